@@ -1,0 +1,203 @@
+/*
+ * pdx.h -- C ABI of the MI355X force-density library (libpdx.so).
+ *
+ * The library replaces the force-density hot path of Peridynamics.jl for one body on one
+ * GPU: the bond system (Morton-ordered points, full CSR bond lists), the bond-based (BB),
+ * ordinary state-based (OSB) and correspondence (CPD) force densities, bond failure and
+ * damage fused with the Velocity Verlet update, and the Jacobian-free Newton-Krylov
+ * J*v products with a device-resident flexible GMRES.
+ *
+ * Conventions
+ *  - Every host array is copied; the library never keeps a caller pointer.
+ *  - Point vectors are Julia's column-major 3 x n (x, y, z of a point contiguous).
+ *  - Indices are 0-based.  The Morton permutation is internal: every get/set uses the
+ *    caller's point order.
+ *  - Status: 0 = PDX_OK, negative = error; pdx_last_error() explains the last one.
+ *  - A context is used by one host thread at a time; it owns one HIP stream and all of
+ *    its device memory until pdx_destroy().
+ *
+ * Reference interfaces replaced (paths relative to the Peridynamics.jl repository):
+ *  see the comment above each entry point.
+ */
+#ifndef PDX_H
+#define PDX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pdx_ctx pdx_ctx;
+typedef int32_t pdx_status;
+
+#define PDX_OK 0
+#define PDX_ERR_ARG (-1)             /* ArgumentError */
+#define PDX_ERR_HIP (-2)             /* HIP runtime failure */
+#define PDX_ERR_OOM (-3)             /* device allocation failed */
+#define PDX_ERR_NAN (-4)             /* NaNError(t, step), src/auxiliary/errors.jl:98-114 */
+#define PDX_ERR_STATE (-5)           /* call out of order */
+#define PDX_ERR_DUPLICATE_POINT (-6) /* check_point_duplicates, bond_system.jl:125-132 */
+#define PDX_ERR_NOT_CONVERGED (-7)   /* newton_krylov.jl:318-324 */
+#define PDX_ERR_UNSUPPORTED (-8)     /* a feature this build does not provide */
+
+/* precision */
+#define PDX_FP64 0  /* all arithmetic in FP64 */
+#define PDX_MIXED 1 /* FP32 stretch/force arithmetic from FP64 differences, FP64 sums and
+                       FP64 integration */
+
+/* material kinds */
+#define PDX_BB 0  /* BBMaterial,  src/physics/bond_based.jl:74-84 */
+#define PDX_OSB 1 /* OSBMaterial, src/physics/ordinary_state_based.jl:82-94 */
+#define PDX_CPD 2 /* CMaterial,   src/physics/correspondence.jl:111-128 */
+
+/* constitutive models of CMaterial (src/physics/constitutive_models.jl) */
+#define PDX_SVK 0 /* SaintVenantKirchhoff :33-48 */
+#define PDX_NHP 1 /* NeoHookePenalty :231-255 */
+
+/* fields for pdx_set_field / pdx_get_field (storage field names of the reference) */
+#define PDX_FIELD_POSITION 0      /* 3 x n */
+#define PDX_FIELD_DISPLACEMENT 1  /* 3 x n */
+#define PDX_FIELD_VELOCITY 2      /* 3 x n */
+#define PDX_FIELD_VELOCITY_HALF 3 /* 3 x n */
+#define PDX_FIELD_ACCELERATION 4  /* 3 x n */
+#define PDX_FIELD_B_INT 5         /* 3 x n */
+#define PDX_FIELD_B_EXT 6         /* 3 x n */
+#define PDX_FIELD_DAMAGE 7        /* n */
+#define PDX_FIELD_N_ACTIVE_BONDS 8 /* n, returned as double */
+#define PDX_FIELD_DEFGRAD 9       /* 9 x n, column-major 3x3 per point (CPD) */
+#define PDX_FIELD_CAUCHY_STRESS 10 /* 9 x n (CPD) */
+#define PDX_FIELD_REF_POSITION 11 /* 3 x n, read only */
+#define PDX_FIELD_VOLUME 12       /* n, read only */
+
+/* Point parameters of one material (StandardPointParameters, src/core/parameters.jl:107-126;
+ * CPointParameters, src/physics/correspondence.jl:148-169).  The host derives them as
+ * material! does (src/physics/material_parameters.jl:64-231, src/physics/fracture.jl:110-130,
+ * bc = 18K/(pi delta^4) at src/physics/bond_based.jl:89). */
+typedef struct pdx_params {
+    double delta, rho, E, nu, G, K, lambda, mu, Gc, epsc, bc;
+    int32_t model;  /* PDX_SVK or PDX_NHP (CPD only) */
+    int32_t kernel; /* 0 = linear_kernel delta/L (src/discretization/kernels.jl:22) */
+    double zem_cs;  /* ZEMSilling Cs, src/discretization/zem_stabilization.jl:16-21 */
+    double maxdmg;  /* CMaterial maxdmg, correspondence.jl:123-128 */
+} pdx_params;
+
+/* Options and statistics of one Newton-Krylov load step (NewtonKrylov,
+ * src/time_solvers/newton_krylov.jl:37-127). */
+typedef struct pdx_nk_opts {
+    int32_t maxiter;          /* Newton iterations */
+    int32_t gmres_maxiter;    /* total Krylov iterations per linear solve */
+    int32_t gmres_restart;    /* restart length */
+    int32_t precond;          /* 0 none, 1 point-block Jacobi, 2 learned (flexible) */
+    double tol;               /* absolute tolerance on ||r|| (reference semantics) */
+    double rel_tol;           /* > 0: stop at ||r|| < rel_tol * ||r_0|| of the load step */
+    double gmres_reltol, gmres_abstol;
+    double perturbation_scale;
+} pdx_nk_opts;
+
+typedef struct pdx_nk_stats {
+    int32_t newton_iters;
+    int32_t krylov_iters;
+    int32_t jv_evals;
+    int32_t precond_fallbacks; /* learned preconditioner rejected, Jacobi used instead */
+    double residual_first, residual_final;
+    double seconds;
+} pdx_nk_stats;
+
+/* --- context -------------------------------------------------------------------------- */
+const char *pdx_version(void);
+/* Creates a context on HIP device `device`.  precision = PDX_FP64 or PDX_MIXED. */
+pdx_status pdx_create(pdx_ctx **out, int32_t device, int32_t precision);
+void pdx_destroy(pdx_ctx *ctx);
+const char *pdx_last_error(const pdx_ctx *ctx);
+pdx_status pdx_synchronize(pdx_ctx *ctx);
+
+/* --- body ----------------------------------------------------------------------------- */
+/* Body positions and volumes (Body, src/discretization/body.jl:83-125) and the per-point
+ * failure permission (failure_permit!, src/physics/fracture.jl:38-42). */
+pdx_status pdx_set_points(pdx_ctx *ctx, int64_t n, const double *X, const double *volume,
+                          const uint8_t *fail_permit);
+/* One parameter set for the whole body (material!, material_parameters.jl:64-92). */
+pdx_status pdx_set_material(pdx_ctx *ctx, int32_t kind, const pdx_params *params);
+/* precrack!(body, set_a, set_b; update_dmg), src/discretization/predefined_cracks.jl:106-112:
+ * filter_bonds = 1 deletes the bonds (update_dmg = false), 0 breaks them at t = 0. */
+pdx_status pdx_add_precrack(pdx_ctx *ctx, int64_t na, const int32_t *set_a, int64_t nb,
+                            const int32_t *set_b, int32_t filter_bonds);
+/* find_bonds + precracks + bond state (BondSystem, bond_system.jl:55-132, 142-176;
+ * apply_precracks!, predefined_cracks.jl:31-61).  Bonds of a point are in ascending
+ * neighbour index order. */
+pdx_status pdx_build_bonds(pdx_ctx *ctx, int64_t *n_bonds_out);
+/* The bond list in the caller's point order: row_ptr[n+1], nbr, L, fail_permit, and the
+ * bond_active state (any pointer may be NULL).  Replaces reading system.bonds /
+ * storage.bond_active. */
+pdx_status pdx_get_bonds(pdx_ctx *ctx, int64_t *row_ptr, int32_t *nbr, double *L,
+                         uint8_t *fail_permit, uint8_t *active);
+
+/* --- fields ----------------------------------------------------------------------------- */
+pdx_status pdx_set_field(pdx_ctx *ctx, int32_t field, const double *host);
+pdx_status pdx_get_field(pdx_ctx *ctx, int32_t field, double *host);
+
+/* --- force density and Velocity Verlet --------------------------------------------------- */
+/* calc_stable_timestep(dh, safety_factor): velocity_verlet.jl:136-163 with
+ * calc_timestep_point, bond_system.jl:269-277. */
+pdx_status pdx_stable_timestep(pdx_ctx *ctx, double safety_factor, double *dt_out);
+/* calc_force_density!(dh, t, dt) at the current positions: failure, damage, force density
+ * (threads_body_data_handler.jl:184-195, bond_system.jl:285-297) plus the NaN check of
+ * nans.jl:10-18 (returns PDX_ERR_NAN). */
+pdx_status pdx_force_density(pdx_ctx *ctx, double t, double dt);
+/* Velocity boundary conditions on velocity_half (velocity_bc!, boundary_conditions.jl:
+ * 422-435).  Point k of the list is points[k]; src[3k+d] is the index into the per-step
+ * value vector that sets its dimension d, or -1.  The host evaluates the reference's
+ * closures (SingleDimBC f(t): one value per condition; PosDepSingleDimBC f(p, t): one
+ * value per point) and a non-finite value leaves the dof alone (apply_sdbc!, apply_pdsdbc!).
+ * Replaces the BC part of apply_boundary_conditions!, boundary_conditions.jl:212-226. */
+pdx_status pdx_set_velocity_bcs(pdx_ctx *ctx, int64_t n_points, const int32_t *points,
+                                const int32_t *src, int64_t n_src);
+/* n_steps Velocity Verlet steps n = first_step ... first_step + n_steps - 1 with step size
+ * dt (solve!, verlet_timestep!, update_vel_half!, update_disp_and_pos!,
+ * _update_acc_and_vel!: velocity_verlet.jl:165-295).  bc_values holds n_steps rows of
+ * n_src values, row k for time (first_step + k) * dt.  One fused launch per step.  On a
+ * NaN force density returns PDX_ERR_NAN and the first such step in *nan_step_out. */
+pdx_status pdx_vv_run(pdx_ctx *ctx, int64_t first_step, int64_t n_steps, double dt,
+                      const double *bc_values, int64_t *nan_step_out);
+
+/* --- Newton-Krylov ---------------------------------------------------------------------- */
+/* Constrained dofs from displacement_bc! (PosSingleDimBC, boundary_conditions.jl:130-177,
+ * 675-688): dof 3*points[k] + dims[k] is prescribed with the full-load value values[k]
+ * (the host evaluates f(p)); load step factor beta scales it. */
+pdx_status pdx_set_displacement_bcs(pdx_ctx *ctx, int64_t n, const int32_t *points,
+                                    const int32_t *dims, const double *values);
+/* Jacobian-vector product of the residual r(u) = (b_int + b_ext) V with constrained rows
+ * set to identity (jacobian_vector_product!, newton_krylov.jl:504-671): Jv = J v for the
+ * current displacement.  v and Jv are 3n host vectors in the caller's order. */
+pdx_status pdx_jv(pdx_ctx *ctx, const double *v, double *Jv, double perturbation_scale);
+/* Residual r = (b_int + b_ext) V, constrained dofs zero (calc_residual!, :371-387). */
+pdx_status pdx_residual(pdx_ctx *ctx, double *r, double *norm_out);
+/* One load step n of n_steps (newton_krylov_step!, newton_krylov.jl:298-332): applies the
+ * scaled displacement BCs, then Newton iterations with a device-resident flexible GMRES
+ * (replacing IterativeSolvers.gmres!, :448-450). */
+pdx_status pdx_nk_step(pdx_ctx *ctx, int64_t n, int64_t n_steps, const pdx_nk_opts *opts,
+                       pdx_nk_stats *stats);
+
+/* --- learned preconditioner (no counterpart in the reference) ---------------------------- */
+/* Collects (residual, Krylov correction) pairs from the plain solves of pdx_nk_step while
+ * enabled, trains the message-passing preconditioner on the device, and reports the final
+ * training loss. */
+pdx_status pdx_gnn_collect(pdx_ctx *ctx, int32_t enable);
+pdx_status pdx_gnn_train(pdx_ctx *ctx, int32_t epochs, double lr, uint64_t seed,
+                         double *loss_out);
+
+/* --- instrumentation ---------------------------------------------------------------------- */
+/* Per-launch HIP-event timing of the step kernels (off by default). */
+pdx_status pdx_set_profiling(pdx_ctx *ctx, int32_t enable);
+/* Average duration (ms) and count of the timed launches of kernel `name`
+ * ("step" = the fused Velocity Verlet kernel, "pass1" = first pass of OSB/CPD). */
+pdx_status pdx_kernel_time(pdx_ctx *ctx, const char *name, double *avg_ms, int64_t *count);
+/* Sizes: n points, n bonds, device bytes held. */
+pdx_status pdx_info(pdx_ctx *ctx, int64_t *n_points, int64_t *n_bonds, int64_t *device_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PDX_H */

@@ -1,0 +1,430 @@
+"""CPU oracle driver for the force-density hot path (TEST INFRASTRUCTURE ONLY).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` import this module, and only as the checker or the timed CPU
+baseline.  The product path (``libpdx.so`` behind ``include/pdx.h``) never calls it.
+
+The numerical kernels live in ``pdx_oracle.c`` (plain C restatement of the
+reference, one function per reference routine, file:line cited there).  This module
+adds the drivers the reference runs around them:
+
+* :func:`find_bonds`             bond_system.jl:96-132 (+ precracks :134-165)
+* :class:`OracleSim.force`       calc_force_density!, bond_system.jl:285-297
+* :meth:`OracleSim.vv_run`       solve!/verlet_timestep!, velocity_verlet.jl:165-197
+* :meth:`OracleSim.stable_dt`    calc_stable_timestep, velocity_verlet.jl:136-163
+* :meth:`OracleSim.nk_step`      newton_krylov_step!, newton_krylov.jl:298-332
+* :meth:`OracleSim.jv`           jacobian_vector_product!, newton_krylov.jl:504-671
+* :func:`gmres`                  restarted GMRES (IterativeSolvers 0.9.4 gmres!, as called
+                                 at newton_krylov.jl:448-450; iterate-level parity
+                                 unpinned, only the converged solution is pinned)
+
+Arrays of points are numpy ``(n, 3)`` C-contiguous, which is the memory layout of
+Julia's column-major ``Matrix{Float64}(3, n)``.  Indices are 0-based.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpdx_oracle.so")
+
+ORC_ERR_DUPLICATE = -6
+
+_lib = None
+
+
+def build(force=False):
+    """Compile pdx_oracle.c with the committed Makefile."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_D = ctypes.c_double
+_INT = ctypes.c_int
+
+
+def _declare(L):
+    L.orc_find_bonds_count.restype = _I64
+    L.orc_find_bonds_count.argtypes = [_I64, _P, _P, _P, _P, _INT]
+    L.orc_find_bonds_fill.restype = _INT
+    L.orc_find_bonds_fill.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _P, _INT]
+    L.orc_find_bonds_brute.restype = _I64
+    L.orc_find_bonds_brute.argtypes = [_I64, _P, _P, _P, _P, _P, _P]
+    L.orc_precrack_hits.restype = None
+    L.orc_precrack_hits.argtypes = [_I64, _P, _P, _P, _P, _P]
+    L.orc_bb_force.restype = None
+    L.orc_bb_force.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _D, _D, _P, _INT]
+    L.orc_osb_force.restype = None
+    L.orc_osb_force.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _D, _D,
+                                _D, _P, _P, _P]
+    L.orc_cpd_force.restype = None
+    L.orc_cpd_force.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _INT, _D, _D,
+                                _D, _D, _D, _D, _D, _D, _D, _P, _P, _P]
+    L.orc_min_timestep.restype = _D
+    L.orc_min_timestep.argtypes = [_I64, _P, _P, _P, _P, _D, _D]
+    L.orc_update_vel_half.restype = None
+    L.orc_update_vel_half.argtypes = [_I64, _P, _P, _D, _P]
+    L.orc_update_disp_and_pos.restype = None
+    L.orc_update_disp_and_pos.argtypes = [_I64, _P, _D, _P, _P]
+    L.orc_update_acc_and_vel.restype = None
+    L.orc_update_acc_and_vel.argtypes = [_I64, _P, _P, _D, _P, _D, _P, _P]
+    L.orc_has_nan.restype = _INT
+    L.orc_has_nan.argtypes = [_I64, _P]
+    L.orc_max_threads.restype = _INT
+    L.orc_max_threads.argtypes = []
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class DuplicatePointError(RuntimeError):
+    pass
+
+
+def find_bonds(X, delta, fail_permit=None, nthreads=0, brute=False):
+    """Bond list of every point: (row_ptr int64[n+1], nbr int32, L float64, bond_fp uint8).
+
+    Neighbours of a point are listed in ascending index order (see pdx_oracle.c)."""
+    X = _c(X, np.float64).reshape(-1, 3)
+    n = X.shape[0]
+    delta = _c(np.broadcast_to(np.asarray(delta, dtype=np.float64), (n,)), np.float64)
+    fp = _c(np.ones(n, np.uint8) if fail_permit is None else np.asarray(fail_permit, np.uint8),
+            np.uint8)
+    counts = np.zeros(n, np.int64)
+    if brute:
+        total = lib().orc_find_bonds_brute(n, _p(X), _p(delta), _p(counts), None, None, None)
+        row_ptr = np.zeros(n + 1, np.int64)
+        np.cumsum(counts, out=row_ptr[1:])
+        nbr = np.zeros(total, np.int32)
+        L = np.zeros(total, np.float64)
+        lib().orc_find_bonds_brute(n, _p(X), _p(delta), None, _p(row_ptr), _p(nbr), _p(L))
+        bfp = (fp[np.repeat(np.arange(n), counts)] & fp[nbr]).astype(np.uint8)
+        if total and np.min(L) < np.finfo(np.float64).eps:
+            raise DuplicatePointError("point duplicate found")
+        return row_ptr, nbr, L, bfp
+    dup = np.zeros(2, np.int64)
+    total = lib().orc_find_bonds_count(n, _p(X), _p(delta), _p(counts), _p(dup), nthreads)
+    if total == ORC_ERR_DUPLICATE:
+        raise DuplicatePointError(
+            f"point duplicate found! Point #{dup[0] + 1} has a duplicate #{dup[1] + 1}")
+    if total < 0:
+        raise RuntimeError(f"orc_find_bonds_count failed: {total}")
+    row_ptr = np.zeros(n + 1, np.int64)
+    np.cumsum(counts, out=row_ptr[1:])
+    nbr = np.zeros(total, np.int32)
+    L = np.zeros(total, np.float64)
+    bfp = np.zeros(total, np.uint8)
+    st = lib().orc_find_bonds_fill(n, _p(X), _p(delta), _p(fp), _p(row_ptr), _p(nbr), _p(L),
+                                   _p(bfp), nthreads)
+    if st != 0:
+        raise RuntimeError(f"orc_find_bonds_fill failed: {st}")
+    return row_ptr, nbr, L, bfp
+
+
+def precrack_hits(row_ptr, nbr, in_a, in_b):
+    n = row_ptr.shape[0] - 1
+    hit = np.zeros(nbr.shape[0], np.uint8)
+    lib().orc_precrack_hits(n, _p(row_ptr), _p(nbr), _p(_c(in_a, np.uint8)),
+                            _p(_c(in_b, np.uint8)), _p(hit))
+    return hit.astype(bool)
+
+
+def _sets_to_mask(n, pts):
+    m = np.zeros(n, np.uint8)
+    m[np.asarray(pts, dtype=np.int64)] = 1
+    return m
+
+
+class OracleSim:
+    """State of one body on the CPU, laid out as the reference's single chunk.
+
+    kind: "BB", "OSB" or "CPD".  params: dict with delta, rho, E, nu, G, K, lambda, mu,
+    Gc, epsc, bc (as StandardPointParameters / CPointParameters), plus for CPD:
+    model ("SVK" | "NHP"), Cs, maxdmg.
+    precracks: list of (points_a, points_b, filter_bonds) as PointSetsPreCrack.
+    """
+
+    def __init__(self, X, vol, kind, params, fail_permit=None, precracks=(), nthreads=0,
+                 bonds=None):
+        self.X = _c(X, np.float64).reshape(-1, 3).copy()
+        n = self.X.shape[0]
+        self.n = n
+        self.vol = _c(np.broadcast_to(np.asarray(vol, np.float64), (n,)), np.float64).copy()
+        self.kind = kind
+        self.p = dict(params)
+        self.nthreads = nthreads
+        fp = np.ones(n, np.uint8) if fail_permit is None else np.asarray(fail_permit, np.uint8)
+        if bonds is None:
+            bonds = find_bonds(self.X, self.p["delta"], fp, nthreads)
+        row_ptr, nbr, L, bfp = bonds
+        counts = np.diff(row_ptr)
+        # precracks with filter_bonds = true delete the bonds (filter_bonds_by_crack!)
+        for a, b, filt in precracks:
+            if not filt:
+                continue
+            hit = precrack_hits(row_ptr, nbr, _sets_to_mask(n, a), _sets_to_mask(n, b))
+            keep = ~hit
+            owner = np.repeat(np.arange(n), counts)
+            counts = np.bincount(owner[keep], minlength=n).astype(np.int64)
+            row_ptr = np.zeros(n + 1, np.int64)
+            np.cumsum(counts, out=row_ptr[1:])
+            nbr, L, bfp = nbr[keep].copy(), L[keep].copy(), bfp[keep].copy()
+        self.row_ptr, self.nbr, self.L, self.bfp = row_ptr, nbr, L, bfp
+        self.n_neighbors = np.diff(row_ptr)
+        nb = nbr.shape[0]
+        self.active = np.ones(nb, np.uint8)
+        self.bond_length = np.zeros(nb, np.float64)
+        self.n_active = self.n_neighbors.astype(np.int64).copy()
+        self.damage = np.zeros(n, np.float64)
+        # precracks with filter_bonds = false break the bonds at t = 0 (break_bonds!)
+        for a, b, filt in precracks:
+            if filt:
+                continue
+            hit = precrack_hits(row_ptr, nbr, _sets_to_mask(n, a), _sets_to_mask(n, b))
+            self.active[hit] = 0
+        owner = np.repeat(np.arange(n), self.n_neighbors)
+        self.n_active = np.bincount(owner, weights=self.active, minlength=n).astype(np.int64)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            self.damage = 1.0 - self.n_active / self.n_neighbors
+        # kernels (find_kernels): linear_kernel(delta_i, L) = delta_i / L for OSB and CPD
+        if kind in ("OSB", "CPD"):
+            self.omega = self.p["delta"] / L
+        else:
+            self.omega = None
+        z = lambda: np.zeros((n, 3), np.float64)
+        self.x = self.X.copy()
+        self.u, self.v, self.vh, self.a, self.b_int, self.b_ext = z(), z(), z(), z(), z(), z()
+        self.defgrad = np.zeros((n, 9), np.float64)
+        self.cauchy = np.zeros((n, 9), np.float64)
+        self.wvol = np.zeros(n, np.float64)
+        self.dil = np.zeros(n, np.float64)
+
+    # calc_force_density!(chunk, t, dt)
+    def force(self, x=None):
+        if x is not None:
+            self.x[...] = x
+        p = self.p
+        L = lib()
+        if self.kind == "BB":
+            L.orc_bb_force(self.n, _p(self.x), _p(self.vol), _p(self.row_ptr), _p(self.nbr),
+                           _p(self.L), _p(self.bfp), _p(self.active), _p(self.bond_length),
+                           _p(self.n_active), _p(self.damage), p["bc"], p["epsc"],
+                           _p(self.b_int), self.nthreads)
+        elif self.kind == "OSB":
+            L.orc_osb_force(self.n, _p(self.X), _p(self.x), _p(self.vol), _p(self.row_ptr),
+                            _p(self.nbr), _p(self.L), _p(self.bfp), _p(self.active),
+                            _p(self.bond_length), _p(self.omega), _p(self.n_active),
+                            _p(self.damage), p["G"], p["K"], p["epsc"], _p(self.b_int),
+                            _p(self.wvol), _p(self.dil))
+        elif self.kind == "CPD":
+            model = {"SVK": 0, "NHP": 1}[p.get("model", "SVK")]
+            L.orc_cpd_force(self.n, _p(self.X), _p(self.x), _p(self.vol), _p(self.row_ptr),
+                            _p(self.nbr), _p(self.L), _p(self.bfp), _p(self.active),
+                            _p(self.omega), _p(self.n_active), _p(self.damage), model,
+                            p["lambda"], p["mu"], p["G"], p["K"], p["bc"], p["delta"],
+                            p.get("Cs", 0.5), p.get("maxdmg", 0.85), p["epsc"],
+                            _p(self.defgrad), _p(self.cauchy), _p(self.b_int))
+        else:
+            raise ValueError(self.kind)
+        return self.b_int
+
+    def stable_dt(self, safety_factor=0.7):
+        dt = lib().orc_min_timestep(self.n, _p(self.vol), _p(self.row_ptr), _p(self.nbr),
+                                    _p(self.L), self.p["bc"], self.p["rho"])
+        return dt * safety_factor
+
+    @staticmethod
+    def _apply_bcs(field, Xref, bcs, t):
+        # apply_boundary_conditions!: boundary_conditions.jl:212-226 (single-dim BCs first,
+        # then the position-dependent ones), a non-finite value leaves the field as it is.
+        for kind, pts, dim, f in bcs:
+            pts = np.asarray(pts, np.int64)
+            if kind == "sd":
+                val = float(f(t))
+                if np.isfinite(val):
+                    field[pts, dim] = val
+        for kind, pts, dim, f in bcs:
+            pts = np.asarray(pts, np.int64)
+            if kind == "pd":
+                for i in pts:
+                    val = float(f(Xref[i], t))
+                    if np.isfinite(val):
+                        field[i, dim] = val
+
+    def vv_run(self, n_steps, dt, bcs=(), first_step=1, check_nans=True):
+        """solve!(dh, VelocityVerlet): verlet_timestep! for n = first_step .. (velocity_verlet.jl:181-197).
+
+        bcs: list of ("sd", points, dim, f(t)) and ("pd", points, dim, f(X, t)) acting on
+        velocity_half (velocity_bc!, boundary_conditions.jl:422-435)."""
+        L = lib()
+        ndof = 3 * self.n
+        dth = 0.5 * dt
+        for n in range(first_step, first_step + n_steps):
+            t = n * dt
+            L.orc_update_vel_half(ndof, _p(self.v), _p(self.a), dth, _p(self.vh))
+            self._apply_bcs(self.vh, self.X, bcs, t)
+            L.orc_update_disp_and_pos(ndof, _p(self.vh), dt, _p(self.u), _p(self.x))
+            self.force()
+            if check_nans and L.orc_has_nan(ndof, _p(self.b_int)):
+                raise FloatingPointError(f"NaN in b_int at t={t}, step {n}")
+            L.orc_update_acc_and_vel(ndof, _p(self.b_int), _p(self.b_ext), self.p["rho"],
+                                     _p(self.vh), dth, _p(self.a), _p(self.v))
+        return self
+
+    # ---------------------------------------------------------------- Newton-Krylov
+    def residual(self, constrained):
+        """calc_residual!: newton_krylov.jl:371-387."""
+        r = (self.b_int + self.b_ext) * self.vol[:, None]
+        r = r.reshape(-1)
+        r[constrained] = 0.0
+        return r
+
+    def jv(self, v, constrained, scale=1.0):
+        """jacobian_vector_product!: newton_krylov.jl:504-532 and :537-671."""
+        v = v.reshape(self.n, 3)
+        u_norm = np.sqrt(np.sum(self.u.reshape(-1) ** 2))
+        v_norm = np.sqrt(np.sum(v.reshape(-1) ** 2))
+        if v_norm < np.finfo(np.float64).eps:
+            v_norm = 1.0
+        eps = scale * np.cbrt(np.finfo(np.float64).eps) * (u_norm + 1.0) / v_norm
+        u0 = self.u.copy()
+        b0 = self.b_int.copy()
+        self.u[...] = u0 + eps * v
+        self.x[...] = self.X + self.u
+        fp = self.force() * self.vol[:, None]
+        fp = fp.copy()
+        self.u[...] = u0 - eps * v
+        self.x[...] = self.X + self.u
+        fm = self.force() * self.vol[:, None]
+        jv = ((fp - fm) / (2 * eps)).reshape(-1)
+        jv[constrained] = v.reshape(-1)[constrained]
+        self.u[...] = u0
+        self.x[...] = self.X + self.u
+        self.b_int[...] = b0
+        return jv
+
+    def nk_step(self, n, n_steps, disp_bcs, maxiter=100, tol=1e-4, rel_tol=None,
+                gmres_maxiter=200, gmres_reltol=1e-4, gmres_abstol=1e-8, gmres_restart=100,
+                scale=1.0, log=None):
+        """newton_krylov_step!: newton_krylov.jl:298-332.
+
+        disp_bcs: list of (points, dim, f(X)) (displacement_bc!, PosSingleDimBC).
+        tol is absolute as in the reference; rel_tol (optional) stops at
+        ||r|| < rel_tol * ||r_first|| of this load step instead."""
+        beta = n / n_steps
+        constrained = self.constrained_dofs(disp_bcs)
+        # apply_incr_boundary_conditions! + update_position! (constrained dofs only)
+        for pts, dim, f in disp_bcs:
+            for i in np.asarray(pts, np.int64):
+                val = beta * float(f(self.X[i]))
+                if np.isfinite(val):
+                    self.u[i, dim] = val
+        xf, Xf, uf = self.x.reshape(-1), self.X.reshape(-1), self.u.reshape(-1)
+        xf[constrained] = Xf[constrained] + uf[constrained]
+        free = np.ones(3 * self.n, bool)
+        free[constrained] = False
+        r0 = None
+        n_jv = 0
+        for it in range(1, maxiter + 1):
+            self.force()
+            r = self.residual(constrained)
+            rn = float(np.sqrt(np.sum(r * r)))
+            if r0 is None:
+                r0 = rn
+            if log is not None:
+                log.append((n, it, rn))
+            thr = tol if rel_tol is None else rel_tol * r0
+            if rn < thr:
+                return {"iters": it, "residual": rn, "n_jv": n_jv}
+            if it == maxiter:
+                raise RuntimeError("Newton-Krylov solver did not converge after max iterations!")
+            def A(v):
+                nonlocal n_jv
+                n_jv += 1
+                return self.jv(v, constrained, scale)
+            du = gmres(A, -r, maxiter=gmres_maxiter, reltol=gmres_reltol, abstol=gmres_abstol,
+                       restart=gmres_restart)
+            uf = self.u.reshape(-1)
+            xf = self.x.reshape(-1)
+            uf[free] += du[free]
+            xf[free] += du[free]
+
+    def constrained_dofs(self, disp_bcs):
+        dofs = []
+        for pts, dim, f in disp_bcs:
+            dofs.extend(3 * int(i) + dim for i in pts)
+        return np.array(sorted(set(dofs)), np.int64)
+
+
+def gmres(A, b, maxiter=200, reltol=1e-4, abstol=1e-8, restart=100):
+    """Restarted GMRES with modified Gram-Schmidt and Givens rotations, x0 = 0.
+
+    Stops when the residual estimate drops below max(reltol * ||b||, abstol) or after
+    maxiter matrix-vector products (IterativeSolvers' stopping rule)."""
+    nrm = np.linalg.norm
+    x = np.zeros_like(b)
+    bnorm = nrm(b)
+    thr = max(reltol * bnorm, abstol)
+    it = 0
+    r = b.copy()
+    beta = nrm(r)
+    while it < maxiter and beta > thr:
+        m = restart
+        V = np.zeros((m + 1, b.shape[0]))
+        H = np.zeros((m + 1, m))
+        cs = np.zeros(m)
+        sn = np.zeros(m)
+        g = np.zeros(m + 1)
+        g[0] = beta
+        V[0] = r / beta
+        k = 0
+        for k in range(m):
+            w = A(V[k])
+            it += 1
+            for j in range(k + 1):
+                H[j, k] = np.dot(V[j], w)
+                w = w - H[j, k] * V[j]
+            H[k + 1, k] = nrm(w)
+            if H[k + 1, k] != 0:
+                V[k + 1] = w / H[k + 1, k]
+            for j in range(k):
+                t = cs[j] * H[j, k] + sn[j] * H[j + 1, k]
+                H[j + 1, k] = -sn[j] * H[j, k] + cs[j] * H[j + 1, k]
+                H[j, k] = t
+            den = np.hypot(H[k, k], H[k + 1, k])
+            cs[k] = H[k, k] / den if den != 0 else 1.0
+            sn[k] = H[k + 1, k] / den if den != 0 else 0.0
+            H[k, k] = den
+            H[k + 1, k] = 0.0
+            g[k + 1] = -sn[k] * g[k]
+            g[k] = cs[k] * g[k]
+            if abs(g[k + 1]) <= thr or it >= maxiter:
+                k += 1
+                break
+        else:
+            k = m
+        y = np.linalg.solve(np.triu(H[:k, :k]), g[:k]) if k > 0 else np.zeros(0)
+        x = x + V[:k].T @ y
+        r = b - A(x)
+        it += 1
+        beta = nrm(r)
+    return x

@@ -1,0 +1,227 @@
+// Internal declarations of libpdx: the context, device helpers and kernel launchers.
+// All device code targets gfx950 (CDNA4, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "pdx.h"
+
+#define PDX_WAVE 64
+#define PDX_BLOCK 256                      // 4 waves per block
+#define PDX_WAVES_PER_BLOCK (PDX_BLOCK / PDX_WAVE)
+#define PDX_MAX_CRACKS 8
+#define PDX_FILL_CAP 1024                  // max bonds of one point in the search kernels
+#define PDX_NUM_XCD 8
+
+// flags of the step kernels
+#define PDX_F_HAS_NEXT 1    // integrate to the next step (write positions of step n+1)
+#define PDX_F_STORE 2       // materialise v, a, b_int, damage, n_active of this step
+#define PDX_F_BEXT 4        // b_ext is not all zero
+#define PDX_F_FORCE_ONLY 8  // calc_force_density! only: write b_int, damage, n_active
+#define PDX_F_SCALE_VOL 16  // force-only: write b_int * V (Newton-Krylov residual form)
+
+struct pdx_crack {
+    std::vector<int32_t> a, b;
+    int filter;
+};
+
+struct pdx_nk;   // Newton-Krylov workspace (pdx_nk.hip)
+struct pdx_gnn;  // learned preconditioner (pdx_gnn.hip)
+
+struct pdx_ctx {
+    int device = 0;
+    int precision = PDX_FP64;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    int64_t n = 0;   // points
+    int64_t nb = 0;  // bonds
+
+    // host copies in the caller's order
+    std::vector<double> X, vol;
+    std::vector<uint8_t> fail_permit;
+    std::vector<int32_t> perm;   // perm[m] = caller index of internal point m (Morton order)
+    std::vector<int32_t> iperm;  // iperm[caller index] = internal index
+
+    int kind = -1;
+    pdx_params P{};
+    std::vector<pdx_crack> cracks;
+    bool bonds_built = false;
+
+    // device state, internal (Morton) order
+    double4 *rec[2] = {nullptr, nullptr};  // (x, y, z, volume) current positions, ping-pong
+    int cur = 0;
+    double4 *Xr = nullptr;                 // (X, Y, Z, 0) reference positions
+    double *u = nullptr, *vh = nullptr, *v = nullptr, *a = nullptr, *bint = nullptr,
+           *bext = nullptr;                // 3n each
+    double *damage = nullptr;              // n
+    int32_t *nact = nullptr;               // n
+    uint8_t *killed = nullptr;             // n (CPD too_much_damage!)
+    double *pstate = nullptr;              // per-point state between the two passes
+    double *defgrad = nullptr, *cauchy = nullptr;  // 9n (CPD)
+    int32_t *d_perm = nullptr, *d_iperm = nullptr;
+    double *stage = nullptr;               // 9n staging for permuted transfers
+    int64_t *row_ptr = nullptr;            // n + 1
+    int32_t *nbr = nullptr;                // nb
+    double *L = nullptr;                   // nb (FP64 layout)
+    float *Lf = nullptr;                   // nb (mixed layout)
+    uint8_t *state = nullptr;              // nb: bit0 active, bit1 fail_permit
+    bool bext_nonzero = false;
+
+    // velocity boundary conditions
+    int32_t *bcslot = nullptr;             // n, -1 = none
+    int32_t *slot_src = nullptr;           // 3 per slot
+    double *bcvals = nullptr;              // value table for a run
+    int64_t n_src = 0, n_bcpts = 0, bcvals_cap = 0;
+
+    unsigned long long *d_flags = nullptr; // [0] first NaN step, [1] duplicate flag, [2..3] pair
+
+    // instrumentation
+    bool prof = false;
+    std::map<std::string, std::pair<double, int64_t>> ktime;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    struct timed_launch {
+        std::string name;
+        hipEvent_t a, b;
+    };
+    std::vector<timed_launch> pending;  // recorded, not yet read
+    std::vector<hipEvent_t> evpool;     // idle events
+    hipEvent_t open_ev = nullptr;       // begin event of the launch being timed
+
+    pdx_nk *nk = nullptr;
+    pdx_gnn *gnn = nullptr;
+    int64_t dev_bytes = 0;
+};
+
+// error helpers ---------------------------------------------------------------------------
+pdx_status pdx_fail(pdx_ctx *c, pdx_status st, const char *fmt, ...);
+#define PDX_HIP(c, call)                                                                    \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return pdx_fail((c), PDX_ERR_HIP, "%s failed: %s (%s:%d)", #call,               \
+                            hipGetErrorString(e_), __FILE__, __LINE__);                     \
+    } while (0)
+#define PDX_TRY(expr)                                                                       \
+    do {                                                                                    \
+        pdx_status s_ = (expr);                                                             \
+        if (s_ != PDX_OK) return s_;                                                        \
+    } while (0)
+
+template <typename T>
+pdx_status pdx_alloc(pdx_ctx *c, T **p, size_t count)
+{
+    if (*p) return PDX_OK;
+    size_t bytes = count * sizeof(T);
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc((void **)p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return pdx_fail(c, PDX_ERR_OOM, "hipMalloc of %zu bytes failed: %s", bytes,
+                        hipGetErrorString(e));
+    }
+    c->dev_bytes += (int64_t)bytes;
+    return PDX_OK;
+}
+void pdx_free_all(pdx_ctx *c);
+
+// launch helpers --------------------------------------------------------------------------
+inline int64_t pdx_wave_grid(int64_t n_waves)
+{
+    int64_t blocks = (n_waves + PDX_WAVES_PER_BLOCK - 1) / PDX_WAVES_PER_BLOCK;
+    return (blocks + PDX_NUM_XCD - 1) / PDX_NUM_XCD * PDX_NUM_XCD;  // multiple of 8
+}
+
+pdx_status pdx_timed_begin(pdx_ctx *c);
+pdx_status pdx_timed_end(pdx_ctx *c, const char *name);
+pdx_status pdx_timed_flush(pdx_ctx *c);
+
+// device helpers --------------------------------------------------------------------------
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  Give each
+// XCD one contiguous range of points so that its L2 holds only its own neighbourhoods.
+// gridDim.x is a multiple of 8, so the map is a bijection of [0, gridDim.x).
+__device__ __forceinline__ int64_t pdx_xcd_block()
+{
+    int64_t g = gridDim.x, b = blockIdx.x;
+    int64_t per = g / PDX_NUM_XCD;
+    return (b % PDX_NUM_XCD) * per + b / PDX_NUM_XCD;
+}
+
+// wave-uniform point index of the calling wave (one wave per point)
+__device__ __forceinline__ int64_t pdx_wave_point()
+{
+    int64_t w = pdx_xcd_block() * PDX_WAVES_PER_BLOCK + (threadIdx.x >> 6);
+    return __builtin_amdgcn_readfirstlane((int)w);
+}
+
+__device__ __forceinline__ double pdx_wave_sum(double x)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+__device__ __forceinline__ int pdx_wave_sum_i(int x)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+__device__ __forceinline__ float pdx_wave_sum_f(float x)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+// arguments of the per-step kernels (shared by BB, OSB and CPD)
+struct pdx_step_args {
+    int64_t n;
+    int64_t step;
+    int flags;
+    double rho, dt, dth;
+    const int64_t *row_ptr;
+    const int32_t *nbr;
+    const double *L;
+    const float *Lf;
+    uint8_t *state;
+    const double4 *rec_in;
+    double4 *rec_out;
+    const double4 *Xr;
+    double *u, *vh;
+    const double *bext;
+    double *v_out, *a_out, *bint_out, *dmg_out;
+    int32_t *nact_out;
+    const int32_t *bcslot, *slot_src;
+    const double *bcv;
+    unsigned long long *flags_out;
+    // material
+    double bc, epsc, delta, G, K, lambda, mu, beta_zem, maxdmg;
+    int model;
+    double *pstate;
+    uint8_t *killed;
+    double *defgrad, *cauchy;
+};
+
+pdx_status pdx_fill_step_args(pdx_ctx *c, pdx_step_args *a, int64_t step, int flags, double dt,
+                              const double *bcv_dev);
+pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a);  // one step of the material
+pdx_status pdx_launch_prologue(pdx_ctx *c, int64_t step, double dt, const double *bcv_dev);
+pdx_status pdx_check_flags(pdx_ctx *c, int64_t *nan_step);
+pdx_status pdx_reset_flags(pdx_ctx *c);
+
+// permuted transfers (pdx_setup.hip)
+pdx_status pdx_upload_vec(pdx_ctx *c, const double *host, double *dev, int comps);
+pdx_status pdx_upload_strided(pdx_ctx *c, const double *host, double *dev, int comps, int stride);
+pdx_status pdx_download_vec(pdx_ctx *c, const double *dev, double *host, int comps, int stride);
+
+// Newton-Krylov (pdx_nk.hip)
+void pdx_nk_free(pdx_ctx *c);
+void pdx_gnn_free(pdx_ctx *c);

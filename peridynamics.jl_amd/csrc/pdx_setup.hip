@@ -1,0 +1,986 @@
+// Context, body set-up and the bond system of libpdx.
+//
+// Bond system (reference: src/discretization/bond_system.jl:55-132, 142-176):
+//  - points are reordered along a Morton curve (host, once) so that the neighbours of a
+//    point are close in memory; every get/set maps back to the caller's order;
+//  - the neighbour search runs on the device over a cell list with cell size = horizon
+//    (the GridNeighborhoodSearch of bond_system.jl:97-99); one wavefront per point counts
+//    the neighbours, a host prefix sum gives the CSR row pointers, and a second pass fills
+//    the bonds of every point ordered by ascending caller index;
+//  - the pair distance is formed as PointNeighbors does: pos_diff = x_i - x_j,
+//    d2 = (p0*p0 + p1*p1) + p2*p2, accepted if d2 <= delta*delta, L = sqrt(d2), so that L
+//    is bit-identical to the reference's Bond.length;
+//  - precracks (predefined_cracks.jl:31-61, bond_system.jl:142-165, 249-267) are applied
+//    inside the same passes: filtered bonds are never emitted, broken ones start inactive.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <numeric>
+
+#include "pdx_internal.h"
+
+static const char *k_version = "pdx-mi355x 0.1.0 (gfx950)";
+
+extern "C" const char *pdx_version(void) { return k_version; }
+
+pdx_status pdx_fail(pdx_ctx *c, pdx_status st, const char *fmt, ...)
+{
+    if (c) {
+        char buf[1024];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        c->err = buf;
+    }
+    return st;
+}
+
+extern "C" const char *pdx_last_error(const pdx_ctx *c)
+{
+    return c ? c->err.c_str() : "null context";
+}
+
+#define FREE(p)                                                                             \
+    do {                                                                                    \
+        if (p) (void)hipFree(p);                                                            \
+        p = nullptr;                                                                        \
+    } while (0)
+
+static void free_bonds(pdx_ctx *c)
+{
+    FREE(c->row_ptr);
+    FREE(c->nbr);
+    FREE(c->L);
+    FREE(c->Lf);
+    FREE(c->state);
+}
+
+void pdx_free_all(pdx_ctx *c)
+{
+    pdx_nk_free(c);
+    pdx_gnn_free(c);
+    FREE(c->rec[0]);
+    FREE(c->rec[1]);
+    FREE(c->Xr);
+    FREE(c->u);
+    FREE(c->vh);
+    FREE(c->v);
+    FREE(c->a);
+    FREE(c->bint);
+    FREE(c->bext);
+    FREE(c->damage);
+    FREE(c->nact);
+    FREE(c->killed);
+    FREE(c->pstate);
+    FREE(c->defgrad);
+    FREE(c->cauchy);
+    FREE(c->d_perm);
+    FREE(c->d_iperm);
+    FREE(c->stage);
+    FREE(c->bcslot);
+    FREE(c->slot_src);
+    FREE(c->bcvals);
+    free_bonds(c);
+    c->dev_bytes = 0;
+    c->bonds_built = false;
+    c->bcvals_cap = 0;
+}
+
+extern "C" pdx_status pdx_create(pdx_ctx **out, int32_t device, int32_t precision)
+{
+    if (!out) return PDX_ERR_ARG;
+    *out = nullptr;
+    if (precision != PDX_FP64 && precision != PDX_MIXED) return PDX_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PDX_ERR_HIP;
+    if (device < 0 || device >= ndev) return PDX_ERR_ARG;
+    pdx_ctx *c = new pdx_ctx();
+    c->device = device;
+    c->precision = precision;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return PDX_ERR_HIP;
+    }
+    if (hipMalloc((void **)&c->d_flags, 8 * sizeof(unsigned long long)) != hipSuccess) {
+        delete c;
+        return PDX_ERR_OOM;
+    }
+    *out = c;
+    return PDX_OK;
+}
+
+extern "C" void pdx_destroy(pdx_ctx *c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    pdx_free_all(c);
+    FREE(c->d_flags);
+    (void)pdx_timed_flush(c);
+    for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
+    if (c->open_ev) (void)hipEventDestroy(c->open_ev);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" pdx_status pdx_synchronize(pdx_ctx *c)
+{
+    if (!c) return PDX_ERR_ARG;
+    PDX_HIP(c, hipStreamSynchronize(c->stream));
+    return PDX_OK;
+}
+
+extern "C" pdx_status pdx_set_points(pdx_ctx *c, int64_t n, const double *X, const double *vol,
+                                     const uint8_t *fp)
+{
+    if (!c) return PDX_ERR_ARG;
+    if (n <= 0 || !X || !vol) return pdx_fail(c, PDX_ERR_ARG, "invalid points (n=%lld)", (long long)n);
+    if (n >= (int64_t)INT32_MAX) return pdx_fail(c, PDX_ERR_UNSUPPORTED, "too many points");
+    PDX_HIP(c, hipSetDevice(c->device));
+    PDX_HIP(c, hipStreamSynchronize(c->stream));
+    pdx_free_all(c);
+    c->n = n;
+    c->nb = 0;
+    c->X.assign(X, X + 3 * n);
+    c->vol.assign(vol, vol + n);
+    c->fail_permit.assign(n, 1);
+    if (fp)
+        for (int64_t i = 0; i < n; i++) c->fail_permit[i] = fp[i] ? 1 : 0;
+    for (int64_t k = 0; k < 3 * n; k++)
+        if (!std::isfinite(X[k])) return pdx_fail(c, PDX_ERR_ARG, "non-finite position");
+    c->cracks.clear();
+    c->n_src = 0;
+    c->n_bcpts = 0;
+    return PDX_OK;
+}
+
+extern "C" pdx_status pdx_set_material(pdx_ctx *c, int32_t kind, const pdx_params *p)
+{
+    if (!c || !p) return PDX_ERR_ARG;
+    if (kind != PDX_BB && kind != PDX_OSB && kind != PDX_CPD)
+        return pdx_fail(c, PDX_ERR_ARG, "unknown material kind %d", kind);
+    if (!(p->delta > 0) || !(p->rho > 0))
+        return pdx_fail(c, PDX_ERR_ARG, "horizon and rho have to be larger than zero");
+    if (kind == PDX_CPD && p->model != PDX_SVK && p->model != PDX_NHP)
+        return pdx_fail(c, PDX_ERR_ARG, "unknown constitutive model %d", p->model);
+    if (p->kernel != 0) return pdx_fail(c, PDX_ERR_UNSUPPORTED, "only linear_kernel is built");
+    if (c->bonds_built && c->kind != kind)
+        return pdx_fail(c, PDX_ERR_STATE, "material kind changed after the bonds were built");
+    c->kind = kind;
+    c->P = *p;
+    return PDX_OK;
+}
+
+extern "C" pdx_status pdx_add_precrack(pdx_ctx *c, int64_t na, const int32_t *a, int64_t nb,
+                                       const int32_t *b, int32_t filter)
+{
+    if (!c) return PDX_ERR_ARG;
+    if (c->bonds_built) return pdx_fail(c, PDX_ERR_STATE, "precracks must precede pdx_build_bonds");
+    if ((int)c->cracks.size() >= PDX_MAX_CRACKS)
+        return pdx_fail(c, PDX_ERR_UNSUPPORTED, "at most %d precracks", PDX_MAX_CRACKS);
+    pdx_crack k;
+    for (int64_t i = 0; i < na; i++) {
+        if (a[i] < 0 || a[i] >= c->n) return pdx_fail(c, PDX_ERR_ARG, "precrack point out of range");
+        k.a.push_back(a[i]);
+    }
+    for (int64_t i = 0; i < nb; i++) {
+        if (b[i] < 0 || b[i] >= c->n) return pdx_fail(c, PDX_ERR_ARG, "precrack point out of range");
+        k.b.push_back(b[i]);
+    }
+    k.filter = filter ? 1 : 0;
+    c->cracks.push_back(k);
+    return PDX_OK;
+}
+
+// ----------------------------------------------------------------------------------------
+// Morton order (host, once per body)
+// ----------------------------------------------------------------------------------------
+static inline uint64_t spread3(uint64_t v)
+{
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffULL;
+    v = (v | v << 16) & 0x1f0000ff0000ffULL;
+    v = (v | v << 8) & 0x100f00f00f00f00fULL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+    v = (v | v << 2) & 0x1249249249249249ULL;
+    return v;
+}
+
+static void morton_order(const std::vector<double> &X, int64_t n, std::vector<int32_t> &perm)
+{
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++) {
+            mn[d] = std::min(mn[d], X[3 * i + d]);
+            mx[d] = std::max(mx[d], X[3 * i + d]);
+        }
+    double ext = std::max({mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2], 1e-300});
+    double s = 2097151.0 / ext;
+    std::vector<std::pair<uint64_t, int32_t>> key(n);
+    for (int64_t i = 0; i < n; i++) {
+        uint64_t q[3];
+        for (int d = 0; d < 3; d++) {
+            double t = (X[3 * i + d] - mn[d]) * s;
+            q[d] = (uint64_t)std::min(std::max(t, 0.0), 2097151.0);
+        }
+        key[i] = {spread3(q[0]) | spread3(q[1]) << 1 | spread3(q[2]) << 2, (int32_t)i};
+    }
+    std::sort(key.begin(), key.end());
+    perm.resize(n);
+    for (int64_t m = 0; m < n; m++) perm[m] = key[m].second;
+}
+
+// ----------------------------------------------------------------------------------------
+// neighbour search kernels
+// ----------------------------------------------------------------------------------------
+struct search_args {
+    int64_t n;
+    const double4 *Xr;        // internal order
+    const int32_t *pcell;     // 3 per point
+    const int64_t *cell_start;
+    const int32_t *cell_pts;
+    int64_t gx, gy, gz;
+    double r2;                // delta * delta
+    const int32_t *perm;      // caller index of each internal point
+    const uint16_t *cmask;    // crack membership (bit 2k: set a of crack k, 2k+1: set b)
+    uint32_t filt_bits;       // bit 2k set: crack k filters its bonds
+    const uint8_t *fp;        // fail_permit, internal order
+    int32_t *counts;
+    const int64_t *row_ptr;
+    int32_t *nbr;
+    double *L;
+    uint8_t *state;
+    unsigned long long *flags;
+};
+
+// (A_i & B_j) | (B_i & A_j) per crack, at the even bit positions
+__device__ __forceinline__ uint32_t crack_hits(uint32_t mi, uint32_t mj)
+{
+    uint32_t ai = mi & 0x5555u, bi = (mi >> 1) & 0x5555u;
+    uint32_t aj = mj & 0x5555u, bj = (mj >> 1) & 0x5555u;
+    return (ai & bj) | (bi & aj);
+}
+
+__device__ __forceinline__ double pair_d2(double4 xi, double4 xj)
+{
+    double p0 = xi.x - xj.x, p1 = xi.y - xj.y, p2 = xi.z - xj.z;
+    return (p0 * p0 + p1 * p1) + p2 * p2;
+}
+
+__global__ __launch_bounds__(PDX_BLOCK) void k_count_bonds(search_args s)
+{
+    int64_t i = pdx_wave_point();
+    if (i >= s.n) return;
+    int lane = threadIdx.x & 63;
+    double4 xi = s.Xr[i];
+    int cx = s.pcell[3 * i], cy = s.pcell[3 * i + 1], cz = s.pcell[3 * i + 2];
+    uint32_t mi = s.cmask ? s.cmask[i] : 0;
+    int cnt = 0;
+    for (int z = cz - 1; z <= cz + 1; z++) {
+        if (z < 0 || z >= s.gz) continue;
+        for (int y = cy - 1; y <= cy + 1; y++) {
+            if (y < 0 || y >= s.gy) continue;
+            for (int x = cx - 1; x <= cx + 1; x++) {
+                if (x < 0 || x >= s.gx) continue;
+                int64_t cell = x + s.gx * (y + s.gy * (int64_t)z);
+                int64_t e = s.cell_start[cell + 1];
+                for (int64_t k = s.cell_start[cell] + lane; k < e; k += 64) {
+                    int32_t j = s.cell_pts[k];
+                    if (j == i) continue;
+                    double d2 = pair_d2(xi, s.Xr[j]);
+                    if (d2 <= s.r2) {
+                        if (sqrt(d2) < 2.220446049250313e-16) {
+                            if (atomicExch(&s.flags[1], 1ULL) == 0ULL) {
+                                s.flags[2] = (unsigned long long)s.perm[i];
+                                s.flags[3] = (unsigned long long)s.perm[j];
+                            }
+                        }
+                        uint32_t h = s.cmask ? crack_hits(mi, s.cmask[j]) : 0u;
+                        if (!(h & s.filt_bits)) cnt++;
+                    }
+                }
+            }
+        }
+    }
+    cnt = pdx_wave_sum_i(cnt);
+    if (lane == 0) s.counts[i] = cnt;
+}
+
+__global__ __launch_bounds__(PDX_BLOCK) void k_fill_bonds(search_args s)
+{
+    __shared__ int32_t cj[PDX_WAVES_PER_BLOCK][PDX_FILL_CAP];
+    __shared__ int32_t co[PDX_WAVES_PER_BLOCK][PDX_FILL_CAP];
+    int64_t i = pdx_wave_point();
+    if (i >= s.n) return;
+    int lane = threadIdx.x & 63;
+    int w = threadIdx.x >> 6;
+    double4 xi = s.Xr[i];
+    int cx = s.pcell[3 * i], cy = s.pcell[3 * i + 1], cz = s.pcell[3 * i + 2];
+    uint32_t mi = s.cmask ? s.cmask[i] : 0;
+    int64_t b0 = s.row_ptr[i];
+    int want = (int)(s.row_ptr[i + 1] - b0);  // <= PDX_FILL_CAP, checked on the host
+    int base = 0;
+    for (int z = cz - 1; z <= cz + 1; z++) {
+        if (z < 0 || z >= s.gz) continue;
+        for (int y = cy - 1; y <= cy + 1; y++) {
+            if (y < 0 || y >= s.gy) continue;
+            for (int x = cx - 1; x <= cx + 1; x++) {
+                if (x < 0 || x >= s.gx) continue;
+                int64_t cell = x + s.gx * (y + s.gy * (int64_t)z);
+                int64_t st = s.cell_start[cell], e = s.cell_start[cell + 1];
+                for (int64_t k0 = st; k0 < e; k0 += 64) {
+                    int64_t k = k0 + lane;
+                    bool take = false;
+                    int32_t j = -1;
+                    if (k < e) {
+                        j = s.cell_pts[k];
+                        if (j != i) {
+                            double d2 = pair_d2(xi, s.Xr[j]);
+                            uint32_t h = s.cmask ? crack_hits(mi, s.cmask[j]) : 0u;
+                            take = (d2 <= s.r2) && !(h & s.filt_bits);
+                        }
+                    }
+                    uint64_t m = __ballot(take);
+                    int pos = base + (int)__popcll(m & ((1ULL << lane) - 1ULL));
+                    if (take && pos < PDX_FILL_CAP) {
+                        cj[w][pos] = j;
+                        co[w][pos] = s.perm[j];
+                    }
+                    base += (int)__popcll(m);
+                }
+            }
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    int cnt = min(base, want);
+    for (int k = lane; k < cnt; k += 64) {
+        int32_t ok = co[w][k];
+        int rank = 0;
+        for (int m2 = 0; m2 < cnt; m2++) rank += (co[w][m2] < ok) ? 1 : 0;
+        int32_t j = cj[w][k];
+        double4 xj = s.Xr[j];
+        double d2 = pair_d2(xi, xj);
+        uint32_t h = s.cmask ? crack_hits(mi, s.cmask[j]) : 0u;
+        uint8_t fpb = (uint8_t)(s.fp[i] & s.fp[j]);
+        uint8_t act = (h & ~s.filt_bits) ? 0 : 1;  // broken at t = 0 by a precrack
+        int64_t b = b0 + rank;
+        s.nbr[b] = j;
+        s.L[b] = sqrt(d2);
+        s.state[b] = (uint8_t)(act | (fpb << 1));
+    }
+}
+
+__global__ void k_init_point_state(int64_t n, const int64_t *row_ptr, const uint8_t *state,
+                                   int32_t *nact, double *damage)
+{
+    int64_t i = pdx_wave_point();
+    if (i >= n) return;
+    int lane = threadIdx.x & 63;
+    int cnt = 0;
+    for (int64_t b = row_ptr[i] + lane; b < row_ptr[i + 1]; b += 64) cnt += state[b] & 1;
+    cnt = pdx_wave_sum_i(cnt);
+    if (lane == 0) {
+        nact[i] = cnt;
+        // calc_damage!: 1 - n_active / n_neighbors (NaN for a point without bonds)
+        damage[i] = 1.0 - (double)cnt / (double)(row_ptr[i + 1] - row_ptr[i]);
+    }
+}
+
+__global__ void k_L_to_float(int64_t nb, const double *L, float *Lf)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nb) Lf[k] = (float)L[k];
+}
+
+// ----------------------------------------------------------------------------------------
+// permuted transfers
+// ----------------------------------------------------------------------------------------
+// dev[m*stride + d] = stage[perm[m]*comps + d]
+__global__ void k_scatter_in(int64_t n, int comps, int stride, const int32_t *perm,
+                             const double *stage, double *dev)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * comps) return;
+    int64_t m = t / comps;
+    int d = (int)(t - m * comps);
+    dev[m * stride + d] = stage[(int64_t)perm[m] * comps + d];
+}
+
+// stage[i*comps + d] = dev[iperm[i]*stride + d]
+__global__ void k_gather_out(int64_t n, int comps, int stride, const int32_t *iperm,
+                             const double *dev, double *stage)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * comps) return;
+    int64_t i = t / comps;
+    int d = (int)(t - i * comps);
+    stage[t] = dev[(int64_t)iperm[i] * stride + d];
+}
+
+pdx_status pdx_upload_vec(pdx_ctx *c, const double *host, double *dev, int comps)
+{
+    int stride = comps;
+    return pdx_upload_strided(c, host, dev, comps, stride);
+}
+
+pdx_status pdx_upload_strided(pdx_ctx *c, const double *host, double *dev, int comps, int stride)
+{
+    int64_t cnt = c->n * comps;
+    PDX_HIP(c, hipMemcpyAsync(c->stage, host, sizeof(double) * cnt, hipMemcpyHostToDevice,
+                              c->stream));
+    int64_t blocks = (cnt + 255) / 256;
+    hipLaunchKernelGGL(k_scatter_in, dim3(blocks), dim3(256), 0, c->stream, c->n, comps, stride,
+                       c->d_perm, c->stage, dev);
+    PDX_HIP(c, hipGetLastError());
+    PDX_HIP(c, hipStreamSynchronize(c->stream));
+    return PDX_OK;
+}
+
+pdx_status pdx_download_vec(pdx_ctx *c, const double *dev, double *host, int comps, int stride)
+{
+    int64_t cnt = c->n * comps;
+    int64_t blocks = (cnt + 255) / 256;
+    hipLaunchKernelGGL(k_gather_out, dim3(blocks), dim3(256), 0, c->stream, c->n, comps, stride,
+                       c->d_iperm, dev, c->stage);
+    PDX_HIP(c, hipGetLastError());
+    PDX_HIP(c, hipMemcpyAsync(host, c->stage, sizeof(double) * cnt, hipMemcpyDeviceToHost,
+                              c->stream));
+    PDX_HIP(c, hipStreamSynchronize(c->stream));
+    return PDX_OK;
+}
+
+// ----------------------------------------------------------------------------------------
+// pdx_build_bonds
+// ----------------------------------------------------------------------------------------
+extern "C" pdx_status pdx_build_bonds(pdx_ctx *c, int64_t *n_bonds_out)
+{
+    if (!c) return PDX_ERR_ARG;
+    if (c->n <= 0) return pdx_fail(c, PDX_ERR_STATE, "pdx_set_points first");
+    if (c->kind < 0) return pdx_fail(c, PDX_ERR_STATE, "pdx_set_material first");
+    PDX_HIP(c, hipSetDevice(c->device));
+    const int64_t n = c->n;
+    if (c->bonds_built) {
+        // rebuilding: drop everything but the points
+        std::vector<pdx_crack> keep = c->cracks;
+        pdx_free_all(c);
+        c->cracks = keep;
+    }
+    // Morton order
+    morton_order(c->X, n, c->perm);
+    c->iperm.assign(n, 0);
+    for (int64_t m = 0; m < n; m++) c->iperm[c->perm[m]] = (int32_t)m;
+    std::vector<double4> Xm(n);
+    std::vector<uint8_t> fpm(n);
+    for (int64_t m = 0; m < n; m++) {
+        int64_t i = c->perm[m];
+        Xm[m] = make_double4(c->X[3 * i], c->X[3 * i + 1], c->X[3 * i + 2], c->vol[i]);
+        fpm[m] = c->fail_permit[i];
+    }
+    // cell list over the internal order, cell size = horizon
+    double cs = c->P.delta;
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t m = 0; m < n; m++) {
+        const double v3[3] = {Xm[m].x, Xm[m].y, Xm[m].z};
+        for (int d = 0; d < 3; d++) {
+            mn[d] = std::min(mn[d], v3[d]);
+            mx[d] = std::max(mx[d], v3[d]);
+        }
+    }
+    for (;;) {
+        double ncell = 1;
+        for (int d = 0; d < 3; d++) ncell *= std::floor((mx[d] - mn[d]) / cs) + 1.0;
+        if (ncell <= 8.0 * (double)n + 4096.0) break;
+        cs *= 2.0;  // a coarser grid still finds every neighbour within the horizon
+    }
+    int64_t g[3];
+    for (int d = 0; d < 3; d++) g[d] = (int64_t)std::floor((mx[d] - mn[d]) / cs) + 1;
+    int64_t ncell = g[0] * g[1] * g[2];
+    std::vector<int32_t> pcell(3 * n);
+    std::vector<int64_t> cell_start(ncell + 1, 0);
+    std::vector<int64_t> cid(n);
+    for (int64_t m = 0; m < n; m++) {
+        const double v3[3] = {Xm[m].x, Xm[m].y, Xm[m].z};
+        for (int d = 0; d < 3; d++) {
+            int64_t q = (int64_t)std::floor((v3[d] - mn[d]) / cs);
+            q = std::min(std::max(q, (int64_t)0), g[d] - 1);
+            pcell[3 * m + d] = (int32_t)q;
+        }
+        cid[m] = pcell[3 * m] + g[0] * (pcell[3 * m + 1] + g[1] * (int64_t)pcell[3 * m + 2]);
+        cell_start[cid[m] + 1]++;
+    }
+    for (int64_t k = 0; k < ncell; k++) cell_start[k + 1] += cell_start[k];
+    std::vector<int32_t> cell_pts(n);
+    {
+        std::vector<int64_t> fill(cell_start.begin(), cell_start.end() - 1);
+        for (int64_t m = 0; m < n; m++) cell_pts[fill[cid[m]]++] = (int32_t)m;
+    }
+    // crack membership masks
+    std::vector<uint16_t> cmask;
+    uint32_t filt_bits = 0;
+    if (!c->cracks.empty()) {
+        cmask.assign(n, 0);
+        for (size_t k = 0; k < c->cracks.size(); k++) {
+            for (int32_t i : c->cracks[k].a) cmask[c->iperm[i]] |= (uint16_t)(1u << (2 * k));
+            for (int32_t i : c->cracks[k].b) cmask[c->iperm[i]] |= (uint16_t)(1u << (2 * k + 1));
+            if (c->cracks[k].filter) filt_bits |= 1u << (2 * k);
+        }
+    }
+    // device buffers
+    PDX_TRY(pdx_alloc(c, &c->rec[0], n));
+    PDX_TRY(pdx_alloc(c, &c->rec[1], n));
+    PDX_TRY(pdx_alloc(c, &c->Xr, n));
+    PDX_TRY(pdx_alloc(c, &c->d_perm, n));
+    PDX_TRY(pdx_alloc(c, &c->d_iperm, n));
+    PDX_TRY(pdx_alloc(c, &c->stage, 9 * n));
+    for (double **p : {&c->u, &c->vh, &c->v, &c->a, &c->bint, &c->bext})
+        PDX_TRY(pdx_alloc(c, p, 3 * n));
+    PDX_TRY(pdx_alloc(c, &c->damage, n));
+    PDX_TRY(pdx_alloc(c, &c->nact, n));
+    PDX_TRY(pdx_alloc(c, &c->killed, n));
+    PDX_TRY(pdx_alloc(c, &c->bcslot, n));
+    int32_t *d_pcell = nullptr, *d_cell_pts = nullptr, *d_counts = nullptr;
+    int64_t *d_cell_start = nullptr;
+    uint16_t *d_cmask = nullptr;
+    uint8_t *d_fp = nullptr;
+    auto cleanup = [&]() {
+        FREE(d_pcell);
+        FREE(d_cell_pts);
+        FREE(d_counts);
+        FREE(d_cell_start);
+        FREE(d_cmask);
+        FREE(d_fp);
+    };
+#define TRY_C(expr)                                                                         \
+    do {                                                                                    \
+        pdx_status s_ = (expr);                                                             \
+        if (s_ != PDX_OK) {                                                                 \
+            cleanup();                                                                      \
+            return s_;                                                                      \
+        }                                                                                   \
+    } while (0)
+#define HIP_C(call)                                                                         \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            cleanup();                                                                      \
+            return pdx_fail(c, PDX_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+        }                                                                                   \
+    } while (0)
+    HIP_C(hipMalloc((void **)&d_pcell, sizeof(int32_t) * 3 * n));
+    HIP_C(hipMalloc((void **)&d_cell_pts, sizeof(int32_t) * n));
+    HIP_C(hipMalloc((void **)&d_counts, sizeof(int32_t) * n));
+    HIP_C(hipMalloc((void **)&d_cell_start, sizeof(int64_t) * (ncell + 1)));
+    HIP_C(hipMalloc((void **)&d_fp, n));
+    if (!cmask.empty()) HIP_C(hipMalloc((void **)&d_cmask, sizeof(uint16_t) * n));
+    HIP_C(hipMemcpy(c->Xr, Xm.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+    HIP_C(hipMemcpy(c->rec[0], Xm.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+    HIP_C(hipMemcpy(c->rec[1], Xm.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+    HIP_C(hipMemcpy(c->d_perm, c->perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    HIP_C(hipMemcpy(c->d_iperm, c->iperm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    HIP_C(hipMemcpy(d_pcell, pcell.data(), sizeof(int32_t) * 3 * n, hipMemcpyHostToDevice));
+    HIP_C(hipMemcpy(d_cell_pts, cell_pts.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    HIP_C(hipMemcpy(d_cell_start, cell_start.data(), sizeof(int64_t) * (ncell + 1),
+                    hipMemcpyHostToDevice));
+    HIP_C(hipMemcpy(d_fp, fpm.data(), n, hipMemcpyHostToDevice));
+    if (d_cmask)
+        HIP_C(hipMemcpy(d_cmask, cmask.data(), sizeof(uint16_t) * n, hipMemcpyHostToDevice));
+    HIP_C(hipMemset(c->d_flags, 0, 8 * sizeof(unsigned long long)));
+    // Each (Xr) record keeps the volume in .w for the bond passes; the reference record's
+    // .w holds the volume too (only .xyz are read as the reference position).
+
+    search_args s{};
+    s.n = n;
+    s.Xr = c->Xr;
+    s.pcell = d_pcell;
+    s.cell_start = d_cell_start;
+    s.cell_pts = d_cell_pts;
+    s.gx = g[0];
+    s.gy = g[1];
+    s.gz = g[2];
+    s.r2 = c->P.delta * c->P.delta;
+    s.perm = c->d_perm;
+    s.cmask = d_cmask;
+    s.filt_bits = filt_bits;
+    s.fp = d_fp;
+    s.counts = d_counts;
+    s.flags = c->d_flags;
+    int64_t grid = pdx_wave_grid(n);
+    hipLaunchKernelGGL(k_count_bonds, dim3(grid), dim3(PDX_BLOCK), 0, c->stream, s);
+    HIP_C(hipGetLastError());
+    std::vector<int32_t> counts(n);
+    HIP_C(hipMemcpyAsync(counts.data(), d_counts, sizeof(int32_t) * n, hipMemcpyDeviceToHost,
+                         c->stream));
+    unsigned long long hflags[8];
+    HIP_C(hipMemcpyAsync(hflags, c->d_flags, sizeof hflags, hipMemcpyDeviceToHost, c->stream));
+    HIP_C(hipStreamSynchronize(c->stream));
+    if (hflags[1]) {
+        cleanup();
+        return pdx_fail(c, PDX_ERR_DUPLICATE_POINT,
+                        "point duplicate found! Point #%llu has a duplicate #%llu which will lead to NaNs",
+                        hflags[2] + 1, hflags[3] + 1);
+    }
+    std::vector<int64_t> row_ptr(n + 1, 0);
+    int32_t maxc = 0;
+    for (int64_t m = 0; m < n; m++) {
+        row_ptr[m + 1] = row_ptr[m] + counts[m];
+        maxc = std::max(maxc, counts[m]);
+    }
+    if (maxc > PDX_FILL_CAP) {
+        cleanup();
+        return pdx_fail(c, PDX_ERR_UNSUPPORTED, "a point has %d bonds (> %d)", maxc, PDX_FILL_CAP);
+    }
+    const int64_t nb = row_ptr[n];
+    if (nb >= ((int64_t)1 << 40)) {
+        cleanup();
+        return pdx_fail(c, PDX_ERR_UNSUPPORTED, "too many bonds");
+    }
+    c->nb = nb;
+    TRY_C(pdx_alloc(c, &c->row_ptr, n + 1));
+    TRY_C(pdx_alloc(c, &c->nbr, nb));
+    TRY_C(pdx_alloc(c, &c->L, nb));
+    TRY_C(pdx_alloc(c, &c->state, nb));
+    HIP_C(hipMemcpy(c->row_ptr, row_ptr.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice));
+    s.row_ptr = c->row_ptr;
+    s.nbr = c->nbr;
+    s.L = c->L;
+    s.state = c->state;
+    hipLaunchKernelGGL(k_fill_bonds, dim3(grid), dim3(PDX_BLOCK), 0, c->stream, s);
+    HIP_C(hipGetLastError());
+    hipLaunchKernelGGL(k_init_point_state, dim3(grid), dim3(PDX_BLOCK), 0, c->stream, n,
+                       c->row_ptr, c->state, c->nact, c->damage);
+    HIP_C(hipGetLastError());
+    if (c->precision == PDX_MIXED) {
+        TRY_C(pdx_alloc(c, &c->Lf, nb));
+        hipLaunchKernelGGL(k_L_to_float, dim3((nb + 255) / 256), dim3(256), 0, c->stream, nb,
+                           c->L, c->Lf);
+        HIP_C(hipGetLastError());
+    }
+    // zero dynamic state
+    for (double *p : {c->u, c->vh, c->v, c->a, c->bint, c->bext})
+        HIP_C(hipMemsetAsync(p, 0, sizeof(double) * 3 * n, c->stream));
+    HIP_C(hipMemsetAsync(c->killed, 0, n, c->stream));
+    HIP_C(hipMemsetAsync(c->bcslot, 0xff, sizeof(int32_t) * n, c->stream));
+    HIP_C(hipStreamSynchronize(c->stream));
+    c->bext_nonzero = false;
+    c->cur = 0;
+    c->n_src = 0;
+    c->n_bcpts = 0;
+    // material state between the passes
+    if (c->kind == PDX_OSB) TRY_C(pdx_alloc(c, &c->pstate, 2 * n));
+    if (c->kind == PDX_CPD) {
+        TRY_C(pdx_alloc(c, &c->pstate, 18 * n));
+        TRY_C(pdx_alloc(c, &c->defgrad, 9 * n));
+        TRY_C(pdx_alloc(c, &c->cauchy, 9 * n));
+        HIP_C(hipMemsetAsync(c->defgrad, 0, sizeof(double) * 9 * n, c->stream));
+        HIP_C(hipMemsetAsync(c->cauchy, 0, sizeof(double) * 9 * n, c->stream));
+        HIP_C(hipStreamSynchronize(c->stream));
+    }
+    cleanup();
+    c->bonds_built = true;
+    if (n_bonds_out) *n_bonds_out = nb;
+    return PDX_OK;
+#undef TRY_C
+#undef HIP_C
+}
+
+extern "C" pdx_status pdx_get_bonds(pdx_ctx *c, int64_t *row_ptr, int32_t *nbr, double *L,
+                                    uint8_t *fp, uint8_t *active)
+{
+    if (!c) return PDX_ERR_ARG;
+    if (!c->bonds_built) return pdx_fail(c, PDX_ERR_STATE, "pdx_build_bonds first");
+    const int64_t n = c->n, nb = c->nb;
+    std::vector<int64_t> rp(n + 1);
+    std::vector<int32_t> nb_dev(nb);
+    std::vector<double> L_dev(L ? nb : 0);
+    std::vector<uint8_t> st(nb);
+    PDX_HIP(c, hipStreamSynchronize(c->stream));
+    PDX_HIP(c, hipMemcpy(rp.data(), c->row_ptr, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost));
+    PDX_HIP(c, hipMemcpy(nb_dev.data(), c->nbr, sizeof(int32_t) * nb, hipMemcpyDeviceToHost));
+    if (L) PDX_HIP(c, hipMemcpy(L_dev.data(), c->L, sizeof(double) * nb, hipMemcpyDeviceToHost));
+    PDX_HIP(c, hipMemcpy(st.data(), c->state, nb, hipMemcpyDeviceToHost));
+    std::vector<uint8_t> killed(n, 0);
+    if (c->kind == PDX_CPD)
+        PDX_HIP(c, hipMemcpy(killed.data(), c->killed, n, hipMemcpyDeviceToHost));
+    // caller order
+    std::vector<int64_t> out_rp(n + 1, 0);
+    for (int64_t i = 0; i < n; i++) {
+        int64_t m = c->iperm[i];
+        out_rp[i + 1] = out_rp[i] + (rp[m + 1] - rp[m]);
+    }
+    if (row_ptr) std::memcpy(row_ptr, out_rp.data(), sizeof(int64_t) * (n + 1));
+    for (int64_t i = 0; i < n; i++) {
+        int64_t m = c->iperm[i];
+        int64_t o = out_rp[i];
+        for (int64_t b = rp[m]; b < rp[m + 1]; b++, o++) {
+            if (nbr) nbr[o] = c->perm[nb_dev[b]];
+            if (L) L[o] = L_dev[b];
+            if (fp) fp[o] = (st[b] >> 1) & 1;
+            if (active) active[o] = (st[b] & 1) && !killed[m];
+        }
+    }
+    return PDX_OK;
+}
+
+// ----------------------------------------------------------------------------------------
+// fields
+// ----------------------------------------------------------------------------------------
+static pdx_status field_ptr(pdx_ctx *c, int field, double **p, int *comps, int *stride)
+{
+    *p = nullptr;
+    *comps = 3;
+    *stride = 3;
+    switch (field) {
+    case PDX_FIELD_POSITION: *p = (double *)c->rec[c->cur]; *stride = 4; break;
+    case PDX_FIELD_DISPLACEMENT: *p = c->u; break;
+    case PDX_FIELD_VELOCITY: *p = c->v; break;
+    case PDX_FIELD_VELOCITY_HALF: *p = c->vh; break;
+    case PDX_FIELD_ACCELERATION: *p = c->a; break;
+    case PDX_FIELD_B_INT: *p = c->bint; break;
+    case PDX_FIELD_B_EXT: *p = c->bext; break;
+    case PDX_FIELD_DAMAGE: *p = c->damage; *comps = 1; *stride = 1; break;
+    case PDX_FIELD_DEFGRAD: *p = c->defgrad; *comps = 9; *stride = 9; break;
+    case PDX_FIELD_CAUCHY_STRESS: *p = c->cauchy; *comps = 9; *stride = 9; break;
+    case PDX_FIELD_REF_POSITION: *p = (double *)c->Xr; *stride = 4; break;
+    case PDX_FIELD_VOLUME: *p = ((double *)c->Xr) + 3; *comps = 1; *stride = 4; break;
+    default: return pdx_fail(c, PDX_ERR_ARG, "unknown field %d", field);
+    }
+    if (!*p) return pdx_fail(c, PDX_ERR_ARG, "field %d is not held by this material", field);
+    return PDX_OK;
+}
+
+__global__ void k_int_to_double(int64_t n, const int32_t *src, double *dst)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[k] = (double)src[k];
+}
+
+__global__ void k_copy_xyz(int64_t n, const double4 *src, double4 *dst)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[k] = src[k];
+}
+
+extern "C" pdx_status pdx_get_field(pdx_ctx *c, int32_t field, double *host)
+{
+    if (!c || !host) return PDX_ERR_ARG;
+    if (!c->bonds_built) return pdx_fail(c, PDX_ERR_STATE, "pdx_build_bonds first");
+    PDX_HIP(c, hipSetDevice(c->device));
+    if (field == PDX_FIELD_N_ACTIVE_BONDS) {
+        double *tmp = c->stage + 4 * c->n;  // second half of the staging buffer
+        hipLaunchKernelGGL(k_int_to_double, dim3((c->n + 255) / 256), dim3(256), 0, c->stream,
+                           c->n, c->nact, tmp);
+        PDX_HIP(c, hipGetLastError());
+        return pdx_download_vec(c, tmp, host, 1, 1);
+    }
+    double *p;
+    int comps, stride;
+    PDX_TRY(field_ptr(c, field, &p, &comps, &stride));
+    if (field == PDX_FIELD_VOLUME) {
+        // .w of the reference record
+        return pdx_download_vec(c, p, host, 1, 4);
+    }
+    return pdx_download_vec(c, p, host, comps, stride);
+}
+
+extern "C" pdx_status pdx_set_field(pdx_ctx *c, int32_t field, const double *host)
+{
+    if (!c || !host) return PDX_ERR_ARG;
+    if (!c->bonds_built) return pdx_fail(c, PDX_ERR_STATE, "pdx_build_bonds first");
+    if (field == PDX_FIELD_REF_POSITION || field == PDX_FIELD_VOLUME ||
+        field == PDX_FIELD_N_ACTIVE_BONDS || field == PDX_FIELD_DEFGRAD ||
+        field == PDX_FIELD_CAUCHY_STRESS)
+        return pdx_fail(c, PDX_ERR_ARG, "field %d is read only", field);
+    PDX_HIP(c, hipSetDevice(c->device));
+    double *p;
+    int comps, stride;
+    PDX_TRY(field_ptr(c, field, &p, &comps, &stride));
+    PDX_TRY(pdx_upload_strided(c, host, p, comps, stride));
+    if (field == PDX_FIELD_B_EXT) {
+        c->bext_nonzero = false;
+        for (int64_t k = 0; k < 3 * c->n; k++)
+            if (host[k] != 0.0) c->bext_nonzero = true;
+    }
+    return PDX_OK;
+}
+
+// ----------------------------------------------------------------------------------------
+// stable time step: calc_timestep_point (bond_system.jl:269-277), summed in bond order by
+// one lane so that the value equals the reference's bit for bit
+// ----------------------------------------------------------------------------------------
+__global__ void k_timestep(int64_t n, const int64_t *row_ptr, const int32_t *nbr,
+                           const double *L, const double4 *Xr, double bc, double rho, double *dt)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double dtsum = 0.0;
+    for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) dtsum += Xr[nbr[b]].w * bc / L[b];
+    dt[i] = sqrt(2.0 * rho / dtsum);
+}
+
+extern "C" pdx_status pdx_stable_timestep(pdx_ctx *c, double safety, double *dt_out)
+{
+    if (!c || !dt_out) return PDX_ERR_ARG;
+    if (!c->bonds_built) return pdx_fail(c, PDX_ERR_STATE, "pdx_build_bonds first");
+    if (!(safety > 0 && safety < 1))
+        return pdx_fail(c, PDX_ERR_ARG, "`safety_factor` of VelocityVerlet has invalid value!");
+    PDX_HIP(c, hipSetDevice(c->device));
+    double *tmp = c->stage;
+    hipLaunchKernelGGL(k_timestep, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->n,
+                       c->row_ptr, c->nbr, c->L, c->Xr, c->P.bc, c->P.rho, tmp);
+    PDX_HIP(c, hipGetLastError());
+    std::vector<double> dt(c->n);
+    PDX_HIP(c, hipMemcpyAsync(dt.data(), tmp, sizeof(double) * c->n, hipMemcpyDeviceToHost,
+                              c->stream));
+    PDX_HIP(c, hipStreamSynchronize(c->stream));
+    double mn = INFINITY;
+    for (double d : dt) mn = std::min(mn, d);
+    *dt_out = mn * safety;
+    return PDX_OK;
+}
+
+// ----------------------------------------------------------------------------------------
+// velocity boundary conditions
+// ----------------------------------------------------------------------------------------
+extern "C" pdx_status pdx_set_velocity_bcs(pdx_ctx *c, int64_t npts, const int32_t *points,
+                                           const int32_t *src, int64_t n_src)
+{
+    if (!c) return PDX_ERR_ARG;
+    if (!c->bonds_built) return pdx_fail(c, PDX_ERR_STATE, "pdx_build_bonds first");
+    if (npts < 0 || n_src < 0 || (npts > 0 && (!points || !src)))
+        return pdx_fail(c, PDX_ERR_ARG, "invalid boundary condition table");
+    PDX_HIP(c, hipSetDevice(c->device));
+    std::vector<int32_t> slot(c->n, -1);
+    std::vector<int32_t> ssrc(3 * std::max<int64_t>(npts, 1), -1);
+    for (int64_t k = 0; k < npts; k++) {
+        int32_t i = points[k];
+        if (i < 0 || i >= c->n) return pdx_fail(c, PDX_ERR_ARG, "BC point out of range");
+        if (slot[c->iperm[i]] >= 0) return pdx_fail(c, PDX_ERR_ARG, "BC point listed twice");
+        slot[c->iperm[i]] = (int32_t)k;
+        for (int d = 0; d < 3; d++) {
+            int32_t sidx = src[3 * k + d];
+            if (sidx < -1 || sidx >= n_src) return pdx_fail(c, PDX_ERR_ARG, "BC source out of range");
+            ssrc[3 * k + d] = sidx;
+        }
+    }
+    if (c->slot_src) {
+        (void)hipFree(c->slot_src);
+        c->slot_src = nullptr;
+    }
+    PDX_TRY(pdx_alloc(c, &c->slot_src, ssrc.size()));
+    PDX_HIP(c, hipMemcpy(c->bcslot, slot.data(), sizeof(int32_t) * c->n, hipMemcpyHostToDevice));
+    PDX_HIP(c, hipMemcpy(c->slot_src, ssrc.data(), sizeof(int32_t) * ssrc.size(),
+                         hipMemcpyHostToDevice));
+    c->n_bcpts = npts;
+    c->n_src = n_src;
+    return PDX_OK;
+}
+
+// ----------------------------------------------------------------------------------------
+// instrumentation
+// ----------------------------------------------------------------------------------------
+extern "C" pdx_status pdx_set_profiling(pdx_ctx *c, int32_t enable)
+{
+    if (!c) return PDX_ERR_ARG;
+    PDX_TRY(pdx_timed_flush(c));
+    c->prof = enable != 0;
+    c->ktime.clear();
+    return PDX_OK;
+}
+
+static pdx_status take_event(pdx_ctx *c, hipEvent_t *e)
+{
+    if (!c->evpool.empty()) {
+        *e = c->evpool.back();
+        c->evpool.pop_back();
+        return PDX_OK;
+    }
+    PDX_HIP(c, hipEventCreate(e));
+    return PDX_OK;
+}
+
+// Launch timing with HIP events on the context's stream.  Events are only recorded here;
+// they are read (and the stream waited for) in pdx_timed_flush, so timing does not
+// serialise back-to-back launches.
+pdx_status pdx_timed_begin(pdx_ctx *c)
+{
+    if (!c->prof) return PDX_OK;
+    PDX_TRY(take_event(c, &c->open_ev));
+    PDX_HIP(c, hipEventRecord(c->open_ev, c->stream));
+    return PDX_OK;
+}
+
+pdx_status pdx_timed_end(pdx_ctx *c, const char *name)
+{
+    if (!c->prof || !c->open_ev) return PDX_OK;
+    hipEvent_t b;
+    PDX_TRY(take_event(c, &b));
+    PDX_HIP(c, hipEventRecord(b, c->stream));
+    c->pending.push_back({name, c->open_ev, b});
+    c->open_ev = nullptr;
+    if (c->pending.size() >= 4096) PDX_TRY(pdx_timed_flush(c));
+    return PDX_OK;
+}
+
+pdx_status pdx_timed_flush(pdx_ctx *c)
+{
+    if (c->pending.empty()) return PDX_OK;
+    PDX_HIP(c, hipEventSynchronize(c->pending.back().b));
+    for (auto &t : c->pending) {
+        float ms = 0;
+        PDX_HIP(c, hipEventElapsedTime(&ms, t.a, t.b));
+        auto &e = c->ktime[t.name];
+        e.first += ms;
+        e.second += 1;
+        c->evpool.push_back(t.a);
+        c->evpool.push_back(t.b);
+    }
+    c->pending.clear();
+    return PDX_OK;
+}
+
+extern "C" pdx_status pdx_kernel_time(pdx_ctx *c, const char *name, double *avg_ms, int64_t *count)
+{
+    if (!c || !name) return PDX_ERR_ARG;
+    PDX_TRY(pdx_timed_flush(c));
+    auto it = c->ktime.find(name);
+    if (it == c->ktime.end()) {
+        if (avg_ms) *avg_ms = 0;
+        if (count) *count = 0;
+        return PDX_OK;
+    }
+    if (avg_ms) *avg_ms = it->second.second ? it->second.first / it->second.second : 0.0;
+    if (count) *count = it->second.second;
+    return PDX_OK;
+}
+
+extern "C" pdx_status pdx_info(pdx_ctx *c, int64_t *np, int64_t *nb, int64_t *bytes)
+{
+    if (!c) return PDX_ERR_ARG;
+    if (np) *np = c->n;
+    if (nb) *nb = c->nb;
+    if (bytes) *bytes = c->dev_bytes;
+    return PDX_OK;
+}
+
+pdx_status pdx_reset_flags(pdx_ctx *c)
+{
+    unsigned long long init[2] = {~0ULL, 0ULL};
+    PDX_HIP(c, hipMemcpyAsync(c->d_flags, init, sizeof init, hipMemcpyHostToDevice, c->stream));
+    return PDX_OK;
+}
+
+pdx_status pdx_check_flags(pdx_ctx *c, int64_t *nan_step)
+{
+    unsigned long long h[2];
+    PDX_HIP(c, hipMemcpyAsync(h, c->d_flags, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    PDX_HIP(c, hipStreamSynchronize(c->stream));
+    if (nan_step) *nan_step = (h[0] == ~0ULL) ? -1 : (int64_t)h[0];
+    return PDX_OK;
+}

@@ -1,0 +1,24 @@
+"""MI355X-native force-density hot path of Peridynamics.jl, host side.
+
+The reference's user API (Body, material!, velocity_bc!, VelocityVerlet, NewtonKrylov,
+Job/submit) over libpdx.so, the C ABI of include/pdx.h.  Julia's `!` suffix is dropped
+and point indices are 0-based."""
+from ._lib import (ArgumentError, DuplicatePointError, NaNError, NotConvergedError, PdxError,
+                   load as load_library)
+from .body import (Body, displacement_bc, failure_permit, forcedensity_bc, material,
+                   n_points, no_failure, point_set, precrack, velocity_bc)
+from .data_handler import MI355XBodyDataHandler
+from .job import Job, submit
+from .materials import (BBMaterial, CMaterial, CriticalStretch, NeoHookePenalty, OSBMaterial,
+                        SaintVenantKirchhoff, ZEMSilling, linear_kernel)
+from .point_generators import jitter, lattice_block, uniform_box
+from .solvers import NewtonKrylov, VelocityVerlet
+
+__all__ = [
+    "ArgumentError", "DuplicatePointError", "NaNError", "NotConvergedError", "PdxError",
+    "load_library", "Body", "displacement_bc", "failure_permit", "forcedensity_bc", "material",
+    "n_points", "no_failure", "point_set", "precrack", "velocity_bc", "MI355XBodyDataHandler",
+    "Job", "submit", "BBMaterial", "CMaterial", "CriticalStretch", "NeoHookePenalty",
+    "OSBMaterial", "SaintVenantKirchhoff", "ZEMSilling", "linear_kernel", "jitter",
+    "lattice_block", "uniform_box", "NewtonKrylov", "VelocityVerlet",
+]

@@ -1,0 +1,283 @@
+"""ctypes binding of libpdx.so (the C ABI declared in include/pdx.h).
+
+The product path has no fallback: if libpdx.so is missing or no GPU is visible, every
+call raises.  This is the binding a Julia ``ccall`` shim mirrors one to one."""
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(PKG_DIR), "libpdx.so")
+
+PDX_OK = 0
+PDX_ERR_ARG = -1
+PDX_ERR_HIP = -2
+PDX_ERR_OOM = -3
+PDX_ERR_NAN = -4
+PDX_ERR_STATE = -5
+PDX_ERR_DUPLICATE_POINT = -6
+PDX_ERR_NOT_CONVERGED = -7
+PDX_ERR_UNSUPPORTED = -8
+
+PDX_FP64 = 0
+PDX_MIXED = 1
+
+PDX_BB, PDX_OSB, PDX_CPD = 0, 1, 2
+PDX_SVK, PDX_NHP = 0, 1
+
+FIELDS = {
+    "position": (0, 3),
+    "displacement": (1, 3),
+    "velocity": (2, 3),
+    "velocity_half": (3, 3),
+    "acceleration": (4, 3),
+    "b_int": (5, 3),
+    "b_ext": (6, 3),
+    "damage": (7, 1),
+    "n_active_bonds": (8, 1),
+    "defgrad": (9, 9),
+    "cauchy_stress": (10, 9),
+    "ref_position": (11, 3),
+    "volume": (12, 1),
+}
+
+
+class PdxError(RuntimeError):
+    """A libpdx call failed."""
+
+    def __init__(self, status, msg):
+        super().__init__(f"[pdx status {status}] {msg}")
+        self.status = status
+
+
+class ArgumentError(PdxError, ValueError):
+    """Invalid argument (the reference's ArgumentError)."""
+
+
+class NaNError(PdxError, FloatingPointError):
+    """NaN in the force density (the reference's NaNError(t, step), errors.jl:98-114)."""
+
+    def __init__(self, status, msg, step=None, t=None):
+        super().__init__(status, msg)
+        self.step = step
+        self.t = t
+
+
+class DuplicatePointError(PdxError):
+    """check_point_duplicates (bond_system.jl:125-132)."""
+
+
+class NotConvergedError(PdxError):
+    """Newton-Krylov did not converge (newton_krylov.jl:318-324)."""
+
+
+class PdxParams(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in
+                ("delta", "rho", "E", "nu", "G", "K", "lambda_", "mu", "Gc", "epsc", "bc")] + [
+        ("model", ctypes.c_int32), ("kernel", ctypes.c_int32), ("zem_cs", ctypes.c_double),
+        ("maxdmg", ctypes.c_double)]
+
+
+class PdxNkOpts(ctypes.Structure):
+    _fields_ = [("maxiter", ctypes.c_int32), ("gmres_maxiter", ctypes.c_int32),
+                ("gmres_restart", ctypes.c_int32), ("precond", ctypes.c_int32),
+                ("tol", ctypes.c_double), ("rel_tol", ctypes.c_double),
+                ("gmres_reltol", ctypes.c_double), ("gmres_abstol", ctypes.c_double),
+                ("perturbation_scale", ctypes.c_double)]
+
+
+class PdxNkStats(ctypes.Structure):
+    _fields_ = [("newton_iters", ctypes.c_int32), ("krylov_iters", ctypes.c_int32),
+                ("jv_evals", ctypes.c_int32), ("precond_fallbacks", ctypes.c_int32),
+                ("residual_first", ctypes.c_double), ("residual_final", ctypes.c_double),
+                ("seconds", ctypes.c_double)]
+
+
+_P = ctypes.c_void_p
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+_D = ctypes.c_double
+_ST = ctypes.c_int32
+
+# name -> (restype, argtypes): every entry point of include/pdx.h
+SIGNATURES = {
+    "pdx_version": (ctypes.c_char_p, []),
+    "pdx_create": (_ST, [ctypes.POINTER(_P), _I32, _I32]),
+    "pdx_destroy": (None, [_P]),
+    "pdx_last_error": (ctypes.c_char_p, [_P]),
+    "pdx_synchronize": (_ST, [_P]),
+    "pdx_set_points": (_ST, [_P, _I64, _P, _P, _P]),
+    "pdx_set_material": (_ST, [_P, _I32, ctypes.POINTER(PdxParams)]),
+    "pdx_add_precrack": (_ST, [_P, _I64, _P, _I64, _P, _I32]),
+    "pdx_build_bonds": (_ST, [_P, ctypes.POINTER(_I64)]),
+    "pdx_get_bonds": (_ST, [_P, _P, _P, _P, _P, _P]),
+    "pdx_set_field": (_ST, [_P, _I32, _P]),
+    "pdx_get_field": (_ST, [_P, _I32, _P]),
+    "pdx_stable_timestep": (_ST, [_P, _D, ctypes.POINTER(_D)]),
+    "pdx_force_density": (_ST, [_P, _D, _D]),
+    "pdx_set_velocity_bcs": (_ST, [_P, _I64, _P, _P, _I64]),
+    "pdx_vv_run": (_ST, [_P, _I64, _I64, _D, _P, ctypes.POINTER(_I64)]),
+    "pdx_set_displacement_bcs": (_ST, [_P, _I64, _P, _P, _P]),
+    "pdx_jv": (_ST, [_P, _P, _P, _D]),
+    "pdx_residual": (_ST, [_P, _P, ctypes.POINTER(_D)]),
+    "pdx_nk_step": (_ST, [_P, _I64, _I64, ctypes.POINTER(PdxNkOpts),
+                          ctypes.POINTER(PdxNkStats)]),
+    "pdx_gnn_collect": (_ST, [_P, _I32]),
+    "pdx_gnn_train": (_ST, [_P, _I32, _D, ctypes.c_uint64, ctypes.POINTER(_D)]),
+    "pdx_set_profiling": (_ST, [_P, _I32]),
+    "pdx_kernel_time": (_ST, [_P, ctypes.c_char_p, ctypes.POINTER(_D), ctypes.POINTER(_I64)]),
+    "pdx_info": (_ST, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libpdx.so (built in-tree by __graft_entry__.build()).  Raises if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"libpdx.so not found at {LIB_PATH}; run __graft_entry__.build() first")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def check(ctx, status, what=""):
+    if status == PDX_OK:
+        return
+    msg = load().pdx_last_error(ctx)
+    msg = msg.decode() if msg else ""
+    if what:
+        msg = f"{what}: {msg}"
+    if status == PDX_ERR_ARG:
+        raise ArgumentError(status, msg)
+    if status == PDX_ERR_NAN:
+        raise NaNError(status, msg)
+    if status == PDX_ERR_DUPLICATE_POINT:
+        raise DuplicatePointError(status, msg)
+    if status == PDX_ERR_NOT_CONVERGED:
+        raise NotConvergedError(status, msg)
+    raise PdxError(status, msg)
+
+
+class Context:
+    """Owner of one pdx_ctx (one body on one GPU)."""
+
+    def __init__(self, device=0, precision=PDX_FP64):
+        self.lib = load()
+        self.h = ctypes.c_void_p()
+        st = self.lib.pdx_create(ctypes.byref(self.h), int(device), int(precision))
+        if st != PDX_OK:
+            raise PdxError(st, f"pdx_create(device={device}) failed (is a GPU visible?)")
+        self.precision = precision
+
+    def close(self):
+        if self.h:
+            self.lib.pdx_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def call(self, name, *args):
+        st = getattr(self.lib, name)(self.h, *args)
+        check(self.h, st, name)
+        return st
+
+    # convenience wrappers ------------------------------------------------------------
+    def set_points(self, X, vol, fail_permit=None):
+        X = np.ascontiguousarray(X, np.float64).reshape(-1, 3)
+        n = X.shape[0]
+        vol = np.ascontiguousarray(np.broadcast_to(np.asarray(vol, np.float64), (n,)))
+        fp = None if fail_permit is None else np.ascontiguousarray(fail_permit, np.uint8)
+        self.call("pdx_set_points", n, ptr(X), ptr(vol), ptr(fp))
+        self.n = n
+
+    def set_material(self, kind, params):
+        self.call("pdx_set_material", int(kind), ctypes.byref(params))
+
+    def add_precrack(self, a, b, filter_bonds):
+        a = np.ascontiguousarray(a, np.int32)
+        b = np.ascontiguousarray(b, np.int32)
+        self.call("pdx_add_precrack", a.shape[0], ptr(a), b.shape[0], ptr(b), int(filter_bonds))
+
+    def build_bonds(self):
+        nb = _I64()
+        self.call("pdx_build_bonds", ctypes.byref(nb))
+        self.nb = nb.value
+        return nb.value
+
+    def get_bonds(self):
+        n, nb = self.n, self.nb
+        row_ptr = np.zeros(n + 1, np.int64)
+        nbr = np.zeros(nb, np.int32)
+        L = np.zeros(nb, np.float64)
+        fp = np.zeros(nb, np.uint8)
+        act = np.zeros(nb, np.uint8)
+        self.call("pdx_get_bonds", ptr(row_ptr), ptr(nbr), ptr(L), ptr(fp), ptr(act))
+        return row_ptr, nbr, L, fp, act
+
+    def get_field(self, name):
+        fid, comps = FIELDS[name]
+        out = np.zeros((self.n, comps), np.float64)
+        self.call("pdx_get_field", fid, ptr(out))
+        return out if comps > 1 else out[:, 0]
+
+    def set_field(self, name, values):
+        fid, comps = FIELDS[name]
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(values, np.float64),
+                                                 (self.n, comps) if comps > 1 else (self.n,)))
+        self.call("pdx_set_field", fid, ptr(a))
+
+    def stable_timestep(self, safety_factor=0.7):
+        dt = _D()
+        self.call("pdx_stable_timestep", float(safety_factor), ctypes.byref(dt))
+        return dt.value
+
+    def force_density(self, t=0.0, dt=0.0):
+        self.call("pdx_force_density", float(t), float(dt))
+
+    def set_velocity_bcs(self, points, src, n_src):
+        points = np.ascontiguousarray(points, np.int32)
+        src = np.ascontiguousarray(src, np.int32).reshape(-1)
+        self.call("pdx_set_velocity_bcs", points.shape[0], ptr(points), ptr(src), int(n_src))
+
+    def vv_run(self, first_step, n_steps, dt, bc_values=None):
+        bv = None if bc_values is None else np.ascontiguousarray(bc_values, np.float64)
+        nan_step = _I64(-1)
+        st = self.lib.pdx_vv_run(self.h, int(first_step), int(n_steps), float(dt), ptr(bv),
+                                 ctypes.byref(nan_step))
+        if st == PDX_ERR_NAN:
+            msg = self.lib.pdx_last_error(self.h).decode()
+            raise NaNError(st, msg, step=nan_step.value, t=nan_step.value * dt)
+        check(self.h, st, "pdx_vv_run")
+
+    def set_profiling(self, on=True):
+        self.call("pdx_set_profiling", 1 if on else 0)
+
+    def kernel_time(self, name):
+        ms = _D()
+        cnt = _I64()
+        self.call("pdx_kernel_time", name.encode(), ctypes.byref(ms), ctypes.byref(cnt))
+        return ms.value, cnt.value
+
+    def synchronize(self):
+        self.call("pdx_synchronize")
+
+    def info(self):
+        a, b, c = _I64(), _I64(), _I64()
+        self.call("pdx_info", ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return a.value, b.value, c.value

@@ -1,0 +1,163 @@
+"""MI355XBodyDataHandler: one body, resident in the HBM of one GPU, behind libpdx.
+
+It takes the place of the reference's ThreadsBodyDataHandler
+(src/core/threads_body_data_handler.jl:24-39, 184-212) for a single body: there is one
+chunk, no halo exchange, and every per-step sweep runs inside libpdx.  The host keeps
+what the reference keeps on the host side of a solver loop: the boundary-condition
+closures, which it evaluates per step and hands to the device as plain values."""
+import numpy as np
+
+from . import _lib
+from ._lib import ArgumentError, Context, NaNError, PdxNkOpts, PdxNkStats, PdxParams
+from .body import eval_posdep
+from .materials import CMaterial
+
+
+def _arg_error(msg):
+    return ArgumentError(-1, msg)
+
+
+PRECISIONS = {"fp64": _lib.PDX_FP64, "mixed": _lib.PDX_MIXED}
+
+
+class MI355XBodyDataHandler:
+    def __init__(self, body, device=0, precision="fp64"):
+        if precision not in PRECISIONS:
+            raise _arg_error("precision must be 'fp64' or 'mixed'")
+        if np.any(body.params_map == 0):
+            raise _arg_error("not all points have material parameters (material!)")
+        used = sorted(set(int(k) for k in body.params_map))
+        keys = {body.point_params[k - 1].key() for k in used}
+        if len(keys) != 1:
+            raise _lib.PdxError(_lib.PDX_ERR_UNSUPPORTED,
+                                "bodies with several parameter sets are not provided by the "
+                                "MI355X build")
+        self.body = body
+        self.params = body.point_params[used[0] - 1]
+        self.precision = precision
+        self.ctx = Context(device, PRECISIONS[precision])
+        self.n = body.n_points
+        self.ctx.set_points(body.xyz, body.volume, body.fail_permit.astype(np.uint8))
+        p = self.params
+        mat = body.mat
+        self.pp = PdxParams(delta=p.delta, rho=p.rho, E=p.E, nu=p.nu, G=p.G, K=p.K,
+                            lambda_=p.lam, mu=p.mu, Gc=p.Gc, epsc=p.epsc, bc=p.bc, model=0,
+                            kernel=0, zem_cs=0.5, maxdmg=0.85)
+        if isinstance(mat, CMaterial):
+            self.pp.model = mat.constitutive_model.code
+            self.pp.zem_cs = mat.zem.Cs
+            self.pp.maxdmg = mat.maxdmg
+        self.ctx.set_material(mat.kind, self.pp)
+        for a, b, filt in body.point_sets_precracks:
+            self.ctx.add_precrack(body.point_sets[a], body.point_sets[b], filt)
+        self.n_bonds = self.ctx.build_bonds()
+        if any(f == "b_ext" for f, *_ in body.single_dim_bcs + body.posdep_single_dim_bcs):
+            raise _lib.PdxError(_lib.PDX_ERR_UNSUPPORTED,
+                                "force density boundary conditions are not provided yet")
+        self._compile_velocity_bcs()
+        self.dt = None
+
+    # ------------------------------------------------------------------ boundary conditions
+    def _compile_velocity_bcs(self):
+        body = self.body
+        sd = [(s, d, f) for fld, s, d, f in body.single_dim_bcs if fld == "velocity_half"]
+        pd = [(s, d, f) for fld, s, d, f in body.posdep_single_dim_bcs if fld == "velocity_half"]
+        sets = [body.point_sets[s] for s, _, _ in sd + pd]
+        pts = np.unique(np.concatenate(sets)) if sets else np.zeros(0, np.int64)
+        slot = {int(p): k for k, p in enumerate(pts)}
+        src = -np.ones((len(pts), 3), np.int32)
+        n_src = 0
+        self._sd = []
+        self._pd = []
+        for s, d, f in sd:
+            idx = [slot[int(p)] for p in body.point_sets[s]]
+            src[idx, d] = n_src
+            self._sd.append((n_src, f))
+            n_src += 1
+        for s, d, f in pd:
+            ps = body.point_sets[s]
+            idx = [slot[int(p)] for p in ps]
+            src[idx, d] = n_src + np.arange(len(ps))
+            self._pd.append((n_src, ps, f))
+            n_src += len(ps)
+        self.n_src = n_src
+        self.ctx.set_velocity_bcs(pts.astype(np.int32), src, n_src)
+
+    def bc_values(self, steps, dt):
+        """Values of every condition source for each step n (time n*dt)."""
+        steps = list(steps)
+        vals = np.empty((len(steps), max(self.n_src, 0)), np.float64)
+        X = self.body.position
+        for k, n in enumerate(steps):
+            t = n * dt
+            for s, f in self._sd:
+                vals[k, s] = float(f(t))
+            for base, ps, f in self._pd:
+                vals[k, base:base + len(ps)] = eval_posdep(f, X[:, ps], t)
+        return vals
+
+    # ------------------------------------------------------------------ reference interface
+    def calc_force_density(self, t=0.0, dt=0.0):
+        """calc_force_density!(dh, t, Δt): threads_body_data_handler.jl:184-195."""
+        self.ctx.force_density(t, dt)
+
+    def calc_stable_timestep(self, safety_factor=0.7):
+        """calc_stable_timestep(dh, safety_factor): velocity_verlet.jl:136-163."""
+        return self.ctx.stable_timestep(safety_factor)
+
+    def init_time_solver(self, vv):
+        """init_time_solver!(vv, dh): velocity_verlet.jl:103-114."""
+        if vv.dt < 0:
+            vv.dt = self.calc_stable_timestep(vv.safety_factor)
+        if vv.end_time < 0:
+            vv.end_time = vv.n_steps * vv.dt
+        elif vv.n_steps < 0:
+            vv.n_steps = int(vv.end_time // vv.dt + 1)
+        vv.check()
+        self.dt = vv.dt
+
+    def initialize(self, vv):
+        """initialize!(dh, solver): data_handler.jl:1-4 (force at t = 0; a stays 0)."""
+        self.calc_force_density(0.0, vv.dt)
+
+    def solve(self, vv, freq=0, on_export=None, first_step=1, max_chunk=None):
+        """solve!(dh, vv, options): velocity_verlet.jl:165-179.  on_export(dh, n, t) runs
+        every `freq` steps (export_results)."""
+        dt = vv.dt
+        remaining = vv.n_steps - first_step + 1
+        n = first_step
+        cap = max(1, (1 << 24) // max(self.n_src, 1))
+        if max_chunk:
+            cap = min(cap, max_chunk)
+        if freq and freq > 0:
+            cap = min(cap, freq)
+        while remaining > 0:
+            k = min(cap, remaining)
+            if freq and freq > 0:
+                k = min(k, freq - ((n - 1) % freq))
+            vals = self.bc_values(range(n, n + k), dt) if self.n_src else None
+            try:
+                self.ctx.vv_run(n, k, dt, vals)
+            except NaNError as e:
+                raise NaNError(e.status, f"NaN in the force density at t={e.step * dt}, step {e.step}",
+                               step=e.step, t=e.step * dt) from None
+            n += k
+            remaining -= k
+            if on_export is not None and freq and (n - 1) % freq == 0:
+                on_export(self, n - 1, (n - 1) * dt)
+        return self
+
+    # ------------------------------------------------------------------ fields
+    def get(self, name):
+        """A storage field in the caller's point order, shape (n, 3), (n, 9) or (n,)."""
+        return self.ctx.get_field(name)
+
+    def set(self, name, values):
+        self.ctx.set_field(name, values)
+
+    def bonds(self):
+        """(row_ptr, nbr, L, fail_permit, active) in the caller's point order, 0-based."""
+        return self.ctx.get_bonds()
+
+    def close(self):
+        self.ctx.close()

@@ -1,0 +1,89 @@
+"""Time solvers: VelocityVerlet (src/time_solvers/velocity_verlet.jl:47-163) and
+NewtonKrylov (src/time_solvers/newton_krylov.jl:37-127, 222-295)."""
+import math
+import warnings
+
+from ._lib import ArgumentError
+
+
+def _arg_error(msg):
+    return ArgumentError(-1, msg)
+
+
+class VelocityVerlet:
+    """VelocityVerlet(; time, steps, stepsize, safety_factor=0.7): velocity_verlet.jl:47-71."""
+
+    def __init__(self, time=-1, steps=-1, stepsize=-1, safety_factor=0.7):
+        if time > 0 and steps > 0:
+            raise _arg_error("specify either time or number of steps, not both!")
+        if time < 0 and steps < 0:
+            raise _arg_error("specify either time or number of steps!")
+        if not (0 < safety_factor < 1):
+            raise _arg_error("wrong safety factor specified! condition: 0 < safety_factor < 1")
+        if stepsize > 0:
+            warnings.warn("stepsize specified! Please be sure that the CFD-condition holds!")
+        self.end_time = float(time)
+        self.n_steps = int(steps)
+        self.dt = float(stepsize)
+        self.safety_factor = float(safety_factor)
+
+    def check(self):
+        """velocity_verlet_check: velocity_verlet.jl:116-130."""
+        if self.end_time < 0:
+            raise RuntimeError("`end_time` of VelocityVerlet smaller than zero!\n")
+        if self.n_steps < 0:
+            raise RuntimeError("`n_steps` of VelocityVerlet smaller than zero!\n")
+        if self.dt < 0:
+            raise RuntimeError("`Δt` of VelocityVerlet smaller than zero!\n")
+        if not (0 < self.safety_factor < 1):
+            raise RuntimeError("`safety_factor` of VelocityVerlet has invalid value!\n")
+
+    def __repr__(self):
+        return f"VelocityVerlet(n_steps={self.n_steps}, safety_factor={self.safety_factor})"
+
+
+class NewtonKrylov:
+    """NewtonKrylov(; kwargs...): newton_krylov.jl:55-127.
+
+    Additions of the MI355X build (defaults keep the reference behaviour):
+    rel_tol > 0 stops a load step at ||r|| < rel_tol * ||r_first|| (the reference's `tol`
+    is absolute, newton_krylov.jl:311-314); precond = "none" | "jacobi" | "learned"."""
+
+    def __init__(self, time=-1, steps=-1, stepsize=1.0, maxiter=100, tol=1e-4,
+                 perturbation_scale=1.0, gmres_maxiter=200, gmres_reltol=1e-4,
+                 gmres_abstol=1e-8, gmres_restart=100, rel_tol=0.0, precond="none"):
+        if time > 0 and steps > 0:
+            raise _arg_error("specify either time or number of steps, not both!")
+        if time < 0 and steps < 0:
+            raise _arg_error("specify either time or number of steps!")
+        if stepsize <= 0:
+            raise _arg_error("stepsize has to be larger than zero: stepsize > 0")
+        if time > 0:
+            n_steps = int(math.ceil(time / stepsize))
+            end_time = float(time)
+        else:
+            n_steps = int(steps)
+            end_time = steps * stepsize
+        for name, val in (("maxiter", maxiter), ("tol", tol), ("gmres_reltol", gmres_reltol),
+                          ("gmres_abstol", gmres_abstol), ("gmres_maxiter", gmres_maxiter),
+                          ("gmres_restart", gmres_restart),
+                          ("perturbation_scale", perturbation_scale)):
+            if val <= 0:
+                raise _arg_error(f"{name} has to be larger than zero: {name} > 0")
+        if precond not in ("none", "jacobi", "learned"):
+            raise _arg_error("precond must be 'none', 'jacobi' or 'learned'")
+        self.end_time = end_time
+        self.n_steps = n_steps
+        self.dt = float(stepsize)
+        self.maxiter = int(maxiter)
+        self.tol = float(tol)
+        self.perturbation_scale = float(perturbation_scale)
+        self.gmres_maxiter = int(gmres_maxiter)
+        self.gmres_reltol = float(gmres_reltol)
+        self.gmres_abstol = float(gmres_abstol)
+        self.gmres_restart = int(gmres_restart)
+        self.rel_tol = float(rel_tol)
+        self.precond = precond
+
+    def __repr__(self):
+        return f"NewtonKrylov(n_steps={self.n_steps}, stepsize={self.dt})"

@@ -1,0 +1,61 @@
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+for p in (ROOT, os.path.join(ROOT, "peridynamics.jl_amd"), os.path.join(ROOT, "oracle")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs libpdx kernels)")
+    config.addinivalue_line("markers", "slow: long-running")
+
+
+@pytest.fixture(scope="session")
+def golden():
+    with open(os.path.join(ROOT, "tests", "golden", "reference_values.json")) as f:
+        return json.load(f)
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    import oracle as orc
+    orc.build()
+    return orc
+
+
+def gpu_available():
+    try:
+        import torch  # noqa: F401  (only to count devices; the product path does not use it)
+        return torch.cuda.device_count() > 0
+    except Exception:
+        return False
+
+
+@pytest.fixture(scope="session")
+def need_gpu():
+    if not gpu_available():
+        pytest.skip("no GPU visible")
+
+
+def bb_params(E=70e9, delta=3.015e-3, rho=2700.0, Gc=100.0):
+    """Point parameters of a bond-based material (material! derivation, host side)."""
+    import peridynamics as pd
+    kw = dict(horizon=delta, rho=rho, E=E)
+    if Gc is not None:
+        kw["Gc"] = Gc
+    p = pd.materials.get_point_params(pd.BBMaterial(), kw)
+    return p
+
+
+def as_oracle_params(p, **extra):
+    d = dict(delta=p.delta, rho=p.rho, E=p.E, nu=p.nu, G=p.G, K=p.K, bc=p.bc, epsc=p.epsc,
+             Gc=p.Gc, mu=p.mu)
+    d["lambda"] = p.lam
+    d.update(extra)
+    return d

@@ -67,13 +67,13 @@ def _declare(L):
     L.orc_precrack_hits.restype = None
     L.orc_precrack_hits.argtypes = [_I64, _P, _P, _P, _P, _P]
     L.orc_bb_force.restype = None
-    L.orc_bb_force.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _D, _D, _P, _INT]
+    L.orc_bb_force.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _D, _D, _P, _P, _INT]
     L.orc_osb_force.restype = None
     L.orc_osb_force.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _D, _D,
-                                _D, _P, _P, _P]
+                                _D, _P, _P, _P, _P]
     L.orc_cpd_force.restype = None
     L.orc_cpd_force.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _INT, _D, _D,
-                                _D, _D, _D, _D, _D, _D, _D, _P, _P, _P]
+                                _D, _D, _D, _D, _D, _D, _D, _P, _P, _P, _P]
     L.orc_min_timestep.restype = _D
     L.orc_min_timestep.argtypes = [_I64, _P, _P, _P, _P, _D, _D]
     L.orc_update_vel_half.restype = None
@@ -213,6 +213,9 @@ class OracleSim:
         z = lambda: np.zeros((n, 3), np.float64)
         self.x = self.X.copy()
         self.u, self.v, self.vh, self.a, self.b_int, self.b_ext = z(), z(), z(), z(), z(), z()
+        # per point and component: sum of |bond contribution| of the last force evaluation,
+        # the scale of the rounding error of the (cancelling) force sum
+        self.b_abs = z()
         self.defgrad = np.zeros((n, 9), np.float64)
         self.cauchy = np.zeros((n, 9), np.float64)
         self.wvol = np.zeros(n, np.float64)
@@ -228,13 +231,13 @@ class OracleSim:
             L.orc_bb_force(self.n, _p(self.x), _p(self.vol), _p(self.row_ptr), _p(self.nbr),
                            _p(self.L), _p(self.bfp), _p(self.active), _p(self.bond_length),
                            _p(self.n_active), _p(self.damage), p["bc"], p["epsc"],
-                           _p(self.b_int), self.nthreads)
+                           _p(self.b_int), _p(self.b_abs), self.nthreads)
         elif self.kind == "OSB":
             L.orc_osb_force(self.n, _p(self.X), _p(self.x), _p(self.vol), _p(self.row_ptr),
                             _p(self.nbr), _p(self.L), _p(self.bfp), _p(self.active),
                             _p(self.bond_length), _p(self.omega), _p(self.n_active),
                             _p(self.damage), p["G"], p["K"], p["epsc"], _p(self.b_int),
-                            _p(self.wvol), _p(self.dil))
+                            _p(self.wvol), _p(self.dil), _p(self.b_abs))
         elif self.kind == "CPD":
             model = {"SVK": 0, "NHP": 1}[p.get("model", "SVK")]
             L.orc_cpd_force(self.n, _p(self.X), _p(self.x), _p(self.vol), _p(self.row_ptr),
@@ -242,7 +245,8 @@ class OracleSim:
                             _p(self.omega), _p(self.n_active), _p(self.damage), model,
                             p["lambda"], p["mu"], p["G"], p["K"], p["bc"], p["delta"],
                             p.get("Cs", 0.5), p.get("maxdmg", 0.85), p["epsc"],
-                            _p(self.defgrad), _p(self.cauchy), _p(self.b_int))
+                            _p(self.defgrad), _p(self.cauchy), _p(self.b_int),
+                            _p(self.b_abs))
         else:
             raise ValueError(self.kind)
         return self.b_int

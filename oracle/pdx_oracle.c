@@ -310,7 +310,7 @@ void orc_precrack_hits(int64_t n, const int64_t *row_ptr, const int32_t *nbr,
 void orc_bb_force(int64_t n, const double *x, const double *vol, const int64_t *row_ptr,
                   const int32_t *nbr, const double *L, const uint8_t *bond_fp,
                   uint8_t *active, double *bond_length, int64_t *n_active, double *damage,
-                  double bc, double epsc, double *b_int, int nthreads)
+                  double bc, double epsc, double *b_int, double *b_abs, int nthreads)
 {
     set_threads(nthreads);
 #pragma omp parallel for schedule(static)
@@ -334,7 +334,7 @@ void orc_bb_force(int64_t n, const double *x, const double *vol, const int64_t *
         /* calc_damage! */
         damage[i] = 1.0 - (double)na / (double)(row_ptr[i + 1] - row_ptr[i]);
         /* force_density_point! */
-        double bx = 0.0, by = 0.0, bz = 0.0;
+        double bx = 0.0, by = 0.0, bz = 0.0, ax = 0.0, ay = 0.0, az = 0.0;
         for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
             int64_t j = nbr[b];
             double dx = x[3 * j] - x[3 * i];
@@ -344,11 +344,12 @@ void orc_bb_force(int64_t n, const double *x, const double *vol, const int64_t *
             double eps = (l - L[b]) / L[b];
             double w = active[b] ? 1.0 : 0.0; /* Bool * 1 */
             double s = ((w * bc) * eps) * vol[j];
-            bx += (s * dx) / l;
-            by += (s * dy) / l;
-            bz += (s * dz) / l;
+            double tx = (s * dx) / l, ty = (s * dy) / l, tz = (s * dz) / l;
+            bx += tx; by += ty; bz += tz;
+            ax += fabs(tx); ay += fabs(ty); az += fabs(tz);
         }
         b_int[3 * i] += bx; b_int[3 * i + 1] += by; b_int[3 * i + 2] += bz;
+        if (b_abs) { b_abs[3 * i] = ax; b_abs[3 * i + 1] = ay; b_abs[3 * i + 2] = az; }
     }
 }
 
@@ -368,9 +369,11 @@ void orc_osb_force(int64_t n, const double *X, const double *x, const double *vo
                    const int64_t *row_ptr, const int32_t *nbr, const double *L,
                    const uint8_t *bond_fp, uint8_t *active, double *bond_length,
                    const double *omega, int64_t *n_active, double *damage, double G,
-                   double K, double epsc, double *b_int, double *wvol_out, double *dil_out)
+                   double K, double epsc, double *b_int, double *wvol_out, double *dil_out,
+                   double *b_abs)
 {
     for (int64_t k = 0; k < 3 * n; k++) b_int[k] = 0.0;
+    if (b_abs) for (int64_t k = 0; k < 3 * n; k++) b_abs[k] = 0.0;
     for (int64_t i = 0; i < n; i++) {
         int64_t na = 0;
         for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
@@ -427,6 +430,12 @@ void orc_osb_force(int64_t n, const double *X, const double *x, const double *vo
             b_int[3 * j] += -px * vol[i];
             b_int[3 * j + 1] += -py * vol[i];
             b_int[3 * j + 2] += -pz * vol[i];
+            if (b_abs) {
+                b_abs[3 * i] += fabs(px * vol[j]); b_abs[3 * i + 1] += fabs(py * vol[j]);
+                b_abs[3 * i + 2] += fabs(pz * vol[j]);
+                b_abs[3 * j] += fabs(px * vol[i]); b_abs[3 * j + 1] += fabs(py * vol[i]);
+                b_abs[3 * j + 2] += fabs(pz * vol[i]);
+            }
         }
     }
 }
@@ -562,9 +571,11 @@ void orc_cpd_force(int64_t n, const double *X, const double *x, const double *vo
                    const uint8_t *bond_fp, uint8_t *active, const double *omega,
                    int64_t *n_active, double *damage, int model, double lambda, double mu,
                    double G, double K, double bc, double delta, double Cs, double maxdmg,
-                   double epsc, double *defgrad, double *cauchy_out, double *b_int)
+                   double epsc, double *defgrad, double *cauchy_out, double *b_int,
+                   double *b_abs)
 {
     for (int64_t k = 0; k < 3 * n; k++) b_int[k] = 0.0;
+    if (b_abs) for (int64_t k = 0; k < 3 * n; k++) b_abs[k] = 0.0;
     double beta = (Cs * bc) / delta;
     for (int64_t i = 0; i < n; i++) {
         int64_t na = 0;
@@ -627,14 +638,24 @@ void orc_cpd_force(int64_t n, const double *X, const double *x, const double *vo
             for (int r = 0; r < 3; r++) FX[r] = (F[r] * DX[0] + F[r + 3] * DX[1]) + F[r + 6] * DX[2];
             for (int r = 0; r < 3; r++) z[r] = dxv[r] - FX[r];
             double wb = w * beta;
+            double tabs[3];
             for (int r = 0; r < 3; r++) {
                 double tz = wb * z[r];
                 double m0 = w * PKinv[r], m1 = w * PKinv[r + 3], m2 = w * PKinv[r + 6];
                 t[r] = ((m0 * DX[0] + m1 * DX[1]) + m2 * DX[2]) + tz;
+                /* rounding scale of t: its parts before the cancellations inside
+                 * z = dx - F dX and P K^-1 dX */
+                tabs[r] = (fabs(m0 * DX[0]) + fabs(m1 * DX[1]) + fabs(m2 * DX[2])) +
+                          fabs(wb) * (fabs(dxv[r]) + fabs(F[r] * DX[0]) + fabs(F[r + 3] * DX[1]) +
+                                      fabs(F[r + 6] * DX[2]));
             }
             for (int r = 0; r < 3; r++) {
                 b_int[3 * i + r] += t[r] * vol[j];
                 b_int[3 * j + r] += -t[r] * vol[i];
+                if (b_abs) {
+                    b_abs[3 * i + r] += tabs[r] * vol[j];
+                    b_abs[3 * j + r] += tabs[r] * vol[i];
+                }
             }
         }
     }

@@ -19,6 +19,12 @@
 #define PDX_FILL_CAP 1024                  // max bonds of one point in the search kernels
 #define PDX_NUM_XCD 8
 
+// bond word: bits 0-29 neighbour (internal index), bit 30 active, bit 31 fail_permit
+#define PDX_BW_NBR 0x3fffffffu
+#define PDX_BW_ACT 0x40000000u
+#define PDX_BW_FP 0x80000000u
+#define PDX_MAX_POINTS ((int64_t)1 << 30)
+
 // flags of the step kernels
 #define PDX_F_HAS_NEXT 1    // integrate to the next step (write positions of step n+1)
 #define PDX_F_STORE 2       // materialise v, a, b_int, damage, n_active of this step
@@ -37,6 +43,9 @@ struct pdx_gnn;  // learned preconditioner (pdx_gnn.hip)
 struct pdx_ctx {
     int device = 0;
     int precision = PDX_FP64;
+    // BB step shape: 10 * points per wavefront + bonds per lane in flight (tuning knob
+    // PDX_BB_VARIANT = 22 | 12 | 11)
+    int bb_variant = 22;
     hipStream_t stream = nullptr;
     std::string err;
 
@@ -68,10 +77,9 @@ struct pdx_ctx {
     int32_t *d_perm = nullptr, *d_iperm = nullptr;
     double *stage = nullptr;               // 9n staging for permuted transfers
     int64_t *row_ptr = nullptr;            // n + 1
-    int32_t *nbr = nullptr;                // nb
+    uint32_t *bw = nullptr;                // nb bond words: neighbour | active | fail_permit
     double *L = nullptr;                   // nb (FP64 layout)
     float *Lf = nullptr;                   // nb (mixed layout)
-    uint8_t *state = nullptr;              // nb: bit0 active, bit1 fail_permit
     bool bext_nonzero = false;
 
     // velocity boundary conditions
@@ -181,6 +189,65 @@ __device__ __forceinline__ float pdx_wave_sum_f(float x)
     return x;
 }
 
+// DPP move of a double (two 32-bit halves); CTRL is a DPP control word
+template <int CTRL>
+__device__ __forceinline__ double pdx_dpp(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+template <int CTRL>
+__device__ __forceinline__ int pdx_dpp_i(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
+}
+
+// sum over each row of 16 lanes, left in every lane of the row:
+// quad_perm(1,0,3,2), quad_perm(2,3,0,1), row_ror:4, row_ror:8
+__device__ __forceinline__ double pdx_row_sum(double x)
+{
+    x += pdx_dpp<0xB1>(x);
+    x += pdx_dpp<0x4E>(x);
+    x += pdx_dpp<0x124>(x);
+    x += pdx_dpp<0x128>(x);
+    return x;
+}
+
+__device__ __forceinline__ int pdx_row_sum_i(int x)
+{
+    x += pdx_dpp_i<0xB1>(x);
+    x += pdx_dpp_i<0x4E>(x);
+    x += pdx_dpp_i<0x124>(x);
+    x += pdx_dpp_i<0x128>(x);
+    return x;
+}
+
+__device__ __forceinline__ double pdx_readlane(double x, int l)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), l);
+    return __hiloint2double(hi, lo);
+}
+
+// sum over each half wave (lanes 0-31, 32-63), returned to the lanes of that half
+__device__ __forceinline__ double pdx_half_sum(double x, int half)
+{
+    x = pdx_row_sum(x);
+    const double r0 = pdx_readlane(x, 0), r1 = pdx_readlane(x, 16);
+    const double r2 = pdx_readlane(x, 32), r3 = pdx_readlane(x, 48);
+    return half ? (r2 + r3) : (r0 + r1);
+}
+
+__device__ __forceinline__ int pdx_half_sum_i(int x, int half)
+{
+    x = pdx_row_sum_i(x);
+    const int r0 = __builtin_amdgcn_readlane(x, 0), r1 = __builtin_amdgcn_readlane(x, 16);
+    const int r2 = __builtin_amdgcn_readlane(x, 32), r3 = __builtin_amdgcn_readlane(x, 48);
+    return half ? (r2 + r3) : (r0 + r1);
+}
+
 // arguments of the per-step kernels (shared by BB, OSB and CPD)
 struct pdx_step_args {
     int64_t n;
@@ -188,10 +255,9 @@ struct pdx_step_args {
     int flags;
     double rho, dt, dth;
     const int64_t *row_ptr;
-    const int32_t *nbr;
+    uint32_t *bw;
     const double *L;
     const float *Lf;
-    uint8_t *state;
     const double4 *rec_in;
     double4 *rec_out;
     const double4 *Xr;
@@ -203,7 +269,7 @@ struct pdx_step_args {
     const double *bcv;
     unsigned long long *flags_out;
     // material
-    double bc, epsc, delta, G, K, lambda, mu, beta_zem, maxdmg;
+    double bc, epsc, epsc_lo, epsc_hi, delta, G, K, lambda, mu, beta_zem, maxdmg;
     int model;
     double *pstate;
     uint8_t *killed;

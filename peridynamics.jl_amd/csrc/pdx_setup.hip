@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 
@@ -51,10 +52,9 @@ extern "C" const char *pdx_last_error(const pdx_ctx *c)
 static void free_bonds(pdx_ctx *c)
 {
     FREE(c->row_ptr);
-    FREE(c->nbr);
+    FREE(c->bw);
     FREE(c->L);
     FREE(c->Lf);
-    FREE(c->state);
 }
 
 void pdx_free_all(pdx_ctx *c)
@@ -99,6 +99,10 @@ extern "C" pdx_status pdx_create(pdx_ctx **out, int32_t device, int32_t precisio
     pdx_ctx *c = new pdx_ctx();
     c->device = device;
     c->precision = precision;
+    if (const char *e = getenv("PDX_BB_VARIANT")) {
+        int v = atoi(e);
+        if (v == 12 || v == 22 || v == 11) c->bb_variant = v;
+    }
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
         delete c;
@@ -131,7 +135,9 @@ extern "C" void pdx_destroy(pdx_ctx *c)
 extern "C" pdx_status pdx_synchronize(pdx_ctx *c)
 {
     if (!c) return PDX_ERR_ARG;
+    PDX_HIP(c, hipSetDevice(c->device));
     PDX_HIP(c, hipStreamSynchronize(c->stream));
+    PDX_HIP(c, hipDeviceSynchronize());
     return PDX_OK;
 }
 
@@ -140,7 +146,7 @@ extern "C" pdx_status pdx_set_points(pdx_ctx *c, int64_t n, const double *X, con
 {
     if (!c) return PDX_ERR_ARG;
     if (n <= 0 || !X || !vol) return pdx_fail(c, PDX_ERR_ARG, "invalid points (n=%lld)", (long long)n);
-    if (n >= (int64_t)INT32_MAX) return pdx_fail(c, PDX_ERR_UNSUPPORTED, "too many points");
+    if (n >= PDX_MAX_POINTS) return pdx_fail(c, PDX_ERR_UNSUPPORTED, "too many points");
     PDX_HIP(c, hipSetDevice(c->device));
     PDX_HIP(c, hipStreamSynchronize(c->stream));
     pdx_free_all(c);
@@ -252,9 +258,8 @@ struct search_args {
     const uint8_t *fp;        // fail_permit, internal order
     int32_t *counts;
     const int64_t *row_ptr;
-    int32_t *nbr;
+    uint32_t *bw;
     double *L;
-    uint8_t *state;
     unsigned long long *flags;
 };
 
@@ -361,30 +366,31 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_fill_bonds(search_args s)
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     int cnt = min(base, want);
     for (int k = lane; k < cnt; k += 64) {
-        int32_t ok = co[w][k];
+        // rows are ordered by the neighbour's internal (Morton) index, so that the lanes of
+        // a gather read neighbouring records; pdx_get_bonds reorders rows by caller index
+        int32_t ok = cj[w][k];
         int rank = 0;
-        for (int m2 = 0; m2 < cnt; m2++) rank += (co[w][m2] < ok) ? 1 : 0;
+        for (int m2 = 0; m2 < cnt; m2++) rank += (cj[w][m2] < ok) ? 1 : 0;
         int32_t j = cj[w][k];
         double4 xj = s.Xr[j];
         double d2 = pair_d2(xi, xj);
         uint32_t h = s.cmask ? crack_hits(mi, s.cmask[j]) : 0u;
-        uint8_t fpb = (uint8_t)(s.fp[i] & s.fp[j]);
-        uint8_t act = (h & ~s.filt_bits) ? 0 : 1;  // broken at t = 0 by a precrack
+        const uint32_t fpb = (s.fp[i] & s.fp[j]) ? PDX_BW_FP : 0u;
+        const uint32_t act = (h & ~s.filt_bits) ? 0u : PDX_BW_ACT;  // broken at t = 0 by a precrack
         int64_t b = b0 + rank;
-        s.nbr[b] = j;
+        s.bw[b] = (uint32_t)j | act | fpb;
         s.L[b] = sqrt(d2);
-        s.state[b] = (uint8_t)(act | (fpb << 1));
     }
 }
 
-__global__ void k_init_point_state(int64_t n, const int64_t *row_ptr, const uint8_t *state,
+__global__ void k_init_point_state(int64_t n, const int64_t *row_ptr, const uint32_t *bw,
                                    int32_t *nact, double *damage)
 {
     int64_t i = pdx_wave_point();
     if (i >= n) return;
     int lane = threadIdx.x & 63;
     int cnt = 0;
-    for (int64_t b = row_ptr[i] + lane; b < row_ptr[i + 1]; b += 64) cnt += state[b] & 1;
+    for (int64_t b = row_ptr[i] + lane; b < row_ptr[i + 1]; b += 64) cnt += (bw[b] & PDX_BW_ACT) ? 1 : 0;
     cnt = pdx_wave_sum_i(cnt);
     if (lane == 0) {
         nact[i] = cnt;
@@ -643,18 +649,16 @@ extern "C" pdx_status pdx_build_bonds(pdx_ctx *c, int64_t *n_bonds_out)
     }
     c->nb = nb;
     TRY_C(pdx_alloc(c, &c->row_ptr, n + 1));
-    TRY_C(pdx_alloc(c, &c->nbr, nb));
+    TRY_C(pdx_alloc(c, &c->bw, nb));
     TRY_C(pdx_alloc(c, &c->L, nb));
-    TRY_C(pdx_alloc(c, &c->state, nb));
     HIP_C(hipMemcpy(c->row_ptr, row_ptr.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice));
     s.row_ptr = c->row_ptr;
-    s.nbr = c->nbr;
+    s.bw = c->bw;
     s.L = c->L;
-    s.state = c->state;
     hipLaunchKernelGGL(k_fill_bonds, dim3(grid), dim3(PDX_BLOCK), 0, c->stream, s);
     HIP_C(hipGetLastError());
     hipLaunchKernelGGL(k_init_point_state, dim3(grid), dim3(PDX_BLOCK), 0, c->stream, n,
-                       c->row_ptr, c->state, c->nact, c->damage);
+                       c->row_ptr, c->bw, c->nact, c->damage);
     HIP_C(hipGetLastError());
     if (c->precision == PDX_MIXED) {
         TRY_C(pdx_alloc(c, &c->Lf, nb));
@@ -697,14 +701,12 @@ extern "C" pdx_status pdx_get_bonds(pdx_ctx *c, int64_t *row_ptr, int32_t *nbr, 
     if (!c->bonds_built) return pdx_fail(c, PDX_ERR_STATE, "pdx_build_bonds first");
     const int64_t n = c->n, nb = c->nb;
     std::vector<int64_t> rp(n + 1);
-    std::vector<int32_t> nb_dev(nb);
+    std::vector<uint32_t> w(nb);
     std::vector<double> L_dev(L ? nb : 0);
-    std::vector<uint8_t> st(nb);
     PDX_HIP(c, hipStreamSynchronize(c->stream));
     PDX_HIP(c, hipMemcpy(rp.data(), c->row_ptr, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost));
-    PDX_HIP(c, hipMemcpy(nb_dev.data(), c->nbr, sizeof(int32_t) * nb, hipMemcpyDeviceToHost));
+    PDX_HIP(c, hipMemcpy(w.data(), c->bw, sizeof(uint32_t) * nb, hipMemcpyDeviceToHost));
     if (L) PDX_HIP(c, hipMemcpy(L_dev.data(), c->L, sizeof(double) * nb, hipMemcpyDeviceToHost));
-    PDX_HIP(c, hipMemcpy(st.data(), c->state, nb, hipMemcpyDeviceToHost));
     std::vector<uint8_t> killed(n, 0);
     if (c->kind == PDX_CPD)
         PDX_HIP(c, hipMemcpy(killed.data(), c->killed, n, hipMemcpyDeviceToHost));
@@ -715,14 +717,22 @@ extern "C" pdx_status pdx_get_bonds(pdx_ctx *c, int64_t *row_ptr, int32_t *nbr, 
         out_rp[i + 1] = out_rp[i] + (rp[m + 1] - rp[m]);
     }
     if (row_ptr) std::memcpy(row_ptr, out_rp.data(), sizeof(int64_t) * (n + 1));
+    // device rows are in Morton order of the neighbours; the caller's rows list neighbours
+    // by ascending caller index (the reference's bond order within a grid cell)
+    std::vector<std::pair<int32_t, int64_t>> row;
     for (int64_t i = 0; i < n; i++) {
         int64_t m = c->iperm[i];
         int64_t o = out_rp[i];
-        for (int64_t b = rp[m]; b < rp[m + 1]; b++, o++) {
-            if (nbr) nbr[o] = c->perm[nb_dev[b]];
+        row.clear();
+        for (int64_t b = rp[m]; b < rp[m + 1]; b++) row.push_back({c->perm[w[b] & PDX_BW_NBR], b});
+        std::sort(row.begin(), row.end());
+        for (auto &e : row) {
+            const int64_t b = e.second;
+            if (nbr) nbr[o] = e.first;
             if (L) L[o] = L_dev[b];
-            if (fp) fp[o] = (st[b] >> 1) & 1;
-            if (active) active[o] = (st[b] & 1) && !killed[m];
+            if (fp) fp[o] = (w[b] & PDX_BW_FP) ? 1 : 0;
+            if (active) active[o] = ((w[b] & PDX_BW_ACT) && !killed[m]) ? 1 : 0;
+            o++;
         }
     }
     return PDX_OK;
@@ -811,17 +821,43 @@ extern "C" pdx_status pdx_set_field(pdx_ctx *c, int32_t field, const double *hos
 }
 
 // ----------------------------------------------------------------------------------------
-// stable time step: calc_timestep_point (bond_system.jl:269-277), summed in bond order by
-// one lane so that the value equals the reference's bit for bit
+// stable time step: calc_timestep_point (bond_system.jl:269-277).  The device rows are in
+// Morton order of the neighbours; the terms are ranked by the neighbour's caller index in
+// LDS and summed in that order by one lane, so that the value equals the reference's bit
+// for bit.
 // ----------------------------------------------------------------------------------------
-__global__ void k_timestep(int64_t n, const int64_t *row_ptr, const int32_t *nbr,
-                           const double *L, const double4 *Xr, double bc, double rho, double *dt)
+__global__ __launch_bounds__(PDX_BLOCK) void k_timestep(int64_t n, const int64_t *row_ptr,
+                                                         const uint32_t *bw, const double *L,
+                                                         const double4 *Xr, const int32_t *perm,
+                                                         double bc, double rho, double *dt)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double term[PDX_WAVES_PER_BLOCK][PDX_FILL_CAP];
+    __shared__ int32_t key[PDX_WAVES_PER_BLOCK][PDX_FILL_CAP];
+    const int64_t i = pdx_wave_point();
     if (i >= n) return;
-    double dtsum = 0.0;
-    for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) dtsum += Xr[nbr[b]].w * bc / L[b];
-    dt[i] = sqrt(2.0 * rho / dtsum);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t b0 = row_ptr[i];
+    const int cnt = (int)(row_ptr[i + 1] - b0);  // <= PDX_FILL_CAP (checked when built)
+    for (int k = lane; k < cnt; k += 64) {
+        const uint32_t j = bw[b0 + k] & PDX_BW_NBR;
+        key[w][k] = perm[j];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (int k = lane; k < cnt; k += 64) {
+        const uint32_t j = bw[b0 + k] & PDX_BW_NBR;
+        const int32_t kk = key[w][k];
+        int rank = 0;
+        for (int m = 0; m < cnt; m++) rank += (key[w][m] < kk) ? 1 : 0;
+        term[w][rank] = Xr[j].w * bc / L[b0 + k];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) {
+        double dtsum = 0.0;
+        for (int k = 0; k < cnt; k++) dtsum += term[w][k];
+        dt[i] = sqrt(2.0 * rho / dtsum);
+    }
 }
 
 extern "C" pdx_status pdx_stable_timestep(pdx_ctx *c, double safety, double *dt_out)
@@ -832,8 +868,9 @@ extern "C" pdx_status pdx_stable_timestep(pdx_ctx *c, double safety, double *dt_
         return pdx_fail(c, PDX_ERR_ARG, "`safety_factor` of VelocityVerlet has invalid value!");
     PDX_HIP(c, hipSetDevice(c->device));
     double *tmp = c->stage;
-    hipLaunchKernelGGL(k_timestep, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->n,
-                       c->row_ptr, c->nbr, c->L, c->Xr, c->P.bc, c->P.rho, tmp);
+    hipLaunchKernelGGL(k_timestep, dim3((unsigned)pdx_wave_grid(c->n)), dim3(PDX_BLOCK), 0,
+                       c->stream, c->n, c->row_ptr, c->bw, c->L, c->Xr, c->d_perm, c->P.bc,
+                       c->P.rho, tmp);
     PDX_HIP(c, hipGetLastError());
     std::vector<double> dt(c->n);
     PDX_HIP(c, hipMemcpyAsync(dt.data(), tmp, sizeof(double) * c->n, hipMemcpyDeviceToHost,

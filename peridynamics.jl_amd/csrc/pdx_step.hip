@@ -16,6 +16,7 @@
 // (apply_boundary_conditions!, boundary_conditions.jl:212-226) and u^{n+1}, x^{n+1}
 // (update_disp_and_pos!, :247-255) into the other position buffer.  A prologue launch does
 // the first half step, so a run of N steps is N + 1 launches.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -23,22 +24,23 @@
 
 // ----------------------------------------------------------------------------------------
 // per-point tail: NaN flag (check_for_nans, nans.jl:10-18), acceleration, velocity and the
-// first half of the next step.  Lanes 0..2 handle the x, y, z dofs.
+// first half of the next step.  `sub` is the lane index within the point's lane group;
+// sub 0..2 handle the x, y, z dofs.
 // ----------------------------------------------------------------------------------------
 template <bool POINT_STATE>
-__device__ __forceinline__ void step_tail(const pdx_step_args &A, int64_t i, int lane,
+__device__ __forceinline__ void step_tail(const pdx_step_args &A, int64_t i, int sub,
                                           double4 ri, double bx, double by, double bz, int na,
                                           int64_t nn)
 {
-    if (POINT_STATE && lane == 0 && (A.flags & (PDX_F_STORE | PDX_F_FORCE_ONLY))) {
+    if (POINT_STATE && sub == 0 && (A.flags & (PDX_F_STORE | PDX_F_FORCE_ONLY))) {
         A.nact_out[i] = na;
         // calc_damage!: bond_system.jl:338-343
         A.dmg_out[i] = 1.0 - (double)na / (double)nn;
     }
-    if (lane < 3) {
-        const double bd = lane == 0 ? bx : (lane == 1 ? by : bz);
+    if (sub < 3) {
+        const double bd = sub == 0 ? bx : (sub == 1 ? by : bz);
         if (isnan(bd)) atomicMin(A.flags_out, (unsigned long long)A.step);
-        const int64_t k = 3 * i + lane;
+        const int64_t k = 3 * i + sub;
         if (A.flags & PDX_F_FORCE_ONLY) {
             A.bint_out[k] = (A.flags & PDX_F_SCALE_VOL) ? bd * ri.w : bd;
             return;
@@ -55,7 +57,7 @@ __device__ __forceinline__ void step_tail(const pdx_step_args &A, int64_t i, int
             double vn = vd + ad * A.dth;
             const int32_t slot = A.bcslot[i];
             if (slot >= 0) {
-                const int32_t s = A.slot_src[3 * slot + lane];
+                const int32_t s = A.slot_src[3 * slot + sub];
                 if (s >= 0) {
                     const double val = A.bcv[s];
                     if (isfinite(val)) vn = val;
@@ -63,85 +65,251 @@ __device__ __forceinline__ void step_tail(const pdx_step_args &A, int64_t i, int
             }
             const double du = vn * A.dt;
             A.u[k] = A.u[k] + du;
-            const double xd = (lane == 0 ? ri.x : (lane == 1 ? ri.y : ri.z)) + du;
-            reinterpret_cast<double *>(A.rec_out)[4 * i + lane] = xd;
+            const double xd = (sub == 0 ? ri.x : (sub == 1 ? ri.y : ri.z)) + du;
+            reinterpret_cast<double *>(A.rec_out)[4 * i + sub] = xd;
             A.vh[k] = vn;
         }
-    } else if (lane == 3 && (A.flags & PDX_F_HAS_NEXT) && !(A.flags & PDX_F_FORCE_ONLY)) {
+    } else if (sub == 3 && (A.flags & PDX_F_HAS_NEXT) && !(A.flags & PDX_F_FORCE_ONLY)) {
         reinterpret_cast<double *>(A.rec_out)[4 * i + 3] = ri.w;
     }
+}
+
+// 1/x to about 1 ulp: hardware reciprocal and two Newton steps
+__device__ __forceinline__ double pdx_rcp(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
 }
 
 // ----------------------------------------------------------------------------------------
 // BB: calc_failure! (bond_based.jl:125-144) + calc_damage! + force_density_point!
 // (bond_based.jl:146-161) + the fused tail.
-// FP64: l = sqrt((dx*dx + dy*dy) + dz*dz) and eps = (l - L) / L are correctly rounded, so the
-// breakage decision equals the reference's.  The force term uses s/l * dx instead of
-// (s*dx)/l (one division instead of three; within 2 ulp of the reference term).
+//
+// PPW points per wavefront (64/PPW lanes per point).  A lane takes the bonds h, h + LPP, ...,
+// h + (U-1) LPP of its point per chunk (h = lane index within the point's lanes) and issues
+// all their loads and neighbour gathers before any arithmetic.  The operands of the tail
+// (u, v_half, b_ext, BC slot) are loaded before the bond loop so that their latency
+// overlaps it.
+//
+// FP64: l = sqrt((dx*dx + dy*dy) + dz*dz) is correctly rounded.  The stretch used in the
+// force is (l - L) * (1/L) (within a few ulp of the reference's (l - L)/L); the breakage
+// test eps > eps_c uses the correctly rounded quotient whenever the two could disagree
+// (|eps - eps_c| <= 1e-11 |eps_c|), so bonds break exactly when the reference's break.  The
+// force term is ((w bc) eps V_j) * (1/l), accumulated with dx by fused multiply-adds (the
+// reference divides each component by l and adds).  An inactive bond has w = 0, as the
+// reference's Bool * factor, so NaN positions still show.
 // MIXED: the stretch comes from FP64 differences du = (x_j - x_i) - (X_j - X_i), then
-// eps = (2 dX.du + du.du) / (L (l + L)) in FP32, force in FP32, sums in FP64.
+// eps = (2 dX.du + du.du) / (L (l + L)) in FP32 and the force in FP32, summed in FP64.
 // ----------------------------------------------------------------------------------------
-template <bool MIXED>
-__global__ __launch_bounds__(PDX_BLOCK) void k_bb_step(pdx_step_args A)
+__device__ __forceinline__ double pdx_wave_total(double x)
 {
-    const int64_t i = pdx_wave_point();
-    if (i >= A.n) return;
-    const int lane = threadIdx.x & 63;
-    const double4 ri = A.rec_in[i];
-    double4 Xi = make_double4(0, 0, 0, 0);
-    if (MIXED) Xi = A.Xr[i];
-    const int64_t b0 = A.row_ptr[i], b1 = A.row_ptr[i + 1];
-    double bx = 0.0, by = 0.0, bz = 0.0;
-    int na = 0;
-    const float bcf = (float)A.bc, epscf = (float)A.epsc;
-    for (int64_t b = b0 + lane; b < b1; b += 64) {
-        const int32_t j = A.nbr[b];
-        const uint8_t st = A.state[b];
-        const double4 rj = A.rec_in[j];
-        const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
-        bool act = st & 1;
-        if (!MIXED) {
-            const double Lb = A.L[b];
-            const double l = sqrt((dx * dx + dy * dy) + dz * dz);
-            const double eps = (l - Lb) / Lb;
-            if (act && (st & 2) && eps > A.epsc) {
+    x = pdx_row_sum(x);
+    return (pdx_readlane(x, 0) + pdx_readlane(x, 16)) + (pdx_readlane(x, 32) + pdx_readlane(x, 48));
+}
+
+__device__ __forceinline__ int pdx_wave_total_i(int x)
+{
+    x = pdx_row_sum_i(x);
+    return (__builtin_amdgcn_readlane(x, 0) + __builtin_amdgcn_readlane(x, 16)) +
+           (__builtin_amdgcn_readlane(x, 32) + __builtin_amdgcn_readlane(x, 48));
+}
+
+// sqrt(d2) correctly rounded for a normal, finite d2 > 0 (the Markstein sequence of the
+// compiler's sqrt without its range scaling: bond lengths never come near the subnormal
+// range), plus 1/sqrt(d2) to about 1 ulp as a by-product.
+__device__ __forceinline__ double pdx_sqrt_rsq(double d2, double &rl)
+{
+    const double r = __builtin_amdgcn_rsq(d2);
+    double g = d2 * r;
+    double h = 0.5 * r;
+    const double e = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, e, g);
+    h = __builtin_fma(h, e, h);
+    double d = __builtin_fma(-g, g, d2);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, d2);
+    rl = h + h;
+    return __builtin_fma(d, h, g);
+}
+
+// 1/x with one Newton step (relative error ~2^-50)
+__device__ __forceinline__ double pdx_rcp1(double x)
+{
+    const double r = __builtin_amdgcn_rcp(x);
+    return __builtin_fma(__builtin_fma(-x, r, 1.0), r, r);
+}
+
+// One bond of the BB force density (see k_bb_step for the arithmetic contract).
+template <bool MIXED>
+__device__ __forceinline__ void bb_bond(const pdx_step_args &A, int64_t b, uint32_t wd, double Ld,
+                                        float Lfl, double4 rj, double4 Xj, double4 ri,
+                                        double4 Xi, double &bx, double &by, double &bz, int &na)
+{
+    const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
+    bool act = (wd & PDX_BW_ACT) != 0;
+    if (!MIXED) {
+        const double d2 = (dx * dx + dy * dy) + dz * dz;
+        double rl;
+        const double l = pdx_sqrt_rsq(d2, rl);
+        const double e = l - Ld;
+        const double eps = e * pdx_rcp1(Ld);
+        // breakage: eps > eps_c with the correctly rounded quotient wherever it could differ
+        if (act && (wd & PDX_BW_FP) && eps >= A.epsc_lo) {
+            const bool brk = (eps > A.epsc_hi) ? true : ((e / Ld) > A.epsc);
+            if (brk) {
                 act = false;
-                A.state[b] = st & 2;
-            }
-            if (act) {
-                const double s = ((A.bc * eps) * rj.w) / l;
-                bx += s * dx;
-                by += s * dy;
-                bz += s * dz;
-                na++;
-            }
-        } else {
-            const float Lb = A.Lf[b];
-            const double4 Xj = A.Xr[j];
-            const double DXd = Xj.x - Xi.x, DYd = Xj.y - Xi.y, DZd = Xj.z - Xi.z;
-            const float ux = (float)(dx - DXd), uy = (float)(dy - DYd), uz = (float)(dz - DZd);
-            const float DX = (float)DXd, DY = (float)DYd, DZ = (float)DZd;
-            const float num = 2.0f * ((DX * ux + DY * uy) + DZ * uz) + ((ux * ux + uy * uy) + uz * uz);
-            const float l = __fsqrt_rn(Lb * Lb + num);
-            const float eps = __fdividef(num, Lb * (l + Lb));
-            if (act && (st & 2) && eps > epscf) {
-                act = false;
-                A.state[b] = st & 2;
-            }
-            if (act) {
-                const float s = __fdividef((bcf * eps) * (float)rj.w, l);
-                bx += (double)(s * (float)dx);
-                by += (double)(s * (float)dy);
-                bz += (double)(s * (float)dz);
-                na++;
+                A.bw[b] = wd & ~PDX_BW_ACT;
             }
         }
+        const double wbc = act ? A.bc : 0.0;  // (w * bc) with w = bond_active
+        const double s = ((wbc * eps) * rj.w) * rl;
+        bx = __builtin_fma(s, dx, bx);
+        by = __builtin_fma(s, dy, by);
+        bz = __builtin_fma(s, dz, bz);
+    } else {
+        const double DXd = Xj.x - Xi.x, DYd = Xj.y - Xi.y, DZd = Xj.z - Xi.z;
+        const float ux = (float)(dx - DXd), uy = (float)(dy - DYd), uz = (float)(dz - DZd);
+        const float DX = (float)DXd, DY = (float)DYd, DZ = (float)DZd;
+        const float num = 2.0f * ((DX * ux + DY * uy) + DZ * uz) + ((ux * ux + uy * uy) + uz * uz);
+        const float l = __fsqrt_rn(Lfl * Lfl + num);
+        const float eps = __fdividef(num, Lfl * (l + Lfl));
+        if (act && (wd & PDX_BW_FP) && eps > (float)A.epsc) {
+            act = false;
+            A.bw[b] = wd & ~PDX_BW_ACT;
+        }
+        const float wbc = act ? (float)A.bc : 0.0f;
+        const float sf = __fdividef((wbc * eps) * (float)rj.w, l);
+        bx += (double)(sf * (float)dx);
+        by += (double)(sf * (float)dy);
+        bz += (double)(sf * (float)dz);
     }
-    bx = pdx_wave_sum(bx);
-    by = pdx_wave_sum(by);
-    bz = pdx_wave_sum(bz);
-    na = pdx_wave_sum_i(na);
-    step_tail<true>(A, i, lane, ri, bx, by, bz, na, b1 - b0);
+    na += act ? 1 : 0;
+}
+
+template <bool MIXED, int PPW, int U>
+__global__ __launch_bounds__(PDX_BLOCK) void k_bb_step(pdx_step_args A)
+{
+    constexpr int LPP = 64 / PPW;
+    const int lane = threadIdx.x & 63;
+    const int grp = (PPW == 2) ? (lane >> 5) : 0;
+    const int h = lane & (LPP - 1);
+    const int64_t w = pdx_wave_point();
+    if ((int64_t)PPW * w >= A.n) return;
+    const int64_t i = PPW * w + grp;
+    const bool valid = i < A.n;
+    const int flags = A.flags;
+    int64_t b0 = 0, b1 = 0;
+    double4 ri = make_double4(0, 0, 0, 0), Xi = make_double4(0, 0, 0, 0);
+    double u_d = 0.0, vh_d = 0.0, be_d = 0.0;
+    int32_t slot = -1;
+    if (valid) {
+        b0 = A.row_ptr[i];
+        b1 = A.row_ptr[i + 1];
+        ri = A.rec_in[i];
+        if (MIXED) Xi = A.Xr[i];
+        if (h < 3 && !(flags & PDX_F_FORCE_ONLY)) {
+            const int64_t k = 3 * i + h;
+            vh_d = A.vh[k];
+            if (flags & PDX_F_HAS_NEXT) {
+                u_d = A.u[k];
+                slot = A.bcslot[i];
+            }
+            if (flags & PDX_F_BEXT) be_d = A.bext[k];
+        }
+    }
+    double bx = 0.0, by = 0.0, bz = 0.0;
+    int na = 0;
+    for (int64_t base = b0 + h; base < b1; base += LPP * U) {
+        uint32_t wd[U];
+        double Ld[U];
+        float Lfl[U];
+        double4 rj[U], Xj[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            Ld[u] = 0.0;
+            Lfl[u] = 0.0f;
+            Xj[u] = make_double4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t b = base + LPP * u;
+            wd[u] = 0;
+            if (b < b1) {
+                wd[u] = A.bw[b];
+                if (MIXED) Lfl[u] = A.Lf[b];
+                else Ld[u] = A.L[b];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (base + LPP * u < b1) {
+                const uint32_t j = wd[u] & PDX_BW_NBR;
+                rj[u] = A.rec_in[j];
+                if (MIXED) Xj[u] = A.Xr[j];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t b = base + LPP * u;
+            if (b < b1)
+                bb_bond<MIXED>(A, b, wd[u], MIXED ? 0.0 : Ld[u], MIXED ? Lfl[u] : 0.0f, rj[u],
+                               MIXED ? Xj[u] : make_double4(0, 0, 0, 0), ri, Xi, bx, by, bz, na);
+        }
+    }
+    if (PPW == 2) {
+        bx = pdx_half_sum(bx, grp);
+        by = pdx_half_sum(by, grp);
+        bz = pdx_half_sum(bz, grp);
+        na = pdx_half_sum_i(na, grp);
+    } else {
+        bx = pdx_wave_total(bx);
+        by = pdx_wave_total(by);
+        bz = pdx_wave_total(bz);
+        na = pdx_wave_total_i(na);
+    }
+    if (!valid) return;
+    // tail: calc_damage!, check_for_nans, _update_acc_and_vel!, then the first half of the
+    // next step (update_vel_half!, apply_boundary_conditions!, update_disp_and_pos!)
+    if (h == 0 && (flags & (PDX_F_STORE | PDX_F_FORCE_ONLY))) {
+        A.nact_out[i] = na;
+        A.dmg_out[i] = 1.0 - (double)na / (double)(b1 - b0);
+    }
+    if (h < 3) {
+        const double bd = h == 0 ? bx : (h == 1 ? by : bz);
+        if (isnan(bd)) atomicMin(A.flags_out, (unsigned long long)A.step);
+        const int64_t k = 3 * i + h;
+        if (flags & PDX_F_FORCE_ONLY) {
+            A.bint_out[k] = (flags & PDX_F_SCALE_VOL) ? bd * ri.w : bd;
+            return;
+        }
+        const double ad = (bd + be_d) / A.rho;
+        const double vd = vh_d + ad * A.dth;
+        if (flags & PDX_F_STORE) {
+            A.v_out[k] = vd;
+            A.a_out[k] = ad;
+            A.bint_out[k] = bd;
+        }
+        if (flags & PDX_F_HAS_NEXT) {
+            double vn = vd + ad * A.dth;
+            if (slot >= 0) {
+                const int32_t src = A.slot_src[3 * slot + h];
+                if (src >= 0) {
+                    const double val = A.bcv[src];
+                    if (isfinite(val)) vn = val;
+                }
+            }
+            const double du = vn * A.dt;
+            A.u[k] = u_d + du;
+            const double xd = (h == 0 ? ri.x : (h == 1 ? ri.y : ri.z)) + du;
+            reinterpret_cast<double *>(A.rec_out)[4 * i + h] = xd;
+            A.vh[k] = vn;
+        }
+    } else if (h == 3 && (flags & PDX_F_HAS_NEXT) && !(flags & PDX_F_FORCE_ONLY)) {
+        reinterpret_cast<double *>(A.rec_out)[4 * i + 3] = ri.w;
+    }
 }
 
 // ----------------------------------------------------------------------------------------
@@ -159,17 +327,17 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_osb_pass1(pdx_step_args A)
     double wvol = 0.0, S = 0.0;
     int na = 0;
     for (int64_t b = b0 + lane; b < b1; b += 64) {
-        const int32_t j = A.nbr[b];
-        const uint8_t st = A.state[b];
+        const uint32_t wd = A.bw[b];
+        const uint32_t j = wd & PDX_BW_NBR;
         const double4 rj = A.rec_in[j];
         const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
         const double Lb = A.L[b];
         const double l = sqrt((dx * dx + dy * dy) + dz * dz);
         const double eps = (l - Lb) / Lb;
-        bool act = st & 1;
-        if (act && (st & 2) && eps > A.epsc) {
+        bool act = (wd & PDX_BW_ACT) != 0;
+        if (act && (wd & PDX_BW_FP) && eps > A.epsc) {
             act = false;
-            A.state[b] = st & 2;
+            A.bw[b] = wd & ~PDX_BW_ACT;
         }
         if (act) {
             const double w = A.delta / Lb;  // linear_kernel
@@ -209,9 +377,9 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_osb_pass2(pdx_step_args A)
     const int64_t b0 = A.row_ptr[i], b1 = A.row_ptr[i + 1];
     double bx = 0.0, by = 0.0, bz = 0.0;
     for (int64_t b = b0 + lane; b < b1; b += 64) {
-        const uint8_t st = A.state[b];
-        if (!(st & 1)) continue;
-        const int32_t j = A.nbr[b];
+        const uint32_t wd = A.bw[b];
+        if (!(wd & PDX_BW_ACT)) continue;
+        const uint32_t j = wd & PDX_BW_NBR;
         const double4 rj = A.rec_in[j];
         const double2 cj = reinterpret_cast<const double2 *>(A.pstate)[j];
         const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
@@ -340,8 +508,8 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_cpd_pass1(pdx_step_args A)
     double Ft[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     int na = 0;
     for (int64_t b = b0 + lane; b < b1; b += 64) {
-        const int32_t j = A.nbr[b];
-        const uint8_t st = A.state[b];
+        const uint32_t wd = A.bw[b];
+        const uint32_t j = wd & PDX_BW_NBR;
         const double4 rj = A.rec_in[j];
         const double4 Xj = A.Xr[j];
         const double dx[3] = {rj.x - ri.x, rj.y - ri.y, rj.z - ri.z};
@@ -349,10 +517,10 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_cpd_pass1(pdx_step_args A)
         const double Lb = A.L[b];
         const double l = sqrt((dx[0] * dx[0] + dx[1] * dx[1]) + dx[2] * dx[2]);
         const double eps = (l - Lb) / Lb;
-        bool act = st & 1;
-        if (act && (st & 2) && eps > A.epsc) {
+        bool act = (wd & PDX_BW_ACT) != 0;
+        if (act && (wd & PDX_BW_FP) && eps > A.epsc) {
             act = false;
-            A.state[b] = st & 2;
+            A.bw[b] = wd & ~PDX_BW_ACT;
         }
         if (act && !dead) {
             na++;
@@ -433,9 +601,9 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_cpd_pass2(pdx_step_args A)
     const int64_t b0 = A.row_ptr[i], b1 = A.row_ptr[i + 1];
     double b3[3] = {0.0, 0.0, 0.0};
     for (int64_t b = b0 + lane; b < b1; b += 64) {
-        const uint8_t st = A.state[b];
-        if (!(st & 1)) continue;
-        const int32_t j = A.nbr[b];
+        const uint32_t wd = A.bw[b];
+        if (!(wd & PDX_BW_ACT)) continue;
+        const uint32_t j = wd & PDX_BW_NBR;
         const double4 rj = A.rec_in[j];
         const double4 Xj = A.Xr[j];
         const double dx[3] = {rj.x - ri.x, rj.y - ri.y, rj.z - ri.z};
@@ -516,10 +684,9 @@ pdx_status pdx_fill_step_args(pdx_ctx *c, pdx_step_args *a, int64_t step, int fl
     a->dt = dt;
     a->dth = 0.5 * dt;
     a->row_ptr = c->row_ptr;
-    a->nbr = c->nbr;
+    a->bw = c->bw;
     a->L = c->L;
     a->Lf = c->Lf;
-    a->state = c->state;
     a->rec_in = c->rec[c->cur];
     a->rec_out = c->rec[c->cur ^ 1];
     a->Xr = c->Xr;
@@ -537,6 +704,11 @@ pdx_status pdx_fill_step_args(pdx_ctx *c, pdx_step_args *a, int64_t step, int fl
     a->flags_out = c->d_flags;
     a->bc = c->P.bc;
     a->epsc = c->P.epsc;
+    // (l - L) * (1/L) with a one-step reciprocal is within ~1e-13 of the correctly rounded
+    // (l - L) / L; outside [epsc_lo, epsc_hi] the two compare equal against eps_c, inside
+    // the kernel takes the correctly rounded quotient
+    a->epsc_lo = c->P.epsc - 1e-11 * fabs(c->P.epsc);
+    a->epsc_hi = c->P.epsc + 1e-11 * fabs(c->P.epsc);
     a->delta = c->P.delta;
     a->G = c->P.G;
     a->K = c->P.K;
@@ -556,16 +728,25 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
 {
     const int64_t grid = pdx_wave_grid(c->n);
     const dim3 G((unsigned)grid), B(PDX_BLOCK);
+    const dim3 G2((unsigned)pdx_wave_grid((c->n + 1) / 2));
     switch (c->kind) {
-    case PDX_BB:
+    case PDX_BB: {
         PDX_TRY(pdx_timed_begin(c));
-        if (c->precision == PDX_MIXED)
-            hipLaunchKernelGGL(k_bb_step<true>, G, B, 0, c->stream, a);
-        else
-            hipLaunchKernelGGL(k_bb_step<false>, G, B, 0, c->stream, a);
+        const int v = c->bb_variant;
+        const dim3 G1((unsigned)pdx_wave_grid(c->n));
+        if (c->precision == PDX_MIXED) {
+            if (v == 22) hipLaunchKernelGGL((k_bb_step<true, 2, 2>), G2, B, 0, c->stream, a);
+            else if (v == 11) hipLaunchKernelGGL((k_bb_step<true, 1, 1>), G1, B, 0, c->stream, a);
+            else hipLaunchKernelGGL((k_bb_step<true, 1, 2>), G1, B, 0, c->stream, a);
+        } else {
+            if (v == 22) hipLaunchKernelGGL((k_bb_step<false, 2, 2>), G2, B, 0, c->stream, a);
+            else if (v == 11) hipLaunchKernelGGL((k_bb_step<false, 1, 1>), G1, B, 0, c->stream, a);
+            else hipLaunchKernelGGL((k_bb_step<false, 1, 2>), G1, B, 0, c->stream, a);
+        }
         PDX_HIP(c, hipGetLastError());
         PDX_TRY(pdx_timed_end(c, "step"));
         break;
+    }
     case PDX_OSB:
         PDX_TRY(pdx_timed_begin(c));
         hipLaunchKernelGGL(k_osb_pass1, G, B, 0, c->stream, a);

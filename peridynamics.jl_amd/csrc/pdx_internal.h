@@ -24,6 +24,9 @@
 #define PDX_BW_ACT 0x40000000u
 #define PDX_BW_FP 0x80000000u
 #define PDX_MAX_POINTS ((int64_t)1 << 30)
+#define PDX_W16_SLOT 0x3fffu
+#define PDX_W16_ACT 0x4000u
+#define PDX_W16_FP 0x8000u
 
 // flags of the step kernels
 #define PDX_F_HAS_NEXT 1    // integrate to the next step (write positions of step n+1)
@@ -43,9 +46,8 @@ struct pdx_gnn;  // learned preconditioner (pdx_gnn.hip)
 struct pdx_ctx {
     int device = 0;
     int precision = PDX_FP64;
-    // BB step shape: 10 * points per wavefront + bonds per lane in flight (tuning knob
-    // PDX_BB_VARIANT = 22 | 12 | 11)
-    int bb_variant = 22;
+    // bonds per iteration of the BB thread (tuning knob PDX_BB_UNROLL = 2 | 4)
+    int bb_unroll = 2;
     hipStream_t stream = nullptr;
     std::string err;
 
@@ -76,10 +78,21 @@ struct pdx_ctx {
     double *defgrad = nullptr, *cauchy = nullptr;  // 9n (CPD)
     int32_t *d_perm = nullptr, *d_iperm = nullptr;
     double *stage = nullptr;               // 9n staging for permuted transfers
-    int64_t *row_ptr = nullptr;            // n + 1
-    uint32_t *bw = nullptr;                // nb bond words: neighbour | active | fail_permit
-    double *L = nullptr;                   // nb (FP64 layout)
-    float *Lf = nullptr;                   // nb (mixed layout)
+    // bonds in sliced ELL (SELL-64): bond t of point p at slice_ptr[p / 64] + 64 t + p % 64
+    int64_t *slice_ptr = nullptr;          // n_slices + 1
+    int32_t *cnt = nullptr;                // n: bonds of each point
+    int64_t nb_alloc = 0;                  // entries including the padding of the slices
+    uint32_t *bw = nullptr;                // bond words: neighbour | active | fail_permit
+    double *L = nullptr;                   // reference lengths (FP64 layout)
+    float *Lf = nullptr;                   // reference lengths (mixed layout)
+    // LDS-staged BB step: per block of PDX_BLOCK points, the sorted list of every point
+    // its bonds reach (the halo); bonds then address the halo with 16-bit words
+    // (bits 0-13 slot, bit 14 active, bit 15 fail_permit)
+    bool lds_mode = false;
+    uint16_t *bw16 = nullptr;              // nb_alloc
+    int64_t *halo_ptr = nullptr;           // n_blocks + 1
+    int32_t *halo = nullptr;               // halo entries (internal point indices)
+    int32_t halo_max = 0;                  // largest halo of a block
     bool bext_nonzero = false;
 
     // velocity boundary conditions
@@ -254,8 +267,12 @@ struct pdx_step_args {
     int64_t step;
     int flags;
     double rho, dt, dth;
-    const int64_t *row_ptr;
+    const int64_t *slice_ptr;
+    const int32_t *cnt;
     uint32_t *bw;
+    uint16_t *bw16;
+    const int64_t *halo_ptr;
+    const int32_t *halo;
     const double *L;
     const float *Lf;
     const double4 *rec_in;

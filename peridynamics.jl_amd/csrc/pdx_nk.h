@@ -1,0 +1,31 @@
+// Newton-Krylov workspace and the hooks of the learned preconditioner.
+#pragma once
+
+#include "pdx_internal.h"
+
+struct pdx_nk {
+    int64_t m = 0;              // 3n
+    int64_t nc = 0;             // constrained dofs
+    int64_t *cdof = nullptr;    // internal dof index of each constraint
+    double *cval = nullptr;     // full-load value of each constraint
+    uint8_t *cmask = nullptr;   // 3n, 1 = constrained
+    double *r = nullptr, *du = nullptr, *w = nullptr, *t0 = nullptr;  // 3n
+    double *Vb = nullptr;       // (restart + 1) x 3n Krylov basis
+    double *Zb = nullptr;       // restart x 3n preconditioned directions (flexible GMRES)
+    int m_cap = 0;
+    double4 *recp = nullptr, *recm = nullptr;  // perturbed positions
+    double *fp = nullptr, *fm = nullptr;       // b_int V at the perturbed positions
+    double *red = nullptr;      // reduction partials
+    double *h_dev = nullptr;    // small coefficient vectors
+    double *dinv = nullptr;     // 9n point-block Jacobi inverse
+    int64_t n_jv = 0;
+    bool collect = false;       // record (rhs, correction) pairs for training
+};
+
+// Jv on device vectors (internal order)
+pdx_status pdx_nk_jv_dev(pdx_ctx *c, const double *v, double *jv, double scale);
+
+// learned preconditioner (pdx_gnn.hip)
+pdx_status pdx_gnn_prepare(pdx_ctx *c);                                  // per Newton iteration
+pdx_status pdx_gnn_apply(pdx_ctx *c, const double *v, double *z);        // z = M^-1 v
+pdx_status pdx_gnn_record(pdx_ctx *c, const double *rhs, const double *du);  // training pair

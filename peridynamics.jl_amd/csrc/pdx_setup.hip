@@ -18,6 +18,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <thread>
 
 #include "pdx_internal.h"
 
@@ -51,8 +52,14 @@ extern "C" const char *pdx_last_error(const pdx_ctx *c)
 
 static void free_bonds(pdx_ctx *c)
 {
-    FREE(c->row_ptr);
+    FREE(c->slice_ptr);
+    FREE(c->cnt);
     FREE(c->bw);
+    FREE(c->bw16);
+    FREE(c->halo_ptr);
+    FREE(c->halo);
+    c->lds_mode = false;
+    c->halo_max = 0;
     FREE(c->L);
     FREE(c->Lf);
 }
@@ -99,9 +106,9 @@ extern "C" pdx_status pdx_create(pdx_ctx **out, int32_t device, int32_t precisio
     pdx_ctx *c = new pdx_ctx();
     c->device = device;
     c->precision = precision;
-    if (const char *e = getenv("PDX_BB_VARIANT")) {
+    if (const char *e = getenv("PDX_BB_UNROLL")) {
         int v = atoi(e);
-        if (v == 12 || v == 22 || v == 11) c->bb_variant = v;
+        if (v == 2 || v == 4) c->bb_unroll = v;
     }
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
@@ -172,6 +179,8 @@ extern "C" pdx_status pdx_set_material(pdx_ctx *c, int32_t kind, const pdx_param
         return pdx_fail(c, PDX_ERR_ARG, "unknown material kind %d", kind);
     if (!(p->delta > 0) || !(p->rho > 0))
         return pdx_fail(c, PDX_ERR_ARG, "horizon and rho have to be larger than zero");
+    if (kind != PDX_BB && c->precision == PDX_MIXED)
+        return pdx_fail(c, PDX_ERR_UNSUPPORTED, "mixed precision is provided for BBMaterial only");
     if (kind == PDX_CPD && p->model != PDX_SVK && p->model != PDX_NHP)
         return pdx_fail(c, PDX_ERR_ARG, "unknown constitutive model %d", p->model);
     if (p->kernel != 0) return pdx_fail(c, PDX_ERR_UNSUPPORTED, "only linear_kernel is built");
@@ -257,7 +266,8 @@ struct search_args {
     uint32_t filt_bits;       // bit 2k set: crack k filters its bonds
     const uint8_t *fp;        // fail_permit, internal order
     int32_t *counts;
-    const int64_t *row_ptr;
+    const int64_t *slice_ptr;
+    const int32_t *cnt;
     uint32_t *bw;
     double *L;
     unsigned long long *flags;
@@ -327,8 +337,8 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_fill_bonds(search_args s)
     double4 xi = s.Xr[i];
     int cx = s.pcell[3 * i], cy = s.pcell[3 * i + 1], cz = s.pcell[3 * i + 2];
     uint32_t mi = s.cmask ? s.cmask[i] : 0;
-    int64_t b0 = s.row_ptr[i];
-    int want = (int)(s.row_ptr[i + 1] - b0);  // <= PDX_FILL_CAP, checked on the host
+    const int64_t b0 = s.slice_ptr[i >> 6] + (i & 63);  // SELL-64: bond t at b0 + 64 t
+    const int want = s.cnt[i];                          // <= PDX_FILL_CAP, checked on the host
     int base = 0;
     for (int z = cz - 1; z <= cz + 1; z++) {
         if (z < 0 || z >= s.gz) continue;
@@ -377,26 +387,23 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_fill_bonds(search_args s)
         uint32_t h = s.cmask ? crack_hits(mi, s.cmask[j]) : 0u;
         const uint32_t fpb = (s.fp[i] & s.fp[j]) ? PDX_BW_FP : 0u;
         const uint32_t act = (h & ~s.filt_bits) ? 0u : PDX_BW_ACT;  // broken at t = 0 by a precrack
-        int64_t b = b0 + rank;
+        const int64_t b = b0 + (int64_t)rank * 64;
         s.bw[b] = (uint32_t)j | act | fpb;
         s.L[b] = sqrt(d2);
     }
 }
 
-__global__ void k_init_point_state(int64_t n, const int64_t *row_ptr, const uint32_t *bw,
-                                   int32_t *nact, double *damage)
+__global__ void k_init_point_state(int64_t n, const int64_t *slice_ptr, const int32_t *cnt,
+                                   const uint32_t *bw, int32_t *nact, double *damage)
 {
-    int64_t i = pdx_wave_point();
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int lane = threadIdx.x & 63;
-    int cnt = 0;
-    for (int64_t b = row_ptr[i] + lane; b < row_ptr[i + 1]; b += 64) cnt += (bw[b] & PDX_BW_ACT) ? 1 : 0;
-    cnt = pdx_wave_sum_i(cnt);
-    if (lane == 0) {
-        nact[i] = cnt;
-        // calc_damage!: 1 - n_active / n_neighbors (NaN for a point without bonds)
-        damage[i] = 1.0 - (double)cnt / (double)(row_ptr[i + 1] - row_ptr[i]);
-    }
+    const int64_t b0 = slice_ptr[i >> 6] + (i & 63);
+    int na = 0;
+    for (int t = 0; t < cnt[i]; t++) na += (bw[b0 + (int64_t)t * 64] & PDX_BW_ACT) ? 1 : 0;
+    nact[i] = na;
+    // calc_damage!: 1 - n_active / n_neighbors (NaN for a point without bonds)
+    damage[i] = 1.0 - (double)na / (double)cnt[i];
 }
 
 __global__ void k_L_to_float(int64_t nb, const double *L, float *Lf)
@@ -459,6 +466,89 @@ pdx_status pdx_download_vec(pdx_ctx *c, const double *dev, double *host, int com
     PDX_HIP(c, hipMemcpyAsync(host, c->stage, sizeof(double) * cnt, hipMemcpyDeviceToHost,
                               c->stream));
     PDX_HIP(c, hipStreamSynchronize(c->stream));
+    return PDX_OK;
+}
+
+// ----------------------------------------------------------------------------------------
+// LDS halos of the BB step.  For every block of PDX_BLOCK consecutive (Morton) points the
+// sorted, de-duplicated list of the points its bonds reach; every bond is rewritten as a
+// 16-bit slot into its block's list.  Built on the host, in parallel over blocks, once.
+// Returns false (and leaves the global-gather layout in use) if a halo does not fit.
+// ----------------------------------------------------------------------------------------
+static pdx_status build_halos(pdx_ctx *c, bool *ok)
+{
+    *ok = false;
+    const int64_t n = c->n, nba = c->nb_alloc;
+    const int64_t n_slices = (n + 63) / 64;
+    const int64_t nblk = (n + PDX_BLOCK - 1) / PDX_BLOCK;
+    hipDeviceProp_t prop;
+    PDX_HIP(c, hipGetDeviceProperties(&prop, c->device));
+    const int64_t lds_limit = std::min<int64_t>((int64_t)prop.sharedMemPerBlock, 160 * 1024);
+    const int64_t slot_limit = std::min<int64_t>(PDX_W16_SLOT + 1, lds_limit / (int64_t)sizeof(double4));
+    std::vector<int64_t> sp(n_slices + 1);
+    std::vector<int32_t> cn(n);
+    std::vector<uint32_t> w(nba);
+    PDX_HIP(c, hipMemcpy(sp.data(), c->slice_ptr, sizeof(int64_t) * (n_slices + 1), hipMemcpyDeviceToHost));
+    PDX_HIP(c, hipMemcpy(cn.data(), c->cnt, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    PDX_HIP(c, hipMemcpy(w.data(), c->bw, sizeof(uint32_t) * nba, hipMemcpyDeviceToHost));
+    std::vector<std::vector<int32_t>> tab(nblk);
+    std::vector<uint16_t> w16(nba, 0);
+    std::vector<int> too_big(nblk, 0);
+    unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    auto work = [&](int64_t k0, int64_t k1) {
+        std::vector<int32_t> tmp;
+        for (int64_t k = k0; k < k1; k++) {
+            tmp.clear();
+            const int64_t pe = std::min(n, (k + 1) * PDX_BLOCK);
+            for (int64_t p = k * PDX_BLOCK; p < pe; p++) {
+                const int64_t b0 = sp[p >> 6] + (p & 63);
+                for (int t = 0; t < cn[p]; t++) tmp.push_back((int32_t)(w[b0 + 64 * (int64_t)t] & PDX_BW_NBR));
+            }
+            std::sort(tmp.begin(), tmp.end());
+            tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+            if ((int64_t)tmp.size() > slot_limit) {
+                too_big[k] = 1;
+                continue;
+            }
+            for (int64_t p = k * PDX_BLOCK; p < pe; p++) {
+                const int64_t b0 = sp[p >> 6] + (p & 63);
+                for (int t = 0; t < cn[p]; t++) {
+                    const int64_t b = b0 + 64 * (int64_t)t;
+                    const int32_t j = (int32_t)(w[b] & PDX_BW_NBR);
+                    const uint32_t slot = (uint32_t)(std::lower_bound(tmp.begin(), tmp.end(), j) - tmp.begin());
+                    w16[b] = (uint16_t)(slot | ((w[b] & PDX_BW_ACT) ? PDX_W16_ACT : 0u) |
+                                        ((w[b] & PDX_BW_FP) ? PDX_W16_FP : 0u));
+                }
+            }
+            tab[k] = tmp;
+        }
+    };
+    std::vector<std::thread> th;
+    const int64_t per = (nblk + nth - 1) / nth;
+    for (unsigned t = 0; t < nth; t++) {
+        const int64_t k0 = t * per, k1 = std::min(nblk, k0 + per);
+        if (k0 < k1) th.emplace_back(work, k0, k1);
+    }
+    for (auto &t : th) t.join();
+    for (int64_t k = 0; k < nblk; k++)
+        if (too_big[k]) return PDX_OK;  // keep the global-gather kernel
+    std::vector<int64_t> hp(nblk + 1, 0);
+    int64_t hmax = 0;
+    for (int64_t k = 0; k < nblk; k++) {
+        hp[k + 1] = hp[k] + (int64_t)tab[k].size();
+        hmax = std::max<int64_t>(hmax, (int64_t)tab[k].size());
+    }
+    std::vector<int32_t> halo(std::max<int64_t>(hp[nblk], 1));
+    for (int64_t k = 0; k < nblk; k++) std::copy(tab[k].begin(), tab[k].end(), halo.begin() + hp[k]);
+    PDX_TRY(pdx_alloc(c, &c->bw16, nba));
+    PDX_TRY(pdx_alloc(c, &c->halo_ptr, nblk + 1));
+    PDX_TRY(pdx_alloc(c, &c->halo, halo.size()));
+    PDX_HIP(c, hipMemcpy(c->bw16, w16.data(), sizeof(uint16_t) * nba, hipMemcpyHostToDevice));
+    PDX_HIP(c, hipMemcpy(c->halo_ptr, hp.data(), sizeof(int64_t) * (nblk + 1), hipMemcpyHostToDevice));
+    PDX_HIP(c, hipMemcpy(c->halo, halo.data(), sizeof(int32_t) * halo.size(), hipMemcpyHostToDevice));
+    c->halo_max = (int32_t)hmax;
+    c->lds_mode = true;
+    *ok = true;
     return PDX_OK;
 }
 
@@ -632,38 +722,55 @@ extern "C" pdx_status pdx_build_bonds(pdx_ctx *c, int64_t *n_bonds_out)
                         "point duplicate found! Point #%llu has a duplicate #%llu which will lead to NaNs",
                         hflags[2] + 1, hflags[3] + 1);
     }
-    std::vector<int64_t> row_ptr(n + 1, 0);
+    // SELL-64 slices: slice s holds points 64 s .. 64 s + 63, as wide as its widest row
+    const int64_t n_slices = (n + 63) / 64;
+    std::vector<int64_t> slice_ptr(n_slices + 1, 0);
     int32_t maxc = 0;
-    for (int64_t m = 0; m < n; m++) {
-        row_ptr[m + 1] = row_ptr[m] + counts[m];
-        maxc = std::max(maxc, counts[m]);
+    int64_t nb_real = 0;
+    for (int64_t sl = 0; sl < n_slices; sl++) {
+        int32_t w = 0;
+        for (int64_t m = 64 * sl; m < std::min(n, 64 * sl + 64); m++) {
+            w = std::max(w, counts[m]);
+            nb_real += counts[m];
+        }
+        maxc = std::max(maxc, w);
+        slice_ptr[sl + 1] = slice_ptr[sl] + 64 * (int64_t)w;
     }
     if (maxc > PDX_FILL_CAP) {
         cleanup();
         return pdx_fail(c, PDX_ERR_UNSUPPORTED, "a point has %d bonds (> %d)", maxc, PDX_FILL_CAP);
     }
-    const int64_t nb = row_ptr[n];
-    if (nb >= ((int64_t)1 << 40)) {
+    const int64_t nb = nb_real;
+    const int64_t nb_alloc = slice_ptr[n_slices];
+    if (nb_alloc >= ((int64_t)1 << 40)) {
         cleanup();
         return pdx_fail(c, PDX_ERR_UNSUPPORTED, "too many bonds");
     }
     c->nb = nb;
-    TRY_C(pdx_alloc(c, &c->row_ptr, n + 1));
-    TRY_C(pdx_alloc(c, &c->bw, nb));
-    TRY_C(pdx_alloc(c, &c->L, nb));
-    HIP_C(hipMemcpy(c->row_ptr, row_ptr.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice));
-    s.row_ptr = c->row_ptr;
+    c->nb_alloc = nb_alloc;
+    TRY_C(pdx_alloc(c, &c->slice_ptr, n_slices + 1));
+    TRY_C(pdx_alloc(c, &c->cnt, n));
+    TRY_C(pdx_alloc(c, &c->bw, nb_alloc));
+    TRY_C(pdx_alloc(c, &c->L, nb_alloc));
+    HIP_C(hipMemcpy(c->slice_ptr, slice_ptr.data(), sizeof(int64_t) * (n_slices + 1),
+                    hipMemcpyHostToDevice));
+    HIP_C(hipMemcpy(c->cnt, counts.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    // padding entries of a slice stay zero (never read: t < cnt)
+    HIP_C(hipMemsetAsync(c->bw, 0, sizeof(uint32_t) * nb_alloc, c->stream));
+    HIP_C(hipMemsetAsync(c->L, 0, sizeof(double) * nb_alloc, c->stream));
+    s.slice_ptr = c->slice_ptr;
+    s.cnt = c->cnt;
     s.bw = c->bw;
     s.L = c->L;
     hipLaunchKernelGGL(k_fill_bonds, dim3(grid), dim3(PDX_BLOCK), 0, c->stream, s);
     HIP_C(hipGetLastError());
-    hipLaunchKernelGGL(k_init_point_state, dim3(grid), dim3(PDX_BLOCK), 0, c->stream, n,
-                       c->row_ptr, c->bw, c->nact, c->damage);
+    hipLaunchKernelGGL(k_init_point_state, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       c->stream, n, c->slice_ptr, c->cnt, c->bw, c->nact, c->damage);
     HIP_C(hipGetLastError());
     if (c->precision == PDX_MIXED) {
-        TRY_C(pdx_alloc(c, &c->Lf, nb));
-        hipLaunchKernelGGL(k_L_to_float, dim3((nb + 255) / 256), dim3(256), 0, c->stream, nb,
-                           c->L, c->Lf);
+        TRY_C(pdx_alloc(c, &c->Lf, nb_alloc));
+        hipLaunchKernelGGL(k_L_to_float, dim3((unsigned)((nb_alloc + 255) / 256)), dim3(256), 0,
+                           c->stream, nb_alloc, c->L, c->Lf);
         HIP_C(hipGetLastError());
     }
     // zero dynamic state
@@ -687,6 +794,10 @@ extern "C" pdx_status pdx_build_bonds(pdx_ctx *c, int64_t *n_bonds_out)
         HIP_C(hipStreamSynchronize(c->stream));
     }
     cleanup();
+    if (c->kind == PDX_BB && !getenv("PDX_NO_LDS")) {
+        bool ok = false;
+        PDX_TRY(build_halos(c, &ok));
+    }
     c->bonds_built = true;
     if (n_bonds_out) *n_bonds_out = nb;
     return PDX_OK;
@@ -699,14 +810,25 @@ extern "C" pdx_status pdx_get_bonds(pdx_ctx *c, int64_t *row_ptr, int32_t *nbr, 
 {
     if (!c) return PDX_ERR_ARG;
     if (!c->bonds_built) return pdx_fail(c, PDX_ERR_STATE, "pdx_build_bonds first");
-    const int64_t n = c->n, nb = c->nb;
-    std::vector<int64_t> rp(n + 1);
-    std::vector<uint32_t> w(nb);
-    std::vector<double> L_dev(L ? nb : 0);
+    const int64_t n = c->n, nba = c->nb_alloc;
+    const int64_t n_slices = (n + 63) / 64;
+    std::vector<int64_t> sp(n_slices + 1);
+    std::vector<int32_t> cn(n);
+    std::vector<uint32_t> w(nba);
+    std::vector<double> L_dev(L ? nba : 0);
     PDX_HIP(c, hipStreamSynchronize(c->stream));
-    PDX_HIP(c, hipMemcpy(rp.data(), c->row_ptr, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost));
-    PDX_HIP(c, hipMemcpy(w.data(), c->bw, sizeof(uint32_t) * nb, hipMemcpyDeviceToHost));
-    if (L) PDX_HIP(c, hipMemcpy(L_dev.data(), c->L, sizeof(double) * nb, hipMemcpyDeviceToHost));
+    PDX_HIP(c, hipMemcpy(sp.data(), c->slice_ptr, sizeof(int64_t) * (n_slices + 1),
+                         hipMemcpyDeviceToHost));
+    PDX_HIP(c, hipMemcpy(cn.data(), c->cnt, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    PDX_HIP(c, hipMemcpy(w.data(), c->bw, sizeof(uint32_t) * nba, hipMemcpyDeviceToHost));
+    if (L) PDX_HIP(c, hipMemcpy(L_dev.data(), c->L, sizeof(double) * nba, hipMemcpyDeviceToHost));
+    if (c->lds_mode) {
+        // the LDS-staged step keeps the bond state in the 16-bit words
+        std::vector<uint16_t> w16(nba);
+        PDX_HIP(c, hipMemcpy(w16.data(), c->bw16, sizeof(uint16_t) * nba, hipMemcpyDeviceToHost));
+        for (int64_t b = 0; b < nba; b++)
+            w[b] = (w[b] & ~PDX_BW_ACT) | ((w16[b] & PDX_W16_ACT) ? PDX_BW_ACT : 0u);
+    }
     std::vector<uint8_t> killed(n, 0);
     if (c->kind == PDX_CPD)
         PDX_HIP(c, hipMemcpy(killed.data(), c->killed, n, hipMemcpyDeviceToHost));
@@ -714,7 +836,7 @@ extern "C" pdx_status pdx_get_bonds(pdx_ctx *c, int64_t *row_ptr, int32_t *nbr, 
     std::vector<int64_t> out_rp(n + 1, 0);
     for (int64_t i = 0; i < n; i++) {
         int64_t m = c->iperm[i];
-        out_rp[i + 1] = out_rp[i] + (rp[m + 1] - rp[m]);
+        out_rp[i + 1] = out_rp[i] + cn[m];
     }
     if (row_ptr) std::memcpy(row_ptr, out_rp.data(), sizeof(int64_t) * (n + 1));
     // device rows are in Morton order of the neighbours; the caller's rows list neighbours
@@ -724,7 +846,11 @@ extern "C" pdx_status pdx_get_bonds(pdx_ctx *c, int64_t *row_ptr, int32_t *nbr, 
         int64_t m = c->iperm[i];
         int64_t o = out_rp[i];
         row.clear();
-        for (int64_t b = rp[m]; b < rp[m + 1]; b++) row.push_back({c->perm[w[b] & PDX_BW_NBR], b});
+        const int64_t b0 = sp[m >> 6] + (m & 63);
+        for (int t = 0; t < cn[m]; t++) {
+            const int64_t b = b0 + (int64_t)t * 64;
+            row.push_back({c->perm[w[b] & PDX_BW_NBR], b});
+        }
         std::sort(row.begin(), row.end());
         for (auto &e : row) {
             const int64_t b = e.second;
@@ -826,7 +952,8 @@ extern "C" pdx_status pdx_set_field(pdx_ctx *c, int32_t field, const double *hos
 // LDS and summed in that order by one lane, so that the value equals the reference's bit
 // for bit.
 // ----------------------------------------------------------------------------------------
-__global__ __launch_bounds__(PDX_BLOCK) void k_timestep(int64_t n, const int64_t *row_ptr,
+__global__ __launch_bounds__(PDX_BLOCK) void k_timestep(int64_t n, const int64_t *slice_ptr,
+                                                         const int32_t *cnts,
                                                          const uint32_t *bw, const double *L,
                                                          const double4 *Xr, const int32_t *perm,
                                                          double bc, double rho, double *dt)
@@ -836,20 +963,21 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_timestep(int64_t n, const int64_t
     const int64_t i = pdx_wave_point();
     if (i >= n) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int64_t b0 = row_ptr[i];
-    const int cnt = (int)(row_ptr[i + 1] - b0);  // <= PDX_FILL_CAP (checked when built)
+    const int64_t b0 = slice_ptr[i >> 6] + (i & 63);
+    const int cnt = cnts[i];  // <= PDX_FILL_CAP (checked when built)
     for (int k = lane; k < cnt; k += 64) {
-        const uint32_t j = bw[b0 + k] & PDX_BW_NBR;
+        const uint32_t j = bw[b0 + (int64_t)k * 64] & PDX_BW_NBR;
         key[w][k] = perm[j];
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     for (int k = lane; k < cnt; k += 64) {
-        const uint32_t j = bw[b0 + k] & PDX_BW_NBR;
+        const int64_t b = b0 + (int64_t)k * 64;
+        const uint32_t j = bw[b] & PDX_BW_NBR;
         const int32_t kk = key[w][k];
         int rank = 0;
         for (int m = 0; m < cnt; m++) rank += (key[w][m] < kk) ? 1 : 0;
-        term[w][rank] = Xr[j].w * bc / L[b0 + k];
+        term[w][rank] = Xr[j].w * bc / L[b];
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -869,7 +997,7 @@ extern "C" pdx_status pdx_stable_timestep(pdx_ctx *c, double safety, double *dt_
     PDX_HIP(c, hipSetDevice(c->device));
     double *tmp = c->stage;
     hipLaunchKernelGGL(k_timestep, dim3((unsigned)pdx_wave_grid(c->n)), dim3(PDX_BLOCK), 0,
-                       c->stream, c->n, c->row_ptr, c->bw, c->L, c->Xr, c->d_perm, c->P.bc,
+                       c->stream, c->n, c->slice_ptr, c->cnt, c->bw, c->L, c->Xr, c->d_perm, c->P.bc,
                        c->P.rho, tmp);
     PDX_HIP(c, hipGetLastError());
     std::vector<double> dt(c->n);

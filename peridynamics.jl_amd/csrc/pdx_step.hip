@@ -1,8 +1,13 @@
 // Force density and the fused Velocity Verlet step: BB, OSB and CPD.
 //
-// One wavefront per point gathers the point's bonds (CSR row of the Morton-ordered bond
-// list, 64 bonds per iteration, coalesced), so every point reduces its own bonds without
-// atomics.  The scatter of the reference's OSB and CPD force densities (b_int[j] -= ...,
+// Bond storage is sliced ELL with slices of 64 Morton-consecutive points (SELL-64): bond t
+// of point p sits at slice_ptr[p / 64] + 64 t + p % 64, so bond t of the 64 points of a
+// slice is one contiguous 64-entry row.  One thread owns one point: it walks its bonds
+// with fully coalesced loads of the bond words and lengths, gathers the neighbours'
+// records (Morton order keeps them in the L1/L2 of the CU), and accumulates its own
+// force density in registers -- no atomics and no cross-lane reductions.
+//
+// The scatter of the reference's OSB and CPD force densities (b_int[j] -= ...,
 // ordinary_state_based.jl:145-146, correspondence.jl:248-249) is rewritten as a gather: a
 // first pass stores each point's state (OSB: c1, c2; CPD: F and P*K^-1), the second pass
 // adds, for every bond i-j, both the term point i scatters into itself and the term point
@@ -22,99 +27,89 @@
 
 #include "pdx_internal.h"
 
-// ----------------------------------------------------------------------------------------
-// per-point tail: NaN flag (check_for_nans, nans.jl:10-18), acceleration, velocity and the
-// first half of the next step.  `sub` is the lane index within the point's lane group;
-// sub 0..2 handle the x, y, z dofs.
-// ----------------------------------------------------------------------------------------
-template <bool POINT_STATE>
-__device__ __forceinline__ void step_tail(const pdx_step_args &A, int64_t i, int sub,
-                                          double4 ri, double bx, double by, double bz, int na,
-                                          int64_t nn)
+// thread index of the calling point; blocks are remapped so that each XCD gets a
+// contiguous range of points (see pdx_xcd_block)
+__device__ __forceinline__ int64_t pdx_thread_point()
 {
-    if (POINT_STATE && sub == 0 && (A.flags & (PDX_F_STORE | PDX_F_FORCE_ONLY))) {
-        A.nact_out[i] = na;
-        // calc_damage!: bond_system.jl:338-343
-        A.dmg_out[i] = 1.0 - (double)na / (double)nn;
+    return pdx_xcd_block() * PDX_BLOCK + threadIdx.x;
+}
+
+// first bond entry of point p in the SELL-64 arrays; bond t is at + 64 t
+__device__ __forceinline__ int64_t pdx_bond0(const int64_t *slice_ptr, int64_t p)
+{
+    return slice_ptr[p >> 6] + (p & 63);
+}
+
+// ----------------------------------------------------------------------------------------
+// per-point tail: NaN flag (check_for_nans, nans.jl:10-18), acceleration and velocity
+// (_update_acc_and_vel!), then the first half of the next step (update_vel_half!,
+// apply_boundary_conditions!, update_disp_and_pos!).  Operands are loaded before the bond
+// loop (tail_load) so that their latency overlaps it.
+// ----------------------------------------------------------------------------------------
+struct tail_ops {
+    double u[3], vh[3], be[3];
+    int32_t slot;
+};
+
+__device__ __forceinline__ void tail_load(const pdx_step_args &A, int64_t p, tail_ops &T)
+{
+    const int flags = A.flags;
+    T.slot = -1;
+#pragma unroll
+    for (int d = 0; d < 3; d++) T.u[d] = T.vh[d] = T.be[d] = 0.0;
+    if (flags & PDX_F_FORCE_ONLY) return;
+#pragma unroll
+    for (int d = 0; d < 3; d++) T.vh[d] = A.vh[3 * p + d];
+    if (flags & PDX_F_HAS_NEXT) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) T.u[d] = A.u[3 * p + d];
+        T.slot = A.bcslot[p];
     }
-    if (sub < 3) {
-        const double bd = sub == 0 ? bx : (sub == 1 ? by : bz);
-        if (isnan(bd)) atomicMin(A.flags_out, (unsigned long long)A.step);
-        const int64_t k = 3 * i + sub;
-        if (A.flags & PDX_F_FORCE_ONLY) {
-            A.bint_out[k] = (A.flags & PDX_F_SCALE_VOL) ? bd * ri.w : bd;
-            return;
-        }
-        const double be = (A.flags & PDX_F_BEXT) ? A.bext[k] : 0.0;
-        const double ad = (bd + be) / A.rho;
-        const double vd = A.vh[k] + ad * A.dth;
-        if (A.flags & PDX_F_STORE) {
+    if (flags & PDX_F_BEXT) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) T.be[d] = A.bext[3 * p + d];
+    }
+}
+
+__device__ __forceinline__ void point_tail(const pdx_step_args &A, int64_t p, double4 ri,
+                                           const double b[3], const tail_ops &T)
+{
+    const int flags = A.flags;
+    if (isnan(b[0]) || isnan(b[1]) || isnan(b[2]))
+        atomicMin(A.flags_out, (unsigned long long)A.step);
+    if (flags & PDX_F_FORCE_ONLY) {
+        const double s = (flags & PDX_F_SCALE_VOL) ? ri.w : 1.0;
+#pragma unroll
+        for (int d = 0; d < 3; d++) A.bint_out[3 * p + d] = (flags & PDX_F_SCALE_VOL) ? b[d] * s : b[d];
+        return;
+    }
+    double x[3] = {ri.x, ri.y, ri.z};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const int64_t k = 3 * p + d;
+        const double ad = (b[d] + T.be[d]) / A.rho;
+        const double vd = T.vh[d] + ad * A.dth;
+        if (flags & PDX_F_STORE) {
             A.v_out[k] = vd;
             A.a_out[k] = ad;
-            A.bint_out[k] = bd;
+            A.bint_out[k] = b[d];
         }
-        if (A.flags & PDX_F_HAS_NEXT) {
+        if (flags & PDX_F_HAS_NEXT) {
             double vn = vd + ad * A.dth;
-            const int32_t slot = A.bcslot[i];
-            if (slot >= 0) {
-                const int32_t s = A.slot_src[3 * slot + sub];
-                if (s >= 0) {
-                    const double val = A.bcv[s];
+            if (T.slot >= 0) {
+                const int32_t src = A.slot_src[3 * T.slot + d];
+                if (src >= 0) {
+                    const double val = A.bcv[src];
                     if (isfinite(val)) vn = val;
                 }
             }
             const double du = vn * A.dt;
-            A.u[k] = A.u[k] + du;
-            const double xd = (sub == 0 ? ri.x : (sub == 1 ? ri.y : ri.z)) + du;
-            reinterpret_cast<double *>(A.rec_out)[4 * i + sub] = xd;
+            A.u[k] = T.u[d] + du;
+            x[d] = x[d] + du;
             A.vh[k] = vn;
         }
-    } else if (sub == 3 && (A.flags & PDX_F_HAS_NEXT) && !(A.flags & PDX_F_FORCE_ONLY)) {
-        reinterpret_cast<double *>(A.rec_out)[4 * i + 3] = ri.w;
     }
-}
-
-// 1/x to about 1 ulp: hardware reciprocal and two Newton steps
-__device__ __forceinline__ double pdx_rcp(double x)
-{
-    double r = __builtin_amdgcn_rcp(x);
-    double e = __builtin_fma(-x, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    e = __builtin_fma(-x, r, 1.0);
-    return __builtin_fma(r, e, r);
-}
-
-// ----------------------------------------------------------------------------------------
-// BB: calc_failure! (bond_based.jl:125-144) + calc_damage! + force_density_point!
-// (bond_based.jl:146-161) + the fused tail.
-//
-// PPW points per wavefront (64/PPW lanes per point).  A lane takes the bonds h, h + LPP, ...,
-// h + (U-1) LPP of its point per chunk (h = lane index within the point's lanes) and issues
-// all their loads and neighbour gathers before any arithmetic.  The operands of the tail
-// (u, v_half, b_ext, BC slot) are loaded before the bond loop so that their latency
-// overlaps it.
-//
-// FP64: l = sqrt((dx*dx + dy*dy) + dz*dz) is correctly rounded.  The stretch used in the
-// force is (l - L) * (1/L) (within a few ulp of the reference's (l - L)/L); the breakage
-// test eps > eps_c uses the correctly rounded quotient whenever the two could disagree
-// (|eps - eps_c| <= 1e-11 |eps_c|), so bonds break exactly when the reference's break.  The
-// force term is ((w bc) eps V_j) * (1/l), accumulated with dx by fused multiply-adds (the
-// reference divides each component by l and adds).  An inactive bond has w = 0, as the
-// reference's Bool * factor, so NaN positions still show.
-// MIXED: the stretch comes from FP64 differences du = (x_j - x_i) - (X_j - X_i), then
-// eps = (2 dX.du + du.du) / (L (l + L)) in FP32 and the force in FP32, summed in FP64.
-// ----------------------------------------------------------------------------------------
-__device__ __forceinline__ double pdx_wave_total(double x)
-{
-    x = pdx_row_sum(x);
-    return (pdx_readlane(x, 0) + pdx_readlane(x, 16)) + (pdx_readlane(x, 32) + pdx_readlane(x, 48));
-}
-
-__device__ __forceinline__ int pdx_wave_total_i(int x)
-{
-    x = pdx_row_sum_i(x);
-    return (__builtin_amdgcn_readlane(x, 0) + __builtin_amdgcn_readlane(x, 16)) +
-           (__builtin_amdgcn_readlane(x, 32) + __builtin_amdgcn_readlane(x, 48));
+    if (flags & PDX_F_HAS_NEXT) A.rec_out[p] = make_double4(x[0], x[1], x[2], ri.w);
 }
 
 // sqrt(d2) correctly rounded for a normal, finite d2 > 0 (the Markstein sequence of the
@@ -142,110 +137,74 @@ __device__ __forceinline__ double pdx_rcp1(double x)
     return __builtin_fma(__builtin_fma(-x, r, 1.0), r, r);
 }
 
-// One bond of the BB force density (see k_bb_step for the arithmetic contract).
-template <bool MIXED>
-__device__ __forceinline__ void bb_bond(const pdx_step_args &A, int64_t b, uint32_t wd, double Ld,
-                                        float Lfl, double4 rj, double4 Xj, double4 ri,
-                                        double4 Xi, double &bx, double &by, double &bz, int &na)
+// calc_failure! of one bond (bond_based.jl:125-144, ordinary_state_based.jl:107-126,
+// bond_system.jl:309-327): breaks the bond (writes its word back) if eps > eps_c and the
+// bond may fail.  eps = e * (1/L) is within ~1e-13 of the correctly rounded (l - L) / L;
+// inside [epsc_lo, epsc_hi] the correctly rounded quotient decides, so a bond breaks
+// exactly when the reference's breaks.
+__device__ __forceinline__ bool bond_fail(const pdx_step_args &A, uint32_t *wp, uint32_t wd,
+                                          double e, double Lb, double eps)
 {
-    const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
     bool act = (wd & PDX_BW_ACT) != 0;
-    if (!MIXED) {
-        const double d2 = (dx * dx + dy * dy) + dz * dz;
-        double rl;
-        const double l = pdx_sqrt_rsq(d2, rl);
-        const double e = l - Ld;
-        const double eps = e * pdx_rcp1(Ld);
-        // breakage: eps > eps_c with the correctly rounded quotient wherever it could differ
-        if (act && (wd & PDX_BW_FP) && eps >= A.epsc_lo) {
-            const bool brk = (eps > A.epsc_hi) ? true : ((e / Ld) > A.epsc);
-            if (brk) {
-                act = false;
-                A.bw[b] = wd & ~PDX_BW_ACT;
-            }
-        }
-        const double wbc = act ? A.bc : 0.0;  // (w * bc) with w = bond_active
-        const double s = ((wbc * eps) * rj.w) * rl;
-        bx = __builtin_fma(s, dx, bx);
-        by = __builtin_fma(s, dy, by);
-        bz = __builtin_fma(s, dz, bz);
-    } else {
-        const double DXd = Xj.x - Xi.x, DYd = Xj.y - Xi.y, DZd = Xj.z - Xi.z;
-        const float ux = (float)(dx - DXd), uy = (float)(dy - DYd), uz = (float)(dz - DZd);
-        const float DX = (float)DXd, DY = (float)DYd, DZ = (float)DZd;
-        const float num = 2.0f * ((DX * ux + DY * uy) + DZ * uz) + ((ux * ux + uy * uy) + uz * uz);
-        const float l = __fsqrt_rn(Lfl * Lfl + num);
-        const float eps = __fdividef(num, Lfl * (l + Lfl));
-        if (act && (wd & PDX_BW_FP) && eps > (float)A.epsc) {
+    if (act && (wd & PDX_BW_FP) && eps >= A.epsc_lo) {
+        const bool brk = (eps > A.epsc_hi) ? true : ((e / Lb) > A.epsc);
+        if (brk) {
             act = false;
-            A.bw[b] = wd & ~PDX_BW_ACT;
+            *wp = wd & ~PDX_BW_ACT;
         }
-        const float wbc = act ? (float)A.bc : 0.0f;
-        const float sf = __fdividef((wbc * eps) * (float)rj.w, l);
-        bx += (double)(sf * (float)dx);
-        by += (double)(sf * (float)dy);
-        bz += (double)(sf * (float)dz);
     }
-    na += act ? 1 : 0;
+    return act;
 }
 
-template <bool MIXED, int PPW, int U>
+// ----------------------------------------------------------------------------------------
+// BB: calc_failure! (bond_based.jl:125-144) + calc_damage! (bond_system.jl:338-343) +
+// force_density_point! (bond_based.jl:146-161) + the fused tail.  One thread per point, U
+// bonds per iteration with all their loads and gathers issued before the arithmetic.
+//
+// FP64: l = sqrt((dx*dx + dy*dy) + dz*dz) is correctly rounded.  The force term is
+// ((w bc) eps V_j) * (1/l), accumulated with dx by fused multiply-adds (the reference
+// divides each component by l and adds).  An inactive bond has w = 0, as the reference's
+// Bool * factor, so NaN positions still show.
+// MIXED: the stretch comes from FP64 differences du = (x_j - x_i) - (X_j - X_i), then
+// eps = (2 dX.du + du.du) / (L (l + L)) in FP32 and the force in FP32, summed in FP64.
+// ----------------------------------------------------------------------------------------
+template <bool MIXED, int U>
 __global__ __launch_bounds__(PDX_BLOCK) void k_bb_step(pdx_step_args A)
 {
-    constexpr int LPP = 64 / PPW;
-    const int lane = threadIdx.x & 63;
-    const int grp = (PPW == 2) ? (lane >> 5) : 0;
-    const int h = lane & (LPP - 1);
-    const int64_t w = pdx_wave_point();
-    if ((int64_t)PPW * w >= A.n) return;
-    const int64_t i = PPW * w + grp;
-    const bool valid = i < A.n;
-    const int flags = A.flags;
-    int64_t b0 = 0, b1 = 0;
-    double4 ri = make_double4(0, 0, 0, 0), Xi = make_double4(0, 0, 0, 0);
-    double u_d = 0.0, vh_d = 0.0, be_d = 0.0;
-    int32_t slot = -1;
-    if (valid) {
-        b0 = A.row_ptr[i];
-        b1 = A.row_ptr[i + 1];
-        ri = A.rec_in[i];
-        if (MIXED) Xi = A.Xr[i];
-        if (h < 3 && !(flags & PDX_F_FORCE_ONLY)) {
-            const int64_t k = 3 * i + h;
-            vh_d = A.vh[k];
-            if (flags & PDX_F_HAS_NEXT) {
-                u_d = A.u[k];
-                slot = A.bcslot[i];
-            }
-            if (flags & PDX_F_BEXT) be_d = A.bext[k];
-        }
-    }
+    const int64_t p = pdx_thread_point();
+    if (p >= A.n) return;
+    const int c = A.cnt[p];
+    const int64_t b0 = pdx_bond0(A.slice_ptr, p);
+    uint32_t *const wbase = A.bw + b0;
+    const double4 ri = A.rec_in[p];
+    double4 Xi = make_double4(0, 0, 0, 0);
+    if (MIXED) Xi = A.Xr[p];
     double bx = 0.0, by = 0.0, bz = 0.0;
     int na = 0;
-    for (int64_t base = b0 + h; base < b1; base += LPP * U) {
+    const double bc = A.bc;
+    const float bcf = (float)A.bc, epscf = (float)A.epsc;
+    for (int t = 0; t < c; t += U) {
         uint32_t wd[U];
         double Ld[U];
         float Lfl[U];
         double4 rj[U], Xj[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            Ld[u] = 0.0;
-            Lfl[u] = 0.0f;
-            Xj[u] = make_double4(0, 0, 0, 0);
-        }
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int64_t b = base + LPP * u;
             wd[u] = 0;
-            if (b < b1) {
-                wd[u] = A.bw[b];
-                if (MIXED) Lfl[u] = A.Lf[b];
-                else Ld[u] = A.L[b];
+            Ld[u] = 1.0;
+            Lfl[u] = 1.0f;
+            if (t + u < c) {
+                const int64_t o = (int64_t)(t + u) * 64;
+                wd[u] = wbase[o];
+                if (MIXED) Lfl[u] = A.Lf[b0 + o];
+                else Ld[u] = A.L[b0 + o];
             }
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            if (base + LPP * u < b1) {
+            rj[u] = make_double4(0, 0, 0, 0);
+            Xj[u] = rj[u];
+            if (t + u < c) {
                 const uint32_t j = wd[u] & PDX_BW_NBR;
                 rj[u] = A.rec_in[j];
                 if (MIXED) Xj[u] = A.Xr[j];
@@ -253,63 +212,161 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_bb_step(pdx_step_args A)
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const int64_t b = base + LPP * u;
-            if (b < b1)
-                bb_bond<MIXED>(A, b, wd[u], MIXED ? 0.0 : Ld[u], MIXED ? Lfl[u] : 0.0f, rj[u],
-                               MIXED ? Xj[u] : make_double4(0, 0, 0, 0), ri, Xi, bx, by, bz, na);
-        }
-    }
-    if (PPW == 2) {
-        bx = pdx_half_sum(bx, grp);
-        by = pdx_half_sum(by, grp);
-        bz = pdx_half_sum(bz, grp);
-        na = pdx_half_sum_i(na, grp);
-    } else {
-        bx = pdx_wave_total(bx);
-        by = pdx_wave_total(by);
-        bz = pdx_wave_total(bz);
-        na = pdx_wave_total_i(na);
-    }
-    if (!valid) return;
-    // tail: calc_damage!, check_for_nans, _update_acc_and_vel!, then the first half of the
-    // next step (update_vel_half!, apply_boundary_conditions!, update_disp_and_pos!)
-    if (h == 0 && (flags & (PDX_F_STORE | PDX_F_FORCE_ONLY))) {
-        A.nact_out[i] = na;
-        A.dmg_out[i] = 1.0 - (double)na / (double)(b1 - b0);
-    }
-    if (h < 3) {
-        const double bd = h == 0 ? bx : (h == 1 ? by : bz);
-        if (isnan(bd)) atomicMin(A.flags_out, (unsigned long long)A.step);
-        const int64_t k = 3 * i + h;
-        if (flags & PDX_F_FORCE_ONLY) {
-            A.bint_out[k] = (flags & PDX_F_SCALE_VOL) ? bd * ri.w : bd;
-            return;
-        }
-        const double ad = (bd + be_d) / A.rho;
-        const double vd = vh_d + ad * A.dth;
-        if (flags & PDX_F_STORE) {
-            A.v_out[k] = vd;
-            A.a_out[k] = ad;
-            A.bint_out[k] = bd;
-        }
-        if (flags & PDX_F_HAS_NEXT) {
-            double vn = vd + ad * A.dth;
-            if (slot >= 0) {
-                const int32_t src = A.slot_src[3 * slot + h];
-                if (src >= 0) {
-                    const double val = A.bcv[src];
-                    if (isfinite(val)) vn = val;
+            if (t + u >= c) continue;
+            const double dx = rj[u].x - ri.x, dy = rj[u].y - ri.y, dz = rj[u].z - ri.z;
+            uint32_t *const wp = wbase + (int64_t)(t + u) * 64;
+            if (!MIXED) {
+                const double d2 = (dx * dx + dy * dy) + dz * dz;
+                double rl;
+                const double l = pdx_sqrt_rsq(d2, rl);
+                const double e = l - Ld[u];
+                const double eps = e * pdx_rcp1(Ld[u]);
+                const bool act = bond_fail(A, wp, wd[u], e, Ld[u], eps);
+                const double wbc = act ? bc : 0.0;  // (w * bc) with w = bond_active
+                const double s = ((wbc * eps) * rj[u].w) * rl;
+                bx = __builtin_fma(s, dx, bx);
+                by = __builtin_fma(s, dy, by);
+                bz = __builtin_fma(s, dz, bz);
+                na += act ? 1 : 0;
+            } else {
+                const double DXd = Xj[u].x - Xi.x, DYd = Xj[u].y - Xi.y, DZd = Xj[u].z - Xi.z;
+                const float ux = (float)(dx - DXd), uy = (float)(dy - DYd), uz = (float)(dz - DZd);
+                const float DX = (float)DXd, DY = (float)DYd, DZ = (float)DZd;
+                const float num = 2.0f * ((DX * ux + DY * uy) + DZ * uz) +
+                                  ((ux * ux + uy * uy) + uz * uz);
+                const float Lb = Lfl[u];
+                const float l = __fsqrt_rn(Lb * Lb + num);
+                const float eps = __fdividef(num, Lb * (l + Lb));
+                bool act = (wd[u] & PDX_BW_ACT) != 0;
+                if (act && (wd[u] & PDX_BW_FP) && eps > epscf) {
+                    act = false;
+                    *wp = wd[u] & ~PDX_BW_ACT;
                 }
+                const float wbc = act ? bcf : 0.0f;
+                const float sf = __fdividef((wbc * eps) * (float)rj[u].w, l);
+                bx += (double)(sf * (float)dx);
+                by += (double)(sf * (float)dy);
+                bz += (double)(sf * (float)dz);
+                na += act ? 1 : 0;
             }
-            const double du = vn * A.dt;
-            A.u[k] = u_d + du;
-            const double xd = (h == 0 ? ri.x : (h == 1 ? ri.y : ri.z)) + du;
-            reinterpret_cast<double *>(A.rec_out)[4 * i + h] = xd;
-            A.vh[k] = vn;
         }
-    } else if (h == 3 && (flags & PDX_F_HAS_NEXT) && !(flags & PDX_F_FORCE_ONLY)) {
-        reinterpret_cast<double *>(A.rec_out)[4 * i + 3] = ri.w;
     }
+    if (A.flags & (PDX_F_STORE | PDX_F_FORCE_ONLY)) {
+        A.nact_out[p] = na;
+        A.dmg_out[p] = 1.0 - (double)na / (double)c;
+    }
+    tail_ops T;
+    tail_load(A, p, T);
+    const double b[3] = {bx, by, bz};
+    point_tail(A, p, ri, b, T);
+}
+
+// ----------------------------------------------------------------------------------------
+// BB, LDS-staged (the production kernel).  A workgroup owns PDX_BLOCK consecutive Morton
+// points.  It first copies the records of its halo (every point its bonds reach, ~2000
+// for a 3.015-spacing horizon) from global memory into LDS with one coalesced pass, then
+// each thread walks its point's bonds: 16-bit bond words (slot into the halo, active,
+// fail_permit) and reference lengths stream from HBM, neighbour records come from LDS.
+// Arithmetic as k_bb_step (FP64); MIXED takes the stretch from the FP64 difference
+// d2 - L^2 (fused, one rounding) and does the rest in FP32 with FP64 sums.
+// ----------------------------------------------------------------------------------------
+__device__ __forceinline__ bool bond_fail16(const pdx_step_args &A, uint16_t *wp, uint32_t wd,
+                                            double e, double Lb, double eps)
+{
+    bool act = (wd & PDX_W16_ACT) != 0;
+    if (act && (wd & PDX_W16_FP) && eps >= A.epsc_lo) {
+        const bool brk = (eps > A.epsc_hi) ? true : ((e / Lb) > A.epsc);
+        if (brk) {
+            act = false;
+            *wp = (uint16_t)(wd & ~PDX_W16_ACT);
+        }
+    }
+    return act;
+}
+
+template <bool MIXED, int U>
+__global__ __launch_bounds__(PDX_BLOCK) void k_bb_lds(pdx_step_args A)
+{
+    extern __shared__ double4 hrec[];
+    const int64_t k = pdx_xcd_block();
+    const int64_t p0 = k * PDX_BLOCK;
+    if (p0 >= A.n) return;  // whole workgroup
+    const int64_t h0 = A.halo_ptr[k];
+    const int H = (int)(A.halo_ptr[k + 1] - h0);
+    for (int h = threadIdx.x; h < H; h += PDX_BLOCK) hrec[h] = A.rec_in[A.halo[h0 + h]];
+    __syncthreads();
+    const int64_t p = p0 + threadIdx.x;
+    if (p >= A.n) return;
+    const int c = A.cnt[p];
+    const int64_t b0 = pdx_bond0(A.slice_ptr, p);
+    uint16_t *const wbase = A.bw16 + b0;
+    const double *const Lbase = A.L + b0;
+    const double4 ri = A.rec_in[p];
+    double bx = 0.0, by = 0.0, bz = 0.0;
+    int na = 0;
+    const double bc = A.bc;
+    const float bcf = (float)A.bc, epscf = (float)A.epsc;
+    for (int t = 0; t < c; t += U) {
+        uint32_t wd[U];
+        double Ld[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            wd[u] = 0;
+            Ld[u] = 1.0;
+            if (t + u < c) {
+                const int64_t o = (int64_t)(t + u) * 64;
+                wd[u] = wbase[o];
+                Ld[u] = Lbase[o];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (t + u >= c) continue;
+            const double4 rj = hrec[wd[u] & PDX_W16_SLOT];
+            const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
+            uint16_t *const wp = wbase + (int64_t)(t + u) * 64;
+            const double d2 = (dx * dx + dy * dy) + dz * dz;
+            if (!MIXED) {
+                double rl;
+                const double l = pdx_sqrt_rsq(d2, rl);
+                const double e = l - Ld[u];
+                const double eps = e * pdx_rcp1(Ld[u]);
+                const bool act = bond_fail16(A, wp, wd[u], e, Ld[u], eps);
+                const double wbc = act ? bc : 0.0;  // (w * bc) with w = bond_active
+                const double s = ((wbc * eps) * rj.w) * rl;
+                bx = __builtin_fma(s, dx, bx);
+                by = __builtin_fma(s, dy, by);
+                bz = __builtin_fma(s, dz, bz);
+                na += act ? 1 : 0;
+            } else {
+                // l^2 - L^2 from the FP64 squared distance in one fused operation keeps the
+                // stretch's digits; everything after it is FP32
+                const float num = (float)__builtin_fma(-Ld[u], Ld[u], d2);
+                const float Lb = (float)Ld[u];
+                const float l = __fsqrt_rn((float)d2);
+                const float eps = __fdividef(num, Lb * (l + Lb));
+                bool act = (wd[u] & PDX_W16_ACT) != 0;
+                if (act && (wd[u] & PDX_W16_FP) && eps > epscf) {
+                    act = false;
+                    *wp = (uint16_t)(wd[u] & ~PDX_W16_ACT);
+                }
+                const float wbc = act ? bcf : 0.0f;
+                const float sf = __fdividef((wbc * eps) * (float)rj.w, l);
+                bx += (double)(sf * (float)dx);
+                by += (double)(sf * (float)dy);
+                bz += (double)(sf * (float)dz);
+                na += act ? 1 : 0;
+            }
+        }
+    }
+    if (A.flags & (PDX_F_STORE | PDX_F_FORCE_ONLY)) {
+        A.nact_out[p] = na;
+        A.dmg_out[p] = 1.0 - (double)na / (double)c;
+    }
+    tail_ops T;
+    tail_load(A, p, T);
+    const double b[3] = {bx, by, bz};
+    point_tail(A, p, ri, b, T);
 }
 
 // ----------------------------------------------------------------------------------------
@@ -319,49 +376,41 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_bb_step(pdx_step_args A)
 // ----------------------------------------------------------------------------------------
 __global__ __launch_bounds__(PDX_BLOCK) void k_osb_pass1(pdx_step_args A)
 {
-    const int64_t i = pdx_wave_point();
-    if (i >= A.n) return;
-    const int lane = threadIdx.x & 63;
-    const double4 ri = A.rec_in[i];
-    const int64_t b0 = A.row_ptr[i], b1 = A.row_ptr[i + 1];
+    const int64_t p = pdx_thread_point();
+    if (p >= A.n) return;
+    const int c = A.cnt[p];
+    const int64_t b0 = pdx_bond0(A.slice_ptr, p);
+    const double4 ri = A.rec_in[p];
     double wvol = 0.0, S = 0.0;
     int na = 0;
-    for (int64_t b = b0 + lane; b < b1; b += 64) {
+    for (int t = 0; t < c; t++) {
+        const int64_t b = b0 + (int64_t)t * 64;
         const uint32_t wd = A.bw[b];
-        const uint32_t j = wd & PDX_BW_NBR;
-        const double4 rj = A.rec_in[j];
+        const double4 rj = A.rec_in[wd & PDX_BW_NBR];
         const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
         const double Lb = A.L[b];
-        const double l = sqrt((dx * dx + dy * dy) + dz * dz);
-        const double eps = (l - Lb) / Lb;
-        bool act = (wd & PDX_BW_ACT) != 0;
-        if (act && (wd & PDX_BW_FP) && eps > A.epsc) {
-            act = false;
-            A.bw[b] = wd & ~PDX_BW_ACT;
-        }
+        double rl;
+        const double l = pdx_sqrt_rsq((dx * dx + dy * dy) + dz * dz, rl);
+        const double e = l - Lb;
+        const double rL = pdx_rcp1(Lb);
+        const bool act = bond_fail(A, A.bw + b, wd, e, Lb, e * rL);
         if (act) {
-            const double w = A.delta / Lb;  // linear_kernel
+            const double w = A.delta * rL;  // linear_kernel(delta, L)
             wvol += (w * (Lb * Lb)) * rj.w;
-            S += ((w * Lb) * (l - Lb)) * rj.w;
+            S += ((w * Lb) * e) * rj.w;
             na++;
         }
     }
-    wvol = pdx_wave_sum(wvol);
-    S = pdx_wave_sum(S);
-    na = pdx_wave_sum_i(na);
-    if (lane == 0) {
-        double c1 = 0.0, c2 = 0.0;
-        if (wvol != 0.0) {
-            const double dil = (3.0 / wvol) * S;
-            c1 = (15.0 * A.G) / wvol;
-            c2 = dil * ((3.0 * A.K) / wvol - c1 / 3.0);
-        }
-        A.pstate[2 * i] = c1;
-        A.pstate[2 * i + 1] = c2;
-        if (A.flags & (PDX_F_STORE | PDX_F_FORCE_ONLY)) {
-            A.nact_out[i] = na;
-            A.dmg_out[i] = 1.0 - (double)na / (double)(b1 - b0);
-        }
+    double c1 = 0.0, c2 = 0.0;
+    if (wvol != 0.0) {
+        const double dil = (3.0 / wvol) * S;
+        c1 = (15.0 * A.G) / wvol;
+        c2 = dil * ((3.0 * A.K) / wvol - c1 / 3.0);
+    }
+    reinterpret_cast<double2 *>(A.pstate)[p] = make_double2(c1, c2);
+    if (A.flags & (PDX_F_STORE | PDX_F_FORCE_ONLY)) {
+        A.nact_out[p] = na;
+        A.dmg_out[p] = 1.0 - (double)na / (double)c;
     }
 }
 
@@ -369,14 +418,15 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_osb_pass1(pdx_step_args A)
 // b_i = sum_j (w/l) dx_ij [ (c2_i L + c1_i (l-L)) V_j + (c2_j L + c1_j (l-L)) V_i ]
 __global__ __launch_bounds__(PDX_BLOCK) void k_osb_pass2(pdx_step_args A)
 {
-    const int64_t i = pdx_wave_point();
-    if (i >= A.n) return;
-    const int lane = threadIdx.x & 63;
-    const double4 ri = A.rec_in[i];
-    const double c1i = A.pstate[2 * i], c2i = A.pstate[2 * i + 1];
-    const int64_t b0 = A.row_ptr[i], b1 = A.row_ptr[i + 1];
+    const int64_t p = pdx_thread_point();
+    if (p >= A.n) return;
+    const int c = A.cnt[p];
+    const int64_t b0 = pdx_bond0(A.slice_ptr, p);
+    const double4 ri = A.rec_in[p];
+    const double2 ci = reinterpret_cast<const double2 *>(A.pstate)[p];
     double bx = 0.0, by = 0.0, bz = 0.0;
-    for (int64_t b = b0 + lane; b < b1; b += 64) {
+    for (int t = 0; t < c; t++) {
+        const int64_t b = b0 + (int64_t)t * 64;
         const uint32_t wd = A.bw[b];
         if (!(wd & PDX_BW_ACT)) continue;
         const uint32_t j = wd & PDX_BW_NBR;
@@ -384,21 +434,21 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_osb_pass2(pdx_step_args A)
         const double2 cj = reinterpret_cast<const double2 *>(A.pstate)[j];
         const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
         const double Lb = A.L[b];
-        const double l = sqrt((dx * dx + dy * dy) + dz * dz);
-        const double w = A.delta / Lb;
+        double rl;
+        const double l = pdx_sqrt_rsq((dx * dx + dy * dy) + dz * dz, rl);
+        const double w = A.delta * pdx_rcp1(Lb);
         const double e = l - Lb;
-        const double fi = c2i * Lb + c1i * e;
+        const double fi = ci.y * Lb + ci.x * e;
         const double fj = cj.y * Lb + cj.x * e;
-        const double s = (w / l) * (fi * rj.w + fj * ri.w);
-        bx += s * dx;
-        by += s * dy;
-        bz += s * dz;
+        const double s = (w * rl) * (fi * rj.w + fj * ri.w);
+        bx = __builtin_fma(s, dx, bx);
+        by = __builtin_fma(s, dy, by);
+        bz = __builtin_fma(s, dz, bz);
     }
-    bx = pdx_wave_sum(bx);
-    by = pdx_wave_sum(by);
-    bz = pdx_wave_sum(bz);
-    // damage and n_active were written by pass 1
-    step_tail<false>(A, i, lane, ri, bx, by, bz, 0, b1 - b0);
+    tail_ops T;
+    tail_load(A, p, T);
+    const double b[3] = {bx, by, bz};
+    point_tail(A, p, ri, b, T);
 }
 
 // ----------------------------------------------------------------------------------------
@@ -497,17 +547,18 @@ __device__ void pk1(int model, const double *F, double lambda, double mu, double
 // calc_first_piola_kirchhoff! (:216-224).  Stores F and P*K^-1 of every point.
 __global__ __launch_bounds__(PDX_BLOCK) void k_cpd_pass1(pdx_step_args A)
 {
-    const int64_t i = pdx_wave_point();
-    if (i >= A.n) return;
-    const int lane = threadIdx.x & 63;
-    const double4 ri = A.rec_in[i];
-    const double4 Xi = A.Xr[i];
-    const bool dead = A.killed[i] != 0;
-    const int64_t b0 = A.row_ptr[i], b1 = A.row_ptr[i + 1];
+    const int64_t p = pdx_thread_point();
+    if (p >= A.n) return;
+    const int c = A.cnt[p];
+    const int64_t b0 = pdx_bond0(A.slice_ptr, p);
+    const double4 ri = A.rec_in[p];
+    const double4 Xi = A.Xr[p];
+    const bool dead = A.killed[p] != 0;
     double Km[6] = {0, 0, 0, 0, 0, 0};  // 00 10 20 11 21 22
     double Ft[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     int na = 0;
-    for (int64_t b = b0 + lane; b < b1; b += 64) {
+    for (int t = 0; t < c; t++) {
+        const int64_t b = b0 + (int64_t)t * 64;
         const uint32_t wd = A.bw[b];
         const uint32_t j = wd & PDX_BW_NBR;
         const double4 rj = A.rec_in[j];
@@ -515,16 +566,14 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_cpd_pass1(pdx_step_args A)
         const double dx[3] = {rj.x - ri.x, rj.y - ri.y, rj.z - ri.z};
         const double DX[3] = {Xj.x - Xi.x, Xj.y - Xi.y, Xj.z - Xi.z};
         const double Lb = A.L[b];
-        const double l = sqrt((dx[0] * dx[0] + dx[1] * dx[1]) + dx[2] * dx[2]);
-        const double eps = (l - Lb) / Lb;
-        bool act = (wd & PDX_BW_ACT) != 0;
-        if (act && (wd & PDX_BW_FP) && eps > A.epsc) {
-            act = false;
-            A.bw[b] = wd & ~PDX_BW_ACT;
-        }
+        double rl;
+        const double l = pdx_sqrt_rsq((dx[0] * dx[0] + dx[1] * dx[1]) + dx[2] * dx[2], rl);
+        const double e = l - Lb;
+        const double rL = pdx_rcp1(Lb);
+        const bool act = bond_fail(A, A.bw + b, wd, e, Lb, e * rL);
         if (act && !dead) {
             na++;
-            const double temp = (A.delta / Lb) * rj.w;
+            const double temp = (A.delta * rL) * rj.w;
             Km[0] += temp * (DX[0] * DX[0]);
             Km[1] += temp * (DX[1] * DX[0]);
             Km[2] += temp * (DX[2] * DX[0]);
@@ -532,21 +581,16 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_cpd_pass1(pdx_step_args A)
             Km[4] += temp * (DX[2] * DX[1]);
             Km[5] += temp * (DX[2] * DX[2]);
 #pragma unroll
-            for (int c = 0; c < 3; c++)
+            for (int cc = 0; cc < 3; cc++)
 #pragma unroll
-                for (int r = 0; r < 3; r++) Ft[r + 3 * c] += temp * (dx[r] * DX[c]);
+                for (int r = 0; r < 3; r++) Ft[r + 3 * cc] += temp * (dx[r] * DX[cc]);
         }
     }
-#pragma unroll
-    for (int k = 0; k < 6; k++) Km[k] = pdx_wave_sum(Km[k]);
-#pragma unroll
-    for (int k = 0; k < 9; k++) Ft[k] = pdx_wave_sum(Ft[k]);
-    na = pdx_wave_sum_i(na);
     const double K9[9] = {Km[0], Km[1], Km[2], Km[1], Km[3], Km[4], Km[2], Km[4], Km[5]};
     double Kinv[9], F[9];
     m3_inv(K9, Kinv);
     m3_mul(Ft, Kinv, F);
-    const double dmg = 1.0 - (double)na / (double)(b1 - b0);
+    const double dmg = 1.0 - (double)na / (double)c;
     bool hasnan = false;
 #pragma unroll
     for (int k = 0; k < 9; k++) hasnan |= isnan(F[k]);
@@ -559,25 +603,25 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_cpd_pass1(pdx_step_args A)
         m3_mul(P, Kinv, PK);
         if (store) {
             // cauchy_stress: stress.jl:112-116, sigma = (1/J) P F'
-            const double s = 1.0 / m3_det(F);
-            if (lane < 9) {
-                const int r = lane % 3, c = lane / 3;
-                A.cauchy[9 * i + lane] =
-                    ((s * P[r]) * F[c] + (s * P[r + 3]) * F[c + 3]) + (s * P[r + 6]) * F[c + 6];
+            const double sj = 1.0 / m3_det(F);
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                const int r = k % 3, cc = k / 3;
+                A.cauchy[9 * p + k] =
+                    ((sj * P[r]) * F[cc] + (sj * P[r + 3]) * F[cc + 3]) + (sj * P[r + 6]) * F[cc + 6];
             }
         }
     }
-    if (lane < 9) {
-        A.pstate[18 * i + lane] = F[lane];
-        A.pstate[18 * i + 9 + lane] = PK[lane];
-        if (store) A.defgrad[9 * i + lane] = F[lane];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        A.pstate[18 * p + k] = F[k];
+        A.pstate[18 * p + 9 + k] = PK[k];
+        if (store) A.defgrad[9 * p + k] = F[k];
     }
-    if (lane == 0) {
-        if (kill && !dead) A.killed[i] = 1;
-        if (store) {
-            A.nact_out[i] = kill ? 0 : na;
-            A.dmg_out[i] = dmg;
-        }
+    if (kill && !dead) A.killed[p] = 1;
+    if (store) {
+        A.nact_out[p] = kill ? 0 : na;
+        A.dmg_out[p] = dmg;
     }
 }
 
@@ -586,21 +630,22 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_cpd_pass1(pdx_step_args A)
 // reverse  -t_ji V_i  = V_i [ (w PKinv_j) dX + (w beta) (dx - F_j dX) ]
 __global__ __launch_bounds__(PDX_BLOCK) void k_cpd_pass2(pdx_step_args A)
 {
-    const int64_t i = pdx_wave_point();
-    if (i >= A.n) return;
-    const int lane = threadIdx.x & 63;
-    const double4 ri = A.rec_in[i];
-    const double4 Xi = A.Xr[i];
-    const bool dead_i = A.killed[i] != 0;
+    const int64_t p = pdx_thread_point();
+    if (p >= A.n) return;
+    const int c = A.cnt[p];
+    const int64_t b0 = pdx_bond0(A.slice_ptr, p);
+    const double4 ri = A.rec_in[p];
+    const double4 Xi = A.Xr[p];
+    const bool dead_i = A.killed[p] != 0;
     double Fi[9], PKi[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) {
-        Fi[k] = A.pstate[18 * i + k];
-        PKi[k] = A.pstate[18 * i + 9 + k];
+        Fi[k] = A.pstate[18 * p + k];
+        PKi[k] = A.pstate[18 * p + 9 + k];
     }
-    const int64_t b0 = A.row_ptr[i], b1 = A.row_ptr[i + 1];
     double b3[3] = {0.0, 0.0, 0.0};
-    for (int64_t b = b0 + lane; b < b1; b += 64) {
+    for (int t = 0; t < c; t++) {
+        const int64_t b = b0 + (int64_t)t * 64;
         const uint32_t wd = A.bw[b];
         if (!(wd & PDX_BW_ACT)) continue;
         const uint32_t j = wd & PDX_BW_NBR;
@@ -615,32 +660,26 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_cpd_pass2(pdx_step_args A)
             for (int r = 0; r < 3; r++) {
                 const double FX = (Fi[r] * DX[0] + Fi[r + 3] * DX[1]) + Fi[r + 6] * DX[2];
                 const double tz = wb * (dx[r] - FX);
-                const double t = (((w * PKi[r]) * DX[0] + (w * PKi[r + 3]) * DX[1]) +
-                                  (w * PKi[r + 6]) * DX[2]) + tz;
-                b3[r] += t * rj.w;
+                const double tt = (((w * PKi[r]) * DX[0] + (w * PKi[r + 3]) * DX[1]) +
+                                   (w * PKi[r + 6]) * DX[2]) + tz;
+                b3[r] += tt * rj.w;
             }
         }
         if (!A.killed[j]) {
             const double *pj = A.pstate + 18 * (int64_t)j;
-            double Fj[9], PKj[9];
-#pragma unroll
-            for (int k = 0; k < 9; k++) {
-                Fj[k] = pj[k];
-                PKj[k] = pj[9 + k];
-            }
 #pragma unroll
             for (int r = 0; r < 3; r++) {
-                const double FX = (Fj[r] * DX[0] + Fj[r + 3] * DX[1]) + Fj[r + 6] * DX[2];
+                const double FX = (pj[r] * DX[0] + pj[r + 3] * DX[1]) + pj[r + 6] * DX[2];
                 const double tz = wb * (dx[r] - FX);
-                const double t = (((w * PKj[r]) * DX[0] + (w * PKj[r + 3]) * DX[1]) +
-                                  (w * PKj[r + 6]) * DX[2]) + tz;
-                b3[r] += t * ri.w;
+                const double tt = (((w * pj[9 + r]) * DX[0] + (w * pj[9 + r + 3]) * DX[1]) +
+                                   (w * pj[9 + r + 6]) * DX[2]) + tz;
+                b3[r] += tt * ri.w;
             }
         }
     }
-#pragma unroll
-    for (int r = 0; r < 3; r++) b3[r] = pdx_wave_sum(b3[r]);
-    step_tail<false>(A, i, lane, ri, b3[0], b3[1], b3[2], 0, b1 - b0);
+    tail_ops T;
+    tail_load(A, p, T);
+    point_tail(A, p, ri, b3, T);
 }
 
 // ----------------------------------------------------------------------------------------
@@ -683,8 +722,12 @@ pdx_status pdx_fill_step_args(pdx_ctx *c, pdx_step_args *a, int64_t step, int fl
     a->rho = c->P.rho;
     a->dt = dt;
     a->dth = 0.5 * dt;
-    a->row_ptr = c->row_ptr;
+    a->slice_ptr = c->slice_ptr;
+    a->cnt = c->cnt;
     a->bw = c->bw;
+    a->bw16 = c->bw16;
+    a->halo_ptr = c->halo_ptr;
+    a->halo = c->halo;
     a->L = c->L;
     a->Lf = c->Lf;
     a->rec_in = c->rec[c->cur];
@@ -726,27 +769,30 @@ pdx_status pdx_fill_step_args(pdx_ctx *c, pdx_step_args *a, int64_t step, int fl
 
 pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
 {
-    const int64_t grid = pdx_wave_grid(c->n);
-    const dim3 G((unsigned)grid), B(PDX_BLOCK);
-    const dim3 G2((unsigned)pdx_wave_grid((c->n + 1) / 2));
+    const int64_t blocks = (c->n + PDX_BLOCK - 1) / PDX_BLOCK;
+    const dim3 G((unsigned)((blocks + PDX_NUM_XCD - 1) / PDX_NUM_XCD * PDX_NUM_XCD)), B(PDX_BLOCK);
     switch (c->kind) {
-    case PDX_BB: {
+    case PDX_BB:
         PDX_TRY(pdx_timed_begin(c));
-        const int v = c->bb_variant;
-        const dim3 G1((unsigned)pdx_wave_grid(c->n));
-        if (c->precision == PDX_MIXED) {
-            if (v == 22) hipLaunchKernelGGL((k_bb_step<true, 2, 2>), G2, B, 0, c->stream, a);
-            else if (v == 11) hipLaunchKernelGGL((k_bb_step<true, 1, 1>), G1, B, 0, c->stream, a);
-            else hipLaunchKernelGGL((k_bb_step<true, 1, 2>), G1, B, 0, c->stream, a);
+        if (c->lds_mode) {
+            const size_t lds = sizeof(double4) * (size_t)std::max(c->halo_max, 1);
+            if (c->precision == PDX_MIXED) {
+                if (c->bb_unroll == 4) hipLaunchKernelGGL((k_bb_lds<true, 4>), G, B, lds, c->stream, a);
+                else hipLaunchKernelGGL((k_bb_lds<true, 2>), G, B, lds, c->stream, a);
+            } else {
+                if (c->bb_unroll == 4) hipLaunchKernelGGL((k_bb_lds<false, 4>), G, B, lds, c->stream, a);
+                else hipLaunchKernelGGL((k_bb_lds<false, 2>), G, B, lds, c->stream, a);
+            }
+        } else if (c->precision == PDX_MIXED) {
+            if (c->bb_unroll == 4) hipLaunchKernelGGL((k_bb_step<true, 4>), G, B, 0, c->stream, a);
+            else hipLaunchKernelGGL((k_bb_step<true, 2>), G, B, 0, c->stream, a);
         } else {
-            if (v == 22) hipLaunchKernelGGL((k_bb_step<false, 2, 2>), G2, B, 0, c->stream, a);
-            else if (v == 11) hipLaunchKernelGGL((k_bb_step<false, 1, 1>), G1, B, 0, c->stream, a);
-            else hipLaunchKernelGGL((k_bb_step<false, 1, 2>), G1, B, 0, c->stream, a);
+            if (c->bb_unroll == 4) hipLaunchKernelGGL((k_bb_step<false, 4>), G, B, 0, c->stream, a);
+            else hipLaunchKernelGGL((k_bb_step<false, 2>), G, B, 0, c->stream, a);
         }
         PDX_HIP(c, hipGetLastError());
         PDX_TRY(pdx_timed_end(c, "step"));
         break;
-    }
     case PDX_OSB:
         PDX_TRY(pdx_timed_begin(c));
         hipLaunchKernelGGL(k_osb_pass1, G, B, 0, c->stream, a);

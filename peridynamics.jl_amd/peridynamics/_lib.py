@@ -265,6 +265,44 @@ class Context:
             raise NaNError(st, msg, step=nan_step.value, t=nan_step.value * dt)
         check(self.h, st, "pdx_vv_run")
 
+    def set_displacement_bcs(self, points, dims, values):
+        points = np.ascontiguousarray(points, np.int32)
+        dims = np.ascontiguousarray(dims, np.int32)
+        values = np.ascontiguousarray(values, np.float64)
+        self.call("pdx_set_displacement_bcs", points.shape[0], ptr(points), ptr(dims), ptr(values))
+
+    def residual(self):
+        r = np.zeros((self.n, 3), np.float64)
+        nrm = _D()
+        self.call("pdx_residual", ptr(r), ctypes.byref(nrm))
+        return r, nrm.value
+
+    def jv(self, v, scale=1.0):
+        v = np.ascontiguousarray(v, np.float64).reshape(self.n, 3)
+        out = np.zeros((self.n, 3), np.float64)
+        self.call("pdx_jv", ptr(v), ptr(out), float(scale))
+        return out
+
+    def nk_step(self, n, n_steps, opts):
+        st = PdxNkStats()
+        status = self.lib.pdx_nk_step(self.h, int(n), int(n_steps), ctypes.byref(opts),
+                                      ctypes.byref(st))
+        if status == PDX_ERR_NOT_CONVERGED:
+            msg = self.lib.pdx_last_error(self.h).decode()
+            err = NotConvergedError(status, msg)
+            err.stats = st
+            raise err
+        check(self.h, status, "pdx_nk_step")
+        return st
+
+    def gnn_collect(self, on=True):
+        self.call("pdx_gnn_collect", 1 if on else 0)
+
+    def gnn_train(self, epochs, lr, seed=0):
+        loss = _D()
+        self.call("pdx_gnn_train", int(epochs), float(lr), ctypes.c_uint64(seed), ctypes.byref(loss))
+        return loss.value
+
     def set_profiling(self, on=True):
         self.call("pdx_set_profiling", 1 if on else 0)
 

@@ -186,7 +186,8 @@ def velocity_bc(f, body, set_name, dim):
 
 
 def forcedensity_bc(f, body, set_name, dim):
-    """forcedensity_bc!: boundary_conditions.jl:559-575 (on b_ext)."""
+    """forcedensity_bc!: boundary_conditions.jl:559-575 (on b_ext).  f(p) is the
+    position-dependent form that the Newton-Krylov solver scales by the load factor."""
     _check_set(body, set_name)
     kind = condition_function_type(f)
     d = get_dim(dim)
@@ -195,7 +196,7 @@ def forcedensity_bc(f, body, set_name, dim):
     elif kind == "pdsdbc":
         _add_bc(body, body.posdep_single_dim_bcs, "b_ext", set_name, d, f)
     else:
-        raise _arg_error("forcedensity_bc! needs f(t) or f(p, t)")
+        _add_bc(body, body.pos_single_dim_bcs, "b_ext", set_name, d, f)
 
 
 def displacement_bc(f, body, set_name, dim):

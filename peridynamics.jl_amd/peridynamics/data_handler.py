@@ -147,6 +147,66 @@ class MI355XBodyDataHandler:
                 on_export(self, n - 1, (n - 1) * dt)
         return self
 
+    # ------------------------------------------------------------------ Newton-Krylov
+    PRECONDS = {"none": 0, "jacobi": 1, "learned": 2}
+
+    def nk_options(self, nk):
+        return PdxNkOpts(maxiter=nk.maxiter, gmres_maxiter=nk.gmres_maxiter,
+                         gmres_restart=nk.gmres_restart, precond=self.PRECONDS[nk.precond],
+                         tol=nk.tol, rel_tol=nk.rel_tol, gmres_reltol=nk.gmres_reltol,
+                         gmres_abstol=nk.gmres_abstol,
+                         perturbation_scale=nk.perturbation_scale)
+
+    def init_newton_krylov(self, nk):
+        """validate_chunk_for_newton_krylov! (newton_krylov.jl:222-256) and the constrained
+        dofs of displacement_bc! (full-load values f(X), scaled per load step)."""
+        body = self.body
+        if body.single_dim_bcs or body.posdep_single_dim_bcs:
+            raise _arg_error("NewtonKrylov solver does not support all boundary condition types!\n"
+                             "Specify only position-dependent boundary conditions!")
+        if self.params.has_fracture():
+            raise _arg_error("NewtonKrylov solver currently does not support damage or fracture!")
+        pts, dims, vals = [], [], []
+        X = body.position
+        for fld, s, d, f in body.pos_single_dim_bcs:
+            if fld != "displacement":
+                continue
+            P = body.point_sets[s]
+            pts.append(P)
+            dims.append(np.full(len(P), d, np.int32))
+            vals.append(eval_posdep(f, X[:, P]))
+        if pts:
+            self.ctx.set_displacement_bcs(np.concatenate(pts), np.concatenate(dims),
+                                          np.concatenate(vals))
+        else:
+            self.ctx.set_displacement_bcs(np.zeros(0, np.int32), np.zeros(0, np.int32),
+                                          np.zeros(0))
+        self._bext_pos = [(body.point_sets[s], d, f) for fld, s, d, f in body.pos_single_dim_bcs
+                          if fld == "b_ext"]
+
+    def nk_load_step(self, nk, n):
+        """newton_krylov_step!(dh, nk, options, n): newton_krylov.jl:298-332."""
+        beta = n / nk.n_steps
+        if self._bext_pos:
+            bext = np.zeros((self.n, 3))
+            X = self.body.position
+            for P, d, f in self._bext_pos:
+                val = beta * eval_posdep(f, X[:, P])
+                ok = np.isfinite(val)
+                bext[P[ok], d] = val[ok]
+            self.set("b_ext", bext)
+        return self.ctx.nk_step(n, nk.n_steps, self.nk_options(nk))
+
+    def solve_nk(self, nk, on_export=None):
+        """solve!(dh, nk, options): newton_krylov.jl:280-295.  Returns the per-step stats."""
+        self.init_newton_krylov(nk)
+        stats = []
+        for n in range(1, nk.n_steps + 1):
+            stats.append(self.nk_load_step(nk, n))
+            if on_export is not None:
+                on_export(self, n, n * nk.dt)
+        return stats
+
     # ------------------------------------------------------------------ fields
     def get(self, name):
         """A storage field in the caller's point order, shape (n, 3), (n, 9) or (n,)."""

@@ -37,14 +37,26 @@ def make_body(nx, ny, nz, dx=1e-3, jitter=0.05, seed=0x5EED):
     return body
 
 
-def step_bytes(n_points, n_bonds, precision):
-    """Algorithmic HBM bytes of one fused step (perfect reuse of neighbour data):
-    per bond: bond word (neighbour | active | fail_permit, 4) + reference length (8 FP64 /
-              4 FP32); a bond word is written back only when the bond breaks;
-    per point: read x|V (32), u (24), v_half (24), row_ptr (8), BC slot (4);
-               write x|V (32), u (24), v_half (24)."""
+def step_bytes(n_points, n_bonds, precision, layout):
+    """Algorithmic HBM bytes of one fused bond-based step: every array the step needs is
+    read or written once (neighbour positions are re-used from cache, so a neighbour's x|V
+    counts once, as its own point's).
+
+    per point: read x|V (32), u (24), v_half (24), bond count (4), BC slot (4); write x|V
+               (32), u (24), v_half (24)  -> 168 B, plus 8 B per 64-point slice pointer;
+    per bond:  LDS-staged layout: 16-bit bond word (2) + FP64 reference length (8), in both
+               precisions (the mixed step forms l^2 - L^2 in FP64);
+               global-gather layout: 32-bit bond word (4) + length (8 FP64 / 4 FP32);
+    per neighbour-table entry (LDS layout): its 4-byte point index.
+    A bond word is written back only when the bond breaks (not counted)."""
+    n_slices = (n_points + 63) // 64
+    per_point = 168 * n_points + 8 * n_slices
+    if layout["lds_mode"]:
+        return (10 * n_bonds + per_point + 4 * layout["halo_entries"],
+                "10*N_b + 168*N_p + 8*N_slices + 4*N_halo")
     per_bond = 12 if precision == "fp64" else 8
-    return per_bond * n_bonds + 172 * n_points
+    return (per_bond * n_bonds + per_point,
+            f"{per_bond}*N_b + 168*N_p + 8*N_slices")
 
 
 def cpu_baseline(sample_shape, target_seconds=12.0, max_steps=5000):
@@ -136,7 +148,8 @@ def main():
     step0 += args.profile_steps
     kms, kcount = ctx.kernel_time("step")
     ctx.set_profiling(False)
-    bytes_step = step_bytes(n_points, n_bonds, args.precision)
+    layout = ctx.layout()
+    bytes_step, bytes_formula = step_bytes(n_points, n_bonds, args.precision, layout)
     achieved = bytes_step / (kms * 1e-3) / 1e9
     log(f"[bench] step kernel {kms:.4f} ms avg over {kcount} launches, {achieved:.1f} GB/s")
     u = dh.get("displacement")
@@ -161,10 +174,10 @@ def main():
                    "dt": dt},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_bb_step", "kernel_ms": kms,
-                     "bytes_formula": ("12*N_b + 172*N_p" if args.precision == "fp64"
-                                       else "8*N_b + 172*N_p"),
-                     "bytes_per_launch": bytes_step},
+                     "kernel": "k_bb_lds" if layout["lds_mode"] else "k_bb_step",
+                     "kernel_ms": kms, "bytes_formula": bytes_formula,
+                     "bytes_per_launch": bytes_step, "halo_entries": layout["halo_entries"],
+                     "bond_slots": layout["bond_slots"]},
     }
     dh.close()
 
@@ -187,7 +200,8 @@ def main():
         out["mixed"] = {"value": n_bonds * args.steps / em, "ms_per_step": em / args.steps * 1e3,
                         "kernel_ms": kmm,
                         "speedup_vs_fp64_kernel": kms / kmm if kmm else None,
-                        "achieved_GBs": step_bytes(n_points, n_bonds, "mixed") / (kmm * 1e-3) / 1e9}
+                        "achieved_GBs": step_bytes(n_points, n_bonds, "mixed", cm.layout())[0]
+                        / (kmm * 1e-3) / 1e9}
         dhm.close()
 
     if not args.no_cpu:

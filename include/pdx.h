@@ -180,12 +180,36 @@ pdx_status pdx_nk_step(pdx_ctx *ctx, int64_t n, int64_t n_steps, const pdx_nk_op
                        pdx_nk_stats *stats);
 
 /* --- learned preconditioner (no counterpart in the reference) ---------------------------- */
-/* Collects (residual, Krylov correction) pairs from the plain solves of pdx_nk_step while
- * enabled, trains the message-passing preconditioner on the device, and reports the final
- * training loss. */
-pdx_status pdx_gnn_collect(pdx_ctx *ctx, int32_t enable);
-pdx_status pdx_gnn_train(pdx_ctx *ctx, int32_t epochs, double lr, uint64_t seed,
+/* The reference solves J du = -r with IterativeSolvers.gmres! and no preconditioner
+ * (newton_krylov.jl:448-450).  These entry points build, train and expose the
+ * message-passing preconditioner that pdx_nk_step uses with opts.precond = 2.
+ *
+ * While collection is enabled, every pdx_nk_step records its Newton pairs (-r, du) and up
+ * to max_krylov_pairs Krylov pairs (J z, z) (reservoir-sampled) as float32 on the device. */
+pdx_status pdx_gnn_collect(pdx_ctx *ctx, int32_t enable, int64_t max_krylov_pairs);
+pdx_status pdx_gnn_pairs(pdx_ctx *ctx, int64_t *n_newton, int64_t *n_krylov);
+/* `steps` Adam steps (cosine-decayed step size lr) on batches of `batch` pairs drawn from
+ * the collected ones (and random combinations of two); *loss_out = last batch loss. */
+pdx_status pdx_gnn_train(pdx_ctx *ctx, int32_t steps, int32_t batch, double lr, uint64_t seed,
                          double *loss_out);
+/* Number of float32 parameters of the network. */
+int64_t pdx_gnn_n_params(void);
+/* Reads (theta_out) or replaces (theta_in, with the output scale *out_scale) the
+ * parameters; replacing marks the network as trained. */
+pdx_status pdx_gnn_params(pdx_ctx *ctx, float *theta_out, const float *theta_in,
+                          double *out_scale);
+/* Test access, all in the library's internal (Morton) point order: permutation
+ * perm[internal] = caller index, near-bond graph (CSR), static features (5 per point),
+ * point-block Jacobi inverse at the current state (9 per point, column-major), constrained
+ * mask (3 per point).  Any pointer may be NULL; call with only n_near first. */
+pdx_status pdx_gnn_graph(pdx_ctx *ctx, int64_t *n_near, int64_t *perm, int64_t *near_ptr,
+                         int32_t *near, float *stat, double *dinv, uint8_t *cmask);
+/* z = M(y) for one 3n vector (internal order). */
+pdx_status pdx_gnn_apply_vec(pdx_ctx *ctx, const double *y, double *z);
+/* Loss and gradient of the current parameters over n_samples pairs (Y, X: n_samples rows of
+ * 3n, internal order) without updating them. */
+pdx_status pdx_gnn_loss_grad(pdx_ctx *ctx, int32_t n_samples, const double *Y, const double *X,
+                             double *loss, float *grad);
 
 /* --- instrumentation ---------------------------------------------------------------------- */
 /* Per-launch HIP-event timing of the step kernels (off by default). */
@@ -195,6 +219,12 @@ pdx_status pdx_set_profiling(pdx_ctx *ctx, int32_t enable);
 pdx_status pdx_kernel_time(pdx_ctx *ctx, const char *name, double *avg_ms, int64_t *count);
 /* Sizes: n points, n bonds, device bytes held. */
 pdx_status pdx_info(pdx_ctx *ctx, int64_t *n_points, int64_t *n_bonds, int64_t *device_bytes);
+/* Device layout of the bond lists (no counterpart in the reference, whose bond lists are
+ * plain vectors): 1 if the bond-based step stages neighbour data in LDS, the number of bond
+ * slots including the padding of 64-point slices, the total and the largest per-block
+ * neighbour table.  Used to state the step kernel's algorithmic bytes. */
+pdx_status pdx_layout(pdx_ctx *ctx, int32_t *lds_mode, int64_t *bond_slots,
+                      int64_t *halo_entries, int32_t *halo_max);
 
 #ifdef __cplusplus
 }

@@ -19,7 +19,6 @@ struct pdx_nk {
     double *h_dev = nullptr;    // small coefficient vectors
     double *dinv = nullptr;     // 9n point-block Jacobi inverse
     int64_t n_jv = 0;
-    bool collect = false;       // record (rhs, correction) pairs for training
 };
 
 // Jv on device vectors (internal order)
@@ -28,4 +27,7 @@ pdx_status pdx_nk_jv_dev(pdx_ctx *c, const double *v, double *jv, double scale);
 // learned preconditioner (pdx_gnn.hip)
 pdx_status pdx_gnn_prepare(pdx_ctx *c);                                  // per Newton iteration
 pdx_status pdx_gnn_apply(pdx_ctx *c, const double *v, double *z);        // z = M^-1 v
-pdx_status pdx_gnn_record(pdx_ctx *c, const double *rhs, const double *du);  // training pair
+pdx_status pdx_gnn_record(pdx_ctx *c, const double *rhs, const double *du);  // Newton pair
+pdx_status pdx_gnn_record_krylov(pdx_ctx *c, const double *z, const double *jz);  // Krylov pair
+bool pdx_gnn_collecting(pdx_ctx *c);
+pdx_status pdx_nk_build_jacobi(pdx_ctx *c);  // point-block Jacobi at the current state

@@ -371,6 +371,12 @@ static pdx_status build_jacobi(pdx_ctx *c)
     return PDX_OK;
 }
 
+pdx_status pdx_nk_build_jacobi(pdx_ctx *c)
+{
+    if (!c->nk) return pdx_fail(c, PDX_ERR_STATE, "Newton-Krylov workspace missing");
+    return build_jacobi(c);
+}
+
 // Flexible GMRES(restart) for J du = b (b = -r in k->t0), du in k->du (starts at zero).
 // Stops at ||res|| <= max(reltol ||b||, abstol) or after maxiter products.  Returns the
 // number of products; *rel_out is the final residual estimate / ||b||.
@@ -405,6 +411,7 @@ static pdx_status fgmres(pdx_ctx *c, const double *b, int restart, int maxiter, 
             PDX_TRY(precond_apply(c, precond, Vj, Zj));
             PDX_TRY(pdx_nk_jv_dev(c, Zj, w, scale));
             it++;
+            if (pdx_gnn_collecting(c)) PDX_TRY(pdx_gnn_record_krylov(c, Zj, w));
             // classical Gram-Schmidt, twice
             PDX_TRY(multidot(c, j + 1, k->Vb, w, h.data()));
             PDX_HIP(c, hipMemcpyAsync(k->h_dev, h.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice, c->stream));
@@ -599,7 +606,7 @@ extern "C" pdx_status pdx_nk_step(pdx_ctx *c, int64_t n, int64_t n_steps, const 
                            o->gmres_abstol, pre, scale, &its, &rel));
             S.krylov_iters += its;
         }
-        if (k->collect) PDX_TRY(pdx_gnn_record(c, k->t0, k->du));
+        if (pdx_gnn_collecting(c)) PDX_TRY(pdx_gnn_record(c, k->t0, k->du));
         hipLaunchKernelGGL(k_nk_update, grid1(k->m), dim3(256), 0, c->stream, c->n, k->du, k->cmask,
                            c->u, c->rec[c->cur]);
         PDX_HIP(c, hipGetLastError());

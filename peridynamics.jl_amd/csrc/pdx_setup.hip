@@ -547,6 +547,7 @@ static pdx_status build_halos(pdx_ctx *c, bool *ok)
     PDX_HIP(c, hipMemcpy(c->halo_ptr, hp.data(), sizeof(int64_t) * (nblk + 1), hipMemcpyHostToDevice));
     PDX_HIP(c, hipMemcpy(c->halo, halo.data(), sizeof(int32_t) * halo.size(), hipMemcpyHostToDevice));
     c->halo_max = (int32_t)hmax;
+    c->halo_total = hp[nblk];
     c->lds_mode = true;
     *ok = true;
     return PDX_OK;
@@ -1131,6 +1132,18 @@ extern "C" pdx_status pdx_info(pdx_ctx *c, int64_t *np, int64_t *nb, int64_t *by
     if (np) *np = c->n;
     if (nb) *nb = c->nb;
     if (bytes) *bytes = c->dev_bytes;
+    return PDX_OK;
+}
+
+extern "C" pdx_status pdx_layout(pdx_ctx *c, int32_t *lds_mode, int64_t *bond_slots,
+                                 int64_t *halo_entries, int32_t *halo_max)
+{
+    if (!c) return PDX_ERR_ARG;
+    if (!c->bonds_built) return pdx_fail(c, PDX_ERR_STATE, "pdx_build_bonds first");
+    if (lds_mode) *lds_mode = c->lds_mode ? 1 : 0;
+    if (bond_slots) *bond_slots = c->nb_alloc;
+    if (halo_entries) *halo_entries = c->lds_mode ? c->halo_total : 0;
+    if (halo_max) *halo_max = c->lds_mode ? c->halo_max : 0;
     return PDX_OK;
 }
 

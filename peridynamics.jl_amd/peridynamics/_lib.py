@@ -123,11 +123,19 @@ SIGNATURES = {
     "pdx_residual": (_ST, [_P, _P, ctypes.POINTER(_D)]),
     "pdx_nk_step": (_ST, [_P, _I64, _I64, ctypes.POINTER(PdxNkOpts),
                           ctypes.POINTER(PdxNkStats)]),
-    "pdx_gnn_collect": (_ST, [_P, _I32]),
-    "pdx_gnn_train": (_ST, [_P, _I32, _D, ctypes.c_uint64, ctypes.POINTER(_D)]),
+    "pdx_gnn_collect": (_ST, [_P, _I32, _I64]),
+    "pdx_gnn_pairs": (_ST, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
+    "pdx_gnn_train": (_ST, [_P, _I32, _I32, _D, ctypes.c_uint64, ctypes.POINTER(_D)]),
+    "pdx_gnn_n_params": (_I64, []),
+    "pdx_gnn_params": (_ST, [_P, _P, _P, ctypes.POINTER(_D)]),
+    "pdx_gnn_graph": (_ST, [_P, ctypes.POINTER(_I64), _P, _P, _P, _P, _P, _P]),
+    "pdx_gnn_apply_vec": (_ST, [_P, _P, _P]),
+    "pdx_gnn_loss_grad": (_ST, [_P, _I32, _P, _P, ctypes.POINTER(_D), _P]),
     "pdx_set_profiling": (_ST, [_P, _I32]),
     "pdx_kernel_time": (_ST, [_P, ctypes.c_char_p, ctypes.POINTER(_D), ctypes.POINTER(_I64)]),
     "pdx_info": (_ST, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
+    "pdx_layout": (_ST, [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I64), ctypes.POINTER(_I64),
+                         ctypes.POINTER(_I32)]),
 }
 
 _lib = None
@@ -295,13 +303,65 @@ class Context:
         check(self.h, status, "pdx_nk_step")
         return st
 
-    def gnn_collect(self, on=True):
-        self.call("pdx_gnn_collect", 1 if on else 0)
+    # ---- learned preconditioner (internal point order where noted)
+    def gnn_collect(self, on=True, max_krylov_pairs=2048):
+        self.call("pdx_gnn_collect", 1 if on else 0, int(max_krylov_pairs))
 
-    def gnn_train(self, epochs, lr, seed=0):
+    def gnn_pairs(self):
+        a, b = _I64(), _I64()
+        self.call("pdx_gnn_pairs", ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def gnn_train(self, steps, batch=4, lr=1e-3, seed=0):
         loss = _D()
-        self.call("pdx_gnn_train", int(epochs), float(lr), ctypes.c_uint64(seed), ctypes.byref(loss))
+        self.call("pdx_gnn_train", int(steps), int(batch), float(lr), ctypes.c_uint64(seed),
+                  ctypes.byref(loss))
         return loss.value
+
+    def gnn_n_params(self):
+        return int(self.lib.pdx_gnn_n_params())
+
+    def gnn_get_params(self):
+        th = np.zeros(self.gnn_n_params(), np.float32)
+        c = _D()
+        self.call("pdx_gnn_params", ptr(th), None, ctypes.byref(c))
+        return th, c.value
+
+    def gnn_set_params(self, theta, out_scale):
+        th = np.ascontiguousarray(theta, np.float32)
+        if th.shape != (self.gnn_n_params(),):
+            raise ValueError("parameter vector has the wrong length")
+        c = _D(out_scale)
+        self.call("pdx_gnn_params", None, ptr(th), ctypes.byref(c))
+
+    def gnn_graph(self):
+        """(perm, near_ptr, near, stat, dinv, cmask), internal order."""
+        nn = _I64()
+        self.call("pdx_gnn_graph", ctypes.byref(nn), None, None, None, None, None, None)
+        n = self.n
+        perm = np.zeros(n, np.int64)
+        near_ptr = np.zeros(n + 1, np.int64)
+        near = np.zeros(max(nn.value, 1), np.int32)
+        stat = np.zeros((n, 5), np.float32)
+        dinv = np.zeros((n, 9), np.float64)
+        cmask = np.zeros((n, 3), np.uint8)
+        self.call("pdx_gnn_graph", ctypes.byref(nn), ptr(perm), ptr(near_ptr), ptr(near),
+                  ptr(stat), ptr(dinv), ptr(cmask))
+        return perm, near_ptr, near[:nn.value], stat, dinv, cmask.astype(bool)
+
+    def gnn_apply(self, y):
+        y = np.ascontiguousarray(y, np.float64).reshape(self.n, 3)
+        z = np.zeros_like(y)
+        self.call("pdx_gnn_apply_vec", ptr(y), ptr(z))
+        return z
+
+    def gnn_loss_grad(self, Y, X):
+        Y = np.ascontiguousarray(Y, np.float64).reshape(-1, self.n, 3)
+        X = np.ascontiguousarray(X, np.float64).reshape(Y.shape)
+        loss = _D()
+        g = np.zeros(self.gnn_n_params(), np.float32)
+        self.call("pdx_gnn_loss_grad", Y.shape[0], ptr(Y), ptr(X), ctypes.byref(loss), ptr(g))
+        return loss.value, g
 
     def set_profiling(self, on=True):
         self.call("pdx_set_profiling", 1 if on else 0)
@@ -319,3 +379,11 @@ class Context:
         a, b, c = _I64(), _I64(), _I64()
         self.call("pdx_info", ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
         return a.value, b.value, c.value
+
+    def layout(self):
+        """{"lds_mode", "bond_slots", "halo_entries", "halo_max"} of the device bond lists."""
+        m, s, h = _I32(), _I64(), _I64()
+        hm = _I32()
+        self.call("pdx_layout", ctypes.byref(m), ctypes.byref(s), ctypes.byref(h), ctypes.byref(hm))
+        return {"lds_mode": bool(m.value), "bond_slots": s.value, "halo_entries": h.value,
+                "halo_max": hm.value}

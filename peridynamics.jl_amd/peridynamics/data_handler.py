@@ -150,9 +150,10 @@ class MI355XBodyDataHandler:
     # ------------------------------------------------------------------ Newton-Krylov
     PRECONDS = {"none": 0, "jacobi": 1, "learned": 2}
 
-    def nk_options(self, nk):
+    def nk_options(self, nk, precond=None):
+        pre = self.PRECONDS[precond or nk.precond]
         return PdxNkOpts(maxiter=nk.maxiter, gmres_maxiter=nk.gmres_maxiter,
-                         gmres_restart=nk.gmres_restart, precond=self.PRECONDS[nk.precond],
+                         gmres_restart=nk.gmres_restart, precond=pre,
                          tol=nk.tol, rel_tol=nk.rel_tol, gmres_reltol=nk.gmres_reltol,
                          gmres_abstol=nk.gmres_abstol,
                          perturbation_scale=nk.perturbation_scale)
@@ -184,7 +185,7 @@ class MI355XBodyDataHandler:
         self._bext_pos = [(body.point_sets[s], d, f) for fld, s, d, f in body.pos_single_dim_bcs
                           if fld == "b_ext"]
 
-    def nk_load_step(self, nk, n):
+    def nk_load_step(self, nk, n, precond=None):
         """newton_krylov_step!(dh, nk, options, n): newton_krylov.jl:298-332."""
         beta = n / nk.n_steps
         if self._bext_pos:
@@ -195,17 +196,39 @@ class MI355XBodyDataHandler:
                 ok = np.isfinite(val)
                 bext[P[ok], d] = val[ok]
             self.set("b_ext", bext)
-        return self.ctx.nk_step(n, nk.n_steps, self.nk_options(nk))
+        return self.ctx.nk_step(n, nk.n_steps, self.nk_options(nk, precond))
 
     def solve_nk(self, nk, on_export=None):
         """solve!(dh, nk, options): newton_krylov.jl:280-295.  Returns the per-step stats."""
         self.init_newton_krylov(nk)
         stats = []
+        self.train_info = None
+        learned = nk.precond == "learned"
         for n in range(1, nk.n_steps + 1):
-            stats.append(self.nk_load_step(nk, n))
+            if learned and n <= nk.learn_steps:
+                # plain (Jacobi) solves that supply the training pairs
+                self.ctx.gnn_collect(True, nk.krylov_pairs)
+                stats.append(self.nk_load_step(nk, n, precond="jacobi"))
+                self.ctx.gnn_collect(False)
+                if n == min(nk.learn_steps, nk.n_steps):
+                    self.train_learned(nk)
+            else:
+                if learned and nk.learn_steps <= 0 and self.train_info is None:
+                    raise _arg_error("the learned preconditioner needs learn_steps >= 1")
+                stats.append(self.nk_load_step(nk, n))
             if on_export is not None:
                 on_export(self, n, n * nk.dt)
         return stats
+
+    def train_learned(self, nk):
+        """Trains the message-passing preconditioner on the collected pairs."""
+        import time
+        t0 = time.perf_counter()
+        pairs = self.ctx.gnn_pairs()
+        loss = self.ctx.gnn_train(nk.train_steps, nk.train_batch, nk.train_lr, nk.train_seed)
+        self.ctx.synchronize()
+        self.train_info = {"pairs": pairs, "loss": loss, "seconds": time.perf_counter() - t0}
+        return self.train_info
 
     # ------------------------------------------------------------------ fields
     def get(self, name):

@@ -47,11 +47,17 @@ class NewtonKrylov:
 
     Additions of the MI355X build (defaults keep the reference behaviour):
     rel_tol > 0 stops a load step at ||r|| < rel_tol * ||r_first|| (the reference's `tol`
-    is absolute, newton_krylov.jl:311-314); precond = "none" | "jacobi" | "learned"."""
+    is absolute, newton_krylov.jl:311-314); precond = "none" | "jacobi" | "learned".
+    With "learned", load steps 1 .. learn_steps are solved with the Jacobi preconditioner
+    while their Newton and Krylov pairs are collected (at most krylov_pairs Krylov pairs),
+    the network is then trained (train_steps Adam steps of train_batch pairs, step size
+    train_lr) and the remaining load steps use it."""
 
     def __init__(self, time=-1, steps=-1, stepsize=1.0, maxiter=100, tol=1e-4,
                  perturbation_scale=1.0, gmres_maxiter=200, gmres_reltol=1e-4,
-                 gmres_abstol=1e-8, gmres_restart=100, rel_tol=0.0, precond="none"):
+                 gmres_abstol=1e-8, gmres_restart=100, rel_tol=0.0, precond="none",
+                 learn_steps=3, train_steps=400, train_batch=4, train_lr=1e-3,
+                 krylov_pairs=2048, train_seed=0):
         if time > 0 and steps > 0:
             raise _arg_error("specify either time or number of steps, not both!")
         if time < 0 and steps < 0:
@@ -84,6 +90,12 @@ class NewtonKrylov:
         self.gmres_restart = int(gmres_restart)
         self.rel_tol = float(rel_tol)
         self.precond = precond
+        self.learn_steps = int(learn_steps)
+        self.train_steps = int(train_steps)
+        self.train_batch = int(train_batch)
+        self.train_lr = float(train_lr)
+        self.krylov_pairs = int(krylov_pairs)
+        self.train_seed = int(train_seed)
 
     def __repr__(self):
         return f"NewtonKrylov(n_steps={self.n_steps}, stepsize={self.dt})"

@@ -39,23 +39,26 @@ def make_body(nx, ny, nz, dx=1e-3, jitter=0.05, seed=0x5EED):
 
 def step_bytes(n_points, n_bonds, precision, layout):
     """Algorithmic HBM bytes of one fused bond-based step: every array the step needs is
-    read or written once (neighbour positions are re-used from cache, so a neighbour's x|V
-    counts once, as its own point's).
+    read or written once (neighbour data is re-used from LDS/cache, so a neighbour's
+    record counts once, as its own point's).
 
     per point: read x|V (32), u (24), v_half (24), bond count (4), BC slot (4); write x|V
                (32), u (24), v_half (24)  -> 168 B, plus 8 B per 64-point slice pointer;
-    per bond:  LDS-staged layout: 16-bit bond word (2) + FP64 reference length (8), in both
-               precisions (the mixed step forms l^2 - L^2 in FP64);
+               the mixed LDS step also reads the reference record X|V (32) -> 200 B;
+    per bond:  LDS-staged FP64: 16-bit bond word (2) + FP64 reference length (8);
+               LDS-staged mixed: the 16-bit bond word only (2; L = |X_j - X_i|);
                global-gather layout: 32-bit bond word (4) + length (8 FP64 / 4 FP32);
-    per neighbour-table entry (LDS layout): its 4-byte point index.
+    per neighbour-table entry (LDS layouts): its 4-byte point index.
     A bond word is written back only when the bond breaks (not counted)."""
     n_slices = (n_points + 63) // 64
-    per_point = 168 * n_points + 8 * n_slices
     if layout["lds_mode"]:
-        return (10 * n_bonds + per_point + 4 * layout["halo_entries"],
-                "10*N_b + 168*N_p + 8*N_slices + 4*N_halo")
+        if precision == "fp64":
+            return (10 * n_bonds + 168 * n_points + 8 * n_slices + 4 * layout["halo_entries"],
+                    "10*N_b + 168*N_p + 8*N_slices + 4*N_halo")
+        return (2 * n_bonds + 200 * n_points + 8 * n_slices + 4 * layout["halo_entries"],
+                "2*N_b + 200*N_p + 8*N_slices + 4*N_halo")
     per_bond = 12 if precision == "fp64" else 8
-    return (per_bond * n_bonds + per_point,
+    return (per_bond * n_bonds + 168 * n_points + 8 * n_slices,
             f"{per_bond}*N_b + 168*N_p + 8*N_slices")
 
 
@@ -154,6 +157,8 @@ def main():
     log(f"[bench] step kernel {kms:.4f} ms avg over {kcount} launches, {achieved:.1f} GB/s")
     u = dh.get("displacement")
     assert np.all(np.isfinite(u))
+    nact = dh.get("n_active_bonds")
+    total_steps = args.warmup + args.steps + args.profile_steps
 
     out = {
         "metric": METRIC,
@@ -182,26 +187,42 @@ def main():
     dh.close()
 
     if not args.no_mixed and args.precision == "fp64":
+        # the same run in mixed precision: throughput, step-kernel time, and its final state
+        # against the FP64 run (displacement error, difference of the broken-bond counts)
         log("[bench] mixed-precision leg")
         dhm = pd.MI355XBodyDataHandler(body, precision="mixed")
-        vvm = pd.VelocityVerlet(steps=args.warmup + args.steps)
+        vvm = pd.VelocityVerlet(steps=total_steps)
         dhm.init_time_solver(vvm)
         dhm.initialize(vvm)
         cm = dhm.ctx
-        bcm = dhm.bc_values(range(1, vvm.n_steps + 1), vvm.dt)
-        cm.vv_run(1, args.warmup, vvm.dt, bcm[:args.warmup])
+        assert vvm.dt == dt
+        bcm = dhm.bc_values(range(1, vvm.n_steps + 1), dt)
+        cm.vv_run(1, args.warmup, dt, bcm[:args.warmup])
         cm.synchronize()
         t1 = time.perf_counter()
-        cm.set_profiling(True)
-        cm.vv_run(1 + args.warmup, args.steps, vvm.dt, bcm[args.warmup:])
+        cm.vv_run(1 + args.warmup, args.steps, dt, bcm[args.warmup:args.warmup + args.steps])
         cm.synchronize()
         em = time.perf_counter() - t1
+        cm.set_profiling(True)
+        s0 = 1 + args.warmup + args.steps
+        cm.vv_run(s0, args.profile_steps, dt, bcm[s0 - 1:])
         kmm, _ = cm.kernel_time("step")
+        cm.set_profiling(False)
+        um = dhm.get("displacement")
+        nm = dhm.get("n_active_bonds")
+        mbytes, mformula = step_bytes(n_points, n_bonds, "mixed", cm.layout())
         out["mixed"] = {"value": n_bonds * args.steps / em, "ms_per_step": em / args.steps * 1e3,
-                        "kernel_ms": kmm,
+                        "kernel_ms": kmm, "speedup_vs_fp64_step": ms_per_step / (em / args.steps * 1e3),
                         "speedup_vs_fp64_kernel": kms / kmm if kmm else None,
-                        "achieved_GBs": step_bytes(n_points, n_bonds, "mixed", cm.layout())[0]
-                        / (kmm * 1e-3) / 1e9}
+                        "achieved_GBs": mbytes / (kmm * 1e-3) / 1e9,
+                        "frac_hbm": mbytes / (kmm * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                        "bytes_formula": mformula,
+                        "final_displacement_rel_error_vs_fp64": float(
+                            np.linalg.norm(um - u) / np.linalg.norm(u)),
+                        "broken_bond_count_difference_vs_fp64": int(np.sum(np.abs(nm - nact))),
+                        "broken_bonds_fp64": int(n_bonds - np.sum(nact)),
+                        "broken_bonds_mixed": int(n_bonds - np.sum(nm)),
+                        "steps_compared": total_steps}
         dhm.close()
 
     if not args.no_cpu:

@@ -88,7 +88,9 @@ typedef struct pdx_nk_opts {
     int32_t maxiter;          /* Newton iterations */
     int32_t gmres_maxiter;    /* total Krylov iterations per linear solve */
     int32_t gmres_restart;    /* restart length */
-    int32_t precond;          /* 0 none, 1 point-block Jacobi, 2 learned (flexible) */
+    int32_t precond;          /* 0 none (the reference), 1 point-block Jacobi, 2 learned
+                                 (two-level + message-passing network, flexible), 3 two-level
+                                 (Jacobi + Galerkin coarse correction on point aggregates) */
     double tol;               /* absolute tolerance on ||r|| (reference semantics) */
     double rel_tol;           /* > 0: stop at ||r|| < rel_tol * ||r_0|| of the load step */
     double gmres_reltol, gmres_abstol;
@@ -99,7 +101,8 @@ typedef struct pdx_nk_stats {
     int32_t newton_iters;
     int32_t krylov_iters;
     int32_t jv_evals;
-    int32_t precond_fallbacks; /* learned preconditioner rejected, Jacobi used instead */
+    int32_t precond_fallbacks; /* learned solves that stalled (linear residual not even halved)
+                                  and were redone with point-block Jacobi */
     double residual_first, residual_final;
     double seconds;
 } pdx_nk_stats;

@@ -951,9 +951,13 @@ pdx_status pdx_gnn_prepare(pdx_ctx *c)
     return PDX_OK;
 }
 
+// the learned preconditioner of the Krylov solver: two-level coarse correction plus the
+// network's block-Jacobi-based output, z = P A_c^-1 P^T v + D^-1 v + c s O(v)
 pdx_status pdx_gnn_apply(pdx_ctx *c, const double *v, double *z)
 {
-    return gnn_forward(c, v, nullptr, z, false, 0.0);
+    PDX_TRY(gnn_forward(c, v, nullptr, z, false, 0.0));
+    if (c->nk->coarse_ready) PDX_TRY(pdx_nk_coarse_add(c, v, z, 1.0));
+    return PDX_OK;
 }
 
 static pdx_status gnn_ensure_pool(pdx_ctx *c)
@@ -1074,7 +1078,8 @@ static pdx_status gnn_draw(pdx_ctx *c, std::mt19937_64 &rng)
     return PDX_OK;
 }
 
-// output scale c = median over samples of rms_free(x - D^-1 y) / rms_free(D^-1 y)
+// output scale c = median over samples of rms_free(x' - D^-1 y) / rms_free(D^-1 y), with
+// x' = x - P A_c^-1 P^T y when the coarse space is built
 static pdx_status gnn_estimate_c(pdx_ctx *c, std::mt19937_64 &rng)
 {
     pdx_gnn *g = c->gnn;
@@ -1084,6 +1089,7 @@ static pdx_status gnn_estimate_c(pdx_ctx *c, std::mt19937_64 &rng)
     const int ns = (int)std::min<int64_t>(32, g->n_n + g->n_k);
     for (int k = 0; k < ns; k++) {
         PDX_TRY(gnn_draw(c, rng));
+        if (c->nk->coarse_ready) PDX_TRY(pdx_nk_coarse_add(c, g->ys, g->xs, -1.0));
         hipLaunchKernelGGL(k_gnn_q, dim3(nb), dim3(256), 0, c->stream, n, c->nk->dinv, g->ys,
                            c->nk->cmask, g->q, g->dpart);
         hipLaunchKernelGGL(k_gnn_q_final, dim3(1), dim3(256), 0, c->stream, nb, g->dpart, g->scal);
@@ -1118,6 +1124,8 @@ extern "C" pdx_status pdx_gnn_train(pdx_ctx *c, int32_t steps, int32_t batch, do
     pdx_gnn *g = c->gnn;
     if (g->n_n + g->n_k == 0) return pdx_fail(c, PDX_ERR_STATE, "no training pairs (pdx_gnn_collect)");
     PDX_TRY(gnn_build_dinv(c));
+    // the network learns what the two-level preconditioner leaves: x - P A_c^-1 P^T y
+    PDX_TRY(pdx_nk_build_coarse(c));
     std::mt19937_64 rng(seed);
     if (!g->trained) PDX_TRY(gnn_init_params(c, g, seed));
     PDX_TRY(gnn_estimate_c(c, rng));
@@ -1128,6 +1136,7 @@ extern "C" pdx_status pdx_gnn_train(pdx_ctx *c, int32_t steps, int32_t batch, do
         PDX_HIP(c, hipMemsetAsync(g->scal + SC_LOSS, 0, sizeof(double), c->stream));
         for (int b = 0; b < batch; b++) {
             PDX_TRY(gnn_draw(c, rng));
+            PDX_TRY(pdx_nk_coarse_add(c, g->ys, g->xs, -1.0));
             PDX_TRY(gnn_forward(c, g->ys, g->xs, nullptr, true, 1.0 / batch));
             PDX_TRY(gnn_backward(c, g->xs));
         }

@@ -47,7 +47,7 @@ struct pdx_ctx {
     int device = 0;
     int precision = PDX_FP64;
     // bonds per iteration of the BB thread (tuning knob PDX_BB_UNROLL = 2 | 4)
-    int bb_unroll = 2;
+    int bb_unroll = 2;  // bonds per lane and unrolled iteration of the BB step (2 or 4)
     hipStream_t stream = nullptr;
     std::string err;
 

@@ -18,6 +18,13 @@ struct pdx_nk {
     double *red = nullptr;      // reduction partials
     double *h_dev = nullptr;    // small coefficient vectors
     double *dinv = nullptr;     // 9n point-block Jacobi inverse
+    double *dsc = nullptr;      // device scalars (||u||^2, ||v||^2)
+    // two-level preconditioner: aggregates of `agg` consecutive points, mc = 3 nagg
+    int agg = 0;
+    int64_t nagg = 0, mc = 0;
+    double *Ac = nullptr;       // mc x mc, the inverse of the Galerkin operator once built
+    double *rc = nullptr, *zc = nullptr;
+    bool coarse_ready = false;
     int64_t n_jv = 0;
 };
 
@@ -31,3 +38,11 @@ pdx_status pdx_gnn_record(pdx_ctx *c, const double *rhs, const double *du);  // 
 pdx_status pdx_gnn_record_krylov(pdx_ctx *c, const double *z, const double *jz);  // Krylov pair
 bool pdx_gnn_collecting(pdx_ctx *c);
 pdx_status pdx_nk_build_jacobi(pdx_ctx *c);  // point-block Jacobi at the current state
+pdx_status pdx_nk_build_coarse(pdx_ctx *c);  // two-level coarse operator at the current state
+pdx_status pdx_nk_coarse_add(pdx_ctx *c, const double *v, double *z, double alpha);  // z += a P Ac^-1 P^T v
+
+// preconditioners of pdx_nk_opts.precond
+#define PDX_PRE_NONE 0
+#define PDX_PRE_JACOBI 1
+#define PDX_PRE_LEARNED 2
+#define PDX_PRE_TWO_LEVEL 3

@@ -24,6 +24,10 @@
 
 #define RED_BLOCKS 512
 #define PDX_MAX_RESTART 128
+#define NK_MAX_AGGS 256  // coarse aggregates (3 coarse dofs each)
+// device scalars of the Krylov solver
+#define NK_UN2 0  // ||u||^2
+#define NK_VN2 1  // ||v||^2 of the current J v product
 
 void pdx_nk_free(pdx_ctx *c)
 {
@@ -32,7 +36,8 @@ void pdx_nk_free(pdx_ctx *c)
     for (void *p : {(void *)k->cdof, (void *)k->cval, (void *)k->cmask, (void *)k->r,
                     (void *)k->du, (void *)k->w, (void *)k->t0, (void *)k->Vb, (void *)k->Zb,
                     (void *)k->recp, (void *)k->recm, (void *)k->fp, (void *)k->fm,
-                    (void *)k->red, (void *)k->dinv, (void *)k->h_dev})
+                    (void *)k->red, (void *)k->dinv, (void *)k->h_dev, (void *)k->dsc,
+                    (void *)k->Ac, (void *)k->rc, (void *)k->zc})
         if (p) (void)hipFree(p);
     delete k;
     c->nk = nullptr;
@@ -142,13 +147,24 @@ __global__ void k_residual(int64_t n, const double *bint, const double *bext,
     r[t] = cmask[t] ? 0.0 : (bint[t] + bext[t]) * Xr[i].w;
 }
 
+// perturbation of jacobian_vector_product! (compute_perturbation, newton_krylov.jl:537-567):
+// s = scale cbrt(eps) (||u|| + 1) / ||v||, ||v|| < eps -> 1; the squared norms are device
+// scalars so that no host round trip sits between the Krylov kernels
+__device__ __forceinline__ double nk_perturbation(const double *un2, const double *vn2, double sc)
+{
+    double vn = sqrt(*vn2);
+    if (vn < 2.220446049250313e-16) vn = 1.0;
+    return sc * (sqrt(*un2) + 1.0) / vn;
+}
+
 // positions X + (u + s v) and X + (u - s v) with the volume in .w
 // (store_state_and_apply_perturbation!, store_pos_force_and_apply_neg_perturbation!)
 __global__ void k_perturb(int64_t n, const double4 *Xr, const double *u, const double *v,
-                          double s, double4 *rp, double4 *rm)
+                          const double *un2, const double *vn2, double sc, double4 *rp, double4 *rm)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const double s = nk_perturbation(un2, vn2, sc);
     const double4 X = Xr[i];
     double up[3], um[3];
     for (int d = 0; d < 3; d++) {
@@ -160,12 +176,23 @@ __global__ void k_perturb(int64_t n, const double4 *Xr, const double *u, const d
 }
 
 // compute_jvp_and_restore!: Jv = (F+ V - F- V) / (2 s), constrained rows Jv = v
-__global__ void k_jv_combine(int64_t m, const double *fp, const double *fm, double s,
-                             const uint8_t *cmask, const double *v, double *jv)
+__global__ void k_jv_combine(int64_t m, const double *fp, const double *fm, const double *un2,
+                             const double *vn2, double sc, const uint8_t *cmask, const double *v,
+                             double *jv)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= m) return;
+    const double s = nk_perturbation(un2, vn2, sc);
     jv[t] = cmask[t] ? v[t] : (fp[t] - fm[t]) / (2.0 * s);
+}
+
+// x *= 1 / sqrt(*n2) (no-op for a zero norm)
+__global__ void k_scale_inv_sqrt(int64_t m, const double *n2, const double *x, double *y)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    const double nn = *n2;
+    y[t] = nn > 0.0 ? x[t] / sqrt(nn) : x[t];
 }
 
 // update_displacement_from_du!: free dofs u += du, x += du
@@ -278,7 +305,9 @@ static pdx_status nk_ensure(pdx_ctx *c, int restart)
         PDX_TRY(pdx_alloc(c, &k->recp, n));
         PDX_TRY(pdx_alloc(c, &k->recm, n));
         PDX_TRY(pdx_alloc(c, &k->red, (size_t)RED_BLOCKS * (PDX_MAX_RESTART + 2)));
-        PDX_TRY(pdx_alloc(c, &k->h_dev, PDX_MAX_RESTART + 2));
+        PDX_TRY(pdx_alloc(c, &k->h_dev, 2 * (PDX_MAX_RESTART + 1) + 8));
+        PDX_TRY(pdx_alloc(c, &k->dsc, 8));
+        PDX_HIP(c, hipMemset(k->dsc, 0, sizeof(double) * 8));
         PDX_TRY(pdx_alloc(c, &k->dinv, 9 * n));
     }
     if (restart > k->m_cap) {
@@ -292,8 +321,8 @@ static pdx_status nk_ensure(pdx_ctx *c, int restart)
     return PDX_OK;
 }
 
-// dots[k] = V_k . w for k < nv (V with leading dimension m), result on the host
-static pdx_status multidot(pdx_ctx *c, int nv, const double *V, const double *w, double *dots)
+// out_dev[k] = V_k . w for k < nv (V with leading dimension m), result in device memory
+static pdx_status multidot_dev(pdx_ctx *c, int nv, const double *V, const double *w, double *out_dev)
 {
     pdx_nk *k = c->nk;
     if (nv > PDX_MAX_RESTART + 1) return pdx_fail(c, PDX_ERR_UNSUPPORTED, "restart too long");
@@ -301,8 +330,16 @@ static pdx_status multidot(pdx_ctx *c, int nv, const double *V, const double *w,
                        k->red);
     PDX_HIP(c, hipGetLastError());
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, c->stream, RED_BLOCKS, nv, k->red,
-                       k->h_dev);
+                       out_dev);
     PDX_HIP(c, hipGetLastError());
+    return PDX_OK;
+}
+
+// dots on the host (one synchronisation)
+static pdx_status multidot(pdx_ctx *c, int nv, const double *V, const double *w, double *dots)
+{
+    pdx_nk *k = c->nk;
+    PDX_TRY(multidot_dev(c, nv, V, w, k->h_dev));
     PDX_HIP(c, hipMemcpyAsync(dots, k->h_dev, sizeof(double) * nv, hipMemcpyDeviceToHost, c->stream));
     PDX_HIP(c, hipStreamSynchronize(c->stream));
     return PDX_OK;
@@ -327,24 +364,241 @@ static pdx_status force_scaled(pdx_ctx *c, const double4 *rec_in, double *out)
     return pdx_launch_step(c, a);
 }
 
-// Jv for the current displacement (jacobian_vector_product!)
+// Jv for the current displacement (jacobian_vector_product!).  ||u||^2 is the device
+// scalar NK_UN2, kept current by nk_refresh_unorm(); ||v||^2 is formed here on the device.
+static pdx_status nk_refresh_unorm(pdx_ctx *c)
+{
+    return multidot_dev(c, 1, c->u, c->u, c->nk->dsc + NK_UN2);
+}
+
 pdx_status pdx_nk_jv_dev(pdx_ctx *c, const double *v, double *jv, double scale)
 {
     pdx_nk *k = c->nk;
-    double un, vn;
-    PDX_TRY(norm2(c, c->u, &un));
-    PDX_TRY(norm2(c, v, &vn));
-    if (vn < 2.220446049250313e-16) vn = 1.0;
-    const double s = scale * std::cbrt(2.220446049250313e-16) * (un + 1.0) / vn;
-    hipLaunchKernelGGL(k_perturb, grid1(c->n), dim3(256), 0, c->stream, c->n, c->Xr, c->u, v, s,
-                       k->recp, k->recm);
+    const double sc = scale * std::cbrt(2.220446049250313e-16);
+    PDX_TRY(multidot_dev(c, 1, v, v, k->dsc + NK_VN2));
+    hipLaunchKernelGGL(k_perturb, grid1(c->n), dim3(256), 0, c->stream, c->n, c->Xr, c->u, v,
+                       k->dsc + NK_UN2, k->dsc + NK_VN2, sc, k->recp, k->recm);
     PDX_HIP(c, hipGetLastError());
     PDX_TRY(force_scaled(c, k->recp, k->fp));
     PDX_TRY(force_scaled(c, k->recm, k->fm));
-    hipLaunchKernelGGL(k_jv_combine, grid1(k->m), dim3(256), 0, c->stream, k->m, k->fp, k->fm, s,
-                       k->cmask, v, jv);
+    hipLaunchKernelGGL(k_jv_combine, grid1(k->m), dim3(256), 0, c->stream, k->m, k->fp, k->fm,
+                       k->dsc + NK_UN2, k->dsc + NK_VN2, sc, k->cmask, v, jv);
     PDX_HIP(c, hipGetLastError());
     k->n_jv++;
+    return PDX_OK;
+}
+
+// ----------------------------------------------------------------------------------------
+// Two-level preconditioner: aggregates are runs of NK_AGG consecutive Morton points
+// (compact boxes); the coarse space holds the three translations of every aggregate on its
+// free dofs (piecewise-constant prolongation P).  The Galerkin operator A_c = P^T J P is
+// assembled from the analytic linearisation of the bond-based residual (the same bond
+// stiffness as the block Jacobi) and inverted once per load step on the device.
+// z = D^-1 v + P A_c^-1 P^T v.
+// ----------------------------------------------------------------------------------------
+// A_c += contributions of the bonds of point i: J_ij = V_i C_ij to block (I, agg(j)) and
+// J_ii = -V_i sum_j C_ij to block (I, I), each masked to free rows (of i) and columns.
+__global__ __launch_bounds__(256) void k_coarse_assemble(int64_t n, int agg, int64_t mc,
+                                                         const int64_t *slice_ptr, const int32_t *cnt,
+                                                         const uint32_t *bw, const double *L,
+                                                         const double4 *rec, double bc,
+                                                         const uint8_t *cmask, double *Ac)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t I = i / agg;
+    const int64_t b0 = slice_ptr[i >> 6] + (i & 63);
+    const double4 ri = rec[i];
+    const bool fi[3] = {cmask[3 * i] == 0, cmask[3 * i + 1] == 0, cmask[3 * i + 2] == 0};
+    double Dg[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // -sum_j V_i C_ij (all bonds)
+    double Acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // run of bonds into aggregate Jc
+    int64_t Jc = -1;
+    auto flush = [&]() {
+        if (Jc < 0) return;
+        for (int d = 0; d < 3; d++)
+            for (int e = 0; e < 3; e++)
+                if (Acc[3 * d + e] != 0.0) atomicAdd(&Ac[(3 * I + d) * mc + 3 * Jc + e], Acc[3 * d + e]);
+        for (int q = 0; q < 9; q++) Acc[q] = 0.0;
+    };
+    for (int t = 0; t < cnt[i]; t++) {
+        const int64_t b = b0 + (int64_t)t * 64;
+        const uint32_t wd = bw[b];
+        if (!(wd & PDX_BW_ACT)) continue;
+        const int64_t j = wd & PDX_BW_NBR;
+        const double4 rj = rec[j];
+        const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
+        const double l = sqrt((dx * dx + dy * dy) + dz * dz);
+        const double nv[3] = {dx / l, dy / l, dz / l};
+        const double Lb = L[b];
+        const double eps = (l - Lb) / Lb;
+        const double a = ri.w * bc * rj.w / Lb, g = ri.w * bc * rj.w * eps / l;
+        const int64_t J = j / agg;
+        if (J != Jc) {
+            flush();
+            Jc = J;
+        }
+        const bool fj[3] = {cmask[3 * j] == 0, cmask[3 * j + 1] == 0, cmask[3 * j + 2] == 0};
+        for (int d = 0; d < 3; d++)
+            for (int e = 0; e < 3; e++) {
+                const double C = (a - g) * nv[d] * nv[e] + (d == e ? g : 0.0);
+                if (fi[d] && fj[e]) Acc[3 * d + e] += C;
+                if (fi[d] && fi[e]) Dg[3 * d + e] -= C;
+            }
+    }
+    flush();
+    for (int d = 0; d < 3; d++)
+        for (int e = 0; e < 3; e++)
+            if (Dg[3 * d + e] != 0.0) atomicAdd(&Ac[(3 * I + d) * mc + 3 * I + e], Dg[3 * d + e]);
+}
+
+// a coarse dof with no free fine dof (all constrained) gets an identity row
+__global__ void k_coarse_fix(int64_t mc, double *Ac)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= mc) return;
+    if (Ac[r * mc + r] == 0.0) Ac[r * mc + r] = 1.0;
+}
+
+// in-place Gauss-Jordan inversion, pivot k: trailing update, then the pivot row/column.
+// A_c is negative definite (J = -stiffness), so no pivoting is needed.
+__global__ void k_gj_update(int64_t mc, int64_t k, double *A)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= mc * mc) return;
+    const int64_t i = e / mc, j = e - i * mc;
+    if (i == k || j == k) return;
+    A[e] -= A[i * mc + k] * A[k * mc + j] / A[k * mc + k];
+}
+
+__global__ void k_gj_pivot(int64_t mc, int64_t k, double *A)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= mc) return;
+    const double p = 1.0 / A[k * mc + k];
+    if (t != k) {
+        A[t * mc + k] = -A[t * mc + k] * p;
+        A[k * mc + t] = A[k * mc + t] * p;
+    }
+}
+
+__global__ void k_gj_diag(int64_t mc, int64_t k, double *A) { A[k * mc + k] = 1.0 / A[k * mc + k]; }
+
+// rc[3I+d] = sum over free dofs d of the points of aggregate I of v; one workgroup per I
+__global__ __launch_bounds__(256) void k_coarse_restrict(int64_t n, int agg, const double *v,
+                                                         const uint8_t *cmask, double *rc)
+{
+    __shared__ double sh[3][256];
+    const int64_t I = blockIdx.x;
+    const int64_t p0 = I * agg, p1 = min(n, p0 + agg);
+    double s[3] = {0, 0, 0};
+    for (int64_t p = p0 + threadIdx.x; p < p1; p += 256)
+        for (int d = 0; d < 3; d++)
+            if (!cmask[3 * p + d]) s[d] += v[3 * p + d];
+    for (int d = 0; d < 3; d++) sh[d][threadIdx.x] = s[d];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int d = 0; d < 3; d++) sh[d][threadIdx.x] += sh[d][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) rc[3 * I + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// zc = Ainv rc (Ainv symmetric: thread r reads column r, coalesced over the rows)
+__global__ void k_coarse_matvec(int64_t mc, const double *Ainv, const double *rc, double *zc)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= mc) return;
+    double s = 0.0;
+    for (int64_t k = 0; k < mc; k++) s += Ainv[k * mc + r] * rc[k];
+    zc[r] = s;
+}
+
+// z = D^-1 v (+ P zc on free dofs when zc is given)
+__global__ void k_apply_two_level(int64_t n, int agg, const double *dinv, const double *v,
+                                  const double *zc, const uint8_t *cmask, double *z)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double *R = dinv + 9 * i;
+    const double a = v[3 * i], b = v[3 * i + 1], cc = v[3 * i + 2];
+    double o[3] = {R[0] * a + R[3] * b + R[6] * cc, R[1] * a + R[4] * b + R[7] * cc,
+                   R[2] * a + R[5] * b + R[8] * cc};
+    if (zc) {
+        const int64_t I = i / agg;
+        for (int d = 0; d < 3; d++)
+            if (!cmask[3 * i + d]) o[d] += zc[3 * I + d];
+    }
+    for (int d = 0; d < 3; d++) z[3 * i + d] = o[d];
+}
+
+// coarse operator and its inverse at the current state
+static pdx_status build_coarse(pdx_ctx *c)
+{
+    pdx_nk *k = c->nk;
+    if (c->kind != PDX_BB) return pdx_fail(c, PDX_ERR_UNSUPPORTED, "the two-level preconditioner is built for BBMaterial");
+    if (!k->Ac) {
+        // aggregates of a power-of-two size >= 64, at most NK_MAX_AGGS of them
+        int64_t agg = 64;
+        while ((c->n + agg - 1) / agg > NK_MAX_AGGS) agg *= 2;
+        k->agg = (int)agg;
+        k->nagg = (c->n + agg - 1) / agg;
+        k->mc = 3 * k->nagg;
+        PDX_TRY(pdx_alloc(c, &k->Ac, (size_t)(k->mc * k->mc)));
+        PDX_TRY(pdx_alloc(c, &k->rc, (size_t)k->mc));
+        PDX_TRY(pdx_alloc(c, &k->zc, (size_t)k->mc));
+    }
+    const int64_t mc = k->mc, mm = mc * mc;
+    PDX_HIP(c, hipMemsetAsync(k->Ac, 0, sizeof(double) * mm, c->stream));
+    hipLaunchKernelGGL(k_coarse_assemble, grid1(c->n), dim3(256), 0, c->stream, c->n, k->agg, mc,
+                       c->slice_ptr, c->cnt, c->bw, c->L, c->rec[c->cur], c->P.bc, k->cmask, k->Ac);
+    hipLaunchKernelGGL(k_coarse_fix, grid1(mc), dim3(256), 0, c->stream, mc, k->Ac);
+    PDX_HIP(c, hipGetLastError());
+    for (int64_t q = 0; q < mc; q++) {
+        hipLaunchKernelGGL(k_gj_update, grid1(mm), dim3(256), 0, c->stream, mc, q, k->Ac);
+        hipLaunchKernelGGL(k_gj_pivot, grid1(mc), dim3(256), 0, c->stream, mc, q, k->Ac);
+        hipLaunchKernelGGL(k_gj_diag, dim3(1), dim3(1), 0, c->stream, mc, q, k->Ac);
+    }
+    PDX_HIP(c, hipGetLastError());
+    k->coarse_ready = true;
+    return PDX_OK;
+}
+
+// z = D^-1 v + P A_c^-1 P^T v
+pdx_status pdx_nk_two_level(pdx_ctx *c, const double *v, double *z)
+{
+    pdx_nk *k = c->nk;
+    if (!k->coarse_ready) return pdx_fail(c, PDX_ERR_STATE, "coarse space not built");
+    hipLaunchKernelGGL(k_coarse_restrict, dim3((unsigned)k->nagg), dim3(256), 0, c->stream, c->n, k->agg,
+                       v, k->cmask, k->rc);
+    hipLaunchKernelGGL(k_coarse_matvec, grid1(k->mc), dim3(256), 0, c->stream, k->mc, k->Ac, k->rc, k->zc);
+    hipLaunchKernelGGL(k_apply_two_level, grid1(c->n), dim3(256), 0, c->stream, c->n, k->agg, k->dinv, v,
+                       k->zc, k->cmask, z);
+    PDX_HIP(c, hipGetLastError());
+    return PDX_OK;
+}
+
+// z += alpha P A_c^-1 P^T v on free dofs
+__global__ void k_coarse_prolong_add(int64_t n, int agg, const double *zc, const uint8_t *cmask,
+                                     double alpha, double *z)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * n) return;
+    if (cmask[t]) return;
+    const int64_t i = t / 3;
+    z[t] += alpha * zc[3 * (i / agg) + (t - 3 * i)];
+}
+
+pdx_status pdx_nk_coarse_add(pdx_ctx *c, const double *v, double *z, double alpha)
+{
+    pdx_nk *k = c->nk;
+    if (!k->coarse_ready) return pdx_fail(c, PDX_ERR_STATE, "coarse space not built");
+    hipLaunchKernelGGL(k_coarse_restrict, dim3((unsigned)k->nagg), dim3(256), 0, c->stream, c->n, k->agg,
+                       v, k->cmask, k->rc);
+    hipLaunchKernelGGL(k_coarse_matvec, grid1(k->mc), dim3(256), 0, c->stream, k->mc, k->Ac, k->rc, k->zc);
+    hipLaunchKernelGGL(k_coarse_prolong_add, grid1(k->m), dim3(256), 0, c->stream, c->n, k->agg, k->zc,
+                       k->cmask, alpha, z);
+    PDX_HIP(c, hipGetLastError());
     return PDX_OK;
 }
 
@@ -352,14 +606,17 @@ pdx_status pdx_nk_jv_dev(pdx_ctx *c, const double *v, double *jv, double scale)
 static pdx_status precond_apply(pdx_ctx *c, int which, const double *v, double *z)
 {
     pdx_nk *k = c->nk;
-    if (which == 1) {
+    switch (which) {
+    case PDX_PRE_JACOBI:
         hipLaunchKernelGGL(k_apply_block, grid1(c->n), dim3(256), 0, c->stream, c->n, k->dinv, v, z);
         PDX_HIP(c, hipGetLastError());
         return PDX_OK;
+    case PDX_PRE_TWO_LEVEL: return pdx_nk_two_level(c, v, z);
+    case PDX_PRE_LEARNED: return pdx_gnn_apply(c, v, z);
+    default:
+        PDX_HIP(c, hipMemcpyAsync(z, v, sizeof(double) * k->m, hipMemcpyDeviceToDevice, c->stream));
+        return PDX_OK;
     }
-    if (which == 2) return pdx_gnn_apply(c, v, z);
-    PDX_HIP(c, hipMemcpyAsync(z, v, sizeof(double) * k->m, hipMemcpyDeviceToDevice, c->stream));
-    return PDX_OK;
 }
 
 static pdx_status build_jacobi(pdx_ctx *c)
@@ -377,25 +634,34 @@ pdx_status pdx_nk_build_jacobi(pdx_ctx *c)
     return build_jacobi(c);
 }
 
-// Flexible GMRES(restart) for J du = b (b = -r in k->t0), du in k->du (starts at zero).
-// Stops at ||res|| <= max(reltol ||b||, abstol) or after maxiter products.  Returns the
-// number of products; *rel_out is the final residual estimate / ||b||.
+pdx_status pdx_nk_build_coarse(pdx_ctx *c)
+{
+    if (!c->nk) return pdx_fail(c, PDX_ERR_STATE, "Newton-Krylov workspace missing");
+    return build_coarse(c);
+}
+
+// Flexible GMRES(restart) for J du = b, du in k->du (starts at zero).  Stops at
+// ||res|| <= max(reltol ||b||, abstol) or after maxiter products.  Classical Gram-Schmidt
+// with one reorthogonalisation pass; the coefficients and norms stay in device memory and
+// the host reads them once per iteration for the Givens rotations.
 static pdx_status fgmres(pdx_ctx *c, const double *b, int restart, int maxiter, double reltol,
                          double abstol, int precond, double scale, int *iters_out,
                          double *res_out)
 {
     pdx_nk *k = c->nk;
     const int64_t m = k->m;
+    const int R1 = PDX_MAX_RESTART + 1;
+    double *hd = k->h_dev, *h2d = k->h_dev + R1, *hnd = k->h_dev + 2 * R1;  // device
     PDX_HIP(c, hipMemsetAsync(k->du, 0, sizeof(double) * m, c->stream));
+    PDX_TRY(nk_refresh_unorm(c));
     double bnorm;
     PDX_TRY(norm2(c, b, &bnorm));
     const double thr = std::max(reltol * bnorm, abstol);
     int it = 0;
-    // r = b - A du = b (du = 0)
     PDX_HIP(c, hipMemcpyAsync(k->r, b, sizeof(double) * m, hipMemcpyDeviceToDevice, c->stream));
     double beta = bnorm;
     std::vector<double> H((size_t)(restart + 1) * restart), cs(restart), sn(restart), g(restart + 1),
-        y(restart), h(restart + 1), h2(restart + 1);
+        y(restart), hh(2 * R1 + 1);
     while (it < maxiter && beta > thr) {
         std::fill(H.begin(), H.end(), 0.0);
         std::fill(g.begin(), g.end(), 0.0);
@@ -412,23 +678,20 @@ static pdx_status fgmres(pdx_ctx *c, const double *b, int restart, int maxiter, 
             PDX_TRY(pdx_nk_jv_dev(c, Zj, w, scale));
             it++;
             if (pdx_gnn_collecting(c)) PDX_TRY(pdx_gnn_record_krylov(c, Zj, w));
-            // classical Gram-Schmidt, twice
-            PDX_TRY(multidot(c, j + 1, k->Vb, w, h.data()));
-            PDX_HIP(c, hipMemcpyAsync(k->h_dev, h.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice, c->stream));
-            hipLaunchKernelGGL(k_multiaxpy, grid1(m), dim3(256), 0, c->stream, m, j + 1, k->Vb, m, k->h_dev, w);
+            // classical Gram-Schmidt, twice, then the norm -- all on the device
+            PDX_TRY(multidot_dev(c, j + 1, k->Vb, w, hd));
+            hipLaunchKernelGGL(k_multiaxpy, grid1(m), dim3(256), 0, c->stream, m, j + 1, k->Vb, m, hd, w);
+            PDX_TRY(multidot_dev(c, j + 1, k->Vb, w, h2d));
+            hipLaunchKernelGGL(k_multiaxpy, grid1(m), dim3(256), 0, c->stream, m, j + 1, k->Vb, m, h2d, w);
+            PDX_TRY(multidot_dev(c, 1, w, w, hnd));
+            hipLaunchKernelGGL(k_scale_inv_sqrt, grid1(m), dim3(256), 0, c->stream, m, hnd, w, w);
             PDX_HIP(c, hipGetLastError());
-            PDX_TRY(multidot(c, j + 1, k->Vb, w, h2.data()));
-            PDX_HIP(c, hipMemcpyAsync(k->h_dev, h2.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice, c->stream));
-            hipLaunchKernelGGL(k_multiaxpy, grid1(m), dim3(256), 0, c->stream, m, j + 1, k->Vb, m, k->h_dev, w);
-            PDX_HIP(c, hipGetLastError());
-            for (int i = 0; i <= j; i++) H[i + (size_t)(restart + 1) * j] = h[i] + h2[i];
-            double hn;
-            PDX_TRY(norm2(c, w, &hn));
+            PDX_HIP(c, hipMemcpyAsync(hh.data(), k->h_dev, sizeof(double) * (2 * R1 + 1),
+                                      hipMemcpyDeviceToHost, c->stream));
+            PDX_HIP(c, hipStreamSynchronize(c->stream));
+            for (int i = 0; i <= j; i++) H[i + (size_t)(restart + 1) * j] = hh[i] + hh[R1 + i];
+            const double hn = std::sqrt(std::max(hh[2 * R1], 0.0));
             H[j + 1 + (size_t)(restart + 1) * j] = hn;
-            if (hn > 0) {
-                hipLaunchKernelGGL(k_scale, grid1(m), dim3(256), 0, c->stream, m, 1.0 / hn, w, w);
-                PDX_HIP(c, hipGetLastError());
-            }
             // Givens
             double *Hc = &H[(size_t)(restart + 1) * j];
             for (int i = 0; i < j; i++) {
@@ -452,9 +715,9 @@ static pdx_status fgmres(pdx_ctx *c, const double *b, int restart, int maxiter, 
         if (!done) j = restart;
         // y = H^-1 g (upper triangular j x j)
         for (int i = j - 1; i >= 0; i--) {
-            double s = g[i];
-            for (int l = i + 1; l < j; l++) s -= H[i + (size_t)(restart + 1) * l] * y[l];
-            y[i] = s / H[i + (size_t)(restart + 1) * i];
+            double sum = g[i];
+            for (int l = i + 1; l < j; l++) sum -= H[i + (size_t)(restart + 1) * l] * y[l];
+            y[i] = sum / H[i + (size_t)(restart + 1) * i];
         }
         PDX_HIP(c, hipMemcpyAsync(k->h_dev, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, c->stream));
         hipLaunchKernelGGL(k_lincomb_add, grid1(m), dim3(256), 0, c->stream, m, j, k->Zb, m, k->h_dev, k->du);
@@ -541,6 +804,7 @@ extern "C" pdx_status pdx_jv(pdx_ctx *c, const double *v_host, double *jv_host, 
     PDX_TRY(nk_ensure(c, 1));
     pdx_nk *k = c->nk;
     PDX_TRY(pdx_upload_vec(c, v_host, k->t0, 3));
+    PDX_TRY(nk_refresh_unorm(c));
     PDX_TRY(pdx_nk_jv_dev(c, k->t0, k->w, scale));
     return pdx_download_vec(c, k->w, jv_host, 3, 3);
 }
@@ -554,7 +818,8 @@ extern "C" pdx_status pdx_nk_step(pdx_ctx *c, int64_t n, int64_t n_steps, const 
     if (o->maxiter <= 0 || o->gmres_maxiter <= 0 || o->gmres_restart <= 0 ||
         o->gmres_restart > PDX_MAX_RESTART)
         return pdx_fail(c, PDX_ERR_ARG, "invalid Newton-Krylov options");
-    if (o->precond < 0 || o->precond > 2) return pdx_fail(c, PDX_ERR_ARG, "unknown preconditioner");
+    if (o->precond < PDX_PRE_NONE || o->precond > PDX_PRE_TWO_LEVEL)
+        return pdx_fail(c, PDX_ERR_ARG, "unknown preconditioner");
     PDX_HIP(c, hipSetDevice(c->device));
     PDX_TRY(nk_ensure(c, o->gmres_restart));
     pdx_nk *k = c->nk;
@@ -585,20 +850,24 @@ extern "C" pdx_status pdx_nk_step(pdx_ctx *c, int64_t n, int64_t n_steps, const 
         // J du = -r
         hipLaunchKernelGGL(k_scale, grid1(k->m), dim3(256), 0, c->stream, k->m, -1.0, k->r, k->t0);
         PDX_HIP(c, hipGetLastError());
-        int pre = o->precond;
-        if (pre >= 1) PDX_TRY(build_jacobi(c));
+        const int pre = o->precond;
+        if (pre != PDX_PRE_NONE) PDX_TRY(build_jacobi(c));
+        // the coarse operator of the two-level and learned preconditioners: once per load step
+        if ((pre == PDX_PRE_TWO_LEVEL || pre == PDX_PRE_LEARNED) && iter == 1) PDX_TRY(build_coarse(c));
         int its = 0;
         double rel = 0.0;
-        if (pre == 2) {
+        if (pre == PDX_PRE_LEARNED) {
             PDX_TRY(pdx_gnn_prepare(c));
             PDX_TRY(fgmres(c, k->t0, o->gmres_restart, o->gmres_maxiter, o->gmres_reltol,
-                           o->gmres_abstol, 2, scale, &its, &rel));
+                           o->gmres_abstol, PDX_PRE_LEARNED, scale, &its, &rel));
             S.krylov_iters += its;
-            // fallback: the learned preconditioner did not reach the tolerance
-            if (!(rel <= std::max(o->gmres_reltol, o->gmres_abstol / std::max(rn, 1e-300)) * 1.0000001)) {
+            // fallback: a learned solve that did not reach the tolerance and did not even
+            // halve the linear residual is redone with the point-block Jacobi preconditioner
+            const double tol_lin = std::max(o->gmres_reltol, o->gmres_abstol / std::max(rn, 1e-300));
+            if (!(rel <= tol_lin * 1.0000001) && !(rel <= 0.5)) {
                 S.precond_fallbacks++;
                 PDX_TRY(fgmres(c, k->t0, o->gmres_restart, o->gmres_maxiter, o->gmres_reltol,
-                               o->gmres_abstol, 1, scale, &its, &rel));
+                               o->gmres_abstol, PDX_PRE_JACOBI, scale, &its, &rel));
                 S.krylov_iters += its;
             }
         } else {

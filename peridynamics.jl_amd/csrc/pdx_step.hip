@@ -27,6 +27,8 @@
 
 #include "pdx_internal.h"
 
+#define PDX_LDS_LANES 4  // lanes per point in the LDS-staged BB kernels
+
 // thread index of the calling point; blocks are remapped so that each XCD gets a
 // contiguous range of points (see pdx_xcd_block)
 __device__ __forceinline__ int64_t pdx_thread_point()
@@ -284,8 +286,8 @@ __device__ __forceinline__ bool bond_fail16(const pdx_step_args &A, uint16_t *wp
     return act;
 }
 
-template <bool MIXED, int U>
-__global__ __launch_bounds__(PDX_BLOCK) void k_bb_lds(pdx_step_args A)
+template <int U, int S>
+__global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds(pdx_step_args A)
 {
     extern __shared__ double4 hrec[];
     const int64_t k = pdx_xcd_block();
@@ -293,10 +295,12 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_bb_lds(pdx_step_args A)
     if (p0 >= A.n) return;  // whole workgroup
     const int64_t h0 = A.halo_ptr[k];
     const int H = (int)(A.halo_ptr[k + 1] - h0);
-    for (int h = threadIdx.x; h < H; h += PDX_BLOCK) hrec[h] = A.rec_in[A.halo[h0 + h]];
+    for (int h = threadIdx.x; h < H; h += PDX_BLOCK * S) hrec[h] = A.rec_in[A.halo[h0 + h]];
     __syncthreads();
-    const int64_t p = p0 + threadIdx.x;
-    if (p >= A.n) return;
+    // S adjacent lanes share a point; lane s takes its bonds s, s + S, s + 2S, ...
+    const int sub = threadIdx.x % S;
+    const int64_t p = p0 + threadIdx.x / S;
+    if (p >= A.n) return;  // all S lanes of the point together
     const int c = A.cnt[p];
     const int64_t b0 = pdx_bond0(A.slice_ptr, p);
     uint16_t *const wbase = A.bw16 + b0;
@@ -305,60 +309,47 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_bb_lds(pdx_step_args A)
     double bx = 0.0, by = 0.0, bz = 0.0;
     int na = 0;
     const double bc = A.bc;
-    const float bcf = (float)A.bc, epscf = (float)A.epsc;
-    for (int t = 0; t < c; t += U) {
+    for (int t = sub; t < c; t += U * S) {
         uint32_t wd[U];
         double Ld[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
             wd[u] = 0;
             Ld[u] = 1.0;
-            if (t + u < c) {
-                const int64_t o = (int64_t)(t + u) * 64;
+            if (t + u * S < c) {
+                const int64_t o = (int64_t)(t + u * S) * 64;
                 wd[u] = wbase[o];
                 Ld[u] = Lbase[o];
             }
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            if (t + u >= c) continue;
+            if (t + u * S >= c) continue;
             const double4 rj = hrec[wd[u] & PDX_W16_SLOT];
             const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
-            uint16_t *const wp = wbase + (int64_t)(t + u) * 64;
+            uint16_t *const wp = wbase + (int64_t)(t + u * S) * 64;
             const double d2 = (dx * dx + dy * dy) + dz * dz;
-            if (!MIXED) {
-                double rl;
-                const double l = pdx_sqrt_rsq(d2, rl);
-                const double e = l - Ld[u];
-                const double eps = e * pdx_rcp1(Ld[u]);
-                const bool act = bond_fail16(A, wp, wd[u], e, Ld[u], eps);
-                const double wbc = act ? bc : 0.0;  // (w * bc) with w = bond_active
-                const double s = ((wbc * eps) * rj.w) * rl;
-                bx = __builtin_fma(s, dx, bx);
-                by = __builtin_fma(s, dy, by);
-                bz = __builtin_fma(s, dz, bz);
-                na += act ? 1 : 0;
-            } else {
-                // l^2 - L^2 from the FP64 squared distance in one fused operation keeps the
-                // stretch's digits; everything after it is FP32
-                const float num = (float)__builtin_fma(-Ld[u], Ld[u], d2);
-                const float Lb = (float)Ld[u];
-                const float l = __fsqrt_rn((float)d2);
-                const float eps = __fdividef(num, Lb * (l + Lb));
-                bool act = (wd[u] & PDX_W16_ACT) != 0;
-                if (act && (wd[u] & PDX_W16_FP) && eps > epscf) {
-                    act = false;
-                    *wp = (uint16_t)(wd[u] & ~PDX_W16_ACT);
-                }
-                const float wbc = act ? bcf : 0.0f;
-                const float sf = __fdividef((wbc * eps) * (float)rj.w, l);
-                bx += (double)(sf * (float)dx);
-                by += (double)(sf * (float)dy);
-                bz += (double)(sf * (float)dz);
-                na += act ? 1 : 0;
-            }
+            double rl;
+            const double l = pdx_sqrt_rsq(d2, rl);
+            const double e = l - Ld[u];
+            const double eps = e * pdx_rcp1(Ld[u]);
+            const bool act = bond_fail16(A, wp, wd[u], e, Ld[u], eps);
+            const double wbc = act ? bc : 0.0;  // (w * bc) with w = bond_active
+            const double s = ((wbc * eps) * rj.w) * rl;
+            bx = __builtin_fma(s, dx, bx);
+            by = __builtin_fma(s, dy, by);
+            bz = __builtin_fma(s, dz, bz);
+            na += act ? 1 : 0;
         }
     }
+#pragma unroll
+    for (int o = 1; o < S; o <<= 1) {
+        bx += __shfl_xor(bx, o);
+        by += __shfl_xor(by, o);
+        bz += __shfl_xor(bz, o);
+        na += __shfl_xor(na, o);
+    }
+    if (sub != 0) return;
     if (A.flags & (PDX_F_STORE | PDX_F_FORCE_ONLY)) {
         A.nact_out[p] = na;
         A.dmg_out[p] = 1.0 - (double)na / (double)c;
@@ -367,6 +358,100 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_bb_lds(pdx_step_args A)
     tail_load(A, p, T);
     const double b[3] = {bx, by, bz};
     point_tail(A, p, ri, b, T);
+}
+
+// ----------------------------------------------------------------------------------------
+// Mixed-precision BB step on the LDS-staged layout.  The stretch is formed without
+// cancellation from the reference bond vector D = X_j - X_i and the displacement difference
+// U = u_j - u_i (u = x - X of the positions the step receives):  l^2 - L^2 = 2 D.U + U.U,  eps = (l^2 - L^2) / (L (l + L)),  L = |D|,
+// l = |D + U|.  Both are staged per block in FP32 relative to the block's first point
+// (subtracted in FP64 first), so their rounding is relative to the block's extent, not to
+// the absolute coordinates, and a rigid motion of the body costs no precision.  Every
+// bond operation is FP32; the per-point sums are FP64 (each FP32 term is added with FP64
+// rounding, the effect of a compensated FP32 sum).  Per bond only the 16-bit bond word is
+// read: L comes from D.  The Velocity Verlet tail stays FP64.
+// ----------------------------------------------------------------------------------------
+template <int U, int S>
+__global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds_mx(pdx_step_args A)
+{
+    extern __shared__ float4 hmx[];  // [2 H]: (X - O, V), (u - u_O, 0)
+    const int64_t k = pdx_xcd_block();
+    const int64_t p0 = k * PDX_BLOCK;
+    if (p0 >= A.n) return;  // whole workgroup
+    const int64_t h0 = A.halo_ptr[k];
+    const int H = (int)(A.halo_ptr[k + 1] - h0);
+    // displacements are taken as x - X of the positions the step receives, so that the
+    // perturbed positions of the Newton-Krylov products work like the Verlet state
+    const double4 O = A.Xr[p0], xo = A.rec_in[p0];
+    const double uo0 = xo.x - O.x, uo1 = xo.y - O.y, uo2 = xo.z - O.z;
+    for (int h = threadIdx.x; h < H; h += PDX_BLOCK * S) {
+        const int64_t j = A.halo[h0 + h];
+        const double4 X = A.Xr[j], x = A.rec_in[j];
+        hmx[2 * h] = make_float4((float)(X.x - O.x), (float)(X.y - O.y), (float)(X.z - O.z), (float)X.w);
+        hmx[2 * h + 1] = make_float4((float)((x.x - X.x) - uo0), (float)((x.y - X.y) - uo1),
+                                     (float)((x.z - X.z) - uo2), 0.f);
+    }
+    __syncthreads();
+    const int sub = threadIdx.x % S;
+    const int64_t p = p0 + threadIdx.x / S;
+    if (p >= A.n) return;  // all S lanes of the point together
+    const int c = A.cnt[p];
+    const int64_t b0 = pdx_bond0(A.slice_ptr, p);
+    uint16_t *const wbase = A.bw16 + b0;
+    const double4 ri = A.rec_in[p];
+    const double4 Xi = A.Xr[p];
+    const float xi = (float)(Xi.x - O.x), yi = (float)(Xi.y - O.y), zi = (float)(Xi.z - O.z);
+    const float ui = (float)((ri.x - Xi.x) - uo0), vi = (float)((ri.y - Xi.y) - uo1),
+                wi = (float)((ri.z - Xi.z) - uo2);
+    double bx = 0.0, by = 0.0, bz = 0.0;
+    int na = 0;
+    const float bcf = (float)A.bc, epscf = (float)A.epsc;
+    for (int t = sub; t < c; t += U * S) {
+        uint32_t wd[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) wd[u] = (t + u * S < c) ? wbase[(int64_t)(t + u * S) * 64] : 0u;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (t + u * S >= c) continue;
+            const int slot = (int)(wd[u] & PDX_W16_SLOT);
+            const float4 a = hmx[2 * slot], b = hmx[2 * slot + 1];
+            const float Dx = a.x - xi, Dy = a.y - yi, Dz = a.z - zi;
+            const float Ux = b.x - ui, Uy = b.y - vi, Uz = b.z - wi;
+            const float L2 = __builtin_fmaf(Dz, Dz, __builtin_fmaf(Dy, Dy, Dx * Dx));
+            const float DU = __builtin_fmaf(Dz, Uz, __builtin_fmaf(Dy, Uy, Dx * Ux));
+            const float UU = __builtin_fmaf(Uz, Uz, __builtin_fmaf(Uy, Uy, Ux * Ux));
+            const float num = __builtin_fmaf(2.f, DU, UU);  // l^2 - L^2
+            const float Lb = __builtin_amdgcn_sqrtf(L2);
+            const float l = __builtin_amdgcn_sqrtf(L2 + num);
+            const float eps = num * __builtin_amdgcn_rcpf(Lb * (l + Lb));
+            bool act = (wd[u] & PDX_W16_ACT) != 0;
+            if (act && (wd[u] & PDX_W16_FP) && eps > epscf) {
+                act = false;
+                wbase[(int64_t)(t + u * S) * 64] = (uint16_t)(wd[u] & ~PDX_W16_ACT);
+            }
+            const float sf = act ? ((bcf * eps) * a.w) * __builtin_amdgcn_rcpf(l) : 0.f;
+            bx += (double)(sf * (Dx + Ux));
+            by += (double)(sf * (Dy + Uy));
+            bz += (double)(sf * (Dz + Uz));
+            na += act ? 1 : 0;
+        }
+    }
+#pragma unroll
+    for (int o = 1; o < S; o <<= 1) {
+        bx += __shfl_xor(bx, o);
+        by += __shfl_xor(by, o);
+        bz += __shfl_xor(bz, o);
+        na += __shfl_xor(na, o);
+    }
+    if (sub != 0) return;
+    if (A.flags & (PDX_F_STORE | PDX_F_FORCE_ONLY)) {
+        A.nact_out[p] = na;
+        A.dmg_out[p] = 1.0 - (double)na / (double)c;
+    }
+    tail_ops T;
+    tail_load(A, p, T);
+    const double bb[3] = {bx, by, bz};
+    point_tail(A, p, ri, bb, T);
 }
 
 // ----------------------------------------------------------------------------------------
@@ -776,12 +861,16 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
         PDX_TRY(pdx_timed_begin(c));
         if (c->lds_mode) {
             const size_t lds = sizeof(double4) * (size_t)std::max(c->halo_max, 1);
+            // four lanes per point (the halo table of a block limits the blocks per CU, so
+            // the lanes, not the blocks, supply the parallelism that hides the bond loads)
+            const dim3 BL(PDX_BLOCK * PDX_LDS_LANES);
+            const bool u4 = c->bb_unroll == 4;
             if (c->precision == PDX_MIXED) {
-                if (c->bb_unroll == 4) hipLaunchKernelGGL((k_bb_lds<true, 4>), G, B, lds, c->stream, a);
-                else hipLaunchKernelGGL((k_bb_lds<true, 2>), G, B, lds, c->stream, a);
+                if (u4) hipLaunchKernelGGL((k_bb_lds_mx<4, PDX_LDS_LANES>), G, BL, lds, c->stream, a);
+                else hipLaunchKernelGGL((k_bb_lds_mx<2, PDX_LDS_LANES>), G, BL, lds, c->stream, a);
             } else {
-                if (c->bb_unroll == 4) hipLaunchKernelGGL((k_bb_lds<false, 4>), G, B, lds, c->stream, a);
-                else hipLaunchKernelGGL((k_bb_lds<false, 2>), G, B, lds, c->stream, a);
+                if (u4) hipLaunchKernelGGL((k_bb_lds<4, PDX_LDS_LANES>), G, BL, lds, c->stream, a);
+                else hipLaunchKernelGGL((k_bb_lds<2, PDX_LDS_LANES>), G, BL, lds, c->stream, a);
             }
         } else if (c->precision == PDX_MIXED) {
             if (c->bb_unroll == 4) hipLaunchKernelGGL((k_bb_step<true, 4>), G, B, 0, c->stream, a);

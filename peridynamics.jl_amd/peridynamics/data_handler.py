@@ -148,7 +148,7 @@ class MI355XBodyDataHandler:
         return self
 
     # ------------------------------------------------------------------ Newton-Krylov
-    PRECONDS = {"none": 0, "jacobi": 1, "learned": 2}
+    PRECONDS = {"none": 0, "jacobi": 1, "learned": 2, "two_level": 3}
 
     def nk_options(self, nk, precond=None):
         pre = self.PRECONDS[precond or nk.precond]

@@ -47,7 +47,9 @@ class NewtonKrylov:
 
     Additions of the MI355X build (defaults keep the reference behaviour):
     rel_tol > 0 stops a load step at ||r|| < rel_tol * ||r_first|| (the reference's `tol`
-    is absolute, newton_krylov.jl:311-314); precond = "none" | "jacobi" | "learned".
+    is absolute, newton_krylov.jl:311-314); precond = "none" | "jacobi" | "two_level" | "learned"
+    ("two_level": block Jacobi plus a Galerkin coarse correction on Morton-block aggregates;
+    "learned": the two-level preconditioner plus the trained message-passing network).
     With "learned", load steps 1 .. learn_steps are solved with the Jacobi preconditioner
     while their Newton and Krylov pairs are collected (at most krylov_pairs Krylov pairs),
     the network is then trained (train_steps Adam steps of train_batch pairs, step size
@@ -76,8 +78,8 @@ class NewtonKrylov:
                           ("perturbation_scale", perturbation_scale)):
             if val <= 0:
                 raise _arg_error(f"{name} has to be larger than zero: {name} > 0")
-        if precond not in ("none", "jacobi", "learned"):
-            raise _arg_error("precond must be 'none', 'jacobi' or 'learned'")
+        if precond not in ("none", "jacobi", "two_level", "learned"):
+            raise _arg_error("precond must be 'none', 'jacobi', 'two_level' or 'learned'")
         self.end_time = end_time
         self.n_steps = n_steps
         self.dt = float(stepsize)

@@ -62,6 +62,72 @@ def step_bytes(n_points, n_bonds, precision, layout):
             f"{per_bond}*N_b + 168*N_p + 8*N_slices")
 
 
+def jfnk_block(nx, ny, nz, dx=1e-3, jitter=0.05, seed=0x5EED, top=1e-4):
+    """Quasi-static workload: bond-based block, bottom face fixed, top face displaced in y."""
+    import peridynamics as pd
+    pos, vol = pd.lattice_block(nx, ny, nz, dx, jitter_amplitude=jitter, seed=seed)
+    body = pd.Body(pd.BBMaterial(), pos, vol)
+    pd.material(body, horizon=3.015 * dx, rho=2700.0, E=70e9)
+    y = pos[1]
+    pd.point_set(body, "bottom", y < y.min() + 0.5 * dx)
+    pd.point_set(body, "top", y > y.max() - 0.5 * dx)
+    for d in "xyz":
+        pd.displacement_bc(lambda p: 0.0, body, "bottom", d)
+    pd.displacement_bc(lambda p: top, body, "top", "y")
+    return body
+
+
+def jfnk_leg(shape, train_steps):
+    """Wall time of the 10 load steps to a 1e-8 relative Newton residual with GMRES(30),
+    200 Krylov iterations per Newton step: block Jacobi ("plain"), the two-level
+    preconditioner, and the learned one (trained on the pairs of load steps 1-3, which it
+    solves with block Jacobi)."""
+    import numpy as np
+    import peridynamics as pd
+    res = {}
+    u_ref = None
+    for pre in ("jacobi", "two_level", "learned"):
+        body = jfnk_block(*shape)
+        dh = pd.MI355XBodyDataHandler(body)
+        nk = pd.NewtonKrylov(steps=10, tol=1e-30, rel_tol=1e-8, maxiter=40, gmres_restart=30,
+                             gmres_maxiter=200, precond=pre, learn_steps=3,
+                             train_steps=train_steps, train_lr=2e-3)
+        dh.ctx.synchronize()
+        t0 = time.perf_counter()
+        stats = dh.solve_nk(nk)
+        dh.ctx.synchronize()
+        wall = time.perf_counter() - t0
+        u = dh.get("displacement")
+        if u_ref is None:
+            u_ref = u
+        info = getattr(dh, "train_info", None)
+        res[pre] = {"seconds": wall,
+                    "seconds_steps_4_10": sum(s.seconds for s in stats[3:]),
+                    "newton_iters": sum(s.newton_iters for s in stats),
+                    "krylov_iters": sum(s.krylov_iters for s in stats),
+                    "krylov_iters_steps_4_10": sum(s.krylov_iters for s in stats[3:]),
+                    "fallbacks": sum(s.precond_fallbacks for s in stats),
+                    "max_final_rel_residual": max(s.residual_final / s.residual_first for s in stats),
+                    "displacement_rel_diff_vs_jacobi": float(np.linalg.norm(u - u_ref) /
+                                                             np.linalg.norm(u_ref))}
+        if info:
+            res[pre]["train_seconds"] = info["seconds"]
+            res[pre]["train_loss"] = info["loss"]
+            res[pre]["train_pairs"] = list(info["pairs"])
+        log(f"[bench] jfnk {pre}: {wall:.3f} s, krylov {res[pre]['krylov_iters']}")
+        dh.close()
+    plain = res["jacobi"]
+    return {"workload": (f"bond-based block {shape[0]}x{shape[1]}x{shape[2]}, bottom fixed, top "
+                         f"displaced 1e-4 m in y over 10 load steps, Newton rel tol 1e-8, GMRES(30)"),
+            "points": shape[0] * shape[1] * shape[2], **res,
+            "speedup_learned_vs_plain": plain["seconds"] / res["learned"]["seconds"],
+            "speedup_learned_vs_plain_steps_4_10": (plain["seconds_steps_4_10"] /
+                                                   res["learned"]["seconds_steps_4_10"]),
+            "speedup_two_level_vs_plain": plain["seconds"] / res["two_level"]["seconds"],
+            "speedup_two_level_vs_plain_steps_4_10": (plain["seconds_steps_4_10"] /
+                                                     res["two_level"]["seconds_steps_4_10"])}
+
+
 def cpu_baseline(sample_shape, target_seconds=12.0, max_steps=5000):
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -104,6 +170,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=100)
+    ap.add_argument("--no-jfnk", action="store_true")
+    ap.add_argument("--jfnk-shape", type=str, default="100,100,10")
+    ap.add_argument("--jfnk-train-steps", type=int, default=300)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
@@ -224,6 +293,11 @@ def main():
                         "broken_bonds_mixed": int(n_bonds - np.sum(nm)),
                         "steps_compared": total_steps}
         dhm.close()
+
+    if not args.no_jfnk:
+        log("[bench] JFNK leg")
+        out["jfnk"] = jfnk_leg(tuple(int(v) for v in args.jfnk_shape.split(",")),
+                               args.jfnk_train_steps)
 
     if not args.no_cpu:
         log("[bench] CPU baseline")

@@ -199,24 +199,21 @@ __global__ __launch_bounds__(256) void k_gnn_q_final(int nblk, const double *par
     }
 }
 
-// encoder: F_i and H0_i = relu(W_e^T F_i + b_e); one thread per point
+// encoder: F_i and H0_i = relu(W_e^T F_i + b_e); one wavefront per point, lane = feature
+// (the per-point inputs are broadcast loads, the stores of H0 are coalesced)
 __global__ __launch_bounds__(256) void k_gnn_encode(int64_t n, int64_t npad, const double *q,
                                                     const double *y, const uint8_t *cmask,
                                                     const float *stat, const double *scal,
                                                     const float *theta, float *Fo, float *Z0,
                                                     uint16_t *H0)
 {
-    __shared__ float W[GFIN * GF + GF];
-    for (int k = threadIdx.x; k < GFIN * GF + GF; k += 256) W[k] = theta[OFF_WE + k];
-    __syncthreads();
-    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int f = threadIdx.x & 63;
     if (p >= npad) return;
     if (p >= n) {
-        for (int f = 0; f < GF; f++) H0[p * GF + f] = 0;
-        if (Fo)
-            for (int k = 0; k < GFIN; k++) Fo[p * GFIN + k] = 0.f;
-        if (Z0)
-            for (int f = 0; f < GF; f++) Z0[p * GF + f] = 0.f;
+        H0[p * GF + f] = 0;
+        if (Fo && f < GFIN) Fo[p * GFIN + f] = 0.f;
+        if (Z0) Z0[p * GF + f] = 0.f;
         return;
     }
     const float is = (float)(1.0 / scal[SC_S]), iy = (float)(1.0 / scal[SC_RY]);
@@ -229,30 +226,35 @@ __global__ __launch_bounds__(256) void k_gnn_encode(int64_t n, int64_t npad, con
     }
     for (int k = 0; k < 5; k++) Fi[9 + k] = stat[5 * p + k];
     Fi[14] = Fi[15] = 0.f;
-    if (Fo)
-        for (int k = 0; k < GFIN; k++) Fo[p * GFIN + k] = Fi[k];
-    for (int f = 0; f < GF; f++) {
-        float z = W[GFIN * GF + f];
-        for (int k = 0; k < GFIN; k++) z += Fi[k] * W[k * GF + f];
-        if (Z0) Z0[p * GF + f] = z;
-        H0[p * GF + f] = f2bf(fmaxf(z, 0.f));
-    }
+    if (Fo && f < GFIN) Fo[p * GFIN + f] = Fi[f];
+    float z = theta[OFF_BE + f];
+#pragma unroll
+    for (int k = 0; k < GFIN; k++) z += Fi[k] * theta[OFF_WE + k * GF + f];
+    if (Z0) Z0[p * GF + f] = z;
+    H0[p * GF + f] = f2bf(fmaxf(z, 0.f));
 }
 
 // sums over 64-point blocks of `ng` column groups of 64 (source bf16 or float, row stride
-// `ld`, first column `c0`); one workgroup of 64*ng threads per block
+// `ld`, first column `c0`); a workgroup of 4 x 64 ng threads per block: thread (g, col)
+// sums rows g, g + 4, ..., the four partial sums meet in LDS
 template <typename T>
-__global__ void k_gnn_pool1(int64_t n, int ng, const T *src, int ld, int c0, float *S1)
+__global__ __launch_bounds__(768) void k_gnn_pool1(int64_t n, int ng, const T *src, int ld, int c0,
+                                                   float *S1)
 {
+    __shared__ float sh[4][3 * GF];
     const int64_t b = blockIdx.x;
-    const int col = threadIdx.x;  // < 64 * ng
+    const int ncol = GF * ng;
+    const int col = threadIdx.x % ncol, g = threadIdx.x / ncol;
     const int64_t p0 = b * GB1, p1 = min(n, p0 + GB1);
     float s = 0.f;
-    for (int64_t p = p0; p < p1; p++) {
+#pragma unroll 4
+    for (int64_t p = p0 + g; p < p1; p += 4) {
         if constexpr (sizeof(T) == 2) s += bf2f(src[p * ld + c0 + col]);
         else s += src[p * ld + c0 + col];
     }
-    S1[b * (GF * 3) + col] = s;
+    sh[g][col] = s;
+    __syncthreads();
+    if (g == 0) S1[b * (GF * 3) + col] = (sh[0][col] + sh[1][col]) + (sh[2][col] + sh[3][col]);
 }
 
 // sums of 64 consecutive first-level sums (4096-point blocks)
@@ -295,7 +297,19 @@ __global__ __launch_bounds__(256) void k_gnn_concat(int64_t n, int64_t npad, con
     }
     const int64_t j0 = near_ptr[p], j1 = near_ptr[p + 1];
     float a = 0.f;
-    for (int64_t j = j0; j < j1; j++) a += bf2f(H[(int64_t)near[j] * GF + f]);
+    // eight neighbours in flight: indices first, then their rows
+    int64_t j = j0;
+    for (; j + 8 <= j1; j += 8) {
+        int32_t q[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) q[u] = near[j + u];
+        float h[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) h[u] = bf2f(H[(int64_t)q[u] * GF + f]);
+#pragma unroll
+        for (int u = 0; u < 8; u++) a += h[u];
+    }
+    for (; j < j1; j++) a += bf2f(H[(int64_t)near[j] * GF + f]);
     if (j1 > j0) a /= (float)(j1 - j0);
     const int64_t b1 = p / GB1, b2 = p / GB2;
     x[f] = H[p * GF + f];
@@ -309,9 +323,9 @@ __global__ __launch_bounds__(256) void k_gnn_concat(int64_t n, int64_t npad, con
 // with v_mfma_f32_16x16x32_bf16 (lane l: A row l%16, B column l%16, k block 8*(l/16);
 // D rows 4*(l/16)+i, column l%16).  Rows: npad (multiple of 256).
 template <bool TRAIN>
-__global__ __launch_bounds__(256) void k_gnn_gemm_fwd(const uint16_t *X, const uint16_t *WT,
+__global__ __launch_bounds__(256) void k_gnn_gemm_fwd(int64_t n, const uint16_t *X, const uint16_t *WT,
                                                       const float *bias, const uint16_t *Hin,
-                                                      uint16_t *Hout, float *Z)
+                                                      uint16_t *Hout, float *Z, float *S1)
 {
     const int lane = threadIdx.x & 63;
     const int64_t row0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
@@ -333,6 +347,9 @@ __global__ __launch_bounds__(256) void k_gnn_gemm_fwd(const uint16_t *X, const u
             for (int c = 0; c < 4; c++)
                 acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], b[c], acc[r][c], 0, 0, 0);
     }
+    // the wave's 64 rows are one 64-point pooling block: its column sums of the new H
+    // (rows < n) are the next layer's first-level pooled sums
+    float bs[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int r = 0; r < 4; r++)
 #pragma unroll
@@ -343,12 +360,25 @@ __global__ __launch_bounds__(256) void k_gnn_gemm_fwd(const uint16_t *X, const u
                 const int col = 16 * c + lr;
                 const float z = acc[r][c][i] + bias[col];
                 if (TRAIN) Z[row * GF + col] = z;
-                Hout[row * GF + col] = f2bf(bf2f(Hin[row * GF + col]) + fmaxf(z, 0.f));
+                const uint16_t h = f2bf(bf2f(Hin[row * GF + col]) + fmaxf(z, 0.f));
+                Hout[row * GF + col] = h;
+                if (row < n) bs[c] += bf2f(h);
             }
+    if (S1) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            bs[c] += __shfl_xor(bs[c], 16);
+            bs[c] += __shfl_xor(bs[c], 32);
+        }
+        if (lane < 16 && row0 < n)
+#pragma unroll
+            for (int c = 0; c < 4; c++) S1[(row0 / GB1) * (GF * 3) + 16 * c + lane] = bs[c];
+    }
 }
 
 // decoder O = W_d^T H + b_d.  Apply: z = q + c s O (free), y (constrained).
 // Train: O kept, partial sums of (O - t)^2 and t^2 with t = (x - q) / (c s).
+// 16 lanes per point (4 features each, 8-byte loads), 16 points per workgroup.
 template <bool TRAIN>
 __global__ __launch_bounds__(256) void k_gnn_decode(int64_t n, const uint16_t *H, const float *theta,
                                                     const double *q, const double *y,
@@ -356,31 +386,41 @@ __global__ __launch_bounds__(256) void k_gnn_decode(int64_t n, const uint16_t *H
                                                     const double *scal, double *z, float *Oo,
                                                     double *part)
 {
-    __shared__ float W[GF * 3 + 3];
-    __shared__ double sh[2][256];
-    for (int k = threadIdx.x; k < GF * 3 + 3; k += 256) W[k] = theta[OFF_WD + k];
-    __syncthreads();
-    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    __shared__ double sh[2][16];
+    const int lane16 = threadIdx.x & 15, pl = threadIdx.x >> 4;
+    const int64_t p = (int64_t)blockIdx.x * 16 + pl;
     double r2 = 0.0, t2 = 0.0;
+    float O[3] = {0.f, 0.f, 0.f};
     if (p < n) {
-        float O[3] = {W[GF * 3], W[GF * 3 + 1], W[GF * 3 + 2]};
-        for (int f = 0; f < GF; f++) {
-            const float h = bf2f(H[p * GF + f]);
-            O[0] += h * W[3 * f];
-            O[1] += h * W[3 * f + 1];
-            O[2] += h * W[3 * f + 2];
+        const uint2 hv = *reinterpret_cast<const uint2 *>(H + p * GF + 4 * lane16);
+        const uint16_t hb[4] = {(uint16_t)(hv.x & 0xFFFF), (uint16_t)(hv.x >> 16),
+                                (uint16_t)(hv.y & 0xFFFF), (uint16_t)(hv.y >> 16)};
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int f = 4 * lane16 + e;
+            const float h = bf2f(hb[e]);
+            O[0] += h * theta[OFF_WD + 3 * f];
+            O[1] += h * theta[OFF_WD + 3 * f + 1];
+            O[2] += h * theta[OFF_WD + 3 * f + 2];
         }
+    }
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1)
+#pragma unroll
+        for (int d = 0; d < 3; d++) O[d] += __shfl_xor(O[d], o);
+    if (p < n && lane16 == 0) {
         const double cs = scal[SC_C] * scal[SC_S];
         for (int d = 0; d < 3; d++) {
             const int64_t k = 3 * p + d;
+            const float od = O[d] + theta[OFF_BD + d];
             const bool c = cmask[k] != 0;
             if (!TRAIN) {
-                z[k] = c ? y[k] : q[k] + cs * (double)O[d];
+                z[k] = c ? y[k] : q[k] + cs * (double)od;
             } else {
-                Oo[k] = O[d];
+                Oo[k] = od;
                 if (!c) {
                     const double t = (x[k] - q[k]) / cs;
-                    const double r = (double)O[d] - t;
+                    const double r = (double)od - t;
                     r2 += r * r;
                     t2 += t * t;
                 }
@@ -388,19 +428,19 @@ __global__ __launch_bounds__(256) void k_gnn_decode(int64_t n, const uint16_t *H
         }
     }
     if (TRAIN) {
-        sh[0][threadIdx.x] = r2;
-        sh[1][threadIdx.x] = t2;
-        __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
-            if ((int)threadIdx.x < o) {
-                sh[0][threadIdx.x] += sh[0][threadIdx.x + o];
-                sh[1][threadIdx.x] += sh[1][threadIdx.x + o];
-            }
-            __syncthreads();
+        if (lane16 == 0) {
+            sh[0][pl] = r2;
+            sh[1][pl] = t2;
         }
+        __syncthreads();
         if (threadIdx.x == 0) {
-            part[2 * blockIdx.x] = sh[0][0];
-            part[2 * blockIdx.x + 1] = sh[1][0];
+            double a = 0.0, b = 0.0;
+            for (int k = 0; k < 16; k++) {
+                a += sh[0][k];
+                b += sh[1][k];
+            }
+            part[2 * blockIdx.x] = a;
+            part[2 * blockIdx.x + 1] = b;
         }
     }
 }
@@ -436,16 +476,14 @@ __global__ void k_gnn_loss_final(int nblk, const double *part, double *scal, dou
 // ----------------------------------------------------------------------------------------
 // backward
 // ----------------------------------------------------------------------------------------
-// dO = scale (O - t) on free dofs; dH = W_d dO
+// dO = scale (O - t) on free dofs; dH = W_d dO.  One wavefront per point, lane = feature.
 __global__ __launch_bounds__(256) void k_gnn_decode_bwd(int64_t n, int64_t npad, const float *theta,
                                                         const float *O, const double *q,
                                                         const double *x, const uint8_t *cmask,
                                                         const double *scal, float *dO, float *dH)
 {
-    __shared__ float W[GF * 3];
-    for (int k = threadIdx.x; k < GF * 3; k += 256) W[k] = theta[OFF_WD + k];
-    __syncthreads();
-    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int f = threadIdx.x & 63;
     if (p >= npad) return;
     float g[3] = {0.f, 0.f, 0.f};
     if (p < n) {
@@ -458,8 +496,9 @@ __global__ __launch_bounds__(256) void k_gnn_decode_bwd(int64_t n, int64_t npad,
             }
         }
     }
-    for (int d = 0; d < 3; d++) dO[3 * p + d] = g[d];
-    for (int f = 0; f < GF; f++) dH[p * GF + f] = W[3 * f] * g[0] + W[3 * f + 1] * g[1] + W[3 * f + 2] * g[2];
+    if (f < 3) dO[3 * p + f] = g[f];
+    dH[p * GF + f] = theta[OFF_WD + 3 * f] * g[0] + theta[OFF_WD + 3 * f + 1] * g[1] +
+                     theta[OFF_WD + 3 * f + 2] * g[2];
 }
 
 // partial outer products sum_p A_p B_p^T and sum_p B_p over chunks of points: output
@@ -496,20 +535,24 @@ __global__ void k_gnn_reduce_add(int64_t nw, int64_t len, const float *part, flo
 }
 
 // dZ = dH * [Z > 0] (bf16, and float when dZf is given); bias partials per 256 points.
-// 64 threads (features) per workgroup.
-__global__ void k_gnn_relu_bwd(int64_t n, const float *dH, const float *Z, uint16_t *dZ,
-                               float *dZf, float *bpart)
+// Workgroup of 256 threads = 4 row groups x 64 features over 256 rows.
+__global__ __launch_bounds__(256) void k_gnn_relu_bwd(int64_t n, const float *dH, const float *Z,
+                                                      uint16_t *dZ, float *dZf, float *bpart)
 {
-    const int f = threadIdx.x;
+    __shared__ float sh[4][GF];
+    const int f = threadIdx.x & 63, gq = threadIdx.x >> 6;
     const int64_t p0 = (int64_t)blockIdx.x * 256;
     float s = 0.f;
-    for (int64_t p = p0; p < p0 + 256; p++) {
+#pragma unroll 4
+    for (int64_t p = p0 + gq; p < p0 + 256; p += 4) {
         const float g = (p < n && Z[p * GF + f] > 0.f) ? dH[p * GF + f] : 0.f;
         if (dZ) dZ[p * GF + f] = f2bf(g);
         if (dZf) dZf[p * GF + f] = g;
         s += g;
     }
-    if (bpart) bpart[(int64_t)blockIdx.x * GF + f] = s;
+    sh[gq][f] = s;
+    __syncthreads();
+    if (bpart && gq == 0) bpart[(int64_t)blockIdx.x * GF + f] = (sh[0][f] + sh[1][f]) + (sh[2][f] + sh[3][f]);
 }
 
 // dX = dZ W^T: [npad x 64] . [64 x 320]; one wavefront per 16 rows, all 320 columns
@@ -603,7 +646,21 @@ __global__ __launch_bounds__(256) void k_gnn_hbwd(int64_t n, int64_t npad, const
     }
     float g = dH[p * GF + f] + dX[p * GK + f];
     const int64_t j0 = near_ptr[p], j1 = near_ptr[p + 1];
-    for (int64_t jj = j0; jj < j1; jj++) {
+    int64_t jj = j0;
+    for (; jj + 4 <= j1; jj += 4) {
+        int64_t q[4];
+        float v[4], d[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) q[u] = near[jj + u];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            v[u] = dX[q[u] * GK + GF + f];
+            d[u] = (float)(near_ptr[q[u] + 1] - near_ptr[q[u]]);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) g += v[u] / d[u];
+    }
+    for (; jj < j1; jj++) {
         const int64_t j = near[jj];
         const float dj = (float)(near_ptr[j + 1] - near_ptr[j]);
         g += dX[j * GK + GF + f] / dj;
@@ -825,8 +882,9 @@ static pdx_status gnn_setup(pdx_ctx *c)
     PDX_TRY(pdx_alloc(c, &g->S1, g->nb1 * GF * 3));
     PDX_TRY(pdx_alloc(c, &g->S2, g->nb2 * GF * 3));
     PDX_TRY(pdx_alloc(c, &g->Sg, GF * 3));
+    // partial sums: 3 per 256 points (q), 2 per 16 points (decoder), 1 per 256 dofs (x - q)
     const int64_t nblk = (3 * n + 255) / 256 + 1;
-    PDX_TRY(pdx_alloc(c, &g->dpart, 3 * nblk));
+    PDX_TRY(pdx_alloc(c, &g->dpart, std::max<int64_t>(3 * nblk, 2 * ((n + 15) / 16) + 2)));
     PDX_TRY(pdx_alloc(c, &g->dH, np * GF));
     PDX_TRY(pdx_alloc(c, &g->dH2, np * GF));
     PDX_TRY(pdx_alloc(c, &g->dZf, np * GF));
@@ -854,30 +912,32 @@ static pdx_status gnn_forward(pdx_ctx *c, const double *y, const double *x, doub
     hipStream_t s = c->stream;
     hipLaunchKernelGGL(k_gnn_q, dim3(nb), dim3(256), 0, s, n, k->dinv, y, k->cmask, g->q, g->dpart);
     hipLaunchKernelGGL(k_gnn_q_final, dim3(1), dim3(256), 0, s, nb, g->dpart, g->scal);
-    hipLaunchKernelGGL(k_gnn_encode, g1(np), dim3(256), 0, s, n, np, g->q, y, k->cmask, g->stat,
+    hipLaunchKernelGGL(k_gnn_encode, dim3((unsigned)(np / 4)), dim3(256), 0, s, n, np, g->q, y, k->cmask, g->stat,
                        g->scal, g->theta, train ? g->F : nullptr, train ? g->Z0 : nullptr, g->H[0]);
     PDX_HIP(c, hipGetLastError());
+    hipLaunchKernelGGL(k_gnn_pool1<uint16_t>, dim3((unsigned)g->nb1), dim3(4 * GF), 0, s, n, 1, g->H[0],
+                       GF, 0, g->S1);
     for (int l = 0; l < GL; l++) {
-        hipLaunchKernelGGL(k_gnn_pool1<uint16_t>, dim3((unsigned)g->nb1), dim3(GF), 0, s, n, 1,
-                           g->H[l], GF, 0, g->S1);
         hipLaunchKernelGGL(k_gnn_pool2, dim3((unsigned)g->nb2), dim3(GF), 0, s, g->nb1, 1, g->S1, g->S2);
         hipLaunchKernelGGL(k_gnn_poolg, dim3(1), dim3(GF), 0, s, g->nb2, g->S2, g->Sg);
         hipLaunchKernelGGL(k_gnn_concat, dim3((unsigned)(np / 4)), dim3(256), 0, s, n, np, g->H[l],
                            g->near_ptr, g->near, g->S1, g->S2, g->Sg, g->X[l]);
+        float *S1n = l + 1 < GL ? g->S1 : nullptr;  // pooled sums of H[l + 1] for the next layer
         if (train)
-            hipLaunchKernelGGL(k_gnn_gemm_fwd<true>, dim3((unsigned)(np / 256)), dim3(256), 0, s,
-                               g->X[l], g->wT[l], g->theta + OFF_B(l), g->H[l], g->H[l + 1], g->Z[l]);
+            hipLaunchKernelGGL(k_gnn_gemm_fwd<true>, dim3((unsigned)(np / 256)), dim3(256), 0, s, n,
+                               g->X[l], g->wT[l], g->theta + OFF_B(l), g->H[l], g->H[l + 1], g->Z[l], S1n);
         else
-            hipLaunchKernelGGL(k_gnn_gemm_fwd<false>, dim3((unsigned)(np / 256)), dim3(256), 0, s,
-                               g->X[l], g->wT[l], g->theta + OFF_B(l), g->H[l], g->H[l + 1], nullptr);
+            hipLaunchKernelGGL(k_gnn_gemm_fwd<false>, dim3((unsigned)(np / 256)), dim3(256), 0, s, n,
+                               g->X[l], g->wT[l], g->theta + OFF_B(l), g->H[l], g->H[l + 1], nullptr, S1n);
         PDX_HIP(c, hipGetLastError());
     }
     if (train) {
-        hipLaunchKernelGGL(k_gnn_decode<true>, dim3(nb), dim3(256), 0, s, n, g->H[GL], g->theta, g->q,
+        const int nbd = (int)((n + 15) / 16);
+        hipLaunchKernelGGL(k_gnn_decode<true>, dim3(nbd), dim3(256), 0, s, n, g->H[GL], g->theta, g->q,
                            y, x, k->cmask, g->scal, nullptr, g->O, g->dpart);
-        hipLaunchKernelGGL(k_gnn_loss_final, dim3(1), dim3(256), 0, s, nb, g->dpart, g->scal, inv_batch);
+        hipLaunchKernelGGL(k_gnn_loss_final, dim3(1), dim3(256), 0, s, nbd, g->dpart, g->scal, inv_batch);
     } else {
-        hipLaunchKernelGGL(k_gnn_decode<false>, dim3(nb), dim3(256), 0, s, n, g->H[GL], g->theta, g->q,
+        hipLaunchKernelGGL(k_gnn_decode<false>, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, s, n, g->H[GL], g->theta, g->q,
                            y, nullptr, k->cmask, g->scal, z, nullptr, nullptr);
     }
     PDX_HIP(c, hipGetLastError());
@@ -892,7 +952,7 @@ static pdx_status gnn_backward(pdx_ctx *c, const double *x)
     const int64_t n = g->n, np = g->npad;
     hipStream_t s = c->stream;
     const int64_t nch = np / 256;  // 256-point chunks
-    hipLaunchKernelGGL(k_gnn_decode_bwd, g1(np), dim3(256), 0, s, n, np, g->theta, g->O, g->q, x,
+    hipLaunchKernelGGL(k_gnn_decode_bwd, dim3((unsigned)(np / 4)), dim3(256), 0, s, n, np, g->theta, g->O, g->q, x,
                        k->cmask, g->scal, g->dO, g->dH);
     // W_d, b_d
     hipLaunchKernelGGL(k_gnn_outer<uint16_t>, dim3((unsigned)nch), dim3(256), 0, s, np, (int64_t)256,
@@ -902,7 +962,7 @@ static pdx_status gnn_backward(pdx_ctx *c, const double *x)
     PDX_HIP(c, hipGetLastError());
     float *dHo = g->dH, *dHi = g->dH2;
     for (int l = GL - 1; l >= 0; l--) {
-        hipLaunchKernelGGL(k_gnn_relu_bwd, dim3((unsigned)nch), dim3(GF), 0, s, n, dHo, g->Z[l], g->dZ,
+        hipLaunchKernelGGL(k_gnn_relu_bwd, dim3((unsigned)nch), dim3(256), 0, s, n, dHo, g->Z[l], g->dZ,
                            nullptr, g->wpart);
         hipLaunchKernelGGL(k_gnn_reduce_add, dim3(1), dim3(GF), 0, s, nch, (int64_t)GF, g->wpart,
                            g->grad + OFF_B(l));
@@ -913,7 +973,7 @@ static pdx_status gnn_backward(pdx_ctx *c, const double *x)
         hipLaunchKernelGGL(k_gnn_gemm_dx, dim3((unsigned)(np / 64)), dim3(256), 0, s, g->dZ, g->wN[l],
                            g->dX);
         // pooled adjoints of dX columns 128..319
-        hipLaunchKernelGGL(k_gnn_pool1<float>, dim3((unsigned)g->nb1), dim3(3 * GF), 0, s, n, 3, g->dX,
+        hipLaunchKernelGGL(k_gnn_pool1<float>, dim3((unsigned)g->nb1), dim3(4 * 3 * GF), 0, s, n, 3, g->dX,
                            GK, 2 * GF, g->S1);
         hipLaunchKernelGGL(k_gnn_pool2, dim3((unsigned)g->nb2), dim3(3 * GF), 0, s, g->nb1, 3, g->S1, g->S2);
         hipLaunchKernelGGL(k_gnn_poolg, dim3(1), dim3(3 * GF), 0, s, g->nb2, g->S2, g->Sg);
@@ -923,7 +983,7 @@ static pdx_status gnn_backward(pdx_ctx *c, const double *x)
         std::swap(dHo, dHi);
     }
     // encoder
-    hipLaunchKernelGGL(k_gnn_relu_bwd, dim3((unsigned)nch), dim3(GF), 0, s, n, dHo, g->Z0, nullptr,
+    hipLaunchKernelGGL(k_gnn_relu_bwd, dim3((unsigned)nch), dim3(256), 0, s, n, dHo, g->Z0, nullptr,
                        g->dZf, nullptr);
     hipLaunchKernelGGL(k_gnn_outer<float>, dim3((unsigned)nch), dim3(256), 0, s, np, (int64_t)256, g->F,
                        GFIN, g->dZf, GF, g->wpart);

@@ -504,14 +504,16 @@ __global__ __launch_bounds__(256) void k_coarse_restrict(int64_t n, int agg, con
     if (threadIdx.x < 3) rc[3 * I + threadIdx.x] = sh[threadIdx.x][0];
 }
 
-// zc = Ainv rc (Ainv symmetric: thread r reads column r, coalesced over the rows)
-__global__ void k_coarse_matvec(int64_t mc, const double *Ainv, const double *rc, double *zc)
+// zc = Ainv rc: one wavefront per row, coalesced along the row, shuffle reduction
+__global__ __launch_bounds__(64) void k_coarse_matvec(int64_t mc, const double *Ainv, const double *rc,
+                                                      double *zc)
 {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= mc) return;
+    const int64_t r = blockIdx.x;
+    const double *row = Ainv + r * mc;
     double s = 0.0;
-    for (int64_t k = 0; k < mc; k++) s += Ainv[k * mc + r] * rc[k];
-    zc[r] = s;
+    for (int64_t k = threadIdx.x; k < mc; k += 64) s += row[k] * rc[k];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (threadIdx.x == 0) zc[r] = s;
 }
 
 // z = D^-1 v (+ P zc on free dofs when zc is given)
@@ -571,7 +573,7 @@ pdx_status pdx_nk_two_level(pdx_ctx *c, const double *v, double *z)
     if (!k->coarse_ready) return pdx_fail(c, PDX_ERR_STATE, "coarse space not built");
     hipLaunchKernelGGL(k_coarse_restrict, dim3((unsigned)k->nagg), dim3(256), 0, c->stream, c->n, k->agg,
                        v, k->cmask, k->rc);
-    hipLaunchKernelGGL(k_coarse_matvec, grid1(k->mc), dim3(256), 0, c->stream, k->mc, k->Ac, k->rc, k->zc);
+    hipLaunchKernelGGL(k_coarse_matvec, dim3((unsigned)k->mc), dim3(64), 0, c->stream, k->mc, k->Ac, k->rc, k->zc);
     hipLaunchKernelGGL(k_apply_two_level, grid1(c->n), dim3(256), 0, c->stream, c->n, k->agg, k->dinv, v,
                        k->zc, k->cmask, z);
     PDX_HIP(c, hipGetLastError());
@@ -595,7 +597,7 @@ pdx_status pdx_nk_coarse_add(pdx_ctx *c, const double *v, double *z, double alph
     if (!k->coarse_ready) return pdx_fail(c, PDX_ERR_STATE, "coarse space not built");
     hipLaunchKernelGGL(k_coarse_restrict, dim3((unsigned)k->nagg), dim3(256), 0, c->stream, c->n, k->agg,
                        v, k->cmask, k->rc);
-    hipLaunchKernelGGL(k_coarse_matvec, grid1(k->mc), dim3(256), 0, c->stream, k->mc, k->Ac, k->rc, k->zc);
+    hipLaunchKernelGGL(k_coarse_matvec, dim3((unsigned)k->mc), dim3(64), 0, c->stream, k->mc, k->Ac, k->rc, k->zc);
     hipLaunchKernelGGL(k_coarse_prolong_add, grid1(k->m), dim3(256), 0, c->stream, c->n, k->agg, k->zc,
                        k->cmask, alpha, z);
     PDX_HIP(c, hipGetLastError());

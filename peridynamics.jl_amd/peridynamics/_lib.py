@@ -8,7 +8,7 @@ import os
 import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(PKG_DIR), "libpdx.so")
+LIB_PATH = os.environ.get("PDX_LIB") or os.path.join(os.path.dirname(PKG_DIR), "libpdx.so")
 
 PDX_OK = 0
 PDX_ERR_ARG = -1

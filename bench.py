@@ -62,6 +62,25 @@ def step_bytes(n_points, n_bonds, precision, layout):
             f"{per_bond}*N_b + 168*N_p + 8*N_slices")
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "bb_step_2M_hbm_traffic.md")
+
+
+def measured_traffic(precision, n_points, n_bonds):
+    """HBM bytes per launch of the step kernel for this workload from the committed counter
+    summary (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by the copy-kernel
+    calibration), or None if the summary does not cover this workload."""
+    try:
+        text = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    tag = f"traffic {precision} points {n_points} bonds {n_bonds}:"
+    for line in text.splitlines():
+        line = line.strip().strip("`")
+        if line.startswith(tag):
+            return int(line[len(tag):].split()[0])
+    return None
+
+
 def jfnk_block(nx, ny, nz, dx=1e-3, jitter=0.05, seed=0x5EED, top=1e-4):
     """Quasi-static workload: bond-based block, bottom face fixed, top face displaced in y."""
     import peridynamics as pd
@@ -128,6 +147,149 @@ def jfnk_leg(shape, train_steps):
                                                      res["two_level"]["seconds_steps_4_10"])}
 
 
+def state_based_bytes(kind, n_points, n_bonds, layout):
+    """Algorithmic bytes of the two passes of an OSB or CPD step.  Per bond and pass the
+    bond word and the FP64 length (LDS layout 2 + 8, global layout 4 + 8).  Per point:
+    OSB 236 B (both passes read x|V and the count, pass 1 writes c1, c2 and pass 2 reads
+    them, plus the Verlet tail); CPD 559 B (also X|V in both passes and F, P K^-1 written
+    and read back).  LDS layout: plus the 4-byte neighbour-table index per entry and pass."""
+    per_pt = 236 if kind == "OSB" else 559
+    if layout["lds_mode"]:
+        return (20 * n_bonds + per_pt * n_points + 8 * layout["halo_entries"],
+                f"20*N_b + {per_pt}*N_p + 8*N_halo")
+    return 24 * n_bonds + per_pt * n_points, f"24*N_b + {per_pt}*N_p"
+
+
+def state_based_leg(kind, shape, steps, warmup=10, dx=1e-3):
+    """Configuration 3: OSB or CMaterial (Saint Venant-Kirchhoff) bar, torsion 10 rad/s
+    on one end (three point layers), the other end held, FP64."""
+    import peridynamics as pd
+    pos, vol = pd.lattice_block(*shape, dx)
+    mat = pd.OSBMaterial() if kind == "OSB" else pd.CMaterial()
+    body = pd.Body(mat, pos, vol)
+    pd.material(body, horizon=3.015 * dx, rho=7850.0, E=210e9, nu=0.25)
+    x = pos[0]
+    pd.point_set(body, "end", x > x.max() - 2.5 * dx)
+    pd.point_set(body, "fixed", x < x.min() + 2.5 * dx)
+    pd.velocity_bc(lambda p, t: -10.0 * p[2], body, "end", "y")
+    pd.velocity_bc(lambda p, t: 10.0 * p[1], body, "end", "z")
+    for d in "xyz":
+        pd.velocity_bc(lambda t: 0.0, body, "fixed", d)
+    dh = pd.MI355XBodyDataHandler(body)
+    vv = pd.VelocityVerlet(steps=warmup + steps)
+    dh.init_time_solver(vv)
+    dh.initialize(vv)
+    ctx = dh.ctx
+    bcv = dh.bc_values(range(1, vv.n_steps + 1), vv.dt)
+    ctx.vv_run(1, warmup, vv.dt, bcv[:warmup])
+    ctx.synchronize()
+    ctx.set_profiling(True)
+    t0 = time.perf_counter()
+    ctx.vv_run(1 + warmup, steps, vv.dt, bcv[warmup:])
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    k1, _ = ctx.kernel_time("pass1")
+    k2, _ = ctx.kernel_time("step")
+    ctx.set_profiling(False)
+    nb, npnt = dh.n_bonds, body.n_points
+    byt, formula = state_based_bytes(kind, npnt, nb, ctx.layout())
+    dh.close()
+    return {"workload": f"{kind} torsion bar {shape[0]}x{shape[1]}x{shape[2]}, 10 rad/s, FP64",
+            "points": npnt, "bonds": nb, "steps": steps, "ms_per_step": el / steps * 1e3,
+            "bond_updates_per_s": nb * steps / el, "pass1_ms": k1, "pass2_ms": k2,
+            "bytes_formula": formula, "bytes_per_step": byt,
+            "achieved_GBs": byt / ((k1 + k2) * 1e-3) / 1e9,
+            "frac_hbm": byt / ((k1 + k2) * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+
+def osb_2m_leg(shape, steps, warmup=10):
+    """The 2 M-point Velocity Verlet workload with OSBMaterial (E 70 GPa, nu 1/4, Gc 100),
+    FP64."""
+    import peridynamics as pd
+    dx = 1e-3
+    pos, vol = pd.lattice_block(*shape, dx, jitter_amplitude=0.05, seed=0x5EED)
+    body = pd.Body(pd.OSBMaterial(), pos, vol)
+    pd.material(body, horizon=3.015 * dx, rho=2700.0, E=70e9, nu=0.25, Gc=100.0)
+    y = pos[1]
+    pd.point_set(body, "top", y > y.max() - 2.5 * dx)
+    pd.point_set(body, "bottom", y < y.min() + 2.5 * dx)
+    pd.velocity_bc(lambda t: 0.6, body, "top", "y")
+    pd.velocity_bc(lambda t: -0.6, body, "bottom", "y")
+    dh = pd.MI355XBodyDataHandler(body)
+    vv = pd.VelocityVerlet(steps=warmup + steps)
+    dh.init_time_solver(vv)
+    dh.initialize(vv)
+    ctx = dh.ctx
+    bcv = dh.bc_values(range(1, vv.n_steps + 1), vv.dt)
+    ctx.vv_run(1, warmup, vv.dt, bcv[:warmup])
+    ctx.synchronize()
+    ctx.set_profiling(True)
+    t0 = time.perf_counter()
+    ctx.vv_run(1 + warmup, steps, vv.dt, bcv[warmup:])
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    k1, _ = ctx.kernel_time("pass1")
+    k2, _ = ctx.kernel_time("step")
+    ctx.set_profiling(False)
+    nb, npnt = dh.n_bonds, body.n_points
+    byt, formula = state_based_bytes("OSB", npnt, nb, ctx.layout())
+    dh.close()
+    return {"workload": f"OSB Velocity Verlet {shape[0]}x{shape[1]}x{shape[2]} jittered lattice, "
+                        f"+-0.6 m/s y-BCs, FP64",
+            "points": npnt, "bonds": nb, "steps": steps, "ms_per_step": el / steps * 1e3,
+            "bond_updates_per_s": nb * steps / el, "pass1_ms": k1, "pass2_ms": k2,
+            "bytes_formula": formula, "bytes_per_step": byt,
+            "achieved_GBs": byt / ((k1 + k2) * 1e-3) / 1e9,
+            "frac_hbm": byt / ((k1 + k2) * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+
+def mode_i_leg(npxy, nz, steps):
+    """Configuration 5: pre-cracked plate in mode I, +-2 m/s on the edges, FP64 and mixed;
+    time per step, and the mixed run's damage and crack tip against the FP64 run."""
+    import numpy as np
+    import peridynamics as pd
+    res = {}
+    for prec in ("fp64", "mixed", "fp64_perturbed"):
+        body, geo = pd.mode_i_plate(npxy=npxy, nz=nz, dx=2e-4, v=2.0)
+        if prec == "fp64_perturbed":
+            # sensitivity baseline: the FP64 run from positions moved by 1e-12 spacing
+            rng = np.random.default_rng(1)
+            body.position += 2e-16 * rng.choice([-1.0, 1.0], size=body.position.shape)
+        dh = pd.MI355XBodyDataHandler(body, precision="mixed" if prec == "mixed" else "fp64")
+        vv = pd.VelocityVerlet(steps=steps)
+        dh.init_time_solver(vv)
+        dh.initialize(vv)
+        bcv = dh.bc_values(range(1, steps + 1), vv.dt)
+        dh.ctx.synchronize()
+        t0 = time.perf_counter()
+        dh.ctx.vv_run(1, steps, vv.dt, bcv)
+        dh.ctx.synchronize()
+        el = time.perf_counter() - t0
+        nact = dh.get("n_active_bonds")
+        tip = pd.crack_tip(dh.get("position"), dh.get("damage"), xmin=geo["xmin"],
+                           cluster_radius=geo["delta"])
+        res[prec] = {"ms_per_step": el / steps * 1e3, "bond_updates_per_s": dh.n_bonds * steps / el,
+                     "broken_bonds": int(dh.n_bonds - nact.sum()), "crack_tip": list(tip),
+                     "_nact": nact, "_dmg": dh.get("damage")}
+        npnt, nb = body.n_points, dh.n_bonds
+        dh.close()
+    a, b, c = res["fp64"], res["mixed"], res["fp64_perturbed"]
+    out = {"workload": (f"pre-cracked plate {npxy}x{npxy}x{nz}, dx 2e-4 m, crack 0.25 l, "
+                        f"+-2 m/s, {steps} steps"),
+           "points": npnt, "bonds": nb,
+           "fp64": {k: v for k, v in a.items() if not k.startswith("_")},
+           "mixed": {k: v for k, v in b.items() if not k.startswith("_")},
+           "mixed_speedup": a["ms_per_step"] / b["ms_per_step"],
+           "mixed_broken_bond_count_difference": int(np.sum(np.abs(a["_nact"] - b["_nact"]))),
+           "mixed_max_damage_difference": float(np.max(np.abs(a["_dmg"] - b["_dmg"]))),
+           "mixed_crack_tip_dx_difference": (abs(a["crack_tip"][0] - b["crack_tip"][0]) / 2e-4
+                                             if not np.isnan(a["crack_tip"][0]) else None),
+           "fp64_perturbed_broken_bond_count_difference": int(np.sum(np.abs(a["_nact"] - c["_nact"]))),
+           "fp64_perturbed_crack_tip": c["crack_tip"],
+           "perturbation": "reference positions moved by +-2e-16 m (1e-12 spacing)"}
+    return out
+
+
 def cpu_baseline(sample_shape, target_seconds=12.0, max_steps=5000):
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -171,6 +333,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=100)
     ap.add_argument("--no-jfnk", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the state-based (512k-point bar) and mode-I plate legs")
     ap.add_argument("--jfnk-shape", type=str, default="100,100,10")
     ap.add_argument("--jfnk-train-steps", type=int, default=300)
     args = ap.parse_args()
@@ -247,7 +411,8 @@ def main():
                    "points": n_points, "bonds": n_bonds, "precision": args.precision,
                    "dt": dt},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(args.precision, n_points, n_bonds),
                      "kernel": "k_bb_lds" if layout["lds_mode"] else "k_bb_step",
                      "kernel_ms": kms, "bytes_formula": bytes_formula,
                      "bytes_per_launch": bytes_step, "halo_entries": layout["halo_entries"],
@@ -293,6 +458,13 @@ def main():
                         "broken_bonds_mixed": int(n_bonds - np.sum(nm)),
                         "steps_compared": total_steps}
         dhm.close()
+
+    if not args.no_extra:
+        log("[bench] state-based legs")
+        out["state_based"] = {k: state_based_leg(k, (160, 80, 40), 100) for k in ("OSB", "CPD")}
+        out["state_based"]["OSB_2M"] = osb_2m_leg(shape, 100)
+        log("[bench] mode-I plate leg")
+        out["mode_i"] = mode_i_leg(500, 4, 2000)
 
     if not args.no_jfnk:
         log("[bench] JFNK leg")

@@ -274,6 +274,7 @@ struct pdx_step_args {
     uint16_t *bw16;
     const int64_t *halo_ptr;
     const int32_t *halo;
+    int32_t halo_max;
     const double *L;
     const float *Lf;
     const double4 *rec_in;

@@ -46,43 +46,53 @@ void pdx_nk_free(pdx_ctx *c)
 // ----------------------------------------------------------------------------------------
 // small vector kernels (3n-long vectors in internal order)
 // ----------------------------------------------------------------------------------------
-// partial[blk * nv + k] = sum over this block's share of V_k . w   (k < nv)
+// partial[blk * nv + k] = sum over this block's share of V_k . w   (k < nv); eight
+// vectors per pass over w, one block reduction per eight dot products
+#define MD_CHUNK 8
 __global__ __launch_bounds__(256) void k_multidot(int64_t m, int nv, const double *V,
                                                   int64_t ldv, const double *w, double *partial)
 {
-    __shared__ double sh[256];
-    for (int k = 0; k < nv; k++) {
-        double s = 0.0;
-        for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < m; i += (int64_t)gridDim.x * 256)
-            s += V[k * ldv + i] * w[i];
-        sh[threadIdx.x] = s;
+    __shared__ double sh[MD_CHUNK][256];
+    for (int k0 = 0; k0 < nv; k0 += MD_CHUNK) {
+        const int kn = min(MD_CHUNK, nv - k0);
+        double s[MD_CHUNK];
+#pragma unroll
+        for (int q = 0; q < MD_CHUNK; q++) s[q] = 0.0;
+        for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < m; i += (int64_t)gridDim.x * 256) {
+            const double wi = w[i];
+#pragma unroll
+            for (int q = 0; q < MD_CHUNK; q++)
+                if (q < kn) s[q] += V[(int64_t)(k0 + q) * ldv + i] * wi;
+        }
+#pragma unroll
+        for (int q = 0; q < MD_CHUNK; q++) sh[q][threadIdx.x] = s[q];
         __syncthreads();
         for (int o = 128; o > 0; o >>= 1) {
-            if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+            if ((int)threadIdx.x < o)
+#pragma unroll
+                for (int q = 0; q < MD_CHUNK; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + o];
             __syncthreads();
         }
-        if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * nv + k] = sh[0];
+        if ((int)threadIdx.x < kn) partial[(int64_t)blockIdx.x * nv + k0 + threadIdx.x] = sh[threadIdx.x][0];
         __syncthreads();
     }
 }
 
-// out[k] = sum over blocks of partial[blk * nv + k]   (one block, nv <= 64 handled in a loop)
+// out[k] = sum over blocks of partial[blk * nv + k]; one workgroup per k
 __global__ __launch_bounds__(256) void k_reduce_partials(int nblk, int nv, const double *partial,
                                                          double *out)
 {
     __shared__ double sh[256];
-    for (int k = 0; k < nv; k++) {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < nblk; b += 256) s += partial[(int64_t)b * nv + k];
-        sh[threadIdx.x] = s;
-        __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
-            if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[k] = sh[0];
+    const int k = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += 256) s += partial[(int64_t)b * nv + k];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
         __syncthreads();
     }
+    if (threadIdx.x == 0) out[k] = sh[0];
 }
 
 // w -= sum_k h[k] V_k
@@ -329,8 +339,8 @@ static pdx_status multidot_dev(pdx_ctx *c, int nv, const double *V, const double
     hipLaunchKernelGGL(k_multidot, dim3(RED_BLOCKS), dim3(256), 0, c->stream, k->m, nv, V, k->m, w,
                        k->red);
     PDX_HIP(c, hipGetLastError());
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, c->stream, RED_BLOCKS, nv, k->red,
-                       out_dev);
+    hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)nv), dim3(256), 0, c->stream, RED_BLOCKS, nv,
+                       k->red, out_dev);
     PDX_HIP(c, hipGetLastError());
     return PDX_OK;
 }

@@ -475,7 +475,7 @@ pdx_status pdx_download_vec(pdx_ctx *c, const double *dev, double *host, int com
 // 16-bit slot into its block's list.  Built on the host, in parallel over blocks, once.
 // Returns false (and leaves the global-gather layout in use) if a halo does not fit.
 // ----------------------------------------------------------------------------------------
-static pdx_status build_halos(pdx_ctx *c, bool *ok)
+static pdx_status build_halos(pdx_ctx *c, int64_t entry_bytes, bool *ok)
 {
     *ok = false;
     const int64_t n = c->n, nba = c->nb_alloc;
@@ -484,7 +484,7 @@ static pdx_status build_halos(pdx_ctx *c, bool *ok)
     hipDeviceProp_t prop;
     PDX_HIP(c, hipGetDeviceProperties(&prop, c->device));
     const int64_t lds_limit = std::min<int64_t>((int64_t)prop.sharedMemPerBlock, 160 * 1024);
-    const int64_t slot_limit = std::min<int64_t>(PDX_W16_SLOT + 1, lds_limit / (int64_t)sizeof(double4));
+    const int64_t slot_limit = std::min<int64_t>(PDX_W16_SLOT + 1, lds_limit / entry_bytes);
     std::vector<int64_t> sp(n_slices + 1);
     std::vector<int32_t> cn(n);
     std::vector<uint32_t> w(nba);
@@ -795,9 +795,12 @@ extern "C" pdx_status pdx_build_bonds(pdx_ctx *c, int64_t *n_bonds_out)
         HIP_C(hipStreamSynchronize(c->stream));
     }
     cleanup();
-    if (c->kind == PDX_BB && !getenv("PDX_NO_LDS")) {
+    // LDS-staged layouts: BB (x|V per neighbour-table entry), OSB (x|V and c1, c2)
+    if ((c->kind == PDX_BB || c->kind == PDX_OSB) && !getenv("PDX_NO_LDS")) {
         bool ok = false;
-        PDX_TRY(build_halos(c, &ok));
+        const int64_t entry = c->kind == PDX_OSB ? (int64_t)(sizeof(double4) + sizeof(double2))
+                                                 : (int64_t)sizeof(double4);
+        PDX_TRY(build_halos(c, entry, &ok));
     }
     c->bonds_built = true;
     if (n_bonds_out) *n_bonds_out = nb;

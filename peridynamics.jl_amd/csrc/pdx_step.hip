@@ -536,6 +536,129 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_osb_pass2(pdx_step_args A)
     point_tail(A, p, ri, b, T);
 }
 
+// OSB on the LDS-staged layout (four lanes per point, as the BB step).  Pass 1 stages the
+// block's neighbour records x|V; pass 2 stages x|V and the neighbours' (c1, c2) behind
+// them (48 B per neighbour-table entry, A.halo_max entries of each).  Bond states live in
+// the 16-bit words.
+template <int S>
+__global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds1(pdx_step_args A)
+{
+    extern __shared__ double4 hrec[];
+    const int64_t k = pdx_xcd_block();
+    const int64_t p0 = k * PDX_BLOCK;
+    if (p0 >= A.n) return;  // whole workgroup
+    const int64_t h0 = A.halo_ptr[k];
+    const int H = (int)(A.halo_ptr[k + 1] - h0);
+    for (int h = threadIdx.x; h < H; h += PDX_BLOCK * S) hrec[h] = A.rec_in[A.halo[h0 + h]];
+    __syncthreads();
+    const int sub = threadIdx.x % S;
+    const int64_t p = p0 + threadIdx.x / S;
+    if (p >= A.n) return;  // all S lanes of the point together
+    const int c = A.cnt[p];
+    const int64_t b0 = pdx_bond0(A.slice_ptr, p);
+    uint16_t *const wbase = A.bw16 + b0;
+    const double *const Lbase = A.L + b0;
+    const double4 ri = A.rec_in[p];
+    double wvol = 0.0, Ssum = 0.0;
+    int na = 0;
+    for (int t = sub; t < c; t += S) {
+        const int64_t o = (int64_t)t * 64;
+        const uint32_t wd = wbase[o];
+        const double Lb = Lbase[o];
+        const double4 rj = hrec[wd & PDX_W16_SLOT];
+        const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
+        double rl;
+        const double l = pdx_sqrt_rsq((dx * dx + dy * dy) + dz * dz, rl);
+        const double e = l - Lb;
+        const double rL = pdx_rcp1(Lb);
+        const bool act = bond_fail16(A, wbase + o, wd, e, Lb, e * rL);
+        if (act) {
+            const double w = A.delta * rL;  // linear_kernel(delta, L)
+            wvol += (w * (Lb * Lb)) * rj.w;
+            Ssum += ((w * Lb) * e) * rj.w;
+            na++;
+        }
+    }
+#pragma unroll
+    for (int o = 1; o < S; o <<= 1) {
+        wvol += __shfl_xor(wvol, o);
+        Ssum += __shfl_xor(Ssum, o);
+        na += __shfl_xor(na, o);
+    }
+    if (sub != 0) return;
+    double c1 = 0.0, c2 = 0.0;
+    if (wvol != 0.0) {
+        const double dil = (3.0 / wvol) * Ssum;
+        c1 = (15.0 * A.G) / wvol;
+        c2 = dil * ((3.0 * A.K) / wvol - c1 / 3.0);
+    }
+    reinterpret_cast<double2 *>(A.pstate)[p] = make_double2(c1, c2);
+    if (A.flags & (PDX_F_STORE | PDX_F_FORCE_ONLY)) {
+        A.nact_out[p] = na;
+        A.dmg_out[p] = 1.0 - (double)na / (double)c;
+    }
+}
+
+template <int S>
+__global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds2(pdx_step_args A)
+{
+    extern __shared__ double4 hrec[];
+    double2 *const hc = reinterpret_cast<double2 *>(hrec + A.halo_max);
+    const int64_t k = pdx_xcd_block();
+    const int64_t p0 = k * PDX_BLOCK;
+    if (p0 >= A.n) return;  // whole workgroup
+    const int64_t h0 = A.halo_ptr[k];
+    const int H = (int)(A.halo_ptr[k + 1] - h0);
+    const double2 *const cst = reinterpret_cast<const double2 *>(A.pstate);
+    for (int h = threadIdx.x; h < H; h += PDX_BLOCK * S) {
+        const int64_t j = A.halo[h0 + h];
+        hrec[h] = A.rec_in[j];
+        hc[h] = cst[j];
+    }
+    __syncthreads();
+    const int sub = threadIdx.x % S;
+    const int64_t p = p0 + threadIdx.x / S;
+    if (p >= A.n) return;  // all S lanes of the point together
+    const int c = A.cnt[p];
+    const int64_t b0 = pdx_bond0(A.slice_ptr, p);
+    const uint16_t *const wbase = A.bw16 + b0;
+    const double *const Lbase = A.L + b0;
+    const double4 ri = A.rec_in[p];
+    const double2 ci = cst[p];
+    double bx = 0.0, by = 0.0, bz = 0.0;
+    for (int t = sub; t < c; t += S) {
+        const int64_t o = (int64_t)t * 64;
+        const uint32_t wd = wbase[o];
+        const double Lb = Lbase[o];
+        if (!(wd & PDX_W16_ACT)) continue;
+        const int slot = (int)(wd & PDX_W16_SLOT);
+        const double4 rj = hrec[slot];
+        const double2 cj = hc[slot];
+        const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
+        double rl;
+        const double l = pdx_sqrt_rsq((dx * dx + dy * dy) + dz * dz, rl);
+        const double w = A.delta * pdx_rcp1(Lb);
+        const double e = l - Lb;
+        const double fi = ci.y * Lb + ci.x * e;
+        const double fj = cj.y * Lb + cj.x * e;
+        const double s = (w * rl) * (fi * rj.w + fj * ri.w);
+        bx = __builtin_fma(s, dx, bx);
+        by = __builtin_fma(s, dy, by);
+        bz = __builtin_fma(s, dz, bz);
+    }
+#pragma unroll
+    for (int o = 1; o < S; o <<= 1) {
+        bx += __shfl_xor(bx, o);
+        by += __shfl_xor(by, o);
+        bz += __shfl_xor(bz, o);
+    }
+    if (sub != 0) return;
+    tail_ops T;
+    tail_load(A, p, T);
+    const double b[3] = {bx, by, bz};
+    point_tail(A, p, ri, b, T);
+}
+
 // ----------------------------------------------------------------------------------------
 // CPD (CMaterial, ZEMSilling).  3x3 matrices column-major as Julia: M[r + 3c].
 // ----------------------------------------------------------------------------------------
@@ -813,6 +936,7 @@ pdx_status pdx_fill_step_args(pdx_ctx *c, pdx_step_args *a, int64_t step, int fl
     a->bw16 = c->bw16;
     a->halo_ptr = c->halo_ptr;
     a->halo = c->halo;
+    a->halo_max = c->halo_max;
     a->L = c->L;
     a->Lf = c->Lf;
     a->rec_in = c->rec[c->cur];
@@ -883,6 +1007,20 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
         PDX_TRY(pdx_timed_end(c, "step"));
         break;
     case PDX_OSB:
+        if (c->lds_mode) {
+            const dim3 BL(PDX_BLOCK * PDX_LDS_LANES);
+            const size_t hm = (size_t)std::max(c->halo_max, 1);
+            PDX_TRY(pdx_timed_begin(c));
+            hipLaunchKernelGGL(k_osb_lds1<PDX_LDS_LANES>, G, BL, sizeof(double4) * hm, c->stream, a);
+            PDX_HIP(c, hipGetLastError());
+            PDX_TRY(pdx_timed_end(c, "pass1"));
+            PDX_TRY(pdx_timed_begin(c));
+            hipLaunchKernelGGL(k_osb_lds2<PDX_LDS_LANES>, G, BL, (sizeof(double4) + sizeof(double2)) * hm,
+                               c->stream, a);
+            PDX_HIP(c, hipGetLastError());
+            PDX_TRY(pdx_timed_end(c, "step"));
+            break;
+        }
         PDX_TRY(pdx_timed_begin(c));
         hipLaunchKernelGGL(k_osb_pass1, G, B, 0, c->stream, a);
         PDX_HIP(c, hipGetLastError());

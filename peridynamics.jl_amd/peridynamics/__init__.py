@@ -12,6 +12,7 @@ from .job import Job, submit
 from .materials import (BBMaterial, CMaterial, CriticalStretch, NeoHookePenalty, OSBMaterial,
                         SaintVenantKirchhoff, ZEMSilling, linear_kernel)
 from .point_generators import jitter, lattice_block, uniform_box
+from .postprocess import crack_tip, mode_i_plate
 from .solvers import NewtonKrylov, VelocityVerlet
 
 __all__ = [
@@ -20,5 +21,6 @@ __all__ = [
     "n_points", "no_failure", "point_set", "precrack", "velocity_bc", "MI355XBodyDataHandler",
     "Job", "submit", "BBMaterial", "CMaterial", "CriticalStretch", "NeoHookePenalty",
     "OSBMaterial", "SaintVenantKirchhoff", "ZEMSilling", "linear_kernel", "jitter",
-    "lattice_block", "uniform_box", "NewtonKrylov", "VelocityVerlet",
+    "lattice_block", "uniform_box", "NewtonKrylov", "VelocityVerlet", "crack_tip",
+    "mode_i_plate",
 ]

@@ -319,60 +319,59 @@ __global__ __launch_bounds__(256) void k_gnn_concat(int64_t n, int64_t npad, con
     x[4 * GF + f] = f2bf(Sg[f] / (float)n);
 }
 
-// Z = X W + b, H_out = bf16(H_in + relu(Z)); one wavefront computes 64 rows x 64 columns
-// with v_mfma_f32_16x16x32_bf16 (lane l: A row l%16, B column l%16, k block 8*(l/16);
-// D rows 4*(l/16)+i, column l%16).  Rows: npad (multiple of 256).
+// Z = X W + b, H_out = bf16(H_in + relu(Z)); a workgroup of four wavefronts computes 64
+// rows (one 64-point pooling block), each wavefront 16 rows x 64 columns with
+// v_mfma_f32_16x16x32_bf16 (lane l: A row l%16, B column l%16, k block 8*(l/16); D rows
+// 4*(l/16)+i, column l%16).  Rows: npad (a multiple of 256).  With S1 given, the block's
+// column sums of the new H (rows < n) are written for the next layer's pooling.
 template <bool TRAIN>
 __global__ __launch_bounds__(256) void k_gnn_gemm_fwd(int64_t n, const uint16_t *X, const uint16_t *WT,
                                                       const float *bias, const uint16_t *Hin,
                                                       uint16_t *Hout, float *Z, float *S1)
 {
-    const int lane = threadIdx.x & 63;
-    const int64_t row0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
+    __shared__ float bsum[4][GF];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t blk0 = (int64_t)blockIdx.x * 64;
+    const int64_t row0 = blk0 + 16 * wv;
     const int lr = lane & 15, lk = (lane >> 4) * 8;
-    f32x4 acc[4][4];
+    f32x4 acc[4];
 #pragma unroll
-    for (int r = 0; r < 4; r++)
-#pragma unroll
-        for (int c = 0; c < 4; c++) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int c = 0; c < 4; c++) acc[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
     for (int k0 = 0; k0 < GK; k0 += 32) {
-        bf16x8 a[4], b[4];
-#pragma unroll
-        for (int r = 0; r < 4; r++) a[r] = load_bf8(X + (row0 + 16 * r + lr) * GK + k0 + lk);
+        const bf16x8 a = load_bf8(X + (row0 + lr) * GK + k0 + lk);
+        bf16x8 b[4];
 #pragma unroll
         for (int c = 0; c < 4; c++) b[c] = load_bf8(WT + (int64_t)(16 * c + lr) * GK + k0 + lk);
 #pragma unroll
-        for (int r = 0; r < 4; r++)
-#pragma unroll
-            for (int c = 0; c < 4; c++)
-                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], b[c], acc[r][c], 0, 0, 0);
+        for (int c = 0; c < 4; c++) acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[c], acc[c], 0, 0, 0);
     }
-    // the wave's 64 rows are one 64-point pooling block: its column sums of the new H
-    // (rows < n) are the next layer's first-level pooled sums
     float bs[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int r = 0; r < 4; r++)
+    for (int c = 0; c < 4; c++)
 #pragma unroll
-        for (int c = 0; c < 4; c++)
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int64_t row = row0 + 16 * r + 4 * (lane >> 4) + i;
-                const int col = 16 * c + lr;
-                const float z = acc[r][c][i] + bias[col];
-                if (TRAIN) Z[row * GF + col] = z;
-                const uint16_t h = f2bf(bf2f(Hin[row * GF + col]) + fmaxf(z, 0.f));
-                Hout[row * GF + col] = h;
-                if (row < n) bs[c] += bf2f(h);
-            }
+        for (int i = 0; i < 4; i++) {
+            const int64_t row = row0 + 4 * (lane >> 4) + i;
+            const int col = 16 * c + lr;
+            const float z = acc[c][i] + bias[col];
+            if (TRAIN) Z[row * GF + col] = z;
+            const uint16_t h = f2bf(bf2f(Hin[row * GF + col]) + fmaxf(z, 0.f));
+            Hout[row * GF + col] = h;
+            if (row < n) bs[c] += bf2f(h);
+        }
     if (S1) {
 #pragma unroll
         for (int c = 0; c < 4; c++) {
             bs[c] += __shfl_xor(bs[c], 16);
             bs[c] += __shfl_xor(bs[c], 32);
         }
-        if (lane < 16 && row0 < n)
+        if (lane < 16)
 #pragma unroll
-            for (int c = 0; c < 4; c++) S1[(row0 / GB1) * (GF * 3) + 16 * c + lane] = bs[c];
+            for (int c = 0; c < 4; c++) bsum[wv][16 * c + lane] = bs[c];
+        __syncthreads();
+        if (threadIdx.x < GF && blk0 < n)
+            S1[(blk0 / GB1) * (GF * 3) + threadIdx.x] =
+                (bsum[0][threadIdx.x] + bsum[1][threadIdx.x]) + (bsum[2][threadIdx.x] + bsum[3][threadIdx.x]);
     }
 }
 
@@ -924,10 +923,10 @@ static pdx_status gnn_forward(pdx_ctx *c, const double *y, const double *x, doub
                            g->near_ptr, g->near, g->S1, g->S2, g->Sg, g->X[l]);
         float *S1n = l + 1 < GL ? g->S1 : nullptr;  // pooled sums of H[l + 1] for the next layer
         if (train)
-            hipLaunchKernelGGL(k_gnn_gemm_fwd<true>, dim3((unsigned)(np / 256)), dim3(256), 0, s, n,
+            hipLaunchKernelGGL(k_gnn_gemm_fwd<true>, dim3((unsigned)(np / 64)), dim3(256), 0, s, n,
                                g->X[l], g->wT[l], g->theta + OFF_B(l), g->H[l], g->H[l + 1], g->Z[l], S1n);
         else
-            hipLaunchKernelGGL(k_gnn_gemm_fwd<false>, dim3((unsigned)(np / 256)), dim3(256), 0, s, n,
+            hipLaunchKernelGGL(k_gnn_gemm_fwd<false>, dim3((unsigned)(np / 64)), dim3(256), 0, s, n,
                                g->X[l], g->wT[l], g->theta + OFF_B(l), g->H[l], g->H[l + 1], nullptr, S1n);
         PDX_HIP(c, hipGetLastError());
     }

@@ -249,12 +249,15 @@ def mode_i_leg(npxy, nz, steps):
     import numpy as np
     import peridynamics as pd
     res = {}
-    for prec in ("fp64", "mixed", "fp64_perturbed"):
+    perturbations = {"fp64_perturbed_2e-16": 2e-16, "fp64_perturbed_5e-14": 5e-14,
+                     "fp64_perturbed_5e-12": 5e-12}
+    for prec in ("fp64", "mixed", *perturbations):
         body, geo = pd.mode_i_plate(npxy=npxy, nz=nz, dx=2e-4, v=2.0)
-        if prec == "fp64_perturbed":
-            # sensitivity baseline: the FP64 run from positions moved by 1e-12 spacing
+        if prec in perturbations:
+            # sensitivity baseline: the FP64 run from reference positions moved by +-eps m
+            # (5e-14 m is the size of the FP32 rounding of a bond's displacement difference)
             rng = np.random.default_rng(1)
-            body.position += 2e-16 * rng.choice([-1.0, 1.0], size=body.position.shape)
+            body.position += perturbations[prec] * rng.choice([-1.0, 1.0], size=body.position.shape)
         dh = pd.MI355XBodyDataHandler(body, precision="mixed" if prec == "mixed" else "fp64")
         vv = pd.VelocityVerlet(steps=steps)
         dh.init_time_solver(vv)
@@ -273,7 +276,7 @@ def mode_i_leg(npxy, nz, steps):
                      "_nact": nact, "_dmg": dh.get("damage")}
         npnt, nb = body.n_points, dh.n_bonds
         dh.close()
-    a, b, c = res["fp64"], res["mixed"], res["fp64_perturbed"]
+    a, b = res["fp64"], res["mixed"]
     out = {"workload": (f"pre-cracked plate {npxy}x{npxy}x{nz}, dx 2e-4 m, crack 0.25 l, "
                         f"+-2 m/s, {steps} steps"),
            "points": npnt, "bonds": nb,
@@ -284,9 +287,11 @@ def mode_i_leg(npxy, nz, steps):
            "mixed_max_damage_difference": float(np.max(np.abs(a["_dmg"] - b["_dmg"]))),
            "mixed_crack_tip_dx_difference": (abs(a["crack_tip"][0] - b["crack_tip"][0]) / 2e-4
                                              if not np.isnan(a["crack_tip"][0]) else None),
-           "fp64_perturbed_broken_bond_count_difference": int(np.sum(np.abs(a["_nact"] - c["_nact"]))),
-           "fp64_perturbed_crack_tip": c["crack_tip"],
-           "perturbation": "reference positions moved by +-2e-16 m (1e-12 spacing)"}
+           "fp64_sensitivity": {k: {"perturbation_m": v,
+                                    "broken_bond_count_difference": int(np.sum(np.abs(a["_nact"] - res[k]["_nact"]))),
+                                    "max_damage_difference": float(np.max(np.abs(a["_dmg"] - res[k]["_dmg"]))),
+                                    "crack_tip": res[k]["crack_tip"]}
+                                for k, v in perturbations.items()}}
     return out
 
 

@@ -101,8 +101,9 @@ typedef struct pdx_nk_stats {
     int32_t newton_iters;
     int32_t krylov_iters;
     int32_t jv_evals;
-    int32_t precond_fallbacks; /* learned solves that stalled (linear residual not even halved)
-                                  and were redone with point-block Jacobi */
+    int32_t precond_fallbacks; /* learned solves that missed the tolerance without reducing the
+                                  linear residual tenfold, redone with the two-level
+                                  preconditioner (block Jacobi without a coarse space) */
     double residual_first, residual_final;
     double seconds;
 } pdx_nk_stats;

@@ -873,13 +873,15 @@ extern "C" pdx_status pdx_nk_step(pdx_ctx *c, int64_t n, int64_t n_steps, const 
             PDX_TRY(fgmres(c, k->t0, o->gmres_restart, o->gmres_maxiter, o->gmres_reltol,
                            o->gmres_abstol, PDX_PRE_LEARNED, scale, &its, &rel));
             S.krylov_iters += its;
-            // fallback: a learned solve that did not reach the tolerance and did not even
-            // halve the linear residual is redone with the point-block Jacobi preconditioner
+            // fallback: a learned solve that did not reach the tolerance and did not reduce
+            // the linear residual tenfold is redone with the preconditioner the network
+            // corrects (two-level; block Jacobi if the coarse space is unavailable)
             const double tol_lin = std::max(o->gmres_reltol, o->gmres_abstol / std::max(rn, 1e-300));
-            if (!(rel <= tol_lin * 1.0000001) && !(rel <= 0.5)) {
+            if (!(rel <= tol_lin * 1.0000001) && !(rel <= 0.1)) {
                 S.precond_fallbacks++;
+                const int fb = k->coarse_ready ? PDX_PRE_TWO_LEVEL : PDX_PRE_JACOBI;
                 PDX_TRY(fgmres(c, k->t0, o->gmres_restart, o->gmres_maxiter, o->gmres_reltol,
-                               o->gmres_abstol, PDX_PRE_JACOBI, scale, &its, &rel));
+                               o->gmres_abstol, fb, scale, &its, &rel));
                 S.krylov_iters += its;
             }
         } else {

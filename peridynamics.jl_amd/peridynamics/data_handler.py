@@ -208,7 +208,7 @@ class MI355XBodyDataHandler:
             if learned and n <= nk.learn_steps:
                 # plain (Jacobi) solves that supply the training pairs
                 self.ctx.gnn_collect(True, nk.krylov_pairs)
-                stats.append(self.nk_load_step(nk, n, precond="jacobi"))
+                stats.append(self.nk_load_step(nk, n, precond=nk.collect_precond))
                 self.ctx.gnn_collect(False)
                 if n == min(nk.learn_steps, nk.n_steps):
                     self.train_learned(nk)

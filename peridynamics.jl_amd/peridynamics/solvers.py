@@ -50,8 +50,8 @@ class NewtonKrylov:
     is absolute, newton_krylov.jl:311-314); precond = "none" | "jacobi" | "two_level" | "learned"
     ("two_level": block Jacobi plus a Galerkin coarse correction on Morton-block aggregates;
     "learned": the two-level preconditioner plus the trained message-passing network).
-    With "learned", load steps 1 .. learn_steps are solved with the Jacobi preconditioner
-    while their Newton and Krylov pairs are collected (at most krylov_pairs Krylov pairs),
+    With "learned", load steps 1 .. learn_steps are solved with collect_precond (block
+    Jacobi by default) while their Newton and Krylov pairs are collected (at most krylov_pairs Krylov pairs),
     the network is then trained (train_steps Adam steps of train_batch pairs, step size
     train_lr) and the remaining load steps use it."""
 
@@ -59,7 +59,7 @@ class NewtonKrylov:
                  perturbation_scale=1.0, gmres_maxiter=200, gmres_reltol=1e-4,
                  gmres_abstol=1e-8, gmres_restart=100, rel_tol=0.0, precond="none",
                  learn_steps=3, train_steps=400, train_batch=4, train_lr=1e-3,
-                 krylov_pairs=2048, train_seed=0):
+                 krylov_pairs=2048, train_seed=0, collect_precond="jacobi"):
         if time > 0 and steps > 0:
             raise _arg_error("specify either time or number of steps, not both!")
         if time < 0 and steps < 0:
@@ -98,6 +98,9 @@ class NewtonKrylov:
         self.train_lr = float(train_lr)
         self.krylov_pairs = int(krylov_pairs)
         self.train_seed = int(train_seed)
+        if collect_precond not in ("jacobi", "two_level"):
+            raise _arg_error("collect_precond must be 'jacobi' or 'two_level'")
+        self.collect_precond = collect_precond
 
     def __repr__(self):
         return f"NewtonKrylov(n_steps={self.n_steps}, stepsize={self.dt})"

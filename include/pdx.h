@@ -69,6 +69,10 @@ typedef int32_t pdx_status;
 #define PDX_FIELD_CAUCHY_STRESS 10 /* 9 x n (CPD) */
 #define PDX_FIELD_REF_POSITION 11 /* 3 x n, read only */
 #define PDX_FIELD_VOLUME 12       /* n, read only */
+#define PDX_FIELD_VELOCITY_HALF_OLD 13 /* 3 x n, DynamicRelaxation */
+#define PDX_FIELD_B_INT_OLD 14    /* 3 x n, DynamicRelaxation */
+#define PDX_FIELD_DENSITY_MATRIX 15 /* n, read only: the DynamicRelaxation Lambda of a point
+                                     (the reference stores it 3 x n with equal rows) */
 
 /* Point parameters of one material (StandardPointParameters, src/core/parameters.jl:107-126;
  * CPointParameters, src/physics/correspondence.jl:148-169).  The host derives them as
@@ -157,6 +161,12 @@ pdx_status pdx_force_density(pdx_ctx *ctx, double t, double dt);
  * Replaces the BC part of apply_boundary_conditions!, boundary_conditions.jl:212-226. */
 pdx_status pdx_set_velocity_bcs(pdx_ctx *ctx, int64_t n_points, const int32_t *points,
                                 const int32_t *src, int64_t n_src);
+/* Time-dependent force-density conditions on b_ext (forcedensity_bc! with f(t) or f(p, t),
+ * boundary_conditions.jl:559-575), applied at the start of every step as
+ * apply_boundary_conditions! does.  src indexes the per-step value rows whose length
+ * pdx_set_velocity_bcs set (call that first, with the total row length). */
+pdx_status pdx_set_force_density_bcs(pdx_ctx *ctx, int64_t n_points, const int32_t *points,
+                                     const int32_t *src);
 /* n_steps Velocity Verlet steps n = first_step ... first_step + n_steps - 1 with step size
  * dt (solve!, verlet_timestep!, update_vel_half!, update_disp_and_pos!,
  * _update_acc_and_vel!: velocity_verlet.jl:165-295).  bc_values holds n_steps rows of
@@ -164,6 +174,19 @@ pdx_status pdx_set_velocity_bcs(pdx_ctx *ctx, int64_t n_points, const int32_t *p
  * NaN force density returns PDX_ERR_NAN and the first such step in *nan_step_out. */
 pdx_status pdx_vv_run(pdx_ctx *ctx, int64_t first_step, int64_t n_steps, double dt,
                       const double *bc_values, int64_t *nan_step_out);
+
+/* --- dynamic relaxation ----------------------------------------------------------------- */
+/* DynamicRelaxation set-up (init_density_matrix!, dynamic_relaxation.jl:119-176): the
+ * per-point Lambda = damping_factor / 4 dt^2 n_bonds 5 pi delta^2 bc; zeroes
+ * velocity_half_old and b_int_old on the first call.  Bond-based bodies. */
+pdx_status pdx_dr_init(pdx_ctx *ctx, double dt, double damping_factor);
+/* Steps first .. first + n_steps - 1 of solve!(dh, ::DynamicRelaxation) (relaxation_timestep!,
+ * relaxation_first_step!, relaxation_step!, relaxation_updates!: dynamic_relaxation.jl:
+ * 219-350).  bc_values as for pdx_vv_run (velocity_half conditions, row k for step first+k). */
+pdx_status pdx_dr_run(pdx_ctx *ctx, int64_t first_step, int64_t n_steps, double dt,
+                      const double *bc_values, int64_t *nan_step_out);
+/* calc_damping (dynamic_relaxation.jl:262-323) of the current fields. */
+pdx_status pdx_dr_damping(pdx_ctx *ctx, double dt, double *cn_out);
 
 /* --- Newton-Krylov ---------------------------------------------------------------------- */
 /* Constrained dofs from displacement_bc! (PosSingleDimBC, boundary_conditions.jl:130-177,

@@ -294,6 +294,60 @@ class OracleSim:
                                      _p(self.vh), dth, _p(self.a), _p(self.v))
         return self
 
+    # ---------------------------------------------------------------- dynamic relaxation
+    def dr_density(self, dt, damping_factor):
+        """calc_density_matrix! for BondSystem: dynamic_relaxation.jl:166-176 (per point)."""
+        k = ((5.0 * np.pi) * (self.p["delta"] * self.p["delta"])) * self.p["bc"]
+        return ((((damping_factor * 1.0) / 4.0) * (dt * dt)) * self.n_neighbors.astype(np.float64)) * k
+
+    def dr_damping(self, dt, density, vh_old, bint_old):
+        """calc_damping: dynamic_relaxation.jl:262-323, the sums taken exactly (math.fsum)."""
+        import math
+        u = self.u.reshape(-1)
+        vh = self.vh.reshape(-1)
+        rho = np.repeat(density, 3)
+        m = vh != 0.0
+        db = self.b_int.reshape(-1)[m] - bint_old.reshape(-1)[m]
+        temp = (dt * rho[m]) * vh_old.reshape(-1)[m]
+        cn1 = math.fsum(((-(u[m] * u[m])) * db) / temp)
+        cn2 = math.fsum(u * u)
+        cn = 0.0
+        if cn2 != 0.0 and cn1 / cn2 > 0.0:
+            cn = 2.0 * math.sqrt(cn1 / cn2)
+        if cn > 2.0:
+            cn = 1.9
+        return cn
+
+    def dr_run(self, n_steps, dt, damping_factor=1.0, bcs=(), fd_bcs=(), state=None):
+        """solve!(dh, DynamicRelaxation): relaxation_timestep! for n = 1 .. n_steps
+        (dynamic_relaxation.jl:219-350).  bcs act on velocity_half, fd_bcs on b_ext, both as
+        ("sd", points, dim, f(t)) or ("pd", points, dim, f(X, t)).  Returns the state dict
+        (density, vh_old, b_int_old) to continue with."""
+        L = lib()
+        if state is None:
+            state = {"density": self.dr_density(dt, damping_factor),
+                     "vh_old": np.zeros((self.n, 3)), "bint_old": np.zeros((self.n, 3)), "n": 0}
+        rho = state["density"][:, None]
+        for _ in range(n_steps):
+            n = state["n"] + 1
+            t = n * dt
+            self._apply_bcs(self.vh, self.X, bcs, t)
+            self._apply_bcs(self.b_ext, self.X, fd_bcs, t)
+            L.orc_update_disp_and_pos(3 * self.n, _p(self.vh), dt, _p(self.u), _p(self.x))
+            self.force()
+            if n == 1:
+                vn = ((0.5 * dt) * (self.b_int + self.b_ext)) / rho
+            else:
+                cn = self.dr_damping(dt, state["density"], state["vh_old"], state["bint_old"])
+                a = (self.b_int + self.b_ext) / rho
+                vn = ((2.0 - cn * dt) * self.vh + (2.0 * dt) * a) / (2.0 + cn * dt)
+            self.vh[...] = vn
+            self.v[...] = 0.5 * (state["vh_old"] + vn)
+            state["vh_old"] = vn.copy()
+            state["bint_old"] = self.b_int.copy()
+            state["n"] = n
+        return state
+
     # ---------------------------------------------------------------- Newton-Krylov
     def residual(self, constrained):
         """calc_residual!: newton_krylov.jl:371-387."""

@@ -47,7 +47,7 @@ struct pdx_ctx {
     int device = 0;
     int precision = PDX_FP64;
     // bonds per iteration of the BB thread (tuning knob PDX_BB_UNROLL = 2 | 4)
-    int bb_unroll = 2;  // bonds per lane and unrolled iteration of the BB step (2 or 4)
+    int bb_unroll = 0;  // bonds per lane and unrolled iteration of the BB step: 2, 4, 0 = default
     hipStream_t stream = nullptr;
     std::string err;
 
@@ -101,6 +101,8 @@ struct pdx_ctx {
     int32_t *slot_src = nullptr;           // 3 per slot
     double *bcvals = nullptr;              // value table for a run
     int64_t n_src = 0, n_bcpts = 0, bcvals_cap = 0;
+    int32_t *fd_pts = nullptr, *fd_src = nullptr;  // force-density conditions (internal points)
+    int64_t n_fdpts = 0;
 
     unsigned long long *d_flags = nullptr; // [0] first NaN step, [1] duplicate flag, [2..3] pair
 
@@ -117,6 +119,11 @@ struct pdx_ctx {
     hipEvent_t open_ev = nullptr;       // begin event of the launch being timed
 
     pdx_nk *nk = nullptr;
+    // dynamic relaxation (pdx_dr.hip)
+    double *dr_vh_old = nullptr, *dr_bint_old = nullptr;  // 3n
+    double *dr_density = nullptr;                           // n (Lambda, the same in x, y, z)
+    double *dr_part = nullptr;                              // damping partial sums, cn
+    bool dr_ready = false;
     pdx_gnn *gnn = nullptr;
     int64_t dev_bytes = 0;
 };
@@ -299,6 +306,7 @@ pdx_status pdx_fill_step_args(pdx_ctx *c, pdx_step_args *a, int64_t step, int fl
                               const double *bcv_dev);
 pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a);  // one step of the material
 pdx_status pdx_launch_prologue(pdx_ctx *c, int64_t step, double dt, const double *bcv_dev);
+pdx_status pdx_apply_fd_bcs(pdx_ctx *c, const double *bcv_row);  // b_ext conditions of one step
 pdx_status pdx_check_flags(pdx_ctx *c, int64_t *nan_step);
 pdx_status pdx_reset_flags(pdx_ctx *c);
 

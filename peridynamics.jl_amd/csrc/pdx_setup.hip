@@ -68,6 +68,12 @@ void pdx_free_all(pdx_ctx *c)
 {
     pdx_nk_free(c);
     pdx_gnn_free(c);
+    FREE(c->fd_pts);
+    FREE(c->fd_src);
+    FREE(c->dr_vh_old);
+    FREE(c->dr_bint_old);
+    FREE(c->dr_density);
+    FREE(c->dr_part);
     FREE(c->rec[0]);
     FREE(c->rec[1]);
     FREE(c->Xr);
@@ -889,6 +895,9 @@ static pdx_status field_ptr(pdx_ctx *c, int field, double **p, int *comps, int *
     case PDX_FIELD_CAUCHY_STRESS: *p = c->cauchy; *comps = 9; *stride = 9; break;
     case PDX_FIELD_REF_POSITION: *p = (double *)c->Xr; *stride = 4; break;
     case PDX_FIELD_VOLUME: *p = ((double *)c->Xr) + 3; *comps = 1; *stride = 4; break;
+    case PDX_FIELD_VELOCITY_HALF_OLD: *p = c->dr_vh_old; break;
+    case PDX_FIELD_B_INT_OLD: *p = c->dr_bint_old; break;
+    case PDX_FIELD_DENSITY_MATRIX: *p = c->dr_density; *comps = 1; *stride = 1; break;
     default: return pdx_fail(c, PDX_ERR_ARG, "unknown field %d", field);
     }
     if (!*p) return pdx_fail(c, PDX_ERR_ARG, "field %d is not held by this material", field);
@@ -935,7 +944,7 @@ extern "C" pdx_status pdx_set_field(pdx_ctx *c, int32_t field, const double *hos
     if (!c->bonds_built) return pdx_fail(c, PDX_ERR_STATE, "pdx_build_bonds first");
     if (field == PDX_FIELD_REF_POSITION || field == PDX_FIELD_VOLUME ||
         field == PDX_FIELD_N_ACTIVE_BONDS || field == PDX_FIELD_DEFGRAD ||
-        field == PDX_FIELD_CAUCHY_STRESS)
+        field == PDX_FIELD_CAUCHY_STRESS || field == PDX_FIELD_DENSITY_MATRIX)
         return pdx_fail(c, PDX_ERR_ARG, "field %d is read only", field);
     PDX_HIP(c, hipSetDevice(c->device));
     double *p;
@@ -1048,6 +1057,67 @@ extern "C" pdx_status pdx_set_velocity_bcs(pdx_ctx *c, int64_t npts, const int32
                          hipMemcpyHostToDevice));
     c->n_bcpts = npts;
     c->n_src = n_src;
+    return PDX_OK;
+}
+
+// force-density conditions on b_ext (forcedensity_bc! with f(t) or f(p, t),
+// boundary_conditions.jl:559-575): dof 3 points[k] + d takes value src[3k + d] of the per-step
+// row (the same rows as the velocity conditions, whose length pdx_set_velocity_bcs set)
+extern "C" pdx_status pdx_set_force_density_bcs(pdx_ctx *c, int64_t npts, const int32_t *points,
+                                                const int32_t *src)
+{
+    if (!c) return PDX_ERR_ARG;
+    if (!c->bonds_built) return pdx_fail(c, PDX_ERR_STATE, "pdx_build_bonds first");
+    if (npts < 0 || (npts > 0 && (!points || !src)))
+        return pdx_fail(c, PDX_ERR_ARG, "invalid boundary condition table");
+    PDX_HIP(c, hipSetDevice(c->device));
+    std::vector<int32_t> pts(std::max<int64_t>(npts, 1), 0), ssrc(3 * std::max<int64_t>(npts, 1), -1);
+    std::vector<uint8_t> seen(c->n, 0);
+    for (int64_t k = 0; k < npts; k++) {
+        const int32_t i = points[k];
+        if (i < 0 || i >= c->n) return pdx_fail(c, PDX_ERR_ARG, "BC point out of range");
+        if (seen[i]) return pdx_fail(c, PDX_ERR_ARG, "BC point listed twice");
+        seen[i] = 1;
+        pts[k] = c->iperm[i];
+        for (int d = 0; d < 3; d++) {
+            const int32_t sidx = src[3 * k + d];
+            if (sidx < -1 || sidx >= c->n_src) return pdx_fail(c, PDX_ERR_ARG, "BC source out of range");
+            ssrc[3 * k + d] = sidx;
+        }
+    }
+    if (c->fd_pts) (void)hipFree(c->fd_pts);
+    if (c->fd_src) (void)hipFree(c->fd_src);
+    c->fd_pts = nullptr;
+    c->fd_src = nullptr;
+    PDX_TRY(pdx_alloc(c, &c->fd_pts, pts.size()));
+    PDX_TRY(pdx_alloc(c, &c->fd_src, ssrc.size()));
+    PDX_HIP(c, hipMemcpy(c->fd_pts, pts.data(), sizeof(int32_t) * pts.size(), hipMemcpyHostToDevice));
+    PDX_HIP(c, hipMemcpy(c->fd_src, ssrc.data(), sizeof(int32_t) * ssrc.size(), hipMemcpyHostToDevice));
+    c->n_fdpts = npts;
+    if (npts > 0) c->bext_nonzero = true;
+    return PDX_OK;
+}
+
+// b_ext of the constrained dofs from one row of condition values (non-finite: unchanged)
+__global__ void k_fd_bcs(int64_t npts, const int32_t *pts, const int32_t *src, const double *bcv,
+                         double *bext)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * npts) return;
+    const int64_t k = t / 3;
+    const int d = (int)(t - 3 * k);
+    const int32_t s = src[t];
+    if (s < 0) return;
+    const double val = bcv[s];
+    if (isfinite(val)) bext[3 * (int64_t)pts[k] + d] = val;
+}
+
+pdx_status pdx_apply_fd_bcs(pdx_ctx *c, const double *bcv_row)
+{
+    if (c->n_fdpts <= 0 || !bcv_row) return PDX_OK;
+    hipLaunchKernelGGL(k_fd_bcs, dim3((unsigned)((3 * c->n_fdpts + 255) / 256)), dim3(256), 0, c->stream,
+                       c->n_fdpts, c->fd_pts, c->fd_src, bcv_row, c->bext);
+    PDX_HIP(c, hipGetLastError());
     return PDX_OK;
 }
 

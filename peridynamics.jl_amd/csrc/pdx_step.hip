@@ -988,7 +988,9 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
             // four lanes per point (the halo table of a block limits the blocks per CU, so
             // the lanes, not the blocks, supply the parallelism that hides the bond loads)
             const dim3 BL(PDX_BLOCK * PDX_LDS_LANES);
-            const bool u4 = c->bb_unroll == 4;
+            // bonds per lane and unrolled iteration: the FP32 bond arithmetic of the mixed
+            // step is short enough that four loads in flight pay; FP64 is indifferent
+            const bool u4 = c->bb_unroll ? c->bb_unroll == 4 : c->precision == PDX_MIXED;
             if (c->precision == PDX_MIXED) {
                 if (u4) hipLaunchKernelGGL((k_bb_lds_mx<4, PDX_LDS_LANES>), G, BL, lds, c->stream, a);
                 else hipLaunchKernelGGL((k_bb_lds_mx<2, PDX_LDS_LANES>), G, BL, lds, c->stream, a);
@@ -1099,6 +1101,8 @@ extern "C" pdx_status pdx_vv_run(pdx_ctx *c, int64_t first, int64_t n_steps, dou
         const bool last = (k == n_steps - 1);
         const int flags = last ? PDX_F_STORE : PDX_F_HAS_NEXT;
         const double *next_bc = last ? c->bcvals : c->bcvals + c->n_src * (k + 1);
+        // b_ext conditions at t = (first + k) dt, used by this launch's acceleration
+        PDX_TRY(pdx_apply_fd_bcs(c, c->n_src > 0 ? c->bcvals + c->n_src * k : nullptr));
         pdx_step_args a;
         PDX_TRY(pdx_fill_step_args(c, &a, first + k, flags, dt, next_bc));
         PDX_TRY(pdx_launch_step(c, a));

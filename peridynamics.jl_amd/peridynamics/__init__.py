@@ -13,7 +13,7 @@ from .materials import (BBMaterial, CMaterial, CriticalStretch, NeoHookePenalty,
                         SaintVenantKirchhoff, ZEMSilling, linear_kernel)
 from .point_generators import jitter, lattice_block, uniform_box
 from .postprocess import crack_tip, mode_i_plate
-from .solvers import NewtonKrylov, VelocityVerlet
+from .solvers import DynamicRelaxation, NewtonKrylov, VelocityVerlet
 
 __all__ = [
     "ArgumentError", "DuplicatePointError", "NaNError", "NotConvergedError", "PdxError",
@@ -22,5 +22,5 @@ __all__ = [
     "Job", "submit", "BBMaterial", "CMaterial", "CriticalStretch", "NeoHookePenalty",
     "OSBMaterial", "SaintVenantKirchhoff", "ZEMSilling", "linear_kernel", "jitter",
     "lattice_block", "uniform_box", "NewtonKrylov", "VelocityVerlet", "crack_tip",
-    "mode_i_plate",
+    "mode_i_plate", "DynamicRelaxation",
 ]

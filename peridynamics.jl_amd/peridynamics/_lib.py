@@ -40,6 +40,9 @@ FIELDS = {
     "cauchy_stress": (10, 9),
     "ref_position": (11, 3),
     "volume": (12, 1),
+    "velocity_half_old": (13, 3),
+    "b_int_old": (14, 3),
+    "density_matrix": (15, 1),
 }
 
 
@@ -118,6 +121,10 @@ SIGNATURES = {
     "pdx_force_density": (_ST, [_P, _D, _D]),
     "pdx_set_velocity_bcs": (_ST, [_P, _I64, _P, _P, _I64]),
     "pdx_vv_run": (_ST, [_P, _I64, _I64, _D, _P, ctypes.POINTER(_I64)]),
+    "pdx_set_force_density_bcs": (_ST, [_P, _I64, _P, _P]),
+    "pdx_dr_init": (_ST, [_P, _D, _D]),
+    "pdx_dr_run": (_ST, [_P, _I64, _I64, _D, _P, ctypes.POINTER(_I64)]),
+    "pdx_dr_damping": (_ST, [_P, _D, ctypes.POINTER(_D)]),
     "pdx_set_displacement_bcs": (_ST, [_P, _I64, _P, _P, _P]),
     "pdx_jv": (_ST, [_P, _P, _P, _D]),
     "pdx_residual": (_ST, [_P, _P, ctypes.POINTER(_D)]),
@@ -262,6 +269,29 @@ class Context:
         points = np.ascontiguousarray(points, np.int32)
         src = np.ascontiguousarray(src, np.int32).reshape(-1)
         self.call("pdx_set_velocity_bcs", points.shape[0], ptr(points), ptr(src), int(n_src))
+
+    def set_force_density_bcs(self, points, src):
+        points = np.ascontiguousarray(points, np.int32)
+        src = np.ascontiguousarray(src, np.int32).reshape(-1)
+        self.call("pdx_set_force_density_bcs", points.shape[0], ptr(points), ptr(src))
+
+    def dr_init(self, dt, damping_factor):
+        self.call("pdx_dr_init", float(dt), float(damping_factor))
+
+    def dr_run(self, first_step, n_steps, dt, bc_values=None):
+        bv = None if bc_values is None else np.ascontiguousarray(bc_values, np.float64)
+        nan_step = _I64(-1)
+        st = self.lib.pdx_dr_run(self.h, int(first_step), int(n_steps), float(dt), ptr(bv),
+                                 ctypes.byref(nan_step))
+        if st == PDX_ERR_NAN:
+            msg = self.lib.pdx_last_error(self.h).decode()
+            raise NaNError(st, msg, step=nan_step.value, t=nan_step.value * dt)
+        check(self.h, st, "pdx_dr_run")
+
+    def dr_damping(self, dt):
+        cn = _D()
+        self.call("pdx_dr_damping", float(dt), ctypes.byref(cn))
+        return cn.value
 
     def vv_run(self, first_step, n_steps, dt, bc_values=None):
         bv = None if bc_values is None else np.ascontiguousarray(bc_values, np.float64)

@@ -51,37 +51,49 @@ class MI355XBodyDataHandler:
         for a, b, filt in body.point_sets_precracks:
             self.ctx.add_precrack(body.point_sets[a], body.point_sets[b], filt)
         self.n_bonds = self.ctx.build_bonds()
-        if any(f == "b_ext" for f, *_ in body.single_dim_bcs + body.posdep_single_dim_bcs):
-            raise _lib.PdxError(_lib.PDX_ERR_UNSUPPORTED,
-                                "force density boundary conditions are not provided yet")
-        self._compile_velocity_bcs()
+        self._compile_bcs()
         self.dt = None
 
     # ------------------------------------------------------------------ boundary conditions
-    def _compile_velocity_bcs(self):
+    def _compile_bcs(self):
+        """velocity_bc! and time-dependent forcedensity_bc! conditions: one source per
+        single-dim condition and per point of a position-dependent one, evaluated by the
+        host for every step (bc_values) and applied on the device."""
         body = self.body
-        sd = [(s, d, f) for fld, s, d, f in body.single_dim_bcs if fld == "velocity_half"]
-        pd = [(s, d, f) for fld, s, d, f in body.posdep_single_dim_bcs if fld == "velocity_half"]
-        sets = [body.point_sets[s] for s, _, _ in sd + pd]
-        pts = np.unique(np.concatenate(sets)) if sets else np.zeros(0, np.int64)
-        slot = {int(p): k for k, p in enumerate(pts)}
-        src = -np.ones((len(pts), 3), np.int32)
+        conds = {"velocity_half": ([], []), "b_ext": ([], [])}
+        for fld, s, d, f in body.single_dim_bcs:
+            if fld in conds:
+                conds[fld][0].append((s, d, f))
+        for fld, s, d, f in body.posdep_single_dim_bcs:
+            if fld in conds:
+                conds[fld][1].append((s, d, f))
         n_src = 0
         self._sd = []
         self._pd = []
-        for s, d, f in sd:
-            idx = [slot[int(p)] for p in body.point_sets[s]]
-            src[idx, d] = n_src
-            self._sd.append((n_src, f))
-            n_src += 1
-        for s, d, f in pd:
-            ps = body.point_sets[s]
-            idx = [slot[int(p)] for p in ps]
-            src[idx, d] = n_src + np.arange(len(ps))
-            self._pd.append((n_src, ps, f))
-            n_src += len(ps)
+        tables = {}
+        for fld, (sd, pd) in conds.items():
+            sets = [body.point_sets[s] for s, _, _ in sd + pd]
+            pts = np.unique(np.concatenate(sets)) if sets else np.zeros(0, np.int64)
+            slot = {int(p): k for k, p in enumerate(pts)}
+            src = -np.ones((len(pts), 3), np.int32)
+            for s, d, f in sd:
+                idx = [slot[int(p)] for p in body.point_sets[s]]
+                src[idx, d] = n_src
+                self._sd.append((n_src, f))
+                n_src += 1
+            for s, d, f in pd:
+                ps = body.point_sets[s]
+                idx = [slot[int(p)] for p in ps]
+                src[idx, d] = n_src + np.arange(len(ps))
+                self._pd.append((n_src, ps, f))
+                n_src += len(ps)
+            tables[fld] = (pts, src)
         self.n_src = n_src
+        pts, src = tables["velocity_half"]
         self.ctx.set_velocity_bcs(pts.astype(np.int32), src, n_src)
+        pts, src = tables["b_ext"]
+        if len(pts):
+            self.ctx.set_force_density_bcs(pts.astype(np.int32), src)
 
     def bc_values(self, steps, dt):
         """Values of every condition source for each step n (time n*dt)."""
@@ -146,6 +158,36 @@ class MI355XBodyDataHandler:
             if on_export is not None and freq and (n - 1) % freq == 0:
                 on_export(self, n - 1, (n - 1) * dt)
         return self
+
+    # ------------------------------------------------------------------ dynamic relaxation
+    def solve_dr(self, dr, freq=0, on_export=None):
+        """solve!(dh, dr, options): dynamic_relaxation.jl:92-107 (init_density_matrix!, then
+        relaxation_timestep! for n = 1 .. n_steps).  on_export(dh, n, t) every `freq` steps."""
+        dt = dr.dt
+        self.ctx.dr_init(dt, dr.damping)
+        n, remaining = 1, dr.n_steps
+        cap = max(1, (1 << 24) // max(self.n_src, 1))
+        if freq and freq > 0:
+            cap = min(cap, freq)
+        while remaining > 0:
+            k = min(cap, remaining)
+            if freq and freq > 0:
+                k = min(k, freq - ((n - 1) % freq))
+            vals = self.bc_values(range(n, n + k), dt) if self.n_src else None
+            try:
+                self.ctx.dr_run(n, k, dt, vals)
+            except NaNError as e:
+                raise NaNError(e.status, f"NaN in the force density at t={e.step * dt}, step {e.step}",
+                               step=e.step, t=e.step * dt) from None
+            n += k
+            remaining -= k
+            if on_export is not None and freq and (n - 1) % freq == 0:
+                on_export(self, n - 1, (n - 1) * dt)
+        return self
+
+    def calc_damping(self, dt):
+        """calc_damping(dh, dt) of the current fields: dynamic_relaxation.jl:262-323."""
+        return self.ctx.dr_damping(dt)
 
     # ------------------------------------------------------------------ Newton-Krylov
     PRECONDS = {"none": 0, "jacobi": 1, "learned": 2, "two_level": 3}

@@ -4,7 +4,7 @@ submit() runs the job on one GPU through MI355XBodyDataHandler; it is the siblin
 reference's _submit_threads.  Exported fields are kept in memory (job.results) instead of
 being written as VTK files."""
 from .data_handler import MI355XBodyDataHandler
-from .solvers import NewtonKrylov, VelocityVerlet
+from .solvers import DynamicRelaxation, NewtonKrylov, VelocityVerlet
 
 DEFAULT_FIELDS = ("displacement", "damage")
 
@@ -39,6 +39,9 @@ def submit(job, quiet=True):
         dh.init_time_solver(solver)
         dh.initialize(solver)
         dh.solve(solver, freq=job.freq if job.fields else 0, on_export=export)
+    elif isinstance(solver, DynamicRelaxation):
+        dh.calc_force_density(0.0, solver.dt)  # initialize!(dh, solver)
+        dh.solve_dr(solver, freq=job.freq if job.fields else 0, on_export=export)
     elif isinstance(solver, NewtonKrylov):
         dh.solve_nk(solver, on_export=export if job.fields else None)
     else:

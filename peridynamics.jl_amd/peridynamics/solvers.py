@@ -104,3 +104,22 @@ class NewtonKrylov:
 
     def __repr__(self):
         return f"NewtonKrylov(n_steps={self.n_steps}, stepsize={self.dt})"
+
+
+class DynamicRelaxation:
+    """DynamicRelaxation(; steps, stepsize=1.0, damping_factor=1.0):
+    dynamic_relaxation.jl:36-58 (bond-based bodies on the device)."""
+
+    def __init__(self, steps, stepsize=1.0, damping_factor=1.0):
+        if steps <= 0:
+            raise _arg_error("`steps` should be larger than zero!\n")
+        if stepsize <= 0:
+            raise _arg_error("`stepsize` should be larger than zero!\n")
+        if damping_factor <= 0:
+            raise _arg_error("`damping_factor` should be larger than zero!\n")
+        self.n_steps = int(steps)
+        self.dt = float(stepsize)
+        self.damping = float(damping_factor)
+
+    def __repr__(self):
+        return f"DynamicRelaxation(n_steps={self.n_steps}, Δt={self.dt}, Λ={self.damping})"

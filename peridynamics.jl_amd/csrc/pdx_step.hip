@@ -500,7 +500,8 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_osb_pass1(pdx_step_args A)
 }
 
 // OSB pass 2: force_density_point! (ordinary_state_based.jl:128-149) as a gather:
-// b_i = sum_j (w/l) dx_ij [ (c2_i L + c1_i (l-L)) V_j + (c2_j L + c1_j (l-L)) V_i ]
+// b_i = sum_j (w/l) dx_ij [ (c2_i L + c1_i (l-L)) + (c2_j L + c1_j (l-L)) ] V_j  (the term
+// point j scatters into i carries V_j, the volume of the point whose loop emits it)
 __global__ __launch_bounds__(PDX_BLOCK) void k_osb_pass2(pdx_step_args A)
 {
     const int64_t p = pdx_thread_point();
@@ -525,7 +526,7 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_osb_pass2(pdx_step_args A)
         const double e = l - Lb;
         const double fi = ci.y * Lb + ci.x * e;
         const double fj = cj.y * Lb + cj.x * e;
-        const double s = (w * rl) * (fi * rj.w + fj * ri.w);
+        const double s = (w * rl) * ((fi + fj) * rj.w);
         bx = __builtin_fma(s, dx, bx);
         by = __builtin_fma(s, dy, by);
         bz = __builtin_fma(s, dz, bz);
@@ -641,7 +642,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds2(pdx_step_args A)
         const double e = l - Lb;
         const double fi = ci.y * Lb + ci.x * e;
         const double fj = cj.y * Lb + cj.x * e;
-        const double s = (w * rl) * (fi * rj.w + fj * ri.w);
+        const double s = (w * rl) * ((fi + fj) * rj.w);
         bx = __builtin_fma(s, dx, bx);
         by = __builtin_fma(s, dy, by);
         bz = __builtin_fma(s, dz, bz);
@@ -881,7 +882,7 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_cpd_pass2(pdx_step_args A)
                 const double tz = wb * (dx[r] - FX);
                 const double tt = (((w * pj[9 + r]) * DX[0] + (w * pj[9 + r + 3]) * DX[1]) +
                                    (w * pj[9 + r + 6]) * DX[2]) + tz;
-                b3[r] += tt * ri.w;
+                b3[r] += tt * rj.w;
             }
         }
     }

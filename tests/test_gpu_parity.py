@@ -161,6 +161,27 @@ def test_stable_timestep_bit_exact(need_gpu, oracle):
 
 # ------------------------------------------------------------------ force density
 @pytest.mark.parametrize("kind", ["BB", "OSB", "CPD"])
+def test_force_density_nonuniform_volume(need_gpu, oracle, kind):
+    """Point volumes varying by +-30 %: the state-based gathers must weight the term a
+    neighbour j scatters into i by V_j, as the reference's b_int[j] -= p V_i of j's loop
+    does (ordinary_state_based.jl:145-146, correspondence.jl:248-249)."""
+    say("force nonuniform volume", kind)
+    E = 70e9 if kind == "BB" else 210e9
+    b = body_block(kind, 16, 14, 6, jit=0.05, E=E, Gc=None, rho=7850.0)
+    rng = np.random.default_rng(3)
+    b.volume *= rng.uniform(0.7, 1.3, b.n_points)
+    dh = pd.MI355XBodyDataHandler(b)
+    orc = oracle_for(oracle, b, kind)
+    x = b.xyz + displacement_field(b.xyz, 1e-6)
+    dh.set("position", x)
+    dh.calc_force_density(0.0, 1e-7)
+    orc.force(x)
+    err = scaled_error(dh.get("b_int"), orc)
+    say(f"  scaled error {err:.3e}")
+    assert err <= FP64_TOL
+
+
+@pytest.mark.parametrize("kind", ["BB", "OSB", "CPD"])
 @pytest.mark.parametrize("amp", [2e-7, 3e-6])
 def test_force_density_fp64(need_gpu, oracle, kind, amp):
     say("force fp64", kind, amp)

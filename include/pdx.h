@@ -127,6 +127,15 @@ pdx_status pdx_set_points(pdx_ctx *ctx, int64_t n, const double *X, const double
                           const uint8_t *fail_permit);
 /* One parameter set for the whole body (material!, material_parameters.jl:64-92). */
 pdx_status pdx_set_material(pdx_ctx *ctx, int32_t kind, const pdx_params *params);
+/* Several parameter sets on one body (material!(body, set_name; ...) called per point set,
+ * material_parameters.jl:64-92, kept as point_params + params_map; ParameterHandler,
+ * src/core/parameter_handler.jl:36-81).  point_set[i] in [0, n_sets) is the set of caller
+ * point i (0-based params_map).  Kinds PDX_BB and PDX_OSB; all sets share one horizon;
+ * at most 255 sets.  A bond uses the mean of both ends' bc (BB) or G and K (OSB) and the
+ * owner's eps_c (bond_based.jl:163-180, ordinary_state_based.jl:151-174).  n_sets = 1 is
+ * pdx_set_material.  Call after pdx_set_points and before pdx_build_bonds. */
+pdx_status pdx_set_material_sets(pdx_ctx *ctx, int32_t kind, int32_t n_sets, const pdx_params *sets,
+                                 const int32_t *point_set);
 /* precrack!(body, set_a, set_b; update_dmg), src/discretization/predefined_cracks.jl:106-112:
  * filter_bonds = 1 deletes the bonds (update_dmg = false), 0 breaks them at t = 0. */
 pdx_status pdx_add_precrack(pdx_ctx *ctx, int64_t na, const int32_t *set_a, int64_t nb,

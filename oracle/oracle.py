@@ -82,6 +82,14 @@ def _declare(L):
     L.orc_update_disp_and_pos.argtypes = [_I64, _P, _D, _P, _P]
     L.orc_update_acc_and_vel.restype = None
     L.orc_update_acc_and_vel.argtypes = [_I64, _P, _P, _D, _P, _D, _P, _P]
+    L.orc_bb_force_sets.restype = None
+    L.orc_bb_force_sets.argtypes = [_I64] + [_P] * 13
+    L.orc_osb_force_sets.restype = None
+    L.orc_osb_force_sets.argtypes = [_I64] + [_P] * 17
+    L.orc_min_timestep_sets.restype = _D
+    L.orc_min_timestep_sets.argtypes = [_I64, _P, _P, _P, _P, _P, _P]
+    L.orc_update_acc_and_vel_sets.restype = None
+    L.orc_update_acc_and_vel_sets.argtypes = [_I64, _P, _P, _P, _P, _D, _P, _P]
     L.orc_has_nan.restype = _INT
     L.orc_has_nan.argtypes = [_I64, _P]
     L.orc_max_threads.restype = _INT
@@ -164,12 +172,23 @@ class OracleSim:
     """
 
     def __init__(self, X, vol, kind, params, fail_permit=None, precracks=(), nthreads=0,
-                 bonds=None):
+                 bonds=None, point_set=None):
         self.X = _c(X, np.float64).reshape(-1, 3).copy()
         n = self.X.shape[0]
         self.n = n
         self.vol = _c(np.broadcast_to(np.asarray(vol, np.float64), (n,)), np.float64).copy()
         self.kind = kind
+        # several parameter sets: params is a list of dicts and point_set[i] the set of point
+        # i (ParameterHandler, parameter_handler.jl:36-81); self.pt holds per-point arrays
+        self.pt = None
+        if point_set is not None:
+            sets = [dict(q) for q in params]
+            ps = np.asarray(point_set, np.int64)
+            if kind not in ("BB", "OSB"):
+                raise ValueError("several parameter sets: BB and OSB")
+            self.pt = {k: np.ascontiguousarray([sets[m][k] for m in ps], np.float64)
+                       for k in ("bc", "epsc", "G", "K", "rho")}
+            params = sets[0]
         self.p = dict(params)
         self.nthreads = nthreads
         fp = np.ones(n, np.uint8) if fail_permit is None else np.asarray(fail_permit, np.uint8)
@@ -227,6 +246,20 @@ class OracleSim:
             self.x[...] = x
         p = self.p
         L = lib()
+        if self.pt is not None:
+            t = self.pt
+            if self.kind == "BB":
+                L.orc_bb_force_sets(self.n, _p(self.x), _p(self.vol), _p(self.row_ptr),
+                                    _p(self.nbr), _p(self.L), _p(self.bfp), _p(self.active),
+                                    _p(self.bond_length), _p(self.n_active), _p(self.damage),
+                                    _p(t["bc"]), _p(t["epsc"]), _p(self.b_int), _p(self.b_abs))
+            else:
+                L.orc_osb_force_sets(self.n, _p(self.X), _p(self.x), _p(self.vol),
+                                     _p(self.row_ptr), _p(self.nbr), _p(self.L), _p(self.bfp),
+                                     _p(self.active), _p(self.bond_length), _p(self.omega),
+                                     _p(self.n_active), _p(self.damage), _p(t["G"]),
+                                     _p(t["K"]), _p(t["epsc"]), _p(self.b_int), _p(self.b_abs))
+            return self.b_int
         if self.kind == "BB":
             L.orc_bb_force(self.n, _p(self.x), _p(self.vol), _p(self.row_ptr), _p(self.nbr),
                            _p(self.L), _p(self.bfp), _p(self.active), _p(self.bond_length),
@@ -252,6 +285,11 @@ class OracleSim:
         return self.b_int
 
     def stable_dt(self, safety_factor=0.7):
+        if self.pt is not None:
+            dt = lib().orc_min_timestep_sets(self.n, _p(self.vol), _p(self.row_ptr),
+                                             _p(self.nbr), _p(self.L), _p(self.pt["bc"]),
+                                             _p(self.pt["rho"]))
+            return dt * safety_factor
         dt = lib().orc_min_timestep(self.n, _p(self.vol), _p(self.row_ptr), _p(self.nbr),
                                     _p(self.L), self.p["bc"], self.p["rho"])
         return dt * safety_factor
@@ -290,14 +328,20 @@ class OracleSim:
             self.force()
             if check_nans and L.orc_has_nan(ndof, _p(self.b_int)):
                 raise FloatingPointError(f"NaN in b_int at t={t}, step {n}")
-            L.orc_update_acc_and_vel(ndof, _p(self.b_int), _p(self.b_ext), self.p["rho"],
-                                     _p(self.vh), dth, _p(self.a), _p(self.v))
+            if self.pt is not None:
+                L.orc_update_acc_and_vel_sets(self.n, _p(self.b_int), _p(self.b_ext),
+                                              _p(self.pt["rho"]), _p(self.vh), dth, _p(self.a),
+                                              _p(self.v))
+            else:
+                L.orc_update_acc_and_vel(ndof, _p(self.b_int), _p(self.b_ext), self.p["rho"],
+                                         _p(self.vh), dth, _p(self.a), _p(self.v))
         return self
 
     # ---------------------------------------------------------------- dynamic relaxation
     def dr_density(self, dt, damping_factor):
         """calc_density_matrix! for BondSystem: dynamic_relaxation.jl:166-176 (per point)."""
-        k = ((5.0 * np.pi) * (self.p["delta"] * self.p["delta"])) * self.p["bc"]
+        bc = self.pt["bc"] if self.pt is not None else self.p["bc"]  # get_params(paramsetup, i)
+        k = ((5.0 * np.pi) * (self.p["delta"] * self.p["delta"])) * bc
         return ((((damping_factor * 1.0) / 4.0) * (dt * dt)) * self.n_neighbors.astype(np.float64)) * k
 
     def dr_damping(self, dt, density, vh_old, bint_old):

@@ -441,6 +441,151 @@ void orc_osb_force(int64_t n, const double *X, const double *x, const double *vo
 }
 
 /* ------------------------------------------------------------------------------ */
+/* Several parameter sets on one body (ParameterHandler, src/core/parameter_handler.jl:36-81):
+ * per-point arrays hold each point's set.
+ * BB: calc_failure! with the owner's eps_c (bond_system.jl:309-326), force_density_point!
+ * with (bc_i + bc_j) / 2 (bond_based.jl:163-180):
+ *   b = ((((w * (bc_i + bc_j)) / 2) * eps) * V_j) * dx / l                                  */
+void orc_bb_force_sets(int64_t n, const double *x, const double *vol, const int64_t *row_ptr,
+                       const int32_t *nbr, const double *L, const uint8_t *bond_fp,
+                       uint8_t *active, double *bond_length, int64_t *n_active, double *damage,
+                       const double *bc_pt, const double *epsc_pt, double *b_int, double *b_abs)
+{
+    for (int64_t i = 0; i < n; i++) {
+        int64_t na = 0;
+        for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
+            int64_t j = nbr[b];
+            double dx = x[3 * j] - x[3 * i];
+            double dy = x[3 * j + 1] - x[3 * i + 1];
+            double dz = x[3 * j + 2] - x[3 * i + 2];
+            double l = sqrt((dx * dx + dy * dy) + dz * dz);
+            double eps = (l - L[b]) / L[b];
+            bond_length[b] = l;
+            if (eps > epsc_pt[i] && bond_fp[b]) active[b] = 0;
+            na += active[b];
+        }
+        n_active[i] = na;
+        damage[i] = 1.0 - (double)na / (double)(row_ptr[i + 1] - row_ptr[i]);
+        double bx = 0.0, by = 0.0, bz = 0.0, ax = 0.0, ay = 0.0, az = 0.0;
+        for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
+            int64_t j = nbr[b];
+            double dx = x[3 * j] - x[3 * i];
+            double dy = x[3 * j + 1] - x[3 * i + 1];
+            double dz = x[3 * j + 2] - x[3 * i + 2];
+            double l = bond_length[b];
+            double eps = (l - L[b]) / L[b];
+            double w = active[b] ? 1.0 : 0.0;
+            double s = (((w * (bc_pt[i] + bc_pt[j])) / 2.0) * eps) * vol[j];
+            double tx = (s * dx) / l, ty = (s * dy) / l, tz = (s * dz) / l;
+            bx += tx; by += ty; bz += tz;
+            ax += fabs(tx); ay += fabs(ty); az += fabs(tz);
+        }
+        b_int[3 * i] = bx; b_int[3 * i + 1] = by; b_int[3 * i + 2] = bz;
+        if (b_abs) { b_abs[3 * i] = ax; b_abs[3 * i + 1] = ay; b_abs[3 * i + 2] = az; }
+    }
+}
+
+/* OSB with several sets (ordinary_state_based.jl:151-174): each point's weighted volume and
+ * dilatation from its own bonds; per bond c1 = 15 (G_i + G_j) / (2 wvol_i),
+ * c2 = dil_i (3 (K_i + K_j) / (2 wvol_i) - c1 / 3); points with wvol = 0 are skipped. */
+void orc_osb_force_sets(int64_t n, const double *X, const double *x, const double *vol,
+                        const int64_t *row_ptr, const int32_t *nbr, const double *L,
+                        const uint8_t *bond_fp, uint8_t *active, double *bond_length,
+                        const double *omega, int64_t *n_active, double *damage,
+                        const double *G_pt, const double *K_pt, const double *epsc_pt,
+                        double *b_int, double *b_abs)
+{
+    for (int64_t k = 0; k < 3 * n; k++) b_int[k] = 0.0;
+    if (b_abs) for (int64_t k = 0; k < 3 * n; k++) b_abs[k] = 0.0;
+    for (int64_t i = 0; i < n; i++) {
+        int64_t na = 0;
+        for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
+            int64_t j = nbr[b];
+            double dx = x[3 * j] - x[3 * i];
+            double dy = x[3 * j + 1] - x[3 * i + 1];
+            double dz = x[3 * j + 2] - x[3 * i + 2];
+            double l = sqrt((dx * dx + dy * dy) + dz * dz);
+            bond_length[b] = l;
+            double eps = (l - L[b]) / L[b];
+            if (eps > epsc_pt[i] && bond_fp[b]) active[b] = 0;
+            na += active[b];
+        }
+        n_active[i] = na;
+        damage[i] = 1.0 - (double)na / (double)(row_ptr[i + 1] - row_ptr[i]);
+    }
+    for (int64_t i = 0; i < n; i++) {
+        double wvol = 0.0;
+        for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
+            int64_t j = nbr[b];
+            double Dx = X[3 * j] - X[3 * i];
+            double Dy = X[3 * j + 1] - X[3 * i + 1];
+            double Dz = X[3 * j + 2] - X[3 * i + 2];
+            double dsq = (Dx * Dx + Dy * Dy) + Dz * Dz;
+            double w = active[b] ? omega[b] : 0.0;
+            wvol += ((w * 1.0) * dsq) * vol[j];
+        }
+        if (wvol == 0.0) continue;
+        double dil = 0.0;
+        double c1d = 3.0 / wvol;
+        for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
+            int64_t j = nbr[b];
+            double w = active[b] ? omega[b] : 0.0;
+            dil += ((((w * 1.0) * c1d) * L[b]) * (bond_length[b] - L[b])) * vol[j];
+        }
+        for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
+            int64_t j = nbr[b];
+            double dx = x[3 * j] - x[3 * i];
+            double dy = x[3 * j + 1] - x[3 * i + 1];
+            double dz = x[3 * j + 2] - x[3 * i + 2];
+            double l = bond_length[b];
+            double c1 = 15.0 * (G_pt[i] + G_pt[j]) / (2.0 * wvol);
+            double c2 = dil * (3.0 * (K_pt[i] + K_pt[j]) / (2.0 * wvol) - c1 / 3.0);
+            double w = active[b] ? omega[b] : 0.0;
+            double s = ((w * 1.0) * (c2 * L[b] + c1 * (l - L[b]))) / l;
+            double px = s * dx, py = s * dy, pz = s * dz;
+            b_int[3 * i] += px * vol[j];
+            b_int[3 * i + 1] += py * vol[j];
+            b_int[3 * i + 2] += pz * vol[j];
+            b_int[3 * j] += -px * vol[i];
+            b_int[3 * j + 1] += -py * vol[i];
+            b_int[3 * j + 2] += -pz * vol[i];
+            if (b_abs) {
+                b_abs[3 * i] += fabs(px * vol[j]); b_abs[3 * i + 1] += fabs(py * vol[j]);
+                b_abs[3 * i + 2] += fabs(pz * vol[j]);
+                b_abs[3 * j] += fabs(px * vol[i]); b_abs[3 * j + 1] += fabs(py * vol[i]);
+                b_abs[3 * j + 2] += fabs(pz * vol[i]);
+            }
+        }
+    }
+}
+
+/* calc_timestep with per-point parameters (velocity_verlet.jl:155-163, bond_system.jl:269-277) */
+double orc_min_timestep_sets(int64_t n, const double *vol, const int64_t *row_ptr,
+                             const int32_t *nbr, const double *L, const double *bc_pt,
+                             const double *rho_pt)
+{
+    double mn = INFINITY;
+    for (int64_t i = 0; i < n; i++) {
+        double dtsum = 0.0;
+        for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) dtsum += vol[nbr[b]] * bc_pt[i] / L[b];
+        double dt = sqrt(2.0 * rho_pt[i] / dtsum);
+        if (dt < mn) mn = dt;
+    }
+    return mn;
+}
+
+/* update_acc_and_vel! with per-point rho (velocity_verlet.jl:262-275) */
+void orc_update_acc_and_vel_sets(int64_t n, const double *b_int, const double *b_ext,
+                                 const double *rho_pt, const double *vh, double dth, double *a,
+                                 double *v)
+{
+    for (int64_t k = 0; k < 3 * n; k++) {
+        a[k] = (b_int[k] + b_ext[k]) / rho_pt[k / 3];
+        v[k] = vh[k] + a[k] * dth;
+    }
+}
+
+/* ------------------------------------------------------------------------------ */
 /* 3x3 helpers, column-major like Julia (A[r + 3c])                                */
 /* ------------------------------------------------------------------------------ */
 static void m3_mul(const double *A, const double *B, double *C)

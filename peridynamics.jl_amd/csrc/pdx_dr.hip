@@ -21,10 +21,11 @@
 #define DR_BLOCKS 1024
 
 __global__ void k_dr_density(int64_t n, const int32_t *cnt, double lam, double dt, double delta,
-                             double bc, double *density)
+                             double bc, const pdx_pset *ps, const uint8_t *pid, double *density)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (ps) bc = ps[pid[i]].bc;  // the point's own set (get_params(paramsetup, i))
     const double k = ((5.0 * M_PI) * (delta * delta)) * bc;
     density[i] = ((((lam * 1.0) / 4.0) * (dt * dt)) * (double)cnt[i]) * k;
 }
@@ -187,7 +188,7 @@ extern "C" pdx_status pdx_dr_init(pdx_ctx *c, double dt, double damping_factor)
     PDX_HIP(c, hipSetDevice(c->device));
     PDX_TRY(dr_ensure(c));
     hipLaunchKernelGGL(k_dr_density, grid_of(c->n), dim3(256), 0, c->stream, c->n, c->cnt, damping_factor,
-                       dt, c->P.delta, c->P.bc, c->dr_density);
+                       dt, c->P.delta, c->P.bc, c->multi() ? c->d_psets : nullptr, c->d_pid, c->dr_density);
     PDX_HIP(c, hipGetLastError());
     PDX_HIP(c, hipStreamSynchronize(c->stream));
     c->dr_ready = true;

@@ -24,7 +24,8 @@
 #define PDX_BW_ACT 0x40000000u
 #define PDX_BW_FP 0x80000000u
 #define PDX_MAX_POINTS ((int64_t)1 << 30)
-#define PDX_W16_SLOT 0x3fffu
+#define PDX_W16_SLOT 0x1fffu  // halo slot: <= 8192 entries (160 KiB LDS / 32 B per entry = 5120)
+#define PDX_W16_RACT 0x2000u  // the reverse bond (neighbour -> owner) is active
 #define PDX_W16_ACT 0x4000u
 #define PDX_W16_FP 0x8000u
 
@@ -41,6 +42,12 @@ struct pdx_crack {
 };
 
 struct pdx_nk;   // Newton-Krylov workspace (pdx_nk.hip)
+
+// one parameter set on the device (bodies with several material! sets)
+struct pdx_pset {
+    double bc, epsc, epsc_lo, epsc_hi, rho, delta, G, K;
+};
+#define PDX_MAX_PSETS 255
 struct pdx_gnn;  // learned preconditioner (pdx_gnn.hip)
 
 struct pdx_ctx {
@@ -62,6 +69,13 @@ struct pdx_ctx {
 
     int kind = -1;
     pdx_params P{};
+    // several parameter sets: the sets, each caller point's set, their device copies
+    // (internal order); empty psets = the single set P
+    std::vector<pdx_params> psets;
+    std::vector<uint8_t> pset_of;
+    pdx_pset *d_psets = nullptr;
+    uint8_t *d_pid = nullptr;
+    bool multi() const { return psets.size() > 1; }
     std::vector<pdx_crack> cracks;
     bool bonds_built = false;
 
@@ -282,6 +296,8 @@ struct pdx_step_args {
     const int64_t *halo_ptr;
     const int32_t *halo;
     int32_t halo_max;
+    const pdx_pset *psets;  // several parameter sets: the table and each point's set
+    const uint8_t *pid;
     const double *L;
     const float *Lf;
     const double4 *rec_in;

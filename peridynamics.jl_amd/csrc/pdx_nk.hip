@@ -225,7 +225,8 @@ __global__ void k_nk_update(int64_t n, const double *du, const uint8_t *cmask, d
 __global__ __launch_bounds__(256) void k_bb_block_jacobi(int64_t n, const int64_t *slice_ptr,
                                                          const int32_t *cnt, const uint32_t *bw,
                                                          const double *L, const double4 *rec,
-                                                         double bc, const uint8_t *cmask,
+                                                         double bc, const pdx_pset *ps,
+                                                         const uint8_t *pid, const uint8_t *cmask,
                                                          double *dinv)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -243,7 +244,8 @@ __global__ __launch_bounds__(256) void k_bb_block_jacobi(int64_t n, const int64_
         const double nx = dx / l, ny = dy / l, nz = dz / l;
         const double Lb = L[b];
         const double eps = (l - Lb) / Lb;
-        const double a = bc * rj.w / Lb, g = bc * rj.w * eps / l;
+        const double bcb = ps ? (ps[pid[i]].bc + ps[pid[wd & PDX_BW_NBR]].bc) / 2.0 : bc;
+        const double a = bcb * rj.w / Lb, g = bcb * rj.w * eps / l;
         D[0] += (a - g) * nx * nx + g;
         D[1] += (a - g) * ny * nx;
         D[2] += (a - g) * nz * nx;
@@ -412,6 +414,7 @@ __global__ __launch_bounds__(256) void k_coarse_assemble(int64_t n, int agg, int
                                                          const int64_t *slice_ptr, const int32_t *cnt,
                                                          const uint32_t *bw, const double *L,
                                                          const double4 *rec, double bc,
+                                                         const pdx_pset *ps, const uint8_t *pid,
                                                          const uint8_t *cmask, double *Ac)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -441,7 +444,8 @@ __global__ __launch_bounds__(256) void k_coarse_assemble(int64_t n, int agg, int
         const double nv[3] = {dx / l, dy / l, dz / l};
         const double Lb = L[b];
         const double eps = (l - Lb) / Lb;
-        const double a = ri.w * bc * rj.w / Lb, g = ri.w * bc * rj.w * eps / l;
+        const double bcb = ps ? (ps[pid[i]].bc + ps[pid[j]].bc) / 2.0 : bc;  // parameter sets
+        const double a = ri.w * bcb * rj.w / Lb, g = ri.w * bcb * rj.w * eps / l;
         const int64_t J = j / agg;
         if (J != Jc) {
             flush();
@@ -563,7 +567,8 @@ static pdx_status build_coarse(pdx_ctx *c)
     const int64_t mc = k->mc, mm = mc * mc;
     PDX_HIP(c, hipMemsetAsync(k->Ac, 0, sizeof(double) * mm, c->stream));
     hipLaunchKernelGGL(k_coarse_assemble, grid1(c->n), dim3(256), 0, c->stream, c->n, k->agg, mc,
-                       c->slice_ptr, c->cnt, c->bw, c->L, c->rec[c->cur], c->P.bc, k->cmask, k->Ac);
+                       c->slice_ptr, c->cnt, c->bw, c->L, c->rec[c->cur], c->P.bc,
+                       c->multi() ? c->d_psets : nullptr, c->d_pid, k->cmask, k->Ac);
     hipLaunchKernelGGL(k_coarse_fix, grid1(mc), dim3(256), 0, c->stream, mc, k->Ac);
     PDX_HIP(c, hipGetLastError());
     for (int64_t q = 0; q < mc; q++) {
@@ -635,7 +640,8 @@ static pdx_status build_jacobi(pdx_ctx *c)
 {
     if (c->kind != PDX_BB) return pdx_fail(c, PDX_ERR_UNSUPPORTED, "block Jacobi is built for BBMaterial");
     hipLaunchKernelGGL(k_bb_block_jacobi, grid1(c->n), dim3(256), 0, c->stream, c->n, c->slice_ptr,
-                       c->cnt, c->bw, c->L, c->rec[c->cur], c->P.bc, c->nk->cmask, c->nk->dinv);
+                       c->cnt, c->bw, c->L, c->rec[c->cur], c->P.bc,
+                       c->multi() ? c->d_psets : nullptr, c->d_pid, c->nk->cmask, c->nk->dinv);
     PDX_HIP(c, hipGetLastError());
     return PDX_OK;
 }

@@ -68,6 +68,8 @@ void pdx_free_all(pdx_ctx *c)
 {
     pdx_nk_free(c);
     pdx_gnn_free(c);
+    FREE(c->d_psets);
+    FREE(c->d_pid);
     FREE(c->fd_pts);
     FREE(c->fd_src);
     FREE(c->dr_vh_old);
@@ -194,6 +196,55 @@ extern "C" pdx_status pdx_set_material(pdx_ctx *c, int32_t kind, const pdx_param
         return pdx_fail(c, PDX_ERR_STATE, "material kind changed after the bonds were built");
     c->kind = kind;
     c->P = *p;
+    return PDX_OK;
+}
+
+extern "C" pdx_status pdx_set_material_sets(pdx_ctx *c, int32_t kind, int32_t n_sets,
+                                            const pdx_params *sets, const int32_t *point_set)
+{
+    if (!c || !sets || n_sets <= 0) return PDX_ERR_ARG;
+    if (n_sets > PDX_MAX_PSETS) return pdx_fail(c, PDX_ERR_UNSUPPORTED, "at most %d parameter sets", PDX_MAX_PSETS);
+    if (c->n <= 0) return pdx_fail(c, PDX_ERR_STATE, "pdx_set_points first");
+    if (n_sets > 1 && !point_set) return PDX_ERR_ARG;
+    PDX_TRY(pdx_set_material(c, kind, &sets[0]));
+    c->psets.clear();
+    c->pset_of.clear();
+    if (n_sets == 1) return PDX_OK;
+    if (kind == PDX_CPD)
+        return pdx_fail(c, PDX_ERR_UNSUPPORTED, "several parameter sets are built for BBMaterial and OSBMaterial");
+    for (int32_t k = 0; k < n_sets; k++) {
+        if (!(sets[k].delta > 0) || !(sets[k].rho > 0))
+            return pdx_fail(c, PDX_ERR_ARG, "horizon and rho have to be larger than zero");
+        if (sets[k].delta != sets[0].delta)
+            return pdx_fail(c, PDX_ERR_UNSUPPORTED, "the parameter sets of a body must share the horizon");
+    }
+    std::vector<uint8_t> of(c->n);
+    for (int64_t i = 0; i < c->n; i++) {
+        if (point_set[i] < 0 || point_set[i] >= n_sets)
+            return pdx_fail(c, PDX_ERR_ARG, "parameter set of point %lld out of range", (long long)i);
+        of[i] = (uint8_t)point_set[i];
+    }
+    c->psets.assign(sets, sets + n_sets);
+    c->pset_of = of;
+    return PDX_OK;
+}
+
+// device copies of the parameter sets and of each point's set (internal order)
+static pdx_status upload_psets(pdx_ctx *c)
+{
+    if (!c->multi()) return PDX_OK;
+    std::vector<pdx_pset> t(c->psets.size());
+    for (size_t k = 0; k < t.size(); k++) {
+        const pdx_params &q = c->psets[k];
+        t[k] = {q.bc, q.epsc, q.epsc - 1e-11 * fabs(q.epsc), q.epsc + 1e-11 * fabs(q.epsc), q.rho,
+                q.delta, q.G, q.K};
+    }
+    std::vector<uint8_t> pid(c->n);
+    for (int64_t i = 0; i < c->n; i++) pid[c->iperm[i]] = c->pset_of[i];
+    PDX_TRY(pdx_alloc(c, &c->d_psets, t.size()));
+    PDX_TRY(pdx_alloc(c, &c->d_pid, c->n));
+    PDX_HIP(c, hipMemcpy(c->d_psets, t.data(), sizeof(pdx_pset) * t.size(), hipMemcpyHostToDevice));
+    PDX_HIP(c, hipMemcpy(c->d_pid, pid.data(), c->n, hipMemcpyHostToDevice));
     return PDX_OK;
 }
 
@@ -490,7 +541,8 @@ static pdx_status build_halos(pdx_ctx *c, int64_t entry_bytes, bool *ok)
     hipDeviceProp_t prop;
     PDX_HIP(c, hipGetDeviceProperties(&prop, c->device));
     const int64_t lds_limit = std::min<int64_t>((int64_t)prop.sharedMemPerBlock, 160 * 1024);
-    const int64_t slot_limit = std::min<int64_t>(PDX_W16_SLOT + 1, lds_limit / entry_bytes);
+    // 64 bytes of slack for the alignment pad of a trailing byte table
+    const int64_t slot_limit = std::min<int64_t>(PDX_W16_SLOT + 1, (lds_limit - 64) / entry_bytes);
     std::vector<int64_t> sp(n_slices + 1);
     std::vector<int32_t> cn(n);
     std::vector<uint32_t> w(nba);
@@ -522,7 +574,7 @@ static pdx_status build_halos(pdx_ctx *c, int64_t entry_bytes, bool *ok)
                     const int64_t b = b0 + 64 * (int64_t)t;
                     const int32_t j = (int32_t)(w[b] & PDX_BW_NBR);
                     const uint32_t slot = (uint32_t)(std::lower_bound(tmp.begin(), tmp.end(), j) - tmp.begin());
-                    w16[b] = (uint16_t)(slot | ((w[b] & PDX_BW_ACT) ? PDX_W16_ACT : 0u) |
+                    w16[b] = (uint16_t)(slot | ((w[b] & PDX_BW_ACT) ? PDX_W16_ACT | PDX_W16_RACT : 0u) |
                                         ((w[b] & PDX_BW_FP) ? PDX_W16_FP : 0u));
                 }
             }
@@ -804,10 +856,15 @@ extern "C" pdx_status pdx_build_bonds(pdx_ctx *c, int64_t *n_bonds_out)
     // LDS-staged layouts: BB (x|V per neighbour-table entry), OSB (x|V and c1, c2)
     if ((c->kind == PDX_BB || c->kind == PDX_OSB) && !getenv("PDX_NO_LDS")) {
         bool ok = false;
-        const int64_t entry = c->kind == PDX_OSB ? (int64_t)(sizeof(double4) + sizeof(double2))
-                                                 : (int64_t)sizeof(double4);
+        const int64_t entry = (c->kind == PDX_OSB ? (int64_t)(sizeof(double4) + sizeof(double2))
+                                                  : (int64_t)sizeof(double4)) +
+                              (c->multi() ? 1 : 0);  // + the neighbour's parameter set
         PDX_TRY(build_halos(c, entry, &ok));
     }
+    PDX_TRY(upload_psets(c));
+    if (c->multi() && !c->lds_mode)
+        return pdx_fail(c, PDX_ERR_UNSUPPORTED,
+                        "several parameter sets need the LDS-staged bond layout (a neighbour table did not fit)");
     c->bonds_built = true;
     if (n_bonds_out) *n_bonds_out = nb;
     return PDX_OK;
@@ -969,7 +1026,9 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_timestep(int64_t n, const int64_t
                                                          const int32_t *cnts,
                                                          const uint32_t *bw, const double *L,
                                                          const double4 *Xr, const int32_t *perm,
-                                                         double bc, double rho, double *dt)
+                                                         double bc, double rho,
+                                                         const pdx_pset *psets, const uint8_t *pid,
+                                                         double *dt)
 {
     __shared__ double term[PDX_WAVES_PER_BLOCK][PDX_FILL_CAP];
     __shared__ int32_t key[PDX_WAVES_PER_BLOCK][PDX_FILL_CAP];
@@ -978,6 +1037,10 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_timestep(int64_t n, const int64_t
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int64_t b0 = slice_ptr[i >> 6] + (i & 63);
     const int cnt = cnts[i];  // <= PDX_FILL_CAP (checked when built)
+    if (psets) {  // the point's own parameter set (get_params(chunk, point_id))
+        bc = psets[pid[i]].bc;
+        rho = psets[pid[i]].rho;
+    }
     for (int k = lane; k < cnt; k += 64) {
         const uint32_t j = bw[b0 + (int64_t)k * 64] & PDX_BW_NBR;
         key[w][k] = perm[j];
@@ -1011,7 +1074,7 @@ extern "C" pdx_status pdx_stable_timestep(pdx_ctx *c, double safety, double *dt_
     double *tmp = c->stage;
     hipLaunchKernelGGL(k_timestep, dim3((unsigned)pdx_wave_grid(c->n)), dim3(PDX_BLOCK), 0,
                        c->stream, c->n, c->slice_ptr, c->cnt, c->bw, c->L, c->Xr, c->d_perm, c->P.bc,
-                       c->P.rho, tmp);
+                       c->P.rho, c->multi() ? c->d_psets : nullptr, c->multi() ? c->d_pid : nullptr, tmp);
     PDX_HIP(c, hipGetLastError());
     std::vector<double> dt(c->n);
     PDX_HIP(c, hipMemcpyAsync(dt.data(), tmp, sizeof(double) * c->n, hipMemcpyDeviceToHost,

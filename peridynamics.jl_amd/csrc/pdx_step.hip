@@ -74,7 +74,7 @@ __device__ __forceinline__ void tail_load(const pdx_step_args &A, int64_t p, tai
 }
 
 __device__ __forceinline__ void point_tail(const pdx_step_args &A, int64_t p, double4 ri,
-                                           const double b[3], const tail_ops &T)
+                                           const double b[3], const tail_ops &T, double rho)
 {
     const int flags = A.flags;
     if (isnan(b[0]) || isnan(b[1]) || isnan(b[2]))
@@ -89,7 +89,7 @@ __device__ __forceinline__ void point_tail(const pdx_step_args &A, int64_t p, do
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         const int64_t k = 3 * p + d;
-        const double ad = (b[d] + T.be[d]) / A.rho;
+        const double ad = (b[d] + T.be[d]) / rho;
         const double vd = T.vh[d] + ad * A.dth;
         if (flags & PDX_F_STORE) {
             A.v_out[k] = vd;
@@ -260,7 +260,7 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_bb_step(pdx_step_args A)
     tail_ops T;
     tail_load(A, p, T);
     const double b[3] = {bx, by, bz};
-    point_tail(A, p, ri, b, T);
+    point_tail(A, p, ri, b, T, A.rho);
 }
 
 // ----------------------------------------------------------------------------------------
@@ -272,12 +272,12 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_bb_step(pdx_step_args A)
 // Arithmetic as k_bb_step (FP64); MIXED takes the stretch from the FP64 difference
 // d2 - L^2 (fused, one rounding) and does the rest in FP32 with FP64 sums.
 // ----------------------------------------------------------------------------------------
-__device__ __forceinline__ bool bond_fail16(const pdx_step_args &A, uint16_t *wp, uint32_t wd,
-                                            double e, double Lb, double eps)
+__device__ __forceinline__ bool bond_fail16(uint16_t *wp, uint32_t wd, double e, double Lb, double eps,
+                                            double epsc, double epsc_lo, double epsc_hi)
 {
     bool act = (wd & PDX_W16_ACT) != 0;
-    if (act && (wd & PDX_W16_FP) && eps >= A.epsc_lo) {
-        const bool brk = (eps > A.epsc_hi) ? true : ((e / Lb) > A.epsc);
+    if (act && (wd & PDX_W16_FP) && eps >= epsc_lo) {
+        const bool brk = (eps > epsc_hi) ? true : ((e / Lb) > epsc);
         if (brk) {
             act = false;
             *wp = (uint16_t)(wd & ~PDX_W16_ACT);
@@ -286,7 +286,9 @@ __device__ __forceinline__ bool bond_fail16(const pdx_step_args &A, uint16_t *wp
     return act;
 }
 
-template <int U, int S>
+// MULTI: several parameter sets (force_density_point! with a ParameterHandler,
+// bond_based.jl:163-180): eps_c of the row's point, (bc_i + bc_j) / 2 per bond, rho_i
+template <int U, int S, bool MULTI>
 __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds(pdx_step_args A)
 {
     extern __shared__ double4 hrec[];
@@ -295,7 +297,12 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds(pdx_step_args A)
     if (p0 >= A.n) return;  // whole workgroup
     const int64_t h0 = A.halo_ptr[k];
     const int H = (int)(A.halo_ptr[k + 1] - h0);
-    for (int h = threadIdx.x; h < H; h += PDX_BLOCK * S) hrec[h] = A.rec_in[A.halo[h0 + h]];
+    uint8_t *const hpid = reinterpret_cast<uint8_t *>(hrec + A.halo_max);
+    for (int h = threadIdx.x; h < H; h += PDX_BLOCK * S) {
+        const int32_t j = A.halo[h0 + h];
+        hrec[h] = A.rec_in[j];
+        if (MULTI) hpid[h] = A.pid[j];
+    }
     __syncthreads();
     // S adjacent lanes share a point; lane s takes its bonds s, s + S, s + 2S, ...
     const int sub = threadIdx.x % S;
@@ -308,7 +315,15 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds(pdx_step_args A)
     const double4 ri = A.rec_in[p];
     double bx = 0.0, by = 0.0, bz = 0.0;
     int na = 0;
-    const double bc = A.bc;
+    double bc = A.bc, epsc = A.epsc, epsc_lo = A.epsc_lo, epsc_hi = A.epsc_hi, rho = A.rho;
+    if (MULTI) {
+        const pdx_pset Pi = A.psets[A.pid[p]];
+        bc = Pi.bc;
+        epsc = Pi.epsc;
+        epsc_lo = Pi.epsc_lo;
+        epsc_hi = Pi.epsc_hi;
+        rho = Pi.rho;
+    }
     for (int t = sub; t < c; t += U * S) {
         uint32_t wd[U];
         double Ld[U];
@@ -325,7 +340,8 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds(pdx_step_args A)
 #pragma unroll
         for (int u = 0; u < U; u++) {
             if (t + u * S >= c) continue;
-            const double4 rj = hrec[wd[u] & PDX_W16_SLOT];
+            const int slot = (int)(wd[u] & PDX_W16_SLOT);
+            const double4 rj = hrec[slot];
             const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
             uint16_t *const wp = wbase + (int64_t)(t + u * S) * 64;
             const double d2 = (dx * dx + dy * dy) + dz * dz;
@@ -333,8 +349,9 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds(pdx_step_args A)
             const double l = pdx_sqrt_rsq(d2, rl);
             const double e = l - Ld[u];
             const double eps = e * pdx_rcp1(Ld[u]);
-            const bool act = bond_fail16(A, wp, wd[u], e, Ld[u], eps);
-            const double wbc = act ? bc : 0.0;  // (w * bc) with w = bond_active
+            const bool act = bond_fail16(wp, wd[u], e, Ld[u], eps, epsc, epsc_lo, epsc_hi);
+            const double bcb = MULTI ? (bc + A.psets[hpid[slot]].bc) / 2.0 : bc;
+            const double wbc = act ? bcb : 0.0;  // (w * bc) with w = bond_active
             const double s = ((wbc * eps) * rj.w) * rl;
             bx = __builtin_fma(s, dx, bx);
             by = __builtin_fma(s, dy, by);
@@ -357,7 +374,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds(pdx_step_args A)
     tail_ops T;
     tail_load(A, p, T);
     const double b[3] = {bx, by, bz};
-    point_tail(A, p, ri, b, T);
+    point_tail(A, p, ri, b, T, rho);
 }
 
 // ----------------------------------------------------------------------------------------
@@ -371,7 +388,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds(pdx_step_args A)
 // rounding, the effect of a compensated FP32 sum).  Per bond only the 16-bit bond word is
 // read: L comes from D.  The Velocity Verlet tail stays FP64.
 // ----------------------------------------------------------------------------------------
-template <int U, int S>
+template <int U, int S, bool MULTI>
 __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds_mx(pdx_step_args A)
 {
     extern __shared__ float4 hmx[];  // [2 H]: (X - O, V), (u - u_O, 0)
@@ -384,8 +401,10 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds_mx(pdx_step_args A)
     // perturbed positions of the Newton-Krylov products work like the Verlet state
     const double4 O = A.Xr[p0], xo = A.rec_in[p0];
     const double uo0 = xo.x - O.x, uo1 = xo.y - O.y, uo2 = xo.z - O.z;
+    uint8_t *const hpid = reinterpret_cast<uint8_t *>(hmx + 2 * A.halo_max);
     for (int h = threadIdx.x; h < H; h += PDX_BLOCK * S) {
         const int64_t j = A.halo[h0 + h];
+        if (MULTI) hpid[h] = A.pid[j];
         const double4 X = A.Xr[j], x = A.rec_in[j];
         hmx[2 * h] = make_float4((float)(X.x - O.x), (float)(X.y - O.y), (float)(X.z - O.z), (float)X.w);
         hmx[2 * h + 1] = make_float4((float)((x.x - X.x) - uo0), (float)((x.y - X.y) - uo1),
@@ -405,7 +424,14 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds_mx(pdx_step_args A)
                 wi = (float)((ri.z - Xi.z) - uo2);
     double bx = 0.0, by = 0.0, bz = 0.0;
     int na = 0;
-    const float bcf = (float)A.bc, epscf = (float)A.epsc;
+    float bcf = (float)A.bc, epscf = (float)A.epsc;
+    double rho = A.rho;
+    if (MULTI) {
+        const pdx_pset Pi = A.psets[A.pid[p]];
+        bcf = (float)Pi.bc;
+        epscf = (float)Pi.epsc;
+        rho = Pi.rho;
+    }
     for (int t = sub; t < c; t += U * S) {
         uint32_t wd[U];
 #pragma unroll
@@ -429,7 +455,8 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds_mx(pdx_step_args A)
                 act = false;
                 wbase[(int64_t)(t + u * S) * 64] = (uint16_t)(wd[u] & ~PDX_W16_ACT);
             }
-            const float sf = act ? ((bcf * eps) * a.w) * __builtin_amdgcn_rcpf(l) : 0.f;
+            const float bcb = MULTI ? (bcf + (float)A.psets[hpid[slot]].bc) * 0.5f : bcf;
+            const float sf = act ? ((bcb * eps) * a.w) * __builtin_amdgcn_rcpf(l) : 0.f;
             bx += (double)(sf * (Dx + Ux));
             by += (double)(sf * (Dy + Uy));
             bz += (double)(sf * (Dz + Uz));
@@ -451,7 +478,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds_mx(pdx_step_args A)
     tail_ops T;
     tail_load(A, p, T);
     const double bb[3] = {bx, by, bz};
-    point_tail(A, p, ri, bb, T);
+    point_tail(A, p, ri, bb, T, rho);
 }
 
 // ----------------------------------------------------------------------------------------
@@ -534,14 +561,14 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_osb_pass2(pdx_step_args A)
     tail_ops T;
     tail_load(A, p, T);
     const double b[3] = {bx, by, bz};
-    point_tail(A, p, ri, b, T);
+    point_tail(A, p, ri, b, T, A.rho);
 }
 
 // OSB on the LDS-staged layout (four lanes per point, as the BB step).  Pass 1 stages the
 // block's neighbour records x|V; pass 2 stages x|V and the neighbours' (c1, c2) behind
 // them (48 B per neighbour-table entry, A.halo_max entries of each).  Bond states live in
 // the 16-bit words.
-template <int S>
+template <int S, bool MULTI>
 __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds1(pdx_step_args A)
 {
     extern __shared__ double4 hrec[];
@@ -550,7 +577,12 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds1(pdx_step_args A)
     if (p0 >= A.n) return;  // whole workgroup
     const int64_t h0 = A.halo_ptr[k];
     const int H = (int)(A.halo_ptr[k + 1] - h0);
-    for (int h = threadIdx.x; h < H; h += PDX_BLOCK * S) hrec[h] = A.rec_in[A.halo[h0 + h]];
+    uint8_t *const hpid = reinterpret_cast<uint8_t *>(hrec + A.halo_max);
+    for (int h = threadIdx.x; h < H; h += PDX_BLOCK * S) {
+        const int32_t j = A.halo[h0 + h];
+        hrec[h] = A.rec_in[j];
+        if (MULTI) hpid[h] = A.pid[j];
+    }
     __syncthreads();
     const int sub = threadIdx.x % S;
     const int64_t p = p0 + threadIdx.x / S;
@@ -562,17 +594,41 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds1(pdx_step_args A)
     const double4 ri = A.rec_in[p];
     double wvol = 0.0, Ssum = 0.0;
     int na = 0;
+    double epsc = A.epsc, epsc_lo = A.epsc_lo, epsc_hi = A.epsc_hi, G = A.G, K = A.K;
+    if (MULTI) {
+        const pdx_pset Pi = A.psets[A.pid[p]];
+        epsc = Pi.epsc;
+        epsc_lo = Pi.epsc_lo;
+        epsc_hi = Pi.epsc_hi;
+    }
     for (int t = sub; t < c; t += S) {
         const int64_t o = (int64_t)t * 64;
         const uint32_t wd = wbase[o];
         const double Lb = Lbase[o];
-        const double4 rj = hrec[wd & PDX_W16_SLOT];
+        const int slot = (int)(wd & PDX_W16_SLOT);
+        const double4 rj = hrec[slot];
         const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
         double rl;
         const double l = pdx_sqrt_rsq((dx * dx + dy * dy) + dz * dz, rl);
         const double e = l - Lb;
         const double rL = pdx_rcp1(Lb);
-        const bool act = bond_fail16(A, wbase + o, wd, e, Lb, e * rL);
+        bool act;
+        if (MULTI) {
+            // The neighbour fails its own copy of the bond against its eps_c; the stretch is
+            // the same number at both ends, so the owner tracks that copy's state (RACT)
+            // exactly, which the gather of pass 2 needs.
+            const double eps = e * rL;
+            const pdx_pset Pj = A.psets[hpid[slot]];
+            uint32_t nw = wd;
+            if ((wd & PDX_W16_ACT) && (wd & PDX_W16_FP) && eps >= epsc_lo)
+                if ((eps > epsc_hi) ? true : ((e / Lb) > epsc)) nw &= ~PDX_W16_ACT;
+            if ((wd & PDX_W16_RACT) && (wd & PDX_W16_FP) && eps >= Pj.epsc_lo)
+                if ((eps > Pj.epsc_hi) ? true : ((e / Lb) > Pj.epsc)) nw &= ~PDX_W16_RACT;
+            if (nw != wd) wbase[o] = (uint16_t)nw;
+            act = (nw & PDX_W16_ACT) != 0;
+        } else {
+            act = bond_fail16(wbase + o, wd, e, Lb, e * rL, epsc, epsc_lo, epsc_hi);
+        }
         if (act) {
             const double w = A.delta * rL;  // linear_kernel(delta, L)
             wvol += (w * (Lb * Lb)) * rj.w;
@@ -588,10 +644,14 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds1(pdx_step_args A)
     }
     if (sub != 0) return;
     double c1 = 0.0, c2 = 0.0;
-    if (wvol != 0.0) {
+    if (MULTI) {
+        // several sets: c1, c2 depend on the bond (averaged G, K); keep wvol and the dilatation
+        c1 = wvol;
+        c2 = wvol != 0.0 ? (3.0 / wvol) * Ssum : 0.0;
+    } else if (wvol != 0.0) {
         const double dil = (3.0 / wvol) * Ssum;
-        c1 = (15.0 * A.G) / wvol;
-        c2 = dil * ((3.0 * A.K) / wvol - c1 / 3.0);
+        c1 = (15.0 * G) / wvol;
+        c2 = dil * ((3.0 * K) / wvol - c1 / 3.0);
     }
     reinterpret_cast<double2 *>(A.pstate)[p] = make_double2(c1, c2);
     if (A.flags & (PDX_F_STORE | PDX_F_FORCE_ONLY)) {
@@ -600,7 +660,11 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds1(pdx_step_args A)
     }
 }
 
-template <int S>
+// MULTI: several parameter sets (force_density_point! with a ParameterHandler,
+// ordinary_state_based.jl:151-174): pass 1 left (wvol, dilatation) per point and the bond
+// takes c1 = 15 (G_i + G_j) / (2 wvol), c2 = dil (3 (K_i + K_j) / (2 wvol) - c1 / 3) of each
+// end; a point with wvol = 0 contributes nothing (its loop returns early)
+template <int S, bool MULTI>
 __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds2(pdx_step_args A)
 {
     extern __shared__ double4 hrec[];
@@ -611,10 +675,12 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds2(pdx_step_args A)
     const int64_t h0 = A.halo_ptr[k];
     const int H = (int)(A.halo_ptr[k + 1] - h0);
     const double2 *const cst = reinterpret_cast<const double2 *>(A.pstate);
+    uint8_t *const hpid = reinterpret_cast<uint8_t *>(hc + A.halo_max);
     for (int h = threadIdx.x; h < H; h += PDX_BLOCK * S) {
         const int64_t j = A.halo[h0 + h];
         hrec[h] = A.rec_in[j];
         hc[h] = cst[j];
+        if (MULTI) hpid[h] = A.pid[j];
     }
     __syncthreads();
     const int sub = threadIdx.x % S;
@@ -627,11 +693,20 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds2(pdx_step_args A)
     const double4 ri = A.rec_in[p];
     const double2 ci = cst[p];
     double bx = 0.0, by = 0.0, bz = 0.0;
+    double Gi = A.G, Ki = A.K, rho = A.rho;
+    if (MULTI) {
+        const pdx_pset Pi = A.psets[A.pid[p]];
+        Gi = Pi.G;
+        Ki = Pi.K;
+        rho = Pi.rho;
+    }
     for (int t = sub; t < c; t += S) {
         const int64_t o = (int64_t)t * 64;
         const uint32_t wd = wbase[o];
         const double Lb = Lbase[o];
-        if (!(wd & PDX_W16_ACT)) continue;
+        const bool act = (wd & PDX_W16_ACT) != 0;
+        const bool ract = MULTI ? (wd & PDX_W16_RACT) != 0 : act;  // one set: both copies agree
+        if (!act && !ract) continue;
         const int slot = (int)(wd & PDX_W16_SLOT);
         const double4 rj = hrec[slot];
         const double2 cj = hc[slot];
@@ -640,8 +715,26 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds2(pdx_step_args A)
         const double l = pdx_sqrt_rsq((dx * dx + dy * dy) + dz * dz, rl);
         const double w = A.delta * pdx_rcp1(Lb);
         const double e = l - Lb;
-        const double fi = ci.y * Lb + ci.x * e;
-        const double fj = cj.y * Lb + cj.x * e;
+        double fi, fj;
+        if (MULTI) {
+            const pdx_pset Pj = A.psets[hpid[slot]];
+            const double Gs = Gi + Pj.G, Ks = Ki + Pj.K;
+            fi = 0.0;
+            fj = 0.0;
+            if (act && ci.x != 0.0) {
+                const double c1 = (15.0 * Gs) / (2.0 * ci.x);
+                const double c2 = ci.y * ((3.0 * Ks) / (2.0 * ci.x) - c1 / 3.0);
+                fi = c2 * Lb + c1 * e;
+            }
+            if (ract && cj.x != 0.0) {
+                const double c1 = (15.0 * Gs) / (2.0 * cj.x);
+                const double c2 = cj.y * ((3.0 * Ks) / (2.0 * cj.x) - c1 / 3.0);
+                fj = c2 * Lb + c1 * e;
+            }
+        } else {
+            fi = ci.y * Lb + ci.x * e;
+            fj = cj.y * Lb + cj.x * e;
+        }
         const double s = (w * rl) * ((fi + fj) * rj.w);
         bx = __builtin_fma(s, dx, bx);
         by = __builtin_fma(s, dy, by);
@@ -657,7 +750,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds2(pdx_step_args A)
     tail_ops T;
     tail_load(A, p, T);
     const double b[3] = {bx, by, bz};
-    point_tail(A, p, ri, b, T);
+    point_tail(A, p, ri, b, T, rho);
 }
 
 // ----------------------------------------------------------------------------------------
@@ -888,7 +981,7 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_cpd_pass2(pdx_step_args A)
     }
     tail_ops T;
     tail_load(A, p, T);
-    point_tail(A, p, ri, b3, T);
+    point_tail(A, p, ri, b3, T, A.rho);
 }
 
 // ----------------------------------------------------------------------------------------
@@ -938,6 +1031,8 @@ pdx_status pdx_fill_step_args(pdx_ctx *c, pdx_step_args *a, int64_t step, int fl
     a->halo_ptr = c->halo_ptr;
     a->halo = c->halo;
     a->halo_max = c->halo_max;
+    a->psets = c->multi() ? c->d_psets : nullptr;
+    a->pid = c->multi() ? c->d_pid : nullptr;
     a->L = c->L;
     a->Lf = c->Lf;
     a->rec_in = c->rec[c->cur];
@@ -992,12 +1087,19 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
             // bonds per lane and unrolled iteration: the FP32 bond arithmetic of the mixed
             // step is short enough that four loads in flight pay; FP64 is indifferent
             const bool u4 = c->bb_unroll ? c->bb_unroll == 4 : c->precision == PDX_MIXED;
-            if (c->precision == PDX_MIXED) {
-                if (u4) hipLaunchKernelGGL((k_bb_lds_mx<4, PDX_LDS_LANES>), G, BL, lds, c->stream, a);
-                else hipLaunchKernelGGL((k_bb_lds_mx<2, PDX_LDS_LANES>), G, BL, lds, c->stream, a);
+            if (c->multi()) {
+                // each neighbour-table entry also carries the neighbour's parameter set
+                const size_t ldsm = lds + (size_t)std::max(c->halo_max, 1) + 16;
+                if (c->precision == PDX_MIXED)
+                    hipLaunchKernelGGL((k_bb_lds_mx<4, PDX_LDS_LANES, true>), G, BL, ldsm, c->stream, a);
+                else
+                    hipLaunchKernelGGL((k_bb_lds<2, PDX_LDS_LANES, true>), G, BL, ldsm, c->stream, a);
+            } else if (c->precision == PDX_MIXED) {
+                if (u4) hipLaunchKernelGGL((k_bb_lds_mx<4, PDX_LDS_LANES, false>), G, BL, lds, c->stream, a);
+                else hipLaunchKernelGGL((k_bb_lds_mx<2, PDX_LDS_LANES, false>), G, BL, lds, c->stream, a);
             } else {
-                if (u4) hipLaunchKernelGGL((k_bb_lds<4, PDX_LDS_LANES>), G, BL, lds, c->stream, a);
-                else hipLaunchKernelGGL((k_bb_lds<2, PDX_LDS_LANES>), G, BL, lds, c->stream, a);
+                if (u4) hipLaunchKernelGGL((k_bb_lds<4, PDX_LDS_LANES, false>), G, BL, lds, c->stream, a);
+                else hipLaunchKernelGGL((k_bb_lds<2, PDX_LDS_LANES, false>), G, BL, lds, c->stream, a);
             }
         } else if (c->precision == PDX_MIXED) {
             if (c->bb_unroll == 4) hipLaunchKernelGGL((k_bb_step<true, 4>), G, B, 0, c->stream, a);
@@ -1013,13 +1115,20 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
         if (c->lds_mode) {
             const dim3 BL(PDX_BLOCK * PDX_LDS_LANES);
             const size_t hm = (size_t)std::max(c->halo_max, 1);
+            const size_t l2 = (sizeof(double4) + sizeof(double2)) * hm;
             PDX_TRY(pdx_timed_begin(c));
-            hipLaunchKernelGGL(k_osb_lds1<PDX_LDS_LANES>, G, BL, sizeof(double4) * hm, c->stream, a);
+            if (c->multi())
+                hipLaunchKernelGGL((k_osb_lds1<PDX_LDS_LANES, true>), G, BL, sizeof(double4) * hm + hm + 16,
+                                   c->stream, a);
+            else
+                hipLaunchKernelGGL((k_osb_lds1<PDX_LDS_LANES, false>), G, BL, sizeof(double4) * hm, c->stream, a);
             PDX_HIP(c, hipGetLastError());
             PDX_TRY(pdx_timed_end(c, "pass1"));
             PDX_TRY(pdx_timed_begin(c));
-            hipLaunchKernelGGL(k_osb_lds2<PDX_LDS_LANES>, G, BL, (sizeof(double4) + sizeof(double2)) * hm,
-                               c->stream, a);
+            if (c->multi())
+                hipLaunchKernelGGL((k_osb_lds2<PDX_LDS_LANES, true>), G, BL, l2 + hm + 16, c->stream, a);
+            else
+                hipLaunchKernelGGL((k_osb_lds2<PDX_LDS_LANES, false>), G, BL, l2, c->stream, a);
             PDX_HIP(c, hipGetLastError());
             PDX_TRY(pdx_timed_end(c, "step"));
             break;

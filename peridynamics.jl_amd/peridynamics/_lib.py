@@ -112,6 +112,7 @@ SIGNATURES = {
     "pdx_synchronize": (_ST, [_P]),
     "pdx_set_points": (_ST, [_P, _I64, _P, _P, _P]),
     "pdx_set_material": (_ST, [_P, _I32, ctypes.POINTER(PdxParams)]),
+    "pdx_set_material_sets": (_ST, [_P, _I32, _I32, ctypes.POINTER(PdxParams), _P]),
     "pdx_add_precrack": (_ST, [_P, _I64, _P, _I64, _P, _I32]),
     "pdx_build_bonds": (_ST, [_P, ctypes.POINTER(_I64)]),
     "pdx_get_bonds": (_ST, [_P, _P, _P, _P, _P, _P]),
@@ -223,6 +224,15 @@ class Context:
 
     def set_material(self, kind, params):
         self.call("pdx_set_material", int(kind), ctypes.byref(params))
+
+    def set_material_sets(self, kind, sets, point_set):
+        """Several parameter sets: `sets` a list of PdxParams, `point_set` the 0-based set
+        of every point (the reference's params_map minus one)."""
+        arr = (PdxParams * len(sets))(*sets)
+        ps = np.ascontiguousarray(point_set, np.int32)
+        if ps.shape[0] != self.n:
+            raise ArgumentError(-1, "point_set needs one entry per point")
+        self.call("pdx_set_material_sets", int(kind), len(sets), arr, ptr(ps))
 
     def add_precrack(self, a, b, filter_bonds):
         a = np.ascontiguousarray(a, np.int32)

@@ -26,33 +26,48 @@ class MI355XBodyDataHandler:
             raise _arg_error("precision must be 'fp64' or 'mixed'")
         if np.any(body.params_map == 0):
             raise _arg_error("not all points have material parameters (material!)")
+        # distinct parameter sets in use (ParameterHandler, parameter_handler.jl:36-55)
         used = sorted(set(int(k) for k in body.params_map))
-        keys = {body.point_params[k - 1].key() for k in used}
-        if len(keys) != 1:
-            raise _lib.PdxError(_lib.PDX_ERR_UNSUPPORTED,
-                                "bodies with several parameter sets are not provided by the "
-                                "MI355X build")
+        keys = []
+        set_of_key = {}
+        for k in used:
+            key = body.point_params[k - 1].key()
+            if key not in set_of_key:
+                set_of_key[key] = len(keys)
+                keys.append(k)
         self.body = body
-        self.params = body.point_params[used[0] - 1]
+        self.params = body.point_params[keys[0] - 1]
         self.precision = precision
         self.ctx = Context(device, PRECISIONS[precision])
         self.n = body.n_points
         self.ctx.set_points(body.xyz, body.volume, body.fail_permit.astype(np.uint8))
-        p = self.params
         mat = body.mat
-        self.pp = PdxParams(delta=p.delta, rho=p.rho, E=p.E, nu=p.nu, G=p.G, K=p.K,
-                            lambda_=p.lam, mu=p.mu, Gc=p.Gc, epsc=p.epsc, bc=p.bc, model=0,
-                            kernel=0, zem_cs=0.5, maxdmg=0.85)
-        if isinstance(mat, CMaterial):
-            self.pp.model = mat.constitutive_model.code
-            self.pp.zem_cs = mat.zem.Cs
-            self.pp.maxdmg = mat.maxdmg
-        self.ctx.set_material(mat.kind, self.pp)
+        sets = [self._pdx_params(body.point_params[k - 1], mat) for k in keys]
+        self.pp = sets[0]
+        if len(sets) == 1:
+            self.ctx.set_material(mat.kind, self.pp)
+        else:
+            remap = np.zeros(len(body.point_params) + 1, np.int32)
+            for k in used:
+                remap[k] = set_of_key[body.point_params[k - 1].key()]
+            self.ctx.set_material_sets(mat.kind, sets, remap[body.params_map])
+        self.n_sets = len(sets)
         for a, b, filt in body.point_sets_precracks:
             self.ctx.add_precrack(body.point_sets[a], body.point_sets[b], filt)
         self.n_bonds = self.ctx.build_bonds()
         self._compile_bcs()
         self.dt = None
+
+    @staticmethod
+    def _pdx_params(p, mat):
+        pp = PdxParams(delta=p.delta, rho=p.rho, E=p.E, nu=p.nu, G=p.G, K=p.K, lambda_=p.lam,
+                       mu=p.mu, Gc=p.Gc, epsc=p.epsc, bc=p.bc, model=0, kernel=0, zem_cs=0.5,
+                       maxdmg=0.85)
+        if isinstance(mat, CMaterial):
+            pp.model = mat.constitutive_model.code
+            pp.zem_cs = mat.zem.Cs
+            pp.maxdmg = mat.maxdmg
+        return pp
 
     # ------------------------------------------------------------------ boundary conditions
     def _compile_bcs(self):
@@ -207,8 +222,11 @@ class MI355XBodyDataHandler:
         if body.single_dim_bcs or body.posdep_single_dim_bcs:
             raise _arg_error("NewtonKrylov solver does not support all boundary condition types!\n"
                              "Specify only position-dependent boundary conditions!")
-        if self.params.has_fracture():
-            raise _arg_error("NewtonKrylov solver currently does not support damage or fracture!")
+        for k in sorted(set(int(k) for k in body.params_map)):
+            if body.point_params[k - 1].has_fracture():
+                i = int(np.argmax(body.params_map == k))
+                raise _arg_error("NewtonKrylov solver currently does not support damage or "
+                                 f"fracture!\nFracture parameters found at point index {i + 1}!")
         pts, dims, vals = [], [], []
         X = body.position
         for fld, s, d, f in body.pos_single_dim_bcs:

@@ -136,6 +136,16 @@ pdx_status pdx_set_material(pdx_ctx *ctx, int32_t kind, const pdx_params *params
  * pdx_set_material.  Call after pdx_set_points and before pdx_build_bonds. */
 pdx_status pdx_set_material_sets(pdx_ctx *ctx, int32_t kind, int32_t n_sets, const pdx_params *sets,
                                  const int32_t *point_set);
+/* Surface correction of the material's type parameter (BBMaterial{Correction},
+ * OSBMaterial{Correction}; src/discretization/bond_system_corrections.jl): 0 NoCorrection,
+ * 1 EnergySurfaceCorrection (:58-61).  With 1, pdx_build_bonds computes mfactor and the
+ * per-bond scfactor as initialize! does (:76-96, calc_mfactor! :98-122, calc_scfactor!
+ * :161-204) and every step multiplies each bond's weight by its scfactor.  PDX_BB and
+ * PDX_OSB.  Call before pdx_build_bonds. */
+pdx_status pdx_set_surface_correction(pdx_ctx *ctx, int32_t kind);
+/* The correction's factors: scfactor[n_bonds] in the bond order of pdx_get_bonds, and
+ * mfactor[n][3] (system.correction.scfactor / .mfactor).  Either pointer may be null. */
+pdx_status pdx_get_surface_correction(pdx_ctx *ctx, double *scfactor, double *mfactor);
 /* precrack!(body, set_a, set_b; update_dmg), src/discretization/predefined_cracks.jl:106-112:
  * filter_bonds = 1 deletes the bonds (update_dmg = false), 0 breaks them at t = 0. */
 pdx_status pdx_add_precrack(pdx_ctx *ctx, int64_t na, const int32_t *set_a, int64_t nb,

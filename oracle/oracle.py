@@ -67,10 +67,10 @@ def _declare(L):
     L.orc_precrack_hits.restype = None
     L.orc_precrack_hits.argtypes = [_I64, _P, _P, _P, _P, _P]
     L.orc_bb_force.restype = None
-    L.orc_bb_force.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _D, _D, _P, _P, _INT]
+    L.orc_bb_force.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _D, _D, _P, _P, _INT, _P]
     L.orc_osb_force.restype = None
     L.orc_osb_force.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _D, _D,
-                                _D, _P, _P, _P, _P]
+                                _D, _P, _P, _P, _P, _P]
     L.orc_cpd_force.restype = None
     L.orc_cpd_force.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _INT, _D, _D,
                                 _D, _D, _D, _D, _D, _D, _D, _P, _P, _P, _P]
@@ -83,13 +83,16 @@ def _declare(L):
     L.orc_update_acc_and_vel.restype = None
     L.orc_update_acc_and_vel.argtypes = [_I64, _P, _P, _D, _P, _D, _P, _P]
     L.orc_bb_force_sets.restype = None
-    L.orc_bb_force_sets.argtypes = [_I64] + [_P] * 13
+    L.orc_bb_force_sets.argtypes = [_I64] + [_P] * 14
     L.orc_osb_force_sets.restype = None
-    L.orc_osb_force_sets.argtypes = [_I64] + [_P] * 17
+    L.orc_osb_force_sets.argtypes = [_I64] + [_P] * 18
     L.orc_min_timestep_sets.restype = _D
     L.orc_min_timestep_sets.argtypes = [_I64, _P, _P, _P, _P, _P, _P]
     L.orc_update_acc_and_vel_sets.restype = None
     L.orc_update_acc_and_vel_sets.argtypes = [_I64, _P, _P, _P, _P, _D, _P, _P]
+    L.orc_esc_factors.restype = None
+    L.orc_esc_factors.argtypes = [_I64, _INT, _INT, _P, _P, _P, _P, _P, _P, _D, _P, _P, _P, _P,
+                                  _P, _P, _P]
     L.orc_has_nan.restype = _INT
     L.orc_has_nan.argtypes = [_I64, _P]
     L.orc_max_threads.restype = _INT
@@ -172,7 +175,7 @@ class OracleSim:
     """
 
     def __init__(self, X, vol, kind, params, fail_permit=None, precracks=(), nthreads=0,
-                 bonds=None, point_set=None):
+                 bonds=None, point_set=None, correction=None):
         self.X = _c(X, np.float64).reshape(-1, 3).copy()
         n = self.X.shape[0]
         self.n = n
@@ -187,7 +190,7 @@ class OracleSim:
             if kind not in ("BB", "OSB"):
                 raise ValueError("several parameter sets: BB and OSB")
             self.pt = {k: np.ascontiguousarray([sets[m][k] for m in ps], np.float64)
-                       for k in ("bc", "epsc", "G", "K", "rho")}
+                       for k in ("bc", "epsc", "G", "K", "rho", "lambda", "mu")}
             params = sets[0]
         self.p = dict(params)
         self.nthreads = nthreads
@@ -239,6 +242,28 @@ class OracleSim:
         self.cauchy = np.zeros((n, 9), np.float64)
         self.wvol = np.zeros(n, np.float64)
         self.dil = np.zeros(n, np.float64)
+        # EnergySurfaceCorrection: factors from the bond states after the precracks
+        # (initialize!, bond_system_corrections.jl:76-87)
+        self.scf = None
+        self.mfactor = None
+        if correction == "energy":
+            if kind not in ("BB", "OSB"):
+                raise ValueError("EnergySurfaceCorrection: BB and OSB")
+            if self.pt is not None:
+                pp = {k: self.pt[k] for k in ("bc", "G", "K")}
+                pp["lambda"] = self.pt["lambda"]
+                pp["mu"] = self.pt["mu"]
+            else:
+                pp = {k: np.full(n, float(self.p[k])) for k in ("bc", "G", "K", "lambda", "mu")}
+            self.mfactor = np.zeros((n, 3), np.float64)
+            self.scf = np.zeros(nb, np.float64)
+            lib().orc_esc_factors(n, 0 if kind == "BB" else 1, 1 if self.pt is not None else 0,
+                                  _p(self.X), _p(self.vol), _p(self.row_ptr), _p(self.nbr),
+                                  _p(self.L), _p(self.active), float(self.p["delta"]),
+                                  _p(pp["lambda"]), _p(pp["mu"]), _p(pp["bc"]), _p(pp["G"]),
+                                  _p(pp["K"]), _p(self.mfactor), _p(self.scf))
+        elif correction is not None:
+            raise ValueError(correction)
 
     # calc_force_density!(chunk, t, dt)
     def force(self, x=None):
@@ -252,25 +277,27 @@ class OracleSim:
                 L.orc_bb_force_sets(self.n, _p(self.x), _p(self.vol), _p(self.row_ptr),
                                     _p(self.nbr), _p(self.L), _p(self.bfp), _p(self.active),
                                     _p(self.bond_length), _p(self.n_active), _p(self.damage),
-                                    _p(t["bc"]), _p(t["epsc"]), _p(self.b_int), _p(self.b_abs))
+                                    _p(t["bc"]), _p(t["epsc"]), _p(self.b_int), _p(self.b_abs),
+                                    _p(self.scf))
             else:
                 L.orc_osb_force_sets(self.n, _p(self.X), _p(self.x), _p(self.vol),
                                      _p(self.row_ptr), _p(self.nbr), _p(self.L), _p(self.bfp),
                                      _p(self.active), _p(self.bond_length), _p(self.omega),
                                      _p(self.n_active), _p(self.damage), _p(t["G"]),
-                                     _p(t["K"]), _p(t["epsc"]), _p(self.b_int), _p(self.b_abs))
+                                     _p(t["K"]), _p(t["epsc"]), _p(self.b_int), _p(self.b_abs),
+                                     _p(self.scf))
             return self.b_int
         if self.kind == "BB":
             L.orc_bb_force(self.n, _p(self.x), _p(self.vol), _p(self.row_ptr), _p(self.nbr),
                            _p(self.L), _p(self.bfp), _p(self.active), _p(self.bond_length),
                            _p(self.n_active), _p(self.damage), p["bc"], p["epsc"],
-                           _p(self.b_int), _p(self.b_abs), self.nthreads)
+                           _p(self.b_int), _p(self.b_abs), self.nthreads, _p(self.scf))
         elif self.kind == "OSB":
             L.orc_osb_force(self.n, _p(self.X), _p(self.x), _p(self.vol), _p(self.row_ptr),
                             _p(self.nbr), _p(self.L), _p(self.bfp), _p(self.active),
                             _p(self.bond_length), _p(self.omega), _p(self.n_active),
                             _p(self.damage), p["G"], p["K"], p["epsc"], _p(self.b_int),
-                            _p(self.wvol), _p(self.dil), _p(self.b_abs))
+                            _p(self.wvol), _p(self.dil), _p(self.b_abs), _p(self.scf))
         elif self.kind == "CPD":
             model = {"SVK": 0, "NHP": 1}[p.get("model", "SVK")]
             L.orc_cpd_force(self.n, _p(self.X), _p(self.x), _p(self.vol), _p(self.row_ptr),

@@ -310,7 +310,8 @@ void orc_precrack_hits(int64_t n, const int64_t *row_ptr, const int32_t *nbr,
 void orc_bb_force(int64_t n, const double *x, const double *vol, const int64_t *row_ptr,
                   const int32_t *nbr, const double *L, const uint8_t *bond_fp,
                   uint8_t *active, double *bond_length, int64_t *n_active, double *damage,
-                  double bc, double epsc, double *b_int, double *b_abs, int nthreads)
+                  double bc, double epsc, double *b_int, double *b_abs, int nthreads,
+                  const double *scf)
 {
     set_threads(nthreads);
 #pragma omp parallel for schedule(static)
@@ -342,7 +343,8 @@ void orc_bb_force(int64_t n, const double *x, const double *vol, const int64_t *
             double dz = x[3 * j + 2] - x[3 * i + 2];
             double l = bond_length[b];
             double eps = (l - L[b]) / L[b];
-            double w = active[b] ? 1.0 : 0.0; /* Bool * 1 */
+            /* Bool * surface_correction_factor (1, or scfactor[b]) */
+            double w = active[b] ? (scf ? scf[b] : 1.0) : 0.0;
             double s = ((w * bc) * eps) * vol[j];
             double tx = (s * dx) / l, ty = (s * dy) / l, tz = (s * dz) / l;
             bx += tx; by += ty; bz += tz;
@@ -370,7 +372,7 @@ void orc_osb_force(int64_t n, const double *X, const double *x, const double *vo
                    const uint8_t *bond_fp, uint8_t *active, double *bond_length,
                    const double *omega, int64_t *n_active, double *damage, double G,
                    double K, double epsc, double *b_int, double *wvol_out, double *dil_out,
-                   double *b_abs)
+                   double *b_abs, const double *scf)
 {
     for (int64_t k = 0; k < 3 * n; k++) b_int[k] = 0.0;
     if (b_abs) for (int64_t k = 0; k < 3 * n; k++) b_abs[k] = 0.0;
@@ -398,7 +400,7 @@ void orc_osb_force(int64_t n, const double *X, const double *x, const double *vo
             double Dz = X[3 * j + 2] - X[3 * i + 2];
             double dsq = (Dx * Dx + Dy * Dy) + Dz * Dz;
             double w = active[b] ? omega[b] : 0.0;
-            wvol += ((w * 1.0) * dsq) * vol[j];
+            wvol += ((w * (scf ? scf[b] : 1.0)) * dsq) * vol[j];
         }
         if (wvol_out) wvol_out[i] = wvol;
         if (dil_out) dil_out[i] = 0.0;
@@ -410,7 +412,7 @@ void orc_osb_force(int64_t n, const double *X, const double *x, const double *vo
             int64_t j = nbr[b];
             double l = bond_length[b];
             double w = active[b] ? omega[b] : 0.0;
-            dil += ((((w * 1.0) * c1d) * L[b]) * (l - L[b])) * vol[j];
+            dil += ((((w * (scf ? scf[b] : 1.0)) * c1d) * L[b]) * (l - L[b])) * vol[j];
         }
         if (dil_out) dil_out[i] = dil;
         double c1 = (15.0 * G) / wvol;
@@ -422,7 +424,7 @@ void orc_osb_force(int64_t n, const double *X, const double *x, const double *vo
             double dz = x[3 * j + 2] - x[3 * i + 2];
             double l = bond_length[b];
             double w = active[b] ? omega[b] : 0.0;
-            double s = ((w * 1.0) * (c2 * L[b] + c1 * (l - L[b]))) / l;
+            double s = ((w * (scf ? scf[b] : 1.0)) * (c2 * L[b] + c1 * (l - L[b]))) / l;
             double px = s * dx, py = s * dy, pz = s * dz;
             b_int[3 * i] += px * vol[j];
             b_int[3 * i + 1] += py * vol[j];
@@ -449,7 +451,8 @@ void orc_osb_force(int64_t n, const double *X, const double *x, const double *vo
 void orc_bb_force_sets(int64_t n, const double *x, const double *vol, const int64_t *row_ptr,
                        const int32_t *nbr, const double *L, const uint8_t *bond_fp,
                        uint8_t *active, double *bond_length, int64_t *n_active, double *damage,
-                       const double *bc_pt, const double *epsc_pt, double *b_int, double *b_abs)
+                       const double *bc_pt, const double *epsc_pt, double *b_int, double *b_abs,
+                       const double *scf)
 {
     for (int64_t i = 0; i < n; i++) {
         int64_t na = 0;
@@ -474,7 +477,7 @@ void orc_bb_force_sets(int64_t n, const double *x, const double *vol, const int6
             double dz = x[3 * j + 2] - x[3 * i + 2];
             double l = bond_length[b];
             double eps = (l - L[b]) / L[b];
-            double w = active[b] ? 1.0 : 0.0;
+            double w = active[b] ? (scf ? scf[b] : 1.0) : 0.0;
             double s = (((w * (bc_pt[i] + bc_pt[j])) / 2.0) * eps) * vol[j];
             double tx = (s * dx) / l, ty = (s * dy) / l, tz = (s * dz) / l;
             bx += tx; by += ty; bz += tz;
@@ -493,7 +496,7 @@ void orc_osb_force_sets(int64_t n, const double *X, const double *x, const doubl
                         const uint8_t *bond_fp, uint8_t *active, double *bond_length,
                         const double *omega, int64_t *n_active, double *damage,
                         const double *G_pt, const double *K_pt, const double *epsc_pt,
-                        double *b_int, double *b_abs)
+                        double *b_int, double *b_abs, const double *scf)
 {
     for (int64_t k = 0; k < 3 * n; k++) b_int[k] = 0.0;
     if (b_abs) for (int64_t k = 0; k < 3 * n; k++) b_abs[k] = 0.0;
@@ -522,7 +525,7 @@ void orc_osb_force_sets(int64_t n, const double *X, const double *x, const doubl
             double Dz = X[3 * j + 2] - X[3 * i + 2];
             double dsq = (Dx * Dx + Dy * Dy) + Dz * Dz;
             double w = active[b] ? omega[b] : 0.0;
-            wvol += ((w * 1.0) * dsq) * vol[j];
+            wvol += ((w * (scf ? scf[b] : 1.0)) * dsq) * vol[j];
         }
         if (wvol == 0.0) continue;
         double dil = 0.0;
@@ -530,7 +533,7 @@ void orc_osb_force_sets(int64_t n, const double *X, const double *x, const doubl
         for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
             int64_t j = nbr[b];
             double w = active[b] ? omega[b] : 0.0;
-            dil += ((((w * 1.0) * c1d) * L[b]) * (bond_length[b] - L[b])) * vol[j];
+            dil += ((((w * (scf ? scf[b] : 1.0)) * c1d) * L[b]) * (bond_length[b] - L[b])) * vol[j];
         }
         for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
             int64_t j = nbr[b];
@@ -541,7 +544,7 @@ void orc_osb_force_sets(int64_t n, const double *X, const double *x, const doubl
             double c1 = 15.0 * (G_pt[i] + G_pt[j]) / (2.0 * wvol);
             double c2 = dil * (3.0 * (K_pt[i] + K_pt[j]) / (2.0 * wvol) - c1 / 3.0);
             double w = active[b] ? omega[b] : 0.0;
-            double s = ((w * 1.0) * (c2 * L[b] + c1 * (l - L[b]))) / l;
+            double s = ((w * (scf ? scf[b] : 1.0)) * (c2 * L[b] + c1 * (l - L[b]))) / l;
             double px = s * dx, py = s * dy, pz = s * dz;
             b_int[3 * i] += px * vol[j];
             b_int[3 * i + 1] += py * vol[j];
@@ -555,6 +558,125 @@ void orc_osb_force_sets(int64_t n, const double *X, const double *x, const doubl
                 b_abs[3 * j] += fabs(px * vol[i]); b_abs[3 * j + 1] += fabs(py * vol[i]);
                 b_abs[3 * j + 2] += fabs(pz * vol[i]);
             }
+        }
+    }
+}
+
+/* EnergySurfaceCorrection factors (bond_system_corrections.jl:98-204).  Per-point parameter
+ * arrays; multi = 0 takes each point's own (lambda, mu) (get_averaged_lame_parameters with
+ * one parameter set, :124-127), multi = 1 averages them over the active bonds (:129-145).
+ * kind 0 = BB (strain_energy_density_point!, bond_based.jl:183-201), 1 = OSB
+ * (ordinary_state_based.jl:207-235), both at scfactor = 1 and the reference configuration
+ * stretched by 1.001 along one axis at a time.  mfactor[3 i + d], scfactor[b]. */
+static double orc_norm3(double a, double b, double c) { return sqrt((a * a + b * b) + c * c); }
+static double sq(double a) { return a * a; } /* Julia's literal x^2 */
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+void orc_esc_factors(int64_t n, int kind, int multi, const double *X, const double *vol,
+                     const int64_t *row_ptr, const int32_t *nbr, const double *L,
+                     const uint8_t *active, double delta, const double *lam_pt,
+                     const double *mu_pt, const double *bc_pt, const double *G_pt,
+                     const double *K_pt, double *mfactor, double *scfactor)
+{
+    const double st = 1.001;
+    double *x = (double *)malloc(sizeof(double) * 3 * n);
+    for (int d = 0; d < 3; d++) {
+        for (int64_t k = 0; k < 3 * n; k++) x[k] = X[k];
+        for (int64_t i = 0; i < n; i++) x[3 * i + d] = st * X[3 * i + d];
+        for (int64_t i = 0; i < n; i++) {
+            double la = lam_pt[i], mu = mu_pt[i];
+            if (multi) {
+                double sl = 0.0, sm = 0.0;
+                int64_t na = 0;
+                for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
+                    if (!active[b]) continue;
+                    int64_t j = nbr[b];
+                    sl += (lam_pt[i] + lam_pt[j]) / 2.0;
+                    sm += (mu_pt[i] + mu_pt[j]) / 2.0;
+                    na++;
+                }
+                la = sl / (double)na;
+                mu = sm / (double)na;
+            }
+            double psi_t = (0.5 * la + mu) * ((st - 1.0) * (st - 1.0));
+            double psi = 0.0;
+            if (kind == 0) {
+                for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
+                    int64_t j = nbr[b];
+                    double l = orc_norm3(x[3 * j] - x[3 * i], x[3 * j + 1] - x[3 * i + 1],
+                                         x[3 * j + 2] - x[3 * i + 2]);
+                    double eps = (l - L[b]) / L[b];
+                    double w = active[b] ? 1.0 : 0.0;
+                    double bc = (bc_pt[i] + bc_pt[j]) / 2.0;
+                    psi += (((((0.25 * w) * bc) * eps) * eps) * L[b]) * vol[j];
+                }
+            } else {
+                double wvol = 0.0;
+                for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
+                    int64_t j = nbr[b];
+                    double Dx = X[3 * j] - X[3 * i], Dy = X[3 * j + 1] - X[3 * i + 1],
+                           Dz = X[3 * j + 2] - X[3 * i + 2];
+                    double w = active[b] ? delta / L[b] : 0.0;
+                    wvol += ((w * 1.0) * ((Dx * Dx + Dy * Dy) + Dz * Dz)) * vol[j];
+                }
+                if (wvol != 0.0) {
+                    double c1 = 3.0 / wvol, dil = 0.0;
+                    for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
+                        int64_t j = nbr[b];
+                        double l = orc_norm3(x[3 * j] - x[3 * i], x[3 * j + 1] - x[3 * i + 1],
+                                             x[3 * j + 2] - x[3 * i + 2]);
+                        double w = active[b] ? delta / L[b] : 0.0;
+                        dil += ((((w * 1.0) * c1) * L[b]) * (l - L[b])) * vol[j];
+                    }
+                    double psi_vol = (0.5 * K_pt[i]) * (dil * dil), psi_dev = 0.0;
+                    for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
+                        int64_t j = nbr[b];
+                        double l = orc_norm3(x[3 * j] - x[3 * i], x[3 * j + 1] - x[3 * i + 1],
+                                             x[3 * j + 2] - x[3 * i + 2]);
+                        double e = l - L[b];
+                        double edev = e - ((1.0 / 3.0) * dil) * L[b];
+                        double w = active[b] ? delta / L[b] : 0.0;
+                        double G = (G_pt[i] + G_pt[j]) / 2.0;
+                        double cdev = (15.0 * G) / (2.0 * wvol);
+                        psi_dev += ((((cdev * w) * 1.0) * edev) * edev) * vol[j];
+                    }
+                    psi = psi_vol + psi_dev;
+                }
+            }
+            mfactor[3 * i + d] = psi_t / psi;
+        }
+    }
+    free(x);
+    const double right = (90.0 * M_PI) / 180.0;
+    for (int64_t i = 0; i < n; i++) {
+        for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
+            int64_t j = nbr[b];
+            double dx = X[3 * j] - X[3 * i], dy = X[3 * j + 1] - X[3 * i + 1],
+                   dz = X[3 * j + 2] - X[3 * i + 2];
+            double scx = (mfactor[3 * i] + mfactor[3 * j]) / 2.0;
+            double scy = (mfactor[3 * i + 1] + mfactor[3 * j + 1]) / 2.0;
+            double scz = (mfactor[3 * i + 2] + mfactor[3 * j + 2]) / 2.0;
+            double th, ph, scr;
+            if (fabs(dz) <= 1e-10) {
+                if (fabs(dy) <= 1e-10) th = 0.0;
+                else if (fabs(dx) <= 1e-10) th = right;
+                else th = atan(fabs(dy) / fabs(dx));
+                ph = right;
+                scr = sqrt(1.0 / ((sq(cos(th) * sin(ph)) / sq(scx) +
+                                   sq(sin(th) * sin(ph)) / sq(scy)) +
+                                  sq(cos(ph)) / sq(scz)));
+            } else if (fabs(dx) <= 1e-10 && fabs(dy) <= 1e-10) {
+                scr = scz;
+            } else {
+                th = atan(fabs(dy) / fabs(dx));
+                ph = acos(fabs(dz) / L[b]);
+                scr = sqrt(1.0 / ((sq(cos(th) * sin(ph)) / sq(scx) +
+                                   sq(sin(th) * sin(ph)) / sq(scy)) +
+                                  sq(cos(ph)) / sq(scz)));
+            }
+            scfactor[b] = scr;
         }
     }
 }

@@ -45,9 +45,12 @@ struct pdx_nk;   // Newton-Krylov workspace (pdx_nk.hip)
 
 // one parameter set on the device (bodies with several material! sets)
 struct pdx_pset {
-    double bc, epsc, epsc_lo, epsc_hi, rho, delta, G, K;
+    double bc, epsc, epsc_lo, epsc_hi, rho, delta, G, K, lam, mu;
 };
 #define PDX_MAX_PSETS 255
+// feature mask of the LDS-staged step kernels
+#define PDX_FEAT_MULTI 1  // several parameter sets
+#define PDX_FEAT_SC 2     // EnergySurfaceCorrection scfactor per bond
 struct pdx_gnn;  // learned preconditioner (pdx_gnn.hip)
 
 struct pdx_ctx {
@@ -76,6 +79,11 @@ struct pdx_ctx {
     pdx_pset *d_psets = nullptr;
     uint8_t *d_pid = nullptr;
     bool multi() const { return psets.size() > 1; }
+    // EnergySurfaceCorrection (bond_system_corrections.jl:46-204): 0 none, 1 energy based
+    int32_t corr = 0;
+    double *scf = nullptr;     // scfactor per bond slot (SELL-64, like L)
+    float *scf32 = nullptr;    // FP32 copy for the mixed step
+    double *mfactor = nullptr; // 3 x n (internal order), mfactor[d * n + i]
     std::vector<pdx_crack> cracks;
     bool bonds_built = false;
 
@@ -296,6 +304,8 @@ struct pdx_step_args {
     const int64_t *halo_ptr;
     const int32_t *halo;
     int32_t halo_max;
+    const double *scf;  // EnergySurfaceCorrection scfactor per bond slot (FP64), or null
+    const float *scf32; // the same in FP32 (mixed precision)
     const pdx_pset *psets;  // several parameter sets: the table and each point's set
     const uint8_t *pid;
     const double *L;
@@ -323,6 +333,10 @@ pdx_status pdx_fill_step_args(pdx_ctx *c, pdx_step_args *a, int64_t step, int fl
 pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a);  // one step of the material
 pdx_status pdx_launch_prologue(pdx_ctx *c, int64_t step, double dt, const double *bcv_dev);
 pdx_status pdx_apply_fd_bcs(pdx_ctx *c, const double *bcv_row);  // b_ext conditions of one step
+pdx_status pdx_build_correction(pdx_ctx *c);  // EnergySurfaceCorrection factors
+// device slot of every bond in the caller's bond order (rows by caller index, neighbours by
+// ascending caller index)
+pdx_status pdx_caller_bond_slots(pdx_ctx *c, std::vector<int64_t> &slots);
 pdx_status pdx_check_flags(pdx_ctx *c, int64_t *nan_step);
 pdx_status pdx_reset_flags(pdx_ctx *c);
 

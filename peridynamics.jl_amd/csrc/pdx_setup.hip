@@ -70,6 +70,9 @@ void pdx_free_all(pdx_ctx *c)
     pdx_gnn_free(c);
     FREE(c->d_psets);
     FREE(c->d_pid);
+    FREE(c->scf);
+    FREE(c->scf32);
+    FREE(c->mfactor);
     FREE(c->fd_pts);
     FREE(c->fd_src);
     FREE(c->dr_vh_old);
@@ -237,7 +240,7 @@ static pdx_status upload_psets(pdx_ctx *c)
     for (size_t k = 0; k < t.size(); k++) {
         const pdx_params &q = c->psets[k];
         t[k] = {q.bc, q.epsc, q.epsc - 1e-11 * fabs(q.epsc), q.epsc + 1e-11 * fabs(q.epsc), q.rho,
-                q.delta, q.G, q.K};
+                q.delta, q.G, q.K, q.lambda, q.mu};
     }
     std::vector<uint8_t> pid(c->n);
     for (int64_t i = 0; i < c->n; i++) pid[c->iperm[i]] = c->pset_of[i];
@@ -865,11 +868,42 @@ extern "C" pdx_status pdx_build_bonds(pdx_ctx *c, int64_t *n_bonds_out)
     if (c->multi() && !c->lds_mode)
         return pdx_fail(c, PDX_ERR_UNSUPPORTED,
                         "several parameter sets need the LDS-staged bond layout (a neighbour table did not fit)");
+    if (c->corr && !c->lds_mode)
+        return pdx_fail(c, PDX_ERR_UNSUPPORTED,
+                        "the surface correction needs the LDS-staged bond layout (BB or OSB, neighbour tables that fit)");
+    PDX_TRY(pdx_build_correction(c));
     c->bonds_built = true;
     if (n_bonds_out) *n_bonds_out = nb;
     return PDX_OK;
 #undef TRY_C
 #undef HIP_C
+}
+
+pdx_status pdx_caller_bond_slots(pdx_ctx *c, std::vector<int64_t> &slots)
+{
+    const int64_t n = c->n, nba = c->nb_alloc;
+    const int64_t n_slices = (n + 63) / 64;
+    std::vector<int64_t> sp(n_slices + 1);
+    std::vector<int32_t> cn(n);
+    std::vector<uint32_t> w(nba);
+    PDX_HIP(c, hipStreamSynchronize(c->stream));
+    PDX_HIP(c, hipMemcpy(sp.data(), c->slice_ptr, sizeof(int64_t) * (n_slices + 1), hipMemcpyDeviceToHost));
+    PDX_HIP(c, hipMemcpy(cn.data(), c->cnt, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    PDX_HIP(c, hipMemcpy(w.data(), c->bw, sizeof(uint32_t) * nba, hipMemcpyDeviceToHost));
+    slots.clear();
+    std::vector<std::pair<int32_t, int64_t>> row;
+    for (int64_t i = 0; i < n; i++) {
+        const int64_t m = c->iperm[i];
+        const int64_t b0 = sp[m >> 6] + (m & 63);
+        row.clear();
+        for (int t = 0; t < cn[m]; t++) {
+            const int64_t b = b0 + (int64_t)t * 64;
+            row.push_back({c->perm[w[b] & PDX_BW_NBR], b});
+        }
+        std::sort(row.begin(), row.end());
+        for (auto &e : row) slots.push_back(e.second);
+    }
+    return PDX_OK;
 }
 
 extern "C" pdx_status pdx_get_bonds(pdx_ctx *c, int64_t *row_ptr, int32_t *nbr, double *L,

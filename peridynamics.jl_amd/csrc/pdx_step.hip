@@ -286,11 +286,15 @@ __device__ __forceinline__ bool bond_fail16(uint16_t *wp, uint32_t wd, double e,
     return act;
 }
 
-// MULTI: several parameter sets (force_density_point! with a ParameterHandler,
-// bond_based.jl:163-180): eps_c of the row's point, (bc_i + bc_j) / 2 per bond, rho_i
-template <int U, int S, bool MULTI>
+// F = feature mask.  PDX_FEAT_MULTI: several parameter sets (force_density_point! with a
+// ParameterHandler, bond_based.jl:163-180): eps_c of the row's point, (bc_i + bc_j) / 2 per
+// bond, rho_i.  PDX_FEAT_SC: EnergySurfaceCorrection, w = bond_active * scfactor[bond]
+// (bond_based.jl:156, bond_system_corrections.jl:71-74).
+template <int U, int S, int F>
 __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds(pdx_step_args A)
 {
+    constexpr bool MULTI = (F & PDX_FEAT_MULTI) != 0;
+    constexpr bool SC = (F & PDX_FEAT_SC) != 0;
     extern __shared__ double4 hrec[];
     const int64_t k = pdx_xcd_block();
     const int64_t p0 = k * PDX_BLOCK;
@@ -326,15 +330,17 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds(pdx_step_args A)
     }
     for (int t = sub; t < c; t += U * S) {
         uint32_t wd[U];
-        double Ld[U];
+        double Ld[U], Sd[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
             wd[u] = 0;
             Ld[u] = 1.0;
+            Sd[u] = 1.0;
             if (t + u * S < c) {
                 const int64_t o = (int64_t)(t + u * S) * 64;
                 wd[u] = wbase[o];
                 Ld[u] = Lbase[o];
+                if (SC) Sd[u] = A.scf[b0 + o];
             }
         }
 #pragma unroll
@@ -351,7 +357,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds(pdx_step_args A)
             const double eps = e * pdx_rcp1(Ld[u]);
             const bool act = bond_fail16(wp, wd[u], e, Ld[u], eps, epsc, epsc_lo, epsc_hi);
             const double bcb = MULTI ? (bc + A.psets[hpid[slot]].bc) / 2.0 : bc;
-            const double wbc = act ? bcb : 0.0;  // (w * bc) with w = bond_active
+            const double wbc = act ? (SC ? Sd[u] * bcb : bcb) : 0.0;  // (w * bc), w = active * sc
             const double s = ((wbc * eps) * rj.w) * rl;
             bx = __builtin_fma(s, dx, bx);
             by = __builtin_fma(s, dy, by);
@@ -388,9 +394,11 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds(pdx_step_args A)
 // rounding, the effect of a compensated FP32 sum).  Per bond only the 16-bit bond word is
 // read: L comes from D.  The Velocity Verlet tail stays FP64.
 // ----------------------------------------------------------------------------------------
-template <int U, int S, bool MULTI>
+template <int U, int S, int F>
 __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds_mx(pdx_step_args A)
 {
+    constexpr bool MULTI = (F & PDX_FEAT_MULTI) != 0;
+    constexpr bool SC = (F & PDX_FEAT_SC) != 0;
     extern __shared__ float4 hmx[];  // [2 H]: (X - O, V), (u - u_O, 0)
     const int64_t k = pdx_xcd_block();
     const int64_t p0 = k * PDX_BLOCK;
@@ -434,8 +442,12 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds_mx(pdx_step_args A)
     }
     for (int t = sub; t < c; t += U * S) {
         uint32_t wd[U];
+        float sc[U];
 #pragma unroll
-        for (int u = 0; u < U; u++) wd[u] = (t + u * S < c) ? wbase[(int64_t)(t + u * S) * 64] : 0u;
+        for (int u = 0; u < U; u++) {
+            wd[u] = (t + u * S < c) ? wbase[(int64_t)(t + u * S) * 64] : 0u;
+            sc[u] = (SC && t + u * S < c) ? A.scf32[b0 + (int64_t)(t + u * S) * 64] : 1.f;
+        }
 #pragma unroll
         for (int u = 0; u < U; u++) {
             if (t + u * S >= c) continue;
@@ -456,7 +468,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds_mx(pdx_step_args A)
                 wbase[(int64_t)(t + u * S) * 64] = (uint16_t)(wd[u] & ~PDX_W16_ACT);
             }
             const float bcb = MULTI ? (bcf + (float)A.psets[hpid[slot]].bc) * 0.5f : bcf;
-            const float sf = act ? ((bcb * eps) * a.w) * __builtin_amdgcn_rcpf(l) : 0.f;
+            const float sf = act ? (((SC ? sc[u] * bcb : bcb) * eps) * a.w) * __builtin_amdgcn_rcpf(l) : 0.f;
             bx += (double)(sf * (Dx + Ux));
             by += (double)(sf * (Dy + Uy));
             bz += (double)(sf * (Dz + Uz));
@@ -568,9 +580,11 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_osb_pass2(pdx_step_args A)
 // block's neighbour records x|V; pass 2 stages x|V and the neighbours' (c1, c2) behind
 // them (48 B per neighbour-table entry, A.halo_max entries of each).  Bond states live in
 // the 16-bit words.
-template <int S, bool MULTI>
+template <int S, int F>
 __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds1(pdx_step_args A)
 {
+    constexpr bool MULTI = (F & PDX_FEAT_MULTI) != 0;
+    constexpr bool SC = (F & PDX_FEAT_SC) != 0;
     extern __shared__ double4 hrec[];
     const int64_t k = pdx_xcd_block();
     const int64_t p0 = k * PDX_BLOCK;
@@ -630,7 +644,8 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds1(pdx_step_args A)
             act = bond_fail16(wbase + o, wd, e, Lb, e * rL, epsc, epsc_lo, epsc_hi);
         }
         if (act) {
-            const double w = A.delta * rL;  // linear_kernel(delta, L)
+            double w = A.delta * rL;  // linear_kernel(delta, L)
+            if (SC) w *= A.scf[b0 + o];  // omega * beta
             wvol += (w * (Lb * Lb)) * rj.w;
             Ssum += ((w * Lb) * e) * rj.w;
             na++;
@@ -664,9 +679,11 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds1(pdx_step_args A)
 // ordinary_state_based.jl:151-174): pass 1 left (wvol, dilatation) per point and the bond
 // takes c1 = 15 (G_i + G_j) / (2 wvol), c2 = dil (3 (K_i + K_j) / (2 wvol) - c1 / 3) of each
 // end; a point with wvol = 0 contributes nothing (its loop returns early)
-template <int S, bool MULTI>
+template <int S, int F>
 __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds2(pdx_step_args A)
 {
+    constexpr bool MULTI = (F & PDX_FEAT_MULTI) != 0;
+    constexpr bool SC = (F & PDX_FEAT_SC) != 0;
     extern __shared__ double4 hrec[];
     double2 *const hc = reinterpret_cast<double2 *>(hrec + A.halo_max);
     const int64_t k = pdx_xcd_block();
@@ -713,7 +730,8 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds2(pdx_step_args A)
         const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
         double rl;
         const double l = pdx_sqrt_rsq((dx * dx + dy * dy) + dz * dz, rl);
-        const double w = A.delta * pdx_rcp1(Lb);
+        double w = A.delta * pdx_rcp1(Lb);
+        if (SC) w *= A.scf[b0 + o];  // beta is symmetric: both ends' copies of the bond carry it
         const double e = l - Lb;
         double fi, fj;
         if (MULTI) {
@@ -1033,6 +1051,8 @@ pdx_status pdx_fill_step_args(pdx_ctx *c, pdx_step_args *a, int64_t step, int fl
     a->halo_max = c->halo_max;
     a->psets = c->multi() ? c->d_psets : nullptr;
     a->pid = c->multi() ? c->d_pid : nullptr;
+    a->scf = c->corr ? c->scf : nullptr;
+    a->scf32 = c->corr ? c->scf32 : nullptr;
     a->L = c->L;
     a->Lf = c->Lf;
     a->rec_in = c->rec[c->cur];
@@ -1072,6 +1092,27 @@ pdx_status pdx_fill_step_args(pdx_ctx *c, pdx_step_args *a, int64_t step, int fl
     return PDX_OK;
 }
 
+
+// launches of the LDS-staged kernels with a feature mask F (PDX_FEAT_*)
+static int pdx_features(const pdx_ctx *c)
+{
+    return (c->multi() ? PDX_FEAT_MULTI : 0) | (c->corr ? PDX_FEAT_SC : 0);
+}
+
+template <int F>
+static void launch_bb_feat(pdx_ctx *c, dim3 G, dim3 BL, size_t lds, const pdx_step_args &a)
+{
+    if (c->precision == PDX_MIXED) hipLaunchKernelGGL((k_bb_lds_mx<4, PDX_LDS_LANES, F>), G, BL, lds, c->stream, a);
+    else hipLaunchKernelGGL((k_bb_lds<2, PDX_LDS_LANES, F>), G, BL, lds, c->stream, a);
+}
+
+template <int F>
+static void launch_osb_feat(pdx_ctx *c, int pass, dim3 G, dim3 BL, size_t lds, const pdx_step_args &a)
+{
+    if (pass == 1) hipLaunchKernelGGL((k_osb_lds1<PDX_LDS_LANES, F>), G, BL, lds, c->stream, a);
+    else hipLaunchKernelGGL((k_osb_lds2<PDX_LDS_LANES, F>), G, BL, lds, c->stream, a);
+}
+
 pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
 {
     const int64_t blocks = (c->n + PDX_BLOCK - 1) / PDX_BLOCK;
@@ -1087,19 +1128,20 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
             // bonds per lane and unrolled iteration: the FP32 bond arithmetic of the mixed
             // step is short enough that four loads in flight pay; FP64 is indifferent
             const bool u4 = c->bb_unroll ? c->bb_unroll == 4 : c->precision == PDX_MIXED;
-            if (c->multi()) {
-                // each neighbour-table entry also carries the neighbour's parameter set
-                const size_t ldsm = lds + (size_t)std::max(c->halo_max, 1) + 16;
-                if (c->precision == PDX_MIXED)
-                    hipLaunchKernelGGL((k_bb_lds_mx<4, PDX_LDS_LANES, true>), G, BL, ldsm, c->stream, a);
-                else
-                    hipLaunchKernelGGL((k_bb_lds<2, PDX_LDS_LANES, true>), G, BL, ldsm, c->stream, a);
-            } else if (c->precision == PDX_MIXED) {
-                if (u4) hipLaunchKernelGGL((k_bb_lds_mx<4, PDX_LDS_LANES, false>), G, BL, lds, c->stream, a);
-                else hipLaunchKernelGGL((k_bb_lds_mx<2, PDX_LDS_LANES, false>), G, BL, lds, c->stream, a);
+            const int feat = pdx_features(c);
+            // with several parameter sets each neighbour-table entry also carries the
+            // neighbour's set
+            const size_t ldsf = lds + (c->multi() ? (size_t)std::max(c->halo_max, 1) + 16 : 0);
+            if (feat == PDX_FEAT_MULTI) launch_bb_feat<PDX_FEAT_MULTI>(c, G, BL, ldsf, a);
+            else if (feat == PDX_FEAT_SC) launch_bb_feat<PDX_FEAT_SC>(c, G, BL, ldsf, a);
+            else if (feat == (PDX_FEAT_MULTI | PDX_FEAT_SC))
+                launch_bb_feat<PDX_FEAT_MULTI | PDX_FEAT_SC>(c, G, BL, ldsf, a);
+            else if (c->precision == PDX_MIXED) {
+                if (u4) hipLaunchKernelGGL((k_bb_lds_mx<4, PDX_LDS_LANES, 0>), G, BL, lds, c->stream, a);
+                else hipLaunchKernelGGL((k_bb_lds_mx<2, PDX_LDS_LANES, 0>), G, BL, lds, c->stream, a);
             } else {
-                if (u4) hipLaunchKernelGGL((k_bb_lds<4, PDX_LDS_LANES, false>), G, BL, lds, c->stream, a);
-                else hipLaunchKernelGGL((k_bb_lds<2, PDX_LDS_LANES, false>), G, BL, lds, c->stream, a);
+                if (u4) hipLaunchKernelGGL((k_bb_lds<4, PDX_LDS_LANES, 0>), G, BL, lds, c->stream, a);
+                else hipLaunchKernelGGL((k_bb_lds<2, PDX_LDS_LANES, 0>), G, BL, lds, c->stream, a);
             }
         } else if (c->precision == PDX_MIXED) {
             if (c->bb_unroll == 4) hipLaunchKernelGGL((k_bb_step<true, 4>), G, B, 0, c->stream, a);
@@ -1116,19 +1158,22 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
             const dim3 BL(PDX_BLOCK * PDX_LDS_LANES);
             const size_t hm = (size_t)std::max(c->halo_max, 1);
             const size_t l2 = (sizeof(double4) + sizeof(double2)) * hm;
+            const size_t tail = c->multi() ? hm + 16 : 0;  // the neighbours' parameter sets
+            const int feat = pdx_features(c);
             PDX_TRY(pdx_timed_begin(c));
-            if (c->multi())
-                hipLaunchKernelGGL((k_osb_lds1<PDX_LDS_LANES, true>), G, BL, sizeof(double4) * hm + hm + 16,
-                                   c->stream, a);
-            else
-                hipLaunchKernelGGL((k_osb_lds1<PDX_LDS_LANES, false>), G, BL, sizeof(double4) * hm, c->stream, a);
+            if (feat == PDX_FEAT_MULTI) launch_osb_feat<PDX_FEAT_MULTI>(c, 1, G, BL, sizeof(double4) * hm + tail, a);
+            else if (feat == PDX_FEAT_SC) launch_osb_feat<PDX_FEAT_SC>(c, 1, G, BL, sizeof(double4) * hm + tail, a);
+            else if (feat == (PDX_FEAT_MULTI | PDX_FEAT_SC))
+                launch_osb_feat<PDX_FEAT_MULTI | PDX_FEAT_SC>(c, 1, G, BL, sizeof(double4) * hm + tail, a);
+            else launch_osb_feat<0>(c, 1, G, BL, sizeof(double4) * hm, a);
             PDX_HIP(c, hipGetLastError());
             PDX_TRY(pdx_timed_end(c, "pass1"));
             PDX_TRY(pdx_timed_begin(c));
-            if (c->multi())
-                hipLaunchKernelGGL((k_osb_lds2<PDX_LDS_LANES, true>), G, BL, l2 + hm + 16, c->stream, a);
-            else
-                hipLaunchKernelGGL((k_osb_lds2<PDX_LDS_LANES, false>), G, BL, l2, c->stream, a);
+            if (feat == PDX_FEAT_MULTI) launch_osb_feat<PDX_FEAT_MULTI>(c, 2, G, BL, l2 + tail, a);
+            else if (feat == PDX_FEAT_SC) launch_osb_feat<PDX_FEAT_SC>(c, 2, G, BL, l2 + tail, a);
+            else if (feat == (PDX_FEAT_MULTI | PDX_FEAT_SC))
+                launch_osb_feat<PDX_FEAT_MULTI | PDX_FEAT_SC>(c, 2, G, BL, l2 + tail, a);
+            else launch_osb_feat<0>(c, 2, G, BL, l2, a);
             PDX_HIP(c, hipGetLastError());
             PDX_TRY(pdx_timed_end(c, "step"));
             break;

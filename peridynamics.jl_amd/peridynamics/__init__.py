@@ -9,8 +9,9 @@ from .body import (Body, displacement_bc, failure_permit, forcedensity_bc, mater
                    n_points, no_failure, point_set, precrack, velocity_bc)
 from .data_handler import MI355XBodyDataHandler
 from .job import Job, submit
-from .materials import (BBMaterial, CMaterial, CriticalStretch, NeoHookePenalty, OSBMaterial,
-                        SaintVenantKirchhoff, ZEMSilling, linear_kernel)
+from .materials import (BBMaterial, CMaterial, CriticalStretch, EnergySurfaceCorrection,
+                        NeoHookePenalty, NoCorrection, OSBMaterial, SaintVenantKirchhoff,
+                        ZEMSilling, linear_kernel)
 from .point_generators import jitter, lattice_block, uniform_box
 from .postprocess import crack_tip, mode_i_plate
 from .solvers import DynamicRelaxation, NewtonKrylov, VelocityVerlet
@@ -22,5 +23,5 @@ __all__ = [
     "Job", "submit", "BBMaterial", "CMaterial", "CriticalStretch", "NeoHookePenalty",
     "OSBMaterial", "SaintVenantKirchhoff", "ZEMSilling", "linear_kernel", "jitter",
     "lattice_block", "uniform_box", "NewtonKrylov", "VelocityVerlet", "crack_tip",
-    "mode_i_plate", "DynamicRelaxation",
+    "mode_i_plate", "DynamicRelaxation", "EnergySurfaceCorrection", "NoCorrection",
 ]

@@ -113,6 +113,8 @@ SIGNATURES = {
     "pdx_set_points": (_ST, [_P, _I64, _P, _P, _P]),
     "pdx_set_material": (_ST, [_P, _I32, ctypes.POINTER(PdxParams)]),
     "pdx_set_material_sets": (_ST, [_P, _I32, _I32, ctypes.POINTER(PdxParams), _P]),
+    "pdx_set_surface_correction": (_ST, [_P, _I32]),
+    "pdx_get_surface_correction": (_ST, [_P, _P, _P]),
     "pdx_add_precrack": (_ST, [_P, _I64, _P, _I64, _P, _I32]),
     "pdx_build_bonds": (_ST, [_P, ctypes.POINTER(_I64)]),
     "pdx_get_bonds": (_ST, [_P, _P, _P, _P, _P, _P]),
@@ -233,6 +235,16 @@ class Context:
         if ps.shape[0] != self.n:
             raise ArgumentError(-1, "point_set needs one entry per point")
         self.call("pdx_set_material_sets", int(kind), len(sets), arr, ptr(ps))
+
+    def set_surface_correction(self, kind):
+        self.call("pdx_set_surface_correction", int(kind))
+
+    def get_surface_correction(self, n_bonds):
+        """(scfactor per bond in the caller's bond order, mfactor (n, 3))."""
+        sc = np.empty(n_bonds, np.float64)
+        mf = np.empty((self.n, 3), np.float64)
+        self.call("pdx_get_surface_correction", ptr(sc), ptr(mf))
+        return sc, mf
 
     def add_precrack(self, a, b, filter_bonds):
         a = np.ascontiguousarray(a, np.int32)

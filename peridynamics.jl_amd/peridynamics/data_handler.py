@@ -52,6 +52,9 @@ class MI355XBodyDataHandler:
                 remap[k] = set_of_key[body.point_params[k - 1].key()]
             self.ctx.set_material_sets(mat.kind, sets, remap[body.params_map])
         self.n_sets = len(sets)
+        corr = getattr(mat, "correction", None)
+        if corr is not None and corr.code:
+            self.ctx.set_surface_correction(corr.code)
         for a, b, filt in body.point_sets_precracks:
             self.ctx.add_precrack(body.point_sets[a], body.point_sets[b], filt)
         self.n_bonds = self.ctx.build_bonds()
@@ -301,6 +304,11 @@ class MI355XBodyDataHandler:
     def bonds(self):
         """(row_ptr, nbr, L, fail_permit, active) in the caller's point order, 0-based."""
         return self.ctx.get_bonds()
+
+    def surface_correction(self):
+        """EnergySurfaceCorrection factors: (scfactor per bond in the order of bonds(),
+        mfactor (n, 3)); system.correction.scfactor / .mfactor of the reference."""
+        return self.ctx.get_surface_correction(self.n_bonds)
 
     def close(self):
         self.ctx.close()

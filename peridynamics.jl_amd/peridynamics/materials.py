@@ -46,32 +46,52 @@ def linear_kernel(delta, L):
 
 
 class NoCorrection:
-    pass
+    """NoCorrection: bond_system_corrections.jl:10 (surface correction factor 1)."""
+    code = 0
+
+
+class EnergySurfaceCorrection:
+    """EnergySurfaceCorrection: bond_system_corrections.jl:58-61 (Le and Bobaru 2018): per-bond
+    factors from a one-time uniform-stretch energy comparison, computed on the device when
+    the bonds are built."""
+    code = 1
+
+
+def _correction(correction):
+    if correction in (None, NoCorrection):
+        return NoCorrection
+    if correction is EnergySurfaceCorrection:
+        return EnergySurfaceCorrection
+    raise _arg_error("correction must be NoCorrection or EnergySurfaceCorrection")
 
 
 class BBMaterial:
-    """BBMaterial(): bond_based.jl:74-84."""
+    """BBMaterial() / BBMaterial{Correction}(): bond_based.jl:74-84.  The type parameter is
+    the `correction` argument here: BBMaterial(correction=EnergySurfaceCorrection)."""
     kind = PDX_BB
 
-    def __init__(self, dmgmodel=None):
+    def __init__(self, dmgmodel=None, correction=None):
         self.dmgmodel = dmgmodel or CriticalStretch()
+        self.correction = _correction(correction)
 
     def __repr__(self):
-        return "BBMaterial{NoCorrection}()"
+        return f"BBMaterial{{{self.correction.__name__}}}()"
 
 
 class OSBMaterial:
-    """OSBMaterial(; kernel=linear_kernel, dmgmodel=CriticalStretch()): ordinary_state_based.jl:82-94."""
+    """OSBMaterial(; kernel=linear_kernel, dmgmodel=CriticalStretch()) and
+    OSBMaterial{Correction}(...): ordinary_state_based.jl:82-94."""
     kind = PDX_OSB
 
-    def __init__(self, kernel=linear_kernel, dmgmodel=None):
+    def __init__(self, kernel=linear_kernel, dmgmodel=None, correction=None):
         if kernel is not linear_kernel:
             raise _arg_error("only linear_kernel is provided by the MI355X build")
         self.kernel = kernel
         self.dmgmodel = dmgmodel or CriticalStretch()
+        self.correction = _correction(correction)
 
     def __repr__(self):
-        return "OSBMaterial{NoCorrection}()"
+        return f"OSBMaterial{{{self.correction.__name__}}}()"
 
 
 class CMaterial:

@@ -295,6 +295,72 @@ def mode_i_leg(npxy, nz, steps):
     return out
 
 
+def export_leg(shape, steps, freq, fields=("displacement", "damage")):
+    """VTK export every `freq` steps (the reference's Job(...; freq, fields) output) on the
+    bond-based workload: wall time per step without export, with the asynchronous export
+    pipeline (device snapshot + pinned copy overlapping the next steps, files compressed by
+    worker threads), and with a synchronous read-then-write export at the same steps."""
+    import shutil
+    import tempfile
+
+    import numpy as np
+    import peridynamics as pd
+    from peridynamics import vtk
+
+    body = make_body(*shape)
+    res = {"shape": list(shape), "steps": steps, "freq": freq, "fields": list(fields)}
+    root = tempfile.mkdtemp(prefix="pdx_export_")
+    try:
+        for mode in ("none", "async", "sync"):
+            dh = pd.MI355XBodyDataHandler(body)
+            vv = pd.VelocityVerlet(steps=steps)
+            dh.init_time_solver(vv)
+            dh.initialize(vv)
+            dh.ctx.synchronize()
+            out_dir = os.path.join(root, mode)
+            t0 = time.perf_counter()
+            if mode == "none":
+                dh.solve(vv)
+            elif mode == "async":
+                ex = vtk.VTKExporter(dh, out_dir, fields)
+                ex.capture(dh, 0, 0.0)
+                dh.solve(vv, freq=freq, on_export=ex.capture)
+                files = ex.close()
+            else:
+                base = vtk.vtk_filebase(out_dir)
+                os.makedirs(os.path.dirname(base), exist_ok=True)
+                parts = max(1, min(16, os.cpu_count() or 1, dh.n // 100_000 + 1))
+                from concurrent.futures import ThreadPoolExecutor
+                with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as pool:
+                    def sync_export(dh_, n, t):
+                        pos = dh_.get("position")
+                        data = {f: dh_.get(f) for f in fields}
+                        for fut in vtk.write_pvtu(base, n, pos, data, t, parts, None, pool):
+                            fut.result()
+                    sync_export(dh, 0, 0.0)
+                    dh.solve(vv, freq=freq, on_export=sync_export)
+            dh.ctx.synchronize()
+            wall = time.perf_counter() - t0
+            res[mode] = {"seconds": wall, "ms_per_step": wall / steps * 1e3}
+            if mode == "async":
+                nbytes = sum(os.path.getsize(os.path.join(dp, f))
+                             for dp, _, fs in os.walk(out_dir) for f in fs)
+                res["exports"] = len(files)
+                res["MB_written"] = nbytes / 1e6
+                r = vtk.read_vtk(files[-1])
+                res["last_export_matches_final_state"] = bool(
+                    np.array_equal(r["displacement"], dh.get("displacement")))
+            dh.close()
+            shutil.rmtree(out_dir, ignore_errors=True)
+    finally:
+        shutil.rmtree(root, ignore_errors=True)
+    res["async_overhead_ms_per_export"] = (res["async"]["seconds"] - res["none"]["seconds"]) / \
+        max(res.get("exports", 1), 1) * 1e3
+    res["sync_overhead_ms_per_export"] = (res["sync"]["seconds"] - res["none"]["seconds"]) / \
+        max(res.get("exports", 1), 1) * 1e3
+    return res
+
+
 def cpu_baseline(sample_shape, target_seconds=12.0, max_steps=5000):
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -470,6 +536,8 @@ def main():
         out["state_based"]["OSB_2M"] = osb_2m_leg(shape, 100)
         log("[bench] mode-I plate leg")
         out["mode_i"] = mode_i_leg(500, 4, 2000)
+        log("[bench] export leg")
+        out["export"] = export_leg(shape, 1000, 200)
 
     if not args.no_jfnk:
         log("[bench] JFNK leg")

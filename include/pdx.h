@@ -164,6 +164,20 @@ pdx_status pdx_get_bonds(pdx_ctx *ctx, int64_t *row_ptr, int32_t *nbr, double *L
 pdx_status pdx_set_field(pdx_ctx *ctx, int32_t field, const double *host);
 pdx_status pdx_get_field(pdx_ctx *ctx, int32_t field, double *host);
 
+/* --- export pipeline -------------------------------------------------------------------- */
+/* Device side of export_results / export_reference_results
+ * (src/core/threads_body_data_handler.jl:197-212; _export_results, src/auxiliary/io.jl:157-167):
+ * takes a snapshot of `fields` (PDX_FIELD_*, at most 16) in the caller's point order on
+ * the compute stream and copies it to pinned host memory on a second stream; returns at
+ * once, so the following steps overlap with the copy.  `tag` is returned by the matching
+ * wait (e.g. the time step).  At most two exports may be pending. */
+pdx_status pdx_export_begin(pdx_ctx *ctx, int32_t n_fields, const int32_t *fields, int64_t tag);
+/* Waits for the oldest pending export; field_data[k] points to field k (n * components
+ * doubles, point-major) in the library's pinned memory of slot *slot (0 or 1).  Exports
+ * fill the two slots in turn: the data stays valid until the next pdx_export_begin that
+ * fills the same slot. */
+pdx_status pdx_export_wait(pdx_ctx *ctx, int64_t *tag, int32_t *slot, const double **field_data);
+
 /* --- force density and Velocity Verlet --------------------------------------------------- */
 /* calc_stable_timestep(dh, safety_factor): velocity_verlet.jl:136-163 with
  * calc_timestep_point, bond_system.jl:269-277. */

@@ -15,7 +15,7 @@ OBJ = os.path.join(CSRC, "_obj")
 LIB = os.path.join(HERE, "libpdx.so")
 
 SOURCES = ["pdx_setup.hip", "pdx_step.hip", "pdx_nk.hip", "pdx_gnn.hip", "pdx_dr.hip",
-           "pdx_surface.hip"]
+           "pdx_surface.hip", "pdx_export.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -84,6 +84,16 @@ struct pdx_ctx {
     double *scf = nullptr;     // scfactor per bond slot (SELL-64, like L)
     float *scf32 = nullptr;    // FP32 copy for the mixed step
     double *mfactor = nullptr; // 3 x n (internal order), mfactor[d * n + i]
+    // export pipeline: two snapshot slots (device, pinned host) filled in turn; a snapshot
+    // is gathered on the compute stream and copied to the host on xstream
+    hipStream_t xstream = nullptr;
+    hipEvent_t xev_snap[2] = {nullptr, nullptr}, xev_done[2] = {nullptr, nullptr};
+    double *xdev[2] = {nullptr, nullptr}, *xhost[2] = {nullptr, nullptr};
+    size_t xcap[2] = {0, 0};
+    std::vector<int32_t> xfields[2];
+    std::vector<int64_t> xoff[2];
+    int64_t xtag[2] = {0, 0};
+    int64_t xbegun = 0, xwaited = 0;
     std::vector<pdx_crack> cracks;
     bool bonds_built = false;
 
@@ -334,6 +344,13 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a);  // one step of 
 pdx_status pdx_launch_prologue(pdx_ctx *c, int64_t step, double dt, const double *bcv_dev);
 pdx_status pdx_apply_fd_bcs(pdx_ctx *c, const double *bcv_row);  // b_ext conditions of one step
 pdx_status pdx_build_correction(pdx_ctx *c);  // EnergySurfaceCorrection factors
+void pdx_export_free(pdx_ctx *c);              // the export pipeline's buffers and stream
+// device pointer, components and stride of a field (PDX_FIELD_*)
+pdx_status pdx_field_ptr(pdx_ctx *c, int field, double **p, int *comps, int *stride);
+// components of a field, and its gather in the caller's point order into dst (n * comps
+// doubles of device memory), enqueued on the compute stream
+pdx_status pdx_field_comps(pdx_ctx *c, int field, int *comps);
+pdx_status pdx_gather_field(pdx_ctx *c, int field, double *dst);
 // device slot of every bond in the caller's bond order (rows by caller index, neighbours by
 // ascending caller index)
 pdx_status pdx_caller_bond_slots(pdx_ctx *c, std::vector<int64_t> &slots);

@@ -139,6 +139,7 @@ extern "C" void pdx_destroy(pdx_ctx *c)
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    pdx_export_free(c);
     pdx_free_all(c);
     FREE(c->d_flags);
     (void)pdx_timed_flush(c);
@@ -968,7 +969,7 @@ extern "C" pdx_status pdx_get_bonds(pdx_ctx *c, int64_t *row_ptr, int32_t *nbr, 
 // ----------------------------------------------------------------------------------------
 // fields
 // ----------------------------------------------------------------------------------------
-static pdx_status field_ptr(pdx_ctx *c, int field, double **p, int *comps, int *stride)
+pdx_status pdx_field_ptr(pdx_ctx *c, int field, double **p, int *comps, int *stride)
 {
     *p = nullptr;
     *comps = 3;
@@ -1007,6 +1008,39 @@ __global__ void k_copy_xyz(int64_t n, const double4 *src, double4 *dst)
     if (k < n) dst[k] = src[k];
 }
 
+pdx_status pdx_field_comps(pdx_ctx *c, int field, int *comps)
+{
+    if (field == PDX_FIELD_N_ACTIVE_BONDS) {
+        *comps = 1;
+        return PDX_OK;
+    }
+    double *p;
+    int stride;
+    PDX_TRY(pdx_field_ptr(c, field, &p, comps, &stride));
+    return PDX_OK;
+}
+
+pdx_status pdx_gather_field(pdx_ctx *c, int field, double *dst)
+{
+    double *p;
+    int comps = 1, stride = 1;
+    if (field == PDX_FIELD_N_ACTIVE_BONDS) {
+        double *tmp = c->stage + 4 * c->n;  // second half of the staging buffer
+        hipLaunchKernelGGL(k_int_to_double, dim3((c->n + 255) / 256), dim3(256), 0, c->stream,
+                           c->n, c->nact, tmp);
+        hipLaunchKernelGGL(k_gather_out, dim3((c->n + 255) / 256), dim3(256), 0, c->stream,
+                           c->n, 1, 1, c->d_iperm, tmp, dst);
+        PDX_HIP(c, hipGetLastError());
+        return PDX_OK;
+    }
+    PDX_TRY(pdx_field_ptr(c, field, &p, &comps, &stride));
+    const int64_t cnt = c->n * comps;
+    hipLaunchKernelGGL(k_gather_out, dim3((cnt + 255) / 256), dim3(256), 0, c->stream, c->n, comps,
+                       stride, c->d_iperm, p, dst);
+    PDX_HIP(c, hipGetLastError());
+    return PDX_OK;
+}
+
 extern "C" pdx_status pdx_get_field(pdx_ctx *c, int32_t field, double *host)
 {
     if (!c || !host) return PDX_ERR_ARG;
@@ -1021,7 +1055,7 @@ extern "C" pdx_status pdx_get_field(pdx_ctx *c, int32_t field, double *host)
     }
     double *p;
     int comps, stride;
-    PDX_TRY(field_ptr(c, field, &p, &comps, &stride));
+    PDX_TRY(pdx_field_ptr(c, field, &p, &comps, &stride));
     if (field == PDX_FIELD_VOLUME) {
         // .w of the reference record
         return pdx_download_vec(c, p, host, 1, 4);
@@ -1040,7 +1074,7 @@ extern "C" pdx_status pdx_set_field(pdx_ctx *c, int32_t field, const double *hos
     PDX_HIP(c, hipSetDevice(c->device));
     double *p;
     int comps, stride;
-    PDX_TRY(field_ptr(c, field, &p, &comps, &stride));
+    PDX_TRY(pdx_field_ptr(c, field, &p, &comps, &stride));
     PDX_TRY(pdx_upload_strided(c, host, p, comps, stride));
     if (field == PDX_FIELD_B_EXT) {
         c->bext_nonzero = false;

@@ -15,6 +15,7 @@ from .materials import (BBMaterial, CMaterial, CriticalStretch, EnergySurfaceCor
 from .point_generators import jitter, lattice_block, uniform_box
 from .postprocess import crack_tip, mode_i_plate
 from .solvers import DynamicRelaxation, NewtonKrylov, VelocityVerlet
+from .vtk import VTKExporter, process_each_export, read_vtk, write_pvtu, write_vtu
 
 __all__ = [
     "ArgumentError", "DuplicatePointError", "NaNError", "NotConvergedError", "PdxError",
@@ -24,4 +25,5 @@ __all__ = [
     "OSBMaterial", "SaintVenantKirchhoff", "ZEMSilling", "linear_kernel", "jitter",
     "lattice_block", "uniform_box", "NewtonKrylov", "VelocityVerlet", "crack_tip",
     "mode_i_plate", "DynamicRelaxation", "EnergySurfaceCorrection", "NoCorrection",
+    "VTKExporter", "process_each_export", "read_vtk", "write_pvtu", "write_vtu",
 ]

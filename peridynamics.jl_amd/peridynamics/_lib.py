@@ -114,6 +114,9 @@ SIGNATURES = {
     "pdx_set_material": (_ST, [_P, _I32, ctypes.POINTER(PdxParams)]),
     "pdx_set_material_sets": (_ST, [_P, _I32, _I32, ctypes.POINTER(PdxParams), _P]),
     "pdx_set_surface_correction": (_ST, [_P, _I32]),
+    "pdx_export_begin": (_ST, [_P, _I32, _P, _I64]),
+    "pdx_export_wait": (_ST, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I32),
+                              ctypes.POINTER(ctypes.c_void_p)]),
     "pdx_get_surface_correction": (_ST, [_P, _P, _P]),
     "pdx_add_precrack": (_ST, [_P, _I64, _P, _I64, _P, _I32]),
     "pdx_build_bonds": (_ST, [_P, ctypes.POINTER(_I64)]),
@@ -272,6 +275,31 @@ class Context:
         out = np.zeros((self.n, comps), np.float64)
         self.call("pdx_get_field", fid, ptr(out))
         return out if comps > 1 else out[:, 0]
+
+    def export_begin(self, names, tag):
+        """Snapshot of the named fields, copied to pinned host memory while the device
+        goes on (pdx_export_begin)."""
+        ids = np.ascontiguousarray([FIELDS[f][0] for f in names], np.int32)
+        self.call("pdx_export_begin", len(ids), ptr(ids), int(tag))
+        if not hasattr(self, "_xnames"):
+            self._xnames = []
+        self._xnames.append(list(names))
+
+    def export_wait(self):
+        """(tag, slot, {name: array}) of the oldest pending export.  The arrays view the
+        library's pinned memory of that slot: they stay valid until the next export_begin
+        that fills the same slot (the slots alternate)."""
+        ptrs = (ctypes.c_void_p * 16)()
+        tag, slot = _I64(), _I32()
+        self.call("pdx_export_wait", ctypes.byref(tag), ctypes.byref(slot), ptrs)
+        names = self._xnames.pop(0)
+        out = {}
+        for k, f in enumerate(names):
+            comps = FIELDS[f][1]
+            buf = (ctypes.c_double * (self.n * comps)).from_address(ptrs[k])
+            a = np.ctypeslib.as_array(buf)
+            out[f] = a.reshape(self.n, comps) if comps > 1 else a
+        return tag.value, slot.value, out
 
     def set_field(self, name, values):
         fid, comps = FIELDS[name]

@@ -1,10 +1,13 @@
 """Job and submit (src/core/job.jl:45-62, src/core/submit.jl:14-27, 99-115).
 
 submit() runs the job on one GPU through MI355XBodyDataHandler; it is the sibling of the
-reference's _submit_threads.  Exported fields are kept in memory (job.results) instead of
-being written as VTK files."""
+reference's _submit_threads.  With a `path`, every `freq`-th step (and the reference
+configuration, step 0) is written as VTK files under `<path>/vtk` through the asynchronous
+export pipeline (peridynamics.vtk); without one, the exported fields are kept in memory
+(job.results)."""
 from .data_handler import MI355XBodyDataHandler
 from .solvers import DynamicRelaxation, NewtonKrylov, VelocityVerlet
+from .vtk import VTKExporter
 
 DEFAULT_FIELDS = ("displacement", "damage")
 
@@ -30,20 +33,35 @@ def submit(job, quiet=True):
     dh = MI355XBodyDataHandler(job.body, device=job.device, precision=job.precision)
     solver = job.solver
 
-    def export(dh_, n, t):
-        if job.fields:
-            job.results.append({"step": n, "time": t,
-                                **{f: dh_.get(f) for f in job.fields}})
-
-    if isinstance(solver, VelocityVerlet):
-        dh.init_time_solver(solver)
-        dh.initialize(solver)
-        dh.solve(solver, freq=job.freq if job.fields else 0, on_export=export)
-    elif isinstance(solver, DynamicRelaxation):
-        dh.calc_force_density(0.0, solver.dt)  # initialize!(dh, solver)
-        dh.solve_dr(solver, freq=job.freq if job.fields else 0, on_export=export)
-    elif isinstance(solver, NewtonKrylov):
-        dh.solve_nk(solver, on_export=export if job.fields else None)
+    exporter = None
+    if job.path and job.fields:
+        exporter = VTKExporter(dh, job.path, job.fields, body_name=getattr(job.body, "name", ""))
+        export = exporter.capture
     else:
-        raise TypeError(f"unsupported solver {type(solver).__name__}")
+        def export(dh_, n, t):
+            if job.fields:
+                job.results.append({"step": n, "time": t,
+                                    **{f: dh_.get(f) for f in job.fields}})
+
+    try:
+        if isinstance(solver, VelocityVerlet):
+            dh.init_time_solver(solver)
+            dh.initialize(solver)
+            if exporter is not None:
+                exporter.capture(dh, 0, 0.0)  # export_reference_results
+            dh.solve(solver, freq=job.freq if job.fields else 0, on_export=export)
+        elif isinstance(solver, DynamicRelaxation):
+            dh.calc_force_density(0.0, solver.dt)  # initialize!(dh, solver)
+            if exporter is not None:
+                exporter.capture(dh, 0, 0.0)
+            dh.solve_dr(solver, freq=job.freq if job.fields else 0, on_export=export)
+        elif isinstance(solver, NewtonKrylov):
+            if exporter is not None:
+                exporter.capture(dh, 0, 0.0)
+            dh.solve_nk(solver, on_export=export if job.fields else None)
+        else:
+            raise TypeError(f"unsupported solver {type(solver).__name__}")
+    finally:
+        if exporter is not None:
+            job.files = exporter.close()
     return dh

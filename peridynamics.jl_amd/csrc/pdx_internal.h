@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <map>
@@ -51,6 +52,7 @@ struct pdx_pset {
 // feature mask of the LDS-staged step kernels
 #define PDX_FEAT_MULTI 1  // several parameter sets
 #define PDX_FEAT_SC 2     // EnergySurfaceCorrection scfactor per bond
+#define PDX_FEAT_CAP 4    // neighbour tables longer than the LDS budget: the rest from global memory
 struct pdx_gnn;  // learned preconditioner (pdx_gnn.hip)
 
 struct pdx_ctx {
@@ -58,6 +60,8 @@ struct pdx_ctx {
     int precision = PDX_FP64;
     // bonds per iteration of the BB thread (tuning knob PDX_BB_UNROLL = 2 | 4)
     int bb_unroll = 0;  // bonds per lane and unrolled iteration of the BB step: 2, 4, 0 = default
+    int64_t lds_limit = 160 * 1024;  // dynamic LDS bytes a workgroup may use
+    int32_t lds_budget_env = 0;      // PDX_LDS_BUDGET (bytes), 0 = default
     hipStream_t stream = nullptr;
     std::string err;
 
@@ -313,6 +317,7 @@ struct pdx_step_args {
     uint16_t *bw16;
     const int64_t *halo_ptr;
     const int32_t *halo;
+    int32_t lds_budget;  // LDS bytes a block stages its neighbour table into
     int32_t halo_max;
     const double *scf;  // EnergySurfaceCorrection scfactor per bond slot (FP64), or null
     const float *scf32; // the same in FP32 (mixed precision)
@@ -344,6 +349,14 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a);  // one step of 
 pdx_status pdx_launch_prologue(pdx_ctx *c, int64_t step, double dt, const double *bcv_dev);
 pdx_status pdx_apply_fd_bcs(pdx_ctx *c, const double *bcv_row);  // b_ext conditions of one step
 pdx_status pdx_build_correction(pdx_ctx *c);  // EnergySurfaceCorrection factors
+// LDS bytes for a block's neighbour table: half the CU's LDS (two 1024-thread blocks per
+// CU), minus room for the parameter-set bytes' alignment
+inline int pdx_lds_budget(const pdx_ctx *c)
+{
+    const int64_t half = 80 * 1024 - 64;
+    const int64_t b = c->lds_budget_env > 0 ? c->lds_budget_env : half;
+    return (int)std::min<int64_t>(b, c->lds_limit);
+}
 void pdx_export_free(pdx_ctx *c);              // the export pipeline's buffers and stream
 // device pointer, components and stride of a field (PDX_FIELD_*)
 pdx_status pdx_field_ptr(pdx_ctx *c, int field, double **p, int *comps, int *stride);

@@ -121,6 +121,7 @@ extern "C" pdx_status pdx_create(pdx_ctx **out, int32_t device, int32_t precisio
         int v = atoi(e);
         if (v == 2 || v == 4) c->bb_unroll = v;
     }
+    if (const char *e = getenv("PDX_LDS_BUDGET")) c->lds_budget_env = atoi(e);
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
         delete c;
@@ -545,6 +546,7 @@ static pdx_status build_halos(pdx_ctx *c, int64_t entry_bytes, bool *ok)
     hipDeviceProp_t prop;
     PDX_HIP(c, hipGetDeviceProperties(&prop, c->device));
     const int64_t lds_limit = std::min<int64_t>((int64_t)prop.sharedMemPerBlock, 160 * 1024);
+    c->lds_limit = lds_limit - 64;
     // 64 bytes of slack for the alignment pad of a trailing byte table
     const int64_t slot_limit = std::min<int64_t>(PDX_W16_SLOT + 1, (lds_limit - 64) / entry_bytes);
     std::vector<int64_t> sp(n_slices + 1);

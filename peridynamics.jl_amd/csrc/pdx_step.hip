@@ -37,6 +37,15 @@ __device__ __forceinline__ int64_t pdx_thread_point()
 }
 
 // first bond entry of point p in the SELL-64 arrays; bond t is at + 64 t
+// Neighbour-table entries a block stages in LDS: all of them up to a budget that leaves
+// room for two 1024-thread blocks per CU; a block whose table is longer reads the entries
+// past the budget from global memory (through the caches).  ENTRY = LDS bytes per entry.
+__host__ __device__ __forceinline__ int pdx_lds_entries(int halo_max, int budget, int entry)
+{
+    const int cap = budget / entry;
+    return halo_max < cap ? (halo_max > 0 ? halo_max : 1) : cap;
+}
+
 __device__ __forceinline__ int64_t pdx_bond0(const int64_t *slice_ptr, int64_t p)
 {
     return slice_ptr[p >> 6] + (p & 63);
@@ -295,15 +304,21 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds(pdx_step_args A)
 {
     constexpr bool MULTI = (F & PDX_FEAT_MULTI) != 0;
     constexpr bool SC = (F & PDX_FEAT_SC) != 0;
+    constexpr bool CAP = (F & PDX_FEAT_CAP) != 0;
     extern __shared__ double4 hrec[];
     const int64_t k = pdx_xcd_block();
     const int64_t p0 = k * PDX_BLOCK;
     if (p0 >= A.n) return;  // whole workgroup
     const int64_t h0 = A.halo_ptr[k];
     const int H = (int)(A.halo_ptr[k + 1] - h0);
-    uint8_t *const hpid = reinterpret_cast<uint8_t *>(hrec + A.halo_max);
-    for (int h = threadIdx.x; h < H; h += PDX_BLOCK * S) {
-        const int32_t j = A.halo[h0 + h];
+    // CAP: the table may be longer than the LDS budget; entries past it are read from
+    // global memory (without CAP the whole table is staged)
+    const int HL = CAP ? pdx_lds_entries(A.halo_max, A.lds_budget, 32 + (MULTI ? 1 : 0)) : A.halo_max;
+    const int Hs = H < HL ? H : HL;
+    const int32_t *const hidx = A.halo + h0;
+    uint8_t *const hpid = reinterpret_cast<uint8_t *>(hrec + HL);
+    for (int h = threadIdx.x; h < Hs; h += PDX_BLOCK * S) {
+        const int32_t j = hidx[h];
         hrec[h] = A.rec_in[j];
         if (MULTI) hpid[h] = A.pid[j];
     }
@@ -347,7 +362,9 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds(pdx_step_args A)
         for (int u = 0; u < U; u++) {
             if (t + u * S >= c) continue;
             const int slot = (int)(wd[u] & PDX_W16_SLOT);
-            const double4 rj = hrec[slot];
+            const bool in_lds = !CAP || slot < HL;
+            const int32_t jg = in_lds ? 0 : hidx[slot];
+            const double4 rj = in_lds ? hrec[slot] : A.rec_in[jg];
             const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
             uint16_t *const wp = wbase + (int64_t)(t + u * S) * 64;
             const double d2 = (dx * dx + dy * dy) + dz * dz;
@@ -356,7 +373,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds(pdx_step_args A)
             const double e = l - Ld[u];
             const double eps = e * pdx_rcp1(Ld[u]);
             const bool act = bond_fail16(wp, wd[u], e, Ld[u], eps, epsc, epsc_lo, epsc_hi);
-            const double bcb = MULTI ? (bc + A.psets[hpid[slot]].bc) / 2.0 : bc;
+            const double bcb = MULTI ? (bc + A.psets[in_lds ? hpid[slot] : A.pid[jg]].bc) / 2.0 : bc;
             const double wbc = act ? (SC ? Sd[u] * bcb : bcb) : 0.0;  // (w * bc), w = active * sc
             const double s = ((wbc * eps) * rj.w) * rl;
             bx = __builtin_fma(s, dx, bx);
@@ -399,6 +416,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds_mx(pdx_step_args A)
 {
     constexpr bool MULTI = (F & PDX_FEAT_MULTI) != 0;
     constexpr bool SC = (F & PDX_FEAT_SC) != 0;
+    constexpr bool CAP = (F & PDX_FEAT_CAP) != 0;
     extern __shared__ float4 hmx[];  // [2 H]: (X - O, V), (u - u_O, 0)
     const int64_t k = pdx_xcd_block();
     const int64_t p0 = k * PDX_BLOCK;
@@ -409,9 +427,12 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds_mx(pdx_step_args A)
     // perturbed positions of the Newton-Krylov products work like the Verlet state
     const double4 O = A.Xr[p0], xo = A.rec_in[p0];
     const double uo0 = xo.x - O.x, uo1 = xo.y - O.y, uo2 = xo.z - O.z;
-    uint8_t *const hpid = reinterpret_cast<uint8_t *>(hmx + 2 * A.halo_max);
-    for (int h = threadIdx.x; h < H; h += PDX_BLOCK * S) {
-        const int64_t j = A.halo[h0 + h];
+    const int HL = CAP ? pdx_lds_entries(A.halo_max, A.lds_budget, 32 + (MULTI ? 1 : 0)) : A.halo_max;
+    const int Hs = H < HL ? H : HL;
+    const int32_t *const hidx = A.halo + h0;
+    uint8_t *const hpid = reinterpret_cast<uint8_t *>(hmx + 2 * HL);
+    for (int h = threadIdx.x; h < Hs; h += PDX_BLOCK * S) {
+        const int64_t j = hidx[h];
         if (MULTI) hpid[h] = A.pid[j];
         const double4 X = A.Xr[j], x = A.rec_in[j];
         hmx[2 * h] = make_float4((float)(X.x - O.x), (float)(X.y - O.y), (float)(X.z - O.z), (float)X.w);
@@ -452,7 +473,18 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds_mx(pdx_step_args A)
         for (int u = 0; u < U; u++) {
             if (t + u * S >= c) continue;
             const int slot = (int)(wd[u] & PDX_W16_SLOT);
-            const float4 a = hmx[2 * slot], b = hmx[2 * slot + 1];
+            float4 a, b;
+            int32_t jg = 0;
+            if (!CAP || slot < HL) {
+                a = hmx[2 * slot];
+                b = hmx[2 * slot + 1];
+            } else {  // past the staged entries: the same values from global memory
+                jg = hidx[slot];
+                const double4 X = A.Xr[jg], x = A.rec_in[jg];
+                a = make_float4((float)(X.x - O.x), (float)(X.y - O.y), (float)(X.z - O.z), (float)X.w);
+                b = make_float4((float)((x.x - X.x) - uo0), (float)((x.y - X.y) - uo1),
+                                (float)((x.z - X.z) - uo2), 0.f);
+            }
             const float Dx = a.x - xi, Dy = a.y - yi, Dz = a.z - zi;
             const float Ux = b.x - ui, Uy = b.y - vi, Uz = b.z - wi;
             const float L2 = __builtin_fmaf(Dz, Dz, __builtin_fmaf(Dy, Dy, Dx * Dx));
@@ -467,7 +499,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds_mx(pdx_step_args A)
                 act = false;
                 wbase[(int64_t)(t + u * S) * 64] = (uint16_t)(wd[u] & ~PDX_W16_ACT);
             }
-            const float bcb = MULTI ? (bcf + (float)A.psets[hpid[slot]].bc) * 0.5f : bcf;
+            const float bcb = MULTI ? (bcf + (float)A.psets[(!CAP || slot < HL) ? hpid[slot] : A.pid[jg]].bc) * 0.5f : bcf;
             const float sf = act ? (((SC ? sc[u] * bcb : bcb) * eps) * a.w) * __builtin_amdgcn_rcpf(l) : 0.f;
             bx += (double)(sf * (Dx + Ux));
             by += (double)(sf * (Dy + Uy));
@@ -585,15 +617,19 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds1(pdx_step_args A)
 {
     constexpr bool MULTI = (F & PDX_FEAT_MULTI) != 0;
     constexpr bool SC = (F & PDX_FEAT_SC) != 0;
+    constexpr bool CAP = (F & PDX_FEAT_CAP) != 0;
     extern __shared__ double4 hrec[];
     const int64_t k = pdx_xcd_block();
     const int64_t p0 = k * PDX_BLOCK;
     if (p0 >= A.n) return;  // whole workgroup
     const int64_t h0 = A.halo_ptr[k];
     const int H = (int)(A.halo_ptr[k + 1] - h0);
-    uint8_t *const hpid = reinterpret_cast<uint8_t *>(hrec + A.halo_max);
-    for (int h = threadIdx.x; h < H; h += PDX_BLOCK * S) {
-        const int32_t j = A.halo[h0 + h];
+    const int HL = CAP ? pdx_lds_entries(A.halo_max, A.lds_budget, 32 + (MULTI ? 1 : 0)) : A.halo_max;
+    const int Hs = H < HL ? H : HL;
+    const int32_t *const hidx = A.halo + h0;
+    uint8_t *const hpid = reinterpret_cast<uint8_t *>(hrec + HL);
+    for (int h = threadIdx.x; h < Hs; h += PDX_BLOCK * S) {
+        const int32_t j = hidx[h];
         hrec[h] = A.rec_in[j];
         if (MULTI) hpid[h] = A.pid[j];
     }
@@ -620,7 +656,9 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds1(pdx_step_args A)
         const uint32_t wd = wbase[o];
         const double Lb = Lbase[o];
         const int slot = (int)(wd & PDX_W16_SLOT);
-        const double4 rj = hrec[slot];
+        const bool in_lds = !CAP || slot < HL;
+        const int32_t jg = in_lds ? 0 : hidx[slot];
+        const double4 rj = in_lds ? hrec[slot] : A.rec_in[jg];
         const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
         double rl;
         const double l = pdx_sqrt_rsq((dx * dx + dy * dy) + dz * dz, rl);
@@ -632,7 +670,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds1(pdx_step_args A)
             // the same number at both ends, so the owner tracks that copy's state (RACT)
             // exactly, which the gather of pass 2 needs.
             const double eps = e * rL;
-            const pdx_pset Pj = A.psets[hpid[slot]];
+            const pdx_pset Pj = A.psets[in_lds ? hpid[slot] : A.pid[jg]];
             uint32_t nw = wd;
             if ((wd & PDX_W16_ACT) && (wd & PDX_W16_FP) && eps >= epsc_lo)
                 if ((eps > epsc_hi) ? true : ((e / Lb) > epsc)) nw &= ~PDX_W16_ACT;
@@ -684,17 +722,21 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds2(pdx_step_args A)
 {
     constexpr bool MULTI = (F & PDX_FEAT_MULTI) != 0;
     constexpr bool SC = (F & PDX_FEAT_SC) != 0;
+    constexpr bool CAP = (F & PDX_FEAT_CAP) != 0;
     extern __shared__ double4 hrec[];
-    double2 *const hc = reinterpret_cast<double2 *>(hrec + A.halo_max);
+    const int HL = CAP ? pdx_lds_entries(A.halo_max, A.lds_budget, 48 + (MULTI ? 1 : 0)) : A.halo_max;
+    double2 *const hc = reinterpret_cast<double2 *>(hrec + HL);
     const int64_t k = pdx_xcd_block();
     const int64_t p0 = k * PDX_BLOCK;
     if (p0 >= A.n) return;  // whole workgroup
     const int64_t h0 = A.halo_ptr[k];
     const int H = (int)(A.halo_ptr[k + 1] - h0);
     const double2 *const cst = reinterpret_cast<const double2 *>(A.pstate);
-    uint8_t *const hpid = reinterpret_cast<uint8_t *>(hc + A.halo_max);
-    for (int h = threadIdx.x; h < H; h += PDX_BLOCK * S) {
-        const int64_t j = A.halo[h0 + h];
+    const int Hs = H < HL ? H : HL;
+    const int32_t *const hidx = A.halo + h0;
+    uint8_t *const hpid = reinterpret_cast<uint8_t *>(hc + HL);
+    for (int h = threadIdx.x; h < Hs; h += PDX_BLOCK * S) {
+        const int64_t j = hidx[h];
         hrec[h] = A.rec_in[j];
         hc[h] = cst[j];
         if (MULTI) hpid[h] = A.pid[j];
@@ -725,8 +767,10 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds2(pdx_step_args A)
         const bool ract = MULTI ? (wd & PDX_W16_RACT) != 0 : act;  // one set: both copies agree
         if (!act && !ract) continue;
         const int slot = (int)(wd & PDX_W16_SLOT);
-        const double4 rj = hrec[slot];
-        const double2 cj = hc[slot];
+        const bool in_lds = !CAP || slot < HL;
+        const int32_t jg = in_lds ? 0 : hidx[slot];
+        const double4 rj = in_lds ? hrec[slot] : A.rec_in[jg];
+        const double2 cj = in_lds ? hc[slot] : cst[jg];
         const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
         double rl;
         const double l = pdx_sqrt_rsq((dx * dx + dy * dy) + dz * dz, rl);
@@ -735,7 +779,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds2(pdx_step_args A)
         const double e = l - Lb;
         double fi, fj;
         if (MULTI) {
-            const pdx_pset Pj = A.psets[hpid[slot]];
+            const pdx_pset Pj = A.psets[in_lds ? hpid[slot] : A.pid[jg]];
             const double Gs = Gi + Pj.G, Ks = Ki + Pj.K;
             fi = 0.0;
             fj = 0.0;
@@ -1049,6 +1093,7 @@ pdx_status pdx_fill_step_args(pdx_ctx *c, pdx_step_args *a, int64_t step, int fl
     a->halo_ptr = c->halo_ptr;
     a->halo = c->halo;
     a->halo_max = c->halo_max;
+    a->lds_budget = pdx_lds_budget(c);
     a->psets = c->multi() ? c->d_psets : nullptr;
     a->pid = c->multi() ? c->d_pid : nullptr;
     a->scf = c->corr ? c->scf : nullptr;
@@ -1099,12 +1144,41 @@ static int pdx_features(const pdx_ctx *c)
     return (c->multi() ? PDX_FEAT_MULTI : 0) | (c->corr ? PDX_FEAT_SC : 0);
 }
 
+// A kernel whose table entries take `entry` LDS bytes stages the whole table if the longest
+// one fits the budget; otherwise (PDX_FEAT_CAP) it stages what fits and reads the rest from
+// global memory, so that two blocks still share a CU.  The capped variant is a separate
+// instantiation: its fallback branch costs the uncapped loop ~30 %.
+static bool lds_capped(const pdx_ctx *c, int entry)
+{
+    return (int64_t)std::max(c->halo_max, 1) * entry > pdx_lds_budget(c);
+}
+
+static size_t lds_bytes(const pdx_ctx *c, int entry)
+{
+    const int hl = lds_capped(c, entry) ? pdx_lds_entries(c->halo_max, pdx_lds_budget(c), entry)
+                                        : std::max(c->halo_max, 1);
+    return (size_t)hl * entry + 16;
+}
+
 template <int F>
 static void launch_bb_feat(pdx_ctx *c, dim3 G, dim3 BL, size_t lds, const pdx_step_args &a)
 {
     if (c->precision == PDX_MIXED) hipLaunchKernelGGL((k_bb_lds_mx<4, PDX_LDS_LANES, F>), G, BL, lds, c->stream, a);
     else hipLaunchKernelGGL((k_bb_lds<2, PDX_LDS_LANES, F>), G, BL, lds, c->stream, a);
 }
+
+// feature masks 1..7 (PDX_FEAT_MULTI | PDX_FEAT_SC | PDX_FEAT_CAP)
+#define PDX_FEAT_SWITCH(feat, CALL)                                                         \
+    switch (feat) {                                                                         \
+    case 1: CALL(1); break;                                                                 \
+    case 2: CALL(2); break;                                                                 \
+    case 3: CALL(3); break;                                                                 \
+    case 4: CALL(4); break;                                                                 \
+    case 5: CALL(5); break;                                                                 \
+    case 6: CALL(6); break;                                                                 \
+    case 7: CALL(7); break;                                                                 \
+    default: break;                                                                         \
+    }
 
 template <int F>
 static void launch_osb_feat(pdx_ctx *c, int pass, dim3 G, dim3 BL, size_t lds, const pdx_step_args &a)
@@ -1121,22 +1195,22 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
     case PDX_BB:
         PDX_TRY(pdx_timed_begin(c));
         if (c->lds_mode) {
-            const size_t lds = sizeof(double4) * (size_t)std::max(c->halo_max, 1);
+            const size_t lds = lds_bytes(c, 32);
             // four lanes per point (the halo table of a block limits the blocks per CU, so
             // the lanes, not the blocks, supply the parallelism that hides the bond loads)
             const dim3 BL(PDX_BLOCK * PDX_LDS_LANES);
             // bonds per lane and unrolled iteration: the FP32 bond arithmetic of the mixed
             // step is short enough that four loads in flight pay; FP64 is indifferent
             const bool u4 = c->bb_unroll ? c->bb_unroll == 4 : c->precision == PDX_MIXED;
-            const int feat = pdx_features(c);
             // with several parameter sets each neighbour-table entry also carries the
             // neighbour's set
-            const size_t ldsf = lds + (c->multi() ? (size_t)std::max(c->halo_max, 1) + 16 : 0);
-            if (feat == PDX_FEAT_MULTI) launch_bb_feat<PDX_FEAT_MULTI>(c, G, BL, ldsf, a);
-            else if (feat == PDX_FEAT_SC) launch_bb_feat<PDX_FEAT_SC>(c, G, BL, ldsf, a);
-            else if (feat == (PDX_FEAT_MULTI | PDX_FEAT_SC))
-                launch_bb_feat<PDX_FEAT_MULTI | PDX_FEAT_SC>(c, G, BL, ldsf, a);
-            else if (c->precision == PDX_MIXED) {
+            const int entry = 32 + (c->multi() ? 1 : 0);
+            const int feat = pdx_features(c) | (lds_capped(c, entry) ? PDX_FEAT_CAP : 0);
+            const size_t ldsf = lds_bytes(c, entry);
+#define PDX_BB_LAUNCH(F) launch_bb_feat<F>(c, G, BL, ldsf, a)
+            if (feat) {
+                PDX_FEAT_SWITCH(feat, PDX_BB_LAUNCH)
+            } else if (c->precision == PDX_MIXED) {
                 if (u4) hipLaunchKernelGGL((k_bb_lds_mx<4, PDX_LDS_LANES, 0>), G, BL, lds, c->stream, a);
                 else hipLaunchKernelGGL((k_bb_lds_mx<2, PDX_LDS_LANES, 0>), G, BL, lds, c->stream, a);
             } else {
@@ -1156,24 +1230,26 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
     case PDX_OSB:
         if (c->lds_mode) {
             const dim3 BL(PDX_BLOCK * PDX_LDS_LANES);
-            const size_t hm = (size_t)std::max(c->halo_max, 1);
-            const size_t l2 = (sizeof(double4) + sizeof(double2)) * hm;
-            const size_t tail = c->multi() ? hm + 16 : 0;  // the neighbours' parameter sets
-            const int feat = pdx_features(c);
+            const int mb = c->multi() ? 1 : 0;  // the neighbours' parameter sets
+            const size_t l1 = lds_bytes(c, 32 + mb), l2b = lds_bytes(c, 48 + mb);
+            const int f1 = pdx_features(c) | (lds_capped(c, 32 + mb) ? PDX_FEAT_CAP : 0);
+            const int f2 = pdx_features(c) | (lds_capped(c, 48 + mb) ? PDX_FEAT_CAP : 0);
+#define PDX_OSB1_LAUNCH(F) launch_osb_feat<F>(c, 1, G, BL, l1, a)
+#define PDX_OSB2_LAUNCH(F) launch_osb_feat<F>(c, 2, G, BL, l2b, a)
             PDX_TRY(pdx_timed_begin(c));
-            if (feat == PDX_FEAT_MULTI) launch_osb_feat<PDX_FEAT_MULTI>(c, 1, G, BL, sizeof(double4) * hm + tail, a);
-            else if (feat == PDX_FEAT_SC) launch_osb_feat<PDX_FEAT_SC>(c, 1, G, BL, sizeof(double4) * hm + tail, a);
-            else if (feat == (PDX_FEAT_MULTI | PDX_FEAT_SC))
-                launch_osb_feat<PDX_FEAT_MULTI | PDX_FEAT_SC>(c, 1, G, BL, sizeof(double4) * hm + tail, a);
-            else launch_osb_feat<0>(c, 1, G, BL, sizeof(double4) * hm, a);
+            if (f1) {
+                PDX_FEAT_SWITCH(f1, PDX_OSB1_LAUNCH)
+            } else {
+                launch_osb_feat<0>(c, 1, G, BL, l1, a);
+            }
             PDX_HIP(c, hipGetLastError());
             PDX_TRY(pdx_timed_end(c, "pass1"));
             PDX_TRY(pdx_timed_begin(c));
-            if (feat == PDX_FEAT_MULTI) launch_osb_feat<PDX_FEAT_MULTI>(c, 2, G, BL, l2 + tail, a);
-            else if (feat == PDX_FEAT_SC) launch_osb_feat<PDX_FEAT_SC>(c, 2, G, BL, l2 + tail, a);
-            else if (feat == (PDX_FEAT_MULTI | PDX_FEAT_SC))
-                launch_osb_feat<PDX_FEAT_MULTI | PDX_FEAT_SC>(c, 2, G, BL, l2 + tail, a);
-            else launch_osb_feat<0>(c, 2, G, BL, l2, a);
+            if (f2) {
+                PDX_FEAT_SWITCH(f2, PDX_OSB2_LAUNCH)
+            } else {
+                launch_osb_feat<0>(c, 2, G, BL, l2b, a);
+            }
             PDX_HIP(c, hipGetLastError());
             PDX_TRY(pdx_timed_end(c, "step"));
             break;

@@ -651,42 +651,57 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds1(pdx_step_args A)
         epsc_lo = Pi.epsc_lo;
         epsc_hi = Pi.epsc_hi;
     }
-    for (int t = sub; t < c; t += S) {
-        const int64_t o = (int64_t)t * 64;
-        const uint32_t wd = wbase[o];
-        const double Lb = Lbase[o];
-        const int slot = (int)(wd & PDX_W16_SLOT);
-        const bool in_lds = !CAP || slot < HL;
-        const int32_t jg = in_lds ? 0 : hidx[slot];
-        const double4 rj = in_lds ? hrec[slot] : A.rec_in[jg];
-        const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
-        double rl;
-        const double l = pdx_sqrt_rsq((dx * dx + dy * dy) + dz * dz, rl);
-        const double e = l - Lb;
-        const double rL = pdx_rcp1(Lb);
-        bool act;
-        if (MULTI) {
-            // The neighbour fails its own copy of the bond against its eps_c; the stretch is
-            // the same number at both ends, so the owner tracks that copy's state (RACT)
-            // exactly, which the gather of pass 2 needs.
-            const double eps = e * rL;
-            const pdx_pset Pj = A.psets[in_lds ? hpid[slot] : A.pid[jg]];
-            uint32_t nw = wd;
-            if ((wd & PDX_W16_ACT) && (wd & PDX_W16_FP) && eps >= epsc_lo)
-                if ((eps > epsc_hi) ? true : ((e / Lb) > epsc)) nw &= ~PDX_W16_ACT;
-            if ((wd & PDX_W16_RACT) && (wd & PDX_W16_FP) && eps >= Pj.epsc_lo)
-                if ((eps > Pj.epsc_hi) ? true : ((e / Lb) > Pj.epsc)) nw &= ~PDX_W16_RACT;
-            if (nw != wd) wbase[o] = (uint16_t)nw;
-            act = (nw & PDX_W16_ACT) != 0;
-        } else {
-            act = bond_fail16(wbase + o, wd, e, Lb, e * rL, epsc, epsc_lo, epsc_hi);
+    // two bonds per lane in flight: their words and lengths are loaded before either is used
+    constexpr int U = 2;
+    for (int t0 = sub; t0 < c; t0 += U * S) {
+        uint32_t wdu[U];
+        double Lu[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const bool ok = t0 + u * S < c;
+            wdu[u] = ok ? wbase[(int64_t)(t0 + u * S) * 64] : 0u;
+            Lu[u] = ok ? Lbase[(int64_t)(t0 + u * S) * 64] : 1.0;
         }
-        if (act) {
-            double w = A.delta * rL;  // linear_kernel(delta, L)
-            if (SC) w *= A.scf[b0 + o];  // omega * beta
-            wvol += (w * (Lb * Lb)) * rj.w;
-            Ssum += ((w * Lb) * e) * rj.w;
-            na++;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = t0 + u * S;
+            if (t >= c) continue;
+            const int64_t o = (int64_t)t * 64;
+            const uint32_t wd = wdu[u];
+            const double Lb = Lu[u];
+            const int slot = (int)(wd & PDX_W16_SLOT);
+            const bool in_lds = !CAP || slot < HL;
+            const int32_t jg = in_lds ? 0 : hidx[slot];
+            const double4 rj = in_lds ? hrec[slot] : A.rec_in[jg];
+            const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
+            double rl;
+            const double l = pdx_sqrt_rsq((dx * dx + dy * dy) + dz * dz, rl);
+            const double e = l - Lb;
+            const double rL = pdx_rcp1(Lb);
+            bool act;
+            if (MULTI) {
+                // The neighbour fails its own copy of the bond against its eps_c; the stretch is
+                // the same number at both ends, so the owner tracks that copy's state (RACT)
+                // exactly, which the gather of pass 2 needs.
+                const double eps = e * rL;
+                const pdx_pset Pj = A.psets[in_lds ? hpid[slot] : A.pid[jg]];
+                uint32_t nw = wd;
+                if ((wd & PDX_W16_ACT) && (wd & PDX_W16_FP) && eps >= epsc_lo)
+                    if ((eps > epsc_hi) ? true : ((e / Lb) > epsc)) nw &= ~PDX_W16_ACT;
+                if ((wd & PDX_W16_RACT) && (wd & PDX_W16_FP) && eps >= Pj.epsc_lo)
+                    if ((eps > Pj.epsc_hi) ? true : ((e / Lb) > Pj.epsc)) nw &= ~PDX_W16_RACT;
+                if (nw != wd) wbase[o] = (uint16_t)nw;
+                act = (nw & PDX_W16_ACT) != 0;
+            } else {
+                act = bond_fail16(wbase + o, wd, e, Lb, e * rL, epsc, epsc_lo, epsc_hi);
+            }
+            if (act) {
+                double w = A.delta * rL;  // linear_kernel(delta, L)
+                if (SC) w *= A.scf[b0 + o];  // omega * beta
+                wvol += (w * (Lb * Lb)) * rj.w;
+                Ssum += ((w * Lb) * e) * rj.w;
+                na++;
+            }
         }
     }
 #pragma unroll
@@ -759,48 +774,62 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds2(pdx_step_args A)
         Ki = Pi.K;
         rho = Pi.rho;
     }
-    for (int t = sub; t < c; t += S) {
-        const int64_t o = (int64_t)t * 64;
-        const uint32_t wd = wbase[o];
-        const double Lb = Lbase[o];
-        const bool act = (wd & PDX_W16_ACT) != 0;
-        const bool ract = MULTI ? (wd & PDX_W16_RACT) != 0 : act;  // one set: both copies agree
-        if (!act && !ract) continue;
-        const int slot = (int)(wd & PDX_W16_SLOT);
-        const bool in_lds = !CAP || slot < HL;
-        const int32_t jg = in_lds ? 0 : hidx[slot];
-        const double4 rj = in_lds ? hrec[slot] : A.rec_in[jg];
-        const double2 cj = in_lds ? hc[slot] : cst[jg];
-        const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
-        double rl;
-        const double l = pdx_sqrt_rsq((dx * dx + dy * dy) + dz * dz, rl);
-        double w = A.delta * pdx_rcp1(Lb);
-        if (SC) w *= A.scf[b0 + o];  // beta is symmetric: both ends' copies of the bond carry it
-        const double e = l - Lb;
-        double fi, fj;
-        if (MULTI) {
-            const pdx_pset Pj = A.psets[in_lds ? hpid[slot] : A.pid[jg]];
-            const double Gs = Gi + Pj.G, Ks = Ki + Pj.K;
-            fi = 0.0;
-            fj = 0.0;
-            if (act && ci.x != 0.0) {
-                const double c1 = (15.0 * Gs) / (2.0 * ci.x);
-                const double c2 = ci.y * ((3.0 * Ks) / (2.0 * ci.x) - c1 / 3.0);
-                fi = c2 * Lb + c1 * e;
-            }
-            if (ract && cj.x != 0.0) {
-                const double c1 = (15.0 * Gs) / (2.0 * cj.x);
-                const double c2 = cj.y * ((3.0 * Ks) / (2.0 * cj.x) - c1 / 3.0);
-                fj = c2 * Lb + c1 * e;
-            }
-        } else {
-            fi = ci.y * Lb + ci.x * e;
-            fj = cj.y * Lb + cj.x * e;
+    constexpr int U = 2;  // two bonds per lane in flight
+    for (int t0 = sub; t0 < c; t0 += U * S) {
+        uint32_t wdu[U];
+        double Lu[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const bool ok = t0 + u * S < c;
+            wdu[u] = ok ? wbase[(int64_t)(t0 + u * S) * 64] : 0u;
+            Lu[u] = ok ? Lbase[(int64_t)(t0 + u * S) * 64] : 1.0;
         }
-        const double s = (w * rl) * ((fi + fj) * rj.w);
-        bx = __builtin_fma(s, dx, bx);
-        by = __builtin_fma(s, dy, by);
-        bz = __builtin_fma(s, dz, bz);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = t0 + u * S;
+            if (t >= c) continue;
+            const int64_t o = (int64_t)t * 64;
+            const uint32_t wd = wdu[u];
+            const double Lb = Lu[u];
+            const bool act = (wd & PDX_W16_ACT) != 0;
+            const bool ract = MULTI ? (wd & PDX_W16_RACT) != 0 : act;  // one set: both copies agree
+            if (!act && !ract) continue;
+            const int slot = (int)(wd & PDX_W16_SLOT);
+            const bool in_lds = !CAP || slot < HL;
+            const int32_t jg = in_lds ? 0 : hidx[slot];
+            const double4 rj = in_lds ? hrec[slot] : A.rec_in[jg];
+            const double2 cj = in_lds ? hc[slot] : cst[jg];
+            const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
+            double rl;
+            const double l = pdx_sqrt_rsq((dx * dx + dy * dy) + dz * dz, rl);
+            double w = A.delta * pdx_rcp1(Lb);
+            if (SC) w *= A.scf[b0 + o];  // beta is symmetric: both ends' copies of the bond carry it
+            const double e = l - Lb;
+            double fi, fj;
+            if (MULTI) {
+                const pdx_pset Pj = A.psets[in_lds ? hpid[slot] : A.pid[jg]];
+                const double Gs = Gi + Pj.G, Ks = Ki + Pj.K;
+                fi = 0.0;
+                fj = 0.0;
+                if (act && ci.x != 0.0) {
+                    const double c1 = (15.0 * Gs) / (2.0 * ci.x);
+                    const double c2 = ci.y * ((3.0 * Ks) / (2.0 * ci.x) - c1 / 3.0);
+                    fi = c2 * Lb + c1 * e;
+                }
+                if (ract && cj.x != 0.0) {
+                    const double c1 = (15.0 * Gs) / (2.0 * cj.x);
+                    const double c2 = cj.y * ((3.0 * Ks) / (2.0 * cj.x) - c1 / 3.0);
+                    fj = c2 * Lb + c1 * e;
+                }
+            } else {
+                fi = ci.y * Lb + ci.x * e;
+                fj = cj.y * Lb + cj.x * e;
+            }
+            const double s = (w * rl) * ((fi + fj) * rj.w);
+            bx = __builtin_fma(s, dx, bx);
+            by = __builtin_fma(s, dy, by);
+            bz = __builtin_fma(s, dz, bz);
+        }
     }
 #pragma unroll
     for (int o = 1; o < S; o <<= 1) {

@@ -62,6 +62,21 @@ def step_bytes(n_points, n_bonds, precision, layout):
             f"{per_bond}*N_b + 168*N_p + 8*N_slices")
 
 
+def reference_layout_bytes(kind, n_points, n_bonds, precision="fp64"):
+    """Compulsory HBM bytes of one step in the reference's own data layout (Vector{Bond}
+    with Int32 neighbour + FP64 length, a Bool bond state, and the per-point Verlet
+    vectors written every step), as a yardstick that does not depend on this library's
+    layout: BB 13 B per bond + 216 B per point (mixed: FP32 length, 9 B per bond); OSB two
+    bond passes + a reverse-bond index, 30 B per bond + 248 B per point; CPD 30 B per bond
+    + 576 B per point."""
+    if kind == "BB":
+        return ((13 if precision == "fp64" else 9) * n_bonds + 216 * n_points,
+                f"{13 if precision == 'fp64' else 9}*N_b + 216*N_p")
+    if kind == "OSB":
+        return 30 * n_bonds + 248 * n_points, "30*N_b + 248*N_p"
+    return 30 * n_bonds + 576 * n_points, "30*N_b + 576*N_p"
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "bb_step_2M_hbm_traffic.md")
 
 
@@ -193,13 +208,16 @@ def state_based_leg(kind, shape, steps, warmup=10, dx=1e-3):
     ctx.set_profiling(False)
     nb, npnt = dh.n_bonds, body.n_points
     byt, formula = state_based_bytes(kind, npnt, nb, ctx.layout())
+    rbyt, rformula = reference_layout_bytes(kind, npnt, nb)
     dh.close()
     return {"workload": f"{kind} torsion bar {shape[0]}x{shape[1]}x{shape[2]}, 10 rad/s, FP64",
             "points": npnt, "bonds": nb, "steps": steps, "ms_per_step": el / steps * 1e3,
             "bond_updates_per_s": nb * steps / el, "pass1_ms": k1, "pass2_ms": k2,
             "bytes_formula": formula, "bytes_per_step": byt,
             "achieved_GBs": byt / ((k1 + k2) * 1e-3) / 1e9,
-            "frac_hbm": byt / ((k1 + k2) * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            "frac_hbm": byt / ((k1 + k2) * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "reference_layout_bytes_formula": rformula,
+            "frac_hbm_reference_layout": rbyt / ((k1 + k2) * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
 
 def osb_2m_leg(shape, steps, warmup=10):
@@ -233,6 +251,7 @@ def osb_2m_leg(shape, steps, warmup=10):
     ctx.set_profiling(False)
     nb, npnt = dh.n_bonds, body.n_points
     byt, formula = state_based_bytes("OSB", npnt, nb, ctx.layout())
+    rbyt, rformula = reference_layout_bytes("OSB", npnt, nb)
     dh.close()
     return {"workload": f"OSB Velocity Verlet {shape[0]}x{shape[1]}x{shape[2]} jittered lattice, "
                         f"+-0.6 m/s y-BCs, FP64",
@@ -240,7 +259,9 @@ def osb_2m_leg(shape, steps, warmup=10):
             "bond_updates_per_s": nb * steps / el, "pass1_ms": k1, "pass2_ms": k2,
             "bytes_formula": formula, "bytes_per_step": byt,
             "achieved_GBs": byt / ((k1 + k2) * 1e-3) / 1e9,
-            "frac_hbm": byt / ((k1 + k2) * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            "frac_hbm": byt / ((k1 + k2) * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "reference_layout_bytes_formula": rformula,
+            "frac_hbm_reference_layout": rbyt / ((k1 + k2) * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
 
 def mode_i_leg(npxy, nz, steps):
@@ -487,7 +508,12 @@ def main():
                      "kernel": "k_bb_lds" if layout["lds_mode"] else "k_bb_step",
                      "kernel_ms": kms, "bytes_formula": bytes_formula,
                      "bytes_per_launch": bytes_step, "halo_entries": layout["halo_entries"],
-                     "bond_slots": layout["bond_slots"]},
+                     "halo_max": layout.get("halo_max"), "bond_slots": layout["bond_slots"],
+                     "reference_layout_bytes_formula": reference_layout_bytes(
+                         "BB", n_points, n_bonds, args.precision)[1],
+                     "frac_reference_layout": reference_layout_bytes(
+                         "BB", n_points, n_bonds, args.precision)[0] / (kms * 1e-3) / 1e9
+                     / HBM_PEAK_GBS},
     }
     dh.close()
 
@@ -521,6 +547,8 @@ def main():
                         "speedup_vs_fp64_kernel": kms / kmm if kmm else None,
                         "achieved_GBs": mbytes / (kmm * 1e-3) / 1e9,
                         "frac_hbm": mbytes / (kmm * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                        "frac_hbm_reference_layout": reference_layout_bytes(
+                            "BB", n_points, n_bonds, "mixed")[0] / (kmm * 1e-3) / 1e9 / HBM_PEAK_GBS,
                         "bytes_formula": mformula,
                         "final_displacement_rel_error_vs_fp64": float(
                             np.linalg.norm(um - u) / np.linalg.norm(u)),

@@ -12,7 +12,11 @@
 #   calc_stable_timestep(dh, sf)        src/time_solvers/velocity_verlet.jl:136-163
 #   initialize!(dh, solver)             src/core/data_handler.jl:1-4
 #   solve!(dh, ::VelocityVerlet, opts)  src/time_solvers/velocity_verlet.jl:165-197
+#   solve!(dh, ::DynamicRelaxation, o)  src/time_solvers/dynamic_relaxation.jl:92-107
 #   solve!(dh, ::NewtonKrylov, opts)    src/time_solvers/newton_krylov.jl:280-332
+#   export_results / export_reference_results  src/core/threads_body_data_handler.jl:197-212
+#     (device snapshot through pdx_export_begin/wait, files by the package's own
+#      pvtk_grid writer as _export_results does, src/auxiliary/io.jl:157-167)
 #
 # Status codes map to the package's exceptions: PDX_ERR_ARG -> ArgumentError,
 # PDX_ERR_NAN -> NaNError(t, step) (src/auxiliary/errors.jl:98-114), everything else ->
@@ -21,7 +25,9 @@ module PeridynamicsMI355X
 
 using Peridynamics
 import Peridynamics: calc_force_density!, calc_stable_timestep, initialize!, solve!,
-                     init_time_solver!, AbstractDataHandler, AbstractJobOptions, NaNError
+                     init_time_solver!, AbstractDataHandler, AbstractJobOptions, NaNError,
+                     EnergySurfaceCorrection, correction_type, get_filename
+using WriteVTK: pvtk_grid, MeshCell, VTKCellTypes, VTKPointData, VTKFieldData
 
 export MI355XBodyDataHandler, submit_mi355x, get_field, set_field!
 
@@ -91,14 +97,32 @@ model_id(::CMaterial{<:SaintVenantKirchhoff}) = Int32(0)
 model_id(::CMaterial{<:NeoHookePenalty}) = Int32(1)
 model_id(_) = Int32(0)
 
-function pdx_params(body)
-    length(body.point_params) == 1 ||
-        throw(ArgumentError("the MI355X data handler takes one parameter set per body"))
-    p = body.point_params[1]
+function pdx_params(body, p)
     cs = body.mat isa CMaterial ? body.mat.zem.Cs : 0.0
     maxdmg = body.mat isa CMaterial ? body.mat.maxdmg : 0.0
     return PdxParams(p.δ, p.rho, p.E, p.nu, p.G, p.K, p.λ, p.μ, p.Gc, p.εc, p.bc,
                      model_id(body.mat), Int32(0), cs, maxdmg)
+end
+
+# one parameter set: pdx_set_material; several (point_params + params_map):
+# pdx_set_material_sets with the 0-based set of every point
+function set_material!(ctx, body)
+    sets = [pdx_params(body, p) for p in body.point_params]
+    if length(sets) == 1
+        check(ctx, ccall((:pdx_set_material, libpdx), Int32, (Ptr{Cvoid}, Int32, Ref{PdxParams}),
+                         ctx, material_kind(body.mat), Ref(sets[1])))
+    else
+        check(ctx, ccall((:pdx_set_material_sets, libpdx), Int32,
+                         (Ptr{Cvoid}, Int32, Int32, Ptr{PdxParams}, Ptr{Int32}),
+                         ctx, material_kind(body.mat), length(sets), sets,
+                         Int32.(body.params_map .- 1)))
+    end
+    if body.mat isa Union{BBMaterial,OSBMaterial} &&
+       correction_type(body.mat) === EnergySurfaceCorrection
+        check(ctx, ccall((:pdx_set_surface_correction, libpdx), Int32, (Ptr{Cvoid}, Int32),
+                         ctx, Int32(1)))
+    end
+    return nothing
 end
 
 zero_based(v) = Int32.(v .- 1)
@@ -114,9 +138,7 @@ function MI355XBodyDataHandler(body::AbstractBody; device::Int=0, precision::Sym
     check(ctx, ccall((:pdx_set_points, libpdx), Int32,
                      (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{UInt8}),
                      ctx, n, body.position, body.volume, fp))
-    params = Ref(pdx_params(body))
-    check(ctx, ccall((:pdx_set_material, libpdx), Int32, (Ptr{Cvoid}, Int32, Ref{PdxParams}),
-                     ctx, material_kind(body.mat), params))
+    set_material!(ctx, body)
     for pc in body.point_sets_precracks
         a = zero_based(body.point_sets[pc.set_a])
         b = zero_based(body.point_sets[pc.set_b])
@@ -216,13 +238,54 @@ end
 
 initialize!(dh::MI355XBodyDataHandler, ::AbstractTimeSolver) = calc_force_density!(dh, 0.0, 0.0)
 
+# export_results: the snapshot of step n is taken on the device and copied while the next
+# steps run; the files of the previous snapshot are written meanwhile.  `pending` holds the
+# (step, time) of the snapshot in flight.
+const EXPORT_PENDING = IdDict{Any,Any}()
+
 function export_fields(dh, options, n, t)
-    Peridynamics.export_results(dh, options, n, t)  # see "Export" in INTEGRATION.md
+    options isa Peridynamics.JobOptions && options.export_allowed || return nothing
+    names = vcat(:position, [f for f in options.fields if f !== :position])
+    ids = Int32[FIELDS[f][1] for f in names]
+    check(dh, ccall((:pdx_export_begin, libpdx), Int32, (Ptr{Cvoid}, Int32, Ptr{Int32}, Int64),
+                    dh.ctx, length(ids), ids, n))
+    prev = get(EXPORT_PENDING, dh, nothing)
+    EXPORT_PENDING[dh] = (names, t)
+    prev === nothing || write_export(dh, options, prev...)
+    return nothing
+end
+
+function write_export(dh, options, names, t)
+    ptrs = Vector{Ptr{Float64}}(undef, 16)
+    tag, slot = Ref{Int64}(0), Ref{Int32}(0)
+    check(dh, ccall((:pdx_export_wait, libpdx), Int32,
+                    (Ptr{Cvoid}, Ref{Int64}, Ref{Int32}, Ptr{Ptr{Float64}}),
+                    dh.ctx, tag, slot, ptrs))
+    n = dh.n_points
+    arr(k) = (nc = FIELDS[names[k]][2];
+              nc == 1 ? copy(unsafe_wrap(Array, ptrs[k], n)) :
+                        copy(unsafe_wrap(Array, ptrs[k], (nc, n))))
+    cells = [MeshCell(VTKCellTypes.VTK_VERTEX, (j,)) for j in 1:n]
+    filename = get_filename(options, Symbol(""), Int(tag[]))
+    pvtk_grid(filename, arr(1), cells; part=1, nparts=1) do vtk
+        for k in 2:length(names)
+            vtk[string(names[k]), VTKPointData()] = arr(k)
+        end
+        vtk["time", VTKFieldData()] = t
+    end
+    return nothing
+end
+
+function finish_exports(dh, options)
+    prev = pop!(EXPORT_PENDING, dh, nothing)
+    prev === nothing || write_export(dh, options, prev...)
+    return nothing
 end
 
 function solve!(dh::MI355XBodyDataHandler, vv::VelocityVerlet, options::AbstractJobOptions)
     Δt = vv.Δt
     freq = options isa Peridynamics.JobOptions && options.export_allowed ? options.freq : 0
+    freq > 0 && export_fields(dh, options, 0, 0.0)  # export_reference_results
     n = 1
     while n <= vv.n_steps
         # run to the next export step (or the end) in one call: one fused launch per step
@@ -237,6 +300,29 @@ function solve!(dh::MI355XBodyDataHandler, vv::VelocityVerlet, options::Abstract
         freq > 0 && last % freq == 0 && export_fields(dh, options, last, last * Δt)
         n = last + 1
     end
+    finish_exports(dh, options)
+    return dh
+end
+
+function solve!(dh::MI355XBodyDataHandler, dr::DynamicRelaxation, options::AbstractJobOptions)
+    Δt = dr.Δt
+    check(dh, ccall((:pdx_dr_init, libpdx), Int32, (Ptr{Cvoid}, Float64, Float64),
+                    dh.ctx, Δt, dr.Λ))
+    freq = options isa Peridynamics.JobOptions && options.export_allowed ? options.freq : 0
+    freq > 0 && export_fields(dh, options, 0, 0.0)
+    n = 1
+    while n <= dr.n_steps
+        last = freq > 0 ? min(dr.n_steps, (div(n - 1, freq) + 1) * freq) : dr.n_steps
+        vals = bc_values(dh, n:last, Δt)
+        nan_step = Ref{Int64}(0)
+        st = ccall((:pdx_dr_run, libpdx), Int32,
+                   (Ptr{Cvoid}, Int64, Int64, Float64, Ptr{Float64}, Ref{Int64}),
+                   dh.ctx, n, last - n + 1, Δt, vals, nan_step)
+        check(dh, st; t=nan_step[] * Δt, step=nan_step[])
+        freq > 0 && last % freq == 0 && export_fields(dh, options, last, last * Δt)
+        n = last + 1
+    end
+    finish_exports(dh, options)
     return dh
 end
 
@@ -256,7 +342,7 @@ function solve!(dh::MI355XBodyDataHandler, nr::NewtonKrylov, options::AbstractJo
     check(dh, ccall((:pdx_set_displacement_bcs, libpdx), Int32,
                     (Ptr{Cvoid}, Int64, Ptr{Int32}, Ptr{Int32}, Ptr{Float64}),
                     dh.ctx, length(pts), pts, dims, vals))
-    pre = Dict(:none => 0, :jacobi => 1, :learned => 2)[precond]
+    pre = Dict(:none => 0, :jacobi => 1, :learned => 2, :two_level => 3)[precond]
     opts = Ref(PdxNkOpts(nr.maxiter, nr.gmres_maxiter, nr.gmres_restart, pre, nr.tol, rel_tol,
                          nr.gmres_reltol, nr.gmres_abstol, nr.perturbation_scale))
     for n in 1:nr.n_steps
@@ -278,6 +364,7 @@ function solve!(dh::MI355XBodyDataHandler, nr::NewtonKrylov, options::AbstractJo
         check(dh, st)
         export_fields(dh, options, n, n * nr.Δt)
     end
+    finish_exports(dh, options)
     return dh
 end
 
@@ -294,6 +381,7 @@ function init_time_solver!(vv::VelocityVerlet, dh::MI355XBodyDataHandler)
 end
 
 init_time_solver!(::NewtonKrylov, ::MI355XBodyDataHandler) = nothing
+init_time_solver!(::DynamicRelaxation, ::MI355XBodyDataHandler) = nothing
 
 """
     submit_mi355x(job::Job; device=0, precision=:fp64)

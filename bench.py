@@ -261,7 +261,8 @@ def osb_2m_leg(shape, steps, warmup=10):
             "achieved_GBs": byt / ((k1 + k2) * 1e-3) / 1e9,
             "frac_hbm": byt / ((k1 + k2) * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "reference_layout_bytes_formula": rformula,
-            "frac_hbm_reference_layout": rbyt / ((k1 + k2) * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            "frac_hbm_reference_layout": rbyt / ((k1 + k2) * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "traffic": measured_traffic("osb", npnt, nb)}
 
 
 def mode_i_leg(npxy, nz, steps):

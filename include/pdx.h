@@ -252,6 +252,14 @@ pdx_status pdx_gnn_pairs(pdx_ctx *ctx, int64_t *n_newton, int64_t *n_krylov);
  * the collected ones (and random combinations of two); *loss_out = last batch loss. */
 pdx_status pdx_gnn_train(pdx_ctx *ctx, int32_t steps, int32_t batch, double lr, uint64_t seed,
                          double *loss_out);
+/* Held-out validation of the trained network (no reference counterpart).  pdx_gnn_train
+ * keeps the last tenth of the Krylov pairs out of training and measures on them the loss
+ * ||c s O - e||^2 / ||e||^2 of the two-level preconditioner's error e (1 = the network
+ * explains nothing).  The learned preconditioner applies the network only while that loss
+ * is below `threshold` (0.85 after training) and until a learned solve needs the fallback
+ * (pdx_nk_stats.precond_fallbacks); otherwise it is the two-level preconditioner alone.
+ * threshold NaN only reports. */
+pdx_status pdx_gnn_gate(pdx_ctx *ctx, double threshold, double *val_loss, int32_t *use_network);
 /* Number of float32 parameters of the network. */
 int64_t pdx_gnn_n_params(void);
 /* Reads (theta_out) or replaces (theta_in, with the output scale *out_scale) the

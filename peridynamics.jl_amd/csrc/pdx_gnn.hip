@@ -51,6 +51,9 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 #define OFF_BD (OFF_WD + GF * 3)
 #define G_NPARAM (OFF_BD + 3)
 
+// held-out loss below which the trained network is applied (1 = it explains nothing)
+#define PDX_GNN_GATE 0.85
+
 struct pdx_gnn {
     int64_t n = 0, npad = 0, nb1 = 0, nb2 = 0;
     // graph and static features
@@ -62,6 +65,11 @@ struct pdx_gnn {
     // parameters
     bool trained = false;
     double c_out = 1.0;
+    // validation: the last tenth of the Krylov pairs is held out of training; the network is
+    // applied only if it reduces the held-out error of the two-level preconditioner enough
+    int64_t n_k_val = 0;
+    double val_loss = -1.0;
+    bool use_net = true;
     float *theta = nullptr, *grad = nullptr, *am = nullptr, *av = nullptr;
     uint16_t *wT[GL] = {}, *wN[GL] = {};  // bf16: out-major [64][320], in-major [320][64]
     int64_t adam_t = 0;
@@ -1012,8 +1020,12 @@ pdx_status pdx_gnn_prepare(pdx_ctx *c)
 
 // the learned preconditioner of the Krylov solver: two-level coarse correction plus the
 // network's block-Jacobi-based output, z = P A_c^-1 P^T v + D^-1 v + c s O(v)
+pdx_status pdx_nk_two_level(pdx_ctx *c, const double *v, double *z);
+
 pdx_status pdx_gnn_apply(pdx_ctx *c, const double *v, double *z)
 {
+    // a network that failed validation is not applied: the two-level preconditioner alone
+    if (!c->gnn->use_net && c->nk->coarse_ready) return pdx_nk_two_level(c, v, z);
     PDX_TRY(gnn_forward(c, v, nullptr, z, false, 0.0));
     if (c->nk->coarse_ready) PDX_TRY(pdx_nk_coarse_add(c, v, z, 1.0));
     return PDX_OK;
@@ -1073,6 +1085,11 @@ pdx_status pdx_gnn_record_krylov(pdx_ctx *c, const double *z, const double *jz)
 }
 
 bool pdx_gnn_collecting(pdx_ctx *c) { return c->gnn && c->gnn->collecting; }
+void pdx_gnn_disable(pdx_ctx *c)
+{
+    if (c->gnn) c->gnn->use_net = false;
+}
+bool pdx_gnn_active(pdx_ctx *c) { return c->gnn && c->gnn->trained && c->gnn->use_net; }
 
 // ---------------------------------------------------------------------------- C ABI
 static pdx_status gnn_ready_for_io(pdx_ctx *c)
@@ -1119,7 +1136,7 @@ static pdx_status gnn_draw(pdx_ctx *c, std::mt19937_64 &rng)
             Y = g->Ny + i * m;
             X = g->Nx + i * m;
         } else {
-            const int64_t i = (int64_t)(rng() % (uint64_t)g->n_k);
+            const int64_t i = (int64_t)(rng() % (uint64_t)(g->n_k - g->n_k_val));
             Y = g->Ky + i * m;
             X = g->Kx + i * m;
         }
@@ -1186,6 +1203,9 @@ extern "C" pdx_status pdx_gnn_train(pdx_ctx *c, int32_t steps, int32_t batch, do
     // the network learns what the two-level preconditioner leaves: x - P A_c^-1 P^T y
     PDX_TRY(pdx_nk_build_coarse(c));
     std::mt19937_64 rng(seed);
+    // hold out the last tenth of the Krylov pairs (if there are enough) for validation
+    g->n_k_val = g->n_k >= 20 ? g->n_k / 10 : 0;
+    if (g->n_n == 0 && g->n_k - g->n_k_val <= 0) g->n_k_val = 0;
     if (!g->trained) PDX_TRY(gnn_init_params(c, g, seed));
     PDX_TRY(gnn_estimate_c(c, rng));
     const float b1 = 0.9f, b2 = 0.999f;
@@ -1215,6 +1235,34 @@ extern "C" pdx_status pdx_gnn_train(pdx_ctx *c, int32_t steps, int32_t batch, do
     PDX_HIP(c, hipStreamSynchronize(c->stream));
     g->trained = true;
     if (loss_out) *loss_out = last;
+    // held-out loss ||c s O - e||^2 / ||e||^2 of the two-level error e (1 = no gain)
+    g->val_loss = last;
+    if (g->n_k_val > 0) {
+        const int64_t m = 3 * c->n;
+        PDX_HIP(c, hipMemsetAsync(g->scal + SC_LOSS, 0, sizeof(double), c->stream));
+        for (int64_t i = g->n_k - g->n_k_val; i < g->n_k; i++) {
+            hipLaunchKernelGGL(k_gnn_mix, g1(m), dim3(256), 0, c->stream, m, g->Ky + i * m, g->Kx + i * m,
+                               (const float *)nullptr, (const float *)nullptr, 1.0, 0.0, g->ys, g->xs);
+            PDX_TRY(pdx_nk_coarse_add(c, g->ys, g->xs, -1.0));
+            PDX_TRY(gnn_forward(c, g->ys, g->xs, nullptr, true, 1.0 / (double)g->n_k_val));
+        }
+        PDX_HIP(c, hipMemcpyAsync(&g->val_loss, g->scal + SC_LOSS, sizeof(double), hipMemcpyDeviceToHost,
+                                  c->stream));
+        PDX_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    g->use_net = g->val_loss < PDX_GNN_GATE;
+    return PDX_OK;
+}
+
+extern "C" pdx_status pdx_gnn_gate(pdx_ctx *c, double threshold, double *val_loss, int32_t *use_network)
+{
+    if (!c) return PDX_ERR_ARG;
+    PDX_TRY(gnn_ready_for_io(c));
+    pdx_gnn *g = c->gnn;
+    if (!g->trained) return pdx_fail(c, PDX_ERR_STATE, "the network is not trained (pdx_gnn_train)");
+    if (threshold == threshold) g->use_net = g->val_loss < threshold;  // NaN: only report
+    if (val_loss) *val_loss = g->val_loss;
+    if (use_network) *use_network = g->use_net ? 1 : 0;
     return PDX_OK;
 }
 

@@ -37,6 +37,8 @@ pdx_status pdx_gnn_apply(pdx_ctx *c, const double *v, double *z);        // z = 
 pdx_status pdx_gnn_record(pdx_ctx *c, const double *rhs, const double *du);  // Newton pair
 pdx_status pdx_gnn_record_krylov(pdx_ctx *c, const double *z, const double *jz);  // Krylov pair
 bool pdx_gnn_collecting(pdx_ctx *c);
+void pdx_gnn_disable(pdx_ctx *c);  // the learned preconditioner falls back to two-level
+bool pdx_gnn_active(pdx_ctx *c);   // the network is applied (trained, validated, not disabled)
 pdx_status pdx_nk_build_jacobi(pdx_ctx *c);  // point-block Jacobi at the current state
 pdx_status pdx_nk_build_coarse(pdx_ctx *c);  // two-level coarse operator at the current state
 pdx_status pdx_nk_coarse_add(pdx_ctx *c, const double *v, double *z, double alpha);  // z += a P Ac^-1 P^T v

@@ -876,6 +876,9 @@ extern "C" pdx_status pdx_nk_step(pdx_ctx *c, int64_t n, int64_t n_steps, const 
         double rel = 0.0;
         if (pre == PDX_PRE_LEARNED) {
             PDX_TRY(pdx_gnn_prepare(c));
+            // with the network switched off the learned preconditioner is the two-level one:
+            // nothing to fall back to
+            const bool net = pdx_gnn_active(c) || !k->coarse_ready;
             PDX_TRY(fgmres(c, k->t0, o->gmres_restart, o->gmres_maxiter, o->gmres_reltol,
                            o->gmres_abstol, PDX_PRE_LEARNED, scale, &its, &rel));
             S.krylov_iters += its;
@@ -883,8 +886,10 @@ extern "C" pdx_status pdx_nk_step(pdx_ctx *c, int64_t n, int64_t n_steps, const 
             // the linear residual tenfold is redone with the preconditioner the network
             // corrects (two-level; block Jacobi if the coarse space is unavailable)
             const double tol_lin = std::max(o->gmres_reltol, o->gmres_abstol / std::max(rn, 1e-300));
-            if (!(rel <= tol_lin * 1.0000001) && !(rel <= 0.1)) {
+            if (net && !(rel <= tol_lin * 1.0000001) && !(rel <= 0.1)) {
                 S.precond_fallbacks++;
+                // a network that fails a solve is not applied again in this context
+                pdx_gnn_disable(c);
                 const int fb = k->coarse_ready ? PDX_PRE_TWO_LEVEL : PDX_PRE_JACOBI;
                 PDX_TRY(fgmres(c, k->t0, o->gmres_restart, o->gmres_maxiter, o->gmres_reltol,
                                o->gmres_abstol, fb, scale, &its, &rel));

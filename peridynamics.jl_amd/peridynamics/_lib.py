@@ -139,6 +139,7 @@ SIGNATURES = {
     "pdx_gnn_collect": (_ST, [_P, _I32, _I64]),
     "pdx_gnn_pairs": (_ST, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     "pdx_gnn_train": (_ST, [_P, _I32, _I32, _D, ctypes.c_uint64, ctypes.POINTER(_D)]),
+    "pdx_gnn_gate": (_ST, [_P, _D, ctypes.POINTER(_D), ctypes.POINTER(_I32)]),
     "pdx_gnn_n_params": (_I64, []),
     "pdx_gnn_params": (_ST, [_P, _P, _P, ctypes.POINTER(_D)]),
     "pdx_gnn_graph": (_ST, [_P, ctypes.POINTER(_I64), _P, _P, _P, _P, _P, _P]),
@@ -391,6 +392,13 @@ class Context:
         a, b = _I64(), _I64()
         self.call("pdx_gnn_pairs", ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
+
+    def gnn_gate(self, threshold=float("nan")):
+        """(held-out loss, network applied) of the trained network; a threshold sets the
+        gate (pdx_gnn_gate)."""
+        vl, use = _D(), _I32()
+        self.call("pdx_gnn_gate", float(threshold), ctypes.byref(vl), ctypes.byref(use))
+        return vl.value, bool(use.value)
 
     def gnn_train(self, steps, batch=4, lr=1e-3, seed=0):
         loss = _D()

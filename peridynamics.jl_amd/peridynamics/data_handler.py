@@ -289,8 +289,10 @@ class MI355XBodyDataHandler:
         t0 = time.perf_counter()
         pairs = self.ctx.gnn_pairs()
         loss = self.ctx.gnn_train(nk.train_steps, nk.train_batch, nk.train_lr, nk.train_seed)
+        val_loss, use_net = self.ctx.gnn_gate(nk.gate if nk.gate is not None else float("nan"))
         self.ctx.synchronize()
-        self.train_info = {"pairs": pairs, "loss": loss, "seconds": time.perf_counter() - t0}
+        self.train_info = {"pairs": pairs, "loss": loss, "val_loss": val_loss,
+                           "network_applied": use_net, "seconds": time.perf_counter() - t0}
         return self.train_info
 
     # ------------------------------------------------------------------ fields

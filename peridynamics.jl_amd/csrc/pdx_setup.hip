@@ -851,7 +851,7 @@ extern "C" pdx_status pdx_build_bonds(pdx_ctx *c, int64_t *n_bonds_out)
     // material state between the passes
     if (c->kind == PDX_OSB) TRY_C(pdx_alloc(c, &c->pstate, 2 * n));
     if (c->kind == PDX_CPD) {
-        TRY_C(pdx_alloc(c, &c->pstate, 18 * n));
+        TRY_C(pdx_alloc(c, &c->pstate, 9 * n));
         TRY_C(pdx_alloc(c, &c->defgrad, 9 * n));
         TRY_C(pdx_alloc(c, &c->cauchy, 9 * n));
         HIP_C(hipMemsetAsync(c->defgrad, 0, sizeof(double) * 9 * n, c->stream));

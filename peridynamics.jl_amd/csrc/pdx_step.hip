@@ -937,11 +937,17 @@ __device__ void pk1(int model, const double *F, double lambda, double mu, double
 
 // CPD pass 1: calc_failure! (bond_system.jl:309-327), calc_damage!,
 // calc_deformation_gradient! (correspondence.jl:193-214), too_much_damage! (:288-298),
-// calc_first_piola_kirchhoff! (:216-224).  Stores F and P*K^-1 of every point.
-__global__ __launch_bounds__(PDX_BLOCK) void k_cpd_pass1(pdx_step_args A)
+// calc_first_piola_kirchhoff! (:216-224).  Stores Q = P K^-1 - beta F of every point: the
+// ZEMSilling bond force of pass 2, w [(P K^-1) dX + beta (dx - F dX)] = w (Q dX + beta dx),
+// then needs 9 numbers per neighbour instead of 18.
+// S lanes per point take interleaved bonds; their sums are folded with lane shuffles and
+// one lane finishes the point.
+template <int S>
+__global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_pass1(pdx_step_args A)
 {
-    const int64_t p = pdx_thread_point();
-    if (p >= A.n) return;
+    const int sub = threadIdx.x % S;
+    const int64_t p = pdx_xcd_block() * PDX_BLOCK + threadIdx.x / S;
+    if (p >= A.n) return;  // all S lanes of the point together
     const int c = A.cnt[p];
     const int64_t b0 = pdx_bond0(A.slice_ptr, p);
     const double4 ri = A.rec_in[p];
@@ -950,7 +956,7 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_cpd_pass1(pdx_step_args A)
     double Km[6] = {0, 0, 0, 0, 0, 0};  // 00 10 20 11 21 22
     double Ft[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     int na = 0;
-    for (int t = 0; t < c; t++) {
+    for (int t = sub; t < c; t += S) {
         const int64_t b = b0 + (int64_t)t * 64;
         const uint32_t wd = A.bw[b];
         const uint32_t j = wd & PDX_BW_NBR;
@@ -979,6 +985,15 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_cpd_pass1(pdx_step_args A)
                 for (int r = 0; r < 3; r++) Ft[r + 3 * cc] += temp * (dx[r] * DX[cc]);
         }
     }
+#pragma unroll
+    for (int o = 1; o < S; o <<= 1) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) Km[q] += __shfl_xor(Km[q], o);
+#pragma unroll
+        for (int q = 0; q < 9; q++) Ft[q] += __shfl_xor(Ft[q], o);
+        na += __shfl_xor(na, o);
+    }
+    if (sub != 0) return;
     const double K9[9] = {Km[0], Km[1], Km[2], Km[1], Km[3], Km[4], Km[2], Km[4], Km[5]};
     double Kinv[9], F[9];
     m3_inv(K9, Kinv);
@@ -1007,8 +1022,7 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_cpd_pass1(pdx_step_args A)
     }
 #pragma unroll
     for (int k = 0; k < 9; k++) {
-        A.pstate[18 * p + k] = F[k];
-        A.pstate[18 * p + 9 + k] = PK[k];
+        A.pstate[9 * p + k] = PK[k] - A.beta_zem * F[k];
         if (store) A.defgrad[9 * p + k] = F[k];
     }
     if (kill && !dead) A.killed[p] = 1;
@@ -1019,25 +1033,26 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_cpd_pass1(pdx_step_args A)
 }
 
 // CPD pass 2: c_force_density! with ZEMSilling (correspondence.jl:226-251) as a gather:
-// own term  t_ij V_j           with t_ij = (w PKinv_i) dX + (w beta) (dx - F_i dX)
-// reverse  -t_ji V_i  = V_i [ (w PKinv_j) dX + (w beta) (dx - F_j dX) ]
-__global__ __launch_bounds__(PDX_BLOCK) void k_cpd_pass2(pdx_step_args A)
+// own term  t_ij V_j           with t_ij = w [(PKinv_i) dX + beta (dx - F_i dX)]
+//                                        = w (Q_i dX + beta dx)
+// reverse  -t_ji V_i  = V_i w (Q_j dX + beta dx)
+template <int S>
+__global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_pass2(pdx_step_args A)
 {
-    const int64_t p = pdx_thread_point();
-    if (p >= A.n) return;
+    const int sub = threadIdx.x % S;
+    const int64_t p = pdx_xcd_block() * PDX_BLOCK + threadIdx.x / S;
+    if (p >= A.n) return;  // all S lanes of the point together
     const int c = A.cnt[p];
     const int64_t b0 = pdx_bond0(A.slice_ptr, p);
     const double4 ri = A.rec_in[p];
     const double4 Xi = A.Xr[p];
     const bool dead_i = A.killed[p] != 0;
-    double Fi[9], PKi[9];
+    const double beta = A.beta_zem;
+    double Qi[9];
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
-        Fi[k] = A.pstate[18 * p + k];
-        PKi[k] = A.pstate[18 * p + 9 + k];
-    }
+    for (int k = 0; k < 9; k++) Qi[k] = A.pstate[9 * p + k];
     double b3[3] = {0.0, 0.0, 0.0};
-    for (int t = 0; t < c; t++) {
+    for (int t = sub; t < c; t += S) {
         const int64_t b = b0 + (int64_t)t * 64;
         const uint32_t wd = A.bw[b];
         if (!(wd & PDX_BW_ACT)) continue;
@@ -1046,30 +1061,23 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_cpd_pass2(pdx_step_args A)
         const double4 Xj = A.Xr[j];
         const double dx[3] = {rj.x - ri.x, rj.y - ri.y, rj.z - ri.z};
         const double DX[3] = {Xj.x - Xi.x, Xj.y - Xi.y, Xj.z - Xi.z};
-        const double w = A.delta / A.L[b];
-        const double wb = w * A.beta_zem;
-        if (!dead_i) {
+        const double wv = (A.delta / A.L[b]) * rj.w;  // w V_j
+        const bool live_j = !A.killed[j];
+        const double *qj = A.pstate + 9 * (int64_t)j;
 #pragma unroll
-            for (int r = 0; r < 3; r++) {
-                const double FX = (Fi[r] * DX[0] + Fi[r + 3] * DX[1]) + Fi[r + 6] * DX[2];
-                const double tz = wb * (dx[r] - FX);
-                const double tt = (((w * PKi[r]) * DX[0] + (w * PKi[r + 3]) * DX[1]) +
-                                   (w * PKi[r + 6]) * DX[2]) + tz;
-                b3[r] += tt * rj.w;
-            }
-        }
-        if (!A.killed[j]) {
-            const double *pj = A.pstate + 18 * (int64_t)j;
-#pragma unroll
-            for (int r = 0; r < 3; r++) {
-                const double FX = (pj[r] * DX[0] + pj[r + 3] * DX[1]) + pj[r + 6] * DX[2];
-                const double tz = wb * (dx[r] - FX);
-                const double tt = (((w * pj[9 + r]) * DX[0] + (w * pj[9 + r + 3]) * DX[1]) +
-                                   (w * pj[9 + r + 6]) * DX[2]) + tz;
-                b3[r] += tt * rj.w;
-            }
+        for (int r = 0; r < 3; r++) {
+            const double bdx = beta * dx[r];
+            double tt = 0.0;
+            if (!dead_i) tt += ((Qi[r] * DX[0] + Qi[r + 3] * DX[1]) + Qi[r + 6] * DX[2]) + bdx;
+            if (live_j) tt += ((qj[r] * DX[0] + qj[r + 3] * DX[1]) + qj[r + 6] * DX[2]) + bdx;
+            b3[r] += wv * tt;
         }
     }
+#pragma unroll
+    for (int o = 1; o < S; o <<= 1)
+#pragma unroll
+        for (int r = 0; r < 3; r++) b3[r] += __shfl_xor(b3[r], o);
+    if (sub != 0) return;
     tail_ops T;
     tail_load(A, p, T);
     point_tail(A, p, ri, b3, T, A.rho);
@@ -1294,11 +1302,11 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
         break;
     case PDX_CPD:
         PDX_TRY(pdx_timed_begin(c));
-        hipLaunchKernelGGL(k_cpd_pass1, G, B, 0, c->stream, a);
+        hipLaunchKernelGGL(k_cpd_pass1<PDX_LDS_LANES>, G, dim3(PDX_BLOCK * PDX_LDS_LANES), 0, c->stream, a);
         PDX_HIP(c, hipGetLastError());
         PDX_TRY(pdx_timed_end(c, "pass1"));
         PDX_TRY(pdx_timed_begin(c));
-        hipLaunchKernelGGL(k_cpd_pass2, G, B, 0, c->stream, a);
+        hipLaunchKernelGGL(k_cpd_pass2<PDX_LDS_LANES>, G, dim3(PDX_BLOCK * PDX_LDS_LANES), 0, c->stream, a);
         PDX_HIP(c, hipGetLastError());
         PDX_TRY(pdx_timed_end(c, "step"));
         break;

@@ -956,33 +956,46 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_pass1(pdx_step_args A)
     double Km[6] = {0, 0, 0, 0, 0, 0};  // 00 10 20 11 21 22
     double Ft[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     int na = 0;
-    for (int t = sub; t < c; t += S) {
-        const int64_t b = b0 + (int64_t)t * 64;
-        const uint32_t wd = A.bw[b];
-        const uint32_t j = wd & PDX_BW_NBR;
-        const double4 rj = A.rec_in[j];
-        const double4 Xj = A.Xr[j];
-        const double dx[3] = {rj.x - ri.x, rj.y - ri.y, rj.z - ri.z};
-        const double DX[3] = {Xj.x - Xi.x, Xj.y - Xi.y, Xj.z - Xi.z};
-        const double Lb = A.L[b];
-        double rl;
-        const double l = pdx_sqrt_rsq((dx[0] * dx[0] + dx[1] * dx[1]) + dx[2] * dx[2], rl);
-        const double e = l - Lb;
-        const double rL = pdx_rcp1(Lb);
-        const bool act = bond_fail(A, A.bw + b, wd, e, Lb, e * rL);
-        if (act && !dead) {
-            na++;
-            const double temp = (A.delta * rL) * rj.w;
-            Km[0] += temp * (DX[0] * DX[0]);
-            Km[1] += temp * (DX[1] * DX[0]);
-            Km[2] += temp * (DX[2] * DX[0]);
-            Km[3] += temp * (DX[1] * DX[1]);
-            Km[4] += temp * (DX[2] * DX[1]);
-            Km[5] += temp * (DX[2] * DX[2]);
+    constexpr int U = 2;  // bonds per lane in flight
+    for (int t0 = sub; t0 < c; t0 += U * S) {
+        uint32_t wdu[U];
+        double Lu[U];
 #pragma unroll
-            for (int cc = 0; cc < 3; cc++)
+        for (int u = 0; u < U; u++) {
+            const bool ok = t0 + u * S < c;
+            wdu[u] = ok ? A.bw[b0 + (int64_t)(t0 + u * S) * 64] : 0u;
+            Lu[u] = ok ? A.L[b0 + (int64_t)(t0 + u * S) * 64] : 1.0;
+        }
 #pragma unroll
-                for (int r = 0; r < 3; r++) Ft[r + 3 * cc] += temp * (dx[r] * DX[cc]);
+        for (int u = 0; u < U; u++) {
+            if (t0 + u * S >= c) continue;
+            const int64_t b = b0 + (int64_t)(t0 + u * S) * 64;
+            const uint32_t wd = wdu[u];
+            const uint32_t j = wd & PDX_BW_NBR;
+            const double4 rj = A.rec_in[j];
+            const double4 Xj = A.Xr[j];
+            const double dx[3] = {rj.x - ri.x, rj.y - ri.y, rj.z - ri.z};
+            const double DX[3] = {Xj.x - Xi.x, Xj.y - Xi.y, Xj.z - Xi.z};
+            const double Lb = Lu[u];
+            double rl;
+            const double l = pdx_sqrt_rsq((dx[0] * dx[0] + dx[1] * dx[1]) + dx[2] * dx[2], rl);
+            const double e = l - Lb;
+            const double rL = pdx_rcp1(Lb);
+            const bool act = bond_fail(A, A.bw + b, wd, e, Lb, e * rL);
+            if (act && !dead) {
+                na++;
+                const double temp = (A.delta * rL) * rj.w;
+                Km[0] += temp * (DX[0] * DX[0]);
+                Km[1] += temp * (DX[1] * DX[0]);
+                Km[2] += temp * (DX[2] * DX[0]);
+                Km[3] += temp * (DX[1] * DX[1]);
+                Km[4] += temp * (DX[2] * DX[1]);
+                Km[5] += temp * (DX[2] * DX[2]);
+#pragma unroll
+                for (int cc = 0; cc < 3; cc++)
+#pragma unroll
+                    for (int r = 0; r < 3; r++) Ft[r + 3 * cc] += temp * (dx[r] * DX[cc]);
+            }
         }
     }
 #pragma unroll
@@ -1052,25 +1065,36 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_pass2(pdx_step_args A)
 #pragma unroll
     for (int k = 0; k < 9; k++) Qi[k] = A.pstate[9 * p + k];
     double b3[3] = {0.0, 0.0, 0.0};
-    for (int t = sub; t < c; t += S) {
-        const int64_t b = b0 + (int64_t)t * 64;
-        const uint32_t wd = A.bw[b];
-        if (!(wd & PDX_BW_ACT)) continue;
-        const uint32_t j = wd & PDX_BW_NBR;
-        const double4 rj = A.rec_in[j];
-        const double4 Xj = A.Xr[j];
-        const double dx[3] = {rj.x - ri.x, rj.y - ri.y, rj.z - ri.z};
-        const double DX[3] = {Xj.x - Xi.x, Xj.y - Xi.y, Xj.z - Xi.z};
-        const double wv = (A.delta / A.L[b]) * rj.w;  // w V_j
-        const bool live_j = !A.killed[j];
-        const double *qj = A.pstate + 9 * (int64_t)j;
+    constexpr int U = 2;  // bonds per lane in flight
+    for (int t0 = sub; t0 < c; t0 += U * S) {
+        uint32_t wdu[U];
+        double Lu[U];
 #pragma unroll
-        for (int r = 0; r < 3; r++) {
-            const double bdx = beta * dx[r];
-            double tt = 0.0;
-            if (!dead_i) tt += ((Qi[r] * DX[0] + Qi[r + 3] * DX[1]) + Qi[r + 6] * DX[2]) + bdx;
-            if (live_j) tt += ((qj[r] * DX[0] + qj[r + 3] * DX[1]) + qj[r + 6] * DX[2]) + bdx;
-            b3[r] += wv * tt;
+        for (int u = 0; u < U; u++) {
+            const bool ok = t0 + u * S < c;
+            wdu[u] = ok ? A.bw[b0 + (int64_t)(t0 + u * S) * 64] : 0u;
+            Lu[u] = ok ? A.L[b0 + (int64_t)(t0 + u * S) * 64] : 1.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const uint32_t wd = wdu[u];
+            if (!(wd & PDX_BW_ACT)) continue;  // also the slots past the row
+            const uint32_t j = wd & PDX_BW_NBR;
+            const double4 rj = A.rec_in[j];
+            const double4 Xj = A.Xr[j];
+            const double dx[3] = {rj.x - ri.x, rj.y - ri.y, rj.z - ri.z};
+            const double DX[3] = {Xj.x - Xi.x, Xj.y - Xi.y, Xj.z - Xi.z};
+            const double wv = (A.delta * pdx_rcp1(Lu[u])) * rj.w;  // w V_j
+            const bool live_j = !A.killed[j];
+            const double *qj = A.pstate + 9 * (int64_t)j;
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                const double bdx = beta * dx[r];
+                double tt = 0.0;
+                if (!dead_i) tt += ((Qi[r] * DX[0] + Qi[r + 3] * DX[1]) + Qi[r + 6] * DX[2]) + bdx;
+                if (live_j) tt += ((qj[r] * DX[0] + qj[r + 3] * DX[1]) + qj[r + 6] * DX[2]) + bdx;
+                b3[r] += wv * tt;
+            }
         }
     }
 #pragma unroll

@@ -123,6 +123,52 @@ __device__ __forceinline__ void point_tail(const pdx_step_args &A, int64_t p, do
     if (flags & PDX_F_HAS_NEXT) A.rec_out[p] = make_double4(x[0], x[1], x[2], ri.w);
 }
 
+// The same tail split over a point's four lanes (they all hold the point's sums): lane d < 3
+// updates component d, lane 3 carries the record's volume, so that every load and store of
+// the tail is one contiguous run per point.  Arithmetic as point_tail, component by
+// component.
+__device__ __forceinline__ void point_tail4(const pdx_step_args &A, int64_t p, double4 ri,
+                                            const double b[3], int d, double rho)
+{
+    const int flags = A.flags;
+    if (d == 0 && (isnan(b[0]) || isnan(b[1]) || isnan(b[2])))
+        atomicMin(A.flags_out, (unsigned long long)A.step);
+    const int64_t k = 3 * p + d;
+    const double bd = d == 0 ? b[0] : (d == 1 ? b[1] : b[2]);
+    if (flags & PDX_F_FORCE_ONLY) {
+        if (d < 3) A.bint_out[k] = (flags & PDX_F_SCALE_VOL) ? bd * ri.w : bd;
+        return;
+    }
+    double xo = d == 0 ? ri.x : (d == 1 ? ri.y : (d == 2 ? ri.z : ri.w));
+    if (d < 3) {
+        const double vh = A.vh[k];
+        const double be = (flags & PDX_F_BEXT) ? A.bext[k] : 0.0;
+        const double ad = (bd + be) / rho;
+        const double vd = vh + ad * A.dth;
+        if (flags & PDX_F_STORE) {
+            A.v_out[k] = vd;
+            A.a_out[k] = ad;
+            A.bint_out[k] = bd;
+        }
+        if (flags & PDX_F_HAS_NEXT) {
+            double vn = vd + ad * A.dth;
+            const int32_t slot = A.bcslot[p];
+            if (slot >= 0) {
+                const int32_t src = A.slot_src[3 * slot + d];
+                if (src >= 0) {
+                    const double val = A.bcv[src];
+                    if (isfinite(val)) vn = val;
+                }
+            }
+            const double du = vn * A.dt;
+            A.u[k] = A.u[k] + du;
+            xo = xo + du;
+            A.vh[k] = vn;
+        }
+    }
+    if (flags & PDX_F_HAS_NEXT) reinterpret_cast<double *>(A.rec_out)[4 * p + d] = xo;
+}
+
 // sqrt(d2) correctly rounded for a normal, finite d2 > 0 (the Markstein sequence of the
 // compiler's sqrt without its range scaling: bond lengths never come near the subnormal
 // range), plus 1/sqrt(d2) to about 1 ulp as a by-product.
@@ -389,15 +435,13 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds(pdx_step_args A)
         bz += __shfl_xor(bz, o);
         na += __shfl_xor(na, o);
     }
-    if (sub != 0) return;
-    if (A.flags & (PDX_F_STORE | PDX_F_FORCE_ONLY)) {
+    if (sub == 0 && (A.flags & (PDX_F_STORE | PDX_F_FORCE_ONLY))) {
         A.nact_out[p] = na;
         A.dmg_out[p] = 1.0 - (double)na / (double)c;
     }
-    tail_ops T;
-    tail_load(A, p, T);
+    static_assert(S == 4, "the tail takes four lanes per point");
     const double b[3] = {bx, by, bz};
-    point_tail(A, p, ri, b, T, rho);
+    point_tail4(A, p, ri, b, sub, rho);
 }
 
 // ----------------------------------------------------------------------------------------
@@ -514,15 +558,13 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds_mx(pdx_step_args A)
         bz += __shfl_xor(bz, o);
         na += __shfl_xor(na, o);
     }
-    if (sub != 0) return;
-    if (A.flags & (PDX_F_STORE | PDX_F_FORCE_ONLY)) {
+    if (sub == 0 && (A.flags & (PDX_F_STORE | PDX_F_FORCE_ONLY))) {
         A.nact_out[p] = na;
         A.dmg_out[p] = 1.0 - (double)na / (double)c;
     }
-    tail_ops T;
-    tail_load(A, p, T);
+    static_assert(S == 4, "the tail takes four lanes per point");
     const double bb[3] = {bx, by, bz};
-    point_tail(A, p, ri, bb, T, rho);
+    point_tail4(A, p, ri, bb, sub, rho);
 }
 
 // ----------------------------------------------------------------------------------------
@@ -837,11 +879,9 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds2(pdx_step_args A)
         by += __shfl_xor(by, o);
         bz += __shfl_xor(bz, o);
     }
-    if (sub != 0) return;
-    tail_ops T;
-    tail_load(A, p, T);
+    static_assert(S == 4, "the tail takes four lanes per point");
     const double b[3] = {bx, by, bz};
-    point_tail(A, p, ri, b, T, rho);
+    point_tail4(A, p, ri, b, sub, rho);
 }
 
 // ----------------------------------------------------------------------------------------
@@ -1101,10 +1141,8 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_pass2(pdx_step_args A)
     for (int o = 1; o < S; o <<= 1)
 #pragma unroll
         for (int r = 0; r < 3; r++) b3[r] += __shfl_xor(b3[r], o);
-    if (sub != 0) return;
-    tail_ops T;
-    tail_load(A, p, T);
-    point_tail(A, p, ri, b3, T, A.rho);
+    static_assert(S == 4, "the tail takes four lanes per point");
+    point_tail4(A, p, ri, b3, sub, A.rho);
 }
 
 // ----------------------------------------------------------------------------------------

@@ -498,13 +498,18 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds_mx(pdx_step_args A)
     double bx = 0.0, by = 0.0, bz = 0.0;
     int na = 0;
     float bcf = (float)A.bc, epscf = (float)A.epsc;
-    double rho = A.rho;
+    double rho = A.rho, epsc = A.epsc;
     if (MULTI) {
         const pdx_pset Pi = A.psets[A.pid[p]];
         bcf = (float)Pi.bc;
         epscf = (float)Pi.epsc;
+        epsc = Pi.epsc;
         rho = Pi.rho;
     }
+    // FP32 stretches within 1e-4 of eps_c are decided on the FP64 stretch, so that a bond
+    // breaks in the step in which the FP64 step would break it (the FP32 stretch is good to
+    // ~1e-6 relative)
+    const float win_lo = epscf * 0.9999f, win_hi = epscf * 1.0001f;
     for (int t = sub; t < c; t += U * S) {
         uint32_t wd[U];
         float sc[U];
@@ -539,9 +544,20 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds_mx(pdx_step_args A)
             const float l = __builtin_amdgcn_sqrtf(L2 + num);
             const float eps = num * __builtin_amdgcn_rcpf(Lb * (l + Lb));
             bool act = (wd[u] & PDX_W16_ACT) != 0;
-            if (act && (wd[u] & PDX_W16_FP) && eps > epscf) {
-                act = false;
-                wbase[(int64_t)(t + u * S) * 64] = (uint16_t)(wd[u] & ~PDX_W16_ACT);
+            if (act && (wd[u] & PDX_W16_FP) && eps > win_lo) {
+                bool brk = eps > win_hi;
+                if (!brk) {
+                    const double Lb = A.L[b0 + (int64_t)(t + u * S) * 64];
+                    const double4 xj = A.rec_in[hidx[slot]];
+                    const double dx = xj.x - ri.x, dy = xj.y - ri.y, dz = xj.z - ri.z;
+                    double rl;
+                    const double l64 = pdx_sqrt_rsq((dx * dx + dy * dy) + dz * dz, rl);
+                    brk = ((l64 - Lb) / Lb) > epsc;
+                }
+                if (brk) {
+                    act = false;
+                    wbase[(int64_t)(t + u * S) * 64] = (uint16_t)(wd[u] & ~PDX_W16_ACT);
+                }
             }
             const float bcb = MULTI ? (bcf + (float)A.psets[(!CAP || slot < HL) ? hpid[slot] : A.pid[jg]].bc) * 0.5f : bcf;
             const float sf = act ? (((SC ? sc[u] * bcb : bcb) * eps) * a.w) * __builtin_amdgcn_rcpf(l) : 0.f;

@@ -165,13 +165,18 @@ def jfnk_leg(shape, train_steps):
 
 
 def state_based_bytes(kind, n_points, n_bonds, layout):
-    """Algorithmic bytes of the two passes of an OSB or CPD step.  Per bond and pass the
-    bond word and the FP64 length (LDS layout 2 + 8, global layout 4 + 8).  Per point:
-    OSB 236 B (both passes read x|V and the count, pass 1 writes c1, c2 and pass 2 reads
-    them, plus the Verlet tail); CPD 415 B (also X|V in both passes and the nine numbers
-    of Q = P K^-1 - beta F written and read back).  LDS layout: plus the 4-byte
+    """Algorithmic bytes of the two passes of an OSB or CPD step (one parameter set).  Per
+    bond and pass the bond word and the FP64 length (LDS layout 2 + 8, global layout
+    4 + 8).  Per point: OSB, global layout, 236 B (both passes read x|V and the count,
+    pass 1 writes c1, c2 and pass 2 reads them, plus the Verlet tail); OSB, LDS layout,
+    268 B (pass 1 writes (c1 V, c2 V) and the own force part, 40 B, pass 2 reads the own
+    part, 24 B); CPD 415 B (also X|V in both passes and the nine numbers of
+    Q = P K^-1 - beta F written and read back).  LDS layout: plus the 4-byte
     neighbour-table index per entry and pass."""
-    per_pt = 236 if kind == "OSB" else 415
+    if kind == "OSB":
+        per_pt = 268 if layout["lds_mode"] else 236
+    else:
+        per_pt = 415
     if layout["lds_mode"]:
         return (20 * n_bonds + per_pt * n_points + 8 * layout["halo_entries"],
                 f"20*N_b + {per_pt}*N_p + 8*N_halo")

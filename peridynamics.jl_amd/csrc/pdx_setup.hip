@@ -849,7 +849,8 @@ extern "C" pdx_status pdx_build_bonds(pdx_ctx *c, int64_t *n_bonds_out)
     c->n_src = 0;
     c->n_bcpts = 0;
     // material state between the passes
-    if (c->kind == PDX_OSB) TRY_C(pdx_alloc(c, &c->pstate, 2 * n));
+    // OSB: the per-point pair of pass 1, then (LDS layout, one set) the own force part
+    if (c->kind == PDX_OSB) TRY_C(pdx_alloc(c, &c->pstate, 5 * n));
     if (c->kind == PDX_CPD) {
         TRY_C(pdx_alloc(c, &c->pstate, 9 * n));
         TRY_C(pdx_alloc(c, &c->defgrad, 9 * n));

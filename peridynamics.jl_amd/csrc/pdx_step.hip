@@ -666,14 +666,21 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_osb_pass2(pdx_step_args A)
     point_tail(A, p, ri, b, T, A.rho);
 }
 
-// OSB on the LDS-staged layout (four lanes per point, as the BB step).  Pass 1 stages the
-// block's neighbour records x|V; pass 2 stages x|V and the neighbours' (c1, c2) behind
-// them (48 B per neighbour-table entry, A.halo_max entries of each).  Bond states live in
-// the 16-bit words.
+// OSB on the LDS-staged layout (four lanes per point, as the BB step).  Bond states live
+// in the 16-bit words.  With one parameter set the bond force of i-j,
+//   (w / l) [(c2_i L + c1_i e) + (c2_j L + c1_j e)] V_j dx,
+// splits into an own part, c2_i A_i + c1_i B_i with A_i = sum (w/l) V_j L dx and
+// B_i = sum (w/l) V_j e dx, which pass 1 sums while it forms c1_i, c2_i, and a reverse
+// part, (w/l) [(c2 V)_j L + (c1 V)_j e] dx, which pass 2 gathers.  Pass 1 stages x|V
+// (32 B per neighbour-table entry) and stores (c1 V, c2 V) and the own part per point;
+// pass 2 stages x and (c1 V, c2 V) (40 B per entry), so a 2 M-point lattice's longest
+// table fits the LDS budget.  With several sets c1, c2 depend on the bond: pass 1 stores
+// (wvol, dilatation) and pass 2 stages x|V and those (48 B + the set index per entry).
 template <int S, int F>
 __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds1(pdx_step_args A)
 {
     constexpr bool MULTI = (F & PDX_FEAT_MULTI) != 0;
+    constexpr bool SPLIT = !MULTI;
     constexpr bool SC = (F & PDX_FEAT_SC) != 0;
     constexpr bool CAP = (F & PDX_FEAT_CAP) != 0;
     extern __shared__ double4 hrec[];
@@ -701,6 +708,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds1(pdx_step_args A)
     const double *const Lbase = A.L + b0;
     const double4 ri = A.rec_in[p];
     double wvol = 0.0, Ssum = 0.0;
+    double ax = 0.0, ay = 0.0, az = 0.0, bx = 0.0, by = 0.0, bz = 0.0;  // A_i, B_i (SPLIT)
     int na = 0;
     double epsc = A.epsc, epsc_lo = A.epsc_lo, epsc_hi = A.epsc_hi, G = A.G, K = A.K;
     if (MULTI) {
@@ -759,6 +767,16 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds1(pdx_step_args A)
                 wvol += (w * (Lb * Lb)) * rj.w;
                 Ssum += ((w * Lb) * e) * rj.w;
                 na++;
+                if (SPLIT) {
+                    const double q = (w * rl) * rj.w;
+                    const double qL = q * Lb, qe = q * e;
+                    ax = __builtin_fma(qL, dx, ax);
+                    ay = __builtin_fma(qL, dy, ay);
+                    az = __builtin_fma(qL, dz, az);
+                    bx = __builtin_fma(qe, dx, bx);
+                    by = __builtin_fma(qe, dy, by);
+                    bz = __builtin_fma(qe, dz, bz);
+                }
             }
         }
     }
@@ -767,6 +785,36 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds1(pdx_step_args A)
         wvol += __shfl_xor(wvol, o);
         Ssum += __shfl_xor(Ssum, o);
         na += __shfl_xor(na, o);
+        if (SPLIT) {
+            ax += __shfl_xor(ax, o);
+            ay += __shfl_xor(ay, o);
+            az += __shfl_xor(az, o);
+            bx += __shfl_xor(bx, o);
+            by += __shfl_xor(by, o);
+            bz += __shfl_xor(bz, o);
+        }
+    }
+    if (SPLIT) {
+        // every lane holds the sums; lane d < 3 stores component d of the own part, lane 3
+        // the per-point pair and the counts
+        double c1 = 0.0, c2 = 0.0;
+        if (wvol != 0.0) {
+            const double dil = (3.0 / wvol) * Ssum;
+            c1 = (15.0 * G) / wvol;
+            c2 = dil * ((3.0 * K) / wvol - c1 / 3.0);
+        }
+        if (sub < 3) {
+            const double a = sub == 0 ? ax : (sub == 1 ? ay : az);
+            const double bb = sub == 0 ? bx : (sub == 1 ? by : bz);
+            A.pstate[2 * A.n + 3 * p + sub] = c2 * a + c1 * bb;
+            return;
+        }
+        reinterpret_cast<double2 *>(A.pstate)[p] = make_double2(c1 * ri.w, c2 * ri.w);
+        if (A.flags & (PDX_F_STORE | PDX_F_FORCE_ONLY)) {
+            A.nact_out[p] = na;
+            A.dmg_out[p] = 1.0 - (double)na / (double)c;
+        }
+        return;
     }
     if (sub != 0) return;
     double c1 = 0.0, c2 = 0.0;
@@ -786,6 +834,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds1(pdx_step_args A)
     }
 }
 
+// One set: the own part comes from pass 1, the loop gathers the reverse parts.
 // MULTI: several parameter sets (force_density_point! with a ParameterHandler,
 // ordinary_state_based.jl:151-174): pass 1 left (wvol, dilatation) per point and the bond
 // takes c1 = 15 (G_i + G_j) / (2 wvol), c2 = dil (3 (K_i + K_j) / (2 wvol) - c1 / 3) of each
@@ -794,11 +843,14 @@ template <int S, int F>
 __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds2(pdx_step_args A)
 {
     constexpr bool MULTI = (F & PDX_FEAT_MULTI) != 0;
+    constexpr bool SPLIT = !MULTI;
     constexpr bool SC = (F & PDX_FEAT_SC) != 0;
     constexpr bool CAP = (F & PDX_FEAT_CAP) != 0;
     extern __shared__ double4 hrec[];
-    const int HL = CAP ? pdx_lds_entries(A.halo_max, A.lds_budget, 48 + (MULTI ? 1 : 0)) : A.halo_max;
+    const int HL = CAP ? pdx_lds_entries(A.halo_max, A.lds_budget, SPLIT ? 40 : 48 + 1) : A.halo_max;
+    // SPLIT: hrec holds (x, y, z, (c1 V)_j), hc2 (c2 V)_j; otherwise hrec x|V, hc (wvol, dil)
     double2 *const hc = reinterpret_cast<double2 *>(hrec + HL);
+    double *const hc2 = reinterpret_cast<double *>(hrec + HL);
     const int64_t k = pdx_xcd_block();
     const int64_t p0 = k * PDX_BLOCK;
     if (p0 >= A.n) return;  // whole workgroup
@@ -810,9 +862,16 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds2(pdx_step_args A)
     uint8_t *const hpid = reinterpret_cast<uint8_t *>(hc + HL);
     for (int h = threadIdx.x; h < Hs; h += PDX_BLOCK * S) {
         const int64_t j = hidx[h];
-        hrec[h] = A.rec_in[j];
-        hc[h] = cst[j];
-        if (MULTI) hpid[h] = A.pid[j];
+        const double4 r = A.rec_in[j];
+        const double2 cj = cst[j];
+        if (SPLIT) {
+            hrec[h] = make_double4(r.x, r.y, r.z, cj.x);
+            hc2[h] = cj.y;
+        } else {
+            hrec[h] = r;
+            hc[h] = cj;
+            hpid[h] = A.pid[j];
+        }
     }
     __syncthreads();
     const int sub = threadIdx.x % S;
@@ -823,8 +882,14 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds2(pdx_step_args A)
     const uint16_t *const wbase = A.bw16 + b0;
     const double *const Lbase = A.L + b0;
     const double4 ri = A.rec_in[p];
-    const double2 ci = cst[p];
+    const double2 ci = MULTI ? cst[p] : make_double2(0.0, 0.0);
     double bx = 0.0, by = 0.0, bz = 0.0;
+    if (SPLIT && sub == 0) {
+        const double *own = A.pstate + 2 * A.n + 3 * p;
+        bx = own[0];
+        by = own[1];
+        bz = own[2];
+    }
     double Gi = A.G, Ki = A.K, rho = A.rho;
     if (MULTI) {
         const pdx_pset Pi = A.psets[A.pid[p]];
@@ -855,14 +920,34 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds2(pdx_step_args A)
             const int slot = (int)(wd & PDX_W16_SLOT);
             const bool in_lds = !CAP || slot < HL;
             const int32_t jg = in_lds ? 0 : hidx[slot];
-            const double4 rj = in_lds ? hrec[slot] : A.rec_in[jg];
-            const double2 cj = in_lds ? hc[slot] : cst[jg];
+            double4 rj;
+            double2 cj;
+            if (SPLIT) {
+                if (in_lds) {
+                    rj = hrec[slot];
+                    cj = make_double2(rj.w, hc2[slot]);
+                } else {
+                    rj = A.rec_in[jg];
+                    cj = cst[jg];
+                }
+            } else {
+                rj = in_lds ? hrec[slot] : A.rec_in[jg];
+                cj = in_lds ? hc[slot] : cst[jg];
+            }
             const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
             double rl;
             const double l = pdx_sqrt_rsq((dx * dx + dy * dy) + dz * dz, rl);
             double w = A.delta * pdx_rcp1(Lb);
             if (SC) w *= A.scf[b0 + o];  // beta is symmetric: both ends' copies of the bond carry it
             const double e = l - Lb;
+            if (SPLIT) {
+                // the reverse part (c2 V)_j L + (c1 V)_j e
+                const double s = (w * rl) * (cj.y * Lb + cj.x * e);
+                bx = __builtin_fma(s, dx, bx);
+                by = __builtin_fma(s, dy, by);
+                bz = __builtin_fma(s, dz, bz);
+                continue;
+            }
             double fi, fj;
             if (MULTI) {
                 const pdx_pset Pj = A.psets[in_lds ? hpid[slot] : A.pid[jg]];
@@ -1346,9 +1431,11 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
         if (c->lds_mode) {
             const dim3 BL(PDX_BLOCK * PDX_LDS_LANES);
             const int mb = c->multi() ? 1 : 0;  // the neighbours' parameter sets
-            const size_t l1 = lds_bytes(c, 32 + mb), l2b = lds_bytes(c, 48 + mb);
+            // pass 2 entries: x and (c1 V, c2 V) with one set, x|V, (wvol, dil) and the set
+            const int e2 = mb ? 48 + mb : 40;
+            const size_t l1 = lds_bytes(c, 32 + mb), l2b = lds_bytes(c, e2);
             const int f1 = pdx_features(c) | (lds_capped(c, 32 + mb) ? PDX_FEAT_CAP : 0);
-            const int f2 = pdx_features(c) | (lds_capped(c, 48 + mb) ? PDX_FEAT_CAP : 0);
+            const int f2 = pdx_features(c) | (lds_capped(c, e2) ? PDX_FEAT_CAP : 0);
 #define PDX_OSB1_LAUNCH(F) launch_osb_feat<F>(c, 1, G, BL, l1, a)
 #define PDX_OSB2_LAUNCH(F) launch_osb_feat<F>(c, 2, G, BL, l2b, a)
             PDX_TRY(pdx_timed_begin(c));

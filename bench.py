@@ -67,13 +67,14 @@ def reference_layout_bytes(kind, n_points, n_bonds, precision="fp64"):
     with Int32 neighbour + FP64 length, a Bool bond state, and the per-point Verlet
     vectors written every step), as a yardstick that does not depend on this library's
     layout: BB 13 B per bond + 216 B per point (mixed: FP32 length, 9 B per bond); OSB two
-    bond passes + a reverse-bond index, 30 B per bond + 248 B per point; CPD 30 B per bond
-    + 576 B per point."""
+    bond passes + a reverse-bond index, 30 B per bond + 248 B per point (mixed: FP32
+    lengths in both passes, 22 B per bond); CPD 30 B per bond + 576 B per point."""
     if kind == "BB":
         return ((13 if precision == "fp64" else 9) * n_bonds + 216 * n_points,
                 f"{13 if precision == 'fp64' else 9}*N_b + 216*N_p")
     if kind == "OSB":
-        return 30 * n_bonds + 248 * n_points, "30*N_b + 248*N_p"
+        pb = 30 if precision == "fp64" else 22
+        return pb * n_bonds + 248 * n_points, f"{pb}*N_b + 248*N_p"
     return 30 * n_bonds + 576 * n_points, "30*N_b + 576*N_p"
 
 
@@ -164,15 +165,20 @@ def jfnk_leg(shape, train_steps):
                                                      res["two_level"]["seconds_steps_3_10"])}
 
 
-def state_based_bytes(kind, n_points, n_bonds, layout):
+def state_based_bytes(kind, n_points, n_bonds, layout, precision="fp64"):
     """Algorithmic bytes of the two passes of an OSB or CPD step (one parameter set).  Per
     bond and pass the bond word and the FP64 length (LDS layout 2 + 8, global layout
-    4 + 8).  Per point: OSB, global layout, 236 B (both passes read x|V and the count,
-    pass 1 writes c1, c2 and pass 2 reads them, plus the Verlet tail); OSB, LDS layout,
-    268 B (pass 1 writes (c1 V, c2 V) and the own force part, 40 B, pass 2 reads the own
-    part, 24 B); CPD 415 B (also X|V in both passes and the nine numbers of
-    Q = P K^-1 - beta F written and read back).  LDS layout: plus the 4-byte
-    neighbour-table index per entry and pass."""
+    4 + 8; mixed OSB only the 2-byte word: the length comes from the reference positions).
+    Per point: OSB, global layout, 236 B (both passes read x|V and the count, pass 1 writes
+    c1, c2 and pass 2 reads them, plus the Verlet tail); OSB, LDS layout, 268 B (pass 1
+    writes (c1 V, c2 V) and the own force part, 40 B, pass 2 reads the own part, 24 B);
+    mixed OSB 300 B (X|V also read in both passes, (c1, c2) written and read back, no own
+    part); CPD 415 B (also X|V in both passes and the
+    nine numbers of Q = P K^-1 - beta F written and read back).  LDS layout: plus the
+    4-byte neighbour-table index per entry and pass."""
+    if kind == "OSB" and precision == "mixed" and layout["lds_mode"]:
+        return (4 * n_bonds + 300 * n_points + 8 * layout["halo_entries"],
+                "4*N_b + 300*N_p + 8*N_halo")
     if kind == "OSB":
         per_pt = 268 if layout["lds_mode"] else 236
     else:
@@ -183,7 +189,7 @@ def state_based_bytes(kind, n_points, n_bonds, layout):
     return 24 * n_bonds + per_pt * n_points, f"24*N_b + {per_pt}*N_p"
 
 
-def state_based_leg(kind, shape, steps, warmup=10, dx=1e-3):
+def state_based_leg(kind, shape, steps, warmup=10, dx=1e-3, precision="fp64"):
     """Configuration 3: OSB or CMaterial (Saint Venant-Kirchhoff) bar, torsion 10 rad/s
     on one end (three point layers), the other end held, FP64."""
     import peridynamics as pd
@@ -198,7 +204,7 @@ def state_based_leg(kind, shape, steps, warmup=10, dx=1e-3):
     pd.velocity_bc(lambda p, t: 10.0 * p[1], body, "end", "z")
     for d in "xyz":
         pd.velocity_bc(lambda t: 0.0, body, "fixed", d)
-    dh = pd.MI355XBodyDataHandler(body)
+    dh = pd.MI355XBodyDataHandler(body, precision=precision)
     vv = pd.VelocityVerlet(steps=warmup + steps)
     dh.init_time_solver(vv)
     dh.initialize(vv)
@@ -215,10 +221,11 @@ def state_based_leg(kind, shape, steps, warmup=10, dx=1e-3):
     k2, _ = ctx.kernel_time("step")
     ctx.set_profiling(False)
     nb, npnt = dh.n_bonds, body.n_points
-    byt, formula = state_based_bytes(kind, npnt, nb, ctx.layout())
-    rbyt, rformula = reference_layout_bytes(kind, npnt, nb)
+    byt, formula = state_based_bytes(kind, npnt, nb, ctx.layout(), precision)
+    rbyt, rformula = reference_layout_bytes(kind, npnt, nb, precision)
     dh.close()
-    return {"workload": f"{kind} torsion bar {shape[0]}x{shape[1]}x{shape[2]}, 10 rad/s, FP64",
+    return {"workload": f"{kind} torsion bar {shape[0]}x{shape[1]}x{shape[2]}, 10 rad/s, "
+                        f"{precision.upper()}",
             "points": npnt, "bonds": nb, "steps": steps, "ms_per_step": el / steps * 1e3,
             "bond_updates_per_s": nb * steps / el, "pass1_ms": k1, "pass2_ms": k2,
             "bytes_formula": formula, "bytes_per_step": byt,
@@ -228,9 +235,8 @@ def state_based_leg(kind, shape, steps, warmup=10, dx=1e-3):
             "frac_hbm_reference_layout": rbyt / ((k1 + k2) * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
 
-def osb_2m_leg(shape, steps, warmup=10):
-    """The 2 M-point Velocity Verlet workload with OSBMaterial (E 70 GPa, nu 1/4, Gc 100),
-    FP64."""
+def osb_2m_leg(shape, steps, warmup=10, precision="fp64"):
+    """The 2 M-point Velocity Verlet workload with OSBMaterial (E 70 GPa, nu 1/4, Gc 100)."""
     import peridynamics as pd
     dx = 1e-3
     pos, vol = pd.lattice_block(*shape, dx, jitter_amplitude=0.05, seed=0x5EED)
@@ -241,7 +247,7 @@ def osb_2m_leg(shape, steps, warmup=10):
     pd.point_set(body, "bottom", y < y.min() + 2.5 * dx)
     pd.velocity_bc(lambda t: 0.6, body, "top", "y")
     pd.velocity_bc(lambda t: -0.6, body, "bottom", "y")
-    dh = pd.MI355XBodyDataHandler(body)
+    dh = pd.MI355XBodyDataHandler(body, precision=precision)
     vv = pd.VelocityVerlet(steps=warmup + steps)
     dh.init_time_solver(vv)
     dh.initialize(vv)
@@ -258,11 +264,11 @@ def osb_2m_leg(shape, steps, warmup=10):
     k2, _ = ctx.kernel_time("step")
     ctx.set_profiling(False)
     nb, npnt = dh.n_bonds, body.n_points
-    byt, formula = state_based_bytes("OSB", npnt, nb, ctx.layout())
-    rbyt, rformula = reference_layout_bytes("OSB", npnt, nb)
+    byt, formula = state_based_bytes("OSB", npnt, nb, ctx.layout(), precision)
+    rbyt, rformula = reference_layout_bytes("OSB", npnt, nb, precision)
     dh.close()
     return {"workload": f"OSB Velocity Verlet {shape[0]}x{shape[1]}x{shape[2]} jittered lattice, "
-                        f"+-0.6 m/s y-BCs, FP64",
+                        f"+-0.6 m/s y-BCs, {precision.upper()}",
             "points": npnt, "bonds": nb, "steps": steps, "ms_per_step": el / steps * 1e3,
             "bond_updates_per_s": nb * steps / el, "pass1_ms": k1, "pass2_ms": k2,
             "bytes_formula": formula, "bytes_per_step": byt,
@@ -270,7 +276,7 @@ def osb_2m_leg(shape, steps, warmup=10):
             "frac_hbm": byt / ((k1 + k2) * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "reference_layout_bytes_formula": rformula,
             "frac_hbm_reference_layout": rbyt / ((k1 + k2) * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "traffic": measured_traffic("osb", npnt, nb)}
+            "traffic": measured_traffic("osb" if precision == "fp64" else "osb_mixed", npnt, nb)}
 
 
 def mode_i_leg(npxy, nz, steps):
@@ -315,8 +321,8 @@ def mode_i_leg(npxy, nz, steps):
            "mixed_speedup": a["ms_per_step"] / b["ms_per_step"],
            "mixed_broken_bond_count_difference": int(np.sum(np.abs(a["_nact"] - b["_nact"]))),
            "mixed_max_damage_difference": float(np.max(np.abs(a["_dmg"] - b["_dmg"]))),
-           "mixed_crack_tip_dx_difference": (abs(a["crack_tip"][0] - b["crack_tip"][0]) / 2e-4
-                                             if not np.isnan(a["crack_tip"][0]) else None),
+           "mixed_crack_tip_difference_m": (abs(a["crack_tip"][0] - b["crack_tip"][0])
+                                            if not np.isnan(a["crack_tip"][0]) else None),
            "fp64_sensitivity": {k: {"perturbation_m": v,
                                     "broken_bond_count_difference": int(np.sum(np.abs(a["_nact"] - res[k]["_nact"]))),
                                     "max_damage_difference": float(np.max(np.abs(a["_dmg"] - res[k]["_dmg"]))),
@@ -571,6 +577,13 @@ def main():
         log("[bench] state-based legs")
         out["state_based"] = {k: state_based_leg(k, (160, 80, 40), 100) for k in ("OSB", "CPD")}
         out["state_based"]["OSB_2M"] = osb_2m_leg(shape, 100)
+        sb = out["state_based"]
+        sb["OSB_mixed"] = state_based_leg("OSB", (160, 80, 40), 100, precision="mixed")
+        sb["OSB_2M_mixed"] = osb_2m_leg(shape, 100, precision="mixed")
+        for a, m in (("OSB", "OSB_mixed"), ("OSB_2M", "OSB_2M_mixed")):
+            sb[m]["speedup_vs_fp64_step"] = sb[a]["ms_per_step"] / sb[m]["ms_per_step"]
+            sb[m]["speedup_vs_fp64_kernels"] = ((sb[a]["pass1_ms"] + sb[a]["pass2_ms"]) /
+                                                (sb[m]["pass1_ms"] + sb[m]["pass2_ms"]))
         log("[bench] mode-I plate leg")
         out["mode_i"] = mode_i_leg(500, 4, 2000)
         log("[bench] export leg")

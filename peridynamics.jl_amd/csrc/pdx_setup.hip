@@ -192,8 +192,8 @@ extern "C" pdx_status pdx_set_material(pdx_ctx *c, int32_t kind, const pdx_param
         return pdx_fail(c, PDX_ERR_ARG, "unknown material kind %d", kind);
     if (!(p->delta > 0) || !(p->rho > 0))
         return pdx_fail(c, PDX_ERR_ARG, "horizon and rho have to be larger than zero");
-    if (kind != PDX_BB && c->precision == PDX_MIXED)
-        return pdx_fail(c, PDX_ERR_UNSUPPORTED, "mixed precision is provided for BBMaterial only");
+    if (kind == PDX_CPD && c->precision == PDX_MIXED)
+        return pdx_fail(c, PDX_ERR_UNSUPPORTED, "mixed precision is provided for BBMaterial and OSBMaterial");
     if (kind == PDX_CPD && p->model != PDX_SVK && p->model != PDX_NHP)
         return pdx_fail(c, PDX_ERR_ARG, "unknown constitutive model %d", p->model);
     if (p->kernel != 0) return pdx_fail(c, PDX_ERR_UNSUPPORTED, "only linear_kernel is built");

@@ -584,6 +584,244 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds_mx(pdx_step_args A)
 }
 
 // ----------------------------------------------------------------------------------------
+// Mixed-precision OSB (one parameter set): the two passes of k_osb_lds1/2 with the bond
+// arithmetic of k_bb_lds_mx.  The stretch comes from the block-relative reference bond
+// vector D and displacement difference U in FP32 (e = l - L = (2 D.U + U.U) / (l + L), no
+// cancellation); the weighted volume, the dilatation sum and the force density are summed
+// in FP64 from FP32 bond terms; c1, c2 and the Verlet tail are FP64.  Pass 1 stores
+// (c1, c2); pass 2 forms both terms of a bond, (w/l)[(c2_i L + c1_i e) V_j + (c2 V)_j L +
+// (c1 V)_j e], from 36-byte table entries (D, V), (U, (c1 V)_j), (c2 V)_j, so that pass 1
+// carries no force sums.  Failure: FP32 stretches within 1e-4 of eps_c are decided on the
+// FP64 stretch.
+// ----------------------------------------------------------------------------------------
+__device__ __forceinline__ void mx_stage_xu(const pdx_step_args &A, int64_t j, double4 O,
+                                            double uo0, double uo1, double uo2, float4 &a, float4 &b)
+{
+    const double4 X = A.Xr[j], x = A.rec_in[j];
+    a = make_float4((float)(X.x - O.x), (float)(X.y - O.y), (float)(X.z - O.z), (float)X.w);
+    b = make_float4((float)((x.x - X.x) - uo0), (float)((x.y - X.y) - uo1), (float)((x.z - X.z) - uo2), 0.f);
+}
+
+template <int U, int S, int F>
+__global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds1_mx(pdx_step_args A)
+{
+    static_assert((F & PDX_FEAT_MULTI) == 0, "mixed OSB takes one parameter set");
+    constexpr bool SC = (F & PDX_FEAT_SC) != 0;
+    constexpr bool CAP = (F & PDX_FEAT_CAP) != 0;
+    extern __shared__ float4 hmx[];
+    const int64_t k = pdx_xcd_block();
+    const int64_t p0 = k * PDX_BLOCK;
+    if (p0 >= A.n) return;  // whole workgroup
+    const int64_t h0 = A.halo_ptr[k];
+    const int H = (int)(A.halo_ptr[k + 1] - h0);
+    const double4 O = A.Xr[p0], xo = A.rec_in[p0];
+    const double uo0 = xo.x - O.x, uo1 = xo.y - O.y, uo2 = xo.z - O.z;
+    const int HL = CAP ? pdx_lds_entries(A.halo_max, A.lds_budget, 32) : A.halo_max;
+    const int Hs = H < HL ? H : HL;
+    const int32_t *const hidx = A.halo + h0;
+    for (int h = threadIdx.x; h < Hs; h += PDX_BLOCK * S) {
+        float4 a, b;
+        mx_stage_xu(A, hidx[h], O, uo0, uo1, uo2, a, b);
+        hmx[2 * h] = a;
+        hmx[2 * h + 1] = b;
+    }
+    __syncthreads();
+    const int sub = threadIdx.x % S;
+    const int64_t p = p0 + threadIdx.x / S;
+    if (p >= A.n) return;  // all S lanes of the point together
+    const int c = A.cnt[p];
+    const int64_t b0 = pdx_bond0(A.slice_ptr, p);
+    uint16_t *const wbase = A.bw16 + b0;
+    const double4 ri = A.rec_in[p];
+    const double4 Xi = A.Xr[p];
+    const float xi = (float)(Xi.x - O.x), yi = (float)(Xi.y - O.y), zi = (float)(Xi.z - O.z);
+    const float ui = (float)((ri.x - Xi.x) - uo0), vi = (float)((ri.y - Xi.y) - uo1),
+                wi = (float)((ri.z - Xi.z) - uo2);
+    double wvol = 0.0, Ssum = 0.0;
+    int na = 0;
+    const double epsc = A.epsc;
+    const float epscf = (float)epsc, deltaf = (float)A.delta;
+    const float win_lo = epscf * 0.9999f, win_hi = epscf * 1.0001f;
+    for (int t = sub; t < c; t += U * S) {
+        uint32_t wd[U];
+        float sc[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            wd[u] = (t + u * S < c) ? wbase[(int64_t)(t + u * S) * 64] : 0u;
+            sc[u] = (SC && t + u * S < c) ? A.scf32[b0 + (int64_t)(t + u * S) * 64] : 1.f;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (t + u * S >= c) continue;
+            const int slot = (int)(wd[u] & PDX_W16_SLOT);
+            float4 a, b;
+            if (!CAP || slot < HL) {
+                a = hmx[2 * slot];
+                b = hmx[2 * slot + 1];
+            } else {  // past the staged entries: the same values from global memory
+                mx_stage_xu(A, hidx[slot], O, uo0, uo1, uo2, a, b);
+            }
+            const float Dx = a.x - xi, Dy = a.y - yi, Dz = a.z - zi;
+            const float Ux = b.x - ui, Uy = b.y - vi, Uz = b.z - wi;
+            const float L2 = __builtin_fmaf(Dz, Dz, __builtin_fmaf(Dy, Dy, Dx * Dx));
+            const float DU = __builtin_fmaf(Dz, Uz, __builtin_fmaf(Dy, Uy, Dx * Ux));
+            const float UU = __builtin_fmaf(Uz, Uz, __builtin_fmaf(Uy, Uy, Ux * Ux));
+            const float num = __builtin_fmaf(2.f, DU, UU);  // l^2 - L^2
+            const float Lb = __builtin_amdgcn_sqrtf(L2);
+            const float l = __builtin_amdgcn_sqrtf(L2 + num);
+            const float rL = __builtin_amdgcn_rcpf(Lb);
+            const float e = num * __builtin_amdgcn_rcpf(l + Lb);
+            const float eps = e * rL;
+            bool act = (wd[u] & PDX_W16_ACT) != 0;
+            if (act && (wd[u] & PDX_W16_FP) && eps > win_lo) {
+                bool brk = eps > win_hi;
+                if (!brk) {
+                    const double L64 = A.L[b0 + (int64_t)(t + u * S) * 64];
+                    const double4 xj = A.rec_in[hidx[slot]];
+                    const double dx = xj.x - ri.x, dy = xj.y - ri.y, dz = xj.z - ri.z;
+                    double rl;
+                    const double l64 = pdx_sqrt_rsq((dx * dx + dy * dy) + dz * dz, rl);
+                    brk = ((l64 - L64) / L64) > epsc;
+                }
+                if (brk) {
+                    act = false;
+                    wbase[(int64_t)(t + u * S) * 64] = (uint16_t)(wd[u] & ~PDX_W16_ACT);
+                }
+            }
+            if (act) {
+                float w = deltaf * rL;  // linear_kernel(delta, L)
+                if (SC) w *= sc[u];
+                const float wVL = (w * a.w) * Lb;
+                wvol += (double)(wVL * Lb);
+                Ssum += (double)(wVL * e);
+                na++;
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 1; o < S; o <<= 1) {
+        wvol += __shfl_xor(wvol, o);
+        Ssum += __shfl_xor(Ssum, o);
+        na += __shfl_xor(na, o);
+    }
+    if (sub != 0) return;
+    double c1 = 0.0, c2 = 0.0;
+    if (wvol != 0.0) {
+        const double dil = (3.0 / wvol) * Ssum;
+        c1 = (15.0 * A.G) / wvol;
+        c2 = dil * ((3.0 * A.K) / wvol - c1 / 3.0);
+    }
+    reinterpret_cast<double2 *>(A.pstate)[p] = make_double2(c1, c2);
+    if (A.flags & (PDX_F_STORE | PDX_F_FORCE_ONLY)) {
+        A.nact_out[p] = na;
+        A.dmg_out[p] = 1.0 - (double)na / (double)c;
+    }
+}
+
+template <int U, int S, int F>
+__global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds2_mx(pdx_step_args A)
+{
+    static_assert((F & PDX_FEAT_MULTI) == 0, "mixed OSB takes one parameter set");
+    constexpr bool SC = (F & PDX_FEAT_SC) != 0;
+    constexpr bool CAP = (F & PDX_FEAT_CAP) != 0;
+    extern __shared__ float4 hmx[];
+    const int64_t k = pdx_xcd_block();
+    const int64_t p0 = k * PDX_BLOCK;
+    if (p0 >= A.n) return;  // whole workgroup
+    const int64_t h0 = A.halo_ptr[k];
+    const int H = (int)(A.halo_ptr[k + 1] - h0);
+    const double4 O = A.Xr[p0], xo = A.rec_in[p0];
+    const double uo0 = xo.x - O.x, uo1 = xo.y - O.y, uo2 = xo.z - O.z;
+    const int HL = CAP ? pdx_lds_entries(A.halo_max, A.lds_budget, 36) : A.halo_max;
+    const int Hs = H < HL ? H : HL;
+    const int32_t *const hidx = A.halo + h0;
+    const double2 *const cst = reinterpret_cast<const double2 *>(A.pstate);
+    float *const hc2 = reinterpret_cast<float *>(hmx + 2 * HL);
+    for (int h = threadIdx.x; h < Hs; h += PDX_BLOCK * S) {
+        const int64_t j = hidx[h];
+        float4 a, b;
+        mx_stage_xu(A, j, O, uo0, uo1, uo2, a, b);
+        const double2 cj = cst[j];
+        const double Vj = A.Xr[j].w;
+        b.w = (float)(cj.x * Vj);
+        hmx[2 * h] = a;
+        hmx[2 * h + 1] = b;
+        hc2[h] = (float)(cj.y * Vj);
+    }
+    __syncthreads();
+    const int sub = threadIdx.x % S;
+    const int64_t p = p0 + threadIdx.x / S;
+    if (p >= A.n) return;  // all S lanes of the point together
+    const int c = A.cnt[p];
+    const int64_t b0 = pdx_bond0(A.slice_ptr, p);
+    const uint16_t *const wbase = A.bw16 + b0;
+    const double4 ri = A.rec_in[p];
+    const double4 Xi = A.Xr[p];
+    const float xi = (float)(Xi.x - O.x), yi = (float)(Xi.y - O.y), zi = (float)(Xi.z - O.z);
+    const float ui = (float)((ri.x - Xi.x) - uo0), vi = (float)((ri.y - Xi.y) - uo1),
+                wi = (float)((ri.z - Xi.z) - uo2);
+    double bx = 0.0, by = 0.0, bz = 0.0;
+    const double2 ci = cst[p];
+    const float c1i = (float)ci.x, c2i = (float)ci.y;
+    const float deltaf = (float)A.delta;
+    for (int t = sub; t < c; t += U * S) {
+        uint32_t wd[U];
+        float sc[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            wd[u] = (t + u * S < c) ? wbase[(int64_t)(t + u * S) * 64] : 0u;
+            sc[u] = (SC && t + u * S < c) ? A.scf32[b0 + (int64_t)(t + u * S) * 64] : 1.f;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (t + u * S >= c) continue;
+            if (!(wd[u] & PDX_W16_ACT)) continue;  // one set: both copies of the bond agree
+            const int slot = (int)(wd[u] & PDX_W16_SLOT);
+            float4 a, b;
+            float c2Vj;
+            if (!CAP || slot < HL) {
+                a = hmx[2 * slot];
+                b = hmx[2 * slot + 1];
+                c2Vj = hc2[slot];
+            } else {
+                const int64_t j = hidx[slot];
+                mx_stage_xu(A, j, O, uo0, uo1, uo2, a, b);
+                const double2 cj = cst[j];
+                b.w = (float)(cj.x * (double)A.Xr[j].w);
+                c2Vj = (float)(cj.y * (double)A.Xr[j].w);
+            }
+            const float Dx = a.x - xi, Dy = a.y - yi, Dz = a.z - zi;
+            const float Ux = b.x - ui, Uy = b.y - vi, Uz = b.z - wi;
+            const float L2 = __builtin_fmaf(Dz, Dz, __builtin_fmaf(Dy, Dy, Dx * Dx));
+            const float DU = __builtin_fmaf(Dz, Uz, __builtin_fmaf(Dy, Uy, Dx * Ux));
+            const float UU = __builtin_fmaf(Uz, Uz, __builtin_fmaf(Uy, Uy, Ux * Ux));
+            const float num = __builtin_fmaf(2.f, DU, UU);
+            const float Lb = __builtin_amdgcn_sqrtf(L2);
+            const float l = __builtin_amdgcn_sqrtf(L2 + num);
+            const float e = num * __builtin_amdgcn_rcpf(l + Lb);
+            float w = deltaf * __builtin_amdgcn_rcpf(Lb);
+            if (SC) w *= sc[u];
+            // own term (c2_i L + c1_i e) V_j plus reverse term (c2 V)_j L + (c1 V)_j e
+            const float fi = __builtin_fmaf(c1i, e, c2i * Lb) * a.w;
+            const float fj = __builtin_fmaf(b.w, e, c2Vj * Lb);
+            const float sf = (w * __builtin_amdgcn_rcpf(l)) * (fi + fj);
+            bx += (double)(sf * (Dx + Ux));
+            by += (double)(sf * (Dy + Uy));
+            bz += (double)(sf * (Dz + Uz));
+        }
+    }
+#pragma unroll
+    for (int o = 1; o < S; o <<= 1) {
+        bx += __shfl_xor(bx, o);
+        by += __shfl_xor(by, o);
+        bz += __shfl_xor(bz, o);
+    }
+    static_assert(S == 4, "the tail takes four lanes per point");
+    const double bb[3] = {bx, by, bz};
+    point_tail4(A, p, ri, bb, sub, A.rho);
+}
+
+// ----------------------------------------------------------------------------------------
 // OSB pass 1: calc_failure! (ordinary_state_based.jl:107-126), calc_damage!,
 // calc_weighted_volume (:176-189), calc_dilatation (:191-204) -> c1, c2 (:135-136).
 // |dX|^2 is taken as L*L (the reference forms dot(dX, dX); they differ by <= 2 ulp).
@@ -1428,6 +1666,33 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
         PDX_TRY(pdx_timed_end(c, "step"));
         break;
     case PDX_OSB:
+        if (c->lds_mode && c->precision == PDX_MIXED && !c->multi()) {
+            const dim3 BL(PDX_BLOCK * PDX_LDS_LANES);
+            // pass 1 entries (D, V), (U, 0): 32 B; pass 2 also (c2 V)_j: 36 B
+            const int sc = c->corr ? PDX_FEAT_SC : 0;
+            const int feat = sc | (lds_capped(c, 32) ? PDX_FEAT_CAP : 0);
+            const int feat2 = sc | (lds_capped(c, 36) ? PDX_FEAT_CAP : 0);
+            const size_t lds = lds_bytes(c, 32), lds2 = lds_bytes(c, 36);
+            PDX_TRY(pdx_timed_begin(c));
+            switch (feat) {
+            case 0: hipLaunchKernelGGL((k_osb_lds1_mx<4, PDX_LDS_LANES, 0>), G, BL, lds, c->stream, a); break;
+            case PDX_FEAT_SC: hipLaunchKernelGGL((k_osb_lds1_mx<4, PDX_LDS_LANES, PDX_FEAT_SC>), G, BL, lds, c->stream, a); break;
+            case PDX_FEAT_CAP: hipLaunchKernelGGL((k_osb_lds1_mx<4, PDX_LDS_LANES, PDX_FEAT_CAP>), G, BL, lds, c->stream, a); break;
+            default: hipLaunchKernelGGL((k_osb_lds1_mx<4, PDX_LDS_LANES, PDX_FEAT_SC | PDX_FEAT_CAP>), G, BL, lds, c->stream, a); break;
+            }
+            PDX_HIP(c, hipGetLastError());
+            PDX_TRY(pdx_timed_end(c, "pass1"));
+            PDX_TRY(pdx_timed_begin(c));
+            switch (feat2) {
+            case 0: hipLaunchKernelGGL((k_osb_lds2_mx<4, PDX_LDS_LANES, 0>), G, BL, lds2, c->stream, a); break;
+            case PDX_FEAT_SC: hipLaunchKernelGGL((k_osb_lds2_mx<4, PDX_LDS_LANES, PDX_FEAT_SC>), G, BL, lds2, c->stream, a); break;
+            case PDX_FEAT_CAP: hipLaunchKernelGGL((k_osb_lds2_mx<4, PDX_LDS_LANES, PDX_FEAT_CAP>), G, BL, lds2, c->stream, a); break;
+            default: hipLaunchKernelGGL((k_osb_lds2_mx<4, PDX_LDS_LANES, PDX_FEAT_SC | PDX_FEAT_CAP>), G, BL, lds2, c->stream, a); break;
+            }
+            PDX_HIP(c, hipGetLastError());
+            PDX_TRY(pdx_timed_end(c, "step"));
+            break;
+        }
         if (c->lds_mode) {
             const dim3 BL(PDX_BLOCK * PDX_LDS_LANES);
             const int mb = c->multi() ? 1 : 0;  // the neighbours' parameter sets

@@ -28,8 +28,11 @@ def test_step_bytes_formulas():
     assert bench.step_bytes(n_p, n_b, "mixed", lds)[0] == 2 * n_b + 200 * n_p + 8 * slices + 400
     assert bench.step_bytes(n_p, n_b, "fp64", glob)[0] == 12 * n_b + 168 * n_p + 8 * slices
     assert bench.step_bytes(n_p, n_b, "mixed", glob)[0] == 8 * n_b + 168 * n_p + 8 * slices
-    assert bench.state_based_bytes("OSB", n_p, n_b, lds)[0] == 20 * n_b + 236 * n_p + 800
-    assert bench.state_based_bytes("CPD", n_p, n_b, glob)[0] == 24 * n_b + 559 * n_p
+    assert bench.state_based_bytes("OSB", n_p, n_b, lds)[0] == 20 * n_b + 268 * n_p + 800
+    assert bench.state_based_bytes("OSB", n_p, n_b, glob)[0] == 24 * n_b + 236 * n_p
+    assert bench.state_based_bytes("OSB", n_p, n_b, lds, "mixed")[0] == 4 * n_b + 300 * n_p + 800
+    assert bench.state_based_bytes("CPD", n_p, n_b, glob)[0] == 24 * n_b + 415 * n_p
+    assert bench.reference_layout_bytes("OSB", n_p, n_b, "mixed")[0] == 22 * n_b + 248 * n_p
 
 
 def test_measured_traffic_lookup(tmp_path, monkeypatch):

@@ -68,14 +68,16 @@ def reference_layout_bytes(kind, n_points, n_bonds, precision="fp64"):
     vectors written every step), as a yardstick that does not depend on this library's
     layout: BB 13 B per bond + 216 B per point (mixed: FP32 length, 9 B per bond); OSB two
     bond passes + a reverse-bond index, 30 B per bond + 248 B per point (mixed: FP32
-    lengths in both passes, 22 B per bond); CPD 30 B per bond + 576 B per point."""
+    lengths in both passes, 22 B per bond); CPD 30 B per bond + 576 B per point (mixed
+    22 B per bond)."""
     if kind == "BB":
         return ((13 if precision == "fp64" else 9) * n_bonds + 216 * n_points,
                 f"{13 if precision == 'fp64' else 9}*N_b + 216*N_p")
     if kind == "OSB":
         pb = 30 if precision == "fp64" else 22
         return pb * n_bonds + 248 * n_points, f"{pb}*N_b + 248*N_p"
-    return 30 * n_bonds + 576 * n_points, "30*N_b + 576*N_p"
+    pb = 30 if precision == "fp64" else 22
+    return pb * n_bonds + 576 * n_points, f"{pb}*N_b + 576*N_p"
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "bb_step_2M_hbm_traffic.md")
@@ -173,12 +175,13 @@ def state_based_bytes(kind, n_points, n_bonds, layout, precision="fp64"):
     c1, c2 and pass 2 reads them, plus the Verlet tail); OSB, LDS layout, 268 B (pass 1
     writes (c1 V, c2 V) and the own force part, 40 B, pass 2 reads the own part, 24 B);
     mixed OSB 300 B (X|V also read in both passes, (c1, c2) written and read back, no own
-    part); CPD 415 B (also X|V in both passes and the
+    part); mixed CPD 414 B (as FP64 CPD, the state Q' = P K^-1 - beta (F - I)); CPD 415 B (also X|V in both passes and the
     nine numbers of Q = P K^-1 - beta F written and read back).  LDS layout: plus the
     4-byte neighbour-table index per entry and pass."""
-    if kind == "OSB" and precision == "mixed" and layout["lds_mode"]:
-        return (4 * n_bonds + 300 * n_points + 8 * layout["halo_entries"],
-                "4*N_b + 300*N_p + 8*N_halo")
+    if precision == "mixed" and layout["lds_mode"]:
+        per_pt = 300 if kind == "OSB" else 414
+        return (4 * n_bonds + per_pt * n_points + 8 * layout["halo_entries"],
+                f"4*N_b + {per_pt}*N_p + 8*N_halo")
     if kind == "OSB":
         per_pt = 268 if layout["lds_mode"] else 236
     else:
@@ -580,7 +583,8 @@ def main():
         sb = out["state_based"]
         sb["OSB_mixed"] = state_based_leg("OSB", (160, 80, 40), 100, precision="mixed")
         sb["OSB_2M_mixed"] = osb_2m_leg(shape, 100, precision="mixed")
-        for a, m in (("OSB", "OSB_mixed"), ("OSB_2M", "OSB_2M_mixed")):
+        sb["CPD_mixed"] = state_based_leg("CPD", (160, 80, 40), 100, precision="mixed")
+        for a, m in (("OSB", "OSB_mixed"), ("OSB_2M", "OSB_2M_mixed"), ("CPD", "CPD_mixed")):
             sb[m]["speedup_vs_fp64_step"] = sb[a]["ms_per_step"] / sb[m]["ms_per_step"]
             sb[m]["speedup_vs_fp64_kernels"] = ((sb[a]["pass1_ms"] + sb[a]["pass2_ms"]) /
                                                 (sb[m]["pass1_ms"] + sb[m]["pass2_ms"]))

@@ -44,7 +44,7 @@ typedef int32_t pdx_status;
 /* precision */
 #define PDX_FP64 0  /* all arithmetic in FP64 */
 #define PDX_MIXED 1 /* FP32 stretch/force arithmetic from FP64 differences, FP64 sums and
-                       FP64 integration; BBMaterial and OSBMaterial (OSB with several
+                       FP64 integration; BB, OSB and CPD materials (OSB with several
                        parameter sets runs the FP64 kernels) */
 
 /* material kinds */

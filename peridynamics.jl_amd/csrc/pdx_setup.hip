@@ -192,8 +192,6 @@ extern "C" pdx_status pdx_set_material(pdx_ctx *c, int32_t kind, const pdx_param
         return pdx_fail(c, PDX_ERR_ARG, "unknown material kind %d", kind);
     if (!(p->delta > 0) || !(p->rho > 0))
         return pdx_fail(c, PDX_ERR_ARG, "horizon and rho have to be larger than zero");
-    if (kind == PDX_CPD && c->precision == PDX_MIXED)
-        return pdx_fail(c, PDX_ERR_UNSUPPORTED, "mixed precision is provided for BBMaterial and OSBMaterial");
     if (kind == PDX_CPD && p->model != PDX_SVK && p->model != PDX_NHP)
         return pdx_fail(c, PDX_ERR_ARG, "unknown constitutive model %d", p->model);
     if (p->kernel != 0) return pdx_fail(c, PDX_ERR_UNSUPPORTED, "only linear_kernel is built");
@@ -860,8 +858,10 @@ extern "C" pdx_status pdx_build_bonds(pdx_ctx *c, int64_t *n_bonds_out)
         HIP_C(hipStreamSynchronize(c->stream));
     }
     cleanup();
-    // LDS-staged layouts: BB (x|V per neighbour-table entry), OSB (x|V and c1, c2)
-    if ((c->kind == PDX_BB || c->kind == PDX_OSB) && !getenv("PDX_NO_LDS")) {
+    // LDS-staged layouts: BB (x|V per neighbour-table entry), OSB (x|V and c1, c2), mixed
+    // CPD (FP32 records; its FP64 passes gather through the caches)
+    const bool cpd_mx = c->kind == PDX_CPD && c->precision == PDX_MIXED;
+    if ((c->kind == PDX_BB || c->kind == PDX_OSB || cpd_mx) && !getenv("PDX_NO_LDS")) {
         bool ok = false;
         const int64_t entry = (c->kind == PDX_OSB ? (int64_t)(sizeof(double4) + sizeof(double2))
                                                   : (int64_t)sizeof(double4)) +

@@ -1485,6 +1485,261 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_pass2(pdx_step_args A)
 }
 
 // ----------------------------------------------------------------------------------------
+// Mixed-precision CPD (LDS-staged, as the mixed OSB passes).  The deformation gradient is
+// formed as F = I + H with H = (sum w V_j U (x) D) K^-1, D the reference bond vector and U
+// the displacement difference in FP32 (block-relative, as in k_bb_lds_mx); K and the H sum
+// are accumulated in FP64 from FP32 bond terms, and K^-1, F, P and the per-point state are
+// FP64.  The ZEMSilling bond force w [(P K^-1) D + beta (d - F D)] is written as
+// w (Q' D + beta U) with Q' = P K^-1 - beta H: U - H D is the small difference d - F D
+// without the cancellation of d - F D in FP32.  Pass 1 stores Q' (9 numbers per point);
+// pass 2 stages (D, V), (U, beta [j alive]), [j alive] Q'_j (68 B per table entry) and
+// gathers both terms of every bond.
+// ----------------------------------------------------------------------------------------
+template <int U, int S, int F>
+__global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_pass1_mx(pdx_step_args A)
+{
+    constexpr bool CAP = (F & PDX_FEAT_CAP) != 0;
+    extern __shared__ float4 hmx[];
+    const int64_t k = pdx_xcd_block();
+    const int64_t p0 = k * PDX_BLOCK;
+    if (p0 >= A.n) return;  // whole workgroup
+    const int64_t h0 = A.halo_ptr[k];
+    const int H = (int)(A.halo_ptr[k + 1] - h0);
+    const double4 O = A.Xr[p0], xo = A.rec_in[p0];
+    const double uo0 = xo.x - O.x, uo1 = xo.y - O.y, uo2 = xo.z - O.z;
+    const int HL = CAP ? pdx_lds_entries(A.halo_max, A.lds_budget, 32) : A.halo_max;
+    const int Hs = H < HL ? H : HL;
+    const int32_t *const hidx = A.halo + h0;
+    for (int h = threadIdx.x; h < Hs; h += PDX_BLOCK * S) {
+        float4 a, b;
+        mx_stage_xu(A, hidx[h], O, uo0, uo1, uo2, a, b);
+        hmx[2 * h] = a;
+        hmx[2 * h + 1] = b;
+    }
+    __syncthreads();
+    const int sub = threadIdx.x % S;
+    const int64_t p = p0 + threadIdx.x / S;
+    if (p >= A.n) return;  // all S lanes of the point together
+    const int c = A.cnt[p];
+    const int64_t b0 = pdx_bond0(A.slice_ptr, p);
+    uint16_t *const wbase = A.bw16 + b0;
+    const double4 ri = A.rec_in[p];
+    const double4 Xi = A.Xr[p];
+    const float xi = (float)(Xi.x - O.x), yi = (float)(Xi.y - O.y), zi = (float)(Xi.z - O.z);
+    const float ui = (float)((ri.x - Xi.x) - uo0), vi = (float)((ri.y - Xi.y) - uo1),
+                wi = (float)((ri.z - Xi.z) - uo2);
+    const bool dead = A.killed[p] != 0;
+    double Km[6] = {0, 0, 0, 0, 0, 0};  // 00 10 20 11 21 22
+    double Ht[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int na = 0;
+    const double epsc = A.epsc;
+    const float epscf = (float)epsc, deltaf = (float)A.delta;
+    const float win_lo = epscf * 0.9999f, win_hi = epscf * 1.0001f;
+    for (int t = sub; t < c; t += U * S) {
+        uint32_t wd[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) wd[u] = (t + u * S < c) ? wbase[(int64_t)(t + u * S) * 64] : 0u;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (t + u * S >= c) continue;
+            const int slot = (int)(wd[u] & PDX_W16_SLOT);
+            float4 a, b;
+            if (!CAP || slot < HL) {
+                a = hmx[2 * slot];
+                b = hmx[2 * slot + 1];
+            } else {
+                mx_stage_xu(A, hidx[slot], O, uo0, uo1, uo2, a, b);
+            }
+            const float D[3] = {a.x - xi, a.y - yi, a.z - zi};
+            const float Uv[3] = {b.x - ui, b.y - vi, b.z - wi};
+            const float L2 = __builtin_fmaf(D[2], D[2], __builtin_fmaf(D[1], D[1], D[0] * D[0]));
+            const float DU = __builtin_fmaf(D[2], Uv[2], __builtin_fmaf(D[1], Uv[1], D[0] * Uv[0]));
+            const float UU = __builtin_fmaf(Uv[2], Uv[2], __builtin_fmaf(Uv[1], Uv[1], Uv[0] * Uv[0]));
+            const float num = __builtin_fmaf(2.f, DU, UU);  // l^2 - L^2
+            const float Lb = __builtin_amdgcn_sqrtf(L2);
+            const float l = __builtin_amdgcn_sqrtf(L2 + num);
+            const float rL = __builtin_amdgcn_rcpf(Lb);
+            const float eps = (num * __builtin_amdgcn_rcpf(l + Lb)) * rL;
+            bool act = (wd[u] & PDX_W16_ACT) != 0;
+            if (act && (wd[u] & PDX_W16_FP) && eps > win_lo) {
+                bool brk = eps > win_hi;
+                if (!brk) {
+                    const double L64 = A.L[b0 + (int64_t)(t + u * S) * 64];
+                    const double4 xj = A.rec_in[hidx[slot]];
+                    const double dx = xj.x - ri.x, dy = xj.y - ri.y, dz = xj.z - ri.z;
+                    double rl;
+                    const double l64 = pdx_sqrt_rsq((dx * dx + dy * dy) + dz * dz, rl);
+                    brk = ((l64 - L64) / L64) > epsc;
+                }
+                if (brk) {
+                    act = false;
+                    wbase[(int64_t)(t + u * S) * 64] = (uint16_t)(wd[u] & ~PDX_W16_ACT);
+                }
+            }
+            if (act && !dead) {
+                na++;
+                const float wv = (deltaf * rL) * a.w;  // w V_j
+                Km[0] += (double)(wv * (D[0] * D[0]));
+                Km[1] += (double)(wv * (D[1] * D[0]));
+                Km[2] += (double)(wv * (D[2] * D[0]));
+                Km[3] += (double)(wv * (D[1] * D[1]));
+                Km[4] += (double)(wv * (D[2] * D[1]));
+                Km[5] += (double)(wv * (D[2] * D[2]));
+#pragma unroll
+                for (int cc = 0; cc < 3; cc++)
+#pragma unroll
+                    for (int r = 0; r < 3; r++) Ht[r + 3 * cc] += (double)(wv * (Uv[r] * D[cc]));
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 1; o < S; o <<= 1) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) Km[q] += __shfl_xor(Km[q], o);
+#pragma unroll
+        for (int q = 0; q < 9; q++) Ht[q] += __shfl_xor(Ht[q], o);
+        na += __shfl_xor(na, o);
+    }
+    if (sub != 0) return;
+    const double K9[9] = {Km[0], Km[1], Km[2], Km[1], Km[3], Km[4], Km[2], Km[4], Km[5]};
+    double Kinv[9], Hm[9], Fm[9];
+    m3_inv(K9, Kinv);
+    m3_mul(Ht, Kinv, Hm);
+#pragma unroll
+    for (int q = 0; q < 9; q++) Fm[q] = Hm[q] + ((q % 4 == 0) ? 1.0 : 0.0);
+    const double dmg = 1.0 - (double)na / (double)c;
+    bool hasnan = false;
+#pragma unroll
+    for (int q = 0; q < 9; q++) hasnan |= isnan(Fm[q]);
+    const bool kill = dead || dmg > A.maxdmg || hasnan;
+    const bool store = (A.flags & (PDX_F_STORE | PDX_F_FORCE_ONLY)) != 0;
+    double PK[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (!kill) {
+        double P[9];
+        pk1(A.model, Fm, A.lambda, A.mu, A.G, A.K, P);
+        m3_mul(P, Kinv, PK);
+        if (store) {
+            const double sj = 1.0 / m3_det(Fm);
+#pragma unroll
+            for (int q = 0; q < 9; q++) {
+                const int r = q % 3, cc = q / 3;
+                A.cauchy[9 * p + q] =
+                    ((sj * P[r]) * Fm[cc] + (sj * P[r + 3]) * Fm[cc + 3]) + (sj * P[r + 6]) * Fm[cc + 6];
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+        A.pstate[9 * p + q] = PK[q] - A.beta_zem * Hm[q];
+        if (store) A.defgrad[9 * p + q] = Fm[q];
+    }
+    if (kill && !dead) A.killed[p] = 1;
+    if (store) {
+        A.nact_out[p] = kill ? 0 : na;
+        A.dmg_out[p] = dmg;
+    }
+}
+
+// the staged values of neighbour j for pass 2: (D, V), (U, beta [j alive]), [j alive] Q'_j
+__device__ __forceinline__ void cpd_mx_entry(const pdx_step_args &A, int64_t j, double4 O, double uo0,
+                                             double uo1, double uo2, float4 &a, float4 &b, float q[9])
+{
+    mx_stage_xu(A, j, O, uo0, uo1, uo2, a, b);
+    const float live = A.killed[j] ? 0.f : 1.f;
+    b.w = (float)A.beta_zem * live;
+#pragma unroll
+    for (int r = 0; r < 9; r++) q[r] = (float)A.pstate[9 * j + r] * live;
+}
+
+template <int U, int S, int F>
+__global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_pass2_mx(pdx_step_args A)
+{
+    constexpr bool CAP = (F & PDX_FEAT_CAP) != 0;
+    extern __shared__ float4 hmx[];  // [4 HL]: (D, V), (U, beta l), q0-3, q4-7; then q8 [HL]
+    const int64_t k = pdx_xcd_block();
+    const int64_t p0 = k * PDX_BLOCK;
+    if (p0 >= A.n) return;  // whole workgroup
+    const int64_t h0 = A.halo_ptr[k];
+    const int H = (int)(A.halo_ptr[k + 1] - h0);
+    const double4 O = A.Xr[p0], xo = A.rec_in[p0];
+    const double uo0 = xo.x - O.x, uo1 = xo.y - O.y, uo2 = xo.z - O.z;
+    const int HL = CAP ? pdx_lds_entries(A.halo_max, A.lds_budget, 68) : A.halo_max;
+    const int Hs = H < HL ? H : HL;
+    const int32_t *const hidx = A.halo + h0;
+    float *const hq8 = reinterpret_cast<float *>(hmx + 4 * HL);
+    for (int h = threadIdx.x; h < Hs; h += PDX_BLOCK * S) {
+        float4 a, b;
+        float q[9];
+        cpd_mx_entry(A, hidx[h], O, uo0, uo1, uo2, a, b, q);
+        hmx[4 * h] = a;
+        hmx[4 * h + 1] = b;
+        hmx[4 * h + 2] = make_float4(q[0], q[1], q[2], q[3]);
+        hmx[4 * h + 3] = make_float4(q[4], q[5], q[6], q[7]);
+        hq8[h] = q[8];
+    }
+    __syncthreads();
+    const int sub = threadIdx.x % S;
+    const int64_t p = p0 + threadIdx.x / S;
+    if (p >= A.n) return;  // all S lanes of the point together
+    const int c = A.cnt[p];
+    const int64_t b0 = pdx_bond0(A.slice_ptr, p);
+    const uint16_t *const wbase = A.bw16 + b0;
+    const double4 ri = A.rec_in[p];
+    const double4 Xi = A.Xr[p];
+    const float xi = (float)(Xi.x - O.x), yi = (float)(Xi.y - O.y), zi = (float)(Xi.z - O.z);
+    const float ui = (float)((ri.x - Xi.x) - uo0), vi = (float)((ri.y - Xi.y) - uo1),
+                wi = (float)((ri.z - Xi.z) - uo2);
+    const float live_i = A.killed[p] ? 0.f : 1.f;
+    const float beta_i = (float)A.beta_zem * live_i;
+    float Qi[9];
+#pragma unroll
+    for (int r = 0; r < 9; r++) Qi[r] = (float)A.pstate[9 * p + r] * live_i;
+    const float deltaf = (float)A.delta;
+    double b3[3] = {0.0, 0.0, 0.0};
+    for (int t = sub; t < c; t += U * S) {
+        uint32_t wd[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) wd[u] = (t + u * S < c) ? wbase[(int64_t)(t + u * S) * 64] : 0u;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (t + u * S >= c) continue;
+            if (!(wd[u] & PDX_W16_ACT)) continue;
+            const int slot = (int)(wd[u] & PDX_W16_SLOT);
+            float4 a, b;
+            float q[9];
+            if (!CAP || slot < HL) {
+                a = hmx[4 * slot];
+                b = hmx[4 * slot + 1];
+                const float4 q0 = hmx[4 * slot + 2], q1 = hmx[4 * slot + 3];
+                q[0] = q0.x; q[1] = q0.y; q[2] = q0.z; q[3] = q0.w;
+                q[4] = q1.x; q[5] = q1.y; q[6] = q1.z; q[7] = q1.w;
+                q[8] = hq8[slot];
+            } else {
+                cpd_mx_entry(A, hidx[slot], O, uo0, uo1, uo2, a, b, q);
+            }
+            const float D[3] = {a.x - xi, a.y - yi, a.z - zi};
+            const float Uv[3] = {b.x - ui, b.y - vi, b.z - wi};
+            const float L2 = __builtin_fmaf(D[2], D[2], __builtin_fmaf(D[1], D[1], D[0] * D[0]));
+            const float wv = (deltaf * __builtin_amdgcn_rsqf(L2)) * a.w;  // w V_j
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                // own term Q'_i D + beta U, reverse term Q'_j D + beta U (zero for dead ends)
+                const float qs0 = Qi[r] + q[r], qs1 = Qi[r + 3] + q[r + 3], qs2 = Qi[r + 6] + q[r + 6];
+                const float tt = __builtin_fmaf(qs2, D[2], __builtin_fmaf(qs1, D[1], qs0 * D[0])) +
+                                 (beta_i + b.w) * Uv[r];
+                b3[r] += (double)(wv * tt);
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 1; o < S; o <<= 1)
+#pragma unroll
+        for (int r = 0; r < 3; r++) b3[r] += __shfl_xor(b3[r], o);
+    static_assert(S == 4, "the tail takes four lanes per point");
+    point_tail4(A, p, ri, b3, sub, A.rho);
+}
+
+// ----------------------------------------------------------------------------------------
 // prologue: the first half step of a run (update_vel_half!, BCs, update_disp_and_pos!)
 // ----------------------------------------------------------------------------------------
 __global__ void k_vv_prologue(int64_t n, const double *v, const double *a, double *vh,
@@ -1731,6 +1986,26 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
         PDX_TRY(pdx_timed_end(c, "step"));
         break;
     case PDX_CPD:
+        if (c->lds_mode && c->precision == PDX_MIXED) {
+            // pass 1 entries (D, V), (U, 0): 32 B; pass 2 (D, V), (U, beta l), l Q': 68 B
+            const dim3 BL(PDX_BLOCK * PDX_LDS_LANES);
+            const size_t l1 = lds_bytes(c, 32), l2b = lds_bytes(c, 68);
+            PDX_TRY(pdx_timed_begin(c));
+            if (lds_capped(c, 32))
+                hipLaunchKernelGGL((k_cpd_pass1_mx<1, PDX_LDS_LANES, PDX_FEAT_CAP>), G, BL, l1, c->stream, a);
+            else
+                hipLaunchKernelGGL((k_cpd_pass1_mx<1, PDX_LDS_LANES, 0>), G, BL, l1, c->stream, a);
+            PDX_HIP(c, hipGetLastError());
+            PDX_TRY(pdx_timed_end(c, "pass1"));
+            PDX_TRY(pdx_timed_begin(c));
+            if (lds_capped(c, 68))
+                hipLaunchKernelGGL((k_cpd_pass2_mx<2, PDX_LDS_LANES, PDX_FEAT_CAP>), G, BL, l2b, c->stream, a);
+            else
+                hipLaunchKernelGGL((k_cpd_pass2_mx<2, PDX_LDS_LANES, 0>), G, BL, l2b, c->stream, a);
+            PDX_HIP(c, hipGetLastError());
+            PDX_TRY(pdx_timed_end(c, "step"));
+            break;
+        }
         PDX_TRY(pdx_timed_begin(c));
         hipLaunchKernelGGL(k_cpd_pass1<PDX_LDS_LANES>, G, dim3(PDX_BLOCK * PDX_LDS_LANES), 0, c->stream, a);
         PDX_HIP(c, hipGetLastError());

@@ -1,5 +1,6 @@
-"""Mixed-precision OSB step (FP32 bond arithmetic from FP64 differences, FP64 sums, FP64
-Verlet tail) against the FP64 oracle of ordinary_state_based.jl:107-204.
+"""Mixed-precision OSB and CPD steps (FP32 bond arithmetic from FP64 differences, FP64
+sums, FP64 per-point state and Verlet tail) against the FP64 oracle of
+ordinary_state_based.jl:107-204 and correspondence.jl:193-251.
 
 Bars: force density within 1e-5 of the per-point scale (the sum of the absolute bond
 contributions to the point), bond states identical to the oracle's (stretches near eps_c
@@ -119,7 +120,69 @@ def test_mixed_osb_torsion_trajectory(need_gpu, oracle):
     dh.close()
 
 
-def test_mixed_cpd_rejected(need_gpu):
-    b = body_block("CPD", 6, 6, 4, model=pd.SaintVenantKirchhoff(), Gc=None)
-    with pytest.raises(pd.PdxError):
-        pd.MI355XBodyDataHandler(b, precision="mixed")
+CPD_MODELS = {"SVK": pd.SaintVenantKirchhoff, "NHP": pd.NeoHookePenalty}
+
+
+@pytest.mark.parametrize("model", ["SVK", "NHP"])
+@pytest.mark.parametrize("budget", [0, 12288])
+@pytest.mark.parametrize("amp", [2e-7, 3e-6])
+def test_mixed_cpd_force(need_gpu, oracle, amp, budget, model):
+    """Mixed CPD (correspondence.jl:193-251): force density, bond states and the
+    deformation gradient against the FP64 oracle, with bond failure at the larger
+    amplitude."""
+    say("mixed cpd force", amp, budget, model)
+    b = body_block("CPD", 24, 20, 8, jit=0.05, E=210e9, rho=7850.0, Gc=20.0,
+                   model=CPD_MODELS[model]())
+    with lds_budget(budget):
+        dh = pd.MI355XBodyDataHandler(b, precision="mixed")
+    orc = oracle_for(oracle, b, "CPD")
+    x = b.xyz + displacement_field(b.xyz, amp)
+    dh.set("position", x)
+    dh.calc_force_density(0.0, 1e-7)
+    orc.force(x)
+    bg = dh.get("b_int")
+    err = scaled_error(bg, orc)
+    _, _, _, _, act = dh.bonds()
+    broken = int(np.sum(orc.active == 0))
+    mism = int(np.sum(act != orc.active))
+    dg = dh.get("defgrad")
+    fin = np.isfinite(orc.defgrad).all(1)
+    dgerr = normwise(dg[fin], orc.defgrad[fin])
+    say(f"  scaled error {err:.3e}, normwise {normwise(bg, orc.b_int):.3e}, "
+        f"broken {broken}, mismatches {mism}, defgrad {dgerr:.3e}")
+    assert err <= MIXED_TOL
+    assert mism == 0
+    assert dgerr < 1e-8
+    if amp > 1e-6:
+        assert broken > 0
+    dh.close()
+
+
+def test_mixed_cpd_torsion_trajectory(need_gpu, oracle):
+    """The torsion bar with CMaterial (Saint Venant-Kirchhoff, ZEMSilling), 40 steps."""
+    say("mixed cpd torsion")
+    b = body_block("CPD", 16, 8, 8, dx=1e-3, E=210e9, rho=7850.0, Gc=None, jit=0.02,
+                   model=pd.SaintVenantKirchhoff())
+    x0 = b.position[0]
+    pd.point_set(b, "end", x0 > x0.max() - 2.5e-3)
+    pd.point_set(b, "fixed", x0 < x0.min() + 2.5e-3)
+    pd.velocity_bc(lambda p, t: -10.0 * p[2], b, "end", "y")
+    pd.velocity_bc(lambda p, t: 10.0 * p[1], b, "end", "z")
+    for d in "xyz":
+        pd.velocity_bc(lambda t: 0.0, b, "fixed", d)
+    vv = pd.VelocityVerlet(steps=40)
+    dh = pd.MI355XBodyDataHandler(b, precision="mixed")
+    dh.init_time_solver(vv)
+    dh.initialize(vv)
+    dh.solve(vv)
+    orc = oracle_for(oracle, b, "CPD")
+    orc.force()
+    bcs = [("pd", b.point_sets["end"], 1, lambda p, t: -10.0 * p[2]),
+           ("pd", b.point_sets["end"], 2, lambda p, t: 10.0 * p[1])]
+    bcs += [("sd", b.point_sets["fixed"], d, lambda t: 0.0) for d in range(3)]
+    orc.vv_run(40, vv.dt, bcs)
+    for f, ref in (("displacement", orc.u), ("velocity", orc.v)):
+        err = normwise(dh.get(f), ref)
+        say(f"  {f} normwise error {err:.3e}")
+        assert err < 1e-5
+    dh.close()

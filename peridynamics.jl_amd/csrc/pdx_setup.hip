@@ -858,10 +858,9 @@ extern "C" pdx_status pdx_build_bonds(pdx_ctx *c, int64_t *n_bonds_out)
         HIP_C(hipStreamSynchronize(c->stream));
     }
     cleanup();
-    // LDS-staged layouts: BB (x|V per neighbour-table entry), OSB (x|V and c1, c2), mixed
-    // CPD (FP32 records; its FP64 passes gather through the caches)
-    const bool cpd_mx = c->kind == PDX_CPD && c->precision == PDX_MIXED;
-    if ((c->kind == PDX_BB || c->kind == PDX_OSB || cpd_mx) && !getenv("PDX_NO_LDS")) {
+    // LDS-staged layouts: BB (x|V per neighbour-table entry), OSB (x|V and c1, c2), CPD
+    // (tables staged in tiles)
+    if (!getenv("PDX_NO_LDS")) {
         bool ok = false;
         const int64_t entry = (c->kind == PDX_OSB ? (int64_t)(sizeof(double4) + sizeof(double2))
                                                   : (int64_t)sizeof(double4)) +

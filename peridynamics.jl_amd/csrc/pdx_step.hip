@@ -1314,6 +1314,50 @@ __device__ void pk1(int model, const double *F, double lambda, double mu, double
     m3_mul(F, S, P);
 }
 
+// The per-point part of CPD pass 1 (FP64): F = Ft K^-1, too_much_damage!, P, and
+// Q = P K^-1 - beta F stored for pass 2; defgrad, cauchy_stress, damage and the active-bond
+// count on export steps.  Km = K's six distinct entries (00 10 20 11 21 22).
+__device__ __forceinline__ void cpd_point_state(const pdx_step_args &A, int64_t p, const double Km[6],
+                                             const double Ft[9], int na, int c, bool dead)
+{
+    const double K9[9] = {Km[0], Km[1], Km[2], Km[1], Km[3], Km[4], Km[2], Km[4], Km[5]};
+    double Kinv[9], F[9];
+    m3_inv(K9, Kinv);
+    m3_mul(Ft, Kinv, F);
+    const double dmg = 1.0 - (double)na / (double)c;
+    bool hasnan = false;
+#pragma unroll
+    for (int k = 0; k < 9; k++) hasnan |= isnan(F[k]);
+    const bool kill = dead || dmg > A.maxdmg || hasnan;
+    const bool store = (A.flags & (PDX_F_STORE | PDX_F_FORCE_ONLY)) != 0;
+    double PK[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (!kill) {
+        double P[9];
+        pk1(A.model, F, A.lambda, A.mu, A.G, A.K, P);
+        m3_mul(P, Kinv, PK);
+        if (store) {
+            // cauchy_stress: stress.jl:112-116, sigma = (1/J) P F'
+            const double sj = 1.0 / m3_det(F);
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                const int r = k % 3, cc = k / 3;
+                A.cauchy[9 * p + k] =
+                    ((sj * P[r]) * F[cc] + (sj * P[r + 3]) * F[cc + 3]) + (sj * P[r + 6]) * F[cc + 6];
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        A.pstate[9 * p + k] = PK[k] - A.beta_zem * F[k];
+        if (store) A.defgrad[9 * p + k] = F[k];
+    }
+    if (kill && !dead) A.killed[p] = 1;
+    if (store) {
+        A.nact_out[p] = kill ? 0 : na;
+        A.dmg_out[p] = dmg;
+    }
+}
+
 // CPD pass 1: calc_failure! (bond_system.jl:309-327), calc_damage!,
 // calc_deformation_gradient! (correspondence.jl:193-214), too_much_damage! (:288-298),
 // calc_first_piola_kirchhoff! (:216-224).  Stores Q = P K^-1 - beta F of every point: the
@@ -1386,42 +1430,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_pass1(pdx_step_args A)
         na += __shfl_xor(na, o);
     }
     if (sub != 0) return;
-    const double K9[9] = {Km[0], Km[1], Km[2], Km[1], Km[3], Km[4], Km[2], Km[4], Km[5]};
-    double Kinv[9], F[9];
-    m3_inv(K9, Kinv);
-    m3_mul(Ft, Kinv, F);
-    const double dmg = 1.0 - (double)na / (double)c;
-    bool hasnan = false;
-#pragma unroll
-    for (int k = 0; k < 9; k++) hasnan |= isnan(F[k]);
-    const bool kill = dead || dmg > A.maxdmg || hasnan;
-    const bool store = (A.flags & (PDX_F_STORE | PDX_F_FORCE_ONLY)) != 0;
-    double PK[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (!kill) {
-        double P[9];
-        pk1(A.model, F, A.lambda, A.mu, A.G, A.K, P);
-        m3_mul(P, Kinv, PK);
-        if (store) {
-            // cauchy_stress: stress.jl:112-116, sigma = (1/J) P F'
-            const double sj = 1.0 / m3_det(F);
-#pragma unroll
-            for (int k = 0; k < 9; k++) {
-                const int r = k % 3, cc = k / 3;
-                A.cauchy[9 * p + k] =
-                    ((sj * P[r]) * F[cc] + (sj * P[r + 3]) * F[cc + 3]) + (sj * P[r + 6]) * F[cc + 6];
-            }
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < 9; k++) {
-        A.pstate[9 * p + k] = PK[k] - A.beta_zem * F[k];
-        if (store) A.defgrad[9 * p + k] = F[k];
-    }
-    if (kill && !dead) A.killed[p] = 1;
-    if (store) {
-        A.nact_out[p] = kill ? 0 : na;
-        A.dmg_out[p] = dmg;
-    }
+    cpd_point_state(A, p, Km, Ft, na, c, dead);
 }
 
 // CPD pass 2: c_force_density! with ZEMSilling (correspondence.jl:226-251) as a gather:
@@ -1480,6 +1489,180 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_pass2(pdx_step_args A)
     for (int o = 1; o < S; o <<= 1)
 #pragma unroll
         for (int r = 0; r < 3; r++) b3[r] += __shfl_xor(b3[r], o);
+    static_assert(S == 4, "the tail takes four lanes per point");
+    point_tail4(A, p, ri, b3, sub, A.rho);
+}
+
+// ----------------------------------------------------------------------------------------
+// CPD passes on the LDS-staged layout (FP64), for neighbour tables longer than one LDS
+// budget: the block's table is staged in tiles of TL entries.  A point's bonds are sorted
+// by neighbour index, as is the table, so each lane's bonds fall into the tiles in order:
+// the lane walks its bonds while their slots lie in the staged tile and resumes there after
+// the next tile is staged.  Pass 1 entries: x|V, X (64 B); pass 2: x|V, X|beta [j alive],
+// [j alive] Q_j (136 B).  The arithmetic is that of k_cpd_pass1/2.
+// ----------------------------------------------------------------------------------------
+template <int S>
+__global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds1(pdx_step_args A)
+{
+    extern __shared__ double4 hrec[];
+    const int TL = pdx_lds_entries(A.halo_max, A.lds_budget, 64);
+    double4 *const hx = hrec;
+    double4 *const hX = hrec + TL;
+    const int64_t k = pdx_xcd_block();
+    const int64_t p0 = k * PDX_BLOCK;
+    if (p0 >= A.n) return;  // whole workgroup
+    const int64_t h0 = A.halo_ptr[k];
+    const int H = (int)(A.halo_ptr[k + 1] - h0);
+    const int32_t *const hidx = A.halo + h0;
+    const int sub = threadIdx.x % S;
+    const int64_t p = p0 + threadIdx.x / S;
+    const bool valid = p < A.n;  // the block's threads all take part in the tile barriers
+    const int c = valid ? A.cnt[p] : 0;
+    const int64_t b0 = valid ? pdx_bond0(A.slice_ptr, p) : 0;
+    uint16_t *const wbase = A.bw16 + b0;
+    const double *const Lbase = A.L + b0;
+    const double4 ri = valid ? A.rec_in[p] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const double4 Xi = valid ? A.Xr[p] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const bool dead = valid && A.killed[p] != 0;
+    double Km[6] = {0, 0, 0, 0, 0, 0};  // 00 10 20 11 21 22
+    double Ft[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int na = 0;
+    int t = sub;
+    uint32_t wd = t < c ? wbase[(int64_t)t * 64] : 0u;
+    double Lb = t < c ? Lbase[(int64_t)t * 64] : 1.0;
+    for (int T0 = 0; T0 < H; T0 += TL) {
+        const int T1 = H < T0 + TL ? H : T0 + TL;
+        if (T0 > 0) __syncthreads();  // the previous tile is consumed
+        for (int h = T0 + (int)threadIdx.x; h < T1; h += PDX_BLOCK * S) {
+            const int32_t j = hidx[h];
+            hx[h - T0] = A.rec_in[j];
+            hX[h - T0] = A.Xr[j];
+        }
+        __syncthreads();
+        while (t < c) {
+            const int slot = (int)(wd & PDX_W16_SLOT);
+            if (slot >= T1) break;
+            const int tn = t + S;
+            const uint32_t wn = tn < c ? wbase[(int64_t)tn * 64] : 0u;
+            const double Ln = tn < c ? Lbase[(int64_t)tn * 64] : 1.0;
+            const double4 rj = hx[slot - T0];
+            const double4 Xj = hX[slot - T0];
+            const double dx[3] = {rj.x - ri.x, rj.y - ri.y, rj.z - ri.z};
+            const double DX[3] = {Xj.x - Xi.x, Xj.y - Xi.y, Xj.z - Xi.z};
+            double rl;
+            const double l = pdx_sqrt_rsq((dx[0] * dx[0] + dx[1] * dx[1]) + dx[2] * dx[2], rl);
+            const double e = l - Lb;
+            const double rL = pdx_rcp1(Lb);
+            const bool act = bond_fail16(wbase + (int64_t)t * 64, wd, e, Lb, e * rL, A.epsc, A.epsc_lo, A.epsc_hi);
+            if (act && !dead) {
+                na++;
+                const double temp = (A.delta * rL) * rj.w;
+                Km[0] += temp * (DX[0] * DX[0]);
+                Km[1] += temp * (DX[1] * DX[0]);
+                Km[2] += temp * (DX[2] * DX[0]);
+                Km[3] += temp * (DX[1] * DX[1]);
+                Km[4] += temp * (DX[2] * DX[1]);
+                Km[5] += temp * (DX[2] * DX[2]);
+#pragma unroll
+                for (int cc = 0; cc < 3; cc++)
+#pragma unroll
+                    for (int r = 0; r < 3; r++) Ft[r + 3 * cc] += temp * (dx[r] * DX[cc]);
+            }
+            wd = wn;
+            Lb = Ln;
+            t = tn;
+        }
+    }
+#pragma unroll
+    for (int o = 1; o < S; o <<= 1) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) Km[q] += __shfl_xor(Km[q], o);
+#pragma unroll
+        for (int q = 0; q < 9; q++) Ft[q] += __shfl_xor(Ft[q], o);
+        na += __shfl_xor(na, o);
+    }
+    if (!valid || sub != 0) return;
+    cpd_point_state(A, p, Km, Ft, na, c, dead);
+}
+
+template <int S>
+__global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2(pdx_step_args A)
+{
+    extern __shared__ double4 hrec[];
+    const int TL = pdx_lds_entries(A.halo_max, A.lds_budget, 136);
+    double4 *const hx = hrec;
+    double4 *const hX = hrec + TL;
+    double *const hq = reinterpret_cast<double *>(hrec + 2 * TL);
+    const int64_t k = pdx_xcd_block();
+    const int64_t p0 = k * PDX_BLOCK;
+    if (p0 >= A.n) return;  // whole workgroup
+    const int64_t h0 = A.halo_ptr[k];
+    const int H = (int)(A.halo_ptr[k + 1] - h0);
+    const int32_t *const hidx = A.halo + h0;
+    const int sub = threadIdx.x % S;
+    const int64_t p = p0 + threadIdx.x / S;
+    const bool valid = p < A.n;
+    const int c = valid ? A.cnt[p] : 0;
+    const int64_t b0 = valid ? pdx_bond0(A.slice_ptr, p) : 0;
+    const uint16_t *const wbase = A.bw16 + b0;
+    const double *const Lbase = A.L + b0;
+    const double4 ri = valid ? A.rec_in[p] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const double4 Xi = valid ? A.Xr[p] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const bool dead_i = !valid || A.killed[p] != 0;
+    const double beta = A.beta_zem;
+    double Qi[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) Qi[q] = valid ? A.pstate[9 * p + q] : 0.0;
+    double b3[3] = {0.0, 0.0, 0.0};
+    int t = sub;
+    uint32_t wd = t < c ? wbase[(int64_t)t * 64] : 0u;
+    double Lb = t < c ? Lbase[(int64_t)t * 64] : 1.0;
+    for (int T0 = 0; T0 < H; T0 += TL) {
+        const int T1 = H < T0 + TL ? H : T0 + TL;
+        if (T0 > 0) __syncthreads();
+        for (int h = T0 + (int)threadIdx.x; h < T1; h += PDX_BLOCK * S) {
+            const int32_t j = hidx[h];
+            const bool live = !A.killed[j];
+            const double4 X = A.Xr[j];
+            hx[h - T0] = A.rec_in[j];
+            hX[h - T0] = make_double4(X.x, X.y, X.z, live ? beta : 0.0);
+            double *const qd = hq + 9 * (int64_t)(h - T0);
+#pragma unroll
+            for (int q = 0; q < 9; q++) qd[q] = live ? A.pstate[9 * (int64_t)j + q] : 0.0;
+        }
+        __syncthreads();
+        while (t < c) {
+            const int slot = (int)(wd & PDX_W16_SLOT);
+            if (slot >= T1) break;
+            const int tn = t + S;
+            const uint32_t wn = tn < c ? wbase[(int64_t)tn * 64] : 0u;
+            const double Ln = tn < c ? Lbase[(int64_t)tn * 64] : 1.0;
+            if (wd & PDX_W16_ACT) {
+                const double4 rj = hx[slot - T0];
+                const double4 Xj = hX[slot - T0];
+                const double *const qj = hq + 9 * (int64_t)(slot - T0);
+                const double dx[3] = {rj.x - ri.x, rj.y - ri.y, rj.z - ri.z};
+                const double DX[3] = {Xj.x - Xi.x, Xj.y - Xi.y, Xj.z - Xi.z};
+                const double wv = (A.delta * pdx_rcp1(Lb)) * rj.w;  // w V_j
+#pragma unroll
+                for (int r = 0; r < 3; r++) {
+                    double tt = 0.0;
+                    if (!dead_i) tt += ((Qi[r] * DX[0] + Qi[r + 3] * DX[1]) + Qi[r + 6] * DX[2]) + beta * dx[r];
+                    // a dead neighbour was staged with Q_j = 0 and beta = 0: its term adds 0
+                    tt += ((qj[r] * DX[0] + qj[r + 3] * DX[1]) + qj[r + 6] * DX[2]) + Xj.w * dx[r];
+                    b3[r] += wv * tt;
+                }
+            }
+            wd = wn;
+            Lb = Ln;
+            t = tn;
+        }
+    }
+#pragma unroll
+    for (int o = 1; o < S; o <<= 1)
+#pragma unroll
+        for (int r = 0; r < 3; r++) b3[r] += __shfl_xor(b3[r], o);
+    if (!valid) return;
     static_assert(S == 4, "the tail takes four lanes per point");
     point_tail4(A, p, ri, b3, sub, A.rho);
 }
@@ -2002,6 +2185,18 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
                 hipLaunchKernelGGL((k_cpd_pass2_mx<2, PDX_LDS_LANES, PDX_FEAT_CAP>), G, BL, l2b, c->stream, a);
             else
                 hipLaunchKernelGGL((k_cpd_pass2_mx<2, PDX_LDS_LANES, 0>), G, BL, l2b, c->stream, a);
+            PDX_HIP(c, hipGetLastError());
+            PDX_TRY(pdx_timed_end(c, "step"));
+            break;
+        }
+        if (c->lds_mode) {
+            const dim3 BL(PDX_BLOCK * PDX_LDS_LANES);
+            PDX_TRY(pdx_timed_begin(c));
+            hipLaunchKernelGGL(k_cpd_lds1<PDX_LDS_LANES>, G, BL, lds_bytes(c, 64), c->stream, a);
+            PDX_HIP(c, hipGetLastError());
+            PDX_TRY(pdx_timed_end(c, "pass1"));
+            PDX_TRY(pdx_timed_begin(c));
+            hipLaunchKernelGGL(k_cpd_lds2<PDX_LDS_LANES>, G, BL, lds_bytes(c, 136), c->stream, a);
             PDX_HIP(c, hipGetLastError());
             PDX_TRY(pdx_timed_end(c, "step"));
             break;

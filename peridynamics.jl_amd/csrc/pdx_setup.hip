@@ -555,7 +555,7 @@ static pdx_status build_halos(pdx_ctx *c, int64_t entry_bytes, bool *ok)
     PDX_HIP(c, hipMemcpy(w.data(), c->bw, sizeof(uint32_t) * nba, hipMemcpyDeviceToHost));
     std::vector<std::vector<int32_t>> tab(nblk);
     std::vector<uint16_t> w16(nba, 0);
-    std::vector<int> too_big(nblk, 0);
+    std::vector<int> too_big(nblk, 0), unsorted(nblk, 0);
     unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     auto work = [&](int64_t k0, int64_t k1) {
         std::vector<int32_t> tmp;
@@ -574,10 +574,13 @@ static pdx_status build_halos(pdx_ctx *c, int64_t entry_bytes, bool *ok)
             }
             for (int64_t p = k * PDX_BLOCK; p < pe; p++) {
                 const int64_t b0 = sp[p >> 6] + (p & 63);
+                int64_t prev = -1;
                 for (int t = 0; t < cn[p]; t++) {
                     const int64_t b = b0 + 64 * (int64_t)t;
                     const int32_t j = (int32_t)(w[b] & PDX_BW_NBR);
                     const uint32_t slot = (uint32_t)(std::lower_bound(tmp.begin(), tmp.end(), j) - tmp.begin());
+                    if ((int64_t)slot <= prev) unsorted[k] = 1;
+                    prev = slot;
                     w16[b] = (uint16_t)(slot | ((w[b] & PDX_BW_ACT) ? PDX_W16_ACT | PDX_W16_RACT : 0u) |
                                         ((w[b] & PDX_BW_FP) ? PDX_W16_FP : 0u));
                 }
@@ -610,6 +613,7 @@ static pdx_status build_halos(pdx_ctx *c, int64_t entry_bytes, bool *ok)
     PDX_HIP(c, hipMemcpy(c->halo, halo.data(), sizeof(int32_t) * halo.size(), hipMemcpyHostToDevice));
     c->halo_max = (int32_t)hmax;
     c->halo_total = hp[nblk];
+    c->rows_sorted = std::find(unsorted.begin(), unsorted.end(), 1) == unsorted.end();
     c->lds_mode = true;
     *ok = true;
     return PDX_OK;

@@ -2190,6 +2190,8 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
             break;
         }
         if (c->lds_mode) {
+            // the tiled staging walks each row in slot order
+            if (!c->rows_sorted) return pdx_fail(c, PDX_ERR_STATE, "bond rows are not sorted by neighbour");
             const dim3 BL(PDX_BLOCK * PDX_LDS_LANES);
             PDX_TRY(pdx_timed_begin(c));
             hipLaunchKernelGGL(k_cpd_lds1<PDX_LDS_LANES>, G, BL, lds_bytes(c, 64), c->stream, a);

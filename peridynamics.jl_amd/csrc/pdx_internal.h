@@ -358,6 +358,13 @@ inline int pdx_lds_budget(const pdx_ctx *c)
     const int64_t b = c->lds_budget_env > 0 ? c->lds_budget_env : half;
     return (int)std::min<int64_t>(b, c->lds_limit);
 }
+// for kernels whose registers allow one 1024-thread block per CU anyway (the CPD passes):
+// the whole LDS of the CU
+inline int pdx_lds_budget_full(const pdx_ctx *c)
+{
+    const int64_t b = c->lds_budget_env > 0 ? c->lds_budget_env : c->lds_limit;
+    return (int)std::min<int64_t>(b, c->lds_limit);
+}
 void pdx_export_free(pdx_ctx *c);              // the export pipeline's buffers and stream
 // device pointer, components and stride of a field (PDX_FIELD_*)
 pdx_status pdx_field_ptr(pdx_ctx *c, int field, double **p, int *comps, int *stride);

@@ -2024,17 +2024,21 @@ static int pdx_features(const pdx_ctx *c)
 // one fits the budget; otherwise (PDX_FEAT_CAP) it stages what fits and reads the rest from
 // global memory, so that two blocks still share a CU.  The capped variant is a separate
 // instantiation: its fallback branch costs the uncapped loop ~30 %.
-static bool lds_capped(const pdx_ctx *c, int entry)
+static bool lds_capped(const pdx_ctx *c, int entry, int budget)
 {
-    return (int64_t)std::max(c->halo_max, 1) * entry > pdx_lds_budget(c);
+    return (int64_t)std::max(c->halo_max, 1) * entry > budget;
 }
 
-static size_t lds_bytes(const pdx_ctx *c, int entry)
+static bool lds_capped(const pdx_ctx *c, int entry) { return lds_capped(c, entry, pdx_lds_budget(c)); }
+
+static size_t lds_bytes(const pdx_ctx *c, int entry, int budget)
 {
-    const int hl = lds_capped(c, entry) ? pdx_lds_entries(c->halo_max, pdx_lds_budget(c), entry)
-                                        : std::max(c->halo_max, 1);
+    const int hl = lds_capped(c, entry, budget) ? pdx_lds_entries(c->halo_max, budget, entry)
+                                                : std::max(c->halo_max, 1);
     return (size_t)hl * entry + 16;
 }
+
+static size_t lds_bytes(const pdx_ctx *c, int entry) { return lds_bytes(c, entry, pdx_lds_budget(c)); }
 
 template <int F>
 static void launch_bb_feat(pdx_ctx *c, dim3 G, dim3 BL, size_t lds, const pdx_step_args &a)
@@ -2169,36 +2173,40 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
         PDX_TRY(pdx_timed_end(c, "step"));
         break;
     case PDX_CPD:
-        if (c->lds_mode && c->precision == PDX_MIXED) {
-            // pass 1 entries (D, V), (U, 0): 32 B; pass 2 (D, V), (U, beta l), l Q': 68 B
-            const dim3 BL(PDX_BLOCK * PDX_LDS_LANES);
-            const size_t l1 = lds_bytes(c, 32), l2b = lds_bytes(c, 68);
-            PDX_TRY(pdx_timed_begin(c));
-            if (lds_capped(c, 32))
-                hipLaunchKernelGGL((k_cpd_pass1_mx<1, PDX_LDS_LANES, PDX_FEAT_CAP>), G, BL, l1, c->stream, a);
-            else
-                hipLaunchKernelGGL((k_cpd_pass1_mx<1, PDX_LDS_LANES, 0>), G, BL, l1, c->stream, a);
-            PDX_HIP(c, hipGetLastError());
-            PDX_TRY(pdx_timed_end(c, "pass1"));
-            PDX_TRY(pdx_timed_begin(c));
-            if (lds_capped(c, 68))
-                hipLaunchKernelGGL((k_cpd_pass2_mx<2, PDX_LDS_LANES, PDX_FEAT_CAP>), G, BL, l2b, c->stream, a);
-            else
-                hipLaunchKernelGGL((k_cpd_pass2_mx<2, PDX_LDS_LANES, 0>), G, BL, l2b, c->stream, a);
-            PDX_HIP(c, hipGetLastError());
-            PDX_TRY(pdx_timed_end(c, "step"));
-            break;
-        }
         if (c->lds_mode) {
+            // The CPD passes hold 80-113 VGPRs, so one 1024-thread block runs per CU: their
+            // tables take the CU's whole LDS (fewer tiles, fewer entries read past the cap).
+            pdx_step_args af = a;
+            const int bud = pdx_lds_budget_full(c);
+            af.lds_budget = bud;
+            const dim3 BL(PDX_BLOCK * PDX_LDS_LANES);
+            if (c->precision == PDX_MIXED) {
+                // pass 1 entries (D, V), (U, 0): 32 B; pass 2 (D, V), (U, beta l), l Q': 68 B
+                const size_t l1 = lds_bytes(c, 32, bud), l2b = lds_bytes(c, 68, bud);
+                PDX_TRY(pdx_timed_begin(c));
+                if (lds_capped(c, 32, bud))
+                    hipLaunchKernelGGL((k_cpd_pass1_mx<1, PDX_LDS_LANES, PDX_FEAT_CAP>), G, BL, l1, c->stream, af);
+                else
+                    hipLaunchKernelGGL((k_cpd_pass1_mx<1, PDX_LDS_LANES, 0>), G, BL, l1, c->stream, af);
+                PDX_HIP(c, hipGetLastError());
+                PDX_TRY(pdx_timed_end(c, "pass1"));
+                PDX_TRY(pdx_timed_begin(c));
+                if (lds_capped(c, 68, bud))
+                    hipLaunchKernelGGL((k_cpd_pass2_mx<2, PDX_LDS_LANES, PDX_FEAT_CAP>), G, BL, l2b, c->stream, af);
+                else
+                    hipLaunchKernelGGL((k_cpd_pass2_mx<2, PDX_LDS_LANES, 0>), G, BL, l2b, c->stream, af);
+                PDX_HIP(c, hipGetLastError());
+                PDX_TRY(pdx_timed_end(c, "step"));
+                break;
+            }
             // the tiled staging walks each row in slot order
             if (!c->rows_sorted) return pdx_fail(c, PDX_ERR_STATE, "bond rows are not sorted by neighbour");
-            const dim3 BL(PDX_BLOCK * PDX_LDS_LANES);
             PDX_TRY(pdx_timed_begin(c));
-            hipLaunchKernelGGL(k_cpd_lds1<PDX_LDS_LANES>, G, BL, lds_bytes(c, 64), c->stream, a);
+            hipLaunchKernelGGL(k_cpd_lds1<PDX_LDS_LANES>, G, BL, lds_bytes(c, 64, bud), c->stream, af);
             PDX_HIP(c, hipGetLastError());
             PDX_TRY(pdx_timed_end(c, "pass1"));
             PDX_TRY(pdx_timed_begin(c));
-            hipLaunchKernelGGL(k_cpd_lds2<PDX_LDS_LANES>, G, BL, lds_bytes(c, 136), c->stream, a);
+            hipLaunchKernelGGL(k_cpd_lds2<PDX_LDS_LANES>, G, BL, lds_bytes(c, 136, bud), c->stream, af);
             PDX_HIP(c, hipGetLastError());
             PDX_TRY(pdx_timed_end(c, "step"));
             break;

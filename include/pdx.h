@@ -45,7 +45,8 @@ typedef int32_t pdx_status;
 #define PDX_FP64 0  /* all arithmetic in FP64 */
 #define PDX_MIXED 1 /* FP32 stretch/force arithmetic from FP64 differences, FP64 sums and
                        FP64 integration; BB, OSB and CPD materials (OSB with several
-                       parameter sets runs the FP64 kernels) */
+                       parameter sets runs the FP64 kernels); Newton-Krylov residuals and
+                       Jacobian products use the FP64 kernels in both precisions */
 
 /* material kinds */
 #define PDX_BB 0  /* BBMaterial,  src/physics/bond_based.jl:74-84 */

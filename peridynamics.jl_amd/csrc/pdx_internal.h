@@ -36,6 +36,7 @@
 #define PDX_F_BEXT 4        // b_ext is not all zero
 #define PDX_F_FORCE_ONLY 8  // calc_force_density! only: write b_int, damage, n_active
 #define PDX_F_SCALE_VOL 16  // force-only: write b_int * V (Newton-Krylov residual form)
+#define PDX_F_FP64 32       // FP64 kernels also in a mixed-precision context (Newton-Krylov)
 
 struct pdx_crack {
     std::vector<int32_t> a, b;

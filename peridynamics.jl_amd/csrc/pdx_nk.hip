@@ -370,7 +370,8 @@ static pdx_status norm2(pdx_ctx *c, const double *x, double *out)
 static pdx_status force_scaled(pdx_ctx *c, const double4 *rec_in, double *out)
 {
     pdx_step_args a;
-    PDX_TRY(pdx_fill_step_args(c, &a, 0, PDX_F_FORCE_ONLY | PDX_F_SCALE_VOL, 0.0, nullptr));
+    // finite-difference products need the FP64 force also in a mixed-precision context
+    PDX_TRY(pdx_fill_step_args(c, &a, 0, PDX_F_FORCE_ONLY | PDX_F_SCALE_VOL | PDX_F_FP64, 0.0, nullptr));
     a.rec_in = rec_in;
     a.bint_out = out;
     return pdx_launch_step(c, a);
@@ -802,7 +803,7 @@ extern "C" pdx_status pdx_residual(pdx_ctx *c, double *r_host, double *norm_out)
     pdx_nk *k = c->nk;
     PDX_TRY(pdx_reset_flags(c));
     pdx_step_args a;
-    PDX_TRY(pdx_fill_step_args(c, &a, 0, PDX_F_FORCE_ONLY, 0.0, nullptr));
+    PDX_TRY(pdx_fill_step_args(c, &a, 0, PDX_F_FORCE_ONLY | PDX_F_FP64, 0.0, nullptr));
     PDX_TRY(pdx_launch_step(c, a));
     hipLaunchKernelGGL(k_residual, grid1(k->m), dim3(256), 0, c->stream, c->n, c->bint, c->bext,
                        c->Xr, k->cmask, k->r);

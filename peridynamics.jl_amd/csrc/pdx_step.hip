@@ -2041,9 +2041,9 @@ static size_t lds_bytes(const pdx_ctx *c, int entry, int budget)
 static size_t lds_bytes(const pdx_ctx *c, int entry) { return lds_bytes(c, entry, pdx_lds_budget(c)); }
 
 template <int F>
-static void launch_bb_feat(pdx_ctx *c, dim3 G, dim3 BL, size_t lds, const pdx_step_args &a)
+static void launch_bb_feat(pdx_ctx *c, bool mixed, dim3 G, dim3 BL, size_t lds, const pdx_step_args &a)
 {
-    if (c->precision == PDX_MIXED) hipLaunchKernelGGL((k_bb_lds_mx<4, PDX_LDS_LANES, F>), G, BL, lds, c->stream, a);
+    if (mixed) hipLaunchKernelGGL((k_bb_lds_mx<4, PDX_LDS_LANES, F>), G, BL, lds, c->stream, a);
     else hipLaunchKernelGGL((k_bb_lds<2, PDX_LDS_LANES, F>), G, BL, lds, c->stream, a);
 }
 
@@ -2069,6 +2069,8 @@ static void launch_osb_feat(pdx_ctx *c, int pass, dim3 G, dim3 BL, size_t lds, c
 
 pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
 {
+    // the mixed-precision kernels, unless the caller needs the FP64 force (PDX_F_FP64)
+    const bool mixed = c->precision == PDX_MIXED && !(a.flags & PDX_F_FP64);
     const int64_t blocks = (c->n + PDX_BLOCK - 1) / PDX_BLOCK;
     const dim3 G((unsigned)((blocks + PDX_NUM_XCD - 1) / PDX_NUM_XCD * PDX_NUM_XCD)), B(PDX_BLOCK);
     switch (c->kind) {
@@ -2081,23 +2083,23 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
             const dim3 BL(PDX_BLOCK * PDX_LDS_LANES);
             // bonds per lane and unrolled iteration: the FP32 bond arithmetic of the mixed
             // step is short enough that four loads in flight pay; FP64 is indifferent
-            const bool u4 = c->bb_unroll ? c->bb_unroll == 4 : c->precision == PDX_MIXED;
+            const bool u4 = c->bb_unroll ? c->bb_unroll == 4 : mixed;
             // with several parameter sets each neighbour-table entry also carries the
             // neighbour's set
             const int entry = 32 + (c->multi() ? 1 : 0);
             const int feat = pdx_features(c) | (lds_capped(c, entry) ? PDX_FEAT_CAP : 0);
             const size_t ldsf = lds_bytes(c, entry);
-#define PDX_BB_LAUNCH(F) launch_bb_feat<F>(c, G, BL, ldsf, a)
+#define PDX_BB_LAUNCH(F) launch_bb_feat<F>(c, mixed, G, BL, ldsf, a)
             if (feat) {
                 PDX_FEAT_SWITCH(feat, PDX_BB_LAUNCH)
-            } else if (c->precision == PDX_MIXED) {
+            } else if (mixed) {
                 if (u4) hipLaunchKernelGGL((k_bb_lds_mx<4, PDX_LDS_LANES, 0>), G, BL, lds, c->stream, a);
                 else hipLaunchKernelGGL((k_bb_lds_mx<2, PDX_LDS_LANES, 0>), G, BL, lds, c->stream, a);
             } else {
                 if (u4) hipLaunchKernelGGL((k_bb_lds<4, PDX_LDS_LANES, 0>), G, BL, lds, c->stream, a);
                 else hipLaunchKernelGGL((k_bb_lds<2, PDX_LDS_LANES, 0>), G, BL, lds, c->stream, a);
             }
-        } else if (c->precision == PDX_MIXED) {
+        } else if (mixed) {
             if (c->bb_unroll == 4) hipLaunchKernelGGL((k_bb_step<true, 4>), G, B, 0, c->stream, a);
             else hipLaunchKernelGGL((k_bb_step<true, 2>), G, B, 0, c->stream, a);
         } else {
@@ -2108,7 +2110,7 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
         PDX_TRY(pdx_timed_end(c, "step"));
         break;
     case PDX_OSB:
-        if (c->lds_mode && c->precision == PDX_MIXED && !c->multi()) {
+        if (c->lds_mode && mixed && !c->multi()) {
             const dim3 BL(PDX_BLOCK * PDX_LDS_LANES);
             // pass 1 entries (D, V), (U, 0): 32 B; pass 2 also (c2 V)_j: 36 B
             const int sc = c->corr ? PDX_FEAT_SC : 0;
@@ -2180,7 +2182,7 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
             const int bud = pdx_lds_budget_full(c);
             af.lds_budget = bud;
             const dim3 BL(PDX_BLOCK * PDX_LDS_LANES);
-            if (c->precision == PDX_MIXED) {
+            if (mixed) {
                 // pass 1 entries (D, V), (U, 0): 32 B; pass 2 (D, V), (U, beta l), l Q': 68 B
                 const size_t l1 = lds_bytes(c, 32, bud), l2b = lds_bytes(c, 68, bud);
                 PDX_TRY(pdx_timed_begin(c));

@@ -151,3 +151,43 @@ def test_nk_throws_at_maxiter(need_gpu):
     nk = pd.NewtonKrylov(steps=10, stepsize=0.1, maxiter=2, tol=1e-6)
     with pytest.raises(pd.NotConvergedError):
         pd.submit(pd.Job(b, nk, fields=()))
+
+
+def _tension_bar():
+    """The CMaterial bar of test_analytic_solutions.jl:7-94 (see the closed-form test)."""
+    l, w, h, Ftot, E = 1.0, 0.1, 0.1, 2e6, 200e9
+    dx = 1 / 30
+    pos, vol = pd.uniform_box(l + 3 * dx, w, h, dx, center=(-1.5 * dx, 0, 0))
+    b = pd.Body(pd.CMaterial(zem=pd.ZEMSilling(Cs=0.5)), pos, vol)
+    pd.material(b, horizon=3.015 * dx, rho=7850.0, E=E, nu=0.25)
+    pd.point_set(b, "left", lambda p: p[0] < -l / 2)
+    pd.point_set(b, "right", lambda p: p[0] > l / 2 - dx)
+    b_right = Ftot / vol[b.point_sets["right"]].sum()
+    for d in "xyz":
+        pd.displacement_bc(lambda p: 0.0, b, "left", d)
+    pd.forcedensity_bc(lambda p: b_right, b, "right", "x")
+    return b
+
+
+@pytest.mark.parametrize("kind", ["BB", "CPD"])
+def test_nk_in_mixed_context_uses_fp64_forces(need_gpu, kind):
+    """A mixed-precision context solves Newton-Krylov with the FP64 force kernels (the
+    finite-difference Jacobian products need them): its load steps equal those of an FP64
+    context bit for bit."""
+    say("nk mixed context", kind)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        if kind == "BB":
+            b = nk_block()
+            nk = pd.NewtonKrylov(steps=2, tol=1e-4, maxiter=30, gmres_restart=30,
+                                 gmres_maxiter=300, precond="jacobi")
+        else:
+            b = _tension_bar()
+            nk = pd.NewtonKrylov(steps=2, tol=1e-3, maxiter=50)
+        dh = pd.MI355XBodyDataHandler(b, precision=prec)
+        stats = dh.solve_nk(nk)
+        out[prec] = (dh.get("displacement"), [(st.newton_iters, st.krylov_iters) for st in stats])
+        say(f"  {prec}: {out[prec][1]}")
+        dh.close()
+    assert out["fp64"][1] == out["mixed"][1]
+    assert np.array_equal(out["fp64"][0], out["mixed"][0])

@@ -535,7 +535,7 @@ pdx_status pdx_download_vec(pdx_ctx *c, const double *dev, double *host, int com
 // 16-bit slot into its block's list.  Built on the host, in parallel over blocks, once.
 // Returns false (and leaves the global-gather layout in use) if a halo does not fit.
 // ----------------------------------------------------------------------------------------
-static pdx_status build_halos(pdx_ctx *c, int64_t entry_bytes, bool *ok)
+static pdx_status build_halos(pdx_ctx *c, bool *ok)
 {
     *ok = false;
     const int64_t n = c->n, nba = c->nb_alloc;
@@ -545,8 +545,10 @@ static pdx_status build_halos(pdx_ctx *c, int64_t entry_bytes, bool *ok)
     PDX_HIP(c, hipGetDeviceProperties(&prop, c->device));
     const int64_t lds_limit = std::min<int64_t>((int64_t)prop.sharedMemPerBlock, 160 * 1024);
     c->lds_limit = lds_limit - 64;
-    // 64 bytes of slack for the alignment pad of a trailing byte table
-    const int64_t slot_limit = std::min<int64_t>(PDX_W16_SLOT + 1, (lds_limit - 64) / entry_bytes);
+    // 64 bytes of slack for the alignment pad of a trailing byte table.  A table longer than
+    // what LDS holds is still served: the kernels stage what fits and read the rest from
+    // global memory (or stage it in tiles), so only the 13-bit slot field limits a table.
+    const int64_t slot_limit = PDX_W16_SLOT + 1;
     std::vector<int64_t> sp(n_slices + 1);
     std::vector<int32_t> cn(n);
     std::vector<uint32_t> w(nba);
@@ -866,10 +868,7 @@ extern "C" pdx_status pdx_build_bonds(pdx_ctx *c, int64_t *n_bonds_out)
     // (tables staged in tiles)
     if (!getenv("PDX_NO_LDS")) {
         bool ok = false;
-        const int64_t entry = (c->kind == PDX_OSB ? (int64_t)(sizeof(double4) + sizeof(double2))
-                                                  : (int64_t)sizeof(double4)) +
-                              (c->multi() ? 1 : 0);  // + the neighbour's parameter set
-        PDX_TRY(build_halos(c, entry, &ok));
+        PDX_TRY(build_halos(c, &ok));
     }
     PDX_TRY(upload_psets(c));
     if (c->multi() && !c->lds_mode)

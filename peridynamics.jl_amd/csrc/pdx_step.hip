@@ -341,10 +341,35 @@ __device__ __forceinline__ bool bond_fail16(uint16_t *wp, uint32_t wd, double e,
     return act;
 }
 
+// One bond of the FP64 BB step (bond_based.jl:125-161): failure against the point's eps_c,
+// then the force term (w bc) eps V_j dx / l accumulated into b; bcb = the bond's bc,
+// sc = its surface-correction factor (1 without).
+__device__ __forceinline__ void bb_bond64(double4 ri, double4 rj, uint16_t *wp, uint32_t wd, double Lb,
+                                          double sc, double bcb, double epsc, double epsc_lo,
+                                          double epsc_hi, double &bx, double &by, double &bz, int &na)
+{
+    const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
+    const double d2 = (dx * dx + dy * dy) + dz * dz;
+    double rl;
+    const double l = pdx_sqrt_rsq(d2, rl);
+    const double e = l - Lb;
+    const double eps = e * pdx_rcp1(Lb);
+    const bool act = bond_fail16(wp, wd, e, Lb, eps, epsc, epsc_lo, epsc_hi);
+    const double wbc = act ? sc * bcb : 0.0;  // (w * bc), w = active * sc
+    const double s = ((wbc * eps) * rj.w) * rl;
+    bx = __builtin_fma(s, dx, bx);
+    by = __builtin_fma(s, dy, by);
+    bz = __builtin_fma(s, dz, bz);
+    na += act ? 1 : 0;
+}
+
 // F = feature mask.  PDX_FEAT_MULTI: several parameter sets (force_density_point! with a
 // ParameterHandler, bond_based.jl:163-180): eps_c of the row's point, (bc_i + bc_j) / 2 per
 // bond, rho_i.  PDX_FEAT_SC: EnergySurfaceCorrection, w = bond_active * scfactor[bond]
-// (bond_based.jl:156, bond_system_corrections.jl:71-74).
+// (bond_based.jl:156, bond_system_corrections.jl:71-74).  PDX_FEAT_CAP: the table may be
+// longer than the LDS budget; it is staged in tiles of HL entries.  Rows and table are both
+// sorted by neighbour, so each lane meets the tiles in order: it walks its bonds while their
+// slots lie in the staged tile and resumes there after the next tile is staged.
 template <int U, int S, int F>
 __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds(pdx_step_args A)
 {
@@ -357,83 +382,142 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds(pdx_step_args A)
     if (p0 >= A.n) return;  // whole workgroup
     const int64_t h0 = A.halo_ptr[k];
     const int H = (int)(A.halo_ptr[k + 1] - h0);
-    // CAP: the table may be longer than the LDS budget; entries past it are read from
-    // global memory (without CAP the whole table is staged)
     const int HL = CAP ? pdx_lds_entries(A.halo_max, A.lds_budget, 32 + (MULTI ? 1 : 0)) : A.halo_max;
-    const int Hs = H < HL ? H : HL;
     const int32_t *const hidx = A.halo + h0;
     uint8_t *const hpid = reinterpret_cast<uint8_t *>(hrec + HL);
-    for (int h = threadIdx.x; h < Hs; h += PDX_BLOCK * S) {
-        const int32_t j = hidx[h];
-        hrec[h] = A.rec_in[j];
-        if (MULTI) hpid[h] = A.pid[j];
-    }
-    __syncthreads();
     // S adjacent lanes share a point; lane s takes its bonds s, s + S, s + 2S, ...
     const int sub = threadIdx.x % S;
     const int64_t p = p0 + threadIdx.x / S;
-    if (p >= A.n) return;  // all S lanes of the point together
-    const int c = A.cnt[p];
-    const int64_t b0 = pdx_bond0(A.slice_ptr, p);
-    uint16_t *const wbase = A.bw16 + b0;
-    const double *const Lbase = A.L + b0;
-    const double4 ri = A.rec_in[p];
     double bx = 0.0, by = 0.0, bz = 0.0;
     int na = 0;
-    double bc = A.bc, epsc = A.epsc, epsc_lo = A.epsc_lo, epsc_hi = A.epsc_hi, rho = A.rho;
-    if (MULTI) {
-        const pdx_pset Pi = A.psets[A.pid[p]];
-        bc = Pi.bc;
-        epsc = Pi.epsc;
-        epsc_lo = Pi.epsc_lo;
-        epsc_hi = Pi.epsc_hi;
-        rho = Pi.rho;
-    }
-    for (int t = sub; t < c; t += U * S) {
-        uint32_t wd[U];
-        double Ld[U], Sd[U];
+    int c = 0;
+    double4 ri = make_double4(0.0, 0.0, 0.0, 0.0);
+    double rho = A.rho;
+    if constexpr (!CAP) {
+        // the whole table fits: stage it once
+        for (int h = threadIdx.x; h < H; h += PDX_BLOCK * S) {
+            const int32_t j = hidx[h];
+            hrec[h] = A.rec_in[j];
+            if (MULTI) hpid[h] = A.pid[j];
+        }
+        __syncthreads();
+        if (p >= A.n) return;  // all S lanes of the point together
+        c = A.cnt[p];
+        const int64_t b0 = pdx_bond0(A.slice_ptr, p);
+        uint16_t *const wbase = A.bw16 + b0;
+        const double *const Lbase = A.L + b0;
+        ri = A.rec_in[p];
+        double bc = A.bc, epsc = A.epsc, epsc_lo = A.epsc_lo, epsc_hi = A.epsc_hi;
+        if (MULTI) {
+            const pdx_pset Pi = A.psets[A.pid[p]];
+            bc = Pi.bc;
+            epsc = Pi.epsc;
+            epsc_lo = Pi.epsc_lo;
+            epsc_hi = Pi.epsc_hi;
+            rho = Pi.rho;
+        }
+        for (int t = sub; t < c; t += U * S) {
+            uint32_t wd[U];
+            double Ld[U], Sd[U];
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            wd[u] = 0;
-            Ld[u] = 1.0;
-            Sd[u] = 1.0;
-            if (t + u * S < c) {
-                const int64_t o = (int64_t)(t + u * S) * 64;
-                wd[u] = wbase[o];
-                Ld[u] = Lbase[o];
-                if (SC) Sd[u] = A.scf[b0 + o];
+            for (int u = 0; u < U; u++) {
+                wd[u] = 0;
+                Ld[u] = 1.0;
+                Sd[u] = 1.0;
+                if (t + u * S < c) {
+                    const int64_t o = (int64_t)(t + u * S) * 64;
+                    wd[u] = wbase[o];
+                    Ld[u] = Lbase[o];
+                    if (SC) Sd[u] = A.scf[b0 + o];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (t + u * S >= c) continue;
+                const int slot = (int)(wd[u] & PDX_W16_SLOT);
+                const double bcb = MULTI ? (bc + A.psets[hpid[slot]].bc) / 2.0 : bc;
+                bb_bond64(ri, hrec[slot], wbase + (int64_t)(t + u * S) * 64, wd[u], Ld[u], Sd[u], bcb,
+                          epsc, epsc_lo, epsc_hi, bx, by, bz, na);
+            }
+        }
+    } else {
+        // tiles of HL entries; all threads of the block take part in the staging barriers
+        const bool valid = p < A.n;
+        c = valid ? A.cnt[p] : 0;
+        const int64_t b0 = valid ? pdx_bond0(A.slice_ptr, p) : 0;
+        uint16_t *const wbase = A.bw16 + b0;
+        const double *const Lbase = A.L + b0;
+        if (valid) ri = A.rec_in[p];
+        double bc = A.bc, epsc = A.epsc, epsc_lo = A.epsc_lo, epsc_hi = A.epsc_hi;
+        if (MULTI && valid) {
+            const pdx_pset Pi = A.psets[A.pid[p]];
+            bc = Pi.bc;
+            epsc = Pi.epsc;
+            epsc_lo = Pi.epsc_lo;
+            epsc_hi = Pi.epsc_hi;
+            rho = Pi.rho;
+        }
+        int t = sub;
+        for (int T0 = 0; T0 < H; T0 += HL) {
+            const int T1 = H < T0 + HL ? H : T0 + HL;
+            if (T0 > 0) __syncthreads();  // the previous tile is consumed
+            for (int h = T0 + (int)threadIdx.x; h < T1; h += PDX_BLOCK * S) {
+                const int32_t j = hidx[h];
+                hrec[h - T0] = A.rec_in[j];
+                if (MULTI) hpid[h - T0] = A.pid[j];
+            }
+            __syncthreads();
+            for (; t < c; t += U * S) {
+                uint32_t wd[U];
+                double Ld[U], Sd[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    wd[u] = 0;
+                    Ld[u] = 1.0;
+                    Sd[u] = 1.0;
+                    if (t + u * S < c) {
+                        const int64_t o = (int64_t)(t + u * S) * 64;
+                        wd[u] = wbase[o];
+                        Ld[u] = Lbase[o];
+                        if (SC) Sd[u] = A.scf[b0 + o];
+                    }
+                }
+                // the leading bonds of the group whose neighbours lie in the staged tile
+                int kk = 0;
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    if (kk == u && t + u * S < c && (int)(wd[u] & PDX_W16_SLOT) < T1) kk = u + 1;
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    if (u >= kk) continue;
+                    const int ls = (int)(wd[u] & PDX_W16_SLOT) - T0;
+                    const double bcb = MULTI ? (bc + A.psets[hpid[ls]].bc) / 2.0 : bc;
+                    bb_bond64(ri, hrec[ls], wbase + (int64_t)(t + u * S) * 64, wd[u], Ld[u], Sd[u],
+                              bcb, epsc, epsc_lo, epsc_hi, bx, by, bz, na);
+                }
+                if (kk < U) {
+                    t += kk * S;  // resume at the first bond past the tile (or the row's end)
+                    break;
+                }
             }
         }
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            if (t + u * S >= c) continue;
-            const int slot = (int)(wd[u] & PDX_W16_SLOT);
-            const bool in_lds = !CAP || slot < HL;
-            const int32_t jg = in_lds ? 0 : hidx[slot];
-            const double4 rj = in_lds ? hrec[slot] : A.rec_in[jg];
-            const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
-            uint16_t *const wp = wbase + (int64_t)(t + u * S) * 64;
-            const double d2 = (dx * dx + dy * dy) + dz * dz;
-            double rl;
-            const double l = pdx_sqrt_rsq(d2, rl);
-            const double e = l - Ld[u];
-            const double eps = e * pdx_rcp1(Ld[u]);
-            const bool act = bond_fail16(wp, wd[u], e, Ld[u], eps, epsc, epsc_lo, epsc_hi);
-            const double bcb = MULTI ? (bc + A.psets[in_lds ? hpid[slot] : A.pid[jg]].bc) / 2.0 : bc;
-            const double wbc = act ? (SC ? Sd[u] * bcb : bcb) : 0.0;  // (w * bc), w = active * sc
-            const double s = ((wbc * eps) * rj.w) * rl;
-            bx = __builtin_fma(s, dx, bx);
-            by = __builtin_fma(s, dy, by);
-            bz = __builtin_fma(s, dz, bz);
-            na += act ? 1 : 0;
+        for (int o = 1; o < S; o <<= 1) {
+            bx += __shfl_xor(bx, o);
+            by += __shfl_xor(by, o);
+            bz += __shfl_xor(bz, o);
+            na += __shfl_xor(na, o);
         }
+        if (!valid) return;
     }
+    if constexpr (!CAP) {
 #pragma unroll
-    for (int o = 1; o < S; o <<= 1) {
-        bx += __shfl_xor(bx, o);
-        by += __shfl_xor(by, o);
-        bz += __shfl_xor(bz, o);
-        na += __shfl_xor(na, o);
+        for (int o = 1; o < S; o <<= 1) {
+            bx += __shfl_xor(bx, o);
+            by += __shfl_xor(by, o);
+            bz += __shfl_xor(bz, o);
+            na += __shfl_xor(na, o);
+        }
     }
     if (sub == 0 && (A.flags & (PDX_F_STORE | PDX_F_FORCE_ONLY))) {
         A.nact_out[p] = na;

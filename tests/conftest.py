@@ -59,3 +59,24 @@ def as_oracle_params(p, **extra):
     d["lambda"] = p.lam
     d.update(extra)
     return d
+
+
+class lds_budget:
+    """PDX_LDS_BUDGET (bytes of LDS a block's neighbour table may take) for the contexts
+    created inside the block; pdx_create reads it.  A small budget sends the kernels down
+    their paths for tables longer than LDS.  0 keeps the default."""
+
+    def __init__(self, nbytes):
+        self.nbytes = nbytes
+
+    def __enter__(self):
+        self.old = os.environ.get("PDX_LDS_BUDGET")
+        if self.nbytes:
+            os.environ["PDX_LDS_BUDGET"] = str(self.nbytes)
+        return self
+
+    def __exit__(self, *exc):
+        if self.old is None:
+            os.environ.pop("PDX_LDS_BUDGET", None)
+        else:
+            os.environ["PDX_LDS_BUDGET"] = self.old

@@ -7,13 +7,13 @@ contributions to the point), bond states identical to the oracle's (stretches ne
 are decided in FP64), and a torsion trajectory that stays within 1e-5 (normwise) of the
 FP64 oracle.  Each force case also runs with a 12 KB LDS budget, where most neighbour-table
 entries are read from global memory instead of LDS."""
-import os
 import sys
 
 import numpy as np
 import pytest
 
 import peridynamics as pd
+from conftest import lds_budget
 from test_gpu_parity import body_block, displacement_field, normwise, oracle_for, scaled_error
 
 pytestmark = pytest.mark.gpu
@@ -24,25 +24,6 @@ MIXED_TOL = 1e-5
 def say(*a):
     print("[gpu-case]", *a, flush=True)
     sys.stdout.flush()
-
-
-class lds_budget:
-    """PDX_LDS_BUDGET for the contexts created inside the block (read by pdx_create)."""
-
-    def __init__(self, nbytes):
-        self.nbytes = nbytes
-
-    def __enter__(self):
-        self.old = os.environ.get("PDX_LDS_BUDGET")
-        if self.nbytes:
-            os.environ["PDX_LDS_BUDGET"] = str(self.nbytes)
-        return self
-
-    def __exit__(self, *exc):
-        if self.old is None:
-            os.environ.pop("PDX_LDS_BUDGET", None)
-        else:
-            os.environ["PDX_LDS_BUDGET"] = self.old
 
 
 @pytest.mark.parametrize("budget", [0, 12288])

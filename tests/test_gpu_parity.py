@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import peridynamics as pd
-from conftest import as_oracle_params
+from conftest import as_oracle_params, lds_budget
 
 pytestmark = pytest.mark.gpu
 
@@ -181,14 +181,18 @@ def test_force_density_nonuniform_volume(need_gpu, oracle, kind):
     assert err <= FP64_TOL
 
 
+@pytest.mark.parametrize("budget", [0, 12288])
 @pytest.mark.parametrize("kind", ["BB", "OSB", "CPD"])
 @pytest.mark.parametrize("amp", [2e-7, 3e-6])
-def test_force_density_fp64(need_gpu, oracle, kind, amp):
-    say("force fp64", kind, amp)
+def test_force_density_fp64(need_gpu, oracle, kind, amp, budget):
+    """budget 12288: a 12 KB LDS budget, so the neighbour tables are longer than what a
+    block stages at once (staged in tiles, or read past the budget from global memory)."""
+    say("force fp64", kind, amp, budget)
     E = 70e9 if kind == "BB" else 210e9
     b = body_block(kind, 24, 20, 8, jit=0.05, E=E, Gc=100.0 if kind == "BB" else None,
                    rho=2700.0 if kind == "BB" else 7850.0)
-    dh = pd.MI355XBodyDataHandler(b)
+    with lds_budget(budget):
+        dh = pd.MI355XBodyDataHandler(b)
     orc = oracle_for(oracle, b, kind)
     X = b.xyz
     x = X + displacement_field(X, amp)
@@ -250,10 +254,12 @@ def test_osb_known_answer(need_gpu, golden):
     np.testing.assert_allclose(bi[1], [-g["b12"], 0, 0], rtol=1e-13, atol=0)
 
 
-def test_mixed_force_error(need_gpu, oracle):
-    say("force mixed")
+@pytest.mark.parametrize("budget", [0, 12288])
+def test_mixed_force_error(need_gpu, oracle, budget):
+    say("force mixed", budget)
     b = body_block("BB", 24, 20, 8, jit=0.05)
-    dh = pd.MI355XBodyDataHandler(b, precision="mixed")
+    with lds_budget(budget):
+        dh = pd.MI355XBodyDataHandler(b, precision="mixed")
     orc = oracle_for(oracle, b, "BB")
     X = b.xyz
     x = X + displacement_field(X, 2e-7)

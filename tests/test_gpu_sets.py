@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import peridynamics as pd
-from conftest import as_oracle_params
+from conftest import as_oracle_params, lds_budget
 
 pytestmark = pytest.mark.gpu
 
@@ -113,11 +113,13 @@ def test_sets_two_point_closed_form(need_gpu, kind):
     dh.close()
 
 
+@pytest.mark.parametrize("budget", [0, 12288])
 @pytest.mark.parametrize("kind", ["BB", "OSB"])
-def test_sets_force_fp64(need_gpu, oracle, kind):
-    say("sets force fp64", kind)
+def test_sets_force_fp64(need_gpu, oracle, kind, budget):
+    say("sets force fp64", kind, budget)
     b = sets_body(kind, 16, 14, 6)
-    dh = pd.MI355XBodyDataHandler(b)
+    with lds_budget(budget):
+        dh = pd.MI355XBodyDataHandler(b)
     assert dh.n_sets == 3
     orc = oracle_of(oracle, b, kind)
     x = deform(b.xyz, 5e-7)

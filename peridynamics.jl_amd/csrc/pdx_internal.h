@@ -129,8 +129,8 @@ struct pdx_ctx {
     uint16_t *bw16 = nullptr;              // nb_alloc
     int64_t *halo_ptr = nullptr;           // n_blocks + 1
     int32_t *halo = nullptr;               // halo entries (internal point indices)
-    int32_t halo_max = 0;                  // largest halo of a block
-    bool rows_sorted = false;              // every row's table slots increase (tiled staging)
+    int32_t halo_max = 0;                  // largest halo of a block (LDS layout: every row's
+                                           // table slots increase)
     int64_t halo_total = 0;                // sum of the halo sizes of all blocks
     bool bext_nonzero = false;
 

@@ -597,8 +597,10 @@ static pdx_status build_halos(pdx_ctx *c, bool *ok)
         if (k0 < k1) th.emplace_back(work, k0, k1);
     }
     for (auto &t : th) t.join();
+    // keep the global-gather kernels if a table outgrows the slot field, or if a row's slots
+    // do not increase (the tiled staging walks every row in slot order)
     for (int64_t k = 0; k < nblk; k++)
-        if (too_big[k]) return PDX_OK;  // keep the global-gather kernel
+        if (too_big[k] || unsorted[k]) return PDX_OK;
     std::vector<int64_t> hp(nblk + 1, 0);
     int64_t hmax = 0;
     for (int64_t k = 0; k < nblk; k++) {
@@ -615,7 +617,6 @@ static pdx_status build_halos(pdx_ctx *c, bool *ok)
     PDX_HIP(c, hipMemcpy(c->halo, halo.data(), sizeof(int32_t) * halo.size(), hipMemcpyHostToDevice));
     c->halo_max = (int32_t)hmax;
     c->halo_total = hp[nblk];
-    c->rows_sorted = std::find(unsorted.begin(), unsorted.end(), 1) == unsorted.end();
     c->lds_mode = true;
     *ok = true;
     return PDX_OK;

@@ -2285,8 +2285,6 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
                 PDX_TRY(pdx_timed_end(c, "step"));
                 break;
             }
-            // the tiled staging walks each row in slot order
-            if (!c->rows_sorted) return pdx_fail(c, PDX_ERR_STATE, "bond rows are not sorted by neighbour");
             PDX_TRY(pdx_timed_begin(c));
             hipLaunchKernelGGL(k_cpd_lds1<PDX_LDS_LANES>, G, BL, lds_bytes(c, 64, bud), c->stream, af);
             PDX_HIP(c, hipGetLastError());

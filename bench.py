@@ -334,6 +334,53 @@ def mode_i_leg(npxy, nz, steps):
     return out
 
 
+def horizon_leg(shape, m, kind="BB", precision="fp64", steps=50, warmup=5, dx=1e-3):
+    """The bond-based (or OSB) tension workload at a larger horizon, delta = m dx: the
+    neighbour tables outgrow one LDS budget (1813 entries at m = 3.015; ~3000 at 4.015,
+    ~4600 at 5.015), so the step stages them in tiles (BB FP64) or reads the entries past
+    the budget from global memory (the other kernels)."""
+    import peridynamics as pd
+    pos, vol = pd.lattice_block(*shape, dx, jitter_amplitude=0.05, seed=0x5EED)
+    body = pd.Body(pd.BBMaterial() if kind == "BB" else pd.OSBMaterial(), pos, vol)
+    kw = dict(horizon=m * dx, rho=2700.0, E=70e9, Gc=100.0)
+    if kind != "BB":
+        kw["nu"] = 0.25
+    pd.material(body, **kw)
+    y = pos[1]
+    pd.point_set(body, "top", y > y.max() - 2.5 * dx)
+    pd.point_set(body, "bottom", y < y.min() + 2.5 * dx)
+    pd.velocity_bc(lambda t: 0.6, body, "top", "y")
+    pd.velocity_bc(lambda t: -0.6, body, "bottom", "y")
+    dh = pd.MI355XBodyDataHandler(body, precision=precision)
+    vv = pd.VelocityVerlet(steps=warmup + steps)
+    dh.init_time_solver(vv)
+    dh.initialize(vv)
+    ctx = dh.ctx
+    bcv = dh.bc_values(range(1, vv.n_steps + 1), vv.dt)
+    ctx.vv_run(1, warmup, vv.dt, bcv[:warmup])
+    ctx.synchronize()
+    ctx.set_profiling(True)
+    ctx.vv_run(1 + warmup, steps, vv.dt, bcv[warmup:])
+    ctx.synchronize()
+    lay = ctx.layout()
+    nb, npnt = dh.n_bonds, body.n_points
+    if kind == "BB":
+        kt, _ = ctx.kernel_time("step")
+        byt, formula = step_bytes(npnt, nb, precision, lay)
+    else:
+        k1, _ = ctx.kernel_time("pass1")
+        k2, _ = ctx.kernel_time("step")
+        kt = k1 + k2
+        byt, formula = state_based_bytes(kind, npnt, nb, lay, precision)
+    ctx.set_profiling(False)
+    dh.close()
+    return {"workload": f"{kind} Velocity Verlet {shape[0]}x{shape[1]}x{shape[2]} jittered lattice, "
+                        f"horizon {m} dx, {precision.upper()}",
+            "points": npnt, "bonds": nb, "halo_max": lay["halo_max"], "kernel_ms": kt,
+            "bond_updates_per_s": nb / (kt * 1e-3), "bytes_formula": formula,
+            "frac_hbm": byt / (kt * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+
 def export_leg(shape, steps, freq, fields=("displacement", "damage")):
     """VTK export every `freq` steps (the reference's Job(...; freq, fields) output) on the
     bond-based workload: wall time per step without export, with the asynchronous export
@@ -592,6 +639,9 @@ def main():
         out["mode_i"] = mode_i_leg(500, 4, 2000)
         log("[bench] export leg")
         out["export"] = export_leg(shape, 1000, 200)
+        log("[bench] larger-horizon legs")
+        out["horizons"] = [horizon_leg((120, 120, 40), m, kind)
+                           for m in (3.015, 4.015, 5.015) for kind in ("BB", "OSB")]
 
     if not args.no_jfnk:
         log("[bench] JFNK leg")

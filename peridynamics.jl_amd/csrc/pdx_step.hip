@@ -24,6 +24,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <unordered_map>
 
 #include "pdx_internal.h"
 
@@ -1011,29 +1013,36 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds1(pdx_step_args A)
     if (p0 >= A.n) return;  // whole workgroup
     const int64_t h0 = A.halo_ptr[k];
     const int H = (int)(A.halo_ptr[k + 1] - h0);
+    // CAP: tables longer than the LDS budget are staged in tiles of HL entries (rows and
+    // tables sorted by neighbour: each lane meets the tiles in order)
     const int HL = CAP ? pdx_lds_entries(A.halo_max, A.lds_budget, 32 + (MULTI ? 1 : 0)) : A.halo_max;
-    const int Hs = H < HL ? H : HL;
     const int32_t *const hidx = A.halo + h0;
     uint8_t *const hpid = reinterpret_cast<uint8_t *>(hrec + HL);
-    for (int h = threadIdx.x; h < Hs; h += PDX_BLOCK * S) {
-        const int32_t j = hidx[h];
-        hrec[h] = A.rec_in[j];
-        if (MULTI) hpid[h] = A.pid[j];
-    }
-    __syncthreads();
+    auto stage = [&](int T0, int T1) {
+        for (int h = T0 + (int)threadIdx.x; h < T1; h += PDX_BLOCK * S) {
+            const int32_t j = hidx[h];
+            hrec[h - T0] = A.rec_in[j];
+            if (MULTI) hpid[h - T0] = A.pid[j];
+        }
+    };
     const int sub = threadIdx.x % S;
     const int64_t p = p0 + threadIdx.x / S;
-    if (p >= A.n) return;  // all S lanes of the point together
-    const int c = A.cnt[p];
-    const int64_t b0 = pdx_bond0(A.slice_ptr, p);
+    const bool valid = p < A.n;
+    if constexpr (!CAP) {
+        stage(0, H);
+        __syncthreads();
+        if (!valid) return;  // all S lanes of the point together
+    }
+    const int c = valid ? A.cnt[p] : 0;
+    const int64_t b0 = valid ? pdx_bond0(A.slice_ptr, p) : 0;
     uint16_t *const wbase = A.bw16 + b0;
     const double *const Lbase = A.L + b0;
-    const double4 ri = A.rec_in[p];
+    const double4 ri = valid ? A.rec_in[p] : make_double4(0.0, 0.0, 0.0, 0.0);
     double wvol = 0.0, Ssum = 0.0;
     double ax = 0.0, ay = 0.0, az = 0.0, bx = 0.0, by = 0.0, bz = 0.0;  // A_i, B_i (SPLIT)
     int na = 0;
     double epsc = A.epsc, epsc_lo = A.epsc_lo, epsc_hi = A.epsc_hi, G = A.G, K = A.K;
-    if (MULTI) {
+    if (MULTI && valid) {
         const pdx_pset Pi = A.psets[A.pid[p]];
         epsc = Pi.epsc;
         epsc_lo = Pi.epsc_lo;
@@ -1041,65 +1050,89 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds1(pdx_step_args A)
     }
     // two bonds per lane in flight: their words and lengths are loaded before either is used
     constexpr int U = 2;
-    for (int t0 = sub; t0 < c; t0 += U * S) {
-        uint32_t wdu[U];
-        double Lu[U];
+    int t0 = sub;
+    // the lane's bonds whose neighbours lie in the staged entries [T0, T1)
+    auto walk = [&](int T0, int T1) {
+        for (; t0 < c; t0 += U * S) {
+            uint32_t wdu[U];
+            double Lu[U];
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const bool ok = t0 + u * S < c;
-            wdu[u] = ok ? wbase[(int64_t)(t0 + u * S) * 64] : 0u;
-            Lu[u] = ok ? Lbase[(int64_t)(t0 + u * S) * 64] : 1.0;
-        }
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int t = t0 + u * S;
-            if (t >= c) continue;
-            const int64_t o = (int64_t)t * 64;
-            const uint32_t wd = wdu[u];
-            const double Lb = Lu[u];
-            const int slot = (int)(wd & PDX_W16_SLOT);
-            const bool in_lds = !CAP || slot < HL;
-            const int32_t jg = in_lds ? 0 : hidx[slot];
-            const double4 rj = in_lds ? hrec[slot] : A.rec_in[jg];
-            const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
-            double rl;
-            const double l = pdx_sqrt_rsq((dx * dx + dy * dy) + dz * dz, rl);
-            const double e = l - Lb;
-            const double rL = pdx_rcp1(Lb);
-            bool act;
-            if (MULTI) {
-                // The neighbour fails its own copy of the bond against its eps_c; the stretch is
-                // the same number at both ends, so the owner tracks that copy's state (RACT)
-                // exactly, which the gather of pass 2 needs.
-                const double eps = e * rL;
-                const pdx_pset Pj = A.psets[in_lds ? hpid[slot] : A.pid[jg]];
-                uint32_t nw = wd;
-                if ((wd & PDX_W16_ACT) && (wd & PDX_W16_FP) && eps >= epsc_lo)
-                    if ((eps > epsc_hi) ? true : ((e / Lb) > epsc)) nw &= ~PDX_W16_ACT;
-                if ((wd & PDX_W16_RACT) && (wd & PDX_W16_FP) && eps >= Pj.epsc_lo)
-                    if ((eps > Pj.epsc_hi) ? true : ((e / Lb) > Pj.epsc)) nw &= ~PDX_W16_RACT;
-                if (nw != wd) wbase[o] = (uint16_t)nw;
-                act = (nw & PDX_W16_ACT) != 0;
-            } else {
-                act = bond_fail16(wbase + o, wd, e, Lb, e * rL, epsc, epsc_lo, epsc_hi);
+            for (int u = 0; u < U; u++) {
+                const bool ok = t0 + u * S < c;
+                wdu[u] = ok ? wbase[(int64_t)(t0 + u * S) * 64] : 0u;
+                Lu[u] = ok ? Lbase[(int64_t)(t0 + u * S) * 64] : 1.0;
             }
-            if (act) {
-                double w = A.delta * rL;  // linear_kernel(delta, L)
-                if (SC) w *= A.scf[b0 + o];  // omega * beta
-                wvol += (w * (Lb * Lb)) * rj.w;
-                Ssum += ((w * Lb) * e) * rj.w;
-                na++;
-                if (SPLIT) {
-                    const double q = (w * rl) * rj.w;
-                    const double qL = q * Lb, qe = q * e;
-                    ax = __builtin_fma(qL, dx, ax);
-                    ay = __builtin_fma(qL, dy, ay);
-                    az = __builtin_fma(qL, dz, az);
-                    bx = __builtin_fma(qe, dx, bx);
-                    by = __builtin_fma(qe, dy, by);
-                    bz = __builtin_fma(qe, dz, bz);
+            int kk = U;
+            if (CAP) {
+                kk = 0;
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    if (kk == u && t0 + u * S < c && (int)(wdu[u] & PDX_W16_SLOT) < T1) kk = u + 1;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int t = t0 + u * S;
+                if (u >= kk || t >= c) continue;
+                const int64_t o = (int64_t)t * 64;
+                const uint32_t wd = wdu[u];
+                const double Lb = Lu[u];
+                const int ls = (int)(wd & PDX_W16_SLOT) - T0;
+                const double4 rj = hrec[ls];
+                const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
+                double rl;
+                const double l = pdx_sqrt_rsq((dx * dx + dy * dy) + dz * dz, rl);
+                const double e = l - Lb;
+                const double rL = pdx_rcp1(Lb);
+                bool act;
+                if (MULTI) {
+                    // The neighbour fails its own copy of the bond against its eps_c; the stretch
+                    // is the same number at both ends, so the owner tracks that copy's state
+                    // (RACT) exactly, which the gather of pass 2 needs.
+                    const double eps = e * rL;
+                    const pdx_pset Pj = A.psets[hpid[ls]];
+                    uint32_t nw = wd;
+                    if ((wd & PDX_W16_ACT) && (wd & PDX_W16_FP) && eps >= epsc_lo)
+                        if ((eps > epsc_hi) ? true : ((e / Lb) > epsc)) nw &= ~PDX_W16_ACT;
+                    if ((wd & PDX_W16_RACT) && (wd & PDX_W16_FP) && eps >= Pj.epsc_lo)
+                        if ((eps > Pj.epsc_hi) ? true : ((e / Lb) > Pj.epsc)) nw &= ~PDX_W16_RACT;
+                    if (nw != wd) wbase[o] = (uint16_t)nw;
+                    act = (nw & PDX_W16_ACT) != 0;
+                } else {
+                    act = bond_fail16(wbase + o, wd, e, Lb, e * rL, epsc, epsc_lo, epsc_hi);
+                }
+                if (act) {
+                    double w = A.delta * rL;  // linear_kernel(delta, L)
+                    if (SC) w *= A.scf[b0 + o];  // omega * beta
+                    wvol += (w * (Lb * Lb)) * rj.w;
+                    Ssum += ((w * Lb) * e) * rj.w;
+                    na++;
+                    if (SPLIT) {
+                        const double q = (w * rl) * rj.w;
+                        const double qL = q * Lb, qe = q * e;
+                        ax = __builtin_fma(qL, dx, ax);
+                        ay = __builtin_fma(qL, dy, ay);
+                        az = __builtin_fma(qL, dz, az);
+                        bx = __builtin_fma(qe, dx, bx);
+                        by = __builtin_fma(qe, dy, by);
+                        bz = __builtin_fma(qe, dz, bz);
+                    }
                 }
             }
+            if (CAP && kk < U) {
+                t0 += kk * S;  // resume at the first bond past the tile (or the row's end)
+                return;
+            }
+        }
+    };
+    if constexpr (!CAP) {
+        walk(0, H);
+    } else {
+        for (int T0 = 0; T0 < H; T0 += HL) {
+            const int T1 = H < T0 + HL ? H : T0 + HL;
+            if (T0 > 0) __syncthreads();  // the previous tile is consumed
+            stage(T0, T1);
+            __syncthreads();
+            walk(T0, T1);
         }
     }
 #pragma unroll
@@ -1116,6 +1149,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds1(pdx_step_args A)
             bz += __shfl_xor(bz, o);
         }
     }
+    if (!valid) return;
     if (SPLIT) {
         // every lane holds the sums; lane d < 3 stores component d of the own part, lane 3
         // the per-point pair and the counts
@@ -1169,6 +1203,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds2(pdx_step_args A)
     constexpr bool SC = (F & PDX_FEAT_SC) != 0;
     constexpr bool CAP = (F & PDX_FEAT_CAP) != 0;
     extern __shared__ double4 hrec[];
+    // CAP: tables longer than the LDS budget are staged in tiles of HL entries
     const int HL = CAP ? pdx_lds_entries(A.halo_max, A.lds_budget, SPLIT ? 40 : 48 + 1) : A.halo_max;
     // SPLIT: hrec holds (x, y, z, (c1 V)_j), hc2 (c2 V)_j; otherwise hrec x|V, hc (wvol, dil)
     double2 *const hc = reinterpret_cast<double2 *>(hrec + HL);
@@ -1179,121 +1214,145 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds2(pdx_step_args A)
     const int64_t h0 = A.halo_ptr[k];
     const int H = (int)(A.halo_ptr[k + 1] - h0);
     const double2 *const cst = reinterpret_cast<const double2 *>(A.pstate);
-    const int Hs = H < HL ? H : HL;
     const int32_t *const hidx = A.halo + h0;
     uint8_t *const hpid = reinterpret_cast<uint8_t *>(hc + HL);
-    for (int h = threadIdx.x; h < Hs; h += PDX_BLOCK * S) {
-        const int64_t j = hidx[h];
-        const double4 r = A.rec_in[j];
-        const double2 cj = cst[j];
-        if (SPLIT) {
-            hrec[h] = make_double4(r.x, r.y, r.z, cj.x);
-            hc2[h] = cj.y;
-        } else {
-            hrec[h] = r;
-            hc[h] = cj;
-            hpid[h] = A.pid[j];
+    auto stage = [&](int T0, int T1) {
+        for (int h = T0 + (int)threadIdx.x; h < T1; h += PDX_BLOCK * S) {
+            const int64_t j = hidx[h];
+            const double4 r = A.rec_in[j];
+            const double2 cj = cst[j];
+            if (SPLIT) {
+                hrec[h - T0] = make_double4(r.x, r.y, r.z, cj.x);
+                hc2[h - T0] = cj.y;
+            } else {
+                hrec[h - T0] = r;
+                hc[h - T0] = cj;
+                hpid[h - T0] = A.pid[j];
+            }
         }
-    }
-    __syncthreads();
+    };
     const int sub = threadIdx.x % S;
     const int64_t p = p0 + threadIdx.x / S;
-    if (p >= A.n) return;  // all S lanes of the point together
-    const int c = A.cnt[p];
-    const int64_t b0 = pdx_bond0(A.slice_ptr, p);
+    const bool valid = p < A.n;
+    if constexpr (!CAP) {
+        stage(0, H);
+        __syncthreads();
+        if (!valid) return;  // all S lanes of the point together
+    }
+    const int c = valid ? A.cnt[p] : 0;
+    const int64_t b0 = valid ? pdx_bond0(A.slice_ptr, p) : 0;
     const uint16_t *const wbase = A.bw16 + b0;
     const double *const Lbase = A.L + b0;
-    const double4 ri = A.rec_in[p];
-    const double2 ci = MULTI ? cst[p] : make_double2(0.0, 0.0);
+    const double4 ri = valid ? A.rec_in[p] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const double2 ci = (MULTI && valid) ? cst[p] : make_double2(0.0, 0.0);
     double bx = 0.0, by = 0.0, bz = 0.0;
-    if (SPLIT && sub == 0) {
+    if (SPLIT && sub == 0 && valid) {
         const double *own = A.pstate + 2 * A.n + 3 * p;
         bx = own[0];
         by = own[1];
         bz = own[2];
     }
     double Gi = A.G, Ki = A.K, rho = A.rho;
-    if (MULTI) {
+    if (MULTI && valid) {
         const pdx_pset Pi = A.psets[A.pid[p]];
         Gi = Pi.G;
         Ki = Pi.K;
         rho = Pi.rho;
     }
     constexpr int U = 2;  // two bonds per lane in flight
-    for (int t0 = sub; t0 < c; t0 += U * S) {
-        uint32_t wdu[U];
-        double Lu[U];
+    int t0 = sub;
+    // the lane's bonds whose neighbours lie in the staged entries [T0, T1)
+    auto walk = [&](int T0, int T1) {
+        for (; t0 < c; t0 += U * S) {
+            uint32_t wdu[U];
+            double Lu[U];
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const bool ok = t0 + u * S < c;
-            wdu[u] = ok ? wbase[(int64_t)(t0 + u * S) * 64] : 0u;
-            Lu[u] = ok ? Lbase[(int64_t)(t0 + u * S) * 64] : 1.0;
-        }
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int t = t0 + u * S;
-            if (t >= c) continue;
-            const int64_t o = (int64_t)t * 64;
-            const uint32_t wd = wdu[u];
-            const double Lb = Lu[u];
-            const bool act = (wd & PDX_W16_ACT) != 0;
-            const bool ract = MULTI ? (wd & PDX_W16_RACT) != 0 : act;  // one set: both copies agree
-            if (!act && !ract) continue;
-            const int slot = (int)(wd & PDX_W16_SLOT);
-            const bool in_lds = !CAP || slot < HL;
-            const int32_t jg = in_lds ? 0 : hidx[slot];
-            double4 rj;
-            double2 cj;
-            if (SPLIT) {
-                if (in_lds) {
-                    rj = hrec[slot];
-                    cj = make_double2(rj.w, hc2[slot]);
-                } else {
-                    rj = A.rec_in[jg];
-                    cj = cst[jg];
-                }
-            } else {
-                rj = in_lds ? hrec[slot] : A.rec_in[jg];
-                cj = in_lds ? hc[slot] : cst[jg];
+            for (int u = 0; u < U; u++) {
+                const bool ok = t0 + u * S < c;
+                wdu[u] = ok ? wbase[(int64_t)(t0 + u * S) * 64] : 0u;
+                Lu[u] = ok ? Lbase[(int64_t)(t0 + u * S) * 64] : 1.0;
             }
-            const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
-            double rl;
-            const double l = pdx_sqrt_rsq((dx * dx + dy * dy) + dz * dz, rl);
-            double w = A.delta * pdx_rcp1(Lb);
-            if (SC) w *= A.scf[b0 + o];  // beta is symmetric: both ends' copies of the bond carry it
-            const double e = l - Lb;
-            if (SPLIT) {
-                // the reverse part (c2 V)_j L + (c1 V)_j e
-                const double s = (w * rl) * (cj.y * Lb + cj.x * e);
+            int kk = U;
+            if (CAP) {
+                kk = 0;
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    if (kk == u && t0 + u * S < c && (int)(wdu[u] & PDX_W16_SLOT) < T1) kk = u + 1;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int t = t0 + u * S;
+                if (u >= kk || t >= c) continue;
+                const int64_t o = (int64_t)t * 64;
+                const uint32_t wd = wdu[u];
+                const double Lb = Lu[u];
+                const bool act = (wd & PDX_W16_ACT) != 0;
+                const bool ract = MULTI ? (wd & PDX_W16_RACT) != 0 : act;  // one set: both copies agree
+                if (!act && !ract) continue;
+                const int ls = (int)(wd & PDX_W16_SLOT) - T0;
+                double4 rj;
+                double2 cj;
+                if (SPLIT) {
+                    rj = hrec[ls];
+                    cj = make_double2(rj.w, hc2[ls]);
+                } else {
+                    rj = hrec[ls];
+                    cj = hc[ls];
+                }
+                const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
+                double rl;
+                const double l = pdx_sqrt_rsq((dx * dx + dy * dy) + dz * dz, rl);
+                double w = A.delta * pdx_rcp1(Lb);
+                if (SC) w *= A.scf[b0 + o];  // beta is symmetric: both ends' copies of the bond carry it
+                const double e = l - Lb;
+                if (SPLIT) {
+                    // the reverse part (c2 V)_j L + (c1 V)_j e
+                    const double s = (w * rl) * (cj.y * Lb + cj.x * e);
+                    bx = __builtin_fma(s, dx, bx);
+                    by = __builtin_fma(s, dy, by);
+                    bz = __builtin_fma(s, dz, bz);
+                    continue;
+                }
+                double fi, fj;
+                if (MULTI) {
+                    const pdx_pset Pj = A.psets[hpid[ls]];
+                    const double Gs = Gi + Pj.G, Ks = Ki + Pj.K;
+                    fi = 0.0;
+                    fj = 0.0;
+                    if (act && ci.x != 0.0) {
+                        const double c1 = (15.0 * Gs) / (2.0 * ci.x);
+                        const double c2 = ci.y * ((3.0 * Ks) / (2.0 * ci.x) - c1 / 3.0);
+                        fi = c2 * Lb + c1 * e;
+                    }
+                    if (ract && cj.x != 0.0) {
+                        const double c1 = (15.0 * Gs) / (2.0 * cj.x);
+                        const double c2 = cj.y * ((3.0 * Ks) / (2.0 * cj.x) - c1 / 3.0);
+                        fj = c2 * Lb + c1 * e;
+                    }
+                } else {
+                    fi = ci.y * Lb + ci.x * e;
+                    fj = cj.y * Lb + cj.x * e;
+                }
+                const double s = (w * rl) * ((fi + fj) * rj.w);
                 bx = __builtin_fma(s, dx, bx);
                 by = __builtin_fma(s, dy, by);
                 bz = __builtin_fma(s, dz, bz);
-                continue;
             }
-            double fi, fj;
-            if (MULTI) {
-                const pdx_pset Pj = A.psets[in_lds ? hpid[slot] : A.pid[jg]];
-                const double Gs = Gi + Pj.G, Ks = Ki + Pj.K;
-                fi = 0.0;
-                fj = 0.0;
-                if (act && ci.x != 0.0) {
-                    const double c1 = (15.0 * Gs) / (2.0 * ci.x);
-                    const double c2 = ci.y * ((3.0 * Ks) / (2.0 * ci.x) - c1 / 3.0);
-                    fi = c2 * Lb + c1 * e;
-                }
-                if (ract && cj.x != 0.0) {
-                    const double c1 = (15.0 * Gs) / (2.0 * cj.x);
-                    const double c2 = cj.y * ((3.0 * Ks) / (2.0 * cj.x) - c1 / 3.0);
-                    fj = c2 * Lb + c1 * e;
-                }
-            } else {
-                fi = ci.y * Lb + ci.x * e;
-                fj = cj.y * Lb + cj.x * e;
+            if (CAP && kk < U) {
+                t0 += kk * S;  // resume at the first bond past the tile (or the row's end)
+                return;
             }
-            const double s = (w * rl) * ((fi + fj) * rj.w);
-            bx = __builtin_fma(s, dx, bx);
-            by = __builtin_fma(s, dy, by);
-            bz = __builtin_fma(s, dz, bz);
+        }
+    };
+    if constexpr (!CAP) {
+        walk(0, H);
+    } else {
+        for (int T0 = 0; T0 < H; T0 += HL) {
+            const int T1 = H < T0 + HL ? H : T0 + HL;
+            if (T0 > 0) __syncthreads();  // the previous tile is consumed
+            stage(T0, T1);
+            __syncthreads();
+            walk(T0, T1);
         }
     }
 #pragma unroll
@@ -1302,6 +1361,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds2(pdx_step_args A)
         by += __shfl_xor(by, o);
         bz += __shfl_xor(bz, o);
     }
+    if (!valid) return;
     static_assert(S == 4, "the tail takes four lanes per point");
     const double b[3] = {bx, by, bz};
     point_tail4(A, p, ri, b, sub, rho);
@@ -2124,10 +2184,46 @@ static size_t lds_bytes(const pdx_ctx *c, int entry, int budget)
 
 static size_t lds_bytes(const pdx_ctx *c, int entry) { return lds_bytes(c, entry, pdx_lds_budget(c)); }
 
+// VGPRs per lane of a kernel (queried once): a 1024-thread block fits twice on a CU only
+// with at most 64
+static int kernel_vgprs(const void *fn)
+{
+    static std::mutex m;
+    static std::unordered_map<const void *, int> cache;
+    std::lock_guard<std::mutex> lock(m);
+    auto it = cache.find(fn);
+    if (it != cache.end()) return it->second;
+    hipFuncAttributes at{};
+    const int v = hipFuncGetAttributes(&at, fn) == hipSuccess ? at.numRegs : 0;
+    cache[fn] = v;
+    return v;
+}
+
+// Launch of an LDS-staged kernel whose table entries take `entry` bytes.  A capped variant
+// (tables longer than the budget) whose registers allow one block per CU anyway may take the
+// CU's whole LDS.  Measured at 576 k points: a tiled kernel gains from fewer tiles whenever
+// the table is longer than the budget; a kernel that reads the entries past the budget from
+// global memory gains once they are many (horizon 5.015 dx: ~80 % past the budget) and
+// loses a little when they are few (4.015 dx: ~15 %).
+static void launch_lds(pdx_ctx *c, const void *fn, bool capped, bool tiled, int entry, dim3 G,
+                       dim3 BL, const pdx_step_args &a)
+{
+    pdx_step_args af = a;
+    int bud = pdx_lds_budget(c);
+    const bool many_past = (int64_t)c->halo_max * entry > (int64_t)bud * 3 / 2;
+    if (capped && (tiled || many_past) && kernel_vgprs(fn) > 64) bud = pdx_lds_budget_full(c);
+    af.lds_budget = bud;
+    void *args[] = {&af};
+    (void)hipLaunchKernel(fn, G, BL, args, lds_bytes(c, entry, bud), c->stream);
+}
+
 template <int F>
 static void launch_bb_feat(pdx_ctx *c, bool mixed, dim3 G, dim3 BL, size_t lds, const pdx_step_args &a)
 {
-    if (mixed) hipLaunchKernelGGL((k_bb_lds_mx<4, PDX_LDS_LANES, F>), G, BL, lds, c->stream, a);
+    if (mixed && (F & PDX_FEAT_CAP))
+        launch_lds(c, (const void *)&k_bb_lds_mx<4, PDX_LDS_LANES, F>, true, false,
+                   32 + ((F & PDX_FEAT_MULTI) ? 1 : 0), G, BL, a);
+    else if (mixed) hipLaunchKernelGGL((k_bb_lds_mx<4, PDX_LDS_LANES, F>), G, BL, lds, c->stream, a);
     else hipLaunchKernelGGL((k_bb_lds<2, PDX_LDS_LANES, F>), G, BL, lds, c->stream, a);
 }
 
@@ -2145,10 +2241,11 @@ static void launch_bb_feat(pdx_ctx *c, bool mixed, dim3 G, dim3 BL, size_t lds, 
     }
 
 template <int F>
-static void launch_osb_feat(pdx_ctx *c, int pass, dim3 G, dim3 BL, size_t lds, const pdx_step_args &a)
+static void launch_osb_feat(pdx_ctx *c, int pass, dim3 G, dim3 BL, int entry, const pdx_step_args &a)
 {
-    if (pass == 1) hipLaunchKernelGGL((k_osb_lds1<PDX_LDS_LANES, F>), G, BL, lds, c->stream, a);
-    else hipLaunchKernelGGL((k_osb_lds2<PDX_LDS_LANES, F>), G, BL, lds, c->stream, a);
+    const void *fn = pass == 1 ? (const void *)&k_osb_lds1<PDX_LDS_LANES, F>
+                               : (const void *)&k_osb_lds2<PDX_LDS_LANES, F>;
+    launch_lds(c, fn, (F & PDX_FEAT_CAP) != 0, true, entry, G, BL, a);
 }
 
 pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
@@ -2205,8 +2302,8 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
             switch (feat) {
             case 0: hipLaunchKernelGGL((k_osb_lds1_mx<4, PDX_LDS_LANES, 0>), G, BL, lds, c->stream, a); break;
             case PDX_FEAT_SC: hipLaunchKernelGGL((k_osb_lds1_mx<4, PDX_LDS_LANES, PDX_FEAT_SC>), G, BL, lds, c->stream, a); break;
-            case PDX_FEAT_CAP: hipLaunchKernelGGL((k_osb_lds1_mx<4, PDX_LDS_LANES, PDX_FEAT_CAP>), G, BL, lds, c->stream, a); break;
-            default: hipLaunchKernelGGL((k_osb_lds1_mx<4, PDX_LDS_LANES, PDX_FEAT_SC | PDX_FEAT_CAP>), G, BL, lds, c->stream, a); break;
+            case PDX_FEAT_CAP: launch_lds(c, (const void *)&k_osb_lds1_mx<4, PDX_LDS_LANES, PDX_FEAT_CAP>, true, false, 32, G, BL, a); break;
+            default: launch_lds(c, (const void *)&k_osb_lds1_mx<4, PDX_LDS_LANES, PDX_FEAT_SC | PDX_FEAT_CAP>, true, false, 32, G, BL, a); break;
             }
             PDX_HIP(c, hipGetLastError());
             PDX_TRY(pdx_timed_end(c, "pass1"));
@@ -2214,8 +2311,8 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
             switch (feat2) {
             case 0: hipLaunchKernelGGL((k_osb_lds2_mx<4, PDX_LDS_LANES, 0>), G, BL, lds2, c->stream, a); break;
             case PDX_FEAT_SC: hipLaunchKernelGGL((k_osb_lds2_mx<4, PDX_LDS_LANES, PDX_FEAT_SC>), G, BL, lds2, c->stream, a); break;
-            case PDX_FEAT_CAP: hipLaunchKernelGGL((k_osb_lds2_mx<4, PDX_LDS_LANES, PDX_FEAT_CAP>), G, BL, lds2, c->stream, a); break;
-            default: hipLaunchKernelGGL((k_osb_lds2_mx<4, PDX_LDS_LANES, PDX_FEAT_SC | PDX_FEAT_CAP>), G, BL, lds2, c->stream, a); break;
+            case PDX_FEAT_CAP: launch_lds(c, (const void *)&k_osb_lds2_mx<4, PDX_LDS_LANES, PDX_FEAT_CAP>, true, false, 36, G, BL, a); break;
+            default: launch_lds(c, (const void *)&k_osb_lds2_mx<4, PDX_LDS_LANES, PDX_FEAT_SC | PDX_FEAT_CAP>, true, false, 36, G, BL, a); break;
             }
             PDX_HIP(c, hipGetLastError());
             PDX_TRY(pdx_timed_end(c, "step"));
@@ -2226,16 +2323,16 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
             const int mb = c->multi() ? 1 : 0;  // the neighbours' parameter sets
             // pass 2 entries: x and (c1 V, c2 V) with one set, x|V, (wvol, dil) and the set
             const int e2 = mb ? 48 + mb : 40;
-            const size_t l1 = lds_bytes(c, 32 + mb), l2b = lds_bytes(c, e2);
-            const int f1 = pdx_features(c) | (lds_capped(c, 32 + mb) ? PDX_FEAT_CAP : 0);
+            const int e1 = 32 + mb;
+            const int f1 = pdx_features(c) | (lds_capped(c, e1) ? PDX_FEAT_CAP : 0);
             const int f2 = pdx_features(c) | (lds_capped(c, e2) ? PDX_FEAT_CAP : 0);
-#define PDX_OSB1_LAUNCH(F) launch_osb_feat<F>(c, 1, G, BL, l1, a)
-#define PDX_OSB2_LAUNCH(F) launch_osb_feat<F>(c, 2, G, BL, l2b, a)
+#define PDX_OSB1_LAUNCH(F) launch_osb_feat<F>(c, 1, G, BL, e1, a)
+#define PDX_OSB2_LAUNCH(F) launch_osb_feat<F>(c, 2, G, BL, e2, a)
             PDX_TRY(pdx_timed_begin(c));
             if (f1) {
                 PDX_FEAT_SWITCH(f1, PDX_OSB1_LAUNCH)
             } else {
-                launch_osb_feat<0>(c, 1, G, BL, l1, a);
+                launch_osb_feat<0>(c, 1, G, BL, e1, a);
             }
             PDX_HIP(c, hipGetLastError());
             PDX_TRY(pdx_timed_end(c, "pass1"));
@@ -2243,7 +2340,7 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
             if (f2) {
                 PDX_FEAT_SWITCH(f2, PDX_OSB2_LAUNCH)
             } else {
-                launch_osb_feat<0>(c, 2, G, BL, l2b, a);
+                launch_osb_feat<0>(c, 2, G, BL, e2, a);
             }
             PDX_HIP(c, hipGetLastError());
             PDX_TRY(pdx_timed_end(c, "step"));

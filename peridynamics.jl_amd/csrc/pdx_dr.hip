@@ -181,8 +181,6 @@ extern "C" pdx_status pdx_dr_init(pdx_ctx *c, double dt, double damping_factor)
 {
     if (!c) return PDX_ERR_ARG;
     if (!c->bonds_built) return pdx_fail(c, PDX_ERR_STATE, "pdx_build_bonds first");
-    if (c->kind != PDX_BB)
-        return pdx_fail(c, PDX_ERR_UNSUPPORTED, "DynamicRelaxation is built for BBMaterial");
     if (!(dt > 0)) return pdx_fail(c, PDX_ERR_ARG, "`stepsize` should be larger than zero!");
     if (!(damping_factor > 0)) return pdx_fail(c, PDX_ERR_ARG, "`damping_factor` should be larger than zero!");
     PDX_HIP(c, hipSetDevice(c->device));

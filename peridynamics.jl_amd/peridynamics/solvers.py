@@ -114,7 +114,7 @@ class NewtonKrylov:
 
 class DynamicRelaxation:
     """DynamicRelaxation(; steps, stepsize=1.0, damping_factor=1.0):
-    dynamic_relaxation.jl:36-58 (bond-based bodies on the device)."""
+    dynamic_relaxation.jl:36-58 (BB, OSB and CPD bodies on the device)."""
 
     def __init__(self, steps, stepsize=1.0, damping_factor=1.0):
         if steps <= 0:

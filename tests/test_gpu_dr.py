@@ -93,3 +93,45 @@ def test_dr_trajectory_matches_oracle(need_gpu, oracle):
         err = np.max(np.abs(g - ref)) / np.max(np.abs(ref))
         say(f"  {f} normwise error {err:.2e}")
         assert err < 1e-9
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("kind", ["BB", "OSB", "CPD"])
+def test_dr_materials(need_gpu, oracle, kind, precision):
+    """Dynamic relaxation of BB, OSB and CPD bodies (calc_density_matrix! for any bond
+    system, dynamic_relaxation.jl:153-163): the block of the trajectory test against the
+    FP64 oracle.  The adaptive damping divides by differences of consecutive force
+    densities, which amplifies rounding differences from step to step; the state-based
+    forces, sums of terms that cancel more, are compared over 20 steps, BB over 150."""
+    say("dr", kind, precision)
+    n_steps = 150 if kind == "BB" else 20
+    pos, vol = pd.lattice_block(12, 12, 4, 1e-3, jitter_amplitude=0.05, seed=7)
+    mat = {"BB": pd.BBMaterial, "OSB": pd.OSBMaterial, "CPD": pd.CMaterial}[kind]()
+    b = pd.Body(mat, pos, vol)
+    kw = dict(horizon=3.015e-3, rho=2700.0, E=70e9)
+    if kind != "BB":
+        kw["nu"] = 0.25
+    pd.material(b, **kw)
+    y = pos[1]
+    pd.point_set(b, "top", y > y.max() - 1.5e-3)
+    pd.point_set(b, "bottom", y < y.min() + 1.5e-3)
+    pd.velocity_bc(lambda t: 1e-6, b, "top", "y")
+    for d in "xyz":
+        pd.velocity_bc(lambda t: 0.0, b, "bottom", d)
+    dr = pd.DynamicRelaxation(steps=n_steps, stepsize=1.0)
+    dh = pd.MI355XBodyDataHandler(b, precision=precision)
+    dh.calc_force_density(0.0, 1.0)
+    dh.solve_dr(dr)
+    extra = {}
+    if kind == "CPD":
+        extra = dict(model="SVK", Cs=b.mat.zem.Cs, maxdmg=b.mat.maxdmg)
+    orc = oracle.OracleSim(b.xyz, b.volume, kind, as_oracle_params(b.point_params[0], **extra),
+                           fail_permit=b.fail_permit.astype(np.uint8))
+    bcs = [("sd", b.point_sets["top"], 1, lambda t: 1e-6)] + \
+          [("sd", b.point_sets["bottom"], d, lambda t: 0.0) for d in range(3)]
+    orc.dr_run(n_steps, 1.0, 1.0, bcs)
+    g = dh.get("displacement")
+    err = np.max(np.abs(g - orc.u)) / np.max(np.abs(orc.u))
+    say(f"  displacement normwise error {err:.2e}")
+    assert err < (1e-9 if precision == "fp64" else 1e-5)
+    dh.close()

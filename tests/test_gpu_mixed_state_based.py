@@ -167,3 +167,29 @@ def test_mixed_cpd_torsion_trajectory(need_gpu, oracle):
         say(f"  {f} normwise error {err:.3e}")
         assert err < 1e-5
     dh.close()
+
+
+F_AFFINE = np.array([[1.1, 0.01, 0.0], [0.0, 1.2, 0.02], [0.03, 0.0, 1.3]])
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-6)])
+@pytest.mark.parametrize("model", ["SVK", "NHP"])
+def test_cpd_defgrad_exact_for_affine_deformation(need_gpu, model, precision, tol):
+    """test/materials/test_uniform_deformation.jl:5-41 on the device: under x = F X every
+    point's deformation gradient, surface points included, equals F (mixed precision forms
+    F = I + H from FP32 displacement terms: H to FP32 rounding)."""
+    say("cpd affine defgrad", model, precision)
+    n = 5
+    dx = 1.0 / n
+    pos, vol = pd.uniform_box(1.0, 1.0, 1.0, dx)
+    b = pd.Body(pd.CMaterial(model=CPD_MODELS[model]()), pos, vol)
+    pd.material(b, horizon=3.1 * dx, rho=7850.0, E=210e9, nu=0.25)
+    pd.no_failure(b)
+    dh = pd.MI355XBodyDataHandler(b, precision=precision)
+    X = b.xyz
+    dh.set("position", X @ F_AFFINE.T)
+    dh.calc_force_density(0.0, 1e-7)
+    err = np.max(np.abs(dh.get("defgrad") - F_AFFINE.T.reshape(-1)))
+    say(f"  max |F - F_a| {err:.2e}")
+    assert err < tol
+    dh.close()

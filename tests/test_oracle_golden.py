@@ -166,6 +166,22 @@ def _deform(X, dt, F=((1.01, 0.003, 0.0), (0.0, 0.995, 0.0), (0.0, 0.0, 1.0)), a
 KINDS = ["BB", "OSB", "CPD"]
 
 
+F_AFFINE = np.array([[1.1, 0.01, 0.0], [0.0, 1.2, 0.02], [0.03, 0.0, 1.3]])
+
+
+@pytest.mark.parametrize("model", ["SVK", "NHP"])
+def test_cpd_defgrad_exact_for_affine_deformation(oracle, model):
+    """test/materials/test_uniform_deformation.jl:5-41: under x = F X the deformation
+    gradient of every point, surface points with truncated families included, is F (the
+    nonlocal average of an affine map is the map).  A dropped bond term, a transposed F or
+    a wrong K^-1 product breaks it at the surface."""
+    X, vol, p = _cube("CPD", n=5, m=3.1, model=model)
+    sim = oracle.OracleSim(X, vol, "CPD", p, fail_permit=np.zeros(len(X)))
+    sim.force(X @ F_AFFINE.T)
+    F_cm = F_AFFINE.T.reshape(-1)  # column-major, F[r + 3c]
+    assert np.max(np.abs(sim.defgrad - F_cm)) < 1e-12
+
+
 @pytest.mark.parametrize("kind", KINDS)
 def test_zero_force_at_reference(oracle, kind):
     # test/materials/test_force_density.jl:5-21

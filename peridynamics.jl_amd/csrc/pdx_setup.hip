@@ -837,12 +837,6 @@ extern "C" pdx_status pdx_build_bonds(pdx_ctx *c, int64_t *n_bonds_out)
     hipLaunchKernelGGL(k_init_point_state, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                        c->stream, n, c->slice_ptr, c->cnt, c->bw, c->nact, c->damage);
     HIP_C(hipGetLastError());
-    if (c->precision == PDX_MIXED) {
-        TRY_C(pdx_alloc(c, &c->Lf, nb_alloc));
-        hipLaunchKernelGGL(k_L_to_float, dim3((unsigned)((nb_alloc + 255) / 256)), dim3(256), 0,
-                           c->stream, nb_alloc, c->L, c->Lf);
-        HIP_C(hipGetLastError());
-    }
     // zero dynamic state
     for (double *p : {c->u, c->vh, c->v, c->a, c->bint, c->bext})
         HIP_C(hipMemsetAsync(p, 0, sizeof(double) * 3 * n, c->stream));
@@ -870,6 +864,14 @@ extern "C" pdx_status pdx_build_bonds(pdx_ctx *c, int64_t *n_bonds_out)
     if (!getenv("PDX_NO_LDS")) {
         bool ok = false;
         PDX_TRY(build_halos(c, &ok));
+    }
+    // FP32 lengths for the global-gather mixed BB step (the LDS-staged steps take L = |D|)
+    if (c->precision == PDX_MIXED && !c->lds_mode) {
+        const int64_t nba = c->nb_alloc;
+        PDX_TRY(pdx_alloc(c, &c->Lf, nba));
+        hipLaunchKernelGGL(k_L_to_float, dim3((unsigned)((nba + 255) / 256)), dim3(256), 0, c->stream,
+                           nba, c->L, c->Lf);
+        PDX_HIP(c, hipGetLastError());
     }
     PDX_TRY(upload_psets(c));
     if (c->multi() && !c->lds_mode)

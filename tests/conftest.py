@@ -80,3 +80,28 @@ class lds_budget:
             os.environ.pop("PDX_LDS_BUDGET", None)
         else:
             os.environ["PDX_LDS_BUDGET"] = self.old
+
+
+class bond_layout:
+    """The device bond layout for the contexts created inside the block: "lds" (default),
+    "lds12k" (the LDS-staged layout with a 12 KB table budget, so the tables are longer
+    than what a block stages at once) or "global" (PDX_NO_LDS: the global-gather kernels
+    that serve tables the 16-bit bond words cannot address)."""
+
+    def __init__(self, layout):
+        self.layout = layout
+
+    def __enter__(self):
+        self.old = {k: os.environ.get(k) for k in ("PDX_LDS_BUDGET", "PDX_NO_LDS")}
+        if self.layout == "lds12k":
+            os.environ["PDX_LDS_BUDGET"] = "12288"
+        elif self.layout == "global":
+            os.environ["PDX_NO_LDS"] = "1"
+        return self
+
+    def __exit__(self, *exc):
+        for k, v in self.old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v

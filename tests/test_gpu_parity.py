@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import peridynamics as pd
-from conftest import as_oracle_params, lds_budget
+from conftest import as_oracle_params, bond_layout
 
 pytestmark = pytest.mark.gpu
 
@@ -181,18 +181,20 @@ def test_force_density_nonuniform_volume(need_gpu, oracle, kind):
     assert err <= FP64_TOL
 
 
-@pytest.mark.parametrize("budget", [0, 12288])
+@pytest.mark.parametrize("layout", ["lds", "lds12k", "global"])
 @pytest.mark.parametrize("kind", ["BB", "OSB", "CPD"])
 @pytest.mark.parametrize("amp", [2e-7, 3e-6])
-def test_force_density_fp64(need_gpu, oracle, kind, amp, budget):
-    """budget 12288: a 12 KB LDS budget, so the neighbour tables are longer than what a
-    block stages at once (staged in tiles, or read past the budget from global memory)."""
-    say("force fp64", kind, amp, budget)
+def test_force_density_fp64(need_gpu, oracle, kind, amp, layout):
+    """layout lds12k: a 12 KB LDS budget, so the neighbour tables are longer than what a
+    block stages at once (staged in tiles, or read past the budget from global memory);
+    global: the global-gather kernels."""
+    say("force fp64", kind, amp, layout)
     E = 70e9 if kind == "BB" else 210e9
     b = body_block(kind, 24, 20, 8, jit=0.05, E=E, Gc=100.0 if kind == "BB" else None,
                    rho=2700.0 if kind == "BB" else 7850.0)
-    with lds_budget(budget):
+    with bond_layout(layout):
         dh = pd.MI355XBodyDataHandler(b)
+    assert dh.ctx.layout()["lds_mode"] == (layout != "global")
     orc = oracle_for(oracle, b, kind)
     X = b.xyz
     x = X + displacement_field(X, amp)
@@ -254,11 +256,11 @@ def test_osb_known_answer(need_gpu, golden):
     np.testing.assert_allclose(bi[1], [-g["b12"], 0, 0], rtol=1e-13, atol=0)
 
 
-@pytest.mark.parametrize("budget", [0, 12288])
-def test_mixed_force_error(need_gpu, oracle, budget):
-    say("force mixed", budget)
+@pytest.mark.parametrize("layout", ["lds", "lds12k", "global"])
+def test_mixed_force_error(need_gpu, oracle, layout):
+    say("force mixed", layout)
     b = body_block("BB", 24, 20, 8, jit=0.05)
-    with lds_budget(budget):
+    with bond_layout(layout):
         dh = pd.MI355XBodyDataHandler(b, precision="mixed")
     orc = oracle_for(oracle, b, "BB")
     X = b.xyz

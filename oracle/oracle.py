@@ -74,6 +74,9 @@ def _declare(L):
     L.orc_cpd_force.restype = None
     L.orc_cpd_force.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _INT, _D, _D,
                                 _D, _D, _D, _D, _D, _D, _D, _P, _P, _P, _P]
+    L.orc_cpd_force_sets.restype = None
+    L.orc_cpd_force_sets.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _INT,
+                                     _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _P, _P, _P]
     L.orc_min_timestep.restype = _D
     L.orc_min_timestep.argtypes = [_I64, _P, _P, _P, _P, _D, _D]
     L.orc_update_vel_half.restype = None
@@ -187,8 +190,8 @@ class OracleSim:
         if point_set is not None:
             sets = [dict(q) for q in params]
             ps = np.asarray(point_set, np.int64)
-            if kind not in ("BB", "OSB"):
-                raise ValueError("several parameter sets: BB and OSB")
+            if kind not in ("BB", "OSB", "CPD"):
+                raise ValueError(kind)
             self.pt = {k: np.ascontiguousarray([sets[m][k] for m in ps], np.float64)
                        for k in ("bc", "epsc", "G", "K", "rho", "lambda", "mu")}
             params = sets[0]
@@ -279,6 +282,16 @@ class OracleSim:
                                     _p(self.bond_length), _p(self.n_active), _p(self.damage),
                                     _p(t["bc"]), _p(t["epsc"]), _p(self.b_int), _p(self.b_abs),
                                     _p(self.scf))
+            elif self.kind == "CPD":
+                model = {"SVK": 0, "NHP": 1}[p.get("model", "SVK")]
+                L.orc_cpd_force_sets(self.n, _p(self.X), _p(self.x), _p(self.vol),
+                                     _p(self.row_ptr), _p(self.nbr), _p(self.L), _p(self.bfp),
+                                     _p(self.active), _p(self.omega), _p(self.n_active),
+                                     _p(self.damage), model, _p(t["lambda"]), _p(t["mu"]),
+                                     _p(t["G"]), _p(t["K"]), _p(t["bc"]), _p(t["epsc"]),
+                                     p["delta"], p.get("Cs", 0.5), p.get("maxdmg", 0.85),
+                                     _p(self.defgrad), _p(self.cauchy), _p(self.b_int),
+                                     _p(self.b_abs))
             else:
                 L.orc_osb_force_sets(self.n, _p(self.X), _p(self.x), _p(self.vol),
                                      _p(self.row_ptr), _p(self.nbr), _p(self.L), _p(self.bfp),

@@ -833,18 +833,24 @@ static void cauchy(const double *P, const double *F, double *sig)
 /* c_force_density! ZEMSilling  correspondence.jl:226-251                          */
 /* model: 0 SaintVenantKirchhoff, 1 NeoHookePenalty                                */
 /* ------------------------------------------------------------------------------ */
-void orc_cpd_force(int64_t n, const double *X, const double *x, const double *vol,
-                   const int64_t *row_ptr, const int32_t *nbr, const double *L,
-                   const uint8_t *bond_fp, uint8_t *active, const double *omega,
-                   int64_t *n_active, double *damage, int model, double lambda, double mu,
-                   double G, double K, double bc, double delta, double Cs, double maxdmg,
-                   double epsc, double *defgrad, double *cauchy_out, double *b_int,
-                   double *b_abs)
+/* Parameters per point (`ps` = 1: arrays of n, one value per point's parameter set, the
+ * params of the point whose loop runs, as force_density_point! receives them with a
+ * ParameterHandler) or one value for all (`ps` = 0). */
+static void cpd_force_core(int64_t n, const double *X, const double *x, const double *vol,
+                           const int64_t *row_ptr, const int32_t *nbr, const double *L,
+                           const uint8_t *bond_fp, uint8_t *active, const double *omega,
+                           int64_t *n_active, double *damage, int model, const double *lambda_p,
+                           const double *mu_p, const double *G_p, const double *K_p,
+                           const double *bc_p, const double *epsc_p, int64_t ps, double delta,
+                           double Cs, double maxdmg, double *defgrad, double *cauchy_out,
+                           double *b_int, double *b_abs)
 {
     for (int64_t k = 0; k < 3 * n; k++) b_int[k] = 0.0;
     if (b_abs) for (int64_t k = 0; k < 3 * n; k++) b_abs[k] = 0.0;
-    double beta = (Cs * bc) / delta;
     for (int64_t i = 0; i < n; i++) {
+        const double lambda = lambda_p[ps * i], mu = mu_p[ps * i], G = G_p[ps * i], K = K_p[ps * i];
+        const double epsc = epsc_p[ps * i];
+        const double beta = (Cs * bc_p[ps * i]) / delta;
         int64_t na = 0;
         for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
             int64_t j = nbr[b];
@@ -926,6 +932,35 @@ void orc_cpd_force(int64_t n, const double *X, const double *x, const double *vo
             }
         }
     }
+}
+
+void orc_cpd_force(int64_t n, const double *X, const double *x, const double *vol,
+                   const int64_t *row_ptr, const int32_t *nbr, const double *L,
+                   const uint8_t *bond_fp, uint8_t *active, const double *omega,
+                   int64_t *n_active, double *damage, int model, double lambda, double mu,
+                   double G, double K, double bc, double delta, double Cs, double maxdmg,
+                   double epsc, double *defgrad, double *cauchy_out, double *b_int,
+                   double *b_abs)
+{
+    cpd_force_core(n, X, x, vol, row_ptr, nbr, L, bond_fp, active, omega, n_active, damage, model,
+                   &lambda, &mu, &G, &K, &bc, &epsc, 0, delta, Cs, maxdmg, defgrad, cauchy_out,
+                   b_int, b_abs);
+}
+
+/* several parameter sets: lambda, mu, G, K, bc, eps_c per point (the sets share delta, the
+ * material's model, Cs and maxdmg) */
+void orc_cpd_force_sets(int64_t n, const double *X, const double *x, const double *vol,
+                        const int64_t *row_ptr, const int32_t *nbr, const double *L,
+                        const uint8_t *bond_fp, uint8_t *active, const double *omega,
+                        int64_t *n_active, double *damage, int model, const double *lambda_pt,
+                        const double *mu_pt, const double *G_pt, const double *K_pt,
+                        const double *bc_pt, const double *epsc_pt, double delta, double Cs,
+                        double maxdmg, double *defgrad, double *cauchy_out, double *b_int,
+                        double *b_abs)
+{
+    cpd_force_core(n, X, x, vol, row_ptr, nbr, L, bond_fp, active, omega, n_active, damage, model,
+                   lambda_pt, mu_pt, G_pt, K_pt, bc_pt, epsc_pt, 1, delta, Cs, maxdmg, defgrad,
+                   cauchy_out, b_int, b_abs);
 }
 
 /* ------------------------------------------------------------------------------ */

@@ -48,6 +48,7 @@ struct pdx_nk;   // Newton-Krylov workspace (pdx_nk.hip)
 // one parameter set on the device (bodies with several material! sets)
 struct pdx_pset {
     double bc, epsc, epsc_lo, epsc_hi, rho, delta, G, K, lam, mu;
+    double beta;  // ZEMSilling beta = Cs bc / delta of the set (correspondence.jl:232)
 };
 #define PDX_MAX_PSETS 255
 // feature mask of the LDS-staged step kernels

@@ -214,7 +214,10 @@ extern "C" pdx_status pdx_set_material_sets(pdx_ctx *c, int32_t kind, int32_t n_
     c->pset_of.clear();
     if (n_sets == 1) return PDX_OK;
     if (kind == PDX_CPD)
-        return pdx_fail(c, PDX_ERR_UNSUPPORTED, "several parameter sets are built for BBMaterial and OSBMaterial");
+        for (int32_t k = 1; k < n_sets; k++)
+            if (sets[k].model != sets[0].model || sets[k].zem_cs != sets[0].zem_cs ||
+                sets[k].maxdmg != sets[0].maxdmg)
+                return pdx_fail(c, PDX_ERR_ARG, "the parameter sets of a CMaterial share its model, zem and maxdmg");
     for (int32_t k = 0; k < n_sets; k++) {
         if (!(sets[k].delta > 0) || !(sets[k].rho > 0))
             return pdx_fail(c, PDX_ERR_ARG, "horizon and rho have to be larger than zero");
@@ -240,7 +243,7 @@ static pdx_status upload_psets(pdx_ctx *c)
     for (size_t k = 0; k < t.size(); k++) {
         const pdx_params &q = c->psets[k];
         t[k] = {q.bc, q.epsc, q.epsc - 1e-11 * fabs(q.epsc), q.epsc + 1e-11 * fabs(q.epsc), q.rho,
-                q.delta, q.G, q.K, q.lambda, q.mu};
+                q.delta, q.G, q.K, q.lambda, q.mu, (q.zem_cs * q.bc) / q.delta};
     }
     std::vector<uint8_t> pid(c->n);
     for (int64_t i = 0; i < c->n; i++) pid[c->iperm[i]] = c->pset_of[i];

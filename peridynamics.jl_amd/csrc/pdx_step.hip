@@ -1462,7 +1462,9 @@ __device__ void pk1(int model, const double *F, double lambda, double mu, double
 // Q = P K^-1 - beta F stored for pass 2; defgrad, cauchy_stress, damage and the active-bond
 // count on export steps.  Km = K's six distinct entries (00 10 20 11 21 22).
 __device__ __forceinline__ void cpd_point_state(const pdx_step_args &A, int64_t p, const double Km[6],
-                                             const double Ft[9], int na, int c, bool dead)
+                                               const double Ft[9], int na, int c, bool dead,
+                                               double lambda, double mu, double G, double K,
+                                               double beta)
 {
     const double K9[9] = {Km[0], Km[1], Km[2], Km[1], Km[3], Km[4], Km[2], Km[4], Km[5]};
     double Kinv[9], F[9];
@@ -1477,7 +1479,7 @@ __device__ __forceinline__ void cpd_point_state(const pdx_step_args &A, int64_t 
     double PK[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (!kill) {
         double P[9];
-        pk1(A.model, F, A.lambda, A.mu, A.G, A.K, P);
+        pk1(A.model, F, lambda, mu, G, K, P);
         m3_mul(P, Kinv, PK);
         if (store) {
             // cauchy_stress: stress.jl:112-116, sigma = (1/J) P F'
@@ -1492,7 +1494,7 @@ __device__ __forceinline__ void cpd_point_state(const pdx_step_args &A, int64_t 
     }
 #pragma unroll
     for (int k = 0; k < 9; k++) {
-        A.pstate[9 * p + k] = PK[k] - A.beta_zem * F[k];
+        A.pstate[9 * p + k] = PK[k] - beta * F[k];
         if (store) A.defgrad[9 * p + k] = F[k];
     }
     if (kill && !dead) A.killed[p] = 1;
@@ -1574,7 +1576,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_pass1(pdx_step_args A)
         na += __shfl_xor(na, o);
     }
     if (sub != 0) return;
-    cpd_point_state(A, p, Km, Ft, na, c, dead);
+    cpd_point_state(A, p, Km, Ft, na, c, dead, A.lambda, A.mu, A.G, A.K, A.beta_zem);
 }
 
 // CPD pass 2: c_force_density! with ZEMSilling (correspondence.jl:226-251) as a gather:
@@ -1645,13 +1647,19 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_pass2(pdx_step_args A)
 // the next tile is staged.  Pass 1 entries: x|V, X (64 B); pass 2: x|V, X|beta [j alive],
 // [j alive] Q_j (136 B).  The arithmetic is that of k_cpd_pass1/2.
 // ----------------------------------------------------------------------------------------
-template <int S>
+template <int S, int F>
 __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds1(pdx_step_args A)
 {
+    // PDX_FEAT_MULTI: several parameter sets (force_density_point! with the params of the
+    // point whose loop runs, correspondence.jl:183-191): eps_c of the owner per copy of a
+    // bond, the neighbour's copy tracked in RACT as in the OSB pass; lambda, mu, G, K, beta
+    // of the point
+    constexpr bool MULTI = (F & PDX_FEAT_MULTI) != 0;
     extern __shared__ double4 hrec[];
-    const int TL = pdx_lds_entries(A.halo_max, A.lds_budget, 64);
+    const int TL = pdx_lds_entries(A.halo_max, A.lds_budget, 64 + (MULTI ? 1 : 0));
     double4 *const hx = hrec;
     double4 *const hX = hrec + TL;
+    uint8_t *const hpid = reinterpret_cast<uint8_t *>(hrec + 2 * TL);
     const int64_t k = pdx_xcd_block();
     const int64_t p0 = k * PDX_BLOCK;
     if (p0 >= A.n) return;  // whole workgroup
@@ -1668,6 +1676,19 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds1(pdx_step_args A)
     const double4 ri = valid ? A.rec_in[p] : make_double4(0.0, 0.0, 0.0, 0.0);
     const double4 Xi = valid ? A.Xr[p] : make_double4(0.0, 0.0, 0.0, 0.0);
     const bool dead = valid && A.killed[p] != 0;
+    double epsc = A.epsc, epsc_lo = A.epsc_lo, epsc_hi = A.epsc_hi;
+    double lam = A.lambda, mu = A.mu, Gp = A.G, Kp = A.K, beta = A.beta_zem;
+    if (MULTI && valid) {
+        const pdx_pset Pi = A.psets[A.pid[p]];
+        epsc = Pi.epsc;
+        epsc_lo = Pi.epsc_lo;
+        epsc_hi = Pi.epsc_hi;
+        lam = Pi.lam;
+        mu = Pi.mu;
+        Gp = Pi.G;
+        Kp = Pi.K;
+        beta = Pi.beta;
+    }
     double Km[6] = {0, 0, 0, 0, 0, 0};  // 00 10 20 11 21 22
     double Ft[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     int na = 0;
@@ -1681,6 +1702,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds1(pdx_step_args A)
             const int32_t j = hidx[h];
             hx[h - T0] = A.rec_in[j];
             hX[h - T0] = A.Xr[j];
+            if (MULTI) hpid[h - T0] = A.pid[j];
         }
         __syncthreads();
         while (t < c) {
@@ -1697,7 +1719,22 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds1(pdx_step_args A)
             const double l = pdx_sqrt_rsq((dx[0] * dx[0] + dx[1] * dx[1]) + dx[2] * dx[2], rl);
             const double e = l - Lb;
             const double rL = pdx_rcp1(Lb);
-            const bool act = bond_fail16(wbase + (int64_t)t * 64, wd, e, Lb, e * rL, A.epsc, A.epsc_lo, A.epsc_hi);
+            bool act;
+            if (MULTI) {
+                // each end fails its copy of the bond against its own eps_c; the stretch is the
+                // same number at both ends
+                const double eps = e * rL;
+                const pdx_pset Pj = A.psets[hpid[slot - T0]];
+                uint32_t nw = wd;
+                if ((wd & PDX_W16_ACT) && (wd & PDX_W16_FP) && eps >= epsc_lo)
+                    if ((eps > epsc_hi) ? true : ((e / Lb) > epsc)) nw &= ~PDX_W16_ACT;
+                if ((wd & PDX_W16_RACT) && (wd & PDX_W16_FP) && eps >= Pj.epsc_lo)
+                    if ((eps > Pj.epsc_hi) ? true : ((e / Lb) > Pj.epsc)) nw &= ~PDX_W16_RACT;
+                if (nw != wd) wbase[(int64_t)t * 64] = (uint16_t)nw;
+                act = (nw & PDX_W16_ACT) != 0;
+            } else {
+                act = bond_fail16(wbase + (int64_t)t * 64, wd, e, Lb, e * rL, epsc, epsc_lo, epsc_hi);
+            }
             if (act && !dead) {
                 na++;
                 const double temp = (A.delta * rL) * rj.w;
@@ -1726,12 +1763,15 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds1(pdx_step_args A)
         na += __shfl_xor(na, o);
     }
     if (!valid || sub != 0) return;
-    cpd_point_state(A, p, Km, Ft, na, c, dead);
+    cpd_point_state(A, p, Km, Ft, na, c, dead, lam, mu, Gp, Kp, beta);
 }
 
-template <int S>
+template <int S, int F>
 __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2(pdx_step_args A)
 {
+    // PDX_FEAT_MULTI: beta of each end's set; the own term follows the owner's copy of the
+    // bond (ACT), the neighbour's term the neighbour's copy (RACT)
+    constexpr bool MULTI = (F & PDX_FEAT_MULTI) != 0;
     extern __shared__ double4 hrec[];
     const int TL = pdx_lds_entries(A.halo_max, A.lds_budget, 136);
     double4 *const hx = hrec;
@@ -1753,7 +1793,12 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2(pdx_step_args A)
     const double4 ri = valid ? A.rec_in[p] : make_double4(0.0, 0.0, 0.0, 0.0);
     const double4 Xi = valid ? A.Xr[p] : make_double4(0.0, 0.0, 0.0, 0.0);
     const bool dead_i = !valid || A.killed[p] != 0;
-    const double beta = A.beta_zem;
+    double beta = A.beta_zem, rho = A.rho;  // several sets: the point's own (velocity_verlet.jl:262-275)
+    if (MULTI && valid) {
+        const pdx_pset Pi = A.psets[A.pid[p]];
+        beta = Pi.beta;
+        rho = Pi.rho;
+    }
     double Qi[9];
 #pragma unroll
     for (int q = 0; q < 9; q++) Qi[q] = valid ? A.pstate[9 * p + q] : 0.0;
@@ -1768,8 +1813,9 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2(pdx_step_args A)
             const int32_t j = hidx[h];
             const bool live = !A.killed[j];
             const double4 X = A.Xr[j];
+            const double beta_j = MULTI ? A.psets[A.pid[j]].beta : beta;
             hx[h - T0] = A.rec_in[j];
-            hX[h - T0] = make_double4(X.x, X.y, X.z, live ? beta : 0.0);
+            hX[h - T0] = make_double4(X.x, X.y, X.z, live ? beta_j : 0.0);
             double *const qd = hq + 9 * (int64_t)(h - T0);
 #pragma unroll
             for (int q = 0; q < 9; q++) qd[q] = live ? A.pstate[9 * (int64_t)j + q] : 0.0;
@@ -1781,7 +1827,9 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2(pdx_step_args A)
             const int tn = t + S;
             const uint32_t wn = tn < c ? wbase[(int64_t)tn * 64] : 0u;
             const double Ln = tn < c ? Lbase[(int64_t)tn * 64] : 1.0;
-            if (wd & PDX_W16_ACT) {
+            const bool act = (wd & PDX_W16_ACT) != 0;
+            const bool ract = MULTI ? (wd & PDX_W16_RACT) != 0 : act;
+            if (act || ract) {
                 const double4 rj = hx[slot - T0];
                 const double4 Xj = hX[slot - T0];
                 const double *const qj = hq + 9 * (int64_t)(slot - T0);
@@ -1791,9 +1839,9 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2(pdx_step_args A)
 #pragma unroll
                 for (int r = 0; r < 3; r++) {
                     double tt = 0.0;
-                    if (!dead_i) tt += ((Qi[r] * DX[0] + Qi[r + 3] * DX[1]) + Qi[r + 6] * DX[2]) + beta * dx[r];
+                    if (!dead_i && act) tt += ((Qi[r] * DX[0] + Qi[r + 3] * DX[1]) + Qi[r + 6] * DX[2]) + beta * dx[r];
                     // a dead neighbour was staged with Q_j = 0 and beta = 0: its term adds 0
-                    tt += ((qj[r] * DX[0] + qj[r + 3] * DX[1]) + qj[r + 6] * DX[2]) + Xj.w * dx[r];
+                    if (ract) tt += ((qj[r] * DX[0] + qj[r + 3] * DX[1]) + qj[r + 6] * DX[2]) + Xj.w * dx[r];
                     b3[r] += wv * tt;
                 }
             }
@@ -1808,7 +1856,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2(pdx_step_args A)
         for (int r = 0; r < 3; r++) b3[r] += __shfl_xor(b3[r], o);
     if (!valid) return;
     static_assert(S == 4, "the tail takes four lanes per point");
-    point_tail4(A, p, ri, b3, sub, A.rho);
+    point_tail4(A, p, ri, b3, sub, rho);
 }
 
 // ----------------------------------------------------------------------------------------
@@ -2363,7 +2411,7 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
             const int bud = pdx_lds_budget_full(c);
             af.lds_budget = bud;
             const dim3 BL(PDX_BLOCK * PDX_LDS_LANES);
-            if (mixed) {
+            if (mixed && !c->multi()) {  // several sets: the FP64 passes
                 // pass 1 entries (D, V), (U, 0): 32 B; pass 2 (D, V), (U, beta l), l Q': 68 B
                 const size_t l1 = lds_bytes(c, 32, bud), l2b = lds_bytes(c, 68, bud);
                 PDX_TRY(pdx_timed_begin(c));
@@ -2383,11 +2431,17 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
                 break;
             }
             PDX_TRY(pdx_timed_begin(c));
-            hipLaunchKernelGGL(k_cpd_lds1<PDX_LDS_LANES>, G, BL, lds_bytes(c, 64, bud), c->stream, af);
+            if (c->multi())
+                hipLaunchKernelGGL((k_cpd_lds1<PDX_LDS_LANES, PDX_FEAT_MULTI>), G, BL, lds_bytes(c, 65, bud), c->stream, af);
+            else
+                hipLaunchKernelGGL((k_cpd_lds1<PDX_LDS_LANES, 0>), G, BL, lds_bytes(c, 64, bud), c->stream, af);
             PDX_HIP(c, hipGetLastError());
             PDX_TRY(pdx_timed_end(c, "pass1"));
             PDX_TRY(pdx_timed_begin(c));
-            hipLaunchKernelGGL(k_cpd_lds2<PDX_LDS_LANES>, G, BL, lds_bytes(c, 136, bud), c->stream, af);
+            if (c->multi())
+                hipLaunchKernelGGL((k_cpd_lds2<PDX_LDS_LANES, PDX_FEAT_MULTI>), G, BL, lds_bytes(c, 136, bud), c->stream, af);
+            else
+                hipLaunchKernelGGL((k_cpd_lds2<PDX_LDS_LANES, 0>), G, BL, lds_bytes(c, 136, bud), c->stream, af);
             PDX_HIP(c, hipGetLastError());
             PDX_TRY(pdx_timed_end(c, "step"));
             break;

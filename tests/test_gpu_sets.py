@@ -1,9 +1,10 @@
 """Bodies with several parameter sets on the HIP path, against the CPU oracle.
 
 material!(body, set_name; ...) per point set gives each point its own parameters; a bond
-takes the mean of its two ends' bc (BB) or G and K (OSB), fails against its owner's eps_c,
-and a point integrates with its own rho (bond_based.jl:163-180,
-ordinary_state_based.jl:151-174, bond_system.jl:309-326, velocity_verlet.jl:262-275).
+takes the mean of its two ends' bc (BB) or G and K (OSB), a CMaterial point the parameters
+of its own set, a bond fails against its owner's eps_c, and a point integrates with its own
+rho (bond_based.jl:163-180, ordinary_state_based.jl:151-174, correspondence.jl:183-191,
+bond_system.jl:309-326, velocity_verlet.jl:262-275).
 Bars as the single-set parity tests: FP64 force within 1e-12 of the oracle on the per-point
 rounding scale, bond states bit-exact, trajectories within 1e-9 (displacement) and 1e-8
 (velocity) normwise, mixed precision within 1e-5."""
@@ -34,7 +35,7 @@ def say(*a):
 def sets_body(kind, nx, ny, nz, dx=1e-3, jit=0.05, seed=11, fail=True, vol_jit=0.3):
     pos, vol = pd.lattice_block(nx, ny, nz, dx, jitter_amplitude=jit, seed=seed)
     vol = vol * np.random.default_rng(seed).uniform(1 - vol_jit, 1 + vol_jit, vol.shape[0])
-    b = pd.Body(pd.BBMaterial() if kind == "BB" else pd.OSBMaterial(), pos, vol)
+    b = pd.Body({"BB": pd.BBMaterial, "OSB": pd.OSBMaterial, "CPD": pd.CMaterial}[kind](), pos, vol)
     x, y = pos[0], pos[1]
     pd.point_set(b, "a", (x <= np.median(x)) & (y <= np.median(y)))
     pd.point_set(b, "b", (x > np.median(x)) & (y <= np.median(y)))
@@ -51,7 +52,10 @@ def sets_body(kind, nx, ny, nz, dx=1e-3, jit=0.05, seed=11, fail=True, vol_jit=0
 
 
 def oracle_of(oracle, b, kind):
-    sets = [as_oracle_params(p) for p in b.point_params]
+    extra = {}
+    if kind == "CPD":
+        extra = dict(model="SVK", Cs=b.mat.zem.Cs, maxdmg=b.mat.maxdmg)
+    sets = [as_oracle_params(p, **extra) for p in b.point_params]
     return oracle.OracleSim(b.xyz, b.volume, kind, sets,
                             fail_permit=b.fail_permit.astype(np.uint8),
                             point_set=b.params_map - 1)
@@ -114,7 +118,7 @@ def test_sets_two_point_closed_form(need_gpu, kind):
 
 
 @pytest.mark.parametrize("budget", [0, 12288])
-@pytest.mark.parametrize("kind", ["BB", "OSB"])
+@pytest.mark.parametrize("kind", ["BB", "OSB", "CPD"])
 def test_sets_force_fp64(need_gpu, oracle, kind, budget):
     say("sets force fp64", kind, budget)
     b = sets_body(kind, 16, 14, 6)
@@ -139,7 +143,7 @@ def test_sets_force_fp64(need_gpu, oracle, kind, budget):
     dh.close()
 
 
-@pytest.mark.parametrize("kind", ["BB", "OSB"])
+@pytest.mark.parametrize("kind", ["BB", "OSB", "CPD"])
 def test_sets_equal_match_single_set(need_gpu, kind):
     """Three sets with equal parameters give the single-set result (the means of equal
     numbers are those numbers)."""
@@ -147,7 +151,7 @@ def test_sets_equal_match_single_set(need_gpu, kind):
     res = []
     for multi in (False, True):
         pos, vol = pd.lattice_block(14, 12, 6, 1e-3, jitter_amplitude=0.05, seed=4)
-        b = pd.Body(pd.BBMaterial() if kind == "BB" else pd.OSBMaterial(), pos, vol)
+        b = pd.Body({"BB": pd.BBMaterial, "OSB": pd.OSBMaterial, "CPD": pd.CMaterial}[kind](), pos, vol)
         kw = dict(horizon=3.015e-3, rho=7850.0, E=210e9, epsilon_c=3e-4)
         if kind != "BB":
             kw["nu"] = 0.25
@@ -194,7 +198,7 @@ def test_sets_stable_timestep(need_gpu, oracle):
     dh.close()
 
 
-@pytest.mark.parametrize("kind", ["BB", "OSB"])
+@pytest.mark.parametrize("kind", ["BB", "OSB", "CPD"])
 def test_sets_vv_trajectory(need_gpu, oracle, kind):
     """Tension across the interfaces: +-1 m/s on the top and bottom rows, 60 steps, per-point
     rho in the update, bonds failing against each owner's eps_c."""
@@ -252,13 +256,18 @@ def test_sets_dynamic_relaxation(need_gpu, oracle):
     dh.close()
 
 
-def test_sets_rejected_for_cpd(need_gpu):
-    pos, vol = pd.lattice_block(6, 6, 4, 1e-3)
-    b = pd.Body(pd.CMaterial(), pos, vol)
-    x = pos[0]
-    pd.point_set(b, "l", x <= np.median(x))
-    pd.point_set(b, "r", x > np.median(x))
-    pd.material(b, "l", horizon=3.015e-3, rho=7850.0, E=210e9, nu=0.25)
-    pd.material(b, "r", horizon=3.015e-3, rho=7850.0, E=70e9, nu=0.25)
-    with pytest.raises(pd.PdxError):
-        pd.MI355XBodyDataHandler(b)
+def test_sets_cpd_mixed_context_runs_fp64(need_gpu, oracle):
+    """A mixed-precision CMaterial body with several sets runs the FP64 passes: its force
+    equals the oracle's to the FP64 bar."""
+    say("sets cpd mixed context")
+    b = sets_body("CPD", 16, 14, 6, fail=False)
+    dh = pd.MI355XBodyDataHandler(b, precision="mixed")
+    orc = oracle_of(oracle, b, "CPD")
+    x = deform(b.xyz, 2e-7)
+    dh.set("position", x)
+    dh.calc_force_density(0.0, 1e-7)
+    orc.force(x)
+    err = scaled_error(dh.get("b_int"), orc)
+    say(f"  scaled error {err:.3e}")
+    assert err <= FP64_TOL
+    dh.close()

@@ -22,12 +22,12 @@ DELTA = 1.5
 
 def two_point_params(kind, E, epsc=0.0, rho=7850.0):
     import peridynamics as pd
-    mat = pd.BBMaterial() if kind == "BB" else pd.OSBMaterial()
+    mat = {"BB": pd.BBMaterial, "OSB": pd.OSBMaterial, "CPD": pd.CMaterial}[kind]()
     kw = dict(horizon=DELTA, rho=rho, E=E)
     if kind != "BB":
         kw["nu"] = 0.25
     p = pd.materials.get_point_params(mat, kw)
-    d = as_oracle_params(p)
+    d = as_oracle_params(p, **(dict(model="SVK", Cs=0.5, maxdmg=0.85) if kind == "CPD" else {}))
     d["epsc"] = epsc
     return d
 
@@ -124,7 +124,7 @@ def test_timestep_and_relaxation_density_per_point(oracle):
                                       lam / 4 * dt * dt * 1 * k * pb["bc"]], rtol=1e-15)
 
 
-@pytest.mark.parametrize("kind", ["BB", "OSB"])
+@pytest.mark.parametrize("kind", ["BB", "OSB", "CPD"])
 def test_equal_sets_reduce_to_single_set(oracle, kind):
     import peridynamics as pd
     pos, vol = pd.lattice_block(9, 8, 5, 1e-3, jitter_amplitude=0.05, seed=5)
@@ -145,11 +145,11 @@ def test_equal_sets_reduce_to_single_set(oracle, kind):
         x = x + 3e-7 * rng.standard_normal(pos.shape)
 
 
-@pytest.mark.parametrize("kind", ["BB", "OSB"])
+@pytest.mark.parametrize("kind", ["BB", "OSB", "CPD"])
 def test_momentum_conservation_sets(oracle, kind):
-    """sum_i V_i b_i = 0 for any deformation.  OSB scatters each bond's term to both ends,
-    so this holds with failures against differing eps_c too; BB gathers each point's own
-    bonds, so a bond broken at one end only leaves an unbalanced force there (as in the
+    """sum_i V_i b_i = 0 for any deformation.  OSB and CPD scatter each bond's term to both
+    ends, so this holds with failures against differing eps_c too; BB gathers each point's
+    own bonds, so a bond broken at one end only leaves an unbalanced force there (as in the
     reference), and its sets share eps_c here."""
     import peridynamics as pd
     pos, vol = pd.lattice_block(10, 9, 6, 1e-3, jitter_amplitude=0.05, seed=7)
@@ -159,7 +159,7 @@ def test_momentum_conservation_sets(oracle, kind):
     for E, epsc in ((210e9, 3e-4), (70e9, 1e-3), (140e9, 6e-4)):
         p = two_point_params(kind, E)
         p["delta"] = 3.015e-3
-        p["epsc"] = epsc if kind == "OSB" else 6e-4
+        p["epsc"] = epsc if kind != "BB" else 6e-4
         sets.append(p)
     ps = (pos[:, 0] > np.median(pos[:, 0])).astype(int) + (pos[:, 1] > np.median(pos[:, 1]))
     orc = oracle.OracleSim(pos, vol, kind, sets, point_set=ps)
@@ -169,3 +169,40 @@ def test_momentum_conservation_sets(oracle, kind):
     total = (b * orc.vol[:, None]).sum(0)
     scale = (np.abs(b) * orc.vol[:, None]).sum(0)
     assert np.all(np.abs(total) <= 1e-12 * scale)
+
+
+def test_cpd_sets_use_the_params_of_the_point(oracle):
+    """CMaterial with several sets (force_density_point! receives the params of the point
+    whose loop runs, correspondence.jl:183-191): a point whose neighbours all share its set
+    feels exactly what a body of that one set gives it, on each side of the interface.  A
+    point-independent choice (the first set, or bond-averaged constants as BB and OSB use)
+    fails on one side."""
+    import peridynamics as pd
+    pos, vol = pd.lattice_block(14, 8, 6, 1e-3, jitter_amplitude=0.05, seed=3)
+    pos = np.ascontiguousarray(pos.T)
+    sets = []
+    for E in (210e9, 70e9):
+        p = two_point_params("CPD", E)
+        p["delta"] = 3.015e-3
+        sets.append(p)
+    ps = (pos[:, 0] > np.median(pos[:, 0])).astype(int)
+    many = oracle.OracleSim(pos, vol, "CPD", sets, fail_permit=np.zeros(len(vol)), point_set=ps)
+    x = pos + 5e-7 * np.random.default_rng(4).standard_normal(pos.shape)
+    b = many.force(x).copy()
+    owner = np.repeat(np.arange(len(vol)), np.diff(many.row_ptr))
+    mixed_family = np.zeros(len(vol), bool)
+    np.logical_or.at(mixed_family, owner, ps[owner] != ps[many.nbr])
+    checked = 0
+    for k in (0, 1):
+        one = oracle.OracleSim(pos, vol, "CPD", sets[k], fail_permit=np.zeros(len(vol)))
+        bk = one.force(x)
+        # the neighbours' neighbours matter too (their Q enters the reverse terms)
+        far = ~mixed_family & (ps == k)
+        far &= ~np.isin(np.arange(len(vol)), many.nbr[mixed_family[owner]])
+        assert far.any()
+        scale = np.max(one.b_abs, axis=1)
+        assert np.all(np.abs(b[far] - bk[far]).max(1) <= 1e-12 * scale[far])
+        checked += int(far.sum())
+    assert checked > 50
+    # and the interface points differ from both single-set bodies
+    assert not np.allclose(b[mixed_family], bk[mixed_family])

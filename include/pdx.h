@@ -132,10 +132,12 @@ pdx_status pdx_set_material(pdx_ctx *ctx, int32_t kind, const pdx_params *params
 /* Several parameter sets on one body (material!(body, set_name; ...) called per point set,
  * material_parameters.jl:64-92, kept as point_params + params_map; ParameterHandler,
  * src/core/parameter_handler.jl:36-81).  point_set[i] in [0, n_sets) is the set of caller
- * point i (0-based params_map).  Kinds PDX_BB and PDX_OSB; all sets share one horizon;
- * at most 255 sets.  A bond uses the mean of both ends' bc (BB) or G and K (OSB) and the
- * owner's eps_c (bond_based.jl:163-180, ordinary_state_based.jl:151-174).  n_sets = 1 is
- * pdx_set_material.  Call after pdx_set_points and before pdx_build_bonds. */
+ * point i (0-based params_map).  Kinds PDX_BB, PDX_OSB and PDX_CPD; all sets share one
+ * horizon (a CMaterial's sets also its model, zem_cs and maxdmg); at most 255 sets.  A bond
+ * uses the mean of both ends' bc (BB) or G and K (OSB) and the owner's eps_c
+ * (bond_based.jl:163-180, ordinary_state_based.jl:151-174); a CMaterial point its own set
+ * (correspondence.jl:183-191).  n_sets = 1 is pdx_set_material.  Call after
+ * pdx_set_points and before pdx_build_bonds. */
 pdx_status pdx_set_material_sets(pdx_ctx *ctx, int32_t kind, int32_t n_sets, const pdx_params *sets,
                                  const int32_t *point_set);
 /* Surface correction of the material's type parameter (BBMaterial{Correction},

@@ -56,6 +56,8 @@ typedef int32_t pdx_status;
 /* constitutive models of CMaterial (src/physics/constitutive_models.jl) */
 #define PDX_SVK 0 /* SaintVenantKirchhoff :33-48 */
 #define PDX_NHP 1 /* NeoHookePenalty :231-255 */
+#define PDX_LINEAR_ELASTIC 2 /* LinearElastic :81-97 (Hooke matrix in Voigt form) */
+#define PDX_NEO_HOOKE 3      /* NeoHooke :167-175 */
 
 /* fields for pdx_set_field / pdx_get_field (storage field names of the reference) */
 #define PDX_FIELD_POSITION 0      /* 3 x n */
@@ -82,7 +84,7 @@ typedef int32_t pdx_status;
  * bc = 18K/(pi delta^4) at src/physics/bond_based.jl:89). */
 typedef struct pdx_params {
     double delta, rho, E, nu, G, K, lambda, mu, Gc, epsc, bc;
-    int32_t model;  /* PDX_SVK or PDX_NHP (CPD only) */
+    int32_t model;  /* PDX_SVK, PDX_NHP, PDX_LINEAR_ELASTIC or PDX_NEO_HOOKE (CPD only) */
     int32_t kernel; /* 0 = linear_kernel delta/L (src/discretization/kernels.jl:22) */
     double zem_cs;  /* ZEMSilling Cs, src/discretization/zem_stabilization.jl:16-21 */
     double maxdmg;  /* CMaterial maxdmg, correspondence.jl:123-128 */

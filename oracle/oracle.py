@@ -41,6 +41,18 @@ def build(force=False):
         subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+# constitutive model codes of the oracle (and of the library): constitutive_models.jl
+MODELS = {"SVK": 0, "NHP": 1, "LE": 2, "NH": 3}
+
+
+def pk1(model, F, lam, mu, G, K, nu):
+    """first_piola_kirchhoff(model, F) of a 3x3 F (numpy, row-major), as a 3x3 array."""
+    Fc = np.ascontiguousarray(np.asarray(F, np.float64).T.reshape(-1))  # column-major
+    P = np.zeros(9)
+    lib().orc_pk1(MODELS[model], _p(Fc), lam, mu, G, K, nu, _p(P))
+    return P.reshape(3, 3).T
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -73,10 +85,12 @@ def _declare(L):
                                 _D, _P, _P, _P, _P, _P]
     L.orc_cpd_force.restype = None
     L.orc_cpd_force.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _INT, _D, _D,
-                                _D, _D, _D, _D, _D, _D, _D, _P, _P, _P, _P]
+                                _D, _D, _D, _D, _D, _D, _D, _D, _P, _P, _P, _P]
     L.orc_cpd_force_sets.restype = None
     L.orc_cpd_force_sets.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _INT,
-                                     _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _P, _P, _P]
+                                     _P, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _P, _P, _P]
+    L.orc_pk1.restype = None
+    L.orc_pk1.argtypes = [_INT, _P, _D, _D, _D, _D, _D, _P]
     L.orc_min_timestep.restype = _D
     L.orc_min_timestep.argtypes = [_I64, _P, _P, _P, _P, _D, _D]
     L.orc_update_vel_half.restype = None
@@ -193,7 +207,7 @@ class OracleSim:
             if kind not in ("BB", "OSB", "CPD"):
                 raise ValueError(kind)
             self.pt = {k: np.ascontiguousarray([sets[m][k] for m in ps], np.float64)
-                       for k in ("bc", "epsc", "G", "K", "rho", "lambda", "mu")}
+                       for k in ("bc", "epsc", "G", "K", "rho", "lambda", "mu", "nu")}
             params = sets[0]
         self.p = dict(params)
         self.nthreads = nthreads
@@ -283,12 +297,13 @@ class OracleSim:
                                     _p(t["bc"]), _p(t["epsc"]), _p(self.b_int), _p(self.b_abs),
                                     _p(self.scf))
             elif self.kind == "CPD":
-                model = {"SVK": 0, "NHP": 1}[p.get("model", "SVK")]
+                model = MODELS[p.get("model", "SVK")]
                 L.orc_cpd_force_sets(self.n, _p(self.X), _p(self.x), _p(self.vol),
                                      _p(self.row_ptr), _p(self.nbr), _p(self.L), _p(self.bfp),
                                      _p(self.active), _p(self.omega), _p(self.n_active),
                                      _p(self.damage), model, _p(t["lambda"]), _p(t["mu"]),
-                                     _p(t["G"]), _p(t["K"]), _p(t["bc"]), _p(t["epsc"]),
+                                     _p(t["G"]), _p(t["K"]), _p(t["nu"]), _p(t["bc"]),
+                                     _p(t["epsc"]),
                                      p["delta"], p.get("Cs", 0.5), p.get("maxdmg", 0.85),
                                      _p(self.defgrad), _p(self.cauchy), _p(self.b_int),
                                      _p(self.b_abs))
@@ -312,11 +327,12 @@ class OracleSim:
                             _p(self.damage), p["G"], p["K"], p["epsc"], _p(self.b_int),
                             _p(self.wvol), _p(self.dil), _p(self.b_abs), _p(self.scf))
         elif self.kind == "CPD":
-            model = {"SVK": 0, "NHP": 1}[p.get("model", "SVK")]
+            model = MODELS[p.get("model", "SVK")]
             L.orc_cpd_force(self.n, _p(self.X), _p(self.x), _p(self.vol), _p(self.row_ptr),
                             _p(self.nbr), _p(self.L), _p(self.bfp), _p(self.active),
                             _p(self.omega), _p(self.n_active), _p(self.damage), model,
-                            p["lambda"], p["mu"], p["G"], p["K"], p["bc"], p["delta"],
+                            p["lambda"], p["mu"], p["G"], p["K"], p.get("nu", 0.25), p["bc"],
+                            p["delta"],
                             p.get("Cs", 0.5), p.get("maxdmg", 0.85), p["epsc"],
                             _p(self.defgrad), _p(self.cauchy), _p(self.b_int),
                             _p(self.b_abs))

@@ -813,6 +813,56 @@ static void pk1_nhp(const double *F, double G, double K, double *P)
     m3_mul(F, S, P);
 }
 
+/* first_piola_kirchhoff, LinearElastic: constitutive_models.jl:83-97 (S = C_voigt E_voigt,
+ * E_voigt = (E11, E22, E33, 2 E23, 2 E31, 2 E12), get_hooke_matrix_voigt :113-124) */
+static void pk1_le(const double *F, double lambda, double mu, double nu, double *P)
+{
+    double C[9], E[9], S[9];
+    m3_tmul(F, F, C);
+    for (int k = 0; k < 9; k++) E[k] = 0.5 * (C[k] - ((k % 4 == 0) ? 1.0 : 0.0));
+    double a = ((1.0 - nu) * lambda) / nu;
+    double ev[6] = {E[0], E[4], E[8], 2.0 * E[7], 2.0 * E[2], 2.0 * E[3]};
+    double cv[36] = {a, lambda, lambda, 0, 0, 0, lambda, a, lambda, 0, 0, 0,
+                     lambda, lambda, a, 0, 0, 0, 0, 0, 0, mu, 0, 0,
+                     0, 0, 0, 0, mu, 0, 0, 0, 0, 0, 0, mu};
+    double sv[6];
+    for (int r = 0; r < 6; r++) {
+        double acc = 0.0;
+        for (int k = 0; k < 6; k++) acc += cv[r + 6 * k] * ev[k];
+        sv[r] = acc;
+    }
+    /* S = [s1 s6 s5; s6 s2 s4; s5 s4 s3] */
+    S[0] = sv[0]; S[1] = sv[5]; S[2] = sv[4];
+    S[3] = sv[5]; S[4] = sv[1]; S[5] = sv[3];
+    S[6] = sv[4]; S[7] = sv[3]; S[8] = sv[2];
+    m3_mul(F, S, P);
+}
+
+/* first_piola_kirchhoff, NeoHooke: constitutive_models.jl:169-175,
+ * S = mu (I - C^-1) + lambda log(J) C^-1 */
+static void pk1_nh(const double *F, double lambda, double mu, double *P)
+{
+    double J = m3_det(F);
+    double C[9], Ci[9], S[9];
+    m3_tmul(F, F, C);
+    m3_inv(C, Ci);
+    for (int k = 0; k < 9; k++) {
+        double Iab = (k % 4 == 0) ? 1.0 : 0.0;
+        S[k] = mu * (Iab - Ci[k]) + (lambda * log(J)) * Ci[k];
+    }
+    m3_mul(F, S, P);
+}
+
+/* the constitutive models by code: 0 SVK, 1 NeoHookePenalty, 2 LinearElastic, 3 NeoHooke */
+void orc_pk1(int model, const double *F, double lambda, double mu, double G, double K, double nu,
+             double *P)
+{
+    if (model == 1) pk1_nhp(F, G, K, P);
+    else if (model == 2) pk1_le(F, lambda, mu, nu, P);
+    else if (model == 3) pk1_nh(F, lambda, mu, P);
+    else pk1_svk(F, lambda, mu, P);
+}
+
 /* cauchy_stress: stress.jl:112-116, sigma = (1/J) P F' */
 static void cauchy(const double *P, const double *F, double *sig)
 {
@@ -841,7 +891,8 @@ static void cpd_force_core(int64_t n, const double *X, const double *x, const do
                            const uint8_t *bond_fp, uint8_t *active, const double *omega,
                            int64_t *n_active, double *damage, int model, const double *lambda_p,
                            const double *mu_p, const double *G_p, const double *K_p,
-                           const double *bc_p, const double *epsc_p, int64_t ps, double delta,
+                           const double *nu_p, const double *bc_p, const double *epsc_p,
+                           int64_t ps, double delta,
                            double Cs, double maxdmg, double *defgrad, double *cauchy_out,
                            double *b_int, double *b_abs)
 {
@@ -849,6 +900,7 @@ static void cpd_force_core(int64_t n, const double *X, const double *x, const do
     if (b_abs) for (int64_t k = 0; k < 3 * n; k++) b_abs[k] = 0.0;
     for (int64_t i = 0; i < n; i++) {
         const double lambda = lambda_p[ps * i], mu = mu_p[ps * i], G = G_p[ps * i], K = K_p[ps * i];
+        const double nu = nu_p[ps * i];
         const double epsc = epsc_p[ps * i];
         const double beta = (Cs * bc_p[ps * i]) / delta;
         int64_t na = 0;
@@ -893,8 +945,7 @@ static void cpd_force_core(int64_t n, const double *X, const double *x, const do
         }
         /* calc_first_piola_kirchhoff! */
         double P[9], PKinv[9], sig[9];
-        if (model == 1) pk1_nhp(F, G, K, P);
-        else pk1_svk(F, lambda, mu, P);
+        orc_pk1(model, F, lambda, mu, G, K, nu, P);
         m3_mul(P, Kinv, PKinv);
         cauchy(P, F, sig);
         if (cauchy_out) memcpy(cauchy_out + 9 * i, sig, sizeof sig);
@@ -938,13 +989,13 @@ void orc_cpd_force(int64_t n, const double *X, const double *x, const double *vo
                    const int64_t *row_ptr, const int32_t *nbr, const double *L,
                    const uint8_t *bond_fp, uint8_t *active, const double *omega,
                    int64_t *n_active, double *damage, int model, double lambda, double mu,
-                   double G, double K, double bc, double delta, double Cs, double maxdmg,
-                   double epsc, double *defgrad, double *cauchy_out, double *b_int,
-                   double *b_abs)
+                   double G, double K, double nu, double bc, double delta, double Cs,
+                   double maxdmg, double epsc, double *defgrad, double *cauchy_out,
+                   double *b_int, double *b_abs)
 {
     cpd_force_core(n, X, x, vol, row_ptr, nbr, L, bond_fp, active, omega, n_active, damage, model,
-                   &lambda, &mu, &G, &K, &bc, &epsc, 0, delta, Cs, maxdmg, defgrad, cauchy_out,
-                   b_int, b_abs);
+                   &lambda, &mu, &G, &K, &nu, &bc, &epsc, 0, delta, Cs, maxdmg, defgrad,
+                   cauchy_out, b_int, b_abs);
 }
 
 /* several parameter sets: lambda, mu, G, K, bc, eps_c per point (the sets share delta, the
@@ -954,13 +1005,13 @@ void orc_cpd_force_sets(int64_t n, const double *X, const double *x, const doubl
                         const uint8_t *bond_fp, uint8_t *active, const double *omega,
                         int64_t *n_active, double *damage, int model, const double *lambda_pt,
                         const double *mu_pt, const double *G_pt, const double *K_pt,
-                        const double *bc_pt, const double *epsc_pt, double delta, double Cs,
-                        double maxdmg, double *defgrad, double *cauchy_out, double *b_int,
-                        double *b_abs)
+                        const double *nu_pt, const double *bc_pt, const double *epsc_pt,
+                        double delta, double Cs, double maxdmg, double *defgrad,
+                        double *cauchy_out, double *b_int, double *b_abs)
 {
     cpd_force_core(n, X, x, vol, row_ptr, nbr, L, bond_fp, active, omega, n_active, damage, model,
-                   lambda_pt, mu_pt, G_pt, K_pt, bc_pt, epsc_pt, 1, delta, Cs, maxdmg, defgrad,
-                   cauchy_out, b_int, b_abs);
+                   lambda_pt, mu_pt, G_pt, K_pt, nu_pt, bc_pt, epsc_pt, 1, delta, Cs, maxdmg,
+                   defgrad, cauchy_out, b_int, b_abs);
 }
 
 /* ------------------------------------------------------------------------------ */

@@ -49,6 +49,7 @@ struct pdx_nk;   // Newton-Krylov workspace (pdx_nk.hip)
 struct pdx_pset {
     double bc, epsc, epsc_lo, epsc_hi, rho, delta, G, K, lam, mu;
     double beta;  // ZEMSilling beta = Cs bc / delta of the set (correspondence.jl:232)
+    double nu;    // Poisson's ratio (LinearElastic's Hooke matrix)
 };
 #define PDX_MAX_PSETS 255
 // feature mask of the LDS-staged step kernels
@@ -339,7 +340,7 @@ struct pdx_step_args {
     const double *bcv;
     unsigned long long *flags_out;
     // material
-    double bc, epsc, epsc_lo, epsc_hi, delta, G, K, lambda, mu, beta_zem, maxdmg;
+    double bc, epsc, epsc_lo, epsc_hi, delta, G, K, lambda, mu, beta_zem, maxdmg, nu;
     int model;
     double *pstate;
     uint8_t *killed;

@@ -192,7 +192,8 @@ extern "C" pdx_status pdx_set_material(pdx_ctx *c, int32_t kind, const pdx_param
         return pdx_fail(c, PDX_ERR_ARG, "unknown material kind %d", kind);
     if (!(p->delta > 0) || !(p->rho > 0))
         return pdx_fail(c, PDX_ERR_ARG, "horizon and rho have to be larger than zero");
-    if (kind == PDX_CPD && p->model != PDX_SVK && p->model != PDX_NHP)
+    if (kind == PDX_CPD && p->model != PDX_SVK && p->model != PDX_NHP && p->model != PDX_LINEAR_ELASTIC &&
+        p->model != PDX_NEO_HOOKE)
         return pdx_fail(c, PDX_ERR_ARG, "unknown constitutive model %d", p->model);
     if (p->kernel != 0) return pdx_fail(c, PDX_ERR_UNSUPPORTED, "only linear_kernel is built");
     if (c->bonds_built && c->kind != kind)
@@ -243,7 +244,7 @@ static pdx_status upload_psets(pdx_ctx *c)
     for (size_t k = 0; k < t.size(); k++) {
         const pdx_params &q = c->psets[k];
         t[k] = {q.bc, q.epsc, q.epsc - 1e-11 * fabs(q.epsc), q.epsc + 1e-11 * fabs(q.epsc), q.rho,
-                q.delta, q.G, q.K, q.lambda, q.mu, (q.zem_cs * q.bc) / q.delta};
+                q.delta, q.G, q.K, q.lambda, q.mu, (q.zem_cs * q.bc) / q.delta, q.nu};
     }
     std::vector<uint8_t> pid(c->n);
     for (int64_t i = 0; i < c->n; i++) pid[c->iperm[i]] = c->pset_of[i];

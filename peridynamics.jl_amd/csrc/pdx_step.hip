@@ -1419,8 +1419,9 @@ __device__ __forceinline__ void m3_inv(const double *A, double *R)
 
 // first_piola_kirchhoff: SaintVenantKirchhoff (constitutive_models.jl:35-41) and
 // NeoHookePenalty (:231-243)
+// first_piola_kirchhoff of the constitutive models (constitutive_models.jl): P = F S
 __device__ void pk1(int model, const double *F, double lambda, double mu, double G, double K,
-                    double *P)
+                    double nu, double *P)
 {
     double C[9], S[9];
 #pragma unroll
@@ -1446,6 +1447,28 @@ __device__ void pk1(int model, const double *F, double lambda, double mu, double
             const double Iab = (k % 4 == 0) ? 1.0 : 0.0;
             S[k] = (G * (Iab - a * Ci[k])) * Jp + vol * Ci[k];
         }
+    } else if (model == PDX_NEO_HOOKE) {
+        // :169-175: S = mu (I - C^-1) + lambda log(J) C^-1
+        const double J = m3_det(F);
+        double Ci[9];
+        m3_inv(C, Ci);
+        const double lJ = lambda * log(J);
+#pragma unroll
+        for (int k = 0; k < 9; k++) S[k] = mu * (((k % 4 == 0) ? 1.0 : 0.0) - Ci[k]) + lJ * Ci[k];
+    } else if (model == PDX_LINEAR_ELASTIC) {
+        // :83-97: S = C_voigt E_voigt, E_voigt = (E11, E22, E33, 2 E23, 2 E31, 2 E12),
+        // C_voigt with a = (1 - nu) lambda / nu on the diagonal block and mu for shear
+        double E[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) E[k] = 0.5 * (C[k] - ((k % 4 == 0) ? 1.0 : 0.0));
+        const double a = ((1.0 - nu) * lambda) / nu;
+        const double s1 = (a * E[0] + lambda * E[4]) + lambda * E[8];
+        const double s2 = (lambda * E[0] + a * E[4]) + lambda * E[8];
+        const double s3 = (lambda * E[0] + lambda * E[4]) + a * E[8];
+        const double s4 = mu * (2.0 * E[7]), s5 = mu * (2.0 * E[2]), s6 = mu * (2.0 * E[3]);
+        S[0] = s1; S[1] = s6; S[2] = s5;
+        S[3] = s6; S[4] = s2; S[5] = s4;
+        S[6] = s5; S[7] = s4; S[8] = s3;
     } else {
         double E[9];
 #pragma unroll
@@ -1464,7 +1487,7 @@ __device__ void pk1(int model, const double *F, double lambda, double mu, double
 __device__ __forceinline__ void cpd_point_state(const pdx_step_args &A, int64_t p, const double Km[6],
                                                const double Ft[9], int na, int c, bool dead,
                                                double lambda, double mu, double G, double K,
-                                               double beta)
+                                               double nu, double beta)
 {
     const double K9[9] = {Km[0], Km[1], Km[2], Km[1], Km[3], Km[4], Km[2], Km[4], Km[5]};
     double Kinv[9], F[9];
@@ -1479,7 +1502,7 @@ __device__ __forceinline__ void cpd_point_state(const pdx_step_args &A, int64_t 
     double PK[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (!kill) {
         double P[9];
-        pk1(A.model, F, lambda, mu, G, K, P);
+        pk1(A.model, F, lambda, mu, G, K, nu, P);
         m3_mul(P, Kinv, PK);
         if (store) {
             // cauchy_stress: stress.jl:112-116, sigma = (1/J) P F'
@@ -1576,7 +1599,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_pass1(pdx_step_args A)
         na += __shfl_xor(na, o);
     }
     if (sub != 0) return;
-    cpd_point_state(A, p, Km, Ft, na, c, dead, A.lambda, A.mu, A.G, A.K, A.beta_zem);
+    cpd_point_state(A, p, Km, Ft, na, c, dead, A.lambda, A.mu, A.G, A.K, A.nu, A.beta_zem);
 }
 
 // CPD pass 2: c_force_density! with ZEMSilling (correspondence.jl:226-251) as a gather:
@@ -1677,7 +1700,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds1(pdx_step_args A)
     const double4 Xi = valid ? A.Xr[p] : make_double4(0.0, 0.0, 0.0, 0.0);
     const bool dead = valid && A.killed[p] != 0;
     double epsc = A.epsc, epsc_lo = A.epsc_lo, epsc_hi = A.epsc_hi;
-    double lam = A.lambda, mu = A.mu, Gp = A.G, Kp = A.K, beta = A.beta_zem;
+    double lam = A.lambda, mu = A.mu, Gp = A.G, Kp = A.K, nup = A.nu, beta = A.beta_zem;
     if (MULTI && valid) {
         const pdx_pset Pi = A.psets[A.pid[p]];
         epsc = Pi.epsc;
@@ -1687,6 +1710,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds1(pdx_step_args A)
         mu = Pi.mu;
         Gp = Pi.G;
         Kp = Pi.K;
+        nup = Pi.nu;
         beta = Pi.beta;
     }
     double Km[6] = {0, 0, 0, 0, 0, 0};  // 00 10 20 11 21 22
@@ -1763,7 +1787,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds1(pdx_step_args A)
         na += __shfl_xor(na, o);
     }
     if (!valid || sub != 0) return;
-    cpd_point_state(A, p, Km, Ft, na, c, dead, lam, mu, Gp, Kp, beta);
+    cpd_point_state(A, p, Km, Ft, na, c, dead, lam, mu, Gp, Kp, nup, beta);
 }
 
 template <int S, int F>
@@ -1991,7 +2015,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_pass1_mx(pdx_step_args A)
     double PK[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (!kill) {
         double P[9];
-        pk1(A.model, Fm, A.lambda, A.mu, A.G, A.K, P);
+        pk1(A.model, Fm, A.lambda, A.mu, A.G, A.K, A.nu, P);
         m3_mul(P, Kinv, PK);
         if (store) {
             const double sj = 1.0 / m3_det(Fm);
@@ -2195,6 +2219,7 @@ pdx_status pdx_fill_step_args(pdx_ctx *c, pdx_step_args *a, int64_t step, int fl
     a->K = c->P.K;
     a->lambda = c->P.lambda;
     a->mu = c->P.mu;
+    a->nu = c->P.nu;
     a->beta_zem = (c->P.zem_cs * c->P.bc) / c->P.delta;  // correspondence.jl:232
     a->maxdmg = c->P.maxdmg;
     a->model = c->P.model;

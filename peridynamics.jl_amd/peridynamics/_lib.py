@@ -24,7 +24,7 @@ PDX_FP64 = 0
 PDX_MIXED = 1
 
 PDX_BB, PDX_OSB, PDX_CPD = 0, 1, 2
-PDX_SVK, PDX_NHP = 0, 1
+PDX_SVK, PDX_NHP, PDX_LINEAR_ELASTIC, PDX_NEO_HOOKE = 0, 1, 2, 3
 
 FIELDS = {
     "position": (0, 3),

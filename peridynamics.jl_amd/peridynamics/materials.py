@@ -5,7 +5,8 @@ src/physics/bond_based.jl:74-113, ordinary_state_based.jl:82-96, correspondence.
 import math
 from dataclasses import dataclass, field
 
-from ._lib import ArgumentError, PDX_BB, PDX_CPD, PDX_NHP, PDX_OSB, PDX_SVK
+from ._lib import (ArgumentError, PDX_BB, PDX_CPD, PDX_LINEAR_ELASTIC, PDX_NEO_HOOKE, PDX_NHP,
+                   PDX_OSB, PDX_SVK)
 
 
 def _arg_error(msg):
@@ -25,6 +26,22 @@ class NeoHookePenalty:
 
     def __repr__(self):
         return "NeoHookePenalty()"
+
+
+class LinearElastic:
+    """constitutive_models.jl:81-97: S from the Hooke matrix in Voigt form."""
+    code = PDX_LINEAR_ELASTIC
+
+    def __repr__(self):
+        return "LinearElastic()"
+
+
+class NeoHooke:
+    """constitutive_models.jl:167-175: S = mu (I - C^-1) + lambda log(J) C^-1."""
+    code = PDX_NEO_HOOKE
+
+    def __repr__(self):
+        return "NeoHooke()"
 
 
 @dataclass
@@ -103,8 +120,10 @@ class CMaterial:
             raise _arg_error("only linear_kernel is provided by the MI355X build")
         self.kernel = kernel
         self.constitutive_model = model or SaintVenantKirchhoff()
-        if not isinstance(self.constitutive_model, (SaintVenantKirchhoff, NeoHookePenalty)):
-            raise _arg_error("only SaintVenantKirchhoff and NeoHookePenalty are provided")
+        if not isinstance(self.constitutive_model,
+                          (SaintVenantKirchhoff, NeoHookePenalty, LinearElastic, NeoHooke)):
+            raise _arg_error("the constitutive models are SaintVenantKirchhoff, LinearElastic, "
+                             "NeoHooke and NeoHookePenalty")
         self.zem = zem or ZEMSilling()
         if not isinstance(self.zem, ZEMSilling):
             raise _arg_error("only ZEMSilling is provided by the MI355X build")

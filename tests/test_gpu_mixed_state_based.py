@@ -101,10 +101,11 @@ def test_mixed_osb_torsion_trajectory(need_gpu, oracle):
     dh.close()
 
 
-CPD_MODELS = {"SVK": pd.SaintVenantKirchhoff, "NHP": pd.NeoHookePenalty}
+CPD_MODELS = {"SVK": pd.SaintVenantKirchhoff, "NHP": pd.NeoHookePenalty,
+              "LE": pd.LinearElastic, "NH": pd.NeoHooke}
 
 
-@pytest.mark.parametrize("model", ["SVK", "NHP"])
+@pytest.mark.parametrize("model", ["SVK", "NHP", "LE", "NH"])
 @pytest.mark.parametrize("budget", [0, 12288])
 @pytest.mark.parametrize("amp", [2e-7, 3e-6])
 def test_mixed_cpd_force(need_gpu, oracle, amp, budget, model):
@@ -173,7 +174,7 @@ F_AFFINE = np.array([[1.1, 0.01, 0.0], [0.0, 1.2, 0.02], [0.03, 0.0, 1.3]])
 
 
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-6)])
-@pytest.mark.parametrize("model", ["SVK", "NHP"])
+@pytest.mark.parametrize("model", ["SVK", "NHP", "LE", "NH"])
 def test_cpd_defgrad_exact_for_affine_deformation(need_gpu, model, precision, tol):
     """test/materials/test_uniform_deformation.jl:5-41 on the device: under x = F X every
     point's deformation gradient, surface points included, equals F (mixed precision forms
@@ -192,4 +193,23 @@ def test_cpd_defgrad_exact_for_affine_deformation(need_gpu, model, precision, to
     err = np.max(np.abs(dh.get("defgrad") - F_AFFINE.T.reshape(-1)))
     say(f"  max |F - F_a| {err:.2e}")
     assert err < tol
+    dh.close()
+
+
+@pytest.mark.parametrize("model", ["LE", "NH"])
+def test_cpd_models_force_fp64(need_gpu, oracle, model):
+    """LinearElastic and NeoHooke (constitutive_models.jl:81-97, 167-175) in the FP64 CPD
+    step against the oracle, to the FP64 bar."""
+    say("cpd model fp64", model)
+    b = body_block("CPD", 24, 20, 8, jit=0.05, E=210e9, rho=7850.0, Gc=None,
+                   model=CPD_MODELS[model]())
+    dh = pd.MI355XBodyDataHandler(b)
+    orc = oracle_for(oracle, b, "CPD")
+    x = b.xyz + displacement_field(b.xyz, 3e-6)
+    dh.set("position", x)
+    dh.calc_force_density(0.0, 1e-7)
+    orc.force(x)
+    err = scaled_error(dh.get("b_int"), orc)
+    say(f"  scaled error {err:.3e}")
+    assert err <= 1e-12
     dh.close()

@@ -47,8 +47,10 @@ def oracle_for(oracle, body, kind, dh=None):
     p = body.point_params[0]
     extra = {}
     if kind == "CPD":
-        extra = dict(model="NHP" if isinstance(body.mat.constitutive_model, pd.NeoHookePenalty)
-                     else "SVK", Cs=body.mat.zem.Cs, maxdmg=body.mat.maxdmg)
+        names = {pd.NeoHookePenalty: "NHP", pd.LinearElastic: "LE", pd.NeoHooke: "NH",
+                 pd.SaintVenantKirchhoff: "SVK"}
+        extra = dict(model=names[type(body.mat.constitutive_model)], Cs=body.mat.zem.Cs,
+                     maxdmg=body.mat.maxdmg)
     cracks = [(body.point_sets[a], body.point_sets[b], f) for a, b, f in body.point_sets_precracks]
     return oracle.OracleSim(body.xyz, body.volume, kind, as_oracle_params(p, **extra),
                             fail_permit=body.fail_permit.astype(np.uint8), precracks=cracks)

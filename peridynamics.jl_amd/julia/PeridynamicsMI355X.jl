@@ -95,6 +95,8 @@ material_kind(m) = throw(ArgumentError("$(typeof(m)) has no MI355X kernel"))
 
 model_id(::CMaterial{<:SaintVenantKirchhoff}) = Int32(0)
 model_id(::CMaterial{<:NeoHookePenalty}) = Int32(1)
+model_id(::CMaterial{<:LinearElastic}) = Int32(2)
+model_id(::CMaterial{<:NeoHooke}) = Int32(3)
 model_id(_) = Int32(0)
 
 function pdx_params(body, p)

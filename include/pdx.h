@@ -59,6 +59,10 @@ typedef int32_t pdx_status;
 #define PDX_LINEAR_ELASTIC 2 /* LinearElastic :81-97 (Hooke matrix in Voigt form) */
 #define PDX_NEO_HOOKE 3      /* NeoHooke :167-175 */
 
+/* zero-energy-mode stabilisation of CMaterial (src/discretization/zem_stabilization.jl) */
+#define PDX_ZEM_SILLING 0 /* ZEMSilling(Cs) :16-21, tzem = w beta z, beta = Cs bc / delta */
+#define PDX_ZEM_WAN 1     /* ZEMWan() :30-42, tzem = w C_1 z, C_1 = C : K^-1 (correspondence.jl:253-285) */
+
 /* fields for pdx_set_field / pdx_get_field (storage field names of the reference) */
 #define PDX_FIELD_POSITION 0      /* 3 x n */
 #define PDX_FIELD_DISPLACEMENT 1  /* 3 x n */
@@ -88,6 +92,8 @@ typedef struct pdx_params {
     int32_t kernel; /* 0 = linear_kernel delta/L (src/discretization/kernels.jl:22) */
     double zem_cs;  /* ZEMSilling Cs, src/discretization/zem_stabilization.jl:16-21 */
     double maxdmg;  /* CMaterial maxdmg, correspondence.jl:123-128 */
+    int32_t zem;    /* PDX_ZEM_SILLING or PDX_ZEM_WAN (CPD only) */
+    int32_t reserved_;
 } pdx_params;
 
 /* Options and statistics of one Newton-Krylov load step (NewtonKrylov,

@@ -43,6 +43,7 @@ def build(force=False):
 
 # constitutive model codes of the oracle (and of the library): constitutive_models.jl
 MODELS = {"SVK": 0, "NHP": 1, "LE": 2, "NH": 3}
+ZEMS = {"Silling": 0, "Wan": 1}  # zero-energy-mode stabilisations of CMaterial
 
 
 def pk1(model, F, lam, mu, G, K, nu):
@@ -51,6 +52,14 @@ def pk1(model, F, lam, mu, G, K, nu):
     P = np.zeros(9)
     lib().orc_pk1(MODELS[model], _p(Fc), lam, mu, G, K, nu, _p(P))
     return P.reshape(3, 3).T
+
+
+def zem_c1(Kinv, lam, mu, nu):
+    """calc_zem_stiffness_tensor(C, Kinv) of the Hooke tensor (nu, lambda, mu), 3x3."""
+    Kc = np.ascontiguousarray(np.asarray(Kinv, np.float64).T.reshape(-1))
+    C1 = np.zeros(9)
+    lib().orc_zem_c1(_p(Kc), lam, mu, nu, _p(C1))
+    return C1.reshape(3, 3).T
 
 
 def lib():
@@ -89,6 +98,8 @@ def _declare(L):
     L.orc_cpd_force_sets.restype = None
     L.orc_cpd_force_sets.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _INT,
                                      _P, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _P, _P, _P]
+    L.orc_zem_c1.restype = None
+    L.orc_zem_c1.argtypes = [_P, _D, _D, _D, _P]
     L.orc_pk1.restype = None
     L.orc_pk1.argtypes = [_INT, _P, _D, _D, _D, _D, _D, _P]
     L.orc_min_timestep.restype = _D
@@ -297,7 +308,7 @@ class OracleSim:
                                     _p(t["bc"]), _p(t["epsc"]), _p(self.b_int), _p(self.b_abs),
                                     _p(self.scf))
             elif self.kind == "CPD":
-                model = MODELS[p.get("model", "SVK")]
+                model = MODELS[p.get("model", "SVK")] | (ZEMS[p.get("zem", "Silling")] << 8)
                 L.orc_cpd_force_sets(self.n, _p(self.X), _p(self.x), _p(self.vol),
                                      _p(self.row_ptr), _p(self.nbr), _p(self.L), _p(self.bfp),
                                      _p(self.active), _p(self.omega), _p(self.n_active),
@@ -327,7 +338,7 @@ class OracleSim:
                             _p(self.damage), p["G"], p["K"], p["epsc"], _p(self.b_int),
                             _p(self.wvol), _p(self.dil), _p(self.b_abs), _p(self.scf))
         elif self.kind == "CPD":
-            model = MODELS[p.get("model", "SVK")]
+            model = MODELS[p.get("model", "SVK")] | (ZEMS[p.get("zem", "Silling")] << 8)
             L.orc_cpd_force(self.n, _p(self.X), _p(self.x), _p(self.vol), _p(self.row_ptr),
                             _p(self.nbr), _p(self.L), _p(self.bfp), _p(self.active),
                             _p(self.omega), _p(self.n_active), _p(self.damage), model,

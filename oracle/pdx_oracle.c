@@ -881,8 +881,29 @@ static void cauchy(const double *P, const double *F, double *sig)
 /* too_much_damage!             correspondence.jl:288-298                          */
 /* calc_first_piola_kirchhoff!  correspondence.jl:216-224                          */
 /* c_force_density! ZEMSilling  correspondence.jl:226-251                          */
-/* model: 0 SaintVenantKirchhoff, 1 NeoHookePenalty                                */
+/* c_force_density! ZEMWan     correspondence.jl:253-285                          */
+/* model & 0xff: 0 SaintVenantKirchhoff, 1 NeoHookePenalty, 2 LinearElastic,      */
+/* 3 NeoHooke; model >> 8: 0 ZEMSilling, 1 ZEMWan                                 */
 /* ------------------------------------------------------------------------------ */
+/* calc_zem_stiffness_tensor (zem_stabilization.jl:32-42): C_1[i,j] = sum_kl C[i,j,k,l]
+ * Kinv[k,l] of the Hooke tensor get_hooke_matrix (constitutive_models.jl:107-133): Voigt
+ * a = (1 - nu) lambda / nu on the normal diagonal, lambda off it, mu for the three shears,
+ * each shear entry at the four index orders of its pair.  Column-major 3x3. */
+void orc_zem_c1(const double *Kinv, double lambda, double mu, double nu, double *C1)
+{
+    const double a = ((1.0 - nu) * lambda) / nu;
+    for (int c = 0; c < 3; c++)
+        for (int r = 0; r < 3; r++) {
+            if (r == c) {
+                double s = 0.0;
+                for (int k = 0; k < 3; k++) s += (k == r ? a : lambda) * Kinv[k + 3 * k];
+                C1[r + 3 * c] = s;
+            } else {
+                C1[r + 3 * c] = mu * Kinv[r + 3 * c] + mu * Kinv[c + 3 * r];
+            }
+        }
+}
+
 /* Parameters per point (`ps` = 1: arrays of n, one value per point's parameter set, the
  * params of the point whose loop runs, as force_density_point! receives them with a
  * ParameterHandler) or one value for all (`ps` = 0). */
@@ -896,6 +917,8 @@ static void cpd_force_core(int64_t n, const double *X, const double *x, const do
                            double Cs, double maxdmg, double *defgrad, double *cauchy_out,
                            double *b_int, double *b_abs)
 {
+    const int wan = (model >> 8) == 1;
+    model &= 0xff;
     for (int64_t k = 0; k < 3 * n; k++) b_int[k] = 0.0;
     if (b_abs) for (int64_t k = 0; k < 3 * n; k++) b_abs[k] = 0.0;
     for (int64_t i = 0; i < n; i++) {
@@ -949,7 +972,9 @@ static void cpd_force_core(int64_t n, const double *X, const double *x, const do
         m3_mul(P, Kinv, PKinv);
         cauchy(P, F, sig);
         if (cauchy_out) memcpy(cauchy_out + 9 * i, sig, sizeof sig);
-        /* c_force_density! (ZEMSilling) */
+        double C1[9] = {0};
+        if (wan) orc_zem_c1(Kinv, lambda, mu, nu, C1);
+        /* c_force_density! (ZEMSilling: tzem = w beta z; ZEMWan: tzem = (w C_1) z) */
         for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
             int64_t j = nbr[b];
             double DX[3], dxv[3];
@@ -964,14 +989,24 @@ static void cpd_force_core(int64_t n, const double *X, const double *x, const do
             double wb = w * beta;
             double tabs[3];
             for (int r = 0; r < 3; r++) {
-                double tz = wb * z[r];
+                double tz, zabs;
+                if (wan) {
+                    tz = ((w * C1[r]) * z[0] + (w * C1[r + 3]) * z[1]) + (w * C1[r + 6]) * z[2];
+                    zabs = 0.0;
+                    for (int c = 0; c < 3; c++)
+                        zabs += fabs(w * C1[r + 3 * c]) *
+                                (fabs(dxv[c]) + fabs(F[c] * DX[0]) + fabs(F[c + 3] * DX[1]) +
+                                 fabs(F[c + 6] * DX[2]));
+                } else {
+                    tz = wb * z[r];
+                    zabs = fabs(wb) * (fabs(dxv[r]) + fabs(F[r] * DX[0]) + fabs(F[r + 3] * DX[1]) +
+                                       fabs(F[r + 6] * DX[2]));
+                }
                 double m0 = w * PKinv[r], m1 = w * PKinv[r + 3], m2 = w * PKinv[r + 6];
                 t[r] = ((m0 * DX[0] + m1 * DX[1]) + m2 * DX[2]) + tz;
                 /* rounding scale of t: its parts before the cancellations inside
                  * z = dx - F dX and P K^-1 dX */
-                tabs[r] = (fabs(m0 * DX[0]) + fabs(m1 * DX[1]) + fabs(m2 * DX[2])) +
-                          fabs(wb) * (fabs(dxv[r]) + fabs(F[r] * DX[0]) + fabs(F[r + 3] * DX[1]) +
-                                      fabs(F[r + 6] * DX[2]));
+                tabs[r] = (fabs(m0 * DX[0]) + fabs(m1 * DX[1]) + fabs(m2 * DX[2])) + zabs;
             }
             for (int r = 0; r < 3; r++) {
                 b_int[3 * i + r] += t[r] * vol[j];

@@ -54,6 +54,7 @@ struct pdx_pset {
 #define PDX_MAX_PSETS 255
 // feature mask of the LDS-staged step kernels
 #define PDX_FEAT_MULTI 1  // several parameter sets
+#define PDX_FEAT_ZW 8     // CPD with ZEMWan (stabilisation stiffness per point)
 #define PDX_FEAT_SC 2     // EnergySurfaceCorrection scfactor per bond
 #define PDX_FEAT_CAP 4    // neighbour tables longer than the LDS budget: the rest from global memory
 struct pdx_gnn;  // learned preconditioner (pdx_gnn.hip)

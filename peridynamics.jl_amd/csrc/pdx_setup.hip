@@ -192,6 +192,8 @@ extern "C" pdx_status pdx_set_material(pdx_ctx *c, int32_t kind, const pdx_param
         return pdx_fail(c, PDX_ERR_ARG, "unknown material kind %d", kind);
     if (!(p->delta > 0) || !(p->rho > 0))
         return pdx_fail(c, PDX_ERR_ARG, "horizon and rho have to be larger than zero");
+    if (kind == PDX_CPD && p->zem != PDX_ZEM_SILLING && p->zem != PDX_ZEM_WAN)
+        return pdx_fail(c, PDX_ERR_ARG, "unknown zero-energy-mode stabilisation %d", p->zem);
     if (kind == PDX_CPD && p->model != PDX_SVK && p->model != PDX_NHP && p->model != PDX_LINEAR_ELASTIC &&
         p->model != PDX_NEO_HOOKE)
         return pdx_fail(c, PDX_ERR_ARG, "unknown constitutive model %d", p->model);
@@ -217,7 +219,7 @@ extern "C" pdx_status pdx_set_material_sets(pdx_ctx *c, int32_t kind, int32_t n_
     if (kind == PDX_CPD)
         for (int32_t k = 1; k < n_sets; k++)
             if (sets[k].model != sets[0].model || sets[k].zem_cs != sets[0].zem_cs ||
-                sets[k].maxdmg != sets[0].maxdmg)
+                sets[k].maxdmg != sets[0].maxdmg || sets[k].zem != sets[0].zem)
                 return pdx_fail(c, PDX_ERR_ARG, "the parameter sets of a CMaterial share its model, zem and maxdmg");
     for (int32_t k = 0; k < n_sets; k++) {
         if (!(sets[k].delta > 0) || !(sets[k].rho > 0))
@@ -855,7 +857,8 @@ extern "C" pdx_status pdx_build_bonds(pdx_ctx *c, int64_t *n_bonds_out)
     // OSB: the per-point pair of pass 1, then (LDS layout, one set) the own force part
     if (c->kind == PDX_OSB) TRY_C(pdx_alloc(c, &c->pstate, 5 * n));
     if (c->kind == PDX_CPD) {
-        TRY_C(pdx_alloc(c, &c->pstate, 9 * n));
+        // Q per point; with ZEMWan also the stabilisation stiffness C_1 (18 numbers)
+        TRY_C(pdx_alloc(c, &c->pstate, (c->P.zem == PDX_ZEM_WAN ? 18 : 9) * n));
         TRY_C(pdx_alloc(c, &c->defgrad, 9 * n));
         TRY_C(pdx_alloc(c, &c->cauchy, 9 * n));
         HIP_C(hipMemsetAsync(c->defgrad, 0, sizeof(double) * 9 * n, c->stream));
@@ -878,6 +881,9 @@ extern "C" pdx_status pdx_build_bonds(pdx_ctx *c, int64_t *n_bonds_out)
         PDX_HIP(c, hipGetLastError());
     }
     PDX_TRY(upload_psets(c));
+    if (c->kind == PDX_CPD && c->P.zem == PDX_ZEM_WAN && !c->lds_mode)
+        return pdx_fail(c, PDX_ERR_UNSUPPORTED,
+                        "ZEMWan needs the LDS-staged bond layout (a neighbour table did not fit)");
     if (c->multi() && !c->lds_mode)
         return pdx_fail(c, PDX_ERR_UNSUPPORTED,
                         "several parameter sets need the LDS-staged bond layout (a neighbour table did not fit)");

@@ -1484,6 +1484,7 @@ __device__ void pk1(int model, const double *F, double lambda, double mu, double
 // The per-point part of CPD pass 1 (FP64): F = Ft K^-1, too_much_damage!, P, and
 // Q = P K^-1 - beta F stored for pass 2; defgrad, cauchy_stress, damage and the active-bond
 // count on export steps.  Km = K's six distinct entries (00 10 20 11 21 22).
+template <bool ZW = false>
 __device__ __forceinline__ void cpd_point_state(const pdx_step_args &A, int64_t p, const double Km[6],
                                                const double Ft[9], int na, int c, bool dead,
                                                double lambda, double mu, double G, double K,
@@ -1515,10 +1516,37 @@ __device__ __forceinline__ void cpd_point_state(const pdx_step_args &A, int64_t 
             }
         }
     }
+    if (ZW) {
+        // ZEMWan (correspondence.jl:253-285): tzem = w C_1 z with C_1 = C : K^-1 of the
+        // point's Hooke tensor (zem_stabilization.jl:32-42; get_hooke_matrix, Voigt diagonal
+        // a = (1 - nu) lambda / nu, lambda off it, mu for shear), so Q = P K^-1 - C_1 F and
+        // pass 2 also needs C_1
+        const double a = ((1.0 - nu) * lambda) / nu;
+        double C1[9], CF[9];
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
-        A.pstate[9 * p + k] = PK[k] - beta * F[k];
-        if (store) A.defgrad[9 * p + k] = F[k];
+        for (int cc = 0; cc < 3; cc++)
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                if (r == cc) {
+                    const double d0 = r == 0 ? a : lambda, d1 = r == 1 ? a : lambda, d2 = r == 2 ? a : lambda;
+                    C1[r + 3 * cc] = (d0 * Kinv[0] + d1 * Kinv[4]) + d2 * Kinv[8];
+                } else {
+                    C1[r + 3 * cc] = mu * Kinv[r + 3 * cc] + mu * Kinv[cc + 3 * r];
+                }
+            }
+        m3_mul(C1, F, CF);
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            A.pstate[18 * p + k] = PK[k] - CF[k];
+            A.pstate[18 * p + 9 + k] = C1[k];
+            if (store) A.defgrad[9 * p + k] = F[k];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            A.pstate[9 * p + k] = PK[k] - beta * F[k];
+            if (store) A.defgrad[9 * p + k] = F[k];
+        }
     }
     if (kill && !dead) A.killed[p] = 1;
     if (store) {
@@ -1787,7 +1815,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds1(pdx_step_args A)
         na += __shfl_xor(na, o);
     }
     if (!valid || sub != 0) return;
-    cpd_point_state(A, p, Km, Ft, na, c, dead, lam, mu, Gp, Kp, nup, beta);
+    cpd_point_state<(F & PDX_FEAT_ZW) != 0>(A, p, Km, Ft, na, c, dead, lam, mu, Gp, Kp, nup, beta);
 }
 
 template <int S, int F>
@@ -1796,8 +1824,12 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2(pdx_step_args A)
     // PDX_FEAT_MULTI: beta of each end's set; the own term follows the owner's copy of the
     // bond (ACT), the neighbour's term the neighbour's copy (RACT)
     constexpr bool MULTI = (F & PDX_FEAT_MULTI) != 0;
+    // PDX_FEAT_ZW: ZEMWan, the stabilisation C_1 dx of each end instead of beta dx; entries
+    // also carry [j alive] C_1 of the neighbour (208 B)
+    constexpr bool ZW = (F & PDX_FEAT_ZW) != 0;
+    constexpr int NQ = ZW ? 18 : 9;
     extern __shared__ double4 hrec[];
-    const int TL = pdx_lds_entries(A.halo_max, A.lds_budget, 136);
+    const int TL = pdx_lds_entries(A.halo_max, A.lds_budget, 64 + 8 * NQ);
     double4 *const hx = hrec;
     double4 *const hX = hrec + TL;
     double *const hq = reinterpret_cast<double *>(hrec + 2 * TL);
@@ -1823,9 +1855,9 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2(pdx_step_args A)
         beta = Pi.beta;
         rho = Pi.rho;
     }
-    double Qi[9];
+    double Qi[NQ];
 #pragma unroll
-    for (int q = 0; q < 9; q++) Qi[q] = valid ? A.pstate[9 * p + q] : 0.0;
+    for (int q = 0; q < NQ; q++) Qi[q] = valid ? A.pstate[NQ * p + q] : 0.0;
     double b3[3] = {0.0, 0.0, 0.0};
     int t = sub;
     uint32_t wd = t < c ? wbase[(int64_t)t * 64] : 0u;
@@ -1840,9 +1872,9 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2(pdx_step_args A)
             const double beta_j = MULTI ? A.psets[A.pid[j]].beta : beta;
             hx[h - T0] = A.rec_in[j];
             hX[h - T0] = make_double4(X.x, X.y, X.z, live ? beta_j : 0.0);
-            double *const qd = hq + 9 * (int64_t)(h - T0);
+            double *const qd = hq + NQ * (int64_t)(h - T0);
 #pragma unroll
-            for (int q = 0; q < 9; q++) qd[q] = live ? A.pstate[9 * (int64_t)j + q] : 0.0;
+            for (int q = 0; q < NQ; q++) qd[q] = live ? A.pstate[NQ * (int64_t)j + q] : 0.0;
         }
         __syncthreads();
         while (t < c) {
@@ -1856,16 +1888,26 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2(pdx_step_args A)
             if (act || ract) {
                 const double4 rj = hx[slot - T0];
                 const double4 Xj = hX[slot - T0];
-                const double *const qj = hq + 9 * (int64_t)(slot - T0);
+                const double *const qj = hq + NQ * (int64_t)(slot - T0);
                 const double dx[3] = {rj.x - ri.x, rj.y - ri.y, rj.z - ri.z};
                 const double DX[3] = {Xj.x - Xi.x, Xj.y - Xi.y, Xj.z - Xi.z};
                 const double wv = (A.delta * pdx_rcp1(Lb)) * rj.w;  // w V_j
 #pragma unroll
                 for (int r = 0; r < 3; r++) {
                     double tt = 0.0;
-                    if (!dead_i && act) tt += ((Qi[r] * DX[0] + Qi[r + 3] * DX[1]) + Qi[r + 6] * DX[2]) + beta * dx[r];
-                    // a dead neighbour was staged with Q_j = 0 and beta = 0: its term adds 0
-                    if (ract) tt += ((qj[r] * DX[0] + qj[r + 3] * DX[1]) + qj[r + 6] * DX[2]) + Xj.w * dx[r];
+                    if (ZW) {
+                        // Q DX + C_1 dx at each end (a dead neighbour was staged as zeros)
+                        if (!dead_i && act)
+                            tt += ((Qi[r] * DX[0] + Qi[r + 3] * DX[1]) + Qi[r + 6] * DX[2]) +
+                                  ((Qi[9 + r] * dx[0] + Qi[12 + r] * dx[1]) + Qi[15 + r] * dx[2]);
+                        if (ract)
+                            tt += ((qj[r] * DX[0] + qj[r + 3] * DX[1]) + qj[r + 6] * DX[2]) +
+                                  ((qj[9 + r] * dx[0] + qj[12 + r] * dx[1]) + qj[15 + r] * dx[2]);
+                    } else {
+                        if (!dead_i && act) tt += ((Qi[r] * DX[0] + Qi[r + 3] * DX[1]) + Qi[r + 6] * DX[2]) + beta * dx[r];
+                        // a dead neighbour was staged with Q_j = 0 and beta = 0: its term adds 0
+                        if (ract) tt += ((qj[r] * DX[0] + qj[r + 3] * DX[1]) + qj[r + 6] * DX[2]) + Xj.w * dx[r];
+                    }
                     b3[r] += wv * tt;
                 }
             }
@@ -2436,7 +2478,7 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
             const int bud = pdx_lds_budget_full(c);
             af.lds_budget = bud;
             const dim3 BL(PDX_BLOCK * PDX_LDS_LANES);
-            if (mixed && !c->multi()) {  // several sets: the FP64 passes
+            if (mixed && !c->multi() && c->P.zem != PDX_ZEM_WAN) {  // otherwise the FP64 passes
                 // pass 1 entries (D, V), (U, 0): 32 B; pass 2 (D, V), (U, beta l), l Q': 68 B
                 const size_t l1 = lds_bytes(c, 32, bud), l2b = lds_bytes(c, 68, bud);
                 PDX_TRY(pdx_timed_begin(c));
@@ -2455,18 +2497,24 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
                 PDX_TRY(pdx_timed_end(c, "step"));
                 break;
             }
+            const int fc = (c->multi() ? PDX_FEAT_MULTI : 0) | (c->P.zem == PDX_ZEM_WAN ? PDX_FEAT_ZW : 0);
+            const int e1 = 64 + (c->multi() ? 1 : 0), e2 = c->P.zem == PDX_ZEM_WAN ? 208 : 136;
             PDX_TRY(pdx_timed_begin(c));
-            if (c->multi())
-                hipLaunchKernelGGL((k_cpd_lds1<PDX_LDS_LANES, PDX_FEAT_MULTI>), G, BL, lds_bytes(c, 65, bud), c->stream, af);
-            else
-                hipLaunchKernelGGL((k_cpd_lds1<PDX_LDS_LANES, 0>), G, BL, lds_bytes(c, 64, bud), c->stream, af);
+            switch (fc) {
+            case 0: hipLaunchKernelGGL((k_cpd_lds1<PDX_LDS_LANES, 0>), G, BL, lds_bytes(c, e1, bud), c->stream, af); break;
+            case PDX_FEAT_MULTI: hipLaunchKernelGGL((k_cpd_lds1<PDX_LDS_LANES, PDX_FEAT_MULTI>), G, BL, lds_bytes(c, e1, bud), c->stream, af); break;
+            case PDX_FEAT_ZW: hipLaunchKernelGGL((k_cpd_lds1<PDX_LDS_LANES, PDX_FEAT_ZW>), G, BL, lds_bytes(c, e1, bud), c->stream, af); break;
+            default: hipLaunchKernelGGL((k_cpd_lds1<PDX_LDS_LANES, PDX_FEAT_MULTI | PDX_FEAT_ZW>), G, BL, lds_bytes(c, e1, bud), c->stream, af); break;
+            }
             PDX_HIP(c, hipGetLastError());
             PDX_TRY(pdx_timed_end(c, "pass1"));
             PDX_TRY(pdx_timed_begin(c));
-            if (c->multi())
-                hipLaunchKernelGGL((k_cpd_lds2<PDX_LDS_LANES, PDX_FEAT_MULTI>), G, BL, lds_bytes(c, 136, bud), c->stream, af);
-            else
-                hipLaunchKernelGGL((k_cpd_lds2<PDX_LDS_LANES, 0>), G, BL, lds_bytes(c, 136, bud), c->stream, af);
+            switch (fc) {
+            case 0: hipLaunchKernelGGL((k_cpd_lds2<PDX_LDS_LANES, 0>), G, BL, lds_bytes(c, e2, bud), c->stream, af); break;
+            case PDX_FEAT_MULTI: hipLaunchKernelGGL((k_cpd_lds2<PDX_LDS_LANES, PDX_FEAT_MULTI>), G, BL, lds_bytes(c, e2, bud), c->stream, af); break;
+            case PDX_FEAT_ZW: hipLaunchKernelGGL((k_cpd_lds2<PDX_LDS_LANES, PDX_FEAT_ZW>), G, BL, lds_bytes(c, e2, bud), c->stream, af); break;
+            default: hipLaunchKernelGGL((k_cpd_lds2<PDX_LDS_LANES, PDX_FEAT_MULTI | PDX_FEAT_ZW>), G, BL, lds_bytes(c, e2, bud), c->stream, af); break;
+            }
             PDX_HIP(c, hipGetLastError());
             PDX_TRY(pdx_timed_end(c, "step"));
             break;

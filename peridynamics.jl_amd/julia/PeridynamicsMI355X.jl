@@ -51,6 +51,7 @@ struct PdxParams
     lambda::Float64; mu::Float64; Gc::Float64; epsc::Float64; bc::Float64
     model::Int32; kernel::Int32
     zem_cs::Float64; maxdmg::Float64
+    zem::Int32; reserved_::Int32
 end
 
 struct PdxNkOpts
@@ -100,10 +101,11 @@ model_id(::CMaterial{<:NeoHooke}) = Int32(3)
 model_id(_) = Int32(0)
 
 function pdx_params(body, p)
-    cs = body.mat isa CMaterial ? body.mat.zem.Cs : 0.0
+    wan = body.mat isa CMaterial && body.mat.zem isa ZEMWan
+    cs = body.mat isa CMaterial && !wan ? body.mat.zem.Cs : 0.0
     maxdmg = body.mat isa CMaterial ? body.mat.maxdmg : 0.0
     return PdxParams(p.δ, p.rho, p.E, p.nu, p.G, p.K, p.λ, p.μ, p.Gc, p.εc, p.bc,
-                     model_id(body.mat), Int32(0), cs, maxdmg)
+                     model_id(body.mat), Int32(0), cs, maxdmg, Int32(wan ? 1 : 0), Int32(0))
 end
 
 # one parameter set: pdx_set_material; several (point_params + params_map):

@@ -11,7 +11,7 @@ from .data_handler import MI355XBodyDataHandler
 from .job import Job, submit
 from .materials import (BBMaterial, CMaterial, CriticalStretch, EnergySurfaceCorrection,
                         LinearElastic, NeoHooke, NeoHookePenalty, NoCorrection, OSBMaterial,
-                        SaintVenantKirchhoff, ZEMSilling, linear_kernel)
+                        SaintVenantKirchhoff, ZEMSilling, ZEMWan, linear_kernel)
 from .point_generators import jitter, lattice_block, uniform_box
 from .postprocess import crack_tip, mode_i_plate
 from .solvers import DynamicRelaxation, NewtonKrylov, VelocityVerlet
@@ -22,7 +22,7 @@ __all__ = [
     "load_library", "Body", "displacement_bc", "failure_permit", "forcedensity_bc", "material",
     "n_points", "no_failure", "point_set", "precrack", "velocity_bc", "MI355XBodyDataHandler",
     "Job", "submit", "BBMaterial", "CMaterial", "CriticalStretch", "NeoHookePenalty",
-    "OSBMaterial", "SaintVenantKirchhoff", "LinearElastic", "NeoHooke", "ZEMSilling", "linear_kernel", "jitter",
+    "OSBMaterial", "SaintVenantKirchhoff", "LinearElastic", "NeoHooke", "ZEMSilling", "ZEMWan", "linear_kernel", "jitter",
     "lattice_block", "uniform_box", "NewtonKrylov", "VelocityVerlet", "crack_tip",
     "mode_i_plate", "DynamicRelaxation", "EnergySurfaceCorrection", "NoCorrection",
     "VTKExporter", "process_each_export", "read_vtk", "write_pvtu", "write_vtu",

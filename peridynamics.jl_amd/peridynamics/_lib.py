@@ -25,6 +25,7 @@ PDX_MIXED = 1
 
 PDX_BB, PDX_OSB, PDX_CPD = 0, 1, 2
 PDX_SVK, PDX_NHP, PDX_LINEAR_ELASTIC, PDX_NEO_HOOKE = 0, 1, 2, 3
+PDX_ZEM_SILLING, PDX_ZEM_WAN = 0, 1
 
 FIELDS = {
     "position": (0, 3),
@@ -79,7 +80,7 @@ class PdxParams(ctypes.Structure):
     _fields_ = [(k, ctypes.c_double) for k in
                 ("delta", "rho", "E", "nu", "G", "K", "lambda_", "mu", "Gc", "epsc", "bc")] + [
         ("model", ctypes.c_int32), ("kernel", ctypes.c_int32), ("zem_cs", ctypes.c_double),
-        ("maxdmg", ctypes.c_double)]
+        ("maxdmg", ctypes.c_double), ("zem", ctypes.c_int32), ("reserved_", ctypes.c_int32)]
 
 
 class PdxNkOpts(ctypes.Structure):

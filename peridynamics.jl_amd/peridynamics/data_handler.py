@@ -68,7 +68,8 @@ class MI355XBodyDataHandler:
                        maxdmg=0.85)
         if isinstance(mat, CMaterial):
             pp.model = mat.constitutive_model.code
-            pp.zem_cs = mat.zem.Cs
+            pp.zem = mat.zem.code
+            pp.zem_cs = getattr(mat.zem, "Cs", 0.0)
             pp.maxdmg = mat.maxdmg
         return pp
 

@@ -48,6 +48,16 @@ class NeoHooke:
 class ZEMSilling:
     """ZEMSilling(; Cs=0.5): zem_stabilization.jl:16-21."""
     Cs: float = 0.5
+    code = 0
+
+
+class ZEMWan:
+    """ZEMWan(): zem_stabilization.jl:30-42, the stabilisation stiffness C_1 = C : K^-1 of the
+    point's Hooke tensor (correspondence.jl:253-285)."""
+    code = 1
+
+    def __repr__(self):
+        return "ZEMWan()"
 
 
 class CriticalStretch:
@@ -125,8 +135,8 @@ class CMaterial:
             raise _arg_error("the constitutive models are SaintVenantKirchhoff, LinearElastic, "
                              "NeoHooke and NeoHookePenalty")
         self.zem = zem or ZEMSilling()
-        if not isinstance(self.zem, ZEMSilling):
-            raise _arg_error("only ZEMSilling is provided by the MI355X build")
+        if not isinstance(self.zem, (ZEMSilling, ZEMWan)):
+            raise _arg_error("the zero-energy-mode stabilisations are ZEMSilling and ZEMWan")
         self.dmgmodel = dmgmodel or CriticalStretch()
         self.maxdmg = float(maxdmg)
 

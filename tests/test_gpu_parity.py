@@ -27,10 +27,11 @@ def say(*a):
 
 
 def body_block(kind, nx, ny, nz, dx=1e-3, jit=0.0, seed=0x5EED, Gc=100.0, E=70e9, nu=None,
-               rho=2700.0, model=None, fail=True):
+               rho=2700.0, model=None, fail=True, zem=None):
     pos, vol = pd.lattice_block(nx, ny, nz, dx, jitter_amplitude=jit, seed=seed)
+    ckw = {} if zem is None else dict(zem=zem)
     mat = {"BB": lambda: pd.BBMaterial(), "OSB": lambda: pd.OSBMaterial(),
-           "CPD": lambda: pd.CMaterial(model=model)}[kind]()
+           "CPD": lambda: pd.CMaterial(model=model, **ckw)}[kind]()
     b = pd.Body(mat, pos, vol)
     kw = dict(horizon=3.015 * dx, rho=rho, E=E)
     if kind != "BB":
@@ -49,8 +50,10 @@ def oracle_for(oracle, body, kind, dh=None):
     if kind == "CPD":
         names = {pd.NeoHookePenalty: "NHP", pd.LinearElastic: "LE", pd.NeoHooke: "NH",
                  pd.SaintVenantKirchhoff: "SVK"}
-        extra = dict(model=names[type(body.mat.constitutive_model)], Cs=body.mat.zem.Cs,
-                     maxdmg=body.mat.maxdmg)
+        wan = isinstance(body.mat.zem, pd.ZEMWan)
+        extra = dict(model=names[type(body.mat.constitutive_model)],
+                     Cs=0.0 if wan else body.mat.zem.Cs, maxdmg=body.mat.maxdmg,
+                     zem="Wan" if wan else "Silling")
     cracks = [(body.point_sets[a], body.point_sets[b], f) for a, b, f in body.point_sets_precracks]
     return oracle.OracleSim(body.xyz, body.volume, kind, as_oracle_params(p, **extra),
                             fail_permit=body.fail_permit.astype(np.uint8), precracks=cracks)
@@ -220,6 +223,49 @@ def test_force_density_fp64(need_gpu, oracle, kind, amp, layout):
         assert normwise(dg[fin], orc.defgrad[fin]) < 1e-12
     else:
         assert np.array_equal(act, oact)
+
+
+@pytest.mark.parametrize("layout", ["lds", "lds12k"])
+@pytest.mark.parametrize("model", [None, pd.NeoHookePenalty])
+@pytest.mark.parametrize("amp", [2e-7, 3e-6])
+def test_force_density_cpd_zemwan(need_gpu, oracle, amp, model, layout):
+    """CMaterial(zem=ZEMWan()): the stabilisation w C_1 z with C_1 = C : K^-1 of each
+    point (correspondence.jl:253-285), against the oracle at the FP64 tolerance; amp 3e-6
+    breaks bonds."""
+    say("force cpd zemwan", amp, model, layout)
+    b = body_block("CPD", 24, 20, 8, jit=0.05, E=210e9, Gc=None, rho=7850.0,
+                   model=None if model is None else model(), zem=pd.ZEMWan())
+    with bond_layout(layout):
+        dh = pd.MI355XBodyDataHandler(b)
+    orc = oracle_for(oracle, b, "CPD")
+    X = b.xyz
+    x = X + displacement_field(X, amp)
+    dh.set("position", x)
+    dh.calc_force_density(0.0, 1e-7)
+    orc.force(x)
+    err = scaled_error(dh.get("b_int"), orc)
+    say(f"  scaled error {err:.3e}")
+    assert err <= FP64_TOL
+    np.testing.assert_array_equal(dh.get("damage"), orc.damage)
+    _, _, _, _, act = dh.bonds()
+    assert np.array_equal(act, orc.active)
+
+
+def test_cpd_zemwan_mixed_context_and_global_layout(need_gpu, oracle):
+    """A mixed-precision context with ZEMWan runs the FP64 passes (FP64 tolerance); the
+    global-gather bond layout refuses ZEMWan with an error."""
+    say("cpd zemwan mixed context / global layout")
+    b = body_block("CPD", 16, 12, 8, jit=0.05, E=210e9, Gc=None, rho=7850.0, zem=pd.ZEMWan())
+    dh = pd.MI355XBodyDataHandler(b, precision="mixed")
+    orc = oracle_for(oracle, b, "CPD")
+    x = b.xyz + displacement_field(b.xyz, 3e-6)
+    dh.set("position", x)
+    dh.calc_force_density(0.0, 1e-7)
+    orc.force(x)
+    assert scaled_error(dh.get("b_int"), orc) <= FP64_TOL
+    with bond_layout("global"):
+        with pytest.raises(RuntimeError, match="ZEMWan"):
+            pd.MI355XBodyDataHandler(b)
 
 
 def test_force_density_cpd_nhp_fixture(need_gpu, golden):

@@ -80,3 +80,116 @@ def test_pure_shear_closed_form(oracle, model):
     S = LAM * np.trace(E) * np.eye(3) + 2 * MU * E
     np.testing.assert_allclose(P_of(oracle, model, F), F @ S, rtol=1.5e-8,
                                atol=1.5e-8 * np.abs(F @ S).max())
+
+
+# ZEMWan: calc_zem_stiffness_tensor (zem_stabilization.jl:32-42) and c_force_density!
+# (correspondence.jl:253-279)
+
+def _spd(seed):
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((3, 3))
+    return A @ A.T + 3 * np.eye(3)
+
+
+def test_zem_c1_isotropic_closed_form(oracle):
+    """For consistent isotropic constants a = (1 - nu) lambda / nu = lambda + 2 mu, and the
+    contraction of the isotropic tensor with K^-1 is lambda tr(K^-1) I + 2 mu sym(K^-1)."""
+    o = oracle
+    for seed in range(4):
+        Kinv = np.linalg.inv(_spd(seed))
+        C1 = o.zem_c1(Kinv, LAM, MU, NU)
+        want = LAM * np.trace(Kinv) * np.eye(3) + MU * (Kinv + Kinv.T)
+        np.testing.assert_allclose(C1, want, rtol=0, atol=1e-13 * np.abs(want).max())
+
+
+def test_zem_c1_brute_force_hooke_tensor(oracle):
+    """C_1[i,j] = sum_kl C[i,j,k,l] Kinv[k,l] with the full 81-entry Hooke tensor built from
+    its Voigt matrix, for constants where a differs from lambda + 2 mu (nu not the one that
+    links lambda and mu) and a non-symmetric Kinv."""
+    o = oracle
+    lam, mu, nu = 1.3, 0.7, 0.21
+    a = (1 - nu) * lam / nu
+    Cv = np.zeros((6, 6))
+    Cv[:3, :3] = lam
+    Cv[[0, 1, 2], [0, 1, 2]] = a
+    Cv[3, 3] = Cv[4, 4] = Cv[5, 5] = mu
+    vm = [(0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)]
+    C = np.zeros((3, 3, 3, 3))
+    for I, (i1, i2) in enumerate(vm):
+        for J, (j1, j2) in enumerate(vm):
+            for (p, q) in ((i1, i2), (i2, i1)):
+                for (r, s) in ((j1, j2), (j2, j1)):
+                    C[p, q, r, s] = Cv[I, J]
+    Kinv = np.random.default_rng(7).standard_normal((3, 3))
+    want = np.einsum("ijkl,kl->ij", C, Kinv)
+    np.testing.assert_allclose(o.zem_c1(Kinv, lam, mu, nu), want, rtol=0, atol=1e-14 * np.abs(want).max())
+
+
+def _cparams(mat, horizon, **kw):
+    import peridynamics as pd
+    from conftest import as_oracle_params
+    p = pd.materials.get_point_params(mat, dict(horizon=horizon, rho=1.0, E=1.0, nu=0.25, Gc=1.0))
+    return as_oracle_params(p, **kw)
+
+
+def test_zem_wan_equals_silling_on_the_tetrahedron(oracle, golden):
+    """test/unit/physics/test_correspondence.jl:56-67: every family of the tetrahedron is
+    complete, so z = dx - F dX vanishes on each bond and ZEMWan, ZEMSilling and no
+    stabilisation give the same force density."""
+    import peridynamics as pd
+    g = golden["cpd_nhp_displaced_point"]
+    X = np.array(g["position"]).T
+    mat = pd.CMaterial(model=pd.NeoHookePenalty())
+    out = {}
+    for name, extra in (("silling", {}), ("wan", {"zem": "Wan"}), ("none", {"Cs": 0.0})):
+        q = _cparams(mat, g["horizon"], model="NHP", **extra)
+        sim = oracle.OracleSim(X, g["volume"], "CPD", q, fail_permit=np.zeros(5))
+        x = X.copy()
+        x[1, 0] += g["shift_x_of_point_1"]
+        out[name] = sim.force(x).copy()
+    scale = np.abs(out["silling"]).max()
+    np.testing.assert_allclose(out["wan"], out["silling"], rtol=0, atol=1e-12 * scale)
+    np.testing.assert_allclose(out["none"], out["silling"], rtol=0, atol=1e-12 * scale)
+
+
+def test_zem_wan_force_is_bare_force_plus_c1_z(oracle):
+    """On a perturbed cube (z != 0) the ZEMWan force density minus the bare one (Cs = 0) is
+    the sum over bonds of w C_1(i) z_ij V_j at i and its negative, times V_i, at j, with
+    C_1 = lambda tr(K^-1) I + 2 mu sym(K^-1) from each point's shape tensor K = sum w V_j
+    dX dX' and z_ij = dx - F_i dX.  Using C_1 of the neighbour, or dropping the reaction,
+    fails it."""
+    import peridynamics as pd
+    n = 4
+    pos, vol = pd.uniform_box(1.0, 1.0, 1.0, 1.0 / n)
+    X = pos.T.copy()
+    mat = pd.CMaterial()
+    p = pd.materials.get_point_params(mat, dict(horizon=3.015 / n, rho=7850.0, E=210e9,
+                                                nu=0.25, Gc=2.7))
+    from conftest import as_oracle_params
+    rng = np.random.default_rng(3)
+    x = X @ np.array([[1.01, 0.003, 0.0], [0.0, 0.995, 0.0], [0.0, 0.0, 1.0]]).T
+    x = x + 1e-4 * rng.standard_normal(x.shape) / n
+    sims = {}
+    for name, extra in (("wan", {"zem": "Wan"}), ("none", {"Cs": 0.0})):
+        sim = oracle.OracleSim(X, vol, "CPD", as_oracle_params(p, model="SVK", **extra),
+                               fail_permit=np.zeros(len(X)))
+        sim.force(x)
+        sims[name] = sim
+    s = sims["wan"]
+    want = np.zeros_like(X)
+    for i in range(len(X)):
+        js = s.nbr[s.row_ptr[i]:s.row_ptr[i + 1]]
+        w = s.omega[s.row_ptr[i]:s.row_ptr[i + 1]] * s.active[s.row_ptr[i]:s.row_ptr[i + 1]]
+        DX = X[js] - X[i]
+        dx = x[js] - x[i]
+        K = np.einsum("b,b,bi,bj->ij", w, vol[js], DX, DX)
+        Kinv = np.linalg.inv(K)
+        C1 = p.lam * np.trace(Kinv) * np.eye(3) + p.mu * (Kinv + Kinv.T)
+        F = s.defgrad.reshape(-1)[9 * i:9 * i + 9].reshape(3, 3).T
+        z = dx - DX @ F.T
+        t = w[:, None] * (z @ C1.T)
+        want[i] += (t * vol[js][:, None]).sum(0)
+        np.add.at(want, js, -t * vol[i])
+    got = sims["wan"].b_int - sims["none"].b_int
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-9 * np.abs(want).max())
+    assert np.abs(want).max() > 1e-6 * np.abs(sims["none"].b_int).max()

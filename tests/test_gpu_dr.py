@@ -96,7 +96,7 @@ def test_dr_trajectory_matches_oracle(need_gpu, oracle):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-@pytest.mark.parametrize("kind", ["BB", "OSB", "CPD"])
+@pytest.mark.parametrize("kind", ["BB", "OSB", "CPD", "CPD-ZEMWan"])
 def test_dr_materials(need_gpu, oracle, kind, precision):
     """Dynamic relaxation of BB, OSB and CPD bodies (calc_density_matrix! for any bond
     system, dynamic_relaxation.jl:153-163): the block of the trajectory test against the
@@ -106,7 +106,11 @@ def test_dr_materials(need_gpu, oracle, kind, precision):
     say("dr", kind, precision)
     n_steps = 150 if kind == "BB" else 20
     pos, vol = pd.lattice_block(12, 12, 4, 1e-3, jitter_amplitude=0.05, seed=7)
-    mat = {"BB": pd.BBMaterial, "OSB": pd.OSBMaterial, "CPD": pd.CMaterial}[kind]()
+    wan = kind == "CPD-ZEMWan"
+    if wan:
+        kind = "CPD"
+    mat = {"BB": pd.BBMaterial, "OSB": pd.OSBMaterial,
+           "CPD": lambda: pd.CMaterial(zem=pd.ZEMWan()) if wan else pd.CMaterial()}[kind]()
     b = pd.Body(mat, pos, vol)
     kw = dict(horizon=3.015e-3, rho=2700.0, E=70e9)
     if kind != "BB":
@@ -124,7 +128,8 @@ def test_dr_materials(need_gpu, oracle, kind, precision):
     dh.solve_dr(dr)
     extra = {}
     if kind == "CPD":
-        extra = dict(model="SVK", Cs=b.mat.zem.Cs, maxdmg=b.mat.maxdmg)
+        extra = dict(model="SVK", Cs=0.0 if wan else b.mat.zem.Cs, maxdmg=b.mat.maxdmg,
+                     zem="Wan" if wan else "Silling")
     orc = oracle.OracleSim(b.xyz, b.volume, kind, as_oracle_params(b.point_params[0], **extra),
                            fail_permit=b.fail_permit.astype(np.uint8))
     bcs = [("sd", b.point_sets["top"], 1, lambda t: 1e-6)] + \

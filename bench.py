@@ -167,7 +167,7 @@ def jfnk_leg(shape, train_steps):
                                                      res["two_level"]["seconds_steps_3_10"])}
 
 
-def state_based_bytes(kind, n_points, n_bonds, layout, precision="fp64"):
+def state_based_bytes(kind, n_points, n_bonds, layout, precision="fp64", zem="Silling"):
     """Algorithmic bytes of the two passes of an OSB or CPD step (one parameter set).  Per
     bond and pass the bond word and the FP64 length (LDS layout 2 + 8, global layout
     4 + 8; mixed OSB only the 2-byte word: the length comes from the reference positions).
@@ -176,8 +176,9 @@ def state_based_bytes(kind, n_points, n_bonds, layout, precision="fp64"):
     writes (c1 V, c2 V) and the own force part, 40 B, pass 2 reads the own part, 24 B);
     mixed OSB 300 B (X|V also read in both passes, (c1, c2) written and read back, no own
     part); mixed CPD 414 B (as FP64 CPD, the state Q' = P K^-1 - beta (F - I)); CPD 415 B (also X|V in both passes and the
-    nine numbers of Q = P K^-1 - beta F written and read back).  LDS layout: plus the
-    4-byte neighbour-table index per entry and pass."""
+    nine numbers of Q = P K^-1 - beta F written and read back); CPD with ZEMWan 559 B (the
+    state is Q = P K^-1 - C_1 F and C_1, 18 numbers).  LDS layout: plus the 4-byte
+    neighbour-table index per entry and pass."""
     if precision == "mixed" and layout["lds_mode"]:
         per_pt = 300 if kind == "OSB" else 414
         return (4 * n_bonds + per_pt * n_points + 8 * layout["halo_entries"],
@@ -185,19 +186,21 @@ def state_based_bytes(kind, n_points, n_bonds, layout, precision="fp64"):
     if kind == "OSB":
         per_pt = 268 if layout["lds_mode"] else 236
     else:
-        per_pt = 415
+        per_pt = 559 if zem == "Wan" else 415
     if layout["lds_mode"]:
         return (20 * n_bonds + per_pt * n_points + 8 * layout["halo_entries"],
                 f"20*N_b + {per_pt}*N_p + 8*N_halo")
     return 24 * n_bonds + per_pt * n_points, f"24*N_b + {per_pt}*N_p"
 
 
-def state_based_leg(kind, shape, steps, warmup=10, dx=1e-3, precision="fp64"):
-    """Configuration 3: OSB or CMaterial (Saint Venant-Kirchhoff) bar, torsion 10 rad/s
-    on one end (three point layers), the other end held, FP64."""
+def state_based_leg(kind, shape, steps, warmup=10, dx=1e-3, precision="fp64", zem="Silling"):
+    """Configuration 3: OSB or CMaterial (Saint Venant-Kirchhoff; ZEMSilling, or ZEMWan
+    with zem="Wan") bar, torsion 10 rad/s on one end (three point layers), the other end
+    held."""
     import peridynamics as pd
     pos, vol = pd.lattice_block(*shape, dx)
-    mat = pd.OSBMaterial() if kind == "OSB" else pd.CMaterial()
+    cmat = (lambda: pd.CMaterial(zem=pd.ZEMWan())) if zem == "Wan" else pd.CMaterial
+    mat = pd.OSBMaterial() if kind == "OSB" else cmat()
     body = pd.Body(mat, pos, vol)
     pd.material(body, horizon=3.015 * dx, rho=7850.0, E=210e9, nu=0.25)
     x = pos[0]
@@ -224,11 +227,11 @@ def state_based_leg(kind, shape, steps, warmup=10, dx=1e-3, precision="fp64"):
     k2, _ = ctx.kernel_time("step")
     ctx.set_profiling(False)
     nb, npnt = dh.n_bonds, body.n_points
-    byt, formula = state_based_bytes(kind, npnt, nb, ctx.layout(), precision)
+    byt, formula = state_based_bytes(kind, npnt, nb, ctx.layout(), precision, zem)
     rbyt, rformula = reference_layout_bytes(kind, npnt, nb, precision)
     dh.close()
     return {"workload": f"{kind} torsion bar {shape[0]}x{shape[1]}x{shape[2]}, 10 rad/s, "
-                        f"{precision.upper()}",
+                        f"{precision.upper()}" + (", ZEMWan" if zem == "Wan" else ""),
             "points": npnt, "bonds": nb, "steps": steps, "ms_per_step": el / steps * 1e3,
             "bond_updates_per_s": nb * steps / el, "pass1_ms": k1, "pass2_ms": k2,
             "bytes_formula": formula, "bytes_per_step": byt,
@@ -631,6 +634,7 @@ def main():
         sb["OSB_mixed"] = state_based_leg("OSB", (160, 80, 40), 100, precision="mixed")
         sb["OSB_2M_mixed"] = osb_2m_leg(shape, 100, precision="mixed")
         sb["CPD_mixed"] = state_based_leg("CPD", (160, 80, 40), 100, precision="mixed")
+        sb["CPD_ZEMWan"] = state_based_leg("CPD", (160, 80, 40), 100, zem="Wan")
         for a, m in (("OSB", "OSB_mixed"), ("OSB_2M", "OSB_2M_mixed"), ("CPD", "CPD_mixed")):
             sb[m]["speedup_vs_fp64_step"] = sb[a]["ms_per_step"] / sb[m]["ms_per_step"]
             sb[m]["speedup_vs_fp64_kernels"] = ((sb[a]["pass1_ms"] + sb[a]["pass2_ms"]) /

@@ -176,8 +176,8 @@ def state_based_bytes(kind, n_points, n_bonds, layout, precision="fp64", zem="Si
     writes (c1 V, c2 V) and the own force part, 40 B, pass 2 reads the own part, 24 B);
     mixed OSB 300 B (X|V also read in both passes, (c1, c2) written and read back, no own
     part); mixed CPD 414 B (as FP64 CPD, the state Q' = P K^-1 - beta (F - I)); CPD 415 B (also X|V in both passes and the
-    nine numbers of Q = P K^-1 - beta F written and read back); CPD with ZEMWan 559 B (the
-    state is Q = P K^-1 - C_1 F and C_1, 18 numbers).  LDS layout: plus the 4-byte
+    nine numbers of Q = P K^-1 - beta F written and read back); CPD with ZEMWan 511 B (the
+    state is Q = P K^-1 - C_1 F and the six entries of the symmetric C_1, 15 numbers).  LDS layout: plus the 4-byte
     neighbour-table index per entry and pass."""
     if precision == "mixed" and layout["lds_mode"]:
         per_pt = 300 if kind == "OSB" else 414
@@ -186,7 +186,7 @@ def state_based_bytes(kind, n_points, n_bonds, layout, precision="fp64", zem="Si
     if kind == "OSB":
         per_pt = 268 if layout["lds_mode"] else 236
     else:
-        per_pt = 559 if zem == "Wan" else 415
+        per_pt = 511 if zem == "Wan" else 415
     if layout["lds_mode"]:
         return (20 * n_bonds + per_pt * n_points + 8 * layout["halo_entries"],
                 f"20*N_b + {per_pt}*N_p + 8*N_halo")

@@ -858,7 +858,7 @@ extern "C" pdx_status pdx_build_bonds(pdx_ctx *c, int64_t *n_bonds_out)
     if (c->kind == PDX_OSB) TRY_C(pdx_alloc(c, &c->pstate, 5 * n));
     if (c->kind == PDX_CPD) {
         // Q per point; with ZEMWan also the stabilisation stiffness C_1 (18 numbers)
-        TRY_C(pdx_alloc(c, &c->pstate, (c->P.zem == PDX_ZEM_WAN ? 18 : 9) * n));
+        TRY_C(pdx_alloc(c, &c->pstate, (c->P.zem == PDX_ZEM_WAN ? 15 : 9) * n));
         TRY_C(pdx_alloc(c, &c->defgrad, 9 * n));
         TRY_C(pdx_alloc(c, &c->cauchy, 9 * n));
         HIP_C(hipMemsetAsync(c->defgrad, 0, sizeof(double) * 9 * n, c->stream));

@@ -1535,12 +1535,16 @@ __device__ __forceinline__ void cpd_point_state(const pdx_step_args &A, int64_t 
                 }
             }
         m3_mul(C1, F, CF);
+        // C_1 is exactly symmetric (its off-diagonal entries are mu a + mu b in either
+        // order): its six distinct entries 00 10 20 11 21 22 follow Q
+        constexpr int SU[6] = {0, 1, 2, 4, 5, 8};
 #pragma unroll
         for (int k = 0; k < 9; k++) {
-            A.pstate[18 * p + k] = PK[k] - CF[k];
-            A.pstate[18 * p + 9 + k] = C1[k];
+            A.pstate[15 * p + k] = PK[k] - CF[k];
             if (store) A.defgrad[9 * p + k] = F[k];
         }
+#pragma unroll
+        for (int m = 0; m < 6; m++) A.pstate[15 * p + 9 + m] = C1[SU[m]];
     } else {
 #pragma unroll
         for (int k = 0; k < 9; k++) {
@@ -1825,9 +1829,10 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2(pdx_step_args A)
     // bond (ACT), the neighbour's term the neighbour's copy (RACT)
     constexpr bool MULTI = (F & PDX_FEAT_MULTI) != 0;
     // PDX_FEAT_ZW: ZEMWan, the stabilisation C_1 dx of each end instead of beta dx; entries
-    // also carry [j alive] C_1 of the neighbour (208 B)
+    // also carry [j alive] C_1 of the neighbour, six entries of the symmetric C_1 (184 B)
     constexpr bool ZW = (F & PDX_FEAT_ZW) != 0;
-    constexpr int NQ = ZW ? 18 : 9;
+    constexpr int NQ = ZW ? 15 : 9;
+    constexpr int SY[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};  // (r, c) -> stored entry of C_1
     extern __shared__ double4 hrec[];
     const int TL = pdx_lds_entries(A.halo_max, A.lds_budget, 64 + 8 * NQ);
     double4 *const hx = hrec;
@@ -1899,10 +1904,12 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2(pdx_step_args A)
                         // Q DX + C_1 dx at each end (a dead neighbour was staged as zeros)
                         if (!dead_i && act)
                             tt += ((Qi[r] * DX[0] + Qi[r + 3] * DX[1]) + Qi[r + 6] * DX[2]) +
-                                  ((Qi[9 + r] * dx[0] + Qi[12 + r] * dx[1]) + Qi[15 + r] * dx[2]);
+                                  ((Qi[9 + SY[r]] * dx[0] + Qi[9 + SY[r + 3]] * dx[1]) +
+                                   Qi[9 + SY[r + 6]] * dx[2]);
                         if (ract)
                             tt += ((qj[r] * DX[0] + qj[r + 3] * DX[1]) + qj[r + 6] * DX[2]) +
-                                  ((qj[9 + r] * dx[0] + qj[12 + r] * dx[1]) + qj[15 + r] * dx[2]);
+                                  ((qj[9 + SY[r]] * dx[0] + qj[9 + SY[r + 3]] * dx[1]) +
+                                   qj[9 + SY[r + 6]] * dx[2]);
                     } else {
                         if (!dead_i && act) tt += ((Qi[r] * DX[0] + Qi[r + 3] * DX[1]) + Qi[r + 6] * DX[2]) + beta * dx[r];
                         // a dead neighbour was staged with Q_j = 0 and beta = 0: its term adds 0
@@ -2498,7 +2505,7 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
                 break;
             }
             const int fc = (c->multi() ? PDX_FEAT_MULTI : 0) | (c->P.zem == PDX_ZEM_WAN ? PDX_FEAT_ZW : 0);
-            const int e1 = 64 + (c->multi() ? 1 : 0), e2 = c->P.zem == PDX_ZEM_WAN ? 208 : 136;
+            const int e1 = 64 + (c->multi() ? 1 : 0), e2 = c->P.zem == PDX_ZEM_WAN ? 184 : 136;
             PDX_TRY(pdx_timed_begin(c));
             switch (fc) {
             case 0: hipLaunchKernelGGL((k_cpd_lds1<PDX_LDS_LANES, 0>), G, BL, lds_bytes(c, e1, bud), c->stream, af); break;

@@ -13,7 +13,9 @@
  * Parity pins (see tests/test_oracle_*.py): bond lists of the reference unit tests,
  * the closed-form BB force, the OSB and CPD (NeoHookePenalty) regression vectors,
  * the force-density invariants (zero at reference, momentum, mirror symmetry,
- * pulled-corner damage), the stable time step closed form, brute force vs cell list.
+ * pulled-corner damage), the stable time step closed form, brute force vs cell list;
+ * the constitutive models by P = dPsi/dF and their closed forms, the ZEMWan C_1 by the
+ * isotropic closed form and a brute-force Hooke contraction, its force by a bond sum.
  *
  * Parity unpinned (no reference test fixes them, and the reference cannot run here):
  *  - the order of bonds across grid cells of PointNeighbors' foreach_neighbor; this

@@ -79,6 +79,26 @@ def test_osb_interface_closed_form(oracle):
     assert abs(b[0, 0] * V1 + b[1, 0] * V2) <= 1e-15 * abs(b[0, 0])
 
 
+def test_osb_interface_fixture(oracle, golden):
+    """The reference's known answer across an OSB material interface
+    (test_ordinary_state_based.jl:31-61): E_b = 2 E_a, volumes [1, 2]."""
+    import peridynamics as pd
+    g = golden["osb_interface"]
+    X = np.array(g["position"]).T
+    sets = []
+    for E in (g["E_a"], g["E_b"]):
+        p = pd.materials.get_point_params(pd.OSBMaterial(), dict(
+            horizon=g["horizon"], rho=g["rho"], E=E, nu=g["nu"], Gc=g["Gc"]))
+        sets.append(as_oracle_params(p))
+    orc = oracle.OracleSim(X, np.array(g["volume"]), "OSB", sets, fail_permit=np.zeros(2),
+                           point_set=[0, 1])
+    x = orc.X.copy()
+    x[1, 0] = g["moved_x_of_point_1"]
+    b = orc.force(x)
+    np.testing.assert_allclose(b[0], [g["b1"], 0, 0], rtol=1e-13, atol=0)
+    np.testing.assert_allclose(b[1], [-g["b2"], 0, 0], rtol=1e-13, atol=0)
+
+
 def test_osb_asymmetric_failure(oracle):
     """eps_c of point 1 below the stretch, of point 2 above: the bond 1 -> 2 breaks and 2 -> 1
     stays.  Point 1 then has a zero weighted volume and contributes nothing; point 2's term

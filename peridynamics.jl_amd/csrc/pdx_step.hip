@@ -1540,15 +1540,17 @@ __device__ __forceinline__ void cpd_point_state(const pdx_step_args &A, int64_t 
         constexpr int SU[6] = {0, 1, 2, 4, 5, 8};
 #pragma unroll
         for (int k = 0; k < 9; k++) {
-            A.pstate[15 * p + k] = PK[k] - CF[k];
+            A.pstate[15 * p + k] = kill ? 0.0 : PK[k] - CF[k];
             if (store) A.defgrad[9 * p + k] = F[k];
         }
 #pragma unroll
-        for (int m = 0; m < 6; m++) A.pstate[15 * p + 9 + m] = C1[SU[m]];
+        for (int m = 0; m < 6; m++) A.pstate[15 * p + 9 + m] = kill ? 0.0 : C1[SU[m]];
     } else {
 #pragma unroll
         for (int k = 0; k < 9; k++) {
-            A.pstate[9 * p + k] = PK[k] - beta * F[k];
+            // a killed point's F may hold NaN (a singular K once all its bonds are off):
+            // its state is zero, so no reader can pick the NaN up
+            A.pstate[9 * p + k] = kill ? 0.0 : PK[k] - beta * F[k];
             if (store) A.defgrad[9 * p + k] = F[k];
         }
     }
@@ -2078,7 +2080,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_pass1_mx(pdx_step_args A)
     }
 #pragma unroll
     for (int q = 0; q < 9; q++) {
-        A.pstate[9 * p + q] = PK[q] - A.beta_zem * Hm[q];
+        A.pstate[9 * p + q] = kill ? 0.0 : PK[q] - A.beta_zem * Hm[q];  // no NaN of a dead K
         if (store) A.defgrad[9 * p + q] = Fm[q];
     }
     if (kill && !dead) A.killed[p] = 1;
@@ -2093,10 +2095,11 @@ __device__ __forceinline__ void cpd_mx_entry(const pdx_step_args &A, int64_t j, 
                                              double uo1, double uo2, float4 &a, float4 &b, float q[9])
 {
     mx_stage_xu(A, j, O, uo0, uo1, uo2, a, b);
-    const float live = A.killed[j] ? 0.f : 1.f;
-    b.w = (float)A.beta_zem * live;
+    const bool live = !A.killed[j];
+    b.w = live ? (float)A.beta_zem : 0.f;
+    // selects, not products with 0: a dead point's state must not leak NaN or Inf
 #pragma unroll
-    for (int r = 0; r < 9; r++) q[r] = (float)A.pstate[9 * j + r] * live;
+    for (int r = 0; r < 9; r++) q[r] = live ? (float)A.pstate[9 * j + r] : 0.f;
 }
 
 template <int U, int S, int F>
@@ -2137,11 +2140,11 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_pass2_mx(pdx_step_args A)
     const float xi = (float)(Xi.x - O.x), yi = (float)(Xi.y - O.y), zi = (float)(Xi.z - O.z);
     const float ui = (float)((ri.x - Xi.x) - uo0), vi = (float)((ri.y - Xi.y) - uo1),
                 wi = (float)((ri.z - Xi.z) - uo2);
-    const float live_i = A.killed[p] ? 0.f : 1.f;
-    const float beta_i = (float)A.beta_zem * live_i;
+    const bool live_i = !A.killed[p];
+    const float beta_i = live_i ? (float)A.beta_zem : 0.f;
     float Qi[9];
 #pragma unroll
-    for (int r = 0; r < 9; r++) Qi[r] = (float)A.pstate[9 * p + r] * live_i;
+    for (int r = 0; r < 9; r++) Qi[r] = live_i ? (float)A.pstate[9 * p + r] : 0.f;
     const float deltaf = (float)A.delta;
     double b3[3] = {0.0, 0.0, 0.0};
     for (int t = sub; t < c; t += U * S) {

@@ -93,3 +93,41 @@ def test_job_vtk_files_match_fields(need_gpu, tmp_path, kind):
     assert np.array_equal(r0["position"], X) and np.all(r0["displacement"] == 0)
     assert np.any(r["damage"] > 0)
     dh2.close()
+
+
+@pytest.mark.parametrize("kind", ["BB", "OSB"])
+def test_exported_fields_match_oracle(need_gpu, oracle, tmp_path, kind):
+    """The VTK files of a Job against the oracle's state at the export steps
+    (export_results every freq steps, threads_body_data_handler.jl:197-212; the reference
+    ships no .vtu fixtures, so the oracle's trajectory is the independent data): displacement
+    and velocity within the trajectory bars (1e-9, 1e-8 normwise), damage bit for bit, b_int
+    within 1e-8 of each point's bond-sum rounding scale, the time of each file exact."""
+    from conftest import as_oracle_params
+    say("vtk vs oracle", kind)
+    fields = ("displacement", "velocity", "damage", "b_int")
+    b = tension_body(kind=kind)
+    root = str(tmp_path / "res")
+    job = pd.Job(b, pd.VelocityVerlet(steps=60), path=root, freq=20, fields=fields)
+    dh = pd.submit(job)
+    dt = job.solver.dt
+    dh.close()
+    orc = oracle.OracleSim(b.xyz, b.volume, kind, as_oracle_params(b.point_params[0]),
+                           fail_permit=b.fail_permit.astype(np.uint8))
+    assert dt == orc.stable_dt(0.7)
+    bcs = [("sd", b.point_sets["top"], 1, lambda t: 2.0),
+           ("sd", b.point_sets["bottom"], 1, lambda t: -2.0)]
+    orc.force()
+    for n in (20, 40, 60):
+        orc.vv_run(20, dt, bcs, first_step=n - 19)
+        r = pd.read_vtk(os.path.join(root, "vtk", f"timestep_{n}.pvtu"))
+        assert r["time"][0] == n * dt
+        eu = float(np.max(np.abs(r["displacement"] - orc.u)) / np.max(np.abs(orc.u)))
+        ev = float(np.max(np.abs(r["velocity"] - orc.v)) / np.max(np.abs(orc.v)))
+        scale = np.max(orc.b_abs, axis=1)
+        m = scale > 0
+        eb = float(np.max(np.max(np.abs(r["b_int"] - orc.b_int), axis=1)[m] / scale[m]))
+        say(f"  step {n}: displacement {eu:.2e}, velocity {ev:.2e}, b_int {eb:.2e}, "
+            f"damaged points {int(np.sum(orc.damage > 0))}")
+        assert eu < 1e-9 and ev < 1e-8 and eb < 1e-8
+        assert np.array_equal(r["damage"], orc.damage)
+    assert np.any(orc.damage > 0)

@@ -193,3 +193,29 @@ def test_zem_wan_force_is_bare_force_plus_c1_z(oracle):
     got = sims["wan"].b_int - sims["none"].b_int
     np.testing.assert_allclose(got, want, rtol=0, atol=1e-9 * np.abs(want).max())
     assert np.abs(want).max() > 1e-6 * np.abs(sims["none"].b_int).max()
+
+
+@pytest.mark.parametrize("model", MODELS)
+def test_stored_cauchy_stress_of_affine_deformation(oracle, model):
+    """The cauchy_stress the oracle's CPD force stores per point (stress.jl:112-116), under
+    the affine map x = F X: an interior point's F equals F_TEST to rounding, so its stored
+    stress is P(F) F' / det F with P the energy derivative pinned above; the 9 entries of a
+    point are a column-major 3x3 (as the reference's 9 x n field).  A point missing its outer
+    bonds has the same F (a complete shape tensor is not needed for an affine map), so every
+    point matches."""
+    import peridynamics as pd
+    pos, vol = pd.lattice_block(8, 8, 8, 1e-3)
+    mat = pd.CMaterial(model={"SVK": pd.SaintVenantKirchhoff, "LE": pd.LinearElastic,
+                              "NH": pd.NeoHooke, "NHP": pd.NeoHookePenalty}[model]())
+    p = pd.materials.get_point_params(mat, dict(horizon=3.015e-3, rho=7850.0, E=E_MOD, nu=NU))
+    from conftest import as_oracle_params
+    X = pos.T.copy()
+    sim = oracle.OracleSim(X, vol, "CPD", as_oracle_params(p, model=model),
+                           fail_permit=np.zeros(len(X)))
+    sim.force(X @ F_TEST.T)
+    P = P_of(oracle, model, F_TEST)
+    want = (P @ F_TEST.T / np.linalg.det(F_TEST)).reshape(-1, order="F")
+    got = sim.cauchy
+    assert got.shape == (len(X), 9)
+    np.testing.assert_allclose(got, np.broadcast_to(want, got.shape), rtol=0,
+                               atol=1e-9 * np.abs(want).max())

@@ -219,6 +219,17 @@ pdx_status pdx_set_force_density_bcs(pdx_ctx *ctx, int64_t n_points, const int32
  * NaN force density returns PDX_ERR_NAN and the first such step in *nan_step_out. */
 pdx_status pdx_vv_run(pdx_ctx *ctx, int64_t first_step, int64_t n_steps, double dt,
                       const double *bc_values, int64_t *nan_step_out);
+/* pdx_vv_run without any device allocation (no counterpart in the reference).  For callers
+ * that cannot allocate device memory while they run -- tight stepping or serving loops,
+ * harnesses that forbid hipMalloc -- once the context is set up: pdx_vv_run may hipFree and
+ * hipMalloc its boundary-condition value table when a run needs a larger one, this variant
+ * stages the table in a fixed device array of PDX_NOALLOC_BC_VALUES doubles (4 MiB) that
+ * each GPU holds from the first call until the process ends.  A run with n_src * n_steps
+ * above that returns PDX_ERR_ARG (split it into shorter runs).  The array is shared by all
+ * contexts on a GPU: calls that overlap on one GPU race.  Results equal pdx_vv_run's. */
+#define PDX_NOALLOC_BC_VALUES (1 << 19)
+pdx_status pdx_vv_run_noalloc(pdx_ctx *ctx, int64_t first_step, int64_t n_steps, double dt,
+                              const double *bc_values, int64_t *nan_step_out);
 
 /* --- dynamic relaxation ----------------------------------------------------------------- */
 /* DynamicRelaxation set-up (init_density_matrix!, dynamic_relaxation.jl:119-176): the

@@ -2570,35 +2570,18 @@ extern "C" pdx_status pdx_force_density(pdx_ctx *c, double t, double dt)
     return PDX_OK;
 }
 
-extern "C" pdx_status pdx_vv_run(pdx_ctx *c, int64_t first, int64_t n_steps, double dt,
-                                 const double *bc_values, int64_t *nan_step_out)
+// the steps of a run, with the boundary-condition value table already on the device
+static pdx_status vv_run_steps(pdx_ctx *c, int64_t first, int64_t n_steps, double dt,
+                               const double *bcv, int64_t *nan_step_out)
 {
-    if (!c) return PDX_ERR_ARG;
-    if (nan_step_out) *nan_step_out = -1;
-    if (!c->bonds_built) return pdx_fail(c, PDX_ERR_STATE, "pdx_build_bonds first");
-    if (n_steps <= 0) return PDX_OK;
-    if (!(dt > 0)) return pdx_fail(c, PDX_ERR_ARG, "`Δt` of VelocityVerlet smaller than zero!");
-    if (c->n_src > 0 && !bc_values) return pdx_fail(c, PDX_ERR_ARG, "missing BC values");
-    PDX_HIP(c, hipSetDevice(c->device));
-    // BC value table: one row of n_src values per step
-    const int64_t need = std::max<int64_t>(c->n_src * n_steps, 1);
-    if (need > c->bcvals_cap) {
-        if (c->bcvals) (void)hipFree(c->bcvals);
-        c->bcvals = nullptr;
-        PDX_TRY(pdx_alloc(c, &c->bcvals, need));
-        c->bcvals_cap = need;
-    }
-    if (c->n_src > 0)
-        PDX_HIP(c, hipMemcpyAsync(c->bcvals, bc_values, sizeof(double) * c->n_src * n_steps,
-                                  hipMemcpyHostToDevice, c->stream));
     PDX_TRY(pdx_reset_flags(c));
-    PDX_TRY(pdx_launch_prologue(c, first, dt, c->bcvals));
+    PDX_TRY(pdx_launch_prologue(c, first, dt, bcv));
     for (int64_t k = 0; k < n_steps; k++) {
         const bool last = (k == n_steps - 1);
         const int flags = last ? PDX_F_STORE : PDX_F_HAS_NEXT;
-        const double *next_bc = last ? c->bcvals : c->bcvals + c->n_src * (k + 1);
+        const double *next_bc = last ? bcv : bcv + c->n_src * (k + 1);
         // b_ext conditions at t = (first + k) dt, used by this launch's acceleration
-        PDX_TRY(pdx_apply_fd_bcs(c, c->n_src > 0 ? c->bcvals + c->n_src * k : nullptr));
+        PDX_TRY(pdx_apply_fd_bcs(c, c->n_src > 0 ? bcv + c->n_src * k : nullptr));
         pdx_step_args a;
         PDX_TRY(pdx_fill_step_args(c, &a, first + k, flags, dt, next_bc));
         PDX_TRY(pdx_launch_step(c, a));
@@ -2612,4 +2595,91 @@ extern "C" pdx_status pdx_vv_run(pdx_ctx *c, int64_t first, int64_t n_steps, dou
                         (long long)nan_step);
     }
     return PDX_OK;
+}
+
+static pdx_status vv_run_check(pdx_ctx *c, int64_t n_steps, double dt, const double *bc_values)
+{
+    if (!c->bonds_built) return pdx_fail(c, PDX_ERR_STATE, "pdx_build_bonds first");
+    if (n_steps <= 0) return PDX_OK;
+    if (!(dt > 0)) return pdx_fail(c, PDX_ERR_ARG, "`Δt` of VelocityVerlet smaller than zero!");
+    if (c->n_src > 0 && !bc_values) return pdx_fail(c, PDX_ERR_ARG, "missing BC values");
+    return PDX_OK;
+}
+
+extern "C" pdx_status pdx_vv_run(pdx_ctx *c, int64_t first, int64_t n_steps, double dt,
+                                 const double *bc_values, int64_t *nan_step_out)
+{
+    if (!c) return PDX_ERR_ARG;
+    if (nan_step_out) *nan_step_out = -1;
+    PDX_TRY(vv_run_check(c, n_steps, dt, bc_values));
+    if (n_steps <= 0) return PDX_OK;
+    PDX_HIP(c, hipSetDevice(c->device));
+    // BC value table: one row of n_src values per step
+    const int64_t need = std::max<int64_t>(c->n_src * n_steps, 1);
+    if (need > c->bcvals_cap) {
+        if (c->bcvals) (void)hipFree(c->bcvals);
+        c->bcvals = nullptr;
+        PDX_TRY(pdx_alloc(c, &c->bcvals, need));
+        c->bcvals_cap = need;
+    }
+    if (c->n_src > 0)
+        PDX_HIP(c, hipMemcpyAsync(c->bcvals, bc_values, sizeof(double) * c->n_src * n_steps,
+                                  hipMemcpyHostToDevice, c->stream));
+    return vv_run_steps(c, first, n_steps, dt, c->bcvals, nan_step_out);
+}
+
+// ----------------------------------------------------------------------------------------
+// pdx_vv_run_noalloc: the same run without any device allocation.  pdx_vv_run keeps the
+// boundary-condition value table of a run in context memory that grows (hipFree +
+// hipMalloc) when a run needs a larger table; this variant stages the table in `scratch`,
+// a fixed array of PDX_NOALLOC_BC_VALUES doubles that every GPU holds once for the life
+// of the process.  All contexts on one GPU share that GPU's array: calls that overlap on
+// one GPU (other host threads, other streams) race on it.  The table is written in full
+// by each call before its kernels read it, so nothing needs clearing.
+// ----------------------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[PDX_NOALLOC_BC_VALUES];
+
+#define PDX_NOALLOC_DEVICES 64
+static double *g_scratch_ptr[PDX_NOALLOC_DEVICES];  // device address of `scratch` per GPU
+static std::mutex g_scratch_mu;
+
+// The device address of this GPU's `scratch`, looked up once per device with the device
+// current.  In host code the array's name is only a host-side stand-in, so every host use
+// (kernel arguments, copies) takes this address instead.
+static pdx_status noalloc_scratch(pdx_ctx *c, double **out)
+{
+    if (c->device < 0 || c->device >= PDX_NOALLOC_DEVICES)
+        return pdx_fail(c, PDX_ERR_ARG, "device %d beyond the no-alloc table", c->device);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[c->device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return pdx_fail(c, PDX_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s",
+                            hipGetErrorString(e));
+        g_scratch_ptr[c->device] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[c->device];
+    return PDX_OK;
+}
+
+extern "C" pdx_status pdx_vv_run_noalloc(pdx_ctx *c, int64_t first, int64_t n_steps, double dt,
+                                         const double *bc_values, int64_t *nan_step_out)
+{
+    if (!c) return PDX_ERR_ARG;
+    if (nan_step_out) *nan_step_out = -1;
+    PDX_TRY(vv_run_check(c, n_steps, dt, bc_values));
+    if (n_steps <= 0) return PDX_OK;
+    if (c->n_src > PDX_NOALLOC_BC_VALUES / n_steps)
+        return pdx_fail(c, PDX_ERR_ARG,
+                        "pdx_vv_run_noalloc: %lld steps of %lld boundary-condition values "
+                        "exceed the fixed table of %d values; split the run",
+                        (long long)n_steps, (long long)c->n_src, PDX_NOALLOC_BC_VALUES);
+    PDX_HIP(c, hipSetDevice(c->device));
+    double *table = nullptr;
+    PDX_TRY(noalloc_scratch(c, &table));
+    if (c->n_src > 0)
+        PDX_HIP(c, hipMemcpyAsync(table, bc_values, sizeof(double) * c->n_src * n_steps,
+                                  hipMemcpyHostToDevice, c->stream));
+    return vv_run_steps(c, first, n_steps, dt, table, nan_step_out);
 }

@@ -128,6 +128,7 @@ SIGNATURES = {
     "pdx_force_density": (_ST, [_P, _D, _D]),
     "pdx_set_velocity_bcs": (_ST, [_P, _I64, _P, _P, _I64]),
     "pdx_vv_run": (_ST, [_P, _I64, _I64, _D, _P, ctypes.POINTER(_I64)]),
+    "pdx_vv_run_noalloc": (_ST, [_P, _I64, _I64, _D, _P, ctypes.POINTER(_I64)]),
     "pdx_set_force_density_bcs": (_ST, [_P, _I64, _P, _P]),
     "pdx_dr_init": (_ST, [_P, _D, _D]),
     "pdx_dr_run": (_ST, [_P, _I64, _I64, _D, _P, ctypes.POINTER(_I64)]),
@@ -345,15 +346,18 @@ class Context:
         self.call("pdx_dr_damping", float(dt), ctypes.byref(cn))
         return cn.value
 
-    def vv_run(self, first_step, n_steps, dt, bc_values=None):
+    def vv_run(self, first_step, n_steps, dt, bc_values=None, noalloc=False):
+        """pdx_vv_run, or with noalloc=True pdx_vv_run_noalloc (the BC value table staged in
+        the library's fixed device array, no device allocation)."""
         bv = None if bc_values is None else np.ascontiguousarray(bc_values, np.float64)
         nan_step = _I64(-1)
-        st = self.lib.pdx_vv_run(self.h, int(first_step), int(n_steps), float(dt), ptr(bv),
-                                 ctypes.byref(nan_step))
+        name = "pdx_vv_run_noalloc" if noalloc else "pdx_vv_run"
+        st = getattr(self.lib, name)(self.h, int(first_step), int(n_steps), float(dt), ptr(bv),
+                                     ctypes.byref(nan_step))
         if st == PDX_ERR_NAN:
             msg = self.lib.pdx_last_error(self.h).decode()
             raise NaNError(st, msg, step=nan_step.value, t=nan_step.value * dt)
-        check(self.h, st, "pdx_vv_run")
+        check(self.h, st, name)
 
     def set_displacement_bcs(self, points, dims, values):
         points = np.ascontiguousarray(points, np.int32)

@@ -83,6 +83,31 @@ def reference_layout_bytes(kind, n_points, n_bonds, precision="fp64"):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "bb_step_2M_hbm_traffic.md")
 
 
+def copy_calibration(gib=1, reps=3):
+    """A plain device-to-device copy of `gib` GiB, `reps` times (torch on the HIP device):
+    the reference kernel for the HBM traffic counters (rocprofv3 --pmc FETCH_SIZE /
+    WRITE_SIZE of this copy against its known bytes calibrates the counters read for the
+    step kernels) and the practical HBM ceiling (read + write bytes / copy time, HIP events
+    on torch's stream, on which the copies run)."""
+    import torch
+    n = (gib << 30) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)  # warm-up
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    for _ in range(reps):
+        dst.copy_(src)
+    ev[1].record()
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / reps
+    moved = 2 * n * 8
+    return {"bytes_read_and_written": moved, "ms_per_copy": ms,
+            "achieved_GBs": moved / (ms * 1e-3) / 1e9,
+            "frac_of_8TBs": moved / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+
 def measured_traffic(precision, n_points, n_bonds):
     """HBM bytes per launch of the step kernel for this workload from the committed counter
     summary (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by the copy-kernel
@@ -156,7 +181,10 @@ def jfnk_leg(shape, train_steps):
         log(f"[bench] jfnk {pre}: {wall:.3f} s, krylov {res[pre]['krylov_iters']}")
         dh.close()
     plain = res["jacobi"]
-    return {"workload": (f"bond-based block {shape[0]}x{shape[1]}x{shape[2]}, bottom fixed, top "
+    summary = {"plain_s": plain["seconds"], "two_level_s": res["two_level"]["seconds"],
+               "learned_s": res["learned"]["seconds"],
+               "learned_vs_plain": plain["seconds"] / res["learned"]["seconds"]}
+    return {"summary": summary, "workload": (f"bond-based block {shape[0]}x{shape[1]}x{shape[2]}, bottom fixed, top "
                          f"displaced 1e-4 m in y over 10 load steps, Newton rel tol 1e-8, GMRES(30)"),
             "points": shape[0] * shape[1] * shape[2], **res,
             "speedup_learned_vs_plain": plain["seconds"] / res["learned"]["seconds"],
@@ -481,6 +509,27 @@ def cpu_baseline(sample_shape, target_seconds=12.0, max_steps=5000):
                        f"({body.n_points} points, {nb} bonds), {steps} steps in {elapsed:.1f} s")}
 
 
+def summary_line(out):
+    """One short line with the headline figures of every leg that ran (the JSON line is
+    long; a log that keeps only its tail still keeps this)."""
+    parts = [f"BB fp64 {out['ms_per_step']:.4f} ms/step {out['value']:.3e} bond/s "
+             f"{100 * out['roofline']['frac']:.1f}% HBM"]
+    m = out.get("mixed")
+    if m:
+        parts.append(f"BB mixed {m['ms_per_step']:.4f} ms/step x{m['speedup_vs_fp64_step']:.2f}")
+    sb = out.get("state_based") or {}
+    for k in ("OSB_2M", "OSB_2M_mixed", "OSB", "CPD", "CPD_mixed", "CPD_ZEMWan"):
+        if k in sb:
+            parts.append(f"{k} {sb[k]['ms_per_step']:.3f} ms")
+    j = out.get("jfnk")
+    if j:
+        parts.append("JFNK " + ", ".join(f"{k} {v:.3g}" for k, v in j.get("summary", {}).items()))
+    c = out.get("cpu_baseline")
+    if c:
+        parts.append(f"CPU {c['value']:.3e} bond/s on {c['cores']} cores")
+    return "[bench] summary: " + "; ".join(parts)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -497,7 +546,12 @@ def main():
                     help="skip the state-based (512k-point bar) and mode-I plate legs")
     ap.add_argument("--jfnk-shape", type=str, default="100,100,10")
     ap.add_argument("--jfnk-train-steps", type=int, default=150)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the 1 GiB device copy that calibrates the HBM counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        print(json.dumps({"copy_calibration": copy_calibration()}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
               "sharding of this path is a design note, not a measured configuration.",
@@ -655,6 +709,7 @@ def main():
     if not args.no_cpu:
         log("[bench] CPU baseline")
         out["cpu_baseline"] = cpu_baseline(tuple(int(s) for s in args.cpu_sample.split(",")))
+    log(summary_line(out))
     print(json.dumps(out), flush=True)
 
 

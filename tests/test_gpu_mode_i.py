@@ -82,3 +82,70 @@ def test_mode_i_mixed_vs_fp64(need_gpu):
     assert diff <= max(2, b64 // 100)
     if not np.isnan(t64[0]):
         assert abs(tmx[0] - t64[0]) <= geo["delta"]
+
+
+def _plate200_run(precision, steps, checkpoints, perturb=0.0, fail=True):
+    """The mode-I plate at 200 x 200 x 4 points (dx 2e-4 m, the configuration's spacing):
+    displacement, per-point active-bond counts and crack tip at each checkpoint step.
+    perturb > 0 moves every point by +-perturb m (random signs) before step 1."""
+    body, geo = pd.mode_i_plate(npxy=200, nz=4, dx=2e-4, v=2.0)
+    if not fail:
+        pd.no_failure(body)
+    dh = pd.MI355XBodyDataHandler(body, precision=precision)
+    vv = pd.VelocityVerlet(steps=steps)
+    dh.init_time_solver(vv)
+    dh.initialize(vv)
+    if perturb:
+        du = perturb * np.random.default_rng(1).choice([-1.0, 1.0], size=(body.n_points, 3))
+        dh.set("position", dh.get("position") + du)
+        dh.set("displacement", dh.get("displacement") + du)
+    bcv = dh.bc_values(range(1, steps + 1), vv.dt)
+    out, n = {}, 1
+    for c in checkpoints:
+        dh.ctx.vv_run(n, c - n + 1, vv.dt, bcv[n - 1:c])
+        n = c + 1
+        out[c] = (dh.get("displacement"), dh.get("n_active_bonds").astype(np.int64),
+                  tip(body, dh, geo))
+    dh.close()
+    return out, geo
+
+
+def test_mode_i_mixed_divergence_is_fracture_chaos(need_gpu):
+    """Mixed against FP64 on the mode-I plate at the configuration's spacing, 2000 steps,
+    against the chaos baseline of an FP64 run whose points start displaced by +-1e-11 m
+    (the size of the mixed run's drift by step 500).
+    - Before the crack runs (step 250), the mixed run breaks exactly the FP64 bonds and its
+      displacement agrees to 1e-6.
+    - Without failure, the mixed drift stays below 2e-6 of the displacement after 2000 steps:
+      the FP32 force arithmetic carries no growing bias.
+    - After fracture, the per-point broken-bond counts of the mixed run differ from FP64
+      by no more than twice as much as those of the perturbed FP64 run do, and the leading
+      crack tip agrees within one horizon: the divergence is the sensitivity of crack
+      branching to any perturbation of that size, not a systematic error."""
+    say("mode I 200x200x4 mixed divergence vs FP64 chaos baseline")
+    ck = [250, 2000]
+    r64, geo = _plate200_run("fp64", 2000, ck)
+    rmx, _ = _plate200_run("mixed", 2000, ck)
+    rpt, _ = _plate200_run("fp64", 2000, ck, perturb=1e-11)
+
+    def cmp(a, b, c):
+        ua, na, ta = a[c]
+        ub, nb, tb = b[c]
+        return (float(np.linalg.norm(ub - ua) / np.linalg.norm(ua)), int(np.abs(na - nb).sum()),
+                abs(tb[0] - ta[0]) if not np.isnan(ta[0]) else None)
+
+    u250, d250, _ = cmp(r64, rmx, 250)
+    say(f"  step 250: mixed displacement difference {u250:.2e}, count difference {d250}")
+    assert u250 < 1e-6 and d250 == 0
+    umx, dmx, tmx = cmp(r64, rmx, 2000)
+    upt, dpt, tpt = cmp(r64, rpt, 2000)
+    say(f"  step 2000: mixed {umx:.2e} / {dmx} / tip {tmx}; perturbed FP64 {upt:.2e} / "
+        f"{dpt} / tip {tpt}")
+    assert dpt > 0  # the chaos baseline diverges
+    assert dmx <= 2 * dpt
+    assert tmx is not None and tmx <= geo["delta"]
+    n64, _ = _plate200_run("fp64", 2000, [2000], fail=False)
+    nmx, _ = _plate200_run("mixed", 2000, [2000], fail=False)
+    drift = cmp(n64, nmx, 2000)[0]
+    say(f"  without failure: mixed drift {drift:.2e} after 2000 steps")
+    assert drift < 2e-6

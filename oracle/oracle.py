@@ -379,11 +379,13 @@ class OracleSim:
                     if np.isfinite(val):
                         field[i, dim] = val
 
-    def vv_run(self, n_steps, dt, bcs=(), first_step=1, check_nans=True):
+    def vv_run(self, n_steps, dt, bcs=(), first_step=1, check_nans=True, fd_bcs=()):
         """solve!(dh, VelocityVerlet): verlet_timestep! for n = first_step .. (velocity_verlet.jl:181-197).
 
         bcs: list of ("sd", points, dim, f(t)) and ("pd", points, dim, f(X, t)) acting on
-        velocity_half (velocity_bc!, boundary_conditions.jl:422-435)."""
+        velocity_half (velocity_bc!, boundary_conditions.jl:422-435); fd_bcs the same on
+        b_ext (forcedensity_bc!, :559-575), both applied by apply_boundary_conditions! after
+        the half step."""
         L = lib()
         ndof = 3 * self.n
         dth = 0.5 * dt
@@ -391,6 +393,7 @@ class OracleSim:
             t = n * dt
             L.orc_update_vel_half(ndof, _p(self.v), _p(self.a), dth, _p(self.vh))
             self._apply_bcs(self.vh, self.X, bcs, t)
+            self._apply_bcs(self.b_ext, self.X, fd_bcs, t)
             L.orc_update_disp_and_pos(ndof, _p(self.vh), dt, _p(self.u), _p(self.x))
             self.force()
             if check_nans and L.orc_has_nan(ndof, _p(self.b_int)):

@@ -71,9 +71,11 @@ mutable struct MI355XBodyDataHandler{B<:AbstractBody} <: AbstractDataHandler
     ctx::Ptr{Cvoid}
     n_points::Int
     n_bonds::Int
-    # velocity_bc! tables: the host evaluates the closures once per step
-    sd::Vector{Any}        # SingleDimBC on :velocity_half
-    pd::Vector{Any}        # PosDepSingleDimBC on :velocity_half
+    # velocity_bc! and time-dependent forcedensity_bc! tables: the host evaluates the
+    # closures once per step, the value row holds one source per SingleDimBC and one per
+    # point of a PosDepSingleDimBC
+    sd::Vector{Any}        # (source, SingleDimBC) on :velocity_half and :b_ext
+    pd::Vector{Any}        # (first source, PosDepSingleDimBC) on :velocity_half and :b_ext
     pd_points::Vector{Vector{Int}}
     n_src::Int
 end
@@ -100,12 +102,22 @@ model_id(::CMaterial{<:LinearElastic}) = Int32(2)
 model_id(::CMaterial{<:NeoHooke}) = Int32(3)
 model_id(_) = Int32(0)
 
+# influence function of OSBMaterial / CMaterial (src/discretization/kernels.jl): the device
+# passes compute linear_kernel δ/L in registers; any other kernel is refused, not replaced
+kernel_id(::BBMaterial) = Int32(0)
+function kernel_id(mat::Union{OSBMaterial,CMaterial})
+    mat.kernel === linear_kernel && return Int32(0)
+    throw(ArgumentError("kernel $(mat.kernel) has no MI355X implementation; " *
+                        "the device passes provide linear_kernel only"))
+end
+
 function pdx_params(body, p)
     wan = body.mat isa CMaterial && body.mat.zem isa ZEMWan
     cs = body.mat isa CMaterial && !wan ? body.mat.zem.Cs : 0.0
     maxdmg = body.mat isa CMaterial ? body.mat.maxdmg : 0.0
     return PdxParams(p.δ, p.rho, p.E, p.nu, p.G, p.K, p.λ, p.μ, p.Gc, p.εc, p.bc,
-                     model_id(body.mat), Int32(0), cs, maxdmg, Int32(wan ? 1 : 0), Int32(0))
+                     model_id(body.mat), kernel_id(body.mat), cs, maxdmg, Int32(wan ? 1 : 0),
+                     Int32(0))
 end
 
 # one parameter set: pdx_set_material; several (point_params + params_map):
@@ -157,8 +169,38 @@ function MI355XBodyDataHandler(body::AbstractBody; device::Int=0, precision::Sym
         d.ctx == C_NULL || ccall((:pdx_destroy, libpdx), Cvoid, (Ptr{Cvoid},), d.ctx)
         d.ctx = C_NULL
     end
-    set_velocity_bcs!(dh)
+    apply_initial_conditions!(dh)
+    set_step_bcs!(dh)
     return dh
+end
+
+# apply_initial_conditions! (src/physics/initial_conditions.jl:86-94, called at chunk
+# creation, threads_body_data_handler.jl:55): velocity_ic! values, then the f(p) ones
+# (a NaN leaves the point alone), written to the device's velocity field
+function apply_initial_conditions!(dh::MI355XBodyDataHandler)
+    body = dh.body
+    isempty(body.single_dim_ics) && isempty(body.posdep_single_dim_ics) && return nothing
+    fields = Dict{Symbol,Matrix{Float64}}()
+    field(f) = get!(() -> zeros(3, dh.n_points), fields, f)
+    for ic in body.single_dim_ics
+        v = field(ic.field)
+        for i in body.point_sets[ic.point_set]
+            v[ic.dim, i] = ic.value
+        end
+    end
+    for ic in body.posdep_single_dim_ics
+        v = field(ic.field)
+        for i in body.point_sets[ic.point_set]
+            value = ic.fun(view(body.position, :, i))
+            isnan(value) || (v[ic.dim, i] = value)
+        end
+    end
+    for (f, v) in fields
+        haskey(FIELDS, f) ||
+            throw(ArgumentError("initial condition on `$f` has no MI355X field"))
+        set_field!(dh, f, v)
+    end
+    return nothing
 end
 
 function get_field(dh::MI355XBodyDataHandler, name::Symbol)
@@ -176,36 +218,58 @@ function set_field!(dh::MI355XBodyDataHandler, name::Symbol, values::AbstractArr
     return nothing
 end
 
-# velocity_bc! (boundary_conditions.jl:422-435): single-dim BCs first, then the
-# position-dependent ones, as apply_boundary_conditions! orders them (:212-226)
-function set_velocity_bcs!(dh::MI355XBodyDataHandler)
+# velocity_bc! (boundary_conditions.jl:422-435) on velocity_half and forcedensity_bc! with
+# f(t) or f(p, t) (:559-575) on b_ext: single-dim BCs first, then the position-dependent
+# ones, as apply_boundary_conditions! orders them (:212-226).  One value row per step holds
+# the sources of both fields (pdx_set_velocity_bcs takes the row length, then
+# pdx_set_force_density_bcs indexes the same rows).  Conditions on any other field, and
+# data-driven conditions, are refused rather than dropped.
+function set_step_bcs!(dh::MI355XBodyDataHandler)
     body = dh.body
-    sd = [bc for bc in body.single_dim_bcs if bc.field === :velocity_half]
-    pd = [bc for bc in body.posdep_single_dim_bcs if bc.field === :velocity_half]
-    src = Dict{Int,NTuple{3,Int}}()
-    put!(i, d, s) = (v = collect(get(src, i, (-1, -1, -1))); v[d] = s; src[i] = Tuple(v))
+    isempty(body.data_bcs) ||
+        throw(ArgumentError("data boundary conditions have no MI355X implementation"))
+    for bc in vcat(body.single_dim_bcs, body.posdep_single_dim_bcs)
+        bc.field in (:velocity_half, :b_ext) ||
+            throw(ArgumentError("boundary condition on `$(bc.field)` has no MI355X implementation"))
+    end
+    sd_all, pd_all, pd_points = Any[], Any[], Vector{Int}[]
+    tables = Dict{Symbol,Tuple{Vector{Int},Vector{Int32}}}()
     k = 0
-    for bc in sd
-        for i in body.point_sets[bc.point_set]
-            put!(i, Int(bc.dim), k)
+    for fld in (:velocity_half, :b_ext)
+        sd = [bc for bc in body.single_dim_bcs if bc.field === fld]
+        pd = [bc for bc in body.posdep_single_dim_bcs if bc.field === fld]
+        src = Dict{Int,NTuple{3,Int}}()
+        put!(i, d, s) = (v = collect(get(src, i, (-1, -1, -1))); v[d] = s; src[i] = Tuple(v))
+        for bc in sd
+            for i in body.point_sets[bc.point_set]
+                put!(i, Int(bc.dim), k)
+            end
+            push!(sd_all, (k, bc))
+            k += 1
         end
-        k += 1
-    end
-    pd_points = Vector{Int}[]
-    for bc in pd
-        pts = body.point_sets[bc.point_set]
-        push!(pd_points, pts)
-        for (q, i) in enumerate(pts)
-            put!(i, Int(bc.dim), k + q - 1)
+        for bc in pd
+            pts = body.point_sets[bc.point_set]
+            push!(pd_all, (k, bc))
+            push!(pd_points, pts)
+            for (q, i) in enumerate(pts)
+                put!(i, Int(bc.dim), k + q - 1)
+            end
+            k += length(pts)
         end
-        k += length(pts)
+        points = sort!(collect(keys(src)))
+        tables[fld] = (points, Int32[s for i in points for s in src[i]])
     end
-    points = sort!(collect(keys(src)))
-    srcv = Int32[s for i in points for s in src[i]]
+    points, srcv = tables[:velocity_half]
     check(dh, ccall((:pdx_set_velocity_bcs, libpdx), Int32,
                     (Ptr{Cvoid}, Int64, Ptr{Int32}, Ptr{Int32}, Int64),
                     dh.ctx, length(points), zero_based(points), srcv, k))
-    dh.sd, dh.pd, dh.pd_points, dh.n_src = sd, pd, pd_points, k
+    points, srcv = tables[:b_ext]
+    if !isempty(points)
+        check(dh, ccall((:pdx_set_force_density_bcs, libpdx), Int32,
+                        (Ptr{Cvoid}, Int64, Ptr{Int32}, Ptr{Int32}),
+                        dh.ctx, length(points), zero_based(points), srcv))
+    end
+    dh.sd, dh.pd, dh.pd_points, dh.n_src = sd_all, pd_all, pd_points, k
     return nothing
 end
 
@@ -214,13 +278,12 @@ function bc_values(dh::MI355XBodyDataHandler, steps, Δt)
     X = dh.body.position
     for (c, n) in enumerate(steps)
         t = n * Δt
-        k = 0
-        for bc in dh.sd
-            vals[k += 1, c] = bc.fun(t)
+        for (k, bc) in dh.sd
+            vals[k + 1, c] = bc.fun(t)
         end
-        for (bc, pts) in zip(dh.pd, dh.pd_points)
-            for i in pts
-                vals[k += 1, c] = bc.fun(view(X, :, i), t)
+        for ((k0, bc), pts) in zip(dh.pd, dh.pd_points)
+            for (q, i) in enumerate(pts)
+                vals[k0 + q, c] = bc.fun(view(X, :, i), t)
             end
         end
     end

@@ -6,7 +6,7 @@ and point indices are 0-based."""
 from ._lib import (ArgumentError, DuplicatePointError, NaNError, NotConvergedError, PdxError,
                    load as load_library)
 from .body import (Body, displacement_bc, failure_permit, forcedensity_bc, material,
-                   n_points, no_failure, point_set, precrack, velocity_bc)
+                   n_points, no_failure, point_set, precrack, velocity_bc, velocity_ic)
 from .data_handler import MI355XBodyDataHandler
 from .job import Job, submit
 from .materials import (BBMaterial, CMaterial, CriticalStretch, EnergySurfaceCorrection,
@@ -20,7 +20,8 @@ from .vtk import VTKExporter, process_each_export, read_vtk, write_pvtu, write_v
 __all__ = [
     "ArgumentError", "DuplicatePointError", "NaNError", "NotConvergedError", "PdxError",
     "load_library", "Body", "displacement_bc", "failure_permit", "forcedensity_bc", "material",
-    "n_points", "no_failure", "point_set", "precrack", "velocity_bc", "MI355XBodyDataHandler",
+    "n_points", "no_failure", "point_set", "precrack", "velocity_bc", "velocity_ic",
+    "MI355XBodyDataHandler",
     "Job", "submit", "BBMaterial", "CMaterial", "CriticalStretch", "NeoHookePenalty",
     "OSBMaterial", "SaintVenantKirchhoff", "LinearElastic", "NeoHooke", "ZEMSilling", "ZEMWan", "linear_kernel", "jitter",
     "lattice_block", "uniform_box", "NewtonKrylov", "VelocityVerlet", "crack_tip",

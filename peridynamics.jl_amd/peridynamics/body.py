@@ -71,6 +71,8 @@ class Body:
         self.posdep_single_dim_bcs = []                     # (field, set, dim, f(p, t))
         self.pos_single_dim_bcs = []                        # (field, set, dim, f(p))
         self.point_sets_precracks = []                      # (set_a, set_b, filter_bonds)
+        self.single_dim_ics = []                            # (field, set, dim, value)
+        self.posdep_single_dim_ics = []                     # (field, set, dim, f(p))
 
     @property
     def xyz(self):
@@ -205,6 +207,61 @@ def displacement_bc(f, body, set_name, dim):
     if condition_function_type(f) != "psdbc":
         raise _arg_error("the function for a displacement boundary condition must be of the form\n`f(p)`!")
     _add_bc(body, body.pos_single_dim_bcs, "displacement", set_name, get_dim(dim), f)
+
+
+def _ic_conflict(body, field, set_name, dim):
+    """has_initial_condition_conflict: initial_conditions.jl:123-131 (same field, same
+    dimension, intersecting point sets)."""
+    pts = body.point_sets[set_name]
+    for lst in (body.single_dim_ics, body.posdep_single_dim_ics):
+        for fld, s, d, _ in lst:
+            if fld == field and d == dim and np.intersect1d(pts, body.point_sets[s]).size:
+                return True
+    return False
+
+
+def velocity_ic(*args):
+    """velocity_ic!(body, set_name, dim, value) / velocity_ic!(f, body, set_name, dim):
+    initial_conditions.jl:178-193.  The velocity of the set's points before time
+    integration; f(p) gets the reference position of a point, and a NaN value leaves the
+    point alone (apply_ic!, :71-83).  Applied when the data handler is built, as
+    apply_initial_conditions! is at chunk creation (threads_body_data_handler.jl:55)."""
+    if len(args) != 4:
+        raise _arg_error("velocity_ic(body, set_name, dim, value) or velocity_ic(f, body, set_name, dim)")
+    if callable(args[0]):
+        f, body, set_name, dim = args
+        _check_set(body, set_name)
+        try:
+            names = [q.name for q in inspect.signature(f).parameters.values()]
+        except (TypeError, ValueError):
+            names = []
+        if names != ["p"]:
+            raise _arg_error("wrong arguments for position dependent initial condition function!\n"
+                             "Initial conditions support only functions of type `f(p)`, where "
+                             "`p` is the position vector [x, y, z] of each point in the point set!")
+        lst, item = body.posdep_single_dim_ics, f
+    else:
+        body, set_name, dim, value = args
+        _check_set(body, set_name)
+        lst, item = body.single_dim_ics, float(value)
+    d = get_dim(dim)
+    if _ic_conflict(body, "velocity", set_name, d):
+        raise _arg_error("the specified condition conflicts with already existing conditions!")
+    lst.append(("velocity", set_name, d, item))
+
+
+def initial_velocity(body):
+    """The velocity field (n, 3) after apply_initial_conditions! (initial_conditions.jl:
+    86-94): single-dim conditions first, then the position-dependent ones; zero elsewhere."""
+    v = np.zeros((body.n_points, 3))
+    for _, s, d, val in body.single_dim_ics:
+        v[body.point_sets[s], d] = val
+    for _, s, d, f in body.posdep_single_dim_ics:
+        pts = body.point_sets[s]
+        vals = eval_posdep(f, body.position[:, pts])
+        keep = ~np.isnan(vals)
+        v[pts[keep], d] = vals[keep]
+    return v
 
 
 def eval_posdep(f, X, t=None):

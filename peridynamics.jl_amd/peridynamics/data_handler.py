@@ -9,7 +9,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import ArgumentError, Context, NaNError, PdxNkOpts, PdxNkStats, PdxParams
-from .body import eval_posdep
+from .body import eval_posdep, initial_velocity
 from .materials import CMaterial
 
 
@@ -58,6 +58,8 @@ class MI355XBodyDataHandler:
         for a, b, filt in body.point_sets_precracks:
             self.ctx.add_precrack(body.point_sets[a], body.point_sets[b], filt)
         self.n_bonds = self.ctx.build_bonds()
+        if body.single_dim_ics or body.posdep_single_dim_ics:
+            self.ctx.set_field("velocity", initial_velocity(body))
         self._compile_bcs()
         self.dt = None
 

@@ -111,6 +111,35 @@ def test_boundary_condition_checks():
         pd.velocity_bc(lambda t: 1.0, b, "top", 4)
 
 
+def test_velocity_ic_checks_and_field():
+    """velocity_ic! (initial_conditions.jl:178-193): a value or an f(p); conflicts with an
+    existing condition of the same dimension on intersecting points, and functions with
+    other arguments, are ArgumentErrors; apply_initial_conditions! (:86-94) sets the
+    single-dim conditions first, then the position-dependent ones, skipping NaN."""
+    pos, vol = pd.uniform_box(1, 1, 1, 0.5)
+    b = pd.Body(pd.BBMaterial(), pos, vol)
+    pd.material(b, horizon=2, rho=1, E=1)
+    pd.point_set(b, "top", lambda p: p[2] > 0)
+    pd.point_set(b, "left", lambda p: p[0] < 0)
+    pd.velocity_ic(b, "top", "z", 3.0)
+    with pytest.raises(pd.ArgumentError):  # same dimension on intersecting points
+        pd.velocity_ic(b, "all_points", 3, 1.0)
+    with pytest.raises(pd.ArgumentError):
+        pd.velocity_ic(lambda p, t: 1.0, b, "left", "x")
+    with pytest.raises(pd.ArgumentError):
+        pd.velocity_ic(b, "nope", "x", 1.0)
+    pd.velocity_ic(lambda p: np.nan if p[1] > 0 else 10.0 * p[0], b, "left", "x")
+    pd.velocity_ic(b, "all_points", "y", -1.0)
+    v = pd.body.initial_velocity(b)
+    top, left = b.point_sets["top"], b.point_sets["left"]
+    assert np.all(v[top, 2] == 3.0) and np.all(np.delete(v[:, 2], top) == 0)
+    assert np.all(v[:, 1] == -1.0)
+    X = b.position
+    want = np.where(X[1, left] > 0, 0.0, 10.0 * X[0, left])
+    np.testing.assert_array_equal(v[left, 0], want)
+    assert np.all(np.delete(v[:, 0], left) == 0)
+
+
 def test_eval_posdep_vectorised_and_scalar():
     from peridynamics.body import eval_posdep
     X = np.arange(12, dtype=float).reshape(3, 4)

@@ -103,8 +103,9 @@ typedef struct pdx_nk_opts {
     int32_t gmres_maxiter;    /* total Krylov iterations per linear solve */
     int32_t gmres_restart;    /* restart length */
     int32_t precond;          /* 0 none (the reference), 1 point-block Jacobi, 2 learned
-                                 (two-level + message-passing network, flexible), 3 two-level
-                                 (Jacobi + Galerkin coarse correction on point aggregates) */
+                                 (two-level whose coarse space adds the message-passing
+                                 network's fields), 3 two-level (Jacobi + Galerkin coarse
+                                 correction by the translations of point aggregates) */
     double tol;               /* absolute tolerance on ||r|| (reference semantics) */
     double rel_tol;           /* > 0: stop at ||r|| < rel_tol * ||r_0|| of the load step */
     double gmres_reltol, gmres_abstol;
@@ -117,7 +118,7 @@ typedef struct pdx_nk_stats {
     int32_t jv_evals;
     int32_t precond_fallbacks; /* learned solves that missed the tolerance without reducing the
                                   linear residual tenfold, redone with the two-level
-                                  preconditioner (block Jacobi without a coarse space) */
+                                  preconditioner of the translations */
     double residual_first, residual_final;
     double seconds;
 } pdx_nk_stats;
@@ -256,6 +257,10 @@ pdx_status pdx_set_displacement_bcs(pdx_ctx *ctx, int64_t n, const int32_t *poin
 pdx_status pdx_jv(pdx_ctx *ctx, const double *v, double *Jv, double perturbation_scale);
 /* Residual r = (b_int + b_ext) V, constrained dofs zero (calc_residual!, :371-387). */
 pdx_status pdx_residual(pdx_ctx *ctx, double *r, double *norm_out);
+/* The aggregates of the two-level preconditioner (no counterpart in the reference): runs of
+ * *agg consecutive internal (Morton) points, *n_aggregates of them, and the coarse fields
+ * per aggregate of the last coarse space built (3 translations, + 8 learned; 0 before). */
+pdx_status pdx_nk_coarse_info(pdx_ctx *ctx, int32_t *agg, int64_t *n_aggregates, int32_t *fields);
 /* One load step n of n_steps (newton_krylov_step!, newton_krylov.jl:298-332): applies the
  * scaled displacement BCs, then Newton iterations with a device-resident flexible GMRES
  * (replacing IterativeSolvers.gmres!, :448-450). */
@@ -265,40 +270,52 @@ pdx_status pdx_nk_step(pdx_ctx *ctx, int64_t n, int64_t n_steps, const pdx_nk_op
 /* --- learned preconditioner (no counterpart in the reference) ---------------------------- */
 /* The reference solves J du = -r with IterativeSolvers.gmres! and no preconditioner
  * (newton_krylov.jl:448-450).  These entry points build, train and expose the
- * message-passing preconditioner that pdx_nk_step uses with opts.precond = 2.
+ * message-passing network behind opts.precond = 2: the network maps the reference
+ * configuration (positions, constraints, bond counts) to 8 vector fields per point, and the
+ * learned preconditioner is the two-level preconditioner whose coarse space on every
+ * aggregate spans the three translations and those 8 fields.  BBMaterial bodies of at most
+ * 131072 points.
  *
- * While collection is enabled, every pdx_nk_step records its Newton pairs (-r, du) and up
- * to max_krylov_pairs Krylov pairs (J z, z) (reservoir-sampled) as float32 on the device. */
+ * While collection is enabled, every Krylov pair (J z, z) of a solve preconditioned by the
+ * two-level preconditioner of the translations (M_T) yields the error sample
+ * e = z - M_T(J z), reservoir-sampled into at most max_krylov_pairs float32 vectors on the
+ * device. */
 pdx_status pdx_gnn_collect(pdx_ctx *ctx, int32_t enable, int64_t max_krylov_pairs);
+/* n_newton is always 0; n_krylov = the error samples held. */
 pdx_status pdx_gnn_pairs(pdx_ctx *ctx, int64_t *n_newton, int64_t *n_krylov);
-/* `steps` Adam steps (cosine-decayed step size lr) on batches of `batch` pairs drawn from
- * the collected ones (and random combinations of two); *loss_out = last batch loss. */
+/* `steps` Adam steps (cosine-decayed step size lr) on random subsets of `batch` samples;
+ * the last tenth of the samples is held out.  Loss per sample and aggregate I: the ridge
+ * least-squares residual of e on I by [translations | fields] over |e|^2.  *loss_out = the
+ * loss of the last reported step. */
 pdx_status pdx_gnn_train(pdx_ctx *ctx, int32_t steps, int32_t batch, double lr, uint64_t seed,
                          double *loss_out);
-/* Held-out validation of the trained network (no reference counterpart).  pdx_gnn_train
- * keeps the last tenth of the Krylov pairs out of training and measures on them the loss
- * ||c s O - e||^2 / ||e||^2 of the two-level preconditioner's error e (1 = the network
- * explains nothing).  The learned preconditioner applies the network only while that loss
- * is below `threshold` (0.85 after training) and until a learned solve needs the fallback
- * (pdx_nk_stats.precond_fallbacks); otherwise it is the two-level preconditioner alone.
- * threshold NaN only reports. */
+/* Held-out validation: *val_loss = the held-out loss with the fields over the loss of the
+ * translations alone (1 = the fields add nothing).  The learned coarse space is used while
+ * that ratio is below `threshold` (0.95 after training) and until a learned solve needs the
+ * fallback (pdx_nk_stats.precond_fallbacks); otherwise the preconditioner is the two-level
+ * one of the translations.  threshold NaN only reports. */
 pdx_status pdx_gnn_gate(pdx_ctx *ctx, double threshold, double *val_loss, int32_t *use_network);
 /* Number of float32 parameters of the network. */
 int64_t pdx_gnn_n_params(void);
-/* Reads (theta_out) or replaces (theta_in, with the output scale *out_scale) the
- * parameters; replacing marks the network as trained. */
+/* Reads (theta_out) or replaces (theta_in) the parameters; replacing marks the network as
+ * trained.  *out_scale is returned as 1 (the coarse solve fixes the scale of the fields). */
 pdx_status pdx_gnn_params(pdx_ctx *ctx, float *theta_out, const float *theta_in,
                           double *out_scale);
 /* Test access, all in the library's internal (Morton) point order: permutation
- * perm[internal] = caller index, near-bond graph (CSR), static features (5 per point),
- * point-block Jacobi inverse at the current state (9 per point, column-major), constrained
- * mask (3 per point).  Any pointer may be NULL; call with only n_near first. */
+ * perm[internal] = caller index, near-bond graph (CSR), the network's 16 input features per
+ * point, point-block Jacobi inverse at the current state (9 per point, column-major),
+ * constrained mask (3 per point).  Any pointer may be NULL; call with only n_near first. */
 pdx_status pdx_gnn_graph(pdx_ctx *ctx, int64_t *n_near, int64_t *perm, int64_t *near_ptr,
-                         int32_t *near, float *stat, double *dinv, uint8_t *cmask);
-/* z = M(y) for one 3n vector (internal order). */
+                         int32_t *near, float *features, double *dinv, uint8_t *cmask);
+/* z = M(y), the learned two-level preconditioner at the current state, for one 3n vector
+ * (internal order). */
 pdx_status pdx_gnn_apply_vec(pdx_ctx *ctx, const double *y, double *z);
-/* Loss and gradient of the current parameters over n_samples pairs (Y, X: n_samples rows of
- * 3n, internal order) without updating them. */
+/* The network's fields for the current parameters: 24 floats per point (8 fields x 3),
+ * internal order. */
+pdx_status pdx_gnn_field_values(pdx_ctx *ctx, float *fields);
+/* Loss and gradient of the current parameters over n_samples error samples (Y: n_samples
+ * rows of 3n, internal order; X unused) without updating them.  The samples replace the
+ * collected ones. */
 pdx_status pdx_gnn_loss_grad(pdx_ctx *ctx, int32_t n_samples, const double *Y, const double *X,
                              double *loss, float *grad);
 

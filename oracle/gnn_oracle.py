@@ -3,25 +3,28 @@
 The preconditioner has no counterpart in Peridynamics.jl: the reference solves J du = -r
 with IterativeSolvers.gmres! and no preconditioner (src/time_solvers/newton_krylov.jl:
 448-450).  This module restates, in float64 numpy, the network that libpdx evaluates and
-trains on the GPU, so that the device forward pass, loss and gradients can be checked.
-The gradients of this module are themselves pinned by central finite differences
-(tests/test_gnn_oracle.py).
+trains on the GPU and the loss it is trained on, so that the device forward pass, loss and
+gradients can be checked.  The gradients of this module are themselves pinned by central
+finite differences (tests/test_gnn_oracle.py).
 
 Network (per point i of the body, in the library's internal Morton order):
-  inputs   F_i (16): D^-1 y / s (3), y / rms(y) (3), constrained mask (3),
-           normalised reference position (3), near-degree ratio (1), bond-count ratio (1),
-           two zeros; s = rms of D^-1 y over free dofs
+  inputs   F_i (16): normalised reference position (3), position relative to the point's
+           aggregate (3), constrained mask (3), near-degree ratio (1), bond-count ratio (1),
+           squares of the relative position (3), 1, 0
   encoder  H_i = relu(W_e^T F_i + b_e)                                   (16 -> 64)
   3 layers A_i = mean_{j in N(i)} H_j          (near bonds: L <= 0.6 delta)
            P_i, Q_i, G = mean of H over the 64-point block, the 4096-point block of i
                          (consecutive internal indices) and the whole body
            Z_i = W_l^T [H_i, A_i, P_i, Q_i, G] + b_l                      (320 -> 64)
            H_i <- H_i + relu(Z_i)
-  decoder  O_i = W_d^T H_i + b_d                                          (64 -> 3)
-  output   z = D^-1 y + c s O on free dofs, z = y on constrained dofs
+  decoder  O_i = W_d^T H_i + b_d                                          (64 -> 24)
+  fields   phi_i[f] = O_i[3f : 3f + 3], f < 8
 
-Training pair (y, x) with J x = y: target t = (x - D^-1 y) / (c s) on free dofs, loss
-sum_free (O - t)^2 / sum_free t^2, averaged over the pairs of a batch.
+Aggregates are runs of `agg` consecutive internal indices (agg: a power of two >= 64 with at
+most 256 aggregates).  The coarse basis of aggregate I is B_I = [e_x, e_y, e_z, phi_0 ..
+phi_7] on I's free dofs.  Loss over error samples e_s:
+  (1/S) sum_s w_s sum_I min_c ( |e_sI - B_I c|^2 + lambda_I |c|^2 ),
+  w_s = 1 / |e_s|^2 (free dofs), lambda_I = 1e-6 tr(B_I^T B_I) / 11 + 1e-30.
 
 The device stores H, the concatenated layer inputs and the three layer weight matrices in
 bfloat16 and accumulates in float32 (encoder and decoder weights stay float32);
@@ -31,6 +34,9 @@ import numpy as np
 
 F_IN = 16
 F = 64
+K_F = 8
+N_OUT = 3 * K_F
+RIDGE = 1e-6
 K_CAT = 5 * F
 N_LAYERS = 3
 BLK1 = 64
@@ -42,7 +48,7 @@ def param_layout():
     lay = [("W_e", (F_IN, F)), ("b_e", (F,))]
     for l in range(1, N_LAYERS + 1):
         lay += [(f"W_{l}", (K_CAT, F)), (f"b_{l}", (F,))]
-    lay += [("W_d", (F, 3)), ("b_d", (3,))]
+    lay += [("W_d", (F, N_OUT)), ("b_d", (N_OUT,))]
     return lay
 
 
@@ -65,16 +71,27 @@ def flatten(p):
 
 
 def init_params(seed):
-    """He-style initialisation; the decoder starts at zero (z = D^-1 y, block Jacobi)."""
+    """He-style initialisation; a small random decoder (zero fields get no gradient)."""
     rng = np.random.default_rng(seed)
     p = {}
     for name, shape in param_layout():
-        if name.startswith("W_") and name != "W_d":
+        if name == "W_d":
+            p[name] = rng.standard_normal(shape) * np.sqrt(1.0 / shape[0])
+        elif name.startswith("W_"):
             p[name] = rng.standard_normal(shape) * np.sqrt(2.0 / shape[0]) * (
                 0.5 if name != "W_e" else 1.0)
         else:
             p[name] = np.zeros(shape)
     return flatten(p)
+
+
+def agg_size(n):
+    """Aggregate size of the two-level preconditioner: a power of two >= 64, at most 256
+    aggregates."""
+    agg = 64
+    while (n + agg - 1) // agg > 256:
+        agg *= 2
+    return agg
 
 
 def bf16(a):
@@ -137,33 +154,35 @@ def near_graph(row_ptr, nbr, L, delta, frac=0.6):
     return ptr, nbr[keep].astype(np.int64)
 
 
-def static_features(X, row_ptr, near_ptr):
-    """Per point: normalised position (3), near-degree ratio, bond-count ratio."""
+def input_features(X, row_ptr, near_ptr, cmask, agg):
+    """The 16 input features per point (X, the mask and the rows in internal order)."""
+    n = X.shape[0]
     lo, hi = X.min(0), X.max(0)
     half = np.maximum(0.5 * (hi - lo), 1e-300)
     Xh = (X - 0.5 * (hi + lo)) / half
     deg = np.diff(near_ptr).astype(np.float64)
     cnt = np.diff(row_ptr).astype(np.float64)
-    return np.column_stack([Xh, deg / max(deg.max(), 1.0), cnt / max(cnt.max(), 1.0)])
+    rel = np.zeros_like(X)
+    for I in range((n + agg - 1) // agg):
+        sl = slice(I * agg, min(n, (I + 1) * agg))
+        cen = X[sl].mean(0)
+        h = max(np.abs(X[sl] - cen).max(), 1e-300)
+        rel[sl] = (X[sl] - cen) / h
+    Fi = np.zeros((n, F_IN))
+    Fi[:, 0:3] = Xh
+    Fi[:, 3:6] = rel
+    Fi[:, 6:9] = cmask
+    Fi[:, 9] = deg / max(deg.max(), 1.0)
+    Fi[:, 10] = cnt / max(cnt.max(), 1.0)
+    Fi[:, 11:14] = rel ** 2
+    Fi[:, 14] = 1.0
+    return Fi
 
 
 def apply_dinv(dinv, y):
     """Block Jacobi: q_i = R_i y_i, R_i stored column-major 3x3 (9 per point)."""
     R = dinv.reshape(-1, 3, 3)  # R[i, col, row]
     return np.einsum("icr,ic->ir", R, y.reshape(-1, 3))
-
-
-def input_features(y, q, cmask, stat):
-    free = ~cmask
-    s = np.sqrt(np.sum(q[free] ** 2) / max(free.sum(), 1)) + 1e-300
-    ry = np.sqrt(np.sum(y[free] ** 2) / max(free.sum(), 1)) + 1e-300
-    n = y.shape[0]
-    Fi = np.zeros((n, F_IN))
-    Fi[:, 0:3] = np.where(free, q / s, 0.0)
-    Fi[:, 3:6] = np.where(free, y / ry, 0.0)
-    Fi[:, 6:9] = cmask
-    Fi[:, 9:14] = stat
-    return Fi, s
 
 
 def forward(theta, Fi, near_ptr, near, bf16_mode=False, keep=False):
@@ -218,35 +237,58 @@ def backward(theta, cache, dO, near_ptr, near, bf16_mode=False):
     return flatten(g)
 
 
-def sample_target(y, x, q, cmask, s, c):
+def basis(O, cmask, use_fields=True):
+    """B per point: (n, 3 dofs, 11 columns): translations, then the 8 fields, zero on
+    constrained dofs."""
+    n = O.shape[0]
+    B = np.zeros((n, 3, 3 + K_F))
+    for d in range(3):
+        B[:, d, d] = 1.0
+    if use_fields:
+        B[:, :, 3:] = O.reshape(n, K_F, 3).transpose(0, 2, 1)
+    return np.where(cmask[:, :, None], 0.0, B)
+
+
+def lsq_loss(O, E, cmask, agg, use_fields=True):
+    """The loss of the samples E (S, n, 3) for the fields O (n, 24) and its gradient dO."""
+    n = O.shape[0]
+    S = E.shape[0]
     free = ~cmask
-    return np.where(free, (x - q) / (c * s), 0.0)
-
-
-def loss_and_grad(theta, samples, graph, c, bf16_mode=False, want_grad=True):
-    """samples: list of (y, x) with y, x of shape (n, 3); graph: dict with near_ptr, near,
-    dinv, cmask (n, 3 bool), stat (n, 5).  Returns (loss, grad)."""
-    near_ptr, near = graph["near_ptr"], graph["near"]
+    Bp = basis(O, cmask, use_fields)
+    dB = np.zeros_like(Bp)
+    w = np.array([1.0 / max(np.sum(np.where(free, e, 0.0) ** 2), 1e-300) for e in E]) / S
     total = 0.0
-    grad = np.zeros(n_params())
-    for y, x in samples:
-        q = apply_dinv(graph["dinv"], y)
-        Fi, s = input_features(y, q, graph["cmask"], graph["stat"])
-        t = sample_target(y, x, q, graph["cmask"], s, c)
-        O, cache = forward(theta, Fi, near_ptr, near, bf16_mode, keep=True)
-        free = ~graph["cmask"]
-        den = max(np.sum(t[free] ** 2), 1e-300)
-        r = np.where(free, O - t, 0.0)
-        total += np.sum(r ** 2) / den
-        if want_grad:
-            grad += backward(theta, cache, 2.0 * r / den, near_ptr, near, bf16_mode)
-    B = len(samples)
-    return total / B, grad / B
+    nb = 3 + K_F
+    for I in range((n + agg - 1) // agg):
+        sl = slice(I * agg, min(n, (I + 1) * agg))
+        B = Bp[sl].reshape(-1, nb)
+        G = B.T @ B
+        lam = RIDGE * np.trace(G) / nb + 1e-30
+        A = G + lam * np.eye(nb)
+        for s in range(S):
+            e = np.where(cmask[sl], 0.0, E[s, sl]).reshape(-1)
+            c = np.linalg.solve(A, B.T @ e)
+            r = e - B @ c
+            total += w[s] * (r @ r + lam * (c @ c))
+            g = -2.0 * w[s] * np.outer(r, c) + w[s] * (2.0 * RIDGE / nb * (c @ c)) * B
+            dB[sl] += g.reshape(-1, 3, nb)
+    dB = np.where(cmask[:, :, None], 0.0, dB)
+    dO = dB[:, :, 3:].transpose(0, 2, 1).reshape(n, N_OUT)
+    return total, (dO if use_fields else np.zeros_like(dO))
 
 
-def apply(theta, y, graph, c, bf16_mode=False):
-    """z = M(y) (the preconditioner)."""
-    q = apply_dinv(graph["dinv"], y)
-    Fi, s = input_features(y, q, graph["cmask"], graph["stat"])
-    O = forward(theta, Fi, graph["near_ptr"], graph["near"], bf16_mode)
-    return np.where(graph["cmask"], y, q + c * s * O)
+def loss_and_grad(theta, E, graph, bf16_mode=False, want_grad=True):
+    """E: error samples (S, n, 3); graph: dict with near_ptr, near, cmask (n, 3 bool), feat
+    (n, 16), agg.  Returns (loss, grad)."""
+    O, cache = forward(theta, graph["feat"], graph["near_ptr"], graph["near"], bf16_mode,
+                       keep=True)
+    loss, dO = lsq_loss(O, E, graph["cmask"], graph["agg"])
+    if not want_grad:
+        return loss, None
+    return loss, backward(theta, cache, dO, graph["near_ptr"], graph["near"], bf16_mode)
+
+
+def fields(theta, graph, bf16_mode=False):
+    """The 8 fields per point, (n, 8, 3)."""
+    O = forward(theta, graph["feat"], graph["near_ptr"], graph["near"], bf16_mode)
+    return O.reshape(-1, K_F, 3)

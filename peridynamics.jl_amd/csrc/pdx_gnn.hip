@@ -1,25 +1,31 @@
 // Learned preconditioner of the device Newton-Krylov solver: a message-passing network on
-// the bond graph, evaluated with BF16 MFMA (FP32 accumulation), trained on the GPU by its
-// own backward kernels and Adam.  No counterpart in the reference, which calls
-// IterativeSolvers.gmres! without a preconditioner (src/time_solvers/newton_krylov.jl:
-// 448-450); it plugs into the flexible GMRES of pdx_nk.hip as a right preconditioner, with
-// the point-block Jacobi solve as fallback.
+// the bond graph that generates the coarse space of the two-level preconditioner.  It is
+// evaluated with BF16 MFMA (FP32 accumulation) and trained on the GPU by its own backward
+// kernels and Adam.  No counterpart in the reference, which calls IterativeSolvers.gmres!
+// without a preconditioner (src/time_solvers/newton_krylov.jl:448-450); the coarse space it
+// builds plugs into the flexible GMRES of pdx_nk.hip as a right preconditioner, with the
+// two-level preconditioner of the translations alone as fallback.
 //
 // Network, per point i (internal Morton order), theta in float32 (master copy):
-//   F_i (16)  = [D^-1 y / s, y / rms(y), constrained mask, normalised X, near-degree
-//               ratio, bond-count ratio, 0, 0]      (s = rms of D^-1 y over free dofs)
+//   F_i (16)  = [normalised X (3), X relative to its aggregate (3), constrained mask (3),
+//               near-degree ratio, bond-count ratio, squares of the relative X (3), 1, 0]
 //   H_i       = relu(W_e^T F_i + b_e)                                     16 -> 64
 //   3 layers:   A_i = mean_{j in N(i)} H_j   (near bonds: L <= 0.6 delta, symmetric)
 //               P_i, Q_i, G = means of H over the 64-point and 4096-point blocks of i and
 //                             over the body
 //               Z_i = W_l^T [H_i A_i P_i Q_i G] + b_l                     320 -> 64 (MFMA)
 //               H_i = H_i + relu(Z_i)
-//   O_i       = W_d^T H_i + b_d                                           64 -> 3
-//   z         = D^-1 y + c s O  (free dofs),  z = y (constrained dofs)
+//   O_i       = W_d^T H_i + b_d                                           64 -> 3 K
+//   fields    phi_i[f] = (O_i[3f], O_i[3f+1], O_i[3f+2]),  f < K = 8
+// The inputs depend only on the reference configuration, so the fields are computed once
+// after training.  The coarse space of aggregate I is spanned by the three translations
+// and the K fields restricted to I (free dofs); pdx_nk.hip assembles its Galerkin operator.
+// Training samples are the errors e = z - M_T(J z) the two-level preconditioner of the
+// translations leaves on the Krylov pairs (J z, z) of the solves it preconditions.  Loss:
+//   (1/S) sum_s sum_I min_c ( |e_sI - B_I c|^2 + lambda_I |c|^2 ) / |e_s|^2,
+// B_I = [translations | fields] on I's free dofs, lambda_I = 1e-6 tr(B_I^T B_I) / (3 + K);
+// by the envelope theorem d/dB_I = -2 (e_sI - B_I c) c^T / (S |e_s|^2).
 // H and the concatenated layer inputs are stored in bfloat16; every sum is float32.
-// Training pairs (y, x) with J x = y come from the plain solves: the Newton corrections
-// (-r, du) and the Krylov directions (J z_j, z_j).  Loss per pair:
-//   sum_free (O - t)^2 / sum_free t^2,   t = (x - D^-1 y) / (c s).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -36,11 +42,16 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 #define GFIN 16        // input features
 #define GK (5 * GF)    // concatenated layer input
 #define GL 3           // message-passing layers
+#define GKF 8          // learned fields
+#define GNO (3 * GKF)  // decoder outputs
+#define GNB (3 + GKF)  // coarse fields per aggregate: translations + learned
 #define GB1 64         // first pooling block
 #define GB2 4096       // second pooling block
 #define GPAD 512       // row padding of every activation buffer
 #define GDW_CHUNK 512  // points per workgroup of the weight-gradient GEMM
 #define G_NEAR_FRAC 0.6
+#define G_AGG_MAX 512  // aggregate size the loss kernel stages in LDS
+#define G_RIDGE 1e-6
 
 // flat parameter layout (float32), identical to the CPU oracle's
 #define OFF_WE 0
@@ -48,27 +59,28 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 #define OFF_W(l) (OFF_BE + GF + (l) * (GK * GF + GF))
 #define OFF_B(l) (OFF_W(l) + GK * GF)
 #define OFF_WD (OFF_BE + GF + GL * (GK * GF + GF))
-#define OFF_BD (OFF_WD + GF * 3)
-#define G_NPARAM (OFF_BD + 3)
+#define OFF_BD (OFF_WD + GF * GNO)
+#define G_NPARAM (OFF_BD + GNO)
 
-// held-out loss below which the trained network is applied (1 = it explains nothing)
-#define PDX_GNN_GATE 0.85
+// held-out loss ratio (fields / translations alone) below which the learned coarse space
+// is used (1 = the fields explain nothing the translations do not)
+#define PDX_GNN_GATE 0.95
 
 struct pdx_gnn {
     int64_t n = 0, npad = 0, nb1 = 0, nb2 = 0;
+    int agg = 0;
+    int64_t nagg = 0;
     // graph and static features
     bool ready = false;
     int64_t n_near = 0;
     int64_t *near_ptr = nullptr;
     int32_t *near = nullptr;
-    float *stat = nullptr;  // 5 per point
     // parameters
     bool trained = false;
-    double c_out = 1.0;
-    // validation: the last tenth of the Krylov pairs is held out of training; the network is
-    // applied only if it reduces the held-out error of the two-level preconditioner enough
-    int64_t n_k_val = 0;
-    double val_loss = -1.0;
+    bool have_fields = false;   // O holds the fields of the current parameters
+    int64_t n_val = 0;          // held-out samples (the last tenth)
+    double val_loss = -1.0;     // held-out loss ratio, fields / translations alone
+    double val_loss_t = -1.0;   // held-out loss of the translations alone
     bool use_net = true;
     float *theta = nullptr, *grad = nullptr, *am = nullptr, *av = nullptr;
     uint16_t *wT[GL] = {}, *wN[GL] = {};  // bf16: out-major [64][320], in-major [320][64]
@@ -78,30 +90,24 @@ struct pdx_gnn {
     uint16_t *H[GL + 1] = {};
     uint16_t *X[GL] = {};
     float *Z[GL] = {};
-    float *O = nullptr;
-    double *q = nullptr;
+    float *O = nullptr;         // npad x GNO: the fields
     float *S1 = nullptr, *S2 = nullptr, *Sg = nullptr;  // pooled sums, 192 columns
-    double *scal = nullptr;                               // device scalars
-    double *dpart = nullptr;                              // double partials
+    double *lpart = nullptr;    // loss partial per aggregate
+    int32_t *sidx = nullptr;    // sample indices of a training step
     // backward
     float *dH = nullptr, *dH2 = nullptr, *dZf = nullptr, *dO = nullptr;
     uint16_t *dZ = nullptr;
     float *dX = nullptr;
-    float *wpart = nullptr;  // weight-gradient partials
+    float *wpart = nullptr;     // weight-gradient partials
     int64_t wpart_len = 0;
-    // training pairs (float32, internal order)
+    // error samples (float32, internal order) and their 1 / |e|^2 over free dofs
     bool collecting = false;
-    int64_t cap_k = 0, n_k = 0, seen_k = 0;
-    float *Ky = nullptr, *Kx = nullptr;
-    int64_t cap_n = 0, n_n = 0;
-    float *Ny = nullptr, *Nx = nullptr;
-    double *ys = nullptr, *xs = nullptr;  // FP64 sample
+    int64_t cap = 0, n_s = 0, seen = 0;
+    float *E = nullptr;
+    float *einv = nullptr;
+    double *tmp = nullptr, *tmp2 = nullptr;  // 3n
     std::mt19937_64 rng{12345};
 };
-
-// device scalars (scal[]): 0 s, 1 rms(y), 2 n_free, 3 sum r^2, 4 sum t^2, 5 loss accum,
-// 6 dO scale, 7 c_out, 8 |x - q|^2 (output-scale estimate)
-enum { SC_S = 0, SC_RY, SC_NFREE, SC_R2, SC_T2, SC_LOSS, SC_DSCALE, SC_C, SC_XQ, SC_N };
 
 // ----------------------------------------------------------------------------------------
 // helpers
@@ -153,93 +159,39 @@ __global__ void k_near_fill(int64_t n, const int64_t *slice_ptr, const int32_t *
 // ----------------------------------------------------------------------------------------
 // forward
 // ----------------------------------------------------------------------------------------
-// q = D^-1 y (point blocks, column-major 3x3), partial sums of q^2, y^2 and the free count
-__global__ __launch_bounds__(256) void k_gnn_q(int64_t n, const double *dinv, const double *y,
-                                               const uint8_t *cmask, double *q, double *part)
-{
-    __shared__ double sh[3][256];
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    double sq = 0, sy = 0, nf = 0;
-    if (i < n) {
-        const double *R = dinv + 9 * i;
-        const double a = y[3 * i], b = y[3 * i + 1], cc = y[3 * i + 2];
-        const double qq[3] = {R[0] * a + R[3] * b + R[6] * cc, R[1] * a + R[4] * b + R[7] * cc,
-                              R[2] * a + R[5] * b + R[8] * cc};
-        for (int d = 0; d < 3; d++) {
-            q[3 * i + d] = qq[d];
-            if (!cmask[3 * i + d]) {
-                sq += qq[d] * qq[d];
-                sy += y[3 * i + d] * y[3 * i + d];
-                nf += 1.0;
-            }
-        }
-    }
-    sh[0][threadIdx.x] = sq;
-    sh[1][threadIdx.x] = sy;
-    sh[2][threadIdx.x] = nf;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o)
-            for (int k = 0; k < 3; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x < 3) part[3 * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
-}
-
-__global__ __launch_bounds__(256) void k_gnn_q_final(int nblk, const double *part, double *scal)
-{
-    __shared__ double sh[3][256];
-    double s[3] = {0, 0, 0};
-    for (int b = threadIdx.x; b < nblk; b += 256)
-        for (int k = 0; k < 3; k++) s[k] += part[3 * b + k];
-    for (int k = 0; k < 3; k++) sh[k][threadIdx.x] = s[k];
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o)
-            for (int k = 0; k < 3; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        const double nf = fmax(sh[2][0], 1.0);
-        scal[SC_S] = sqrt(sh[0][0] / nf) + 1e-300;
-        scal[SC_RY] = sqrt(sh[1][0] / nf) + 1e-300;
-        scal[SC_NFREE] = sh[2][0];
-    }
-}
-
-// encoder: F_i and H0_i = relu(W_e^T F_i + b_e); one wavefront per point, lane = feature
-// (the per-point inputs are broadcast loads, the stores of H0 are coalesced)
-__global__ __launch_bounds__(256) void k_gnn_encode(int64_t n, int64_t npad, const double *q,
-                                                    const double *y, const uint8_t *cmask,
-                                                    const float *stat, const double *scal,
-                                                    const float *theta, float *Fo, float *Z0,
-                                                    uint16_t *H0)
+// encoder: H0_i = relu(W_e^T F_i + b_e); one wavefront per point, lane = feature (the
+// inputs are broadcast loads, the stores of H0 coalesced); Z0 kept for training
+__global__ __launch_bounds__(256) void k_gnn_encode(int64_t n, int64_t npad, const float *Fi,
+                                                    const float *theta, float *Z0, uint16_t *H0)
 {
     const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int f = threadIdx.x & 63;
     if (p >= npad) return;
     if (p >= n) {
         H0[p * GF + f] = 0;
-        if (Fo && f < GFIN) Fo[p * GFIN + f] = 0.f;
         if (Z0) Z0[p * GF + f] = 0.f;
         return;
     }
-    const float is = (float)(1.0 / scal[SC_S]), iy = (float)(1.0 / scal[SC_RY]);
-    float Fi[GFIN];
-    for (int d = 0; d < 3; d++) {
-        const bool c = cmask[3 * p + d] != 0;
-        Fi[d] = c ? 0.f : (float)q[3 * p + d] * is;
-        Fi[3 + d] = c ? 0.f : (float)y[3 * p + d] * iy;
-        Fi[6 + d] = c ? 1.f : 0.f;
-    }
-    for (int k = 0; k < 5; k++) Fi[9 + k] = stat[5 * p + k];
-    Fi[14] = Fi[15] = 0.f;
-    if (Fo && f < GFIN) Fo[p * GFIN + f] = Fi[f];
     float z = theta[OFF_BE + f];
 #pragma unroll
-    for (int k = 0; k < GFIN; k++) z += Fi[k] * theta[OFF_WE + k * GF + f];
+    for (int k = 0; k < GFIN; k++) z += Fi[p * GFIN + k] * theta[OFF_WE + k * GF + f];
     if (Z0) Z0[p * GF + f] = z;
     H0[p * GF + f] = f2bf(fmaxf(z, 0.f));
+}
+
+// decoder O = W_d^T H + b_d (64 -> GNO): ten points per workgroup, a thread per output
+__global__ __launch_bounds__(256) void k_gnn_decode(int64_t n, int64_t npad, const uint16_t *H,
+                                                    const float *theta, float *O)
+{
+    const int pl = threadIdx.x / GNO, o = threadIdx.x % GNO;
+    const int64_t p = (int64_t)blockIdx.x * (256 / GNO) + pl;
+    if (pl >= 256 / GNO || p >= npad) return;
+    float s = 0.f;
+    if (p < n) {
+        s = theta[OFF_BD + o];
+        for (int f = 0; f < GF; f++) s += bf2f(H[p * GF + f]) * theta[OFF_WD + f * GNO + o];
+    }
+    O[p * GNO + o] = s;
 }
 
 // sums over 64-point blocks of `ng` column groups of 64 (source bf16 or float, row stride
@@ -383,129 +335,160 @@ __global__ __launch_bounds__(256) void k_gnn_gemm_fwd(int64_t n, const uint16_t 
     }
 }
 
-// decoder O = W_d^T H + b_d.  Apply: z = q + c s O (free), y (constrained).
-// Train: O kept, partial sums of (O - t)^2 and t^2 with t = (x - q) / (c s).
-// 16 lanes per point (4 features each, 8-byte loads), 16 points per workgroup.
-template <bool TRAIN>
-__global__ __launch_bounds__(256) void k_gnn_decode(int64_t n, const uint16_t *H, const float *theta,
-                                                    const double *q, const double *y,
-                                                    const double *x, const uint8_t *cmask,
-                                                    const double *scal, double *z, float *Oo,
-                                                    double *part)
+// ----------------------------------------------------------------------------------------
+// loss: per aggregate, the ridge least-squares fit of every error sample by the coarse
+// fields, and the gradient with respect to the fields.  One workgroup per aggregate; the
+// aggregate's basis B (rows = its 3 |I| dofs, columns = 3 translations + GKF fields, zero on
+// constrained dofs) and the gradient accumulate in LDS.
+// ----------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_gnn_lsq(int64_t n, int agg, const float *O,
+                                                 const uint8_t *cmask, const float *E,
+                                                 const float *einv, const int32_t *sidx,
+                                                 int ns, int64_t m, int use_fields, float wscale,
+                                                 float *dO, double *lpart)
 {
-    __shared__ double sh[2][16];
-    const int lane16 = threadIdx.x & 15, pl = threadIdx.x >> 4;
-    const int64_t p = (int64_t)blockIdx.x * 16 + pl;
-    double r2 = 0.0, t2 = 0.0;
-    float O[3] = {0.f, 0.f, 0.f};
-    if (p < n) {
-        const uint2 hv = *reinterpret_cast<const uint2 *>(H + p * GF + 4 * lane16);
-        const uint16_t hb[4] = {(uint16_t)(hv.x & 0xFFFF), (uint16_t)(hv.x >> 16),
-                                (uint16_t)(hv.y & 0xFFFF), (uint16_t)(hv.y >> 16)};
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const int f = 4 * lane16 + e;
-            const float h = bf2f(hb[e]);
-            O[0] += h * theta[OFF_WD + 3 * f];
-            O[1] += h * theta[OFF_WD + 3 * f + 1];
-            O[2] += h * theta[OFF_WD + 3 * f + 2];
+    __shared__ float B[3 * G_AGG_MAX][GNB];
+    __shared__ float dB[3 * G_AGG_MAX][GKF];
+    __shared__ double G[GNB][GNB], Lc[GNB][GNB];
+    __shared__ float red[GNB][256];
+    __shared__ double bv[GNB], cv[GNB];
+    __shared__ double lsum[256];
+    const int64_t I = blockIdx.x;
+    const int64_t p0 = I * agg, p1 = min(n, p0 + agg);
+    const int nr = (int)(3 * (p1 - p0));
+    const int t = threadIdx.x;
+    for (int r = t; r < nr; r += 256) {
+        const int64_t p = p0 + r / 3;
+        const int d = r % 3;
+        const bool fr = !cmask[3 * p + d];
+        for (int f = 0; f < 3; f++) B[r][f] = (fr && f == d) ? 1.f : 0.f;
+        for (int f = 0; f < GKF; f++) {
+            B[r][3 + f] = (fr && use_fields) ? O[p * GNO + 3 * f + d] : 0.f;
+            dB[r][f] = 0.f;
         }
     }
-#pragma unroll
-    for (int o = 1; o < 16; o <<= 1)
-#pragma unroll
-        for (int d = 0; d < 3; d++) O[d] += __shfl_xor(O[d], o);
-    if (p < n && lane16 == 0) {
-        const double cs = scal[SC_C] * scal[SC_S];
-        for (int d = 0; d < 3; d++) {
-            const int64_t k = 3 * p + d;
-            const float od = O[d] + theta[OFF_BD + d];
-            const bool c = cmask[k] != 0;
-            if (!TRAIN) {
-                z[k] = c ? y[k] : q[k] + cs * (double)od;
-            } else {
-                Oo[k] = od;
-                if (!c) {
-                    const double t = (x[k] - q[k]) / cs;
-                    const double r = (double)od - t;
-                    r2 += r * r;
-                    t2 += t * t;
-                }
+    __syncthreads();
+    if (t < GNB * GNB) {
+        const int a = t / GNB, b = t % GNB;
+        double s = 0.0;
+        for (int r = 0; r < nr; r++) s += (double)B[r][a] * (double)B[r][b];
+        G[a][b] = s;
+    }
+    __syncthreads();
+    if (t == 0) {
+        double tr = 0.0;
+        for (int a = 0; a < GNB; a++) tr += G[a][a];
+        const double lam = G_RIDGE * tr / GNB + 1e-30;
+        // Cholesky of G + lam I (lower)
+        for (int a = 0; a < GNB; a++)
+            for (int b = 0; b <= a; b++) {
+                double s = G[a][b] + (a == b ? lam : 0.0);
+                for (int k = 0; k < b; k++) s -= Lc[a][k] * Lc[b][k];
+                Lc[a][b] = (a == b) ? sqrt(fmax(s, 1e-300)) : s / Lc[b][b];
             }
-        }
+        G[0][0] = lam;  // G is no longer needed: keep lambda
     }
-    if (TRAIN) {
-        if (lane16 == 0) {
-            sh[0][pl] = r2;
-            sh[1][pl] = t2;
+    __syncthreads();
+    const double lam = G[0][0];
+    double lacc = 0.0;
+    for (int q = 0; q < ns; q++) {
+        const int64_t sI = sidx ? sidx[q] : q;
+        const float *e = E + sI * m + 3 * p0;
+        float ev[(3 * G_AGG_MAX + 255) / 256];
+        float part[GNB];
+        for (int a = 0; a < GNB; a++) part[a] = 0.f;
+        int u = 0;
+        for (int r = t; r < nr; r += 256, u++) {
+            ev[u] = e[r];
+            for (int a = 0; a < GNB; a++) part[a] += B[r][a] * ev[u];
+        }
+        for (int a = 0; a < GNB; a++) red[a][t] = part[a];
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (t < o)
+                for (int a = 0; a < GNB; a++) red[a][t] += red[a][t + o];
+            __syncthreads();
+        }
+        if (t == 0) {
+            // c = (G + lam I)^-1 B^T e by the Cholesky factor
+            double y[GNB];
+            for (int a = 0; a < GNB; a++) {
+                double s = red[a][0];
+                for (int k = 0; k < a; k++) s -= Lc[a][k] * y[k];
+                y[a] = s / Lc[a][a];
+            }
+            for (int a = GNB - 1; a >= 0; a--) {
+                double s = y[a];
+                for (int k = a + 1; k < GNB; k++) s -= Lc[k][a] * cv[k];
+                cv[a] = s / Lc[a][a];
+            }
+            double c2 = 0.0;
+            for (int a = 0; a < GNB; a++) c2 += cv[a] * cv[a];
+            bv[0] = c2;
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            double a = 0.0, b = 0.0;
-            for (int k = 0; k < 16; k++) {
-                a += sh[0][k];
-                b += sh[1][k];
-            }
-            part[2 * blockIdx.x] = a;
-            part[2 * blockIdx.x + 1] = b;
+        const float w = einv[sI] * wscale;
+        double r2 = 0.0;
+        u = 0;
+        for (int r = t; r < nr; r += 256, u++) {
+            double bc = 0.0;
+            for (int a = 0; a < GNB; a++) bc += (double)B[r][a] * cv[a];
+            const float res = (float)((double)ev[u] - bc);
+            r2 += (double)res * res;
+            for (int f = 0; f < GKF; f++) dB[r][f] += -2.f * w * res * (float)cv[3 + f];
         }
+        // lambda = ridge tr(B^T B) / (3 + K) depends on the fields too: d lambda / dB = 2 ridge B / (3 + K)
+        u = 0;
+        for (int r = t; r < nr; r += 256, u++)
+            for (int f = 0; f < GKF; f++)
+                dB[r][f] += w * (float)(2.0 * G_RIDGE / GNB * bv[0]) * B[r][3 + f];
+        lacc += (double)w * r2;
+        if (t == 0) lacc += (double)w * lam * bv[0];
+        __syncthreads();
     }
-}
-
-// loss of one sample, the dO scale 2 / (B sum t^2)
-__global__ void k_gnn_loss_final(int nblk, const double *part, double *scal, double inv_batch)
-{
-    __shared__ double sh[2][256];
-    double a = 0, b = 0;
-    for (int k = threadIdx.x; k < nblk; k += 256) {
-        a += part[2 * k];
-        b += part[2 * k + 1];
+    for (int r = t; r < nr; r += 256) {
+        const int64_t p = p0 + r / 3;
+        const int d = r % 3;
+        for (int f = 0; f < GKF; f++) dO[p * GNO + 3 * f + d] = dB[r][f];
     }
-    sh[0][threadIdx.x] = a;
-    sh[1][threadIdx.x] = b;
+    lsum[t] = lacc;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) {
-            sh[0][threadIdx.x] += sh[0][threadIdx.x + o];
-            sh[1][threadIdx.x] += sh[1][threadIdx.x + o];
-        }
+        if (t < o) lsum[t] += lsum[t + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        const double t2 = fmax(sh[1][0], 1e-300);
-        scal[SC_R2] = sh[0][0];
-        scal[SC_T2] = t2;
-        scal[SC_LOSS] += sh[0][0] / t2 * inv_batch;
-        scal[SC_DSCALE] = 2.0 * inv_batch / t2;
+    if (t == 0) lpart[I] = lsum[0];
+}
+
+// sum of the loss partials into out[0]
+__global__ void k_gnn_lsum(int64_t nagg, const double *lpart, double *out)
+{
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (int64_t k = threadIdx.x; k < nagg; k += 256) s += lpart[k];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) out[0] = sh[0];
 }
 
 // ----------------------------------------------------------------------------------------
 // backward
 // ----------------------------------------------------------------------------------------
-// dO = scale (O - t) on free dofs; dH = W_d dO.  One wavefront per point, lane = feature.
+// dH = W_d dO (GNO -> 64); one wavefront per point, lane = feature
 __global__ __launch_bounds__(256) void k_gnn_decode_bwd(int64_t n, int64_t npad, const float *theta,
-                                                        const float *O, const double *q,
-                                                        const double *x, const uint8_t *cmask,
-                                                        const double *scal, float *dO, float *dH)
+                                                        const float *dO, float *dH)
 {
     const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int f = threadIdx.x & 63;
     if (p >= npad) return;
-    float g[3] = {0.f, 0.f, 0.f};
-    if (p < n) {
-        const double cs = scal[SC_C] * scal[SC_S];
-        for (int d = 0; d < 3; d++) {
-            const int64_t k = 3 * p + d;
-            if (!cmask[k]) {
-                const double t = (x[k] - q[k]) / cs;
-                g[d] = (float)(scal[SC_DSCALE] * ((double)O[k] - t));
-            }
-        }
-    }
-    if (f < 3) dO[3 * p + f] = g[f];
-    dH[p * GF + f] = theta[OFF_WD + 3 * f] * g[0] + theta[OFF_WD + 3 * f + 1] * g[1] +
-                     theta[OFF_WD + 3 * f + 2] * g[2];
+    float s = 0.f;
+    if (p < n)
+        for (int o = 0; o < GNO; o++) s += theta[OFF_WD + f * GNO + o] * dO[p * GNO + o];
+    dH[p * GF + f] = s;
 }
 
 // partial outer products sum_p A_p B_p^T and sum_p B_p over chunks of points: output
@@ -711,39 +694,18 @@ __global__ void k_gnn_wcopy(const float *theta, uint16_t *wT0, uint16_t *wN0, ui
     wN[i * GF + o] = h;
 }
 
-// training sample: y = a Y[i] + b Y[j], x likewise (float pairs -> FP64)
-__global__ void k_gnn_mix(int64_t m, const float *Yi, const float *Xi, const float *Yj,
-                          const float *Xj, double a, double b, double *y, double *x)
-{
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= m) return;
-    double yy = a * (double)Yi[k], xx = a * (double)Xi[k];
-    if (Yj) {
-        yy += b * (double)Yj[k];
-        xx += b * (double)Xj[k];
-    }
-    y[k] = yy;
-    x[k] = xx;
-}
-
-__global__ void k_gnn_store_pair(int64_t m, const double *y, const double *x, float *Y, float *X)
-{
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= m) return;
-    Y[k] = (float)y[k];
-    X[k] = (float)x[k];
-}
-
-// sum over free dofs of (x - q)^2 (output-scale estimate), block partials
-__global__ __launch_bounds__(256) void k_gnn_xq(int64_t m, const double *x, const double *q,
-                                                const uint8_t *cmask, double *part)
+// error sample e = z - t on the dofs (t = M_T(J z)), stored in float, and block partials of
+// |e|^2 over free dofs
+__global__ __launch_bounds__(256) void k_gnn_store_err(int64_t m, const double *z, const double *tt,
+                                                       const uint8_t *cmask, float *E, double *part)
 {
     __shared__ double sh[256];
     const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
     double s = 0.0;
-    if (k < m && !cmask[k]) {
-        const double d = x[k] - q[k];
-        s = d * d;
+    if (k < m) {
+        const double e = cmask[k] ? 0.0 : z[k] - tt[k];
+        E[k] = (float)e;
+        s = e * e;
     }
     sh[threadIdx.x] = s;
     __syncthreads();
@@ -754,6 +716,21 @@ __global__ __launch_bounds__(256) void k_gnn_xq(int64_t m, const double *x, cons
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
+// einv[slot] = 1 / sum(part) (0 for a zero sample)
+__global__ void k_gnn_einv(int nblk, const double *part, float *einv, int64_t slot)
+{
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (int k = threadIdx.x; k < nblk; k += 256) s += part[k];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) einv[slot] = sh[0] > 0.0 ? (float)(1.0 / sh[0]) : 0.f;
+}
+
 // ----------------------------------------------------------------------------------------
 // host
 // ----------------------------------------------------------------------------------------
@@ -761,10 +738,10 @@ void pdx_gnn_free(pdx_ctx *c)
 {
     pdx_gnn *g = c->gnn;
     if (!g) return;
-    std::vector<void *> ps = {g->near_ptr, g->near, g->stat, g->theta, g->grad, g->am, g->av,
-                              g->F, g->Z0, g->O, g->q, g->S1, g->S2, g->Sg, g->scal, g->dpart,
-                              g->dH, g->dH2, g->dZf, g->dO, g->dZ, g->dX, g->wpart, g->Ky,
-                              g->Kx, g->Ny, g->Nx, g->ys, g->xs};
+    std::vector<void *> ps = {g->near_ptr, g->near, g->theta, g->grad, g->am, g->av, g->F, g->Z0,
+                              g->O, g->S1, g->S2, g->Sg, g->lpart, g->sidx, g->dH, g->dH2,
+                              g->dZf, g->dO, g->dZ, g->dX, g->wpart, g->E, g->einv, g->tmp,
+                              g->tmp2};
     for (int l = 0; l < GL; l++) {
         ps.push_back(g->wT[l]);
         ps.push_back(g->wN[l]);
@@ -789,8 +766,6 @@ static pdx_status gnn_alloc_params(pdx_ctx *c, pdx_gnn *g)
         PDX_TRY(pdx_alloc(c, &g->wT[l], GK * GF));
         PDX_TRY(pdx_alloc(c, &g->wN[l], GK * GF));
     }
-    PDX_TRY(pdx_alloc(c, &g->scal, SC_N));
-    PDX_HIP(c, hipMemset(g->scal, 0, sizeof(double) * SC_N));
     return PDX_OK;
 }
 
@@ -802,7 +777,8 @@ static pdx_status gnn_wcopy(pdx_ctx *c, pdx_gnn *g)
     return PDX_OK;
 }
 
-// He-style initial weights, zero decoder (the untrained network is block Jacobi)
+// He-style initial weights; the decoder starts small and random (a zero decoder would give
+// zero fields, which receive no gradient)
 static pdx_status gnn_init_params(pdx_ctx *c, pdx_gnn *g, uint64_t seed)
 {
     std::vector<float> th(G_NPARAM, 0.f);
@@ -811,14 +787,17 @@ static pdx_status gnn_init_params(pdx_ctx *c, pdx_gnn *g, uint64_t seed)
     for (int k = 0; k < GFIN * GF; k++) th[OFF_WE + k] = nd(rng) * std::sqrt(2.f / GFIN);
     for (int l = 0; l < GL; l++)
         for (int k = 0; k < GK * GF; k++) th[OFF_W(l) + k] = 0.5f * nd(rng) * std::sqrt(2.f / GK);
+    for (int k = 0; k < GF * GNO; k++) th[OFF_WD + k] = nd(rng) * std::sqrt(1.f / GF);
     PDX_HIP(c, hipMemcpy(g->theta, th.data(), sizeof(float) * G_NPARAM, hipMemcpyHostToDevice));
     PDX_HIP(c, hipMemset(g->am, 0, sizeof(float) * G_NPARAM));
     PDX_HIP(c, hipMemset(g->av, 0, sizeof(float) * G_NPARAM));
     g->adam_t = 0;
+    g->have_fields = false;
     return gnn_wcopy(c, g);
 }
 
-// graph, static features and activation buffers (once per context)
+// graph, input features and activation buffers (once per context, after the constrained
+// dofs are known)
 static pdx_status gnn_setup(pdx_ctx *c)
 {
     if (!c->gnn) c->gnn = new pdx_gnn();
@@ -831,6 +810,12 @@ static pdx_status gnn_setup(pdx_ctx *c)
     g->npad = (n + GPAD - 1) / GPAD * GPAD;
     g->nb1 = (n + GB1 - 1) / GB1;
     g->nb2 = (n + GB2 - 1) / GB2;
+    PDX_TRY(pdx_nk_aggregates(c));
+    g->agg = c->nk->agg;
+    g->nagg = c->nk->nagg;
+    if (g->agg > G_AGG_MAX)
+        return pdx_fail(c, PDX_ERR_UNSUPPORTED, "the learned preconditioner takes at most %lld points",
+                        (long long)G_AGG_MAX * 256);
     const double lmax = G_NEAR_FRAC * c->P.delta;
     int32_t *dcnt = nullptr;
     PDX_TRY(pdx_alloc(c, &dcnt, n));
@@ -838,8 +823,10 @@ static pdx_status gnn_setup(pdx_ctx *c)
                        lmax, dcnt);
     PDX_HIP(c, hipGetLastError());
     std::vector<int32_t> nc(n), bc(n);
+    std::vector<uint8_t> cm(3 * n);
     PDX_HIP(c, hipMemcpyAsync(nc.data(), dcnt, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream));
     PDX_HIP(c, hipMemcpyAsync(bc.data(), c->cnt, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream));
+    PDX_HIP(c, hipMemcpyAsync(cm.data(), c->nk->cmask, 3 * n, hipMemcpyDeviceToHost, c->stream));
     PDX_HIP(c, hipStreamSynchronize(c->stream));
     (void)hipFree(dcnt);
     std::vector<int64_t> ptr(n + 1, 0);
@@ -851,7 +838,7 @@ static pdx_status gnn_setup(pdx_ctx *c)
     hipLaunchKernelGGL(k_near_fill, g1(n), dim3(256), 0, c->stream, n, c->slice_ptr, c->cnt, c->bw,
                        c->L, lmax, g->near_ptr, g->near);
     PDX_HIP(c, hipGetLastError());
-    // static features from the reference positions
+    // input features from the reference positions
     std::vector<double> Xr(4 * n);
     PDX_HIP(c, hipMemcpy(Xr.data(), c->Xr, sizeof(double) * 4 * n, hipMemcpyDeviceToHost));
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -864,63 +851,71 @@ static pdx_status gnn_setup(pdx_ctx *c)
         nmax = std::max(nmax, nc[p]);
         bmax = std::max(bmax, bc[p]);
     }
-    std::vector<float> st(5 * n);
-    for (int64_t p = 0; p < n; p++) {
-        for (int d = 0; d < 3; d++) {
-            const double half = std::max(0.5 * (hi[d] - lo[d]), 1e-300);
-            st[5 * p + d] = (float)((Xr[4 * p + d] - 0.5 * (hi[d] + lo[d])) / half);
+    std::vector<float> F((size_t)g->npad * GFIN, 0.f);
+    for (int64_t I = 0; I < g->nagg; I++) {
+        const int64_t p0 = I * g->agg, p1 = std::min(n, p0 + g->agg);
+        double cen[3] = {0, 0, 0};
+        for (int64_t p = p0; p < p1; p++)
+            for (int d = 0; d < 3; d++) cen[d] += Xr[4 * p + d];
+        for (int d = 0; d < 3; d++) cen[d] /= (double)(p1 - p0);
+        double h = 0.0;
+        for (int64_t p = p0; p < p1; p++)
+            for (int d = 0; d < 3; d++) h = std::max(h, std::fabs(Xr[4 * p + d] - cen[d]));
+        h = std::max(h, 1e-300);
+        for (int64_t p = p0; p < p1; p++) {
+            float *f = &F[p * GFIN];
+            for (int d = 0; d < 3; d++) {
+                const double half = std::max(0.5 * (hi[d] - lo[d]), 1e-300);
+                f[d] = (float)((Xr[4 * p + d] - 0.5 * (hi[d] + lo[d])) / half);
+                const double rel = (Xr[4 * p + d] - cen[d]) / h;
+                f[3 + d] = (float)rel;
+                f[6 + d] = cm[3 * p + d] ? 1.f : 0.f;
+                f[11 + d] = (float)(rel * rel);
+            }
+            f[9] = (float)nc[p] / (float)nmax;
+            f[10] = (float)bc[p] / (float)bmax;
+            f[14] = 1.f;
+            f[15] = 0.f;
         }
-        st[5 * p + 3] = (float)nc[p] / (float)nmax;
-        st[5 * p + 4] = (float)bc[p] / (float)bmax;
     }
-    PDX_TRY(pdx_alloc(c, &g->stat, 5 * n));
-    PDX_HIP(c, hipMemcpy(g->stat, st.data(), sizeof(float) * 5 * n, hipMemcpyHostToDevice));
-    // activations
     const int64_t np = g->npad;
     PDX_TRY(pdx_alloc(c, &g->F, np * GFIN));
+    PDX_HIP(c, hipMemcpy(g->F, F.data(), sizeof(float) * np * GFIN, hipMemcpyHostToDevice));
+    // activations
     PDX_TRY(pdx_alloc(c, &g->Z0, np * GF));
     for (int l = 0; l <= GL; l++) PDX_TRY(pdx_alloc(c, &g->H[l], np * GF));
     for (int l = 0; l < GL; l++) {
         PDX_TRY(pdx_alloc(c, &g->X[l], np * GK));
         PDX_TRY(pdx_alloc(c, &g->Z[l], np * GF));
     }
-    PDX_TRY(pdx_alloc(c, &g->O, 3 * n));
-    PDX_TRY(pdx_alloc(c, &g->q, 3 * n));
+    PDX_TRY(pdx_alloc(c, &g->O, np * GNO));
     PDX_TRY(pdx_alloc(c, &g->S1, g->nb1 * GF * 3));
     PDX_TRY(pdx_alloc(c, &g->S2, g->nb2 * GF * 3));
     PDX_TRY(pdx_alloc(c, &g->Sg, GF * 3));
-    // partial sums: 3 per 256 points (q), 2 per 16 points (decoder), 1 per 256 dofs (x - q)
-    const int64_t nblk = (3 * n + 255) / 256 + 1;
-    PDX_TRY(pdx_alloc(c, &g->dpart, std::max<int64_t>(3 * nblk, 2 * ((n + 15) / 16) + 2)));
+    PDX_TRY(pdx_alloc(c, &g->lpart, g->nagg + 8));
     PDX_TRY(pdx_alloc(c, &g->dH, np * GF));
     PDX_TRY(pdx_alloc(c, &g->dH2, np * GF));
     PDX_TRY(pdx_alloc(c, &g->dZf, np * GF));
-    PDX_TRY(pdx_alloc(c, &g->dO, 3 * np));
+    PDX_TRY(pdx_alloc(c, &g->dO, np * GNO));
     PDX_TRY(pdx_alloc(c, &g->dZ, np * GF));
     PDX_TRY(pdx_alloc(c, &g->dX, np * GK));
     g->wpart_len = std::max<int64_t>((np / GDW_CHUNK) * GK * GF, (np / 256 + 1) * (GK + 1) * GF);
     PDX_TRY(pdx_alloc(c, &g->wpart, g->wpart_len));
-    PDX_TRY(pdx_alloc(c, &g->ys, 3 * n));
-    PDX_TRY(pdx_alloc(c, &g->xs, 3 * n));
+    PDX_TRY(pdx_alloc(c, &g->tmp, 3 * n));
+    PDX_TRY(pdx_alloc(c, &g->tmp2, (3 * n + 255) / 256 + 8));
     PDX_TRY(gnn_alloc_params(c, g));
     g->ready = true;
     return PDX_OK;
 }
 
-// forward pass for the FP64 vector y (internal order).  TRAIN keeps the activations and
-// accumulates the loss against x; otherwise z = M(y).
-static pdx_status gnn_forward(pdx_ctx *c, const double *y, const double *x, double *z, bool train,
-                              double inv_batch)
+// the fields of the current parameters into O; TRAIN keeps the activations for backward
+static pdx_status gnn_forward(pdx_ctx *c, bool train)
 {
     pdx_gnn *g = c->gnn;
-    pdx_nk *k = c->nk;
     const int64_t n = g->n, np = g->npad;
-    const int nb = (int)((n + 255) / 256);
     hipStream_t s = c->stream;
-    hipLaunchKernelGGL(k_gnn_q, dim3(nb), dim3(256), 0, s, n, k->dinv, y, k->cmask, g->q, g->dpart);
-    hipLaunchKernelGGL(k_gnn_q_final, dim3(1), dim3(256), 0, s, nb, g->dpart, g->scal);
-    hipLaunchKernelGGL(k_gnn_encode, dim3((unsigned)(np / 4)), dim3(256), 0, s, n, np, g->q, y, k->cmask, g->stat,
-                       g->scal, g->theta, train ? g->F : nullptr, train ? g->Z0 : nullptr, g->H[0]);
+    hipLaunchKernelGGL(k_gnn_encode, dim3((unsigned)(np / 4)), dim3(256), 0, s, n, np, g->F, g->theta,
+                       train ? g->Z0 : nullptr, g->H[0]);
     PDX_HIP(c, hipGetLastError());
     hipLaunchKernelGGL(k_gnn_pool1<uint16_t>, dim3((unsigned)g->nb1), dim3(4 * GF), 0, s, n, 1, g->H[0],
                        GF, 0, g->S1);
@@ -938,33 +933,45 @@ static pdx_status gnn_forward(pdx_ctx *c, const double *y, const double *x, doub
                                g->X[l], g->wT[l], g->theta + OFF_B(l), g->H[l], g->H[l + 1], nullptr, S1n);
         PDX_HIP(c, hipGetLastError());
     }
-    if (train) {
-        const int nbd = (int)((n + 15) / 16);
-        hipLaunchKernelGGL(k_gnn_decode<true>, dim3(nbd), dim3(256), 0, s, n, g->H[GL], g->theta, g->q,
-                           y, x, k->cmask, g->scal, nullptr, g->O, g->dpart);
-        hipLaunchKernelGGL(k_gnn_loss_final, dim3(1), dim3(256), 0, s, nbd, g->dpart, g->scal, inv_batch);
-    } else {
-        hipLaunchKernelGGL(k_gnn_decode<false>, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, s, n, g->H[GL], g->theta, g->q,
-                           y, nullptr, k->cmask, g->scal, z, nullptr, nullptr);
-    }
+    const int ppb = 256 / GNO;
+    hipLaunchKernelGGL(k_gnn_decode, dim3((unsigned)((np + ppb - 1) / ppb)), dim3(256), 0, s, n, np,
+                       g->H[GL], g->theta, g->O);
     PDX_HIP(c, hipGetLastError());
+    g->have_fields = !train;
     return PDX_OK;
 }
 
-// adds the gradient of the last training forward pass to g->grad
-static pdx_status gnn_backward(pdx_ctx *c, const double *x)
+// loss of the samples sidx[0 .. ns) (all of E[0 .. ns) when sidx is null) for the fields in
+// O (or the translations alone), and dO; the loss lands in tmp2[0]
+static pdx_status gnn_loss(pdx_ctx *c, const int32_t *sidx, int ns, bool use_fields, double *loss)
 {
     pdx_gnn *g = c->gnn;
-    pdx_nk *k = c->nk;
+    hipStream_t s = c->stream;
+    PDX_HIP(c, hipMemsetAsync(g->dO, 0, sizeof(float) * g->npad * GNO, s));
+    hipLaunchKernelGGL(k_gnn_lsq, dim3((unsigned)g->nagg), dim3(256), 0, s, g->n, g->agg, g->O,
+                       c->nk->cmask, g->E, g->einv, sidx, ns, 3 * g->n, use_fields ? 1 : 0,
+                       1.f / (float)ns, g->dO, g->lpart);
+    hipLaunchKernelGGL(k_gnn_lsum, dim3(1), dim3(256), 0, s, g->nagg, g->lpart, g->tmp2);
+    PDX_HIP(c, hipGetLastError());
+    if (loss) {
+        PDX_HIP(c, hipMemcpyAsync(loss, g->tmp2, sizeof(double), hipMemcpyDeviceToHost, s));
+        PDX_HIP(c, hipStreamSynchronize(s));
+    }
+    return PDX_OK;
+}
+
+// adds the gradient of the last training forward pass (dO from gnn_loss) to g->grad
+static pdx_status gnn_backward(pdx_ctx *c)
+{
+    pdx_gnn *g = c->gnn;
     const int64_t n = g->n, np = g->npad;
     hipStream_t s = c->stream;
     const int64_t nch = np / 256;  // 256-point chunks
-    hipLaunchKernelGGL(k_gnn_decode_bwd, dim3((unsigned)(np / 4)), dim3(256), 0, s, n, np, g->theta, g->O, g->q, x,
-                       k->cmask, g->scal, g->dO, g->dH);
+    hipLaunchKernelGGL(k_gnn_decode_bwd, dim3((unsigned)(np / 4)), dim3(256), 0, s, n, np, g->theta, g->dO, g->dH);
     // W_d, b_d
     hipLaunchKernelGGL(k_gnn_outer<uint16_t>, dim3((unsigned)nch), dim3(256), 0, s, np, (int64_t)256,
-                       g->H[GL], GF, g->dO, 3, g->wpart);
-    hipLaunchKernelGGL(k_gnn_reduce_add, g1((GF + 1) * 3), dim3(256), 0, s, nch, (int64_t)(GF + 1) * 3,
+                       g->H[GL], GF, g->dO, GNO, g->wpart);
+    hipLaunchKernelGGL(k_gnn_reduce_add, g1((GF + 1) * GNO), dim3(256), 0, s, nch, (int64_t)(GF + 1) * GNO,
                        g->wpart, g->grad + OFF_WD);
     PDX_HIP(c, hipGetLastError());
     float *dHo = g->dH, *dHi = g->dH2;
@@ -1000,15 +1007,6 @@ static pdx_status gnn_backward(pdx_ctx *c, const double *x)
     return PDX_OK;
 }
 
-static pdx_status gnn_set_c(pdx_ctx *c, double cval)
-{
-    pdx_gnn *g = c->gnn;
-    g->c_out = cval;
-    PDX_HIP(c, hipMemcpyAsync(g->scal + SC_C, &g->c_out, sizeof(double), hipMemcpyHostToDevice, c->stream));
-    PDX_HIP(c, hipStreamSynchronize(c->stream));
-    return PDX_OK;
-}
-
 // ---------------------------------------------------------------------------- NK hooks
 pdx_status pdx_gnn_prepare(pdx_ctx *c)
 {
@@ -1018,69 +1016,51 @@ pdx_status pdx_gnn_prepare(pdx_ctx *c)
     return PDX_OK;
 }
 
-// the learned preconditioner of the Krylov solver: two-level coarse correction plus the
-// network's block-Jacobi-based output, z = P A_c^-1 P^T v + D^-1 v + c s O(v)
-pdx_status pdx_nk_two_level(pdx_ctx *c, const double *v, double *z);
-
-pdx_status pdx_gnn_apply(pdx_ctx *c, const double *v, double *z)
+// the network's fields (GKF per point, 3 floats each, internal order) for the coarse space
+pdx_status pdx_gnn_fields(pdx_ctx *c, const float **fields, int *n_fields)
 {
-    // a network that failed validation is not applied: the two-level preconditioner alone
-    if (!c->gnn->use_net && c->nk->coarse_ready) return pdx_nk_two_level(c, v, z);
-    PDX_TRY(gnn_forward(c, v, nullptr, z, false, 0.0));
-    if (c->nk->coarse_ready) PDX_TRY(pdx_nk_coarse_add(c, v, z, 1.0));
-    return PDX_OK;
-}
-
-static pdx_status gnn_ensure_pool(pdx_ctx *c)
-{
+    PDX_TRY(pdx_gnn_prepare(c));
     pdx_gnn *g = c->gnn;
-    const int64_t m = 3 * c->n;
-    if (!g->Ny) {
-        g->cap_n = 256;
-        PDX_TRY(pdx_alloc(c, &g->Ny, g->cap_n * m));
-        PDX_TRY(pdx_alloc(c, &g->Nx, g->cap_n * m));
-    }
-    if (!g->Ky && g->cap_k > 0) {
-        PDX_TRY(pdx_alloc(c, &g->Ky, g->cap_k * m));
-        PDX_TRY(pdx_alloc(c, &g->Kx, g->cap_k * m));
-    }
+    if (!g->have_fields) PDX_TRY(gnn_forward(c, false));
+    *fields = g->O;
+    *n_fields = GKF;
     return PDX_OK;
 }
 
-// Newton pair (y = -r, x = du)
-pdx_status pdx_gnn_record(pdx_ctx *c, const double *rhs, const double *du)
-{
-    pdx_gnn *g = c->gnn;
-    if (!g || !g->collecting) return PDX_OK;
-    PDX_TRY(gnn_ensure_pool(c));
-    const int64_t m = 3 * c->n;
-    const int64_t slot = g->n_n < g->cap_n ? g->n_n : (int64_t)(g->rng() % (uint64_t)g->cap_n);
-    hipLaunchKernelGGL(k_gnn_store_pair, g1(m), dim3(256), 0, c->stream, m, rhs, du, g->Ny + slot * m,
-                       g->Nx + slot * m);
-    PDX_HIP(c, hipGetLastError());
-    if (g->n_n < g->cap_n) g->n_n++;
-    return PDX_OK;
-}
+// the preconditioner with the current coarse space (built by pdx_nk_step with the fields)
+pdx_status pdx_gnn_apply(pdx_ctx *c, const double *v, double *z) { return pdx_nk_two_level(c, v, z); }
 
-// Krylov pair (y = J z, x = z), reservoir-sampled into cap_k slots
+// Newton pairs carry no error sample of their own: the Krylov pairs are exact
+pdx_status pdx_gnn_record(pdx_ctx *c, const double *rhs, const double *du) { return PDX_OK; }
+
+// Krylov pair (J z, z) of a solve preconditioned by the translations' two-level
+// preconditioner M_T: the error sample e = z - M_T(J z), reservoir-sampled into cap slots
 pdx_status pdx_gnn_record_krylov(pdx_ctx *c, const double *z, const double *jz)
 {
     pdx_gnn *g = c->gnn;
-    if (!g || !g->collecting || g->cap_k <= 0) return PDX_OK;
-    PDX_TRY(gnn_ensure_pool(c));
+    if (!g || !g->collecting || g->cap <= 0) return PDX_OK;
+    pdx_nk *k = c->nk;
+    if (!k->coarse_ready || k->nf != 3) return PDX_OK;  // collection runs under M_T
     const int64_t m = 3 * c->n;
-    int64_t slot = -1;
-    if (g->seen_k < g->cap_k) slot = g->seen_k;
-    else {
-        const uint64_t r = g->rng() % (uint64_t)(g->seen_k + 1);
-        if ((int64_t)r < g->cap_k) slot = (int64_t)r;
+    if (!g->E) {
+        PDX_TRY(pdx_alloc(c, &g->E, (size_t)g->cap * m));
+        PDX_TRY(pdx_alloc(c, &g->einv, (size_t)g->cap));
     }
-    g->seen_k++;
+    int64_t slot = -1;
+    if (g->seen < g->cap) slot = g->seen;
+    else {
+        const uint64_t r = g->rng() % (uint64_t)(g->seen + 1);
+        if ((int64_t)r < g->cap) slot = (int64_t)r;
+    }
+    g->seen++;
     if (slot < 0) return PDX_OK;
-    hipLaunchKernelGGL(k_gnn_store_pair, g1(m), dim3(256), 0, c->stream, m, jz, z, g->Ky + slot * m,
-                       g->Kx + slot * m);
+    PDX_TRY(pdx_nk_two_level(c, jz, g->tmp));
+    const int nb = (int)((m + 255) / 256);
+    hipLaunchKernelGGL(k_gnn_store_err, dim3(nb), dim3(256), 0, c->stream, m, z, g->tmp, k->cmask,
+                       g->E + slot * m, g->tmp2);
+    hipLaunchKernelGGL(k_gnn_einv, dim3(1), dim3(256), 0, c->stream, nb, g->tmp2, g->einv, slot);
     PDX_HIP(c, hipGetLastError());
-    g->n_k = std::min(g->seen_k, g->cap_k);
+    g->n_s = std::min(g->seen, g->cap);
     return PDX_OK;
 }
 
@@ -1100,13 +1080,13 @@ static pdx_status gnn_ready_for_io(pdx_ctx *c)
     return gnn_setup(c);
 }
 
-extern "C" pdx_status pdx_gnn_collect(pdx_ctx *c, int32_t enable, int64_t max_krylov_pairs)
+extern "C" pdx_status pdx_gnn_collect(pdx_ctx *c, int32_t enable, int64_t max_samples)
 {
     if (!c) return PDX_ERR_ARG;
-    if (max_krylov_pairs < 0) return pdx_fail(c, PDX_ERR_ARG, "max_krylov_pairs < 0");
+    if (max_samples < 0) return pdx_fail(c, PDX_ERR_ARG, "max_krylov_pairs < 0");
     PDX_TRY(gnn_ready_for_io(c));
     pdx_gnn *g = c->gnn;
-    if (enable && !g->collecting && g->Ky == nullptr) g->cap_k = max_krylov_pairs;
+    if (enable && !g->collecting && g->E == nullptr) g->cap = max_samples;
     g->collecting = enable != 0;
     return PDX_OK;
 }
@@ -1115,80 +1095,21 @@ extern "C" pdx_status pdx_gnn_pairs(pdx_ctx *c, int64_t *n_newton, int64_t *n_kr
 {
     if (!c) return PDX_ERR_ARG;
     pdx_gnn *g = c->gnn;
-    if (n_newton) *n_newton = g ? g->n_n : 0;
-    if (n_krylov) *n_krylov = g ? g->n_k : 0;
+    if (n_newton) *n_newton = 0;
+    if (n_krylov) *n_krylov = g ? g->n_s : 0;
     return PDX_OK;
 }
 
-static pdx_status gnn_build_dinv(pdx_ctx *c);
-
-// one sample into ys/xs: Newton pairs, Krylov pairs, and combinations of two
-static pdx_status gnn_draw(pdx_ctx *c, std::mt19937_64 &rng)
+static pdx_status gnn_adam_step(pdx_ctx *c, double lr_k)
 {
     pdx_gnn *g = c->gnn;
-    const int64_t m = 3 * c->n;
-    std::uniform_real_distribution<double> ud(0.0, 1.0);
-    std::normal_distribution<double> nd(0.0, 1.0);
-    auto pick = [&](const float *&Y, const float *&X) {
-        const bool newton = g->n_k == 0 || (g->n_n > 0 && ud(rng) < 0.5);
-        if (newton) {
-            const int64_t i = (int64_t)(rng() % (uint64_t)g->n_n);
-            Y = g->Ny + i * m;
-            X = g->Nx + i * m;
-        } else {
-            const int64_t i = (int64_t)(rng() % (uint64_t)(g->n_k - g->n_k_val));
-            Y = g->Ky + i * m;
-            X = g->Kx + i * m;
-        }
-    };
-    const float *Y1, *X1, *Y2 = nullptr, *X2 = nullptr;
-    pick(Y1, X1);
-    double a = 1.0, b = 0.0;
-    if (ud(rng) < 0.5) {
-        pick(Y2, X2);
-        a = nd(rng);
-        b = nd(rng);
-    }
-    hipLaunchKernelGGL(k_gnn_mix, g1(m), dim3(256), 0, c->stream, m, Y1, X1, Y2, X2, a, b, g->ys, g->xs);
+    const float b1 = 0.9f, b2 = 0.999f;
+    g->adam_t++;
+    const float c1 = 1.f - std::pow(b1, (float)g->adam_t), c2 = 1.f - std::pow(b2, (float)g->adam_t);
+    hipLaunchKernelGGL(k_gnn_adam, g1(G_NPARAM), dim3(256), 0, c->stream, (int64_t)G_NPARAM, g->theta,
+                       g->grad, g->am, g->av, (float)lr_k, b1, b2, c1, c2, 1e-8f);
     PDX_HIP(c, hipGetLastError());
-    return PDX_OK;
-}
-
-// output scale c = median over samples of rms_free(x' - D^-1 y) / rms_free(D^-1 y), with
-// x' = x - P A_c^-1 P^T y when the coarse space is built
-static pdx_status gnn_estimate_c(pdx_ctx *c, std::mt19937_64 &rng)
-{
-    pdx_gnn *g = c->gnn;
-    const int64_t m = 3 * c->n, n = c->n;
-    const int nb = (int)((n + 255) / 256), nbm = (int)((m + 255) / 256);
-    std::vector<double> ratios;
-    const int ns = (int)std::min<int64_t>(32, g->n_n + g->n_k);
-    for (int k = 0; k < ns; k++) {
-        PDX_TRY(gnn_draw(c, rng));
-        if (c->nk->coarse_ready) PDX_TRY(pdx_nk_coarse_add(c, g->ys, g->xs, -1.0));
-        hipLaunchKernelGGL(k_gnn_q, dim3(nb), dim3(256), 0, c->stream, n, c->nk->dinv, g->ys,
-                           c->nk->cmask, g->q, g->dpart);
-        hipLaunchKernelGGL(k_gnn_q_final, dim3(1), dim3(256), 0, c->stream, nb, g->dpart, g->scal);
-        double sc[3];
-        PDX_HIP(c, hipMemcpyAsync(sc, g->scal, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream));
-        hipLaunchKernelGGL(k_gnn_xq, dim3(nbm), dim3(256), 0, c->stream, m, g->xs, g->q, c->nk->cmask,
-                           g->dpart);
-        std::vector<double> part(nbm);
-        PDX_HIP(c, hipMemcpyAsync(part.data(), g->dpart, sizeof(double) * nbm, hipMemcpyDeviceToHost,
-                                  c->stream));
-        PDX_HIP(c, hipStreamSynchronize(c->stream));
-        double xq = 0.0;
-        for (double v : part) xq += v;
-        const double nf = std::max(sc[SC_NFREE], 1.0);
-        const double r = std::sqrt(xq / nf) / sc[SC_S];
-        if (std::isfinite(r) && r > 0) ratios.push_back(r);
-    }
-    double cval = 1.0;
-    if (!ratios.empty()) {
-        std::sort(ratios.begin(), ratios.end());
-        cval = ratios[ratios.size() / 2];
-    }
-    return gnn_set_c(c, cval);
+    return gnn_wcopy(c, g);
 }
 
 extern "C" pdx_status pdx_gnn_train(pdx_ctx *c, int32_t steps, int32_t batch, double lr,
@@ -1198,57 +1119,46 @@ extern "C" pdx_status pdx_gnn_train(pdx_ctx *c, int32_t steps, int32_t batch, do
     if (steps < 0 || batch <= 0 || !(lr > 0)) return pdx_fail(c, PDX_ERR_ARG, "invalid training options");
     PDX_TRY(gnn_ready_for_io(c));
     pdx_gnn *g = c->gnn;
-    if (g->n_n + g->n_k == 0) return pdx_fail(c, PDX_ERR_STATE, "no training pairs (pdx_gnn_collect)");
-    PDX_TRY(gnn_build_dinv(c));
-    // the network learns what the two-level preconditioner leaves: x - P A_c^-1 P^T y
-    PDX_TRY(pdx_nk_build_coarse(c));
+    if (g->n_s == 0) return pdx_fail(c, PDX_ERR_STATE, "no training samples (pdx_gnn_collect)");
     std::mt19937_64 rng(seed);
-    // hold out the last tenth of the Krylov pairs (if there are enough) for validation
-    g->n_k_val = g->n_k >= 20 ? g->n_k / 10 : 0;
-    if (g->n_n == 0 && g->n_k - g->n_k_val <= 0) g->n_k_val = 0;
+    // hold out the last tenth of the samples (if there are enough) for validation
+    g->n_val = g->n_s >= 20 ? g->n_s / 10 : 0;
+    const int64_t ntr = g->n_s - g->n_val;
+    const int nb = (int)std::min<int64_t>(batch, ntr);
+    if (!g->sidx) PDX_TRY(pdx_alloc(c, &g->sidx, (size_t)std::max<int64_t>(g->n_s, 1)));
     if (!g->trained) PDX_TRY(gnn_init_params(c, g, seed));
-    PDX_TRY(gnn_estimate_c(c, rng));
-    const float b1 = 0.9f, b2 = 0.999f;
+    std::vector<int32_t> order(ntr);
+    for (int64_t i = 0; i < ntr; i++) order[i] = (int32_t)i;
     double last = 0.0;
     for (int it = 0; it < steps; it++) {
+        // a random subset of nb training samples
+        for (int q = 0; q < nb; q++) std::swap(order[q], order[q + rng() % (uint64_t)(ntr - q)]);
+        PDX_HIP(c, hipMemcpyAsync(g->sidx, order.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice,
+                                  c->stream));
         PDX_HIP(c, hipMemsetAsync(g->grad, 0, sizeof(float) * G_NPARAM, c->stream));
-        PDX_HIP(c, hipMemsetAsync(g->scal + SC_LOSS, 0, sizeof(double), c->stream));
-        for (int b = 0; b < batch; b++) {
-            PDX_TRY(gnn_draw(c, rng));
-            PDX_TRY(pdx_nk_coarse_add(c, g->ys, g->xs, -1.0));
-            PDX_TRY(gnn_forward(c, g->ys, g->xs, nullptr, true, 1.0 / batch));
-            PDX_TRY(gnn_backward(c, g->xs));
-        }
-        g->adam_t++;
-        const float c1 = 1.f - std::pow(b1, (float)g->adam_t), c2 = 1.f - std::pow(b2, (float)g->adam_t);
+        PDX_TRY(gnn_forward(c, true));
+        const bool report = it == steps - 1 || (it % 25) == 24;
+        PDX_TRY(gnn_loss(c, g->sidx, nb, true, report ? &last : nullptr));
+        PDX_TRY(gnn_backward(c));
         // cosine decay of the step size
-        const float lrk = (float)(lr * 0.5 * (1.0 + std::cos(M_PI * it / std::max(steps, 1))));
-        hipLaunchKernelGGL(k_gnn_adam, g1(G_NPARAM), dim3(256), 0, c->stream, (int64_t)G_NPARAM, g->theta,
-                           g->grad, g->am, g->av, lrk, b1, b2, c1, c2, 1e-8f);
-        PDX_TRY(gnn_wcopy(c, g));
-        if (it == steps - 1 || (it % 50) == 49) {
-            PDX_HIP(c, hipMemcpyAsync(&last, g->scal + SC_LOSS, sizeof(double), hipMemcpyDeviceToHost,
-                                      c->stream));
-            PDX_HIP(c, hipStreamSynchronize(c->stream));
-        }
+        PDX_TRY(gnn_adam_step(c, lr * 0.5 * (1.0 + std::cos(M_PI * it / std::max(steps, 1)))));
     }
+    PDX_TRY(gnn_forward(c, false));
     PDX_HIP(c, hipStreamSynchronize(c->stream));
     g->trained = true;
     if (loss_out) *loss_out = last;
-    // held-out loss ||c s O - e||^2 / ||e||^2 of the two-level error e (1 = no gain)
-    g->val_loss = last;
-    if (g->n_k_val > 0) {
-        const int64_t m = 3 * c->n;
-        PDX_HIP(c, hipMemsetAsync(g->scal + SC_LOSS, 0, sizeof(double), c->stream));
-        for (int64_t i = g->n_k - g->n_k_val; i < g->n_k; i++) {
-            hipLaunchKernelGGL(k_gnn_mix, g1(m), dim3(256), 0, c->stream, m, g->Ky + i * m, g->Kx + i * m,
-                               (const float *)nullptr, (const float *)nullptr, 1.0, 0.0, g->ys, g->xs);
-            PDX_TRY(pdx_nk_coarse_add(c, g->ys, g->xs, -1.0));
-            PDX_TRY(gnn_forward(c, g->ys, g->xs, nullptr, true, 1.0 / (double)g->n_k_val));
-        }
-        PDX_HIP(c, hipMemcpyAsync(&g->val_loss, g->scal + SC_LOSS, sizeof(double), hipMemcpyDeviceToHost,
+    // held-out loss with the fields over the loss of the translations alone (1 = no gain)
+    g->val_loss = 1.0;
+    if (g->n_val > 0) {
+        std::vector<int32_t> vi(g->n_val);
+        for (int64_t q = 0; q < g->n_val; q++) vi[q] = (int32_t)(ntr + q);
+        PDX_HIP(c, hipMemcpyAsync(g->sidx, vi.data(), sizeof(int32_t) * g->n_val, hipMemcpyHostToDevice,
                                   c->stream));
-        PDX_HIP(c, hipStreamSynchronize(c->stream));
+        double lf = 0.0, lt = 0.0;
+        PDX_TRY(gnn_loss(c, g->sidx, (int)g->n_val, true, &lf));
+        PDX_TRY(gnn_loss(c, g->sidx, (int)g->n_val, false, &lt));
+        g->val_loss_t = lt;
+        g->val_loss = lt > 0 ? lf / lt : 1.0;
     }
     g->use_net = g->val_loss < PDX_GNN_GATE;
     return PDX_OK;
@@ -1266,10 +1176,6 @@ extern "C" pdx_status pdx_gnn_gate(pdx_ctx *c, double threshold, double *val_los
     return PDX_OK;
 }
 
-// point-block Jacobi at the current state (pdx_nk.hip)
-pdx_status pdx_nk_build_jacobi(pdx_ctx *c);
-static pdx_status gnn_build_dinv(pdx_ctx *c) { return pdx_nk_build_jacobi(c); }
-
 extern "C" pdx_status pdx_gnn_params(pdx_ctx *c, float *theta_out, const float *theta_in,
                                      double *out_scale)
 {
@@ -1282,11 +1188,10 @@ extern "C" pdx_status pdx_gnn_params(pdx_ctx *c, float *theta_out, const float *
         PDX_HIP(c, hipMemset(g->av, 0, sizeof(float) * G_NPARAM));
         g->adam_t = 0;
         PDX_TRY(gnn_wcopy(c, g));
-        if (out_scale) PDX_TRY(gnn_set_c(c, *out_scale));
         g->trained = true;
-    } else if (out_scale) {
-        *out_scale = g->c_out;
+        g->have_fields = false;
     }
+    if (out_scale) *out_scale = 1.0;
     if (theta_out) PDX_HIP(c, hipMemcpy(theta_out, g->theta, sizeof(float) * G_NPARAM, hipMemcpyDeviceToHost));
     PDX_HIP(c, hipDeviceSynchronize());
     return PDX_OK;
@@ -1295,7 +1200,7 @@ extern "C" pdx_status pdx_gnn_params(pdx_ctx *c, float *theta_out, const float *
 extern "C" int64_t pdx_gnn_n_params(void) { return G_NPARAM; }
 
 extern "C" pdx_status pdx_gnn_graph(pdx_ctx *c, int64_t *n_near, int64_t *perm, int64_t *near_ptr,
-                                    int32_t *near, float *stat, double *dinv, uint8_t *cmask)
+                                    int32_t *near, float *feat, double *dinv, uint8_t *cmask)
 {
     if (!c) return PDX_ERR_ARG;
     PDX_TRY(gnn_ready_for_io(c));
@@ -1308,9 +1213,9 @@ extern "C" pdx_status pdx_gnn_graph(pdx_ctx *c, int64_t *n_near, int64_t *perm, 
     }
     if (near_ptr) PDX_HIP(c, hipMemcpy(near_ptr, g->near_ptr, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost));
     if (near) PDX_HIP(c, hipMemcpy(near, g->near, sizeof(int32_t) * g->n_near, hipMemcpyDeviceToHost));
-    if (stat) PDX_HIP(c, hipMemcpy(stat, g->stat, sizeof(float) * 5 * n, hipMemcpyDeviceToHost));
+    if (feat) PDX_HIP(c, hipMemcpy(feat, g->F, sizeof(float) * GFIN * n, hipMemcpyDeviceToHost));
     if (dinv) {
-        PDX_TRY(gnn_build_dinv(c));
+        PDX_TRY(pdx_nk_build_jacobi(c));
         PDX_HIP(c, hipMemcpy(dinv, c->nk->dinv, sizeof(double) * 9 * n, hipMemcpyDeviceToHost));
     }
     if (cmask) PDX_HIP(c, hipMemcpy(cmask, c->nk->cmask, 3 * n, hipMemcpyDeviceToHost));
@@ -1323,31 +1228,67 @@ extern "C" pdx_status pdx_gnn_apply_vec(pdx_ctx *c, const double *y, double *z)
     PDX_TRY(gnn_ready_for_io(c));
     const int64_t m = 3 * c->n;
     pdx_gnn *g = c->gnn;
-    PDX_TRY(gnn_build_dinv(c));
-    PDX_HIP(c, hipMemcpy(g->ys, y, sizeof(double) * m, hipMemcpyHostToDevice));
-    PDX_TRY(gnn_forward(c, g->ys, nullptr, g->xs, false, 0.0));
-    PDX_HIP(c, hipMemcpy(z, g->xs, sizeof(double) * m, hipMemcpyDeviceToHost));
+    const float *fl = nullptr;
+    int nl = 0;
+    PDX_TRY(pdx_gnn_fields(c, &fl, &nl));
+    PDX_TRY(pdx_nk_build_jacobi(c));
+    PDX_TRY(pdx_nk_build_coarse_fields(c, fl, nl));
+    PDX_HIP(c, hipMemcpy(g->tmp, y, sizeof(double) * m, hipMemcpyHostToDevice));
+    double *zd = c->nk->w;
+    PDX_TRY(pdx_nk_two_level(c, g->tmp, zd));
+    PDX_HIP(c, hipMemcpy(z, zd, sizeof(double) * m, hipMemcpyDeviceToHost));
+    return PDX_OK;
+}
+
+extern "C" pdx_status pdx_gnn_field_values(pdx_ctx *c, float *fields)
+{
+    if (!c || !fields) return PDX_ERR_ARG;
+    PDX_TRY(gnn_ready_for_io(c));
+    pdx_gnn *g = c->gnn;
+    if (!g->trained) return pdx_fail(c, PDX_ERR_STATE, "the network has no parameters (pdx_gnn_params)");
+    PDX_TRY(gnn_forward(c, false));
+    PDX_HIP(c, hipMemcpy(fields, g->O, sizeof(float) * GNO * c->n, hipMemcpyDeviceToHost));
     return PDX_OK;
 }
 
 extern "C" pdx_status pdx_gnn_loss_grad(pdx_ctx *c, int32_t n_samples, const double *Y,
                                         const double *X, double *loss, float *grad)
 {
-    if (!c || n_samples <= 0 || !Y || !X) return PDX_ERR_ARG;
+    if (!c || n_samples <= 0 || !Y) return PDX_ERR_ARG;
     PDX_TRY(gnn_ready_for_io(c));
     pdx_gnn *g = c->gnn;
+    if (!g->trained) return pdx_fail(c, PDX_ERR_STATE, "the network has no parameters (pdx_gnn_params)");
     const int64_t m = 3 * c->n;
-    PDX_TRY(gnn_build_dinv(c));
-    PDX_HIP(c, hipMemsetAsync(g->grad, 0, sizeof(float) * G_NPARAM, c->stream));
-    PDX_HIP(c, hipMemsetAsync(g->scal + SC_LOSS, 0, sizeof(double), c->stream));
+    // the samples replace the collected ones
+    if (g->E && g->cap < n_samples) {
+        (void)hipFree(g->E);
+        (void)hipFree(g->einv);
+        g->E = nullptr;
+        g->einv = nullptr;
+    }
+    if (!g->E) {
+        g->cap = std::max<int64_t>(g->cap, n_samples);
+        PDX_TRY(pdx_alloc(c, &g->E, (size_t)g->cap * m));
+        PDX_TRY(pdx_alloc(c, &g->einv, (size_t)g->cap));
+    }
+    const int nb = (int)((m + 255) / 256);
+    double *zd = c->nk->w;
+    PDX_HIP(c, hipMemsetAsync(g->tmp, 0, sizeof(double) * m, c->stream));
     for (int s = 0; s < n_samples; s++) {
-        PDX_HIP(c, hipMemcpyAsync(g->ys, Y + s * m, sizeof(double) * m, hipMemcpyHostToDevice, c->stream));
-        PDX_HIP(c, hipMemcpyAsync(g->xs, X + s * m, sizeof(double) * m, hipMemcpyHostToDevice, c->stream));
-        PDX_TRY(gnn_forward(c, g->ys, g->xs, nullptr, true, 1.0 / n_samples));
-        PDX_TRY(gnn_backward(c, g->xs));
+        PDX_HIP(c, hipMemcpyAsync(zd, Y + s * m, sizeof(double) * m, hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_gnn_store_err, dim3(nb), dim3(256), 0, c->stream, m, zd, g->tmp,
+                           c->nk->cmask, g->E + s * m, g->tmp2);
+        hipLaunchKernelGGL(k_gnn_einv, dim3(1), dim3(256), 0, c->stream, nb, g->tmp2, g->einv, (int64_t)s);
         PDX_HIP(c, hipStreamSynchronize(c->stream));
     }
-    if (loss) PDX_HIP(c, hipMemcpy(loss, g->scal + SC_LOSS, sizeof(double), hipMemcpyDeviceToHost));
+    g->n_s = g->seen = n_samples;
+    PDX_HIP(c, hipMemsetAsync(g->grad, 0, sizeof(float) * G_NPARAM, c->stream));
+    PDX_TRY(gnn_forward(c, true));
+    double l = 0.0;
+    PDX_TRY(gnn_loss(c, nullptr, n_samples, true, &l));
+    PDX_TRY(gnn_backward(c));
+    PDX_HIP(c, hipStreamSynchronize(c->stream));
+    if (loss) *loss = l;
     if (grad) PDX_HIP(c, hipMemcpy(grad, g->grad, sizeof(float) * G_NPARAM, hipMemcpyDeviceToHost));
     return PDX_OK;
 }

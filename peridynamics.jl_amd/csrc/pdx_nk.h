@@ -1,6 +1,8 @@
 // Newton-Krylov workspace and the hooks of the learned preconditioner.
 #pragma once
 
+#include <rocblas/rocblas.h>
+
 #include "pdx_internal.h"
 
 struct pdx_nk {
@@ -19,29 +21,55 @@ struct pdx_nk {
     double *h_dev = nullptr;    // small coefficient vectors
     double *dinv = nullptr;     // 9n point-block Jacobi inverse
     double *dsc = nullptr;      // device scalars (||u||^2, ||v||^2)
-    // two-level preconditioner: aggregates of `agg` consecutive points, mc = 3 nagg
+    // two-level preconditioner: aggregates of `agg` consecutive points, nf basis fields per
+    // aggregate, mc = nf nagg coarse dofs
     int agg = 0;
     int64_t nagg = 0, mc = 0;
-    double *Ac = nullptr;       // mc x mc, the inverse of the Galerkin operator once built
+    int nf = 0, nf_cap = 0;
+    int maxs = 0;                // neighbouring aggregates of an aggregate (incl. itself)
+    int32_t *aslot = nullptr;    // nagg x maxs neighbour aggregates, -1 padded
+    int32_t *pJ = nullptr;       // n x NK_TP aggregates reached by each point's bonds
+    int *dflag = nullptr;        // overflow flag and the rocSOLVER infos
+    double *phi = nullptr;       // n x nf x 3 basis fields, orthonormal per aggregate
+    double *Ycs = nullptr;       // n x NK_TP x nf x 3 per-point stiffness products
+    double *Ac = nullptr;        // mc x mc Galerkin operator, then its Cholesky inverse
+    float *Ainv = nullptr;       // mc x mc inverse of the Galerkin operator (float32)
     double *rc = nullptr, *zc = nullptr;
+    rocblas_handle rb = nullptr;
     bool coarse_ready = false;
     int64_t n_jv = 0;
+    bool verbose = false;       // PDX_NK_VERBOSE: a line per Newton iteration on stderr
 };
+
+// aggregate size of the two-level preconditioner: a power of two >= 64, at most 256
+// aggregates (pdx_nk_aggregates doubles it where the bonds of a point would reach too many)
+inline int pdx_nk_agg_size(int64_t n)
+{
+    int64_t agg = 64;
+    while ((n + agg - 1) / agg > 256) agg *= 2;
+    return (int)agg;
+}
 
 // Jv on device vectors (internal order)
 pdx_status pdx_nk_jv_dev(pdx_ctx *c, const double *v, double *jv, double scale);
 
 // learned preconditioner (pdx_gnn.hip)
-pdx_status pdx_gnn_prepare(pdx_ctx *c);                                  // per Newton iteration
+pdx_status pdx_gnn_prepare(pdx_ctx *c);                                  // trained check
 pdx_status pdx_gnn_apply(pdx_ctx *c, const double *v, double *z);        // z = M^-1 v
+// the network's coarse fields (n_fields per point, 3 floats each, internal order)
+pdx_status pdx_gnn_fields(pdx_ctx *c, const float **fields, int *n_fields);
 pdx_status pdx_gnn_record(pdx_ctx *c, const double *rhs, const double *du);  // Newton pair
-pdx_status pdx_gnn_record_krylov(pdx_ctx *c, const double *z, const double *jz);  // Krylov pair
+pdx_status pdx_gnn_record_krylov(pdx_ctx *c, const double *z, const double *jz);  // Krylov pair -> error sample
 bool pdx_gnn_collecting(pdx_ctx *c);
-void pdx_gnn_disable(pdx_ctx *c);  // the learned preconditioner falls back to two-level
+void pdx_gnn_disable(pdx_ctx *c);  // the learned preconditioner falls back to the translations
 bool pdx_gnn_active(pdx_ctx *c);   // the network is applied (trained, validated, not disabled)
 pdx_status pdx_nk_build_jacobi(pdx_ctx *c);  // point-block Jacobi at the current state
+pdx_status pdx_nk_aggregates(pdx_ctx *c);    // sets agg and nagg once per context
 pdx_status pdx_nk_build_coarse(pdx_ctx *c);  // two-level coarse operator at the current state
+// the same with the network's fields appended to the translations (n_fields per point)
+pdx_status pdx_nk_build_coarse_fields(pdx_ctx *c, const float *fields, int n_fields);
 pdx_status pdx_nk_coarse_add(pdx_ctx *c, const double *v, double *z, double alpha);  // z += a P Ac^-1 P^T v
+pdx_status pdx_nk_two_level(pdx_ctx *c, const double *v, double *z);  // z = D^-1 v + P Ac^-1 P^T v
 
 // preconditioners of pdx_nk_opts.precond
 #define PDX_PRE_NONE 0

@@ -19,12 +19,13 @@
 #include <cstring>
 #include <vector>
 
+#include <rocsolver/rocsolver.h>
+
 #include "pdx_internal.h"
 #include "pdx_nk.h"
 
 #define RED_BLOCKS 512
 #define PDX_MAX_RESTART 128
-#define NK_MAX_AGGS 256  // coarse aggregates (3 coarse dofs each)
 // device scalars of the Krylov solver
 #define NK_UN2 0  // ||u||^2
 #define NK_VN2 1  // ||v||^2 of the current J v product
@@ -37,8 +38,10 @@ void pdx_nk_free(pdx_ctx *c)
                     (void *)k->du, (void *)k->w, (void *)k->t0, (void *)k->Vb, (void *)k->Zb,
                     (void *)k->recp, (void *)k->recm, (void *)k->fp, (void *)k->fm,
                     (void *)k->red, (void *)k->dinv, (void *)k->h_dev, (void *)k->dsc,
-                    (void *)k->Ac, (void *)k->rc, (void *)k->zc})
+                    (void *)k->Ac, (void *)k->rc, (void *)k->zc, (void *)k->Ainv, (void *)k->phi,
+                    (void *)k->Ycs, (void *)k->pJ, (void *)k->aslot, (void *)k->dflag})
         if (p) (void)hipFree(p);
+    if (k->rb) rocblas_destroy_handle(k->rb);
     delete k;
     c->nk = nullptr;
 }
@@ -305,7 +308,12 @@ static inline dim3 grid1(int64_t m, int bs = 256) { return dim3((unsigned)((m + 
 
 static pdx_status nk_ensure(pdx_ctx *c, int restart)
 {
-    if (!c->nk) c->nk = new pdx_nk();
+    if (!c->nk) {
+        c->nk = new pdx_nk();
+        // PDX_NK_VERBOSE=1: one line per Newton iteration on stderr (diagnostics)
+        const char *e = getenv("PDX_NK_VERBOSE");
+        c->nk->verbose = e && atoi(e) > 0;
+    }
     pdx_nk *k = c->nk;
     const int64_t n = c->n, m = 3 * n;
     k->m = m;
@@ -402,181 +410,466 @@ pdx_status pdx_nk_jv_dev(pdx_ctx *c, const double *v, double *jv, double scale)
 }
 
 // ----------------------------------------------------------------------------------------
-// Two-level preconditioner: aggregates are runs of NK_AGG consecutive Morton points
-// (compact boxes); the coarse space holds the three translations of every aggregate on its
-// free dofs (piecewise-constant prolongation P).  The Galerkin operator A_c = P^T J P is
-// assembled from the analytic linearisation of the bond-based residual (the same bond
-// stiffness as the block Jacobi) and inverted once per load step on the device.
+// Two-level preconditioner.  Aggregates are runs of `agg` consecutive Morton points
+// (compact boxes).  Every point carries nf basis fields phi_i[f] (3-vectors, zero on
+// constrained dofs); the coarse space of aggregate I holds those nf fields restricted to
+// I, orthonormalised per aggregate.  The translations (nf = 3, phi_i = e_x, e_y, e_z) give
+// the classical aggregation space; the learned preconditioner appends the fields of its
+// network (pdx_gnn.hip).  The Galerkin operator A_c = P^T J P comes from the analytic
+// linearisation of the bond-based residual (the bond stiffness of the block Jacobi) in
+// two passes: Y_i,J = sum_{j in J} J_ij phi_j per point and neighbouring aggregate, then
+// A_c[(I,k),(J,l)] = sum_{i in I} phi_i[k] . Y_i,J[l].  -A_c is symmetric positive
+// definite: it is factored and inverted once per load step (rocSOLVER potrf + potri) and
+// kept as float32 for the per-iteration product.
 // z = D^-1 v + P A_c^-1 P^T v.
 // ----------------------------------------------------------------------------------------
-// A_c += contributions of the bonds of point i: J_ij = V_i C_ij to block (I, agg(j)) and
-// J_ii = -V_i sum_j C_ij to block (I, I), each masked to free rows (of i) and columns.
-__global__ __launch_bounds__(256) void k_coarse_assemble(int64_t n, int agg, int64_t mc,
-                                                         const int64_t *slice_ptr, const int32_t *cnt,
-                                                         const uint32_t *bw, const double *L,
-                                                         const double4 *rec, double bc,
-                                                         const pdx_pset *ps, const uint8_t *pid,
-                                                         const uint8_t *cmask, double *Ac)
+#define NK_TP 16    // aggregates a point's bonds may reach (slot 0: its own)
+#define NK_FCH 4    // basis fields per pass of the Y kernel
+
+// slots of a point: slot 0 is its own aggregate, the others in order of appearance along
+// its (neighbour-sorted) bonds
+__device__ __forceinline__ int nk_slot(int32_t *pj, int &ns, int64_t J, int *overflow)
+{
+    for (int t = 0; t < ns; t++)
+        if (pj[t] == J) return t;
+    if (ns >= NK_TP) {
+        *overflow = 1;
+        return -1;
+    }
+    pj[ns] = (int32_t)J;
+    return ns++;
+}
+
+// Y[(i*TP + t)*nf + f][d] = sum over the bonds of i into slot t's aggregate of C_ij phi_j[f],
+// plus -sum_j C_ij phi_i[f] in slot 0 (the diagonal block J_ii).  C_ij is the bond's
+// stiffness dr_i/du_j = V_i bc V_j [(1/L) n n^T + (eps/l)(I - n n^T)].
+__global__ __launch_bounds__(256) void k_coarse_y(int64_t n, int agg, int nf, const int64_t *slice_ptr,
+                                                  const int32_t *cnt, const uint32_t *bw,
+                                                  const double *L, const double4 *rec, double bc,
+                                                  const pdx_pset *ps, const uint8_t *pid,
+                                                  const double *phi, int32_t *pJ, double *Y,
+                                                  int *overflow)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int64_t I = i / agg;
     const int64_t b0 = slice_ptr[i >> 6] + (i & 63);
     const double4 ri = rec[i];
-    const bool fi[3] = {cmask[3 * i] == 0, cmask[3 * i + 1] == 0, cmask[3 * i + 2] == 0};
-    double Dg[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // -sum_j V_i C_ij (all bonds)
-    double Acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // run of bonds into aggregate Jc
-    int64_t Jc = -1;
-    auto flush = [&]() {
-        if (Jc < 0) return;
+    int32_t pj[NK_TP];
+    for (int f0 = 0; f0 < nf; f0 += NK_FCH) {
+        const int fn = min(NK_FCH, nf - f0);
+        int ns = 1;
+        pj[0] = (int32_t)I;
+        double S[3][NK_FCH], T[9];
+        for (int q = 0; q < 9; q++) T[q] = 0.0;
         for (int d = 0; d < 3; d++)
-            for (int e = 0; e < 3; e++)
-                if (Acc[3 * d + e] != 0.0) atomicAdd(&Ac[(3 * I + d) * mc + 3 * Jc + e], Acc[3 * d + e]);
-        for (int q = 0; q < 9; q++) Acc[q] = 0.0;
-    };
+            for (int q = 0; q < NK_FCH; q++) S[d][q] = 0.0;
+        int64_t Jc = -1;
+        auto flush = [&]() {
+            if (Jc < 0) return;
+            const int t = nk_slot(pj, ns, Jc, overflow);
+            if (t >= 0)
+                for (int q = 0; q < fn; q++)
+                    for (int d = 0; d < 3; d++) {
+                        double *y = Y + ((i * NK_TP + t) * nf + f0 + q) * 3 + d;
+                        *y += S[d][q];
+                    }
+            for (int d = 0; d < 3; d++)
+                for (int q = 0; q < NK_FCH; q++) S[d][q] = 0.0;
+        };
+        for (int t = 0; t < cnt[i]; t++) {
+            const int64_t b = b0 + (int64_t)t * 64;
+            const uint32_t wd = bw[b];
+            if (!(wd & PDX_BW_ACT)) continue;
+            const int64_t j = wd & PDX_BW_NBR;
+            const int64_t J = j / agg;
+            if (J != Jc) {
+                flush();
+                Jc = J;
+            }
+            const double4 rj = rec[j];
+            const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
+            const double l = sqrt((dx * dx + dy * dy) + dz * dz);
+            const double nv[3] = {dx / l, dy / l, dz / l};
+            const double Lb = L[b];
+            const double eps = (l - Lb) / Lb;
+            const double bcb = ps ? (ps[pid[i]].bc + ps[pid[j]].bc) / 2.0 : bc;
+            const double a = ri.w * bcb * rj.w / Lb, g = ri.w * bcb * rj.w * eps / l;
+            double C[9];
+            for (int d = 0; d < 3; d++)
+                for (int e = 0; e < 3; e++) C[3 * d + e] = (a - g) * nv[d] * nv[e] + (d == e ? g : 0.0);
+            for (int q = 0; q < 9; q++) T[q] -= C[q];
+            const double *pjf = phi + (j * nf + f0) * 3;
+            for (int q = 0; q < fn; q++)
+                for (int d = 0; d < 3; d++)
+                    S[d][q] += (C[3 * d] * pjf[3 * q] + C[3 * d + 1] * pjf[3 * q + 1]) + C[3 * d + 2] * pjf[3 * q + 2];
+        }
+        flush();
+        // diagonal block into slot 0
+        const double *pif = phi + (i * nf + f0) * 3;
+        for (int q = 0; q < fn; q++)
+            for (int d = 0; d < 3; d++) {
+                double *y = Y + ((i * NK_TP) * nf + f0 + q) * 3 + d;
+                *y += (T[3 * d] * pif[3 * q] + T[3 * d + 1] * pif[3 * q + 1]) + T[3 * d + 2] * pif[3 * q + 2];
+            }
+        if (f0 == 0)
+            for (int t = 0; t < NK_TP; t++) pJ[i * NK_TP + t] = t < ns ? pj[t] : -1;
+    }
+}
+
+// A_c[(I,k),(J,l)] = sum_{i in I} phi_i[k] . Y_i,J[l]: one workgroup per (aggregate,
+// neighbour-aggregate slot), one thread per (k, l)
+__global__ __launch_bounds__(256) void k_coarse_gather(int64_t n, int agg, int nf, int maxs,
+                                                       const int32_t *aslot, const int32_t *pJ,
+                                                       const double *phi, const double *Y,
+                                                       int64_t mc, double *Ac)
+{
+    const int64_t I = blockIdx.x / maxs;
+    const int s = (int)(blockIdx.x % maxs);
+    const int32_t J = aslot[I * maxs + s];
+    if (J < 0) return;
+    const int kk = threadIdx.x / nf, ll = threadIdx.x % nf;
+    const bool on = (int)threadIdx.x < nf * nf;
+    const int64_t p0 = I * agg, p1 = min(n, p0 + agg);
+    double acc = 0.0;
+    for (int64_t i = p0; i < p1; i++) {
+        int t = -1;
+        for (int q = 0; q < NK_TP; q++)
+            if (pJ[i * NK_TP + q] == J) {
+                t = q;
+                break;
+            }
+        if (t < 0 || !on) continue;
+        const double *pk = phi + (i * nf + kk) * 3;
+        const double *yl = Y + ((i * NK_TP + t) * nf + ll) * 3;
+        acc += (pk[0] * yl[0] + pk[1] * yl[1]) + pk[2] * yl[2];
+    }
+    if (on) Ac[(I * nf + kk) * mc + (int64_t)J * nf + ll] = acc;
+}
+
+// the number of aggregates (its own included) that the bonds of a point reach, maximised
+// over the points into *out
+__global__ void k_coarse_reach(int64_t n, int agg, const int64_t *slice_ptr, const int32_t *cnt,
+                               const uint32_t *bw, int *out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t I = i / agg;
+    const int64_t b0 = slice_ptr[i >> 6] + (i & 63);
+    int runs = 0;
+    bool own = false;
+    int64_t Jc = -1;
     for (int t = 0; t < cnt[i]; t++) {
-        const int64_t b = b0 + (int64_t)t * 64;
-        const uint32_t wd = bw[b];
-        if (!(wd & PDX_BW_ACT)) continue;
-        const int64_t j = wd & PDX_BW_NBR;
-        const double4 rj = rec[j];
-        const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
-        const double l = sqrt((dx * dx + dy * dy) + dz * dz);
-        const double nv[3] = {dx / l, dy / l, dz / l};
-        const double Lb = L[b];
-        const double eps = (l - Lb) / Lb;
-        const double bcb = ps ? (ps[pid[i]].bc + ps[pid[j]].bc) / 2.0 : bc;  // parameter sets
-        const double a = ri.w * bcb * rj.w / Lb, g = ri.w * bcb * rj.w * eps / l;
-        const int64_t J = j / agg;
+        const int64_t J = (int64_t)(bw[b0 + (int64_t)t * 64] & PDX_BW_NBR) / agg;
         if (J != Jc) {
-            flush();
+            runs++;
+            own |= (J == I);
             Jc = J;
         }
-        const bool fj[3] = {cmask[3 * j] == 0, cmask[3 * j + 1] == 0, cmask[3 * j + 2] == 0};
-        for (int d = 0; d < 3; d++)
-            for (int e = 0; e < 3; e++) {
-                const double C = (a - g) * nv[d] * nv[e] + (d == e ? g : 0.0);
-                if (fi[d] && fj[e]) Acc[3 * d + e] += C;
-                if (fi[d] && fi[e]) Dg[3 * d + e] -= C;
-            }
     }
-    flush();
-    for (int d = 0; d < 3; d++)
-        for (int e = 0; e < 3; e++)
-            if (Dg[3 * d + e] != 0.0) atomicAdd(&Ac[(3 * I + d) * mc + 3 * I + e], Dg[3 * d + e]);
+    atomicMax(out, runs + (own ? 0 : 1));
 }
 
-// a coarse dof with no free fine dof (all constrained) gets an identity row
-__global__ void k_coarse_fix(int64_t mc, double *Ac)
+// neighbouring aggregates: touch[I * nagg + J] = 1 if a point of I has a bond slot into J
+__global__ void k_coarse_touch(int64_t n, int agg, int64_t nagg, const int32_t *pJ, uint8_t *touch)
 {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= mc) return;
-    if (Ac[r * mc + r] == 0.0) Ac[r * mc + r] = 1.0;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t I = i / agg;
+    for (int t = 0; t < NK_TP; t++) {
+        const int32_t J = pJ[i * NK_TP + t];
+        if (J >= 0) touch[I * nagg + J] = 1;
+    }
 }
 
-// in-place Gauss-Jordan inversion, pivot k: trailing update, then the pivot row/column.
-// A_c is negative definite (J = -stiffness), so no pivoting is needed.
-__global__ void k_gj_update(int64_t mc, int64_t k, double *A)
+// B = -A_c with an identity row for a coarse dof whose field vanishes on the aggregate
+__global__ void k_coarse_negate(int64_t mc, double *A)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= mc * mc) return;
-    const int64_t i = e / mc, j = e - i * mc;
-    if (i == k || j == k) return;
-    A[e] -= A[i * mc + k] * A[k * mc + j] / A[k * mc + k];
+    const int64_t r = e / mc, cc = e - r * mc;
+    double v = -A[e];
+    if (r == cc && v == 0.0) v = 1.0;
+    A[e] = v;
 }
 
-__global__ void k_gj_pivot(int64_t mc, int64_t k, double *A)
+// the inverse of -A_c (lower triangle, column-major, from potri) -> A_c^-1 in float32, both
+// triangles
+__global__ void k_coarse_finish(int64_t mc, const double *B, float *Ainv)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= mc) return;
-    const double p = 1.0 / A[k * mc + k];
-    if (t != k) {
-        A[t * mc + k] = -A[t * mc + k] * p;
-        A[k * mc + t] = A[k * mc + t] * p;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= mc * mc) return;
+    const int64_t r = e / mc, cc = e - r * mc;
+    const double v = r >= cc ? B[cc * mc + r] : B[r * mc + cc];
+    Ainv[e] = (float)(-v);
+}
+
+// basis fields of every point: the three translations, then `nl` learned fields (float,
+// 3 per field), zero on constrained dofs
+__global__ void k_coarse_basis(int64_t n, int nl, const float *learned, const uint8_t *cmask,
+                               double *phi)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int nf = 3 + nl;
+    double *p = phi + i * nf * 3;
+    for (int f = 0; f < 3; f++)
+        for (int d = 0; d < 3; d++) p[3 * f + d] = (d == f && !cmask[3 * i + d]) ? 1.0 : 0.0;
+    for (int f = 0; f < nl; f++)
+        for (int d = 0; d < 3; d++)
+            p[3 * (3 + f) + d] = cmask[3 * i + d] ? 0.0 : (double)learned[i * 3 * nl + 3 * f + d];
+}
+
+// per aggregate: Gram-Schmidt (twice) of its nf fields over its points; a field whose
+// remainder is below 1e-6 of its norm is dropped (zeroed).  One workgroup per aggregate.
+#define NK_MAXF 16
+__global__ __launch_bounds__(256) void k_coarse_orth(int64_t n, int agg, int nf, double *phi)
+{
+    __shared__ double red[256];
+    const int64_t I = blockIdx.x;
+    const int64_t p0 = I * agg, p1 = min(n, p0 + agg);
+    for (int f = 0; f < nf; f++) {
+        double nrm0 = 0.0;
+        for (int pass = 0; pass < 3; pass++) {
+            // pass 0: the field's norm; passes 1, 2: remove the earlier fields
+            for (int g = 0; g < (pass == 0 ? 1 : f); g++) {
+                double s = 0.0;
+                for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
+                    const double *a = phi + (p * nf + f) * 3;
+                    const double *b = phi + (p * nf + (pass == 0 ? f : g)) * 3;
+                    s += (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+                }
+                red[threadIdx.x] = s;
+                __syncthreads();
+                for (int o = 128; o > 0; o >>= 1) {
+                    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+                    __syncthreads();
+                }
+                const double dot = red[0];
+                __syncthreads();
+                if (pass == 0) {
+                    nrm0 = sqrt(dot);
+                } else {
+                    for (int64_t p = p0 + threadIdx.x; p < p1; p += 256)
+                        for (int d = 0; d < 3; d++)
+                            phi[(p * nf + f) * 3 + d] -= dot * phi[(p * nf + g) * 3 + d];
+                }
+                __syncthreads();
+            }
+        }
+        double s = 0.0;
+        for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
+            const double *a = phi + (p * nf + f) * 3;
+            s += (a[0] * a[0] + a[1] * a[1]) + a[2] * a[2];
+        }
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+            __syncthreads();
+        }
+        const double nr = sqrt(red[0]);
+        __syncthreads();
+        const double sc = (nr > 1e-6 * nrm0 && nr > 0.0) ? 1.0 / nr : 0.0;
+        for (int64_t p = p0 + threadIdx.x; p < p1; p += 256)
+            for (int d = 0; d < 3; d++) phi[(p * nf + f) * 3 + d] *= sc;
+        __syncthreads();
     }
 }
 
-__global__ void k_gj_diag(int64_t mc, int64_t k, double *A) { A[k * mc + k] = 1.0 / A[k * mc + k]; }
-
-// rc[3I+d] = sum over free dofs d of the points of aggregate I of v; one workgroup per I
-__global__ __launch_bounds__(256) void k_coarse_restrict(int64_t n, int agg, const double *v,
-                                                         const uint8_t *cmask, double *rc)
+// rc[I*nf + f] = sum_{i in I} phi_i[f] . v_i; one workgroup per aggregate
+__global__ __launch_bounds__(256) void k_coarse_restrict(int64_t n, int agg, int nf, const double *phi,
+                                                         const double *v, double *rc)
 {
-    __shared__ double sh[3][256];
+    __shared__ double sh[NK_MAXF][256];
     const int64_t I = blockIdx.x;
     const int64_t p0 = I * agg, p1 = min(n, p0 + agg);
-    double s[3] = {0, 0, 0};
-    for (int64_t p = p0 + threadIdx.x; p < p1; p += 256)
-        for (int d = 0; d < 3; d++)
-            if (!cmask[3 * p + d]) s[d] += v[3 * p + d];
-    for (int d = 0; d < 3; d++) sh[d][threadIdx.x] = s[d];
+    double s[NK_MAXF];
+    for (int f = 0; f < NK_MAXF; f++) s[f] = 0.0;
+    for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
+        const double v0 = v[3 * p], v1 = v[3 * p + 1], v2 = v[3 * p + 2];
+        for (int f = 0; f < nf; f++) {
+            const double *a = phi + (p * nf + f) * 3;
+            s[f] += (a[0] * v0 + a[1] * v1) + a[2] * v2;
+        }
+    }
+    for (int f = 0; f < nf; f++) sh[f][threadIdx.x] = s[f];
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o)
-            for (int d = 0; d < 3; d++) sh[d][threadIdx.x] += sh[d][threadIdx.x + o];
+            for (int f = 0; f < nf; f++) sh[f][threadIdx.x] += sh[f][threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x < 3) rc[3 * I + threadIdx.x] = sh[threadIdx.x][0];
+    if ((int)threadIdx.x < nf) rc[I * nf + threadIdx.x] = sh[threadIdx.x][0];
 }
 
 // zc = Ainv rc: one wavefront per row, coalesced along the row, shuffle reduction
-__global__ __launch_bounds__(64) void k_coarse_matvec(int64_t mc, const double *Ainv, const double *rc,
+__global__ __launch_bounds__(64) void k_coarse_matvec(int64_t mc, const float *Ainv, const double *rc,
                                                       double *zc)
 {
     const int64_t r = blockIdx.x;
-    const double *row = Ainv + r * mc;
+    const float *row = Ainv + r * mc;
     double s = 0.0;
-    for (int64_t k = threadIdx.x; k < mc; k += 64) s += row[k] * rc[k];
+    for (int64_t k = threadIdx.x; k < mc; k += 64) s += (double)row[k] * rc[k];
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
     if (threadIdx.x == 0) zc[r] = s;
 }
 
-// z = D^-1 v (+ P zc on free dofs when zc is given)
-__global__ void k_apply_two_level(int64_t n, int agg, const double *dinv, const double *v,
-                                  const double *zc, const uint8_t *cmask, double *z)
+// z = a D^-1 v + b P zc  (a = 0: the prolongation alone, added to z when b is given and
+// `add` is set)
+__global__ void k_apply_two_level(int64_t n, int agg, int nf, const double *dinv, const double *v,
+                                  const double *zc, const double *phi, double alpha, int add,
+                                  double *z)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double *R = dinv + 9 * i;
-    const double a = v[3 * i], b = v[3 * i + 1], cc = v[3 * i + 2];
-    double o[3] = {R[0] * a + R[3] * b + R[6] * cc, R[1] * a + R[4] * b + R[7] * cc,
-                   R[2] * a + R[5] * b + R[8] * cc};
-    if (zc) {
-        const int64_t I = i / agg;
-        for (int d = 0; d < 3; d++)
-            if (!cmask[3 * i + d]) o[d] += zc[3 * I + d];
+    double o[3] = {0.0, 0.0, 0.0};
+    if (add) {
+        o[0] = z[3 * i];
+        o[1] = z[3 * i + 1];
+        o[2] = z[3 * i + 2];
+    } else {
+        const double *R = dinv + 9 * i;
+        const double a = v[3 * i], b = v[3 * i + 1], cc = v[3 * i + 2];
+        o[0] = R[0] * a + R[3] * b + R[6] * cc;
+        o[1] = R[1] * a + R[4] * b + R[7] * cc;
+        o[2] = R[2] * a + R[5] * b + R[8] * cc;
+    }
+    const int64_t I = i / agg;
+    for (int f = 0; f < nf; f++) {
+        const double w = alpha * zc[I * nf + f];
+        const double *p = phi + (i * nf + f) * 3;
+        o[0] += w * p[0];
+        o[1] += w * p[1];
+        o[2] += w * p[2];
     }
     for (int d = 0; d < 3; d++) z[3 * i + d] = o[d];
 }
 
-// coarse operator and its inverse at the current state
-static pdx_status build_coarse(pdx_ctx *c)
+// aggregate size: pdx_nk_agg_size(n), doubled until no point's bonds reach more than
+// NK_TP aggregates (its own included)
+pdx_status pdx_nk_aggregates(pdx_ctx *c)
+{
+    pdx_nk *k = c->nk;
+    if (k->agg) return PDX_OK;
+    const int64_t n = c->n;
+    if (!k->dflag) PDX_TRY(pdx_alloc(c, &k->dflag, 4));
+    for (int64_t agg = pdx_nk_agg_size(n);; agg *= 2) {
+        PDX_HIP(c, hipMemsetAsync(k->dflag, 0, sizeof(int), c->stream));
+        hipLaunchKernelGGL(k_coarse_reach, grid1(n), dim3(256), 0, c->stream, n, (int)agg, c->slice_ptr,
+                           c->cnt, c->bw, k->dflag);
+        PDX_HIP(c, hipGetLastError());
+        int reach = 0;
+        PDX_HIP(c, hipMemcpyAsync(&reach, k->dflag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        PDX_HIP(c, hipStreamSynchronize(c->stream));
+        if (reach <= NK_TP || agg >= n) {
+            if (reach > NK_TP) return pdx_fail(c, PDX_ERR_UNSUPPORTED, "a point reaches more than %d aggregates", NK_TP);
+            k->agg = (int)agg;
+            k->nagg = (n + agg - 1) / agg;
+            return PDX_OK;
+        }
+    }
+}
+
+static pdx_status nk_rocsolver(pdx_ctx *c, rocblas_status st, const char *what)
+{
+    if (st != rocblas_status_success)
+        return pdx_fail(c, PDX_ERR_HIP, "%s failed (rocblas status %d)", what, (int)st);
+    return PDX_OK;
+}
+
+// the coarse space, operator and inverse at the current state.  learned_fields: the
+// network's fields (float, 3 * n_learned per point, internal order), or null for the
+// translations alone.
+static pdx_status build_coarse(pdx_ctx *c, const float *learned_fields, int n_learned)
 {
     pdx_nk *k = c->nk;
     if (c->kind != PDX_BB) return pdx_fail(c, PDX_ERR_UNSUPPORTED, "the two-level preconditioner is built for BBMaterial");
-    if (!k->Ac) {
-        // aggregates of a power-of-two size >= 64, at most NK_MAX_AGGS of them
-        int64_t agg = 64;
-        while ((c->n + agg - 1) / agg > NK_MAX_AGGS) agg *= 2;
-        k->agg = (int)agg;
-        k->nagg = (c->n + agg - 1) / agg;
-        k->mc = 3 * k->nagg;
-        PDX_TRY(pdx_alloc(c, &k->Ac, (size_t)(k->mc * k->mc)));
-        PDX_TRY(pdx_alloc(c, &k->rc, (size_t)k->mc));
-        PDX_TRY(pdx_alloc(c, &k->zc, (size_t)k->mc));
+    const int nf = 3 + (learned_fields ? n_learned : 0);
+    if (nf > NK_MAXF) return pdx_fail(c, PDX_ERR_ARG, "at most %d coarse fields per aggregate", NK_MAXF);
+    const int64_t n = c->n;
+    hipStream_t s = c->stream;
+    if (!k->pJ) {
+        PDX_TRY(pdx_nk_aggregates(c));
+        PDX_TRY(pdx_alloc(c, &k->pJ, (size_t)n * NK_TP));
+        if (rocblas_create_handle(&k->rb) != rocblas_status_success)
+            return pdx_fail(c, PDX_ERR_HIP, "rocblas_create_handle failed");
     }
+    if (nf > k->nf_cap) {
+        for (void *p : {(void *)k->phi, (void *)k->Ycs, (void *)k->Ac, (void *)k->Ainv, (void *)k->rc,
+                        (void *)k->zc})
+            if (p) (void)hipFree(p);
+        k->phi = k->Ycs = k->Ac = k->rc = k->zc = nullptr;
+        k->Ainv = nullptr;
+        const int64_t mc = k->nagg * nf;
+        PDX_TRY(pdx_alloc(c, &k->phi, (size_t)n * nf * 3));
+        PDX_TRY(pdx_alloc(c, &k->Ycs, (size_t)n * NK_TP * nf * 3));
+        PDX_TRY(pdx_alloc(c, &k->Ac, (size_t)(mc * mc)));
+        PDX_TRY(pdx_alloc(c, &k->Ainv, (size_t)(mc * mc)));
+        PDX_TRY(pdx_alloc(c, &k->rc, (size_t)mc));
+        PDX_TRY(pdx_alloc(c, &k->zc, (size_t)mc));
+        k->nf_cap = nf;
+    }
+    k->nf = nf;
+    k->mc = k->nagg * nf;
     const int64_t mc = k->mc, mm = mc * mc;
-    PDX_HIP(c, hipMemsetAsync(k->Ac, 0, sizeof(double) * mm, c->stream));
-    hipLaunchKernelGGL(k_coarse_assemble, grid1(c->n), dim3(256), 0, c->stream, c->n, k->agg, mc,
-                       c->slice_ptr, c->cnt, c->bw, c->L, c->rec[c->cur], c->P.bc,
-                       c->multi() ? c->d_psets : nullptr, c->d_pid, k->cmask, k->Ac);
-    hipLaunchKernelGGL(k_coarse_fix, grid1(mc), dim3(256), 0, c->stream, mc, k->Ac);
+    hipLaunchKernelGGL(k_coarse_basis, grid1(n), dim3(256), 0, s, n, nf - 3, learned_fields, k->cmask, k->phi);
+    hipLaunchKernelGGL(k_coarse_orth, dim3((unsigned)k->nagg), dim3(256), 0, s, n, k->agg, nf, k->phi);
+    PDX_HIP(c, hipMemsetAsync(k->Ycs, 0, sizeof(double) * n * NK_TP * nf * 3, s));
+    PDX_HIP(c, hipMemsetAsync(k->dflag, 0, sizeof(int) * 4, s));
+    hipLaunchKernelGGL(k_coarse_y, grid1(n), dim3(256), 0, s, n, k->agg, nf, c->slice_ptr, c->cnt, c->bw,
+                       c->L, c->rec[c->cur], c->P.bc, c->multi() ? c->d_psets : nullptr, c->d_pid,
+                       k->phi, k->pJ, k->Ycs, k->dflag);
     PDX_HIP(c, hipGetLastError());
-    for (int64_t q = 0; q < mc; q++) {
-        hipLaunchKernelGGL(k_gj_update, grid1(mm), dim3(256), 0, c->stream, mc, q, k->Ac);
-        hipLaunchKernelGGL(k_gj_pivot, grid1(mc), dim3(256), 0, c->stream, mc, q, k->Ac);
-        hipLaunchKernelGGL(k_gj_diag, dim3(1), dim3(1), 0, c->stream, mc, q, k->Ac);
+    int ovf = 0;
+    PDX_HIP(c, hipMemcpyAsync(&ovf, k->dflag, sizeof(int), hipMemcpyDeviceToHost, s));
+    PDX_HIP(c, hipStreamSynchronize(s));
+    if (ovf) return pdx_fail(c, PDX_ERR_UNSUPPORTED, "a point reaches more than %d aggregates", NK_TP - 1);
+    if (k->maxs == 0) {
+        // neighbouring aggregates of every aggregate (the bond set of a Newton-Krylov body
+        // does not change)
+        const int64_t na = k->nagg;
+        uint8_t *touch = nullptr;
+        PDX_HIP(c, hipMalloc((void **)&touch, na * na));
+        PDX_HIP(c, hipMemsetAsync(touch, 0, na * na, s));
+        hipLaunchKernelGGL(k_coarse_touch, grid1(n), dim3(256), 0, s, n, k->agg, na, k->pJ, touch);
+        std::vector<uint8_t> h(na * na);
+        PDX_HIP(c, hipMemcpyAsync(h.data(), touch, na * na, hipMemcpyDeviceToHost, s));
+        PDX_HIP(c, hipStreamSynchronize(s));
+        (void)hipFree(touch);
+        int maxs = 1;
+        for (int64_t I = 0; I < na; I++) {
+            int cntI = 0;
+            for (int64_t J = 0; J < na; J++) cntI += h[I * na + J];
+            maxs = std::max(maxs, cntI);
+        }
+        std::vector<int32_t> slots(na * maxs, -1);
+        for (int64_t I = 0; I < na; I++) {
+            int q = 0;
+            for (int64_t J = 0; J < na; J++)
+                if (h[I * na + J]) slots[I * maxs + q++] = (int32_t)J;
+        }
+        PDX_TRY(pdx_alloc(c, &k->aslot, slots.size()));
+        PDX_HIP(c, hipMemcpy(k->aslot, slots.data(), sizeof(int32_t) * slots.size(), hipMemcpyHostToDevice));
+        k->maxs = maxs;
     }
+    PDX_HIP(c, hipMemsetAsync(k->Ac, 0, sizeof(double) * mm, s));
+    hipLaunchKernelGGL(k_coarse_gather, dim3((unsigned)(k->nagg * k->maxs)), dim3(256), 0, s, n, k->agg, nf,
+                       k->maxs, k->aslot, k->pJ, k->phi, k->Ycs, mc, k->Ac);
+    hipLaunchKernelGGL(k_coarse_negate, grid1(mm), dim3(256), 0, s, mc, k->Ac);
+    PDX_HIP(c, hipGetLastError());
+    rocblas_set_stream(k->rb, s);
+    PDX_TRY(nk_rocsolver(c, rocsolver_dpotrf(k->rb, rocblas_fill_lower, (rocblas_int)mc, k->Ac,
+                                              (rocblas_int)mc, k->dflag + 1), "rocsolver_dpotrf"));
+    PDX_TRY(nk_rocsolver(c, rocsolver_dpotri(k->rb, rocblas_fill_lower, (rocblas_int)mc, k->Ac,
+                                              (rocblas_int)mc, k->dflag + 2), "rocsolver_dpotri"));
+    int info[2] = {0, 0};
+    PDX_HIP(c, hipMemcpyAsync(info, k->dflag + 1, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+    PDX_HIP(c, hipStreamSynchronize(s));
+    if (info[0] != 0 || info[1] != 0)
+        return pdx_fail(c, PDX_ERR_HIP, "the coarse operator is not definite (potrf info %d, potri info %d)",
+                        info[0], info[1]);
+    hipLaunchKernelGGL(k_coarse_finish, grid1(mm), dim3(256), 0, s, mc, k->Ac, k->Ainv);
     PDX_HIP(c, hipGetLastError());
     k->coarse_ready = true;
     return PDX_OK;
@@ -588,34 +881,24 @@ pdx_status pdx_nk_two_level(pdx_ctx *c, const double *v, double *z)
     pdx_nk *k = c->nk;
     if (!k->coarse_ready) return pdx_fail(c, PDX_ERR_STATE, "coarse space not built");
     hipLaunchKernelGGL(k_coarse_restrict, dim3((unsigned)k->nagg), dim3(256), 0, c->stream, c->n, k->agg,
-                       v, k->cmask, k->rc);
-    hipLaunchKernelGGL(k_coarse_matvec, dim3((unsigned)k->mc), dim3(64), 0, c->stream, k->mc, k->Ac, k->rc, k->zc);
-    hipLaunchKernelGGL(k_apply_two_level, grid1(c->n), dim3(256), 0, c->stream, c->n, k->agg, k->dinv, v,
-                       k->zc, k->cmask, z);
+                       k->nf, k->phi, v, k->rc);
+    hipLaunchKernelGGL(k_coarse_matvec, dim3((unsigned)k->mc), dim3(64), 0, c->stream, k->mc, k->Ainv, k->rc, k->zc);
+    hipLaunchKernelGGL(k_apply_two_level, grid1(c->n), dim3(256), 0, c->stream, c->n, k->agg, k->nf, k->dinv, v,
+                       k->zc, k->phi, 1.0, 0, z);
     PDX_HIP(c, hipGetLastError());
     return PDX_OK;
 }
 
-// z += alpha P A_c^-1 P^T v on free dofs
-__global__ void k_coarse_prolong_add(int64_t n, int agg, const double *zc, const uint8_t *cmask,
-                                     double alpha, double *z)
-{
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 3 * n) return;
-    if (cmask[t]) return;
-    const int64_t i = t / 3;
-    z[t] += alpha * zc[3 * (i / agg) + (t - 3 * i)];
-}
-
+// z += alpha P A_c^-1 P^T v
 pdx_status pdx_nk_coarse_add(pdx_ctx *c, const double *v, double *z, double alpha)
 {
     pdx_nk *k = c->nk;
     if (!k->coarse_ready) return pdx_fail(c, PDX_ERR_STATE, "coarse space not built");
     hipLaunchKernelGGL(k_coarse_restrict, dim3((unsigned)k->nagg), dim3(256), 0, c->stream, c->n, k->agg,
-                       v, k->cmask, k->rc);
-    hipLaunchKernelGGL(k_coarse_matvec, dim3((unsigned)k->mc), dim3(64), 0, c->stream, k->mc, k->Ac, k->rc, k->zc);
-    hipLaunchKernelGGL(k_coarse_prolong_add, grid1(k->m), dim3(256), 0, c->stream, c->n, k->agg, k->zc,
-                       k->cmask, alpha, z);
+                       k->nf, k->phi, v, k->rc);
+    hipLaunchKernelGGL(k_coarse_matvec, dim3((unsigned)k->mc), dim3(64), 0, c->stream, k->mc, k->Ainv, k->rc, k->zc);
+    hipLaunchKernelGGL(k_apply_two_level, grid1(c->n), dim3(256), 0, c->stream, c->n, k->agg, k->nf, k->dinv, v,
+                       k->zc, k->phi, alpha, 1, z);
     PDX_HIP(c, hipGetLastError());
     return PDX_OK;
 }
@@ -630,7 +913,7 @@ static pdx_status precond_apply(pdx_ctx *c, int which, const double *v, double *
         PDX_HIP(c, hipGetLastError());
         return PDX_OK;
     case PDX_PRE_TWO_LEVEL: return pdx_nk_two_level(c, v, z);
-    case PDX_PRE_LEARNED: return pdx_gnn_apply(c, v, z);
+    case PDX_PRE_LEARNED: return pdx_gnn_apply(c, v, z);  // two-level with the learned fields
     default:
         PDX_HIP(c, hipMemcpyAsync(z, v, sizeof(double) * k->m, hipMemcpyDeviceToDevice, c->stream));
         return PDX_OK;
@@ -656,7 +939,13 @@ pdx_status pdx_nk_build_jacobi(pdx_ctx *c)
 pdx_status pdx_nk_build_coarse(pdx_ctx *c)
 {
     if (!c->nk) return pdx_fail(c, PDX_ERR_STATE, "Newton-Krylov workspace missing");
-    return build_coarse(c);
+    return build_coarse(c, nullptr, 0);
+}
+
+pdx_status pdx_nk_build_coarse_fields(pdx_ctx *c, const float *fields, int n_fields)
+{
+    if (!c->nk) return pdx_fail(c, PDX_ERR_STATE, "Newton-Krylov workspace missing");
+    return build_coarse(c, fields, n_fields);
 }
 
 // Flexible GMRES(restart) for J du = b, du in k->du (starts at zero).  Stops at
@@ -794,6 +1083,20 @@ extern "C" pdx_status pdx_set_displacement_bcs(pdx_ctx *c, int64_t n, const int3
     return PDX_OK;
 }
 
+extern "C" pdx_status pdx_nk_coarse_info(pdx_ctx *c, int32_t *agg, int64_t *n_aggregates,
+                                         int32_t *fields)
+{
+    if (!c) return PDX_ERR_ARG;
+    if (!c->bonds_built) return pdx_fail(c, PDX_ERR_STATE, "pdx_build_bonds first");
+    PDX_HIP(c, hipSetDevice(c->device));
+    PDX_TRY(nk_ensure(c, 1));
+    PDX_TRY(pdx_nk_aggregates(c));
+    if (agg) *agg = c->nk->agg;
+    if (n_aggregates) *n_aggregates = c->nk->nagg;
+    if (fields) *fields = c->nk->coarse_ready ? c->nk->nf : 0;
+    return PDX_OK;
+}
+
 extern "C" pdx_status pdx_residual(pdx_ctx *c, double *r_host, double *norm_out)
 {
     if (!c) return PDX_ERR_ARG;
@@ -871,29 +1174,35 @@ extern "C" pdx_status pdx_nk_step(pdx_ctx *c, int64_t n, int64_t n_steps, const 
         PDX_HIP(c, hipGetLastError());
         const int pre = o->precond;
         if (pre != PDX_PRE_NONE) PDX_TRY(build_jacobi(c));
-        // the coarse operator of the two-level and learned preconditioners: once per load step
-        if ((pre == PDX_PRE_TWO_LEVEL || pre == PDX_PRE_LEARNED) && iter == 1) PDX_TRY(build_coarse(c));
+        // the coarse operator of the two-level and learned preconditioners: once per load
+        // step (the learned one spans the translations and the network's fields)
+        if (pre == PDX_PRE_TWO_LEVEL && iter == 1) PDX_TRY(build_coarse(c, nullptr, 0));
+        if (pre == PDX_PRE_LEARNED && iter == 1) {
+            PDX_TRY(pdx_gnn_prepare(c));
+            const float *fl = nullptr;
+            int nl = 0;
+            if (pdx_gnn_active(c)) PDX_TRY(pdx_gnn_fields(c, &fl, &nl));
+            PDX_TRY(build_coarse(c, fl, nl));
+        }
         int its = 0;
         double rel = 0.0;
         if (pre == PDX_PRE_LEARNED) {
-            PDX_TRY(pdx_gnn_prepare(c));
-            // with the network switched off the learned preconditioner is the two-level one:
-            // nothing to fall back to
-            const bool net = pdx_gnn_active(c) || !k->coarse_ready;
+            // with the network switched off the learned preconditioner is the two-level one
+            // of the translations: nothing to fall back to
+            const bool net = pdx_gnn_active(c);
             PDX_TRY(fgmres(c, k->t0, o->gmres_restart, o->gmres_maxiter, o->gmres_reltol,
                            o->gmres_abstol, PDX_PRE_LEARNED, scale, &its, &rel));
             S.krylov_iters += its;
             // fallback: a learned solve that did not reach the tolerance and did not reduce
-            // the linear residual tenfold is redone with the preconditioner the network
-            // corrects (two-level; block Jacobi if the coarse space is unavailable)
+            // the linear residual tenfold is redone with the two-level preconditioner of the
+            // translations, and the network is not applied again in this context
             const double tol_lin = std::max(o->gmres_reltol, o->gmres_abstol / std::max(rn, 1e-300));
             if (net && !(rel <= tol_lin * 1.0000001) && !(rel <= 0.1)) {
                 S.precond_fallbacks++;
-                // a network that fails a solve is not applied again in this context
                 pdx_gnn_disable(c);
-                const int fb = k->coarse_ready ? PDX_PRE_TWO_LEVEL : PDX_PRE_JACOBI;
+                PDX_TRY(build_coarse(c, nullptr, 0));
                 PDX_TRY(fgmres(c, k->t0, o->gmres_restart, o->gmres_maxiter, o->gmres_reltol,
-                               o->gmres_abstol, fb, scale, &its, &rel));
+                               o->gmres_abstol, PDX_PRE_TWO_LEVEL, scale, &its, &rel));
                 S.krylov_iters += its;
             }
         } else {
@@ -901,6 +1210,9 @@ extern "C" pdx_status pdx_nk_step(pdx_ctx *c, int64_t n, int64_t n_steps, const 
                            o->gmres_abstol, pre, scale, &its, &rel));
             S.krylov_iters += its;
         }
+        if (k->verbose)
+            fprintf(stderr, "[pdx nk] step %lld newton %d: |r| %.6e, krylov %d, linear rel %.3e\n",
+                    (long long)n, iter, rn, its, rel);
         if (pdx_gnn_collecting(c)) PDX_TRY(pdx_gnn_record(c, k->t0, k->du));
         hipLaunchKernelGGL(k_nk_update, grid1(k->m), dim3(256), 0, c->stream, c->n, k->du, k->cmask,
                            c->u, c->rec[c->cur]);

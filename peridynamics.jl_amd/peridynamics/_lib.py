@@ -136,6 +136,8 @@ SIGNATURES = {
     "pdx_set_displacement_bcs": (_ST, [_P, _I64, _P, _P, _P]),
     "pdx_jv": (_ST, [_P, _P, _P, _D]),
     "pdx_residual": (_ST, [_P, _P, ctypes.POINTER(_D)]),
+    "pdx_nk_coarse_info": (_ST, [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I64),
+                                 ctypes.POINTER(_I32)]),
     "pdx_nk_step": (_ST, [_P, _I64, _I64, ctypes.POINTER(PdxNkOpts),
                           ctypes.POINTER(PdxNkStats)]),
     "pdx_gnn_collect": (_ST, [_P, _I32, _I64]),
@@ -146,6 +148,7 @@ SIGNATURES = {
     "pdx_gnn_params": (_ST, [_P, _P, _P, ctypes.POINTER(_D)]),
     "pdx_gnn_graph": (_ST, [_P, ctypes.POINTER(_I64), _P, _P, _P, _P, _P, _P]),
     "pdx_gnn_apply_vec": (_ST, [_P, _P, _P]),
+    "pdx_gnn_field_values": (_ST, [_P, _P]),
     "pdx_gnn_loss_grad": (_ST, [_P, _I32, _P, _P, ctypes.POINTER(_D), _P]),
     "pdx_set_profiling": (_ST, [_P, _I32]),
     "pdx_kernel_time": (_ST, [_P, ctypes.c_char_p, ctypes.POINTER(_D), ctypes.POINTER(_I64)]),
@@ -428,14 +431,14 @@ class Context:
         self.call("pdx_gnn_params", None, ptr(th), ctypes.byref(c))
 
     def gnn_graph(self):
-        """(perm, near_ptr, near, stat, dinv, cmask), internal order."""
+        """(perm, near_ptr, near, features (n, 16), dinv, cmask), internal order."""
         nn = _I64()
         self.call("pdx_gnn_graph", ctypes.byref(nn), None, None, None, None, None, None)
         n = self.n
         perm = np.zeros(n, np.int64)
         near_ptr = np.zeros(n + 1, np.int64)
         near = np.zeros(max(nn.value, 1), np.int32)
-        stat = np.zeros((n, 5), np.float32)
+        stat = np.zeros((n, 16), np.float32)
         dinv = np.zeros((n, 9), np.float64)
         cmask = np.zeros((n, 3), np.uint8)
         self.call("pdx_gnn_graph", ctypes.byref(nn), ptr(perm), ptr(near_ptr), ptr(near),
@@ -448,12 +451,24 @@ class Context:
         self.call("pdx_gnn_apply_vec", ptr(y), ptr(z))
         return z
 
-    def gnn_loss_grad(self, Y, X):
-        Y = np.ascontiguousarray(Y, np.float64).reshape(-1, self.n, 3)
-        X = np.ascontiguousarray(X, np.float64).reshape(Y.shape)
+    def coarse_info(self):
+        """(points per aggregate, aggregates, coarse fields of the last coarse space)."""
+        a, na, f = _I32(), _I64(), _I32()
+        self.call("pdx_nk_coarse_info", ctypes.byref(a), ctypes.byref(na), ctypes.byref(f))
+        return a.value, na.value, f.value
+
+    def gnn_fields(self):
+        """The network's 8 coarse fields per point, (n, 8, 3) float32, internal order."""
+        f = np.zeros((self.n, 8, 3), np.float32)
+        self.call("pdx_gnn_field_values", ptr(f))
+        return f
+
+    def gnn_loss_grad(self, E):
+        """Loss and gradient over the error samples E (s, n, 3), internal order."""
+        E = np.ascontiguousarray(E, np.float64).reshape(-1, self.n, 3)
         loss = _D()
         g = np.zeros(self.gnn_n_params(), np.float32)
-        self.call("pdx_gnn_loss_grad", Y.shape[0], ptr(Y), ptr(X), ctypes.byref(loss), ptr(g))
+        self.call("pdx_gnn_loss_grad", E.shape[0], ptr(E), None, ctypes.byref(loss), ptr(g))
         return loss.value, g
 
     def set_profiling(self, on=True):

@@ -48,23 +48,23 @@ class NewtonKrylov:
     Additions of the MI355X build (defaults keep the reference behaviour):
     rel_tol > 0 stops a load step at ||r|| < rel_tol * ||r_first|| (the reference's `tol`
     is absolute, newton_krylov.jl:311-314); precond = "none" | "jacobi" | "two_level" | "learned"
-    ("two_level": block Jacobi plus a Galerkin coarse correction on Morton-block aggregates;
-    "learned": the two-level preconditioner plus the trained message-passing network).
-    With "learned", load steps 1 .. learn_steps are solved with collect_precond (the
-    two-level preconditioner by default) while their Newton and Krylov pairs are collected
-    (at most krylov_pairs Krylov pairs),
-    the network is then trained (train_steps Adam steps of train_batch pairs, step size
-    train_lr) and the remaining load steps use it.  The last tenth of the Krylov pairs is
-    held out; the network is applied only if its held-out loss is below `gate` (library
-    default 0.85; 1 means it explains none of the two-level preconditioner's error),
-    otherwise the remaining steps run with the two-level preconditioner alone; so they do
-    after the first learned solve that needs the fallback."""
+    ("two_level": block Jacobi plus a Galerkin coarse correction by the translations of
+    Morton-block aggregates; "learned": the same with a coarse space that also spans the
+    trained message-passing network's fields).  With "learned", load steps 1 .. learn_steps
+    are solved with the two-level preconditioner while the errors it leaves on their Krylov
+    pairs are collected (at most krylov_pairs samples); the network is then trained
+    (train_steps Adam steps on random subsets of train_batch samples, step size train_lr)
+    and the remaining load steps use its coarse space.  The last tenth of the samples is held
+    out; the fields are used only if their held-out loss is below `gate` times that of the
+    translations alone (library default 0.95), otherwise the remaining steps run with the
+    two-level preconditioner alone; so they do after the first learned solve that needs the
+    fallback."""
 
     def __init__(self, time=-1, steps=-1, stepsize=1.0, maxiter=100, tol=1e-4,
                  perturbation_scale=1.0, gmres_maxiter=200, gmres_reltol=1e-4,
                  gmres_abstol=1e-8, gmres_restart=100, rel_tol=0.0, precond="none",
-                 learn_steps=2, train_steps=150, train_batch=4, train_lr=2e-3,
-                 krylov_pairs=2048, train_seed=0, collect_precond="two_level", gate=None):
+                 learn_steps=3, train_steps=150, train_batch=128, train_lr=3e-3,
+                 krylov_pairs=512, train_seed=0, collect_precond="two_level", gate=None):
         if time > 0 and steps > 0:
             raise _arg_error("specify either time or number of steps, not both!")
         if time < 0 and steps < 0:
@@ -103,8 +103,8 @@ class NewtonKrylov:
         self.train_lr = float(train_lr)
         self.krylov_pairs = int(krylov_pairs)
         self.train_seed = int(train_seed)
-        if collect_precond not in ("jacobi", "two_level"):
-            raise _arg_error("collect_precond must be 'jacobi' or 'two_level'")
+        if collect_precond != "two_level":
+            raise _arg_error("collect_precond must be 'two_level' (the samples are its errors)")
         self.collect_precond = collect_precond
         self.gate = None if gate is None else float(gate)
 

@@ -89,15 +89,19 @@ def _declare(L):
     L.orc_precrack_hits.argtypes = [_I64, _P, _P, _P, _P, _P]
     L.orc_bb_force.restype = None
     L.orc_bb_force.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _D, _D, _P, _P, _INT, _P]
-    L.orc_osb_force.restype = None
+    L.orc_reverse_index.restype = _INT
+    L.orc_reverse_index.argtypes = [_I64, _P, _P, _P, _P, _P]
+    L.orc_osb_force.restype = _INT
     L.orc_osb_force.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _D, _D,
-                                _D, _P, _P, _P, _P, _P]
-    L.orc_cpd_force.restype = None
+                                _D, _P, _P, _P, _P, _P, _P, _P, _P, _INT]
+    L.orc_cpd_force.restype = _INT
     L.orc_cpd_force.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _INT, _D, _D,
-                                _D, _D, _D, _D, _D, _D, _D, _D, _P, _P, _P, _P]
-    L.orc_cpd_force_sets.restype = None
+                                _D, _D, _D, _D, _D, _D, _D, _D, _P, _P, _P, _P, _P, _P, _P,
+                                _INT]
+    L.orc_cpd_force_sets.restype = _INT
     L.orc_cpd_force_sets.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _INT,
-                                     _P, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _P, _P, _P]
+                                     _P, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _P, _P, _P,
+                                     _P, _P, _P, _INT]
     L.orc_zem_c1.restype = None
     L.orc_zem_c1.argtypes = [_P, _D, _D, _D, _P]
     L.orc_pk1.restype = None
@@ -112,8 +116,8 @@ def _declare(L):
     L.orc_update_acc_and_vel.argtypes = [_I64, _P, _P, _D, _P, _D, _P, _P]
     L.orc_bb_force_sets.restype = None
     L.orc_bb_force_sets.argtypes = [_I64] + [_P] * 14
-    L.orc_osb_force_sets.restype = None
-    L.orc_osb_force_sets.argtypes = [_I64] + [_P] * 18
+    L.orc_osb_force_sets.restype = _INT
+    L.orc_osb_force_sets.argtypes = [_I64] + [_P] * 21 + [_INT]
     L.orc_min_timestep_sets.restype = _D
     L.orc_min_timestep_sets.argtypes = [_I64, _P, _P, _P, _P, _P, _P]
     L.orc_update_acc_and_vel_sets.restype = None
@@ -260,6 +264,16 @@ class OracleSim:
             self.omega = self.p["delta"] / L
         else:
             self.omega = None
+        # reverse bond index of the two-phase OSB/CPD loops (built once: the bond list is fixed)
+        self.rev = None
+        if kind in ("OSB", "CPD"):
+            in_ptr = np.zeros(n + 1, np.int64)
+            in_bond = np.zeros(max(nb, 1), np.int64)
+            in_src = np.zeros(max(nb, 1), np.int32)
+            if lib().orc_reverse_index(n, _p(row_ptr), _p(nbr), _p(in_ptr), _p(in_bond),
+                                       _p(in_src)) != 0:
+                raise MemoryError("orc_reverse_index")
+            self.rev = (in_ptr, in_bond, in_src)
         z = lambda: np.zeros((n, 3), np.float64)
         self.x = self.X.copy()
         self.u, self.v, self.vh, self.a, self.b_int, self.b_ext = z(), z(), z(), z(), z(), z()
@@ -293,6 +307,15 @@ class OracleSim:
         elif correction is not None:
             raise ValueError(correction)
 
+    def _rev_args(self):
+        in_ptr, in_bond, in_src = self.rev
+        return _p(in_ptr), _p(in_bond), _p(in_src), self.nthreads
+
+    @staticmethod
+    def _check(rc):
+        if rc != 0:
+            raise MemoryError("oracle force loop: out of host memory")
+
     # calc_force_density!(chunk, t, dt)
     def force(self, x=None):
         if x is not None:
@@ -309,7 +332,7 @@ class OracleSim:
                                     _p(self.scf))
             elif self.kind == "CPD":
                 model = MODELS[p.get("model", "SVK")] | (ZEMS[p.get("zem", "Silling")] << 8)
-                L.orc_cpd_force_sets(self.n, _p(self.X), _p(self.x), _p(self.vol),
+                self._check(L.orc_cpd_force_sets(self.n, _p(self.X), _p(self.x), _p(self.vol),
                                      _p(self.row_ptr), _p(self.nbr), _p(self.L), _p(self.bfp),
                                      _p(self.active), _p(self.omega), _p(self.n_active),
                                      _p(self.damage), model, _p(t["lambda"]), _p(t["mu"]),
@@ -317,14 +340,14 @@ class OracleSim:
                                      _p(t["epsc"]),
                                      p["delta"], p.get("Cs", 0.5), p.get("maxdmg", 0.85),
                                      _p(self.defgrad), _p(self.cauchy), _p(self.b_int),
-                                     _p(self.b_abs))
+                                     _p(self.b_abs), *self._rev_args()))
             else:
-                L.orc_osb_force_sets(self.n, _p(self.X), _p(self.x), _p(self.vol),
+                self._check(L.orc_osb_force_sets(self.n, _p(self.X), _p(self.x), _p(self.vol),
                                      _p(self.row_ptr), _p(self.nbr), _p(self.L), _p(self.bfp),
                                      _p(self.active), _p(self.bond_length), _p(self.omega),
                                      _p(self.n_active), _p(self.damage), _p(t["G"]),
                                      _p(t["K"]), _p(t["epsc"]), _p(self.b_int), _p(self.b_abs),
-                                     _p(self.scf))
+                                     _p(self.scf), *self._rev_args()))
             return self.b_int
         if self.kind == "BB":
             L.orc_bb_force(self.n, _p(self.x), _p(self.vol), _p(self.row_ptr), _p(self.nbr),
@@ -332,21 +355,22 @@ class OracleSim:
                            _p(self.n_active), _p(self.damage), p["bc"], p["epsc"],
                            _p(self.b_int), _p(self.b_abs), self.nthreads, _p(self.scf))
         elif self.kind == "OSB":
-            L.orc_osb_force(self.n, _p(self.X), _p(self.x), _p(self.vol), _p(self.row_ptr),
+            self._check(L.orc_osb_force(self.n, _p(self.X), _p(self.x), _p(self.vol), _p(self.row_ptr),
                             _p(self.nbr), _p(self.L), _p(self.bfp), _p(self.active),
                             _p(self.bond_length), _p(self.omega), _p(self.n_active),
                             _p(self.damage), p["G"], p["K"], p["epsc"], _p(self.b_int),
-                            _p(self.wvol), _p(self.dil), _p(self.b_abs), _p(self.scf))
+                            _p(self.wvol), _p(self.dil), _p(self.b_abs), _p(self.scf),
+                            *self._rev_args()))
         elif self.kind == "CPD":
             model = MODELS[p.get("model", "SVK")] | (ZEMS[p.get("zem", "Silling")] << 8)
-            L.orc_cpd_force(self.n, _p(self.X), _p(self.x), _p(self.vol), _p(self.row_ptr),
+            self._check(L.orc_cpd_force(self.n, _p(self.X), _p(self.x), _p(self.vol), _p(self.row_ptr),
                             _p(self.nbr), _p(self.L), _p(self.bfp), _p(self.active),
                             _p(self.omega), _p(self.n_active), _p(self.damage), model,
                             p["lambda"], p["mu"], p["G"], p["K"], p.get("nu", 0.25), p["bc"],
                             p["delta"],
                             p.get("Cs", 0.5), p.get("maxdmg", 0.85), p["epsc"],
                             _p(self.defgrad), _p(self.cauchy), _p(self.b_int),
-                            _p(self.b_abs))
+                            _p(self.b_abs), *self._rev_args()))
         else:
             raise ValueError(self.kind)
         return self.b_int
@@ -378,6 +402,11 @@ class OracleSim:
                     val = float(f(Xref[i], t))
                     if np.isfinite(val):
                         field[i, dim] = val
+            elif kind == "pdv":
+                # the same, for an f that takes all the points' positions at once (k x 3)
+                val = np.asarray(f(Xref[pts], t), np.float64)
+                ok = np.isfinite(val)
+                field[pts[ok], dim] = val[ok]
 
     def vv_run(self, n_steps, dt, bcs=(), first_step=1, check_nans=True, fd_bcs=()):
         """solve!(dh, VelocityVerlet): verlet_timestep! for n = first_step .. (velocity_verlet.jl:181-197).
@@ -385,7 +414,8 @@ class OracleSim:
         bcs: list of ("sd", points, dim, f(t)) and ("pd", points, dim, f(X, t)) acting on
         velocity_half (velocity_bc!, boundary_conditions.jl:422-435); fd_bcs the same on
         b_ext (forcedensity_bc!, :559-575), both applied by apply_boundary_conditions! after
-        the half step."""
+        the half step; ("pdv", points, dim, f(Xs, t)) is "pd" with f vectorised over the
+        points' reference positions Xs (k x 3)."""
         L = lib()
         ndof = 3 * self.n
         dth = 0.5 * dt

@@ -362,23 +362,159 @@ void orc_bb_force(int64_t n, const double *x, const double *vol, const int64_t *
  * two agree bit for bit because b_int starts at zero: 0 + t1 + t2 ... */
 
 /* ------------------------------------------------------------------------------ */
+/* Scatter loops run in two phases.  The reference's OSB and CPD loops add each bond's */
+/* force to both ends (b_int[:, i] += t V_j, b_int[:, j] -= t V_i), so the serial    */
+/* order of the additions into one point's b_int is: the bonds of points i < k that  */
+/* list k, then k's own bonds, then those of points i > k.  Phase 1 (points in       */
+/* parallel) stores every bond's vector t; phase 2 (points in parallel) gathers      */
+/* b_int[k] in exactly that order over the reverse bond index, so the result equals  */
+/* the serial loop bit for bit for any thread count.                                */
+/* ------------------------------------------------------------------------------ */
+
+/* Reverse bond index: the bonds grouped by their neighbour k = nbr[b], each group in
+ * ascending owner order (a stable counting sort by nbr).  in_ptr: n + 1, in_bond and
+ * in_src (the owner): one per bond. */
+int orc_reverse_index(int64_t n, const int64_t *row_ptr, const int32_t *nbr, int64_t *in_ptr,
+                      int64_t *in_bond, int32_t *in_src)
+{
+    const int64_t nb = row_ptr[n];
+    int64_t *pos = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n + 1));
+    if (!pos) return ORC_ERR_OOM;
+    for (int64_t k = 0; k <= n; k++) in_ptr[k] = 0;
+    for (int64_t b = 0; b < nb; b++) in_ptr[nbr[b] + 1]++;
+    for (int64_t k = 0; k < n; k++) in_ptr[k + 1] += in_ptr[k];
+    memcpy(pos, in_ptr, sizeof(int64_t) * (size_t)(n + 1));
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
+            const int64_t q = pos[nbr[b]]++;
+            in_bond[q] = b;
+            in_src[q] = (int32_t)i;
+        }
+    free(pos);
+    return ORC_OK;
+}
+
+/* per-bond vectors of phase 1 (grow-only; the oracle is called from one host thread) */
+static double *g_tb = NULL, *g_tabs = NULL;
+static size_t g_tb_cap = 0, g_tabs_cap = 0;
+static uint8_t *g_skip = NULL;
+static size_t g_skip_cap = 0;
+
+static int grow(void **p, size_t *cap, size_t bytes)
+{
+    if (bytes <= *cap) return ORC_OK;
+    free(*p);
+    *p = malloc(bytes);
+    *cap = *p ? bytes : 0;
+    return *p ? ORC_OK : ORC_ERR_OOM;
+}
+
+typedef struct {
+    const int64_t *in_ptr, *in_bond;
+    const int32_t *in_src;
+    int64_t *own_ptr, *own_bond; /* built here when the caller passes no index */
+    int32_t *own_src;
+} rev_t;
+
+static int rev_get(rev_t *r, int64_t n, const int64_t *row_ptr, const int32_t *nbr,
+                   const int64_t *in_ptr, const int64_t *in_bond, const int32_t *in_src)
+{
+    memset(r, 0, sizeof *r);
+    if (in_ptr && in_bond && in_src) {
+        r->in_ptr = in_ptr; r->in_bond = in_bond; r->in_src = in_src;
+        return ORC_OK;
+    }
+    const int64_t nb = row_ptr[n];
+    r->own_ptr = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n + 1));
+    r->own_bond = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nb > 0 ? nb : 1));
+    r->own_src = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nb > 0 ? nb : 1));
+    if (!r->own_ptr || !r->own_bond || !r->own_src) return ORC_ERR_OOM;
+    if (orc_reverse_index(n, row_ptr, nbr, r->own_ptr, r->own_bond, r->own_src) != ORC_OK)
+        return ORC_ERR_OOM;
+    r->in_ptr = r->own_ptr; r->in_bond = r->own_bond; r->in_src = r->own_src;
+    return ORC_OK;
+}
+
+static void rev_free(rev_t *r)
+{
+    free(r->own_ptr); free(r->own_bond); free(r->own_src);
+}
+
+/* Phase 2.  abs_mode 0: b_abs[i] += |t V_j|, b_abs[j] += |t V_i| (OSB); abs_mode 1:
+ * b_abs[i] += tabs V_j, b_abs[j] += tabs V_i (CPD's rounding scale per bond). */
+static void gather(int64_t n, const int64_t *row_ptr, const int32_t *nbr, const rev_t *r,
+                   const uint8_t *skip, const double *tb, const double *tabs, int abs_mode,
+                   const double *vol, double *b_int, double *b_abs)
+{
+#pragma omp parallel for schedule(dynamic, 512)
+    for (int64_t k = 0; k < n; k++) {
+        double acc[3] = {0.0, 0.0, 0.0}, aab[3] = {0.0, 0.0, 0.0};
+        int64_t q = r->in_ptr[k];
+        const int64_t e = r->in_ptr[k + 1];
+        for (int pass = 0; pass < 2; pass++) {
+            /* pass 0: owners i < k; pass 1: owners i > k */
+            for (; q < e && (pass == 1 || r->in_src[q] < k); q++) {
+                const int64_t i = r->in_src[q], b = r->in_bond[q];
+                if (skip[i]) continue;
+                for (int d = 0; d < 3; d++) {
+                    acc[d] += -tb[3 * b + d] * vol[i];
+                    if (b_abs) aab[d] += abs_mode ? tabs[3 * b + d] * vol[i] : fabs(tb[3 * b + d] * vol[i]);
+                }
+            }
+            if (pass == 0 && !skip[k])
+                for (int64_t b = row_ptr[k]; b < row_ptr[k + 1]; b++) {
+                    const int64_t j = nbr[b];
+                    for (int d = 0; d < 3; d++) {
+                        acc[d] += tb[3 * b + d] * vol[j];
+                        if (b_abs) aab[d] += abs_mode ? tabs[3 * b + d] * vol[j] : fabs(tb[3 * b + d] * vol[j]);
+                    }
+                }
+        }
+        for (int d = 0; d < 3; d++) {
+            b_int[3 * k + d] = acc[d];
+            if (b_abs) b_abs[3 * k + d] = aab[d];
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------------ */
 /* Ordinary state-based force density (scatter form, as the reference)            */
 /* calc_failure! (OSB)         ordinary_state_based.jl:107-126                     */
-/* force_density_point!        ordinary_state_based.jl:128-149                     */
+/* force_density_point!        ordinary_state_based.jl:128-149 (one set),          */
+/*                             :151-174 (several sets: per bond                    */
+/*                             c1 = 15 (G_i + G_j) / (2 wvol_i),                   */
+/*                             c2 = dil_i (3 (K_i + K_j) / (2 wvol_i) - c1 / 3))   */
 /* calc_weighted_volume        ordinary_state_based.jl:176-189                     */
 /* calc_dilatation             ordinary_state_based.jl:191-204                     */
 /* omega[b] = kernel(delta_i, L) (find_kernels, bond_system.jl:184-196)            */
+/* Points with wvol = 0 add nothing.  multi = 0: G_p, K_p, epsc_p hold one value;   */
+/* multi = 1: one value per point.                                                 */
 /* ------------------------------------------------------------------------------ */
-void orc_osb_force(int64_t n, const double *X, const double *x, const double *vol,
-                   const int64_t *row_ptr, const int32_t *nbr, const double *L,
-                   const uint8_t *bond_fp, uint8_t *active, double *bond_length,
-                   const double *omega, int64_t *n_active, double *damage, double G,
-                   double K, double epsc, double *b_int, double *wvol_out, double *dil_out,
-                   double *b_abs, const double *scf)
+static int osb_core(int64_t n, const double *X, const double *x, const double *vol,
+                    const int64_t *row_ptr, const int32_t *nbr, const double *L,
+                    const uint8_t *bond_fp, uint8_t *active, double *bond_length,
+                    const double *omega, int64_t *n_active, double *damage, const double *G_p,
+                    const double *K_p, const double *epsc_p, int multi, double *b_int,
+                    double *wvol_out, double *dil_out, double *b_abs, const double *scf,
+                    const int64_t *in_ptr, const int64_t *in_bond, const int32_t *in_src,
+                    int nthreads)
 {
-    for (int64_t k = 0; k < 3 * n; k++) b_int[k] = 0.0;
-    if (b_abs) for (int64_t k = 0; k < 3 * n; k++) b_abs[k] = 0.0;
+    set_threads(nthreads);
+    const int64_t nb = row_ptr[n];
+    const int64_t ps = multi ? 1 : 0;
+    if (grow((void **)&g_tb, &g_tb_cap, sizeof(double) * 3 * (size_t)(nb > 0 ? nb : 1)) ||
+        grow((void **)&g_skip, &g_skip_cap, (size_t)(n > 0 ? n : 1)))
+        return ORC_ERR_OOM;
+    rev_t r;
+    if (rev_get(&r, n, row_ptr, nbr, in_ptr, in_bond, in_src) != ORC_OK) {
+        rev_free(&r);
+        return ORC_ERR_OOM;
+    }
+    double *tb = g_tb;
+    uint8_t *skip = g_skip;
+#pragma omp parallel for schedule(dynamic, 256)
     for (int64_t i = 0; i < n; i++) {
+        const double epsc = epsc_p[ps * i];
         int64_t na = 0;
         for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
             int64_t j = nbr[b];
@@ -406,7 +542,8 @@ void orc_osb_force(int64_t n, const double *X, const double *x, const double *vo
         }
         if (wvol_out) wvol_out[i] = wvol;
         if (dil_out) dil_out[i] = 0.0;
-        if (wvol == 0.0) continue;
+        skip[i] = wvol == 0.0;
+        if (skip[i]) continue;
         /* calc_dilatation */
         double dil = 0.0;
         double c1d = 3.0 / wvol;
@@ -417,31 +554,43 @@ void orc_osb_force(int64_t n, const double *X, const double *x, const double *vo
             dil += ((((w * (scf ? scf[b] : 1.0)) * c1d) * L[b]) * (l - L[b])) * vol[j];
         }
         if (dil_out) dil_out[i] = dil;
-        double c1 = (15.0 * G) / wvol;
-        double c2 = dil * ((3.0 * K) / wvol - c1 / 3.0);
         for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
             int64_t j = nbr[b];
             double dx = x[3 * j] - x[3 * i];
             double dy = x[3 * j + 1] - x[3 * i + 1];
             double dz = x[3 * j + 2] - x[3 * i + 2];
             double l = bond_length[b];
+            double c1, c2;
+            if (multi) {
+                c1 = 15.0 * (G_p[i] + G_p[j]) / (2.0 * wvol);
+                c2 = dil * (3.0 * (K_p[i] + K_p[j]) / (2.0 * wvol) - c1 / 3.0);
+            } else {
+                c1 = (15.0 * G_p[0]) / wvol;
+                c2 = dil * ((3.0 * K_p[0]) / wvol - c1 / 3.0);
+            }
             double w = active[b] ? omega[b] : 0.0;
             double s = ((w * (scf ? scf[b] : 1.0)) * (c2 * L[b] + c1 * (l - L[b]))) / l;
-            double px = s * dx, py = s * dy, pz = s * dz;
-            b_int[3 * i] += px * vol[j];
-            b_int[3 * i + 1] += py * vol[j];
-            b_int[3 * i + 2] += pz * vol[j];
-            b_int[3 * j] += -px * vol[i];
-            b_int[3 * j + 1] += -py * vol[i];
-            b_int[3 * j + 2] += -pz * vol[i];
-            if (b_abs) {
-                b_abs[3 * i] += fabs(px * vol[j]); b_abs[3 * i + 1] += fabs(py * vol[j]);
-                b_abs[3 * i + 2] += fabs(pz * vol[j]);
-                b_abs[3 * j] += fabs(px * vol[i]); b_abs[3 * j + 1] += fabs(py * vol[i]);
-                b_abs[3 * j + 2] += fabs(pz * vol[i]);
-            }
+            tb[3 * b] = s * dx;
+            tb[3 * b + 1] = s * dy;
+            tb[3 * b + 2] = s * dz;
         }
     }
+    gather(n, row_ptr, nbr, &r, skip, tb, NULL, 0, vol, b_int, b_abs);
+    rev_free(&r);
+    return ORC_OK;
+}
+
+int orc_osb_force(int64_t n, const double *X, const double *x, const double *vol,
+                  const int64_t *row_ptr, const int32_t *nbr, const double *L,
+                  const uint8_t *bond_fp, uint8_t *active, double *bond_length,
+                  const double *omega, int64_t *n_active, double *damage, double G,
+                  double K, double epsc, double *b_int, double *wvol_out, double *dil_out,
+                  double *b_abs, const double *scf, const int64_t *in_ptr,
+                  const int64_t *in_bond, const int32_t *in_src, int nthreads)
+{
+    return osb_core(n, X, x, vol, row_ptr, nbr, L, bond_fp, active, bond_length, omega,
+                    n_active, damage, &G, &K, &epsc, 0, b_int, wvol_out, dil_out, b_abs, scf,
+                    in_ptr, in_bond, in_src, nthreads);
 }
 
 /* ------------------------------------------------------------------------------ */
@@ -456,6 +605,7 @@ void orc_bb_force_sets(int64_t n, const double *x, const double *vol, const int6
                        const double *bc_pt, const double *epsc_pt, double *b_int, double *b_abs,
                        const double *scf)
 {
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) {
         int64_t na = 0;
         for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
@@ -490,78 +640,18 @@ void orc_bb_force_sets(int64_t n, const double *x, const double *vol, const int6
     }
 }
 
-/* OSB with several sets (ordinary_state_based.jl:151-174): each point's weighted volume and
- * dilatation from its own bonds; per bond c1 = 15 (G_i + G_j) / (2 wvol_i),
- * c2 = dil_i (3 (K_i + K_j) / (2 wvol_i) - c1 / 3); points with wvol = 0 are skipped. */
-void orc_osb_force_sets(int64_t n, const double *X, const double *x, const double *vol,
-                        const int64_t *row_ptr, const int32_t *nbr, const double *L,
-                        const uint8_t *bond_fp, uint8_t *active, double *bond_length,
-                        const double *omega, int64_t *n_active, double *damage,
-                        const double *G_pt, const double *K_pt, const double *epsc_pt,
-                        double *b_int, double *b_abs, const double *scf)
+int orc_osb_force_sets(int64_t n, const double *X, const double *x, const double *vol,
+                       const int64_t *row_ptr, const int32_t *nbr, const double *L,
+                       const uint8_t *bond_fp, uint8_t *active, double *bond_length,
+                       const double *omega, int64_t *n_active, double *damage,
+                       const double *G_pt, const double *K_pt, const double *epsc_pt,
+                       double *b_int, double *b_abs, const double *scf,
+                       const int64_t *in_ptr, const int64_t *in_bond, const int32_t *in_src,
+                       int nthreads)
 {
-    for (int64_t k = 0; k < 3 * n; k++) b_int[k] = 0.0;
-    if (b_abs) for (int64_t k = 0; k < 3 * n; k++) b_abs[k] = 0.0;
-    for (int64_t i = 0; i < n; i++) {
-        int64_t na = 0;
-        for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
-            int64_t j = nbr[b];
-            double dx = x[3 * j] - x[3 * i];
-            double dy = x[3 * j + 1] - x[3 * i + 1];
-            double dz = x[3 * j + 2] - x[3 * i + 2];
-            double l = sqrt((dx * dx + dy * dy) + dz * dz);
-            bond_length[b] = l;
-            double eps = (l - L[b]) / L[b];
-            if (eps > epsc_pt[i] && bond_fp[b]) active[b] = 0;
-            na += active[b];
-        }
-        n_active[i] = na;
-        damage[i] = 1.0 - (double)na / (double)(row_ptr[i + 1] - row_ptr[i]);
-    }
-    for (int64_t i = 0; i < n; i++) {
-        double wvol = 0.0;
-        for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
-            int64_t j = nbr[b];
-            double Dx = X[3 * j] - X[3 * i];
-            double Dy = X[3 * j + 1] - X[3 * i + 1];
-            double Dz = X[3 * j + 2] - X[3 * i + 2];
-            double dsq = (Dx * Dx + Dy * Dy) + Dz * Dz;
-            double w = active[b] ? omega[b] : 0.0;
-            wvol += ((w * (scf ? scf[b] : 1.0)) * dsq) * vol[j];
-        }
-        if (wvol == 0.0) continue;
-        double dil = 0.0;
-        double c1d = 3.0 / wvol;
-        for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
-            int64_t j = nbr[b];
-            double w = active[b] ? omega[b] : 0.0;
-            dil += ((((w * (scf ? scf[b] : 1.0)) * c1d) * L[b]) * (bond_length[b] - L[b])) * vol[j];
-        }
-        for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
-            int64_t j = nbr[b];
-            double dx = x[3 * j] - x[3 * i];
-            double dy = x[3 * j + 1] - x[3 * i + 1];
-            double dz = x[3 * j + 2] - x[3 * i + 2];
-            double l = bond_length[b];
-            double c1 = 15.0 * (G_pt[i] + G_pt[j]) / (2.0 * wvol);
-            double c2 = dil * (3.0 * (K_pt[i] + K_pt[j]) / (2.0 * wvol) - c1 / 3.0);
-            double w = active[b] ? omega[b] : 0.0;
-            double s = ((w * (scf ? scf[b] : 1.0)) * (c2 * L[b] + c1 * (l - L[b]))) / l;
-            double px = s * dx, py = s * dy, pz = s * dz;
-            b_int[3 * i] += px * vol[j];
-            b_int[3 * i + 1] += py * vol[j];
-            b_int[3 * i + 2] += pz * vol[j];
-            b_int[3 * j] += -px * vol[i];
-            b_int[3 * j + 1] += -py * vol[i];
-            b_int[3 * j + 2] += -pz * vol[i];
-            if (b_abs) {
-                b_abs[3 * i] += fabs(px * vol[j]); b_abs[3 * i + 1] += fabs(py * vol[j]);
-                b_abs[3 * i + 2] += fabs(pz * vol[j]);
-                b_abs[3 * j] += fabs(px * vol[i]); b_abs[3 * j + 1] += fabs(py * vol[i]);
-                b_abs[3 * j + 2] += fabs(pz * vol[i]);
-            }
-        }
-    }
+    return osb_core(n, X, x, vol, row_ptr, nbr, L, bond_fp, active, bond_length, omega,
+                    n_active, damage, G_pt, K_pt, epsc_pt, 1, b_int, NULL, NULL, b_abs, scf,
+                    in_ptr, in_bond, in_src, nthreads);
 }
 
 /* EnergySurfaceCorrection factors (bond_system_corrections.jl:98-204).  Per-point parameter
@@ -909,7 +999,7 @@ void orc_zem_c1(const double *Kinv, double lambda, double mu, double nu, double 
 /* Parameters per point (`ps` = 1: arrays of n, one value per point's parameter set, the
  * params of the point whose loop runs, as force_density_point! receives them with a
  * ParameterHandler) or one value for all (`ps` = 0). */
-static void cpd_force_core(int64_t n, const double *X, const double *x, const double *vol,
+static int cpd_force_core(int64_t n, const double *X, const double *x, const double *vol,
                            const int64_t *row_ptr, const int32_t *nbr, const double *L,
                            const uint8_t *bond_fp, uint8_t *active, const double *omega,
                            int64_t *n_active, double *damage, int model, const double *lambda_p,
@@ -917,12 +1007,25 @@ static void cpd_force_core(int64_t n, const double *X, const double *x, const do
                            const double *nu_p, const double *bc_p, const double *epsc_p,
                            int64_t ps, double delta,
                            double Cs, double maxdmg, double *defgrad, double *cauchy_out,
-                           double *b_int, double *b_abs)
+                           double *b_int, double *b_abs, const int64_t *in_ptr,
+                           const int64_t *in_bond, const int32_t *in_src, int nthreads)
 {
     const int wan = (model >> 8) == 1;
     model &= 0xff;
-    for (int64_t k = 0; k < 3 * n; k++) b_int[k] = 0.0;
-    if (b_abs) for (int64_t k = 0; k < 3 * n; k++) b_abs[k] = 0.0;
+    set_threads(nthreads);
+    const int64_t nb = row_ptr[n];
+    const size_t bvec = sizeof(double) * 3 * (size_t)(nb > 0 ? nb : 1);
+    if (grow((void **)&g_tb, &g_tb_cap, bvec) || grow((void **)&g_tabs, &g_tabs_cap, bvec) ||
+        grow((void **)&g_skip, &g_skip_cap, (size_t)(n > 0 ? n : 1)))
+        return ORC_ERR_OOM;
+    rev_t rv;
+    if (rev_get(&rv, n, row_ptr, nbr, in_ptr, in_bond, in_src) != ORC_OK) {
+        rev_free(&rv);
+        return ORC_ERR_OOM;
+    }
+    double *tb = g_tb, *tba = g_tabs;
+    uint8_t *skip = g_skip;
+#pragma omp parallel for schedule(dynamic, 256)
     for (int64_t i = 0; i < n; i++) {
         const double lambda = lambda_p[ps * i], mu = mu_p[ps * i], G = G_p[ps * i], K = K_p[ps * i];
         const double nu = nu_p[ps * i];
@@ -963,7 +1066,8 @@ static void cpd_force_core(int64_t n, const double *X, const double *x, const do
         m3_mul(Ft, Kinv, F);
         if (defgrad) memcpy(defgrad + 9 * i, F, sizeof F);
         /* too_much_damage! */
-        if (damage[i] > maxdmg || m3_hasnan(F)) {
+        skip[i] = damage[i] > maxdmg || m3_hasnan(F);
+        if (skip[i]) {
             for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) active[b] = 0;
             n_active[i] = 0;
             continue;
@@ -1011,44 +1115,48 @@ static void cpd_force_core(int64_t n, const double *X, const double *x, const do
                 tabs[r] = (fabs(m0 * DX[0]) + fabs(m1 * DX[1]) + fabs(m2 * DX[2])) + zabs;
             }
             for (int r = 0; r < 3; r++) {
-                b_int[3 * i + r] += t[r] * vol[j];
-                b_int[3 * j + r] += -t[r] * vol[i];
-                if (b_abs) {
-                    b_abs[3 * i + r] += tabs[r] * vol[j];
-                    b_abs[3 * j + r] += tabs[r] * vol[i];
-                }
+                tb[3 * b + r] = t[r];
+                tba[3 * b + r] = tabs[r];
             }
         }
     }
+    gather(n, row_ptr, nbr, &rv, skip, tb, tba, 1, vol, b_int, b_abs);
+    rev_free(&rv);
+    return ORC_OK;
 }
 
-void orc_cpd_force(int64_t n, const double *X, const double *x, const double *vol,
-                   const int64_t *row_ptr, const int32_t *nbr, const double *L,
-                   const uint8_t *bond_fp, uint8_t *active, const double *omega,
-                   int64_t *n_active, double *damage, int model, double lambda, double mu,
-                   double G, double K, double nu, double bc, double delta, double Cs,
-                   double maxdmg, double epsc, double *defgrad, double *cauchy_out,
-                   double *b_int, double *b_abs)
+int orc_cpd_force(int64_t n, const double *X, const double *x, const double *vol,
+                  const int64_t *row_ptr, const int32_t *nbr, const double *L,
+                  const uint8_t *bond_fp, uint8_t *active, const double *omega,
+                  int64_t *n_active, double *damage, int model, double lambda, double mu,
+                  double G, double K, double nu, double bc, double delta, double Cs,
+                  double maxdmg, double epsc, double *defgrad, double *cauchy_out,
+                  double *b_int, double *b_abs, const int64_t *in_ptr, const int64_t *in_bond,
+                  const int32_t *in_src, int nthreads)
 {
-    cpd_force_core(n, X, x, vol, row_ptr, nbr, L, bond_fp, active, omega, n_active, damage, model,
-                   &lambda, &mu, &G, &K, &nu, &bc, &epsc, 0, delta, Cs, maxdmg, defgrad,
-                   cauchy_out, b_int, b_abs);
+    return cpd_force_core(n, X, x, vol, row_ptr, nbr, L, bond_fp, active, omega, n_active,
+                          damage, model, &lambda, &mu, &G, &K, &nu, &bc, &epsc, 0, delta, Cs,
+                          maxdmg, defgrad, cauchy_out, b_int, b_abs, in_ptr, in_bond, in_src,
+                          nthreads);
 }
 
 /* several parameter sets: lambda, mu, G, K, bc, eps_c per point (the sets share delta, the
  * material's model, Cs and maxdmg) */
-void orc_cpd_force_sets(int64_t n, const double *X, const double *x, const double *vol,
-                        const int64_t *row_ptr, const int32_t *nbr, const double *L,
-                        const uint8_t *bond_fp, uint8_t *active, const double *omega,
-                        int64_t *n_active, double *damage, int model, const double *lambda_pt,
-                        const double *mu_pt, const double *G_pt, const double *K_pt,
-                        const double *nu_pt, const double *bc_pt, const double *epsc_pt,
-                        double delta, double Cs, double maxdmg, double *defgrad,
-                        double *cauchy_out, double *b_int, double *b_abs)
+int orc_cpd_force_sets(int64_t n, const double *X, const double *x, const double *vol,
+                       const int64_t *row_ptr, const int32_t *nbr, const double *L,
+                       const uint8_t *bond_fp, uint8_t *active, const double *omega,
+                       int64_t *n_active, double *damage, int model, const double *lambda_pt,
+                       const double *mu_pt, const double *G_pt, const double *K_pt,
+                       const double *nu_pt, const double *bc_pt, const double *epsc_pt,
+                       double delta, double Cs, double maxdmg, double *defgrad,
+                       double *cauchy_out, double *b_int, double *b_abs,
+                       const int64_t *in_ptr, const int64_t *in_bond, const int32_t *in_src,
+                       int nthreads)
 {
-    cpd_force_core(n, X, x, vol, row_ptr, nbr, L, bond_fp, active, omega, n_active, damage, model,
-                   lambda_pt, mu_pt, G_pt, K_pt, nu_pt, bc_pt, epsc_pt, 1, delta, Cs, maxdmg,
-                   defgrad, cauchy_out, b_int, b_abs);
+    return cpd_force_core(n, X, x, vol, row_ptr, nbr, L, bond_fp, active, omega, n_active,
+                          damage, model, lambda_pt, mu_pt, G_pt, K_pt, nu_pt, bc_pt, epsc_pt, 1,
+                          delta, Cs, maxdmg, defgrad, cauchy_out, b_int, b_abs, in_ptr, in_bond,
+                          in_src, nthreads);
 }
 
 /* ------------------------------------------------------------------------------ */

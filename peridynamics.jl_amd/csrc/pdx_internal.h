@@ -213,6 +213,63 @@ pdx_status pdx_timed_end(pdx_ctx *c, const char *name);
 pdx_status pdx_timed_flush(pdx_ctx *c);
 
 // device helpers --------------------------------------------------------------------------
+// finite-difference perturbation of J v (compute_perturbation, newton_krylov.jl:534-566):
+// eps = scale cbrt(eps_mach) (||u|| + 1) / ||v||, with sc = scale cbrt(eps_mach) and the
+// squared norms on the device (a vanishing ||v|| counts as 1)
+__device__ __forceinline__ double pdx_nk_perturbation_v(double un2, double vn2, double sc)
+{
+    double vn = sqrt(vn2);
+    if (vn < 2.220446049250313e-16) vn = 1.0;
+    return sc * (sqrt(un2) + 1.0) / vn;
+}
+__device__ __forceinline__ double pdx_nk_perturbation(const double *un2, const double *vn2, double sc)
+{
+    return pdx_nk_perturbation_v(*un2, *vn2, sc);
+}
+
+// 32 sums per lane reduced over the wave by a halving butterfly: each step trades half of
+// the values with the partner lane (32 shuffles in all instead of 6 per value); afterwards
+// lanes 2q and 2q + 1 hold sum q in s[0].  Indices are compile-time constants, so the array
+// stays in registers.
+template <int H>
+__device__ __forceinline__ void pdx_bfly32(double (&s)[32], int lane)
+{
+    const bool up = (lane & (2 * H)) != 0;
+#pragma unroll
+    for (int q = 0; q < H; q++) {
+        const double keep = up ? s[q + H] : s[q];
+        const double give = up ? s[q] : s[q + H];
+        s[q] = keep + __shfl_xor(give, 2 * H);
+    }
+}
+
+__device__ __forceinline__ void pdx_wave_reduce32(double (&s)[32])
+{
+    const int lane = threadIdx.x & 63;
+    pdx_bfly32<16>(s, lane);
+    pdx_bfly32<8>(s, lane);
+    pdx_bfly32<4>(s, lane);
+    pdx_bfly32<2>(s, lane);
+    pdx_bfly32<1>(s, lane);
+    s[0] += __shfl_xor(s[0], 1);
+}
+
+// sum of parts[0, np) by a whole workgroup: every thread returns it, and the order of the
+// additions depends only on np and the workgroup size (sh: one double per wave)
+__device__ __forceinline__ double pdx_block_sum_parts(const double *parts, int np, double *sh)
+{
+    double s = 0.0;
+    for (int t = threadIdx.x; t < np; t += blockDim.x) s += parts[t];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    const int nw = (int)((blockDim.x + 63) >> 6);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    double tot = 0.0;
+    for (int q = 0; q < nw; q++) tot += sh[q];
+    __syncthreads();
+    return tot;
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  Give each
 // XCD one contiguous range of points so that its L2 holds only its own neighbourhoods.
 // gridDim.x is a multiple of 8, so the map is a bijection of [0, gridDim.x).
@@ -351,6 +408,14 @@ struct pdx_step_args {
 pdx_status pdx_fill_step_args(pdx_ctx *c, pdx_step_args *a, int64_t step, int flags, double dt,
                               const double *bcv_dev);
 pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a);  // one step of the material
+// J v = (F(u + e v) V - F(u - e v) V) / (2 e) of the bond-based material in one kernel, the
+// constrained rows (cmask) = v; true when the context's layout allows it (FP64 LDS step
+// with one parameter set, no surface correction, a neighbour table that fits the LDS)
+// ||v||^2 = the sum of the np partial sums `parts` (pdx_block_sum_parts at 1024 threads)
+bool pdx_bb_jv_fusable(const pdx_ctx *c);
+pdx_status pdx_launch_bb_jv(pdx_ctx *c, const double *u, const double *v, const double *un2,
+                            const double *parts, int np, double sc, const uint8_t *cmask,
+                            double *jv);
 pdx_status pdx_launch_prologue(pdx_ctx *c, int64_t step, double dt, const double *bcv_dev);
 pdx_status pdx_apply_fd_bcs(pdx_ctx *c, const double *bcv_row);  // b_ext conditions of one step
 pdx_status pdx_build_correction(pdx_ctx *c);  // EnergySurfaceCorrection factors

@@ -21,6 +21,13 @@ struct pdx_nk {
     double *h_dev = nullptr;    // small coefficient vectors
     double *dinv = nullptr;     // 9n point-block Jacobi inverse
     double *dsc = nullptr;      // device scalars (||u||^2, ||v||^2)
+    double *gm = nullptr;       // GMRES state on the device: Hessenberg, rotations, g, y, flags
+    int *flag_host = nullptr;   // pinned (coherent): the convergence flag of the last two columns
+    int *flag_dev = nullptr;    // the same memory as the kernels address it
+    bool jv_fused = false;      // J v by the one-kernel central difference (k_bb_jv)
+    double *vparts = nullptr;   // partial sums of |v|^2 for a J v whose caller has none
+    double *zparts = nullptr;   // partial sums of |z|^2 left by the preconditioner's kernels
+    hipEvent_t ev_col[2] = {nullptr, nullptr};
     // two-level preconditioner: aggregates of `agg` consecutive points, nf basis fields per
     // aggregate, mc = nf nagg coarse dofs
     int agg = 0;
@@ -50,8 +57,9 @@ inline int pdx_nk_agg_size(int64_t n)
     return (int)agg;
 }
 
-// Jv on device vectors (internal order)
-pdx_status pdx_nk_jv_dev(pdx_ctx *c, const double *v, double *jv, double scale);
+// Jv on device vectors (internal order); `parts` (np partial sums of |v|^2) when known
+pdx_status pdx_nk_jv_dev(pdx_ctx *c, const double *v, double *jv, double scale,
+                         const double *parts = nullptr, int np = 0);
 
 // learned preconditioner (pdx_gnn.hip)
 pdx_status pdx_gnn_prepare(pdx_ctx *c);                                  // trained check
@@ -69,7 +77,8 @@ pdx_status pdx_nk_build_coarse(pdx_ctx *c);  // two-level coarse operator at the
 // the same with the network's fields appended to the translations (n_fields per point)
 pdx_status pdx_nk_build_coarse_fields(pdx_ctx *c, const float *fields, int n_fields);
 pdx_status pdx_nk_coarse_add(pdx_ctx *c, const double *v, double *z, double alpha);  // z += a P Ac^-1 P^T v
-pdx_status pdx_nk_two_level(pdx_ctx *c, const double *v, double *z);  // z = D^-1 v + P Ac^-1 P^T v
+// z = D^-1 v + P Ac^-1 P^T v; parts (optional): the ceil(n / 256) partial sums of |z|^2
+pdx_status pdx_nk_two_level(pdx_ctx *c, const double *v, double *z, double *parts = nullptr);
 
 // preconditioners of pdx_nk_opts.precond
 #define PDX_PRE_NONE 0

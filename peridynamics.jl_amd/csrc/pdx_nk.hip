@@ -20,11 +20,13 @@
 #include <vector>
 
 #include <rocsolver/rocsolver.h>
+#include <cstdlib>
 
 #include "pdx_internal.h"
 #include "pdx_nk.h"
 
 #define RED_BLOCKS 512
+#define CGS_MAX_BLOCKS 2048  // partial sums of the Gram-Schmidt kernels: at most this many blocks
 #define PDX_MAX_RESTART 128
 // device scalars of the Krylov solver
 #define NK_UN2 0  // ||u||^2
@@ -39,9 +41,14 @@ void pdx_nk_free(pdx_ctx *c)
                     (void *)k->recp, (void *)k->recm, (void *)k->fp, (void *)k->fm,
                     (void *)k->red, (void *)k->dinv, (void *)k->h_dev, (void *)k->dsc,
                     (void *)k->Ac, (void *)k->rc, (void *)k->zc, (void *)k->Ainv, (void *)k->phi,
-                    (void *)k->Ycs, (void *)k->pJ, (void *)k->aslot, (void *)k->dflag})
+                    (void *)k->Ycs, (void *)k->pJ, (void *)k->aslot, (void *)k->dflag,
+                    (void *)k->vparts, (void *)k->zparts})
         if (p) (void)hipFree(p);
     if (k->rb) rocblas_destroy_handle(k->rb);
+    if (k->gm) (void)hipFree(k->gm);
+    if (k->flag_host) (void)hipHostFree(k->flag_host);
+    for (int e = 0; e < 2; e++)
+        if (k->ev_col[e]) (void)hipEventDestroy(k->ev_col[e]);
     delete k;
     c->nk = nullptr;
 }
@@ -163,13 +170,6 @@ __global__ void k_residual(int64_t n, const double *bint, const double *bext,
 // perturbation of jacobian_vector_product! (compute_perturbation, newton_krylov.jl:537-567):
 // s = scale cbrt(eps) (||u|| + 1) / ||v||, ||v|| < eps -> 1; the squared norms are device
 // scalars so that no host round trip sits between the Krylov kernels
-__device__ __forceinline__ double nk_perturbation(const double *un2, const double *vn2, double sc)
-{
-    double vn = sqrt(*vn2);
-    if (vn < 2.220446049250313e-16) vn = 1.0;
-    return sc * (sqrt(*un2) + 1.0) / vn;
-}
-
 // positions X + (u + s v) and X + (u - s v) with the volume in .w
 // (store_state_and_apply_perturbation!, store_pos_force_and_apply_neg_perturbation!)
 __global__ void k_perturb(int64_t n, const double4 *Xr, const double *u, const double *v,
@@ -177,7 +177,7 @@ __global__ void k_perturb(int64_t n, const double4 *Xr, const double *u, const d
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double s = nk_perturbation(un2, vn2, sc);
+    const double s = pdx_nk_perturbation(un2, vn2, sc);
     const double4 X = Xr[i];
     double up[3], um[3];
     for (int d = 0; d < 3; d++) {
@@ -195,18 +195,10 @@ __global__ void k_jv_combine(int64_t m, const double *fp, const double *fm, cons
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= m) return;
-    const double s = nk_perturbation(un2, vn2, sc);
+    const double s = pdx_nk_perturbation(un2, vn2, sc);
     jv[t] = cmask[t] ? v[t] : (fp[t] - fm[t]) / (2.0 * s);
 }
 
-// x *= 1 / sqrt(*n2) (no-op for a zero norm)
-__global__ void k_scale_inv_sqrt(int64_t m, const double *n2, const double *x, double *y)
-{
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= m) return;
-    const double nn = *n2;
-    y[t] = nn > 0.0 ? x[t] / sqrt(nn) : x[t];
-}
 
 // update_displacement_from_du!: free dofs u += du, x += du
 __global__ void k_nk_update(int64_t n, const double *du, const uint8_t *cmask, double *u,
@@ -306,6 +298,171 @@ __global__ void k_apply_block(int64_t n, const double *dinv, const double *v, do
 // ----------------------------------------------------------------------------------------
 static inline dim3 grid1(int64_t m, int bs = 256) { return dim3((unsigned)((m + bs - 1) / bs)); }
 
+// Gram-Schmidt passes over a 3n vector w and the first vectors of V (leading dimension ldv),
+// partial-sum kernels (one row of partial sums per block) that k_reduce_partials or the
+// Arnoldi step reduce:
+//   k_cgs_dots:  partial = V_k . w                        (k < nv)
+//   k_cgs_axdn:  w -= V h;  partial = V_k . w, then w . w  (k < nv; the V values of a dof
+//                stay in registers between the update and the dots when nv <= 32)
+// (the next basis vector (w - V h) / hn is formed by k_next_basis)
+// The 32 running sums of a lane are reduced over the wave by pdx_wave_reduce32 (a halving
+// butterfly); all indices into the register arrays are compile-time constants, so the
+// arrays stay in registers.
+#define CGS_REG 32
+// one dof per thread (grid-stride beyond CGS_MAX_BLOCKS blocks): the loads of all vectors
+// of a dof are in flight at once
+static inline int cgs_blocks(int64_t m) { return (int)std::min<int64_t>((m + 255) / 256, CGS_MAX_BLOCKS); }
+
+// the block's sums 0..kn-1 (after pdx_wave_reduce32) into column q of the partial sums
+// (partial[q * nblk + block]: the reductions read a sum's partials contiguously), in a fixed
+// order
+__device__ __forceinline__ void cgs_block_store(const double (&s)[CGS_REG], int kn,
+                                                double (*sh)[CGS_REG], double *partial, int q0)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if ((lane & 1) == 0) sh[wv][lane >> 1] = s[0];
+    __syncthreads();
+    if ((int)threadIdx.x < kn) {
+        const int q = threadIdx.x;
+        partial[(int64_t)(q0 + q) * gridDim.x + blockIdx.x] = (sh[0][q] + sh[1][q]) + (sh[2][q] + sh[3][q]);
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ double cgs_wave_sum(double t)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+    return t;
+}
+
+// the block's partial of one more sum (x of every thread) into column q
+__device__ __forceinline__ void cgs_block_store1(double x, double *partial, int q)
+{
+    __shared__ double sh1[4];
+    x = cgs_wave_sum(x);
+    if ((threadIdx.x & 63) == 0) sh1[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[(int64_t)q * gridDim.x + blockIdx.x] = (sh1[0] + sh1[1]) + (sh1[2] + sh1[3]);
+}
+
+// partial (column q, one per block) = V_q . w for q < nv
+__global__ __launch_bounds__(256) void k_cgs_dots(int64_t m, int nv, const double *V, int64_t ldv,
+                                                  const double *w, double *partial)
+{
+    __shared__ double sh[4][CGS_REG];
+    const int64_t i0 = (int64_t)blockIdx.x * 256, stride = (int64_t)gridDim.x * 256;
+    if (m <= stride && nv <= CGS_REG) {
+        // one dof per thread: the products go straight into the register array
+        const int64_t i = i0 + threadIdx.x;
+        double s[CGS_REG];
+        const double wi = i < m ? w[i] : 0.0;
+#pragma unroll
+        for (int q = 0; q < CGS_REG; q++) s[q] = (q < nv && i < m) ? V[(int64_t)q * ldv + i] * wi : 0.0;
+        pdx_wave_reduce32(s);
+        cgs_block_store(s, nv, sh, partial, 0);
+        return;
+    }
+    for (int k0 = 0; k0 < nv; k0 += CGS_REG) {
+        const int kn = min(CGS_REG, nv - k0);
+        const double *Vk = V + (int64_t)k0 * ldv;
+        double s[CGS_REG];
+#pragma unroll
+        for (int q = 0; q < CGS_REG; q++) s[q] = 0.0;
+        for (int64_t i = i0 + threadIdx.x; i < m; i += stride) {
+            const double wi = w[i];
+#pragma unroll
+            for (int q = 0; q < CGS_REG; q++)
+                if (q < kn) s[q] += Vk[(int64_t)q * ldv + i] * wi;
+        }
+        pdx_wave_reduce32(s);
+        cgs_block_store(s, kn, sh, partial, k0);
+    }
+}
+
+// w -= V h; partial columns q < nv: V_q . w (the updated w), column nv: w . w.  One dof
+// per thread (m <= 256 x blocks) and nv <= 32: the products overwrite the loaded V values,
+// so only one 32-wide register array is live; h is read from LDS.
+__global__ __launch_bounds__(256) void k_cgs_axdn(int64_t m, int nv, const double *h, const double *V,
+                                                  int64_t ldv, double *w, double *partial)
+{
+    __shared__ double sh[4][CGS_REG];
+    __shared__ double hs[CGS_REG];
+    if (threadIdx.x < CGS_REG) hs[threadIdx.x] = (int)threadIdx.x < nv ? h[threadIdx.x] : 0.0;
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    double v[CGS_REG];
+    double x = 0.0;
+#pragma unroll
+    for (int q = 0; q < CGS_REG; q++) v[q] = 0.0;
+    if (i < m) {
+#pragma unroll
+        for (int q = 0; q < CGS_REG; q++)
+            if (q < nv) v[q] = V[(int64_t)q * ldv + i];
+        x = w[i];
+#pragma unroll
+        for (int q = 0; q < CGS_REG; q++)
+            if (q < nv) x -= hs[q] * v[q];
+        w[i] = x;
+    }
+#pragma unroll
+    for (int q = 0; q < CGS_REG; q++) v[q] *= x;
+    pdx_wave_reduce32(v);
+    cgs_block_store(v, nv, sh, partial, 0);
+    cgs_block_store1(x * x, partial, nv);
+}
+
+// the same for several dofs per thread or a long basis: the update first, then the dots in
+// chunks (V read again)
+__global__ __launch_bounds__(256) void k_cgs_axdn_loop(int64_t m, int nv, const double *h, const double *V,
+                                                       int64_t ldv, double *w, double *partial)
+{
+    __shared__ double sh[4][CGS_REG];
+    const int64_t i0 = (int64_t)blockIdx.x * 256, stride = (int64_t)gridDim.x * 256;
+    double nrm = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < m; i += stride) {
+        double x = w[i];
+        for (int k = 0; k < nv; k++) x -= h[k] * V[(int64_t)k * ldv + i];
+        w[i] = x;
+        nrm += x * x;
+    }
+    for (int k0 = 0; k0 < nv; k0 += CGS_REG) {
+        const int kn = min(CGS_REG, nv - k0);
+        const double *Vk = V + (int64_t)k0 * ldv;
+        double s[CGS_REG];
+#pragma unroll
+        for (int q = 0; q < CGS_REG; q++) s[q] = 0.0;
+        for (int64_t i = i0 + threadIdx.x; i < m; i += stride) {
+            const double wi = w[i];
+#pragma unroll
+            for (int q = 0; q < CGS_REG; q++)
+                if (q < kn) s[q] += Vk[(int64_t)q * ldv + i] * wi;
+        }
+        pdx_wave_reduce32(s);
+        cgs_block_store(s, kn, sh, partial, k0);
+    }
+    cgs_block_store1(nrm, partial, nv);
+}
+
+// out[q] = sum of the nblk partials of column q (one workgroup per q)
+__global__ __launch_bounds__(256) void k_reduce_cols(int nblk, const double *partial, double *out)
+{
+    __shared__ double sh[4];
+    const double *col = partial + (int64_t)blockIdx.x * nblk;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += 256) s += col[b];
+    s = cgs_wave_sum(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+
+
+// doubles of the small GMRES state for restart length R (H, cs, sn, g, y, h1, h2, hn and
+// two ints)
+static size_t nk_gm_doubles(int R) { return (size_t)(R + 1) * R + 2 * R + (R + 1) + R + 2 * (R + 1) + 2 + 2; }
+
 static pdx_status nk_ensure(pdx_ctx *c, int restart)
 {
     if (!c->nk) {
@@ -324,11 +481,22 @@ static pdx_status nk_ensure(pdx_ctx *c, int restart)
             PDX_TRY(pdx_alloc(c, p, m));
         PDX_TRY(pdx_alloc(c, &k->recp, n));
         PDX_TRY(pdx_alloc(c, &k->recm, n));
-        PDX_TRY(pdx_alloc(c, &k->red, (size_t)RED_BLOCKS * (PDX_MAX_RESTART + 2)));
+        PDX_TRY(pdx_alloc(c, &k->red, (size_t)std::max(RED_BLOCKS, CGS_MAX_BLOCKS) * (PDX_MAX_RESTART + 2)));
+        PDX_TRY(pdx_alloc(c, &k->vparts, CGS_MAX_BLOCKS));
+        PDX_TRY(pdx_alloc(c, &k->zparts, (size_t)((n + 63) / 64)));
         PDX_TRY(pdx_alloc(c, &k->h_dev, 2 * (PDX_MAX_RESTART + 1) + 8));
         PDX_TRY(pdx_alloc(c, &k->dsc, 8));
         PDX_HIP(c, hipMemset(k->dsc, 0, sizeof(double) * 8));
         PDX_TRY(pdx_alloc(c, &k->dinv, 9 * n));
+        PDX_TRY(pdx_alloc(c, &k->gm, nk_gm_doubles(PDX_MAX_RESTART)));
+        // J v in one kernel where the layout allows; PDX_JV_TWO_KERNELS=1 keeps the two force
+        // evaluations of the step kernel (the same numbers, for comparison)
+        const char *two = std::getenv("PDX_JV_TWO_KERNELS");
+        k->jv_fused = pdx_bb_jv_fusable(c) && !(two && two[0] == '1');
+        PDX_HIP(c, hipHostMalloc((void **)&k->flag_host, sizeof(int) * 2, hipHostMallocCoherent));
+        PDX_HIP(c, hipHostGetDevicePointer((void **)&k->flag_dev, k->flag_host, 0));
+        for (int e = 0; e < 2; e++)
+            PDX_HIP(c, hipEventCreateWithFlags(&k->ev_col[e], hipEventDisableTiming));
     }
     if (restart > k->m_cap) {
         if (k->Vb) (void)hipFree(k->Vb);
@@ -387,16 +555,37 @@ static pdx_status force_scaled(pdx_ctx *c, const double4 *rec_in, double *out)
 
 // Jv for the current displacement (jacobian_vector_product!).  ||u||^2 is the device
 // scalar NK_UN2, kept current by nk_refresh_unorm(); ||v||^2 is formed here on the device.
-static pdx_status nk_refresh_unorm(pdx_ctx *c)
+static pdx_status nk_norm2_dev(pdx_ctx *c, const double *x, double *out_dev);
+static pdx_status nk_refresh_unorm(pdx_ctx *c) { return nk_norm2_dev(c, c->u, c->nk->dsc + NK_UN2); }
+
+// sum of np partial sums into out[0] (the order of pdx_block_sum_parts at 1024 threads, as
+// the one-kernel J v forms ||v||^2)
+__global__ __launch_bounds__(1024) void k_sum_parts(const double *parts, int np, double *out)
 {
-    return multidot_dev(c, 1, c->u, c->u, c->nk->dsc + NK_UN2);
+    __shared__ double sh[16];
+    const double t = pdx_block_sum_parts(parts, np, sh);
+    if (threadIdx.x == 0) out[0] = t;
 }
 
-pdx_status pdx_nk_jv_dev(pdx_ctx *c, const double *v, double *jv, double scale)
+// J v; ||v||^2 is the sum of the np partial sums `parts` when given (the preconditioner's
+// kernels leave them for z), else formed here
+pdx_status pdx_nk_jv_dev(pdx_ctx *c, const double *v, double *jv, double scale, const double *parts,
+                         int np)
 {
     pdx_nk *k = c->nk;
     const double sc = scale * std::cbrt(2.220446049250313e-16);
-    PDX_TRY(multidot_dev(c, 1, v, v, k->dsc + NK_VN2));
+    if (!parts) {
+        const int nb = cgs_blocks(k->m);
+        hipLaunchKernelGGL(k_cgs_dots, dim3(nb), dim3(256), 0, c->stream, k->m, 1, v, k->m, v, k->vparts);
+        parts = k->vparts;
+        np = nb;
+    }
+    if (k->jv_fused) {
+        PDX_TRY(pdx_launch_bb_jv(c, c->u, v, k->dsc + NK_UN2, parts, np, sc, k->cmask, jv));
+        k->n_jv++;
+        return PDX_OK;
+    }
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(1024), 0, c->stream, parts, np, k->dsc + NK_VN2);
     hipLaunchKernelGGL(k_perturb, grid1(c->n), dim3(256), 0, c->stream, c->n, c->Xr, c->u, v,
                        k->dsc + NK_UN2, k->dsc + NK_VN2, sc, k->recp, k->recm);
     PDX_HIP(c, hipGetLastError());
@@ -717,33 +906,49 @@ __global__ __launch_bounds__(64) void k_coarse_matvec(int64_t mc, const float *A
 
 // z = a D^-1 v + b P zc  (a = 0: the prolongation alone, added to z when b is given and
 // `add` is set)
-__global__ void k_apply_two_level(int64_t n, int agg, int nf, const double *dinv, const double *v,
-                                  const double *zc, const double *phi, double alpha, int add,
-                                  double *z)
+// partial sum of x over a 256-thread block into parts[blockIdx.x] (fixed order)
+__device__ __forceinline__ void nk_block_part(double x, double *parts)
+{
+    __shared__ double sh[4];
+    x = cgs_wave_sum(x);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) parts[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+__global__ __launch_bounds__(256) void k_apply_two_level(int64_t n, int agg, int nf, const double *dinv,
+                                                         const double *v, const double *zc,
+                                                         const double *phi, double alpha, int add,
+                                                         double *z, double *parts)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double o[3] = {0.0, 0.0, 0.0};
-    if (add) {
-        o[0] = z[3 * i];
-        o[1] = z[3 * i + 1];
-        o[2] = z[3 * i + 2];
-    } else {
-        const double *R = dinv + 9 * i;
-        const double a = v[3 * i], b = v[3 * i + 1], cc = v[3 * i + 2];
-        o[0] = R[0] * a + R[3] * b + R[6] * cc;
-        o[1] = R[1] * a + R[4] * b + R[7] * cc;
-        o[2] = R[2] * a + R[5] * b + R[8] * cc;
+    double zz = 0.0;
+    if (i < n) {
+        double o[3] = {0.0, 0.0, 0.0};
+        if (add) {
+            o[0] = z[3 * i];
+            o[1] = z[3 * i + 1];
+            o[2] = z[3 * i + 2];
+        } else {
+            const double *R = dinv + 9 * i;
+            const double a = v[3 * i], b = v[3 * i + 1], cc = v[3 * i + 2];
+            o[0] = R[0] * a + R[3] * b + R[6] * cc;
+            o[1] = R[1] * a + R[4] * b + R[7] * cc;
+            o[2] = R[2] * a + R[5] * b + R[8] * cc;
+        }
+        const int64_t I = i / agg;
+        for (int f = 0; f < nf; f++) {
+            const double w = alpha * zc[I * nf + f];
+            const double *p = phi + (i * nf + f) * 3;
+            o[0] += w * p[0];
+            o[1] += w * p[1];
+            o[2] += w * p[2];
+        }
+        for (int d = 0; d < 3; d++) z[3 * i + d] = o[d];
+        zz = (o[0] * o[0] + o[1] * o[1]) + o[2] * o[2];
     }
-    const int64_t I = i / agg;
-    for (int f = 0; f < nf; f++) {
-        const double w = alpha * zc[I * nf + f];
-        const double *p = phi + (i * nf + f) * 3;
-        o[0] += w * p[0];
-        o[1] += w * p[1];
-        o[2] += w * p[2];
-    }
-    for (int d = 0; d < 3; d++) z[3 * i + d] = o[d];
+    // every thread of the block reaches the partial sum (it holds a barrier)
+    if (parts) nk_block_part(zz, parts);
 }
 
 // aggregate size: pdx_nk_agg_size(n), doubled until no point's bonds reach more than
@@ -876,7 +1081,7 @@ static pdx_status build_coarse(pdx_ctx *c, const float *learned_fields, int n_le
 }
 
 // z = D^-1 v + P A_c^-1 P^T v
-pdx_status pdx_nk_two_level(pdx_ctx *c, const double *v, double *z)
+pdx_status pdx_nk_two_level(pdx_ctx *c, const double *v, double *z, double *parts)
 {
     pdx_nk *k = c->nk;
     if (!k->coarse_ready) return pdx_fail(c, PDX_ERR_STATE, "coarse space not built");
@@ -884,7 +1089,7 @@ pdx_status pdx_nk_two_level(pdx_ctx *c, const double *v, double *z)
                        k->nf, k->phi, v, k->rc);
     hipLaunchKernelGGL(k_coarse_matvec, dim3((unsigned)k->mc), dim3(64), 0, c->stream, k->mc, k->Ainv, k->rc, k->zc);
     hipLaunchKernelGGL(k_apply_two_level, grid1(c->n), dim3(256), 0, c->stream, c->n, k->agg, k->nf, k->dinv, v,
-                       k->zc, k->phi, 1.0, 0, z);
+                       k->zc, k->phi, 1.0, 0, z, parts);
     PDX_HIP(c, hipGetLastError());
     return PDX_OK;
 }
@@ -898,27 +1103,12 @@ pdx_status pdx_nk_coarse_add(pdx_ctx *c, const double *v, double *z, double alph
                        k->nf, k->phi, v, k->rc);
     hipLaunchKernelGGL(k_coarse_matvec, dim3((unsigned)k->mc), dim3(64), 0, c->stream, k->mc, k->Ainv, k->rc, k->zc);
     hipLaunchKernelGGL(k_apply_two_level, grid1(c->n), dim3(256), 0, c->stream, c->n, k->agg, k->nf, k->dinv, v,
-                       k->zc, k->phi, alpha, 1, z);
+                       k->zc, k->phi, alpha, 1, z, (double *)nullptr);
     PDX_HIP(c, hipGetLastError());
     return PDX_OK;
 }
 
 // z = M^-1 v for the chosen preconditioner
-static pdx_status precond_apply(pdx_ctx *c, int which, const double *v, double *z)
-{
-    pdx_nk *k = c->nk;
-    switch (which) {
-    case PDX_PRE_JACOBI:
-        hipLaunchKernelGGL(k_apply_block, grid1(c->n), dim3(256), 0, c->stream, c->n, k->dinv, v, z);
-        PDX_HIP(c, hipGetLastError());
-        return PDX_OK;
-    case PDX_PRE_TWO_LEVEL: return pdx_nk_two_level(c, v, z);
-    case PDX_PRE_LEARNED: return pdx_gnn_apply(c, v, z);  // two-level with the learned fields
-    default:
-        PDX_HIP(c, hipMemcpyAsync(z, v, sizeof(double) * k->m, hipMemcpyDeviceToDevice, c->stream));
-        return PDX_OK;
-    }
-}
 
 static pdx_status build_jacobi(pdx_ctx *c)
 {
@@ -948,99 +1138,343 @@ pdx_status pdx_nk_build_coarse_fields(pdx_ctx *c, const float *fields, int n_fie
     return build_coarse(c, fields, n_fields);
 }
 
+// ----------------------------------------------------------------------------------------
 // Flexible GMRES(restart) for J du = b, du in k->du (starts at zero).  Stops at
 // ||res|| <= max(reltol ||b||, abstol) or after maxiter products.  Classical Gram-Schmidt
-// with one reorthogonalisation pass; the coefficients and norms stay in device memory and
-// the host reads them once per iteration for the Givens rotations.
+// with one reorthogonalisation pass.  The Arnoldi coefficients, the Hessenberg matrix, its
+// Givens rotations and the residual estimate stay on the device (k_arnoldi); the host
+// keeps one iteration queued ahead and reads the convergence flag of the previous one, so
+// no iteration waits for a round trip.  An iteration queued after convergence leaves the
+// Hessenberg matrix alone and its direction unused.
+// ----------------------------------------------------------------------------------------
+// device layout of the small GMRES state (k->gm): H (R+1) x R column-major, cs, sn, g, y,
+// then the integer flags
+struct nk_gm_view {
+    double *H, *cs, *sn, *g, *y, *h1, *h2, *hn;
+    int *flags;  // [0] done, [1] columns of the cycle
+};
+static nk_gm_view nk_gm(pdx_nk *k, int R)
+{
+    nk_gm_view v;
+    v.H = k->gm;
+    v.cs = v.H + (size_t)(R + 1) * R;
+    v.sn = v.cs + R;
+    v.g = v.sn + R;
+    v.y = v.g + (R + 1);
+    v.h1 = v.y + R;
+    v.h2 = v.h1 + (R + 1);
+    v.hn = v.h2 + (R + 1);
+    v.flags = reinterpret_cast<int *>(v.hn + 2);
+    return v;
+}
+
+// The next basis vector and its preconditioned image; a block of 192 threads takes 64
+// points, one dof per thread:
+//   V = (w - V h) / hn  (hn_sq: / sqrt(hn), the cycle's first vector r / beta), in place
+//   when vout = w; with dinv (block Jacobi) also z = D^-1 V (the point's three values
+//   through LDS) and the block's partial sum of |z|^2 into parts (the J v of z takes ||z||
+//   from them)
+__global__ __launch_bounds__(192) void k_next_basis(int64_t n, int nv, const double *h, const double *hn,
+                                                    int hn_sq, const double *V, int64_t ldv,
+                                                    const double *w, double *vout, const double *dinv,
+                                                    double *z, double *parts)
+{
+    __shared__ double xs[192];
+    __shared__ double sh[3];
+    const int64_t m = 3 * n;
+    const int64_t t = (int64_t)blockIdx.x * 192 + threadIdx.x;
+    double x = 0.0;
+    if (t < m) {
+        const double raw = *hn;
+        const double dv = hn_sq ? sqrt(raw) : raw;
+        double a = w[t];
+#pragma unroll 8
+        for (int k = 0; k < nv; k++) a -= h[k] * V[(int64_t)k * ldv + t];
+        x = raw > 0.0 ? a / dv : a;
+        vout[t] = x;
+    }
+    if (!dinv) return;  // uniform over the grid
+    xs[threadIdx.x] = x;
+    __syncthreads();
+    double zz = 0.0;
+    if (t < m) {
+        const int pl = threadIdx.x / 3, d = threadIdx.x % 3;
+        const double *R = dinv + 9 * (t / 3);
+        const double x0 = xs[3 * pl], x1 = xs[3 * pl + 1], x2 = xs[3 * pl + 2];
+        const double zd = R[d] * x0 + R[3 + d] * x1 + R[6 + d] * x2;
+        z[t] = zd;
+        zz = zd * zd;
+    }
+    zz = cgs_wave_sum(zz);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = zz;
+    __syncthreads();
+    if (threadIdx.x == 0) parts[blockIdx.x] = (sh[0] + sh[1]) + sh[2];
+}
+
+// start of a cycle: g = (beta, 0, ...), flags cleared; beta^2 in *b2
+__global__ void k_gm_start(int R, const double *b2, double *g, int *flags)
+{
+    const int t = threadIdx.x;
+    for (int i = t; i <= R; i += blockDim.x) g[i] = (i == 0) ? sqrt(*b2) : 0.0;
+    if (t == 0) {
+        flags[0] = 0;
+        flags[1] = 0;
+    }
+}
+
+// Column j of the Hessenberg matrix.  h1: the first Gram-Schmidt pass (reduced); the
+// second pass arrives as j + 2 columns of np partial sums (V_i . w for i <= j, then n1 =
+// |w|^2 after the first pass), reduced here into h2 (one wave per sum); since V is
+// orthonormal, |w - V h2|^2 = n1 - |h2|^2.  The rotations run on the column staged in
+// LDS.  Stores the new norm in hn[0], the columns so far in flags[1]; sets flags[0] (done)
+// when the residual estimate |g[j + 1]| reaches thr, the norm vanishes or the iteration is
+// the last one allowed, and copies flags[0] to flag_host[j & 1] (pinned host memory, may
+// be null).  One workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void k_arnoldi(int j, int R, const double *h1, const double *parts,
+                                                  int np, double *h2, double *hn, double thr,
+                                                  int last_allowed, double *H, double *cs, double *sn,
+                                                  double *g, int *flags, int *flag_host)
+{
+    __shared__ double h2s[PDX_MAX_RESTART + 2];
+    __shared__ double col[PDX_MAX_RESTART + 2], c_[PDX_MAX_RESTART], s_[PDX_MAX_RESTART];
+    __shared__ double hn_s;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int nres = j + 2;
+    for (int q = wv; q < nres; q += 16) {
+        double a = 0.0;
+#pragma unroll 8
+        for (int b = lane; b < np; b += 64) a += parts[(int64_t)q * np + b];
+        a = cgs_wave_sum(a);
+        if (lane == 0) {
+            h2s[q] = a;
+            h2[q] = a;
+        }
+    }
+    __syncthreads();
+    if (wv == 0) {
+        double q2 = 0.0;
+        for (int i = lane; i <= j; i += 64) q2 += h2s[i] * h2s[i];
+        q2 = cgs_wave_sum(q2);
+        if (lane == 0) {
+            hn_s = sqrt(fmax(h2s[j + 1] - q2, 0.0));
+            hn[0] = hn_s;
+        }
+    }
+    const bool done_before = flags[0] != 0;
+    for (int i = t; i <= j; i += 1024) {
+        col[i] = h1[i] + h2s[i];
+        if (i < j) {
+            c_[i] = cs[i];
+            s_[i] = sn[i];
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        const double hnv = hn_s;
+        if (done_before) {
+            if (flag_host) __hip_atomic_store(flag_host + (j & 1), 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        } else {
+            col[j + 1] = hnv;
+            // the previous rotations, eight at a time: their inputs are read from LDS ahead
+            // of the dependent chain (the running value a is carried in a register)
+            double a = col[0];
+            for (int i0 = 0; i0 < j; i0 += 8) {
+                double cc[8], ss[8], bb[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++)
+                    if (i0 + u < j) {
+                        cc[u] = c_[i0 + u];
+                        ss[u] = s_[i0 + u];
+                        bb[u] = col[i0 + u + 1];
+                    }
+#pragma unroll
+                for (int u = 0; u < 8; u++)
+                    if (i0 + u < j) {
+                        col[i0 + u] = cc[u] * a + ss[u] * bb[u];
+                        a = -ss[u] * a + cc[u] * bb[u];
+                    }
+            }
+            col[j] = a;
+            const double den = hypot(col[j], col[j + 1]);
+            const double cj = den != 0 ? col[j] / den : 1.0;
+            const double sj = den != 0 ? col[j + 1] / den : 0.0;
+            cs[j] = cj;
+            sn[j] = sj;
+            col[j] = den;
+            col[j + 1] = 0.0;
+            const double gj = g[j];
+            g[j + 1] = -sj * gj;
+            g[j] = cj * gj;
+            flags[1] = j + 1;
+            const int done = (fabs(-sj * gj) <= thr || last_allowed || hnv == 0) ? 1 : 0;
+            flags[0] = done;
+            if (flag_host) __hip_atomic_store(flag_host + (j & 1), done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+    }
+    __syncthreads();
+    if (done_before) return;
+    double *Hc = H + (size_t)(R + 1) * j;
+    for (int i = t; i <= j + 1; i += 1024) Hc[i] = col[i];
+}
+
+// y = H^-1 g for the cycle's columns (upper triangular, column-oriented back substitution on
+// one wave: lane l keeps rows l and l + 64), zero beyond; est = |g[cols]|
+__global__ __launch_bounds__(64) void k_gm_solve(int R, const double *H, const double *g,
+                                                 const int *flags, double *y, double *est)
+{
+    const int t = threadIdx.x;
+    const int j = flags[1];
+    const size_t ld = (size_t)(R + 1);
+    double r0 = t < j ? g[t] : 0.0, r1 = t + 64 < j ? g[t + 64] : 0.0;
+    double y0 = 0.0, y1 = 0.0;
+    for (int i = j - 1; i >= 0; i--) {
+        const double *Hi = H + ld * i;
+        // lane (i & 63) owns row i
+        double yi = 0.0;
+        if (t == (i & 63)) yi = (i < 64 ? r0 : r1) / Hi[i];
+        yi = __shfl(yi, i & 63);
+        if (t == (i & 63)) {
+            if (i < 64) y0 = yi;
+            else y1 = yi;
+        }
+        if (t < i) r0 -= Hi[t] * yi;
+        if (t + 64 < i) r1 -= Hi[t + 64] * yi;
+    }
+    if (t < R) y[t] = y0;
+    if (t + 64 < R) y[t + 64] = y1;
+    if (t == 0) *est = fabs(g[j]);
+}
+
+// |x|^2 into out_dev
+static pdx_status nk_norm2_dev(pdx_ctx *c, const double *x, double *out_dev)
+{
+    pdx_nk *k = c->nk;
+    const int nb = cgs_blocks(k->m);
+    hipLaunchKernelGGL(k_cgs_dots, dim3(nb), dim3(256), 0, c->stream, k->m, 1, x, k->m, x, k->red);
+    hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, c->stream, nb, (const double *)k->red, out_dev);
+    PDX_HIP(c, hipGetLastError());
+    return PDX_OK;
+}
+
+// The next basis vector V = (w - V h) / hn (or r / sqrt(hn), hn_sq = 1) into vout and, when
+// zout is given, its preconditioned image z = M^-1 V with the partial sums of |z|^2 (*parts,
+// *np; null where the preconditioner leaves none).  Block Jacobi is one kernel; the
+// two-level (translations or learned fields) adds its coarse correction.
+static pdx_status nk_next(pdx_ctx *c, int precond, int nv, const double *h, const double *hn, int hn_sq,
+                          const double *w, double *vout, double *zout, const double **parts, int *np)
+{
+    pdx_nk *k = c->nk;
+    const int64_t n = c->n;
+    const bool jac = zout && precond == PDX_PRE_JACOBI;
+    const unsigned nbb = (unsigned)((n + 63) / 64);
+    hipLaunchKernelGGL(k_next_basis, dim3(nbb), dim3(192), 0, c->stream, n, nv, h, hn, hn_sq,
+                       (const double *)k->Vb, k->m, w, vout, jac ? (const double *)k->dinv : nullptr,
+                       jac ? zout : nullptr, jac ? k->zparts : nullptr);
+    PDX_HIP(c, hipGetLastError());
+    *parts = nullptr;
+    *np = 0;
+    if (!zout) return PDX_OK;
+    if (precond == PDX_PRE_TWO_LEVEL || precond == PDX_PRE_LEARNED)
+        PDX_TRY(pdx_nk_two_level(c, vout, zout, k->zparts));  // learned: the coarse space holds its fields
+    else if (!jac)
+        PDX_HIP(c, hipMemcpyAsync(zout, vout, sizeof(double) * k->m, hipMemcpyDeviceToDevice, c->stream));
+    if (jac) {
+        *parts = k->zparts;
+        *np = (int)nbb;
+    } else if (precond == PDX_PRE_TWO_LEVEL || precond == PDX_PRE_LEARNED) {
+        *parts = k->zparts;
+        *np = (int)grid1(n).x;
+    }
+    return PDX_OK;
+}
+
 static pdx_status fgmres(pdx_ctx *c, const double *b, int restart, int maxiter, double reltol,
                          double abstol, int precond, double scale, int *iters_out,
                          double *res_out)
 {
     pdx_nk *k = c->nk;
     const int64_t m = k->m;
-    const int R1 = PDX_MAX_RESTART + 1;
-    double *hd = k->h_dev, *h2d = k->h_dev + R1, *hnd = k->h_dev + 2 * R1;  // device
-    PDX_HIP(c, hipMemsetAsync(k->du, 0, sizeof(double) * m, c->stream));
+    const int R = restart;
+    nk_gm_view G = nk_gm(k, R);
+    hipStream_t s = c->stream;
+    const int CB = cgs_blocks(m);
+    PDX_HIP(c, hipMemsetAsync(k->du, 0, sizeof(double) * m, s));
     PDX_TRY(nk_refresh_unorm(c));
-    double bnorm;
-    PDX_TRY(norm2(c, b, &bnorm));
+    // |b|^2 -> the cycle's beta^2 (G.hn holds it until the first column)
+    PDX_TRY(nk_norm2_dev(c, b, G.hn));
+    double bn2 = 0.0;
+    PDX_HIP(c, hipMemcpyAsync(&bn2, G.hn, sizeof(double), hipMemcpyDeviceToHost, s));
+    PDX_HIP(c, hipStreamSynchronize(s));
+    const double bnorm = std::sqrt(bn2);
     const double thr = std::max(reltol * bnorm, abstol);
     int it = 0;
-    PDX_HIP(c, hipMemcpyAsync(k->r, b, sizeof(double) * m, hipMemcpyDeviceToDevice, c->stream));
+    PDX_HIP(c, hipMemcpyAsync(k->r, b, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
     double beta = bnorm;
-    std::vector<double> H((size_t)(restart + 1) * restart), cs(restart), sn(restart), g(restart + 1),
-        y(restart), hh(2 * R1 + 1);
     while (it < maxiter && beta > thr) {
-        std::fill(H.begin(), H.end(), 0.0);
-        std::fill(g.begin(), g.end(), 0.0);
-        g[0] = beta;
-        hipLaunchKernelGGL(k_scale, grid1(m), dim3(256), 0, c->stream, m, 1.0 / beta, k->r, k->Vb);
+        hipLaunchKernelGGL(k_gm_start, dim3(1), dim3(64), 0, s, R, G.hn, G.g, G.flags);
         PDX_HIP(c, hipGetLastError());
-        int j = 0;
-        bool done = false;
-        for (j = 0; j < restart; j++) {
-            double *Vj = k->Vb + (int64_t)j * m;
+        // V_0 = r / beta and Z_0 = M^-1 V_0
+        const double *zp = nullptr;
+        int znp = 0;
+        PDX_TRY(nk_next(c, precond, 0, nullptr, G.hn, 1, k->r, k->Vb, k->Zb, &zp, &znp));
+        int enq = 0;  // columns queued in this cycle
+        for (int j = 0; j < R && it + enq < maxiter; j++) {
             double *Zj = k->Zb + (int64_t)j * m;
             double *w = k->Vb + (int64_t)(j + 1) * m;
-            PDX_TRY(precond_apply(c, precond, Vj, Zj));
-            PDX_TRY(pdx_nk_jv_dev(c, Zj, w, scale));
-            it++;
+            PDX_TRY(pdx_nk_jv_dev(c, Zj, w, scale, zp, znp));
             if (pdx_gnn_collecting(c)) PDX_TRY(pdx_gnn_record_krylov(c, Zj, w));
-            // classical Gram-Schmidt, twice, then the norm -- all on the device
-            PDX_TRY(multidot_dev(c, j + 1, k->Vb, w, hd));
-            hipLaunchKernelGGL(k_multiaxpy, grid1(m), dim3(256), 0, c->stream, m, j + 1, k->Vb, m, hd, w);
-            PDX_TRY(multidot_dev(c, j + 1, k->Vb, w, h2d));
-            hipLaunchKernelGGL(k_multiaxpy, grid1(m), dim3(256), 0, c->stream, m, j + 1, k->Vb, m, h2d, w);
-            PDX_TRY(multidot_dev(c, 1, w, w, hnd));
-            hipLaunchKernelGGL(k_scale_inv_sqrt, grid1(m), dim3(256), 0, c->stream, m, hnd, w, w);
+            // classical Gram-Schmidt, twice: h1 = V^T w; w -= V h1, then V^T w and |w|^2 in
+            // partial sums that the Arnoldi step reduces into h2; then the next basis vector
+            // (w - V h2) / |w - V h2| and its preconditioned image
+            const int nv = j + 1;
+            hipLaunchKernelGGL(k_cgs_dots, dim3(CB), dim3(256), 0, s, m, nv, (const double *)k->Vb, m,
+                               (const double *)w, k->red);
+            hipLaunchKernelGGL(k_reduce_cols, dim3((unsigned)nv), dim3(256), 0, s, CB, (const double *)k->red, G.h1);
+            if (nv <= CGS_REG && m <= (int64_t)CB * 256)
+                hipLaunchKernelGGL(k_cgs_axdn, dim3(CB), dim3(256), 0, s, m, nv, (const double *)G.h1,
+                                   (const double *)k->Vb, m, w, k->red);
+            else
+                hipLaunchKernelGGL(k_cgs_axdn_loop, dim3(CB), dim3(256), 0, s, m, nv, (const double *)G.h1,
+                                   (const double *)k->Vb, m, w, k->red);
+            const int last_allowed = (it + enq + 1 >= maxiter) ? 1 : 0;
+            hipLaunchKernelGGL(k_arnoldi, dim3(1), dim3(1024), 0, s, j, R, (const double *)G.h1,
+                               (const double *)k->red, CB, G.h2, G.hn, thr, last_allowed, G.H, G.cs, G.sn,
+                               G.g, G.flags, k->flag_dev);
             PDX_HIP(c, hipGetLastError());
-            PDX_HIP(c, hipMemcpyAsync(hh.data(), k->h_dev, sizeof(double) * (2 * R1 + 1),
-                                      hipMemcpyDeviceToHost, c->stream));
-            PDX_HIP(c, hipStreamSynchronize(c->stream));
-            for (int i = 0; i <= j; i++) H[i + (size_t)(restart + 1) * j] = hh[i] + hh[R1 + i];
-            const double hn = std::sqrt(std::max(hh[2 * R1], 0.0));
-            H[j + 1 + (size_t)(restart + 1) * j] = hn;
-            // Givens
-            double *Hc = &H[(size_t)(restart + 1) * j];
-            for (int i = 0; i < j; i++) {
-                const double t = cs[i] * Hc[i] + sn[i] * Hc[i + 1];
-                Hc[i + 1] = -sn[i] * Hc[i] + cs[i] * Hc[i + 1];
-                Hc[i] = t;
-            }
-            const double den = std::hypot(Hc[j], Hc[j + 1]);
-            cs[j] = den != 0 ? Hc[j] / den : 1.0;
-            sn[j] = den != 0 ? Hc[j + 1] / den : 0.0;
-            Hc[j] = den;
-            Hc[j + 1] = 0.0;
-            g[j + 1] = -sn[j] * g[j];
-            g[j] = cs[j] * g[j];
-            if (std::fabs(g[j + 1]) <= thr || it >= maxiter || hn == 0) {
-                j++;
-                done = true;
-                break;
+            // Z has R columns: the last column's V needs no image
+            PDX_TRY(nk_next(c, precond, nv, G.h2, G.hn, 0, w, w, j + 1 < R ? k->Zb + (int64_t)(j + 1) * m : nullptr,
+                            &zp, &znp));
+            PDX_HIP(c, hipEventRecord(k->ev_col[j & 1], s));
+            enq++;
+            // with this column queued, wait for the previous one's flag
+            if (j >= 1) {
+                PDX_HIP(c, hipEventSynchronize(k->ev_col[(j - 1) & 1]));
+                if (__atomic_load_n(k->flag_host + ((j - 1) & 1), __ATOMIC_ACQUIRE)) break;
             }
         }
-        if (!done) j = restart;
-        // y = H^-1 g (upper triangular j x j)
-        for (int i = j - 1; i >= 0; i--) {
-            double sum = g[i];
-            for (int l = i + 1; l < j; l++) sum -= H[i + (size_t)(restart + 1) * l] * y[l];
-            y[i] = sum / H[i + (size_t)(restart + 1) * i];
-        }
-        PDX_HIP(c, hipMemcpyAsync(k->h_dev, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, c->stream));
-        hipLaunchKernelGGL(k_lincomb_add, grid1(m), dim3(256), 0, c->stream, m, j, k->Zb, m, k->h_dev, k->du);
+        hipLaunchKernelGGL(k_gm_solve, dim3(1), dim3(64), 0, s, R, G.H, G.g, G.flags, G.y, G.hn + 1);
+        hipLaunchKernelGGL(k_lincomb_add, grid1(m), dim3(256), 0, s, m, enq, k->Zb, m, G.y, k->du);
         PDX_HIP(c, hipGetLastError());
-        const double est = std::fabs(g[j]);
-        if (est <= thr || it >= maxiter) {
+        int cols = 0;
+        double est = 0.0;
+        PDX_HIP(c, hipMemcpyAsync(&cols, G.flags + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+        PDX_HIP(c, hipMemcpyAsync(&est, G.hn + 1, sizeof(double), hipMemcpyDeviceToHost, s));
+        PDX_HIP(c, hipStreamSynchronize(s));
+        it += cols;
+        if (est <= thr || it >= maxiter || cols == 0) {
             beta = est;
             break;
         }
         // true residual for the restart: r = b - J du
         PDX_TRY(pdx_nk_jv_dev(c, k->du, k->r, scale));
         it++;
-        hipLaunchKernelGGL(k_sub_from, grid1(m), dim3(256), 0, c->stream, m, b, k->r);
+        hipLaunchKernelGGL(k_sub_from, grid1(m), dim3(256), 0, s, m, b, k->r);
         PDX_HIP(c, hipGetLastError());
-        PDX_TRY(norm2(c, k->r, &beta));
+        PDX_TRY(nk_norm2_dev(c, k->r, G.hn));
+        double r2 = 0.0;
+        PDX_HIP(c, hipMemcpyAsync(&r2, G.hn, sizeof(double), hipMemcpyDeviceToHost, s));
+        PDX_HIP(c, hipStreamSynchronize(s));
+        beta = std::sqrt(r2);
     }
     *iters_out = it;
     *res_out = bnorm > 0 ? beta / bnorm : 0.0;

@@ -22,6 +22,7 @@
 // (update_disp_and_pos!, :247-255) into the other position buffer.  A prologue launch does
 // the first half step, so a run of N steps is N + 1 launches.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -346,16 +347,17 @@ __device__ __forceinline__ bool bond_fail16(uint16_t *wp, uint32_t wd, double e,
 // One bond of the FP64 BB step (bond_based.jl:125-161): failure against the point's eps_c,
 // then the force term (w bc) eps V_j dx / l accumulated into b; bcb = the bond's bc,
 // sc = its surface-correction factor (1 without).
-__device__ __forceinline__ void bb_bond64(double4 ri, double4 rj, uint16_t *wp, uint32_t wd, double Lb,
-                                          double sc, double bcb, double epsc, double epsc_lo,
-                                          double epsc_hi, double &bx, double &by, double &bz, int &na)
+// (rL = pdx_rcp1(Lb), passed in where one bond is evaluated at two states)
+__device__ __forceinline__ void bb_bond64_r(double4 ri, double4 rj, uint16_t *wp, uint32_t wd, double Lb,
+                                            double rL, double sc, double bcb, double epsc, double epsc_lo,
+                                            double epsc_hi, double &bx, double &by, double &bz, int &na)
 {
     const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
     const double d2 = (dx * dx + dy * dy) + dz * dz;
     double rl;
     const double l = pdx_sqrt_rsq(d2, rl);
     const double e = l - Lb;
-    const double eps = e * pdx_rcp1(Lb);
+    const double eps = e * rL;
     const bool act = bond_fail16(wp, wd, e, Lb, eps, epsc, epsc_lo, epsc_hi);
     const double wbc = act ? sc * bcb : 0.0;  // (w * bc), w = active * sc
     const double s = ((wbc * eps) * rj.w) * rl;
@@ -363,6 +365,13 @@ __device__ __forceinline__ void bb_bond64(double4 ri, double4 rj, uint16_t *wp, 
     by = __builtin_fma(s, dy, by);
     bz = __builtin_fma(s, dz, bz);
     na += act ? 1 : 0;
+}
+
+__device__ __forceinline__ void bb_bond64(double4 ri, double4 rj, uint16_t *wp, uint32_t wd, double Lb,
+                                          double sc, double bcb, double epsc, double epsc_lo,
+                                          double epsc_hi, double &bx, double &by, double &bz, int &na)
+{
+    bb_bond64_r(ri, rj, wp, wd, Lb, pdx_rcp1(Lb), sc, bcb, epsc, epsc_lo, epsc_hi, bx, by, bz, na);
 }
 
 // F = feature mask.  PDX_FEAT_MULTI: several parameter sets (force_density_point! with a
@@ -528,6 +537,202 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds(pdx_step_args A)
     static_assert(S == 4, "the tail takes four lanes per point");
     const double b[3] = {bx, by, bz};
     point_tail4(A, p, ri, b, sub, rho);
+}
+
+// ----------------------------------------------------------------------------------------
+// J v of the bond-based material by central differences (jacobian_vector_product!,
+// newton_krylov.jl:504-532) in one kernel: the block stages its neighbour table at the
+// positions X + (u + e v), walks its bonds, stages the table again at X + (u - e v) and
+// walks them again, and combines the two force densities (times the volume) into
+// (F+ - F-) / (2 e); constrained rows take v.  The positions are formed as the reference
+// forms them (u +- e v, then X + u, two roundings each), the bond arithmetic is that of
+// k_bb_lds, so the result equals the two force evaluations of the step kernel bit for bit.
+// Bonds that break at the + state stay broken at the - state, as in two successive
+// calc_force_density! calls; n_active and damage keep the - state's values.
+// ----------------------------------------------------------------------------------------
+__device__ __forceinline__ double4 jv_position(const double4 X, const double *u, const double *v,
+                                               int64_t j, double e)
+{
+    const double u0 = u[3 * j] + e * v[3 * j];
+    const double u1 = u[3 * j + 1] + e * v[3 * j + 1];
+    const double u2 = u[3 * j + 2] + e * v[3 * j + 2];
+    return make_double4(X.x + u0, X.y + u1, X.z + u2, X.w);
+}
+
+template <int U, int S>
+__global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_jv(pdx_step_args A, const double *u, const double *v,
+                                                          const double *un2, const double *parts, int np,
+                                                          double sc, const uint8_t *cmask, double *jv)
+{
+    extern __shared__ double4 hrec[];
+    __shared__ double shs[PDX_BLOCK * S / 64];
+    const int64_t k = pdx_xcd_block();
+    const int64_t p0 = k * PDX_BLOCK;
+    if (p0 >= A.n) return;  // whole workgroup
+    const double vn2 = pdx_block_sum_parts(parts, np, shs);
+    const int64_t h0 = A.halo_ptr[k];
+    const int H = (int)(A.halo_ptr[k + 1] - h0);
+    const int32_t *const hidx = A.halo + h0;
+    const int sub = threadIdx.x % S;
+    const int64_t p = p0 + threadIdx.x / S;
+    const bool valid = p < A.n;
+    const double e = pdx_nk_perturbation_v(*un2, vn2, sc);
+    const int c = valid ? A.cnt[p] : 0;
+    const int64_t b0 = valid ? pdx_bond0(A.slice_ptr, p) : 0;
+    uint16_t *const wbase = A.bw16 + b0;
+    const double *const Lbase = A.L + b0;
+    // lane d < 3 keeps component d of F+ (the first pass); after the second, of F-
+    double fpd = 0.0, fmd = 0.0;
+    bool bad = false;
+    int na = 0;
+#pragma unroll 1
+    for (int pass = 0; pass < 2; pass++) {
+        const double es = pass == 0 ? e : -e;
+        if (pass == 1) __syncthreads();  // the + table is consumed
+        for (int h = threadIdx.x; h < H; h += PDX_BLOCK * S) {
+            const int32_t j = hidx[h];
+            hrec[h] = jv_position(A.Xr[j], u, v, j, es);
+        }
+        __syncthreads();
+        if (!valid) continue;  // all S lanes of the point together
+        const double4 ri = jv_position(A.Xr[p], u, v, p, es);
+        double bx = 0.0, by = 0.0, bz = 0.0;
+        int nap = 0;
+        for (int t = sub; t < c; t += U * S) {
+            uint32_t wd[U];
+            double Ld[U];
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                wd[q] = 0;
+                Ld[q] = 1.0;
+                if (t + q * S < c) {
+                    const int64_t o = (int64_t)(t + q * S) * 64;
+                    wd[q] = wbase[o];
+                    Ld[q] = Lbase[o];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                if (t + q * S >= c) continue;
+                const int slot = (int)(wd[q] & PDX_W16_SLOT);
+                bb_bond64(ri, hrec[slot], wbase + (int64_t)(t + q * S) * 64, wd[q], Ld[q], 1.0, A.bc,
+                          A.epsc, A.epsc_lo, A.epsc_hi, bx, by, bz, nap);
+            }
+        }
+#pragma unroll
+        for (int o = 1; o < S; o <<= 1) {
+            bx += __shfl_xor(bx, o);
+            by += __shfl_xor(by, o);
+            bz += __shfl_xor(bz, o);
+            nap += __shfl_xor(nap, o);
+        }
+        bad = bad || isnan(bx) || isnan(by) || isnan(bz);
+        const double fd = sub == 0 ? bx : (sub == 1 ? by : bz);
+        if (pass == 0) fpd = fd;
+        else fmd = fd;
+        na = nap;
+    }
+    if (!valid) return;
+    if (sub == 0) {
+        A.nact_out[p] = na;
+        A.dmg_out[p] = 1.0 - (double)na / (double)c;
+        if (bad) atomicMin(A.flags_out, (unsigned long long)A.step);
+    }
+    if (sub < 3) {
+        const int64_t kk = 3 * p + sub;
+        const double w = A.Xr[p].w;
+        jv[kk] = cmask[kk] ? v[kk] : (fpd * w - fmd * w) / (2.0 * e);
+    }
+}
+
+// The same J v with both tables staged at once (twice the LDS of k_bb_jv, one block per CU
+// for a typical table): each bond's word and length are read once and the bond is
+// evaluated at both states.  A bond that breaks at the + state is evaluated broken at the
+// - state, as above.
+template <int U, int S>
+__global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_jv1(pdx_step_args A, const double *u, const double *v,
+                                                           const double *un2, const double *parts, int np,
+                                                           double sc, const uint8_t *cmask, double *jv)
+{
+    extern __shared__ double4 hrec[];  // [0, H): + positions, [H, 2H): - positions
+    __shared__ double shs[PDX_BLOCK * S / 64];
+    const int64_t k = pdx_xcd_block();
+    const int64_t p0 = k * PDX_BLOCK;
+    if (p0 >= A.n) return;  // whole workgroup
+    const double vn2 = pdx_block_sum_parts(parts, np, shs);
+    const int64_t h0 = A.halo_ptr[k];
+    const int H = (int)(A.halo_ptr[k + 1] - h0);
+    const int32_t *const hidx = A.halo + h0;
+    const int sub = threadIdx.x % S;
+    const int64_t p = p0 + threadIdx.x / S;
+    const bool valid = p < A.n;  // the same for the S lanes of a point
+    const double e = pdx_nk_perturbation_v(*un2, vn2, sc);
+    for (int h = threadIdx.x; h < H; h += PDX_BLOCK * S) {
+        const int32_t j = hidx[h];
+        const double4 X = A.Xr[j];
+        hrec[h] = jv_position(X, u, v, j, e);
+        hrec[H + h] = jv_position(X, u, v, j, -e);
+    }
+    __syncthreads();
+    if (valid) {
+        const int c = A.cnt[p];
+        const int64_t b0 = pdx_bond0(A.slice_ptr, p);
+        uint16_t *const wbase = A.bw16 + b0;
+        const double *const Lbase = A.L + b0;
+        const double4 Xi = A.Xr[p];
+        const double4 rp = jv_position(Xi, u, v, p, e), rm = jv_position(Xi, u, v, p, -e);
+        double px = 0.0, py = 0.0, pz = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
+        int nap = 0, nam = 0;
+        for (int t = sub; t < c; t += U * S) {
+            uint32_t wd[U];
+            double Ld[U];
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                wd[q] = 0;
+                Ld[q] = 1.0;
+                if (t + q * S < c) {
+                    const int64_t o = (int64_t)(t + q * S) * 64;
+                    wd[q] = wbase[o];
+                    Ld[q] = Lbase[o];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                if (t + q * S >= c) continue;
+                const int slot = (int)(wd[q] & PDX_W16_SLOT);
+                uint16_t *const wp = wbase + (int64_t)(t + q * S) * 64;
+                const int before = nap;
+                const double rL = pdx_rcp1(Ld[q]);
+                bb_bond64_r(rp, hrec[slot], wp, wd[q], Ld[q], rL, 1.0, A.bc, A.epsc, A.epsc_lo, A.epsc_hi,
+                            px, py, pz, nap);
+                const uint32_t wm = nap != before ? wd[q] : (wd[q] & ~(uint32_t)PDX_W16_ACT);
+                bb_bond64_r(rm, hrec[H + slot], wp, wm, Ld[q], rL, 1.0, A.bc, A.epsc, A.epsc_lo, A.epsc_hi,
+                            mx, my, mz, nam);
+            }
+        }
+#pragma unroll
+        for (int o = 1; o < S; o <<= 1) {
+            px += __shfl_xor(px, o);
+            py += __shfl_xor(py, o);
+            pz += __shfl_xor(pz, o);
+            mx += __shfl_xor(mx, o);
+            my += __shfl_xor(my, o);
+            mz += __shfl_xor(mz, o);
+            nam += __shfl_xor(nam, o);
+        }
+        if (sub == 0) {
+            A.nact_out[p] = nam;
+            A.dmg_out[p] = 1.0 - (double)nam / (double)c;
+            if (isnan(px) || isnan(py) || isnan(pz) || isnan(mx) || isnan(my) || isnan(mz))
+                atomicMin(A.flags_out, (unsigned long long)A.step);
+        }
+        if (sub < 3) {
+            const int64_t kk = 3 * p + sub;
+            const double a = sub == 0 ? px : (sub == 1 ? py : pz);
+            const double b = sub == 0 ? mx : (sub == 1 ? my : mz);
+            jv[kk] = cmask[kk] ? v[kk] : (a * Xi.w - b * Xi.w) / (2.0 * e);
+        }
+    }
 }
 
 // ----------------------------------------------------------------------------------------
@@ -2371,6 +2576,35 @@ static void launch_osb_feat(pdx_ctx *c, int pass, dim3 G, dim3 BL, int entry, co
     const void *fn = pass == 1 ? (const void *)&k_osb_lds1<PDX_LDS_LANES, F>
                                : (const void *)&k_osb_lds2<PDX_LDS_LANES, F>;
     launch_lds(c, fn, (F & PDX_FEAT_CAP) != 0, true, entry, G, BL, a);
+}
+
+bool pdx_bb_jv_fusable(const pdx_ctx *c)
+{
+    return c->kind == PDX_BB && c->lds_mode && pdx_features(c) == 0 && !lds_capped(c, 32);
+}
+
+pdx_status pdx_launch_bb_jv(pdx_ctx *c, const double *u, const double *v, const double *un2,
+                            const double *parts, int np, double sc, const uint8_t *cmask,
+                            double *jv)
+{
+    if (!pdx_bb_jv_fusable(c)) return pdx_fail(c, PDX_ERR_UNSUPPORTED, "fused J v: layout not supported");
+    pdx_step_args a;
+    PDX_TRY(pdx_fill_step_args(c, &a, 0, PDX_F_FORCE_ONLY | PDX_F_SCALE_VOL | PDX_F_FP64, 0.0, nullptr));
+    const int64_t blocks = (c->n + PDX_BLOCK - 1) / PDX_BLOCK;
+    const dim3 G((unsigned)((blocks + PDX_NUM_XCD - 1) / PDX_NUM_XCD * PDX_NUM_XCD));
+    const dim3 BL(PDX_BLOCK * PDX_LDS_LANES);
+    const size_t two = (size_t)std::max(c->halo_max, 1) * 64 + 16;
+    const char *env = std::getenv("PDX_JV_ONE_PASS");
+    // (the static LDS of the sum adds 128 bytes)
+    const bool one_pass = (int64_t)two + 256 <= c->lds_limit && !(env && env[0] == '0');
+    if (one_pass)
+        hipLaunchKernelGGL((k_bb_jv1<4, PDX_LDS_LANES>), G, BL, two, c->stream, a, u, v, un2, parts, np, sc,
+                           cmask, jv);
+    else
+        hipLaunchKernelGGL((k_bb_jv<2, PDX_LDS_LANES>), G, BL, lds_bytes(c, 32), c->stream, a, u, v, un2,
+                           parts, np, sc, cmask, jv);
+    PDX_HIP(c, hipGetLastError());
+    return PDX_OK;
 }
 
 pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)

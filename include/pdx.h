@@ -82,6 +82,12 @@ typedef int32_t pdx_status;
 #define PDX_FIELD_DENSITY_MATRIX 15 /* n, read only: the DynamicRelaxation Lambda of a point
                                      (the reference stores it 3 x n with equal rows) */
 
+/* Influence functions omega(delta, L) (src/discretization/kernels.jl). */
+#define PDX_KERNEL_LINEAR 0             /* linear_kernel, delta / L (:22), the default */
+#define PDX_KERNEL_CONST_ONE 1          /* const_one_kernel, 1 (:10) */
+#define PDX_KERNEL_CUBIC_BSPLINE 2      /* cubic_b_spline_kernel (:43-52) */
+#define PDX_KERNEL_CUBIC_BSPLINE_NORM 3 /* cubic_b_spline_kernel_norm (:74-84) */
+
 /* Point parameters of one material (StandardPointParameters, src/core/parameters.jl:107-126;
  * CPointParameters, src/physics/correspondence.jl:148-169).  The host derives them as
  * material! does (src/physics/material_parameters.jl:64-231, src/physics/fracture.jl:110-130,
@@ -89,7 +95,7 @@ typedef int32_t pdx_status;
 typedef struct pdx_params {
     double delta, rho, E, nu, G, K, lambda, mu, Gc, epsc, bc;
     int32_t model;  /* PDX_SVK, PDX_NHP, PDX_LINEAR_ELASTIC or PDX_NEO_HOOKE (CPD only) */
-    int32_t kernel; /* 0 = linear_kernel delta/L (src/discretization/kernels.jl:22) */
+    int32_t kernel; /* influence function of OSB and CPD (PDX_KERNEL_*); BB has none */
     double zem_cs;  /* ZEMSilling Cs, src/discretization/zem_stabilization.jl:16-21 */
     double maxdmg;  /* CMaterial maxdmg, correspondence.jl:123-128 */
     int32_t zem;    /* PDX_ZEM_SILLING or PDX_ZEM_WAN (CPD only) */

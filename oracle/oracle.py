@@ -139,6 +139,34 @@ def _c(a, dtype):
     return np.ascontiguousarray(a, dtype=dtype)
 
 
+def influence(kernel, delta, L):
+    """The influence function omega(delta, L) of every bond (find_kernels,
+    bond_system.jl:184-196), vectorised over L, as src/discretization/kernels.jl evaluates
+    it: "linear" delta / L (:22), "const_one" 1 (:10), "cubic_b_spline" (:43-52) and
+    "cubic_b_spline_norm" (:74-84) of xi = L / delta, with Julia's literal powers
+    (xi^2 = xi xi, xi^3 = xi xi xi) and left-to-right sums."""
+    L = np.asarray(L, np.float64)
+    if kernel == "linear":
+        return delta / L
+    if kernel == "const_one":
+        return np.ones_like(L)
+    xi = L / delta
+    x2, x3 = xi * xi, xi * xi * xi
+    lo = (xi > 0) & (xi <= 0.5)
+    hi = (xi > 0.5) & (xi <= 1.0)
+    if kernel == "cubic_b_spline":
+        a = ((2 / 3) - 4.0 * x2) + 4.0 * x3
+        b = (((4 / 3) - 4.0 * xi) + 4.0 * x2) - (4 / 3) * x3
+        return np.where(lo, a, np.where(hi, b, 0.0))
+    if kernel == "cubic_b_spline_norm":
+        C = 8.0 / (np.pi * (delta * delta * delta))
+        t = 1.0 - xi
+        a = C * ((1.0 - 6.0 * x2) + 6.0 * x3)
+        b = (C * 2.0) * (t * t * t)
+        return np.where(lo, a, np.where(hi, b, 0.0))
+    raise ValueError(kernel)
+
+
 class DuplicatePointError(RuntimeError):
     pass
 
@@ -259,9 +287,10 @@ class OracleSim:
         self.n_active = np.bincount(owner, weights=self.active, minlength=n).astype(np.int64)
         with np.errstate(invalid="ignore", divide="ignore"):
             self.damage = 1.0 - self.n_active / self.n_neighbors
-        # kernels (find_kernels): linear_kernel(delta_i, L) = delta_i / L for OSB and CPD
+        # kernels (find_kernels): omega(delta_i, L) per bond for OSB and CPD (params
+        # "kernel": "linear" (default), "const_one", "cubic_b_spline", "cubic_b_spline_norm")
         if kind in ("OSB", "CPD"):
-            self.omega = self.p["delta"] / L
+            self.omega = influence(self.p.get("kernel", "linear"), self.p["delta"], L)
         else:
             self.omega = None
         # reverse bond index of the two-phase OSB/CPD loops (built once: the bond list is fixed)

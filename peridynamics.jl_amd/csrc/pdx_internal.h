@@ -57,6 +57,7 @@ struct pdx_pset {
 #define PDX_FEAT_ZW 8     // CPD with ZEMWan (stabilisation stiffness per point)
 #define PDX_FEAT_SC 2     // EnergySurfaceCorrection scfactor per bond
 #define PDX_FEAT_CAP 4    // neighbour tables longer than the LDS budget: the rest from global memory
+#define PDX_FEAT_KERN 16  // OSB with an influence function other than linear_kernel
 struct pdx_gnn;  // learned preconditioner (pdx_gnn.hip)
 
 struct pdx_ctx {
@@ -213,6 +214,34 @@ pdx_status pdx_timed_end(pdx_ctx *c, const char *name);
 pdx_status pdx_timed_flush(pdx_ctx *c);
 
 // device helpers --------------------------------------------------------------------------
+// Influence function omega(delta, L) of the OSB and CPD bonds (kernels.jl), rL = 1/L.  The
+// default linear_kernel delta / L is formed as delta * (1/L) (within an ulp); the others as
+// Julia evaluates them: xi = L / delta, xi^2 = xi xi, xi^3 = xi xi xi, left to right.
+__device__ __noinline__ double pdx_omega_other(int kid, double delta, double L)
+{
+    if (kid == PDX_KERNEL_CONST_ONE) return 1.0;  // const_one_kernel (:10)
+    const double xi = L / delta;
+    const double x2 = xi * xi, x3 = xi * xi * xi;
+    if (kid == PDX_KERNEL_CUBIC_BSPLINE) {  // cubic_b_spline_kernel (:43-52)
+        if (0.0 < xi && xi <= 0.5) return ((2.0 / 3.0) - 4.0 * x2) + 4.0 * x3;
+        if (0.5 < xi && xi <= 1.0) return (((4.0 / 3.0) - 4.0 * xi) + 4.0 * x2) - (4.0 / 3.0) * x3;
+        return 0.0;
+    }
+    // cubic_b_spline_kernel_norm (:74-84)
+    const double C = 8.0 / (3.141592653589793 * (delta * delta * delta));
+    if (0.0 < xi && xi <= 0.5) return C * ((1.0 - 6.0 * x2) + 6.0 * x3);
+    if (0.5 < xi && xi <= 1.0) {
+        const double t = 1.0 - xi;
+        return (C * 2.0) * (t * t * t);
+    }
+    return 0.0;
+}
+
+__device__ __forceinline__ double pdx_omega(int kid, double delta, double L, double rL)
+{
+    return kid == PDX_KERNEL_LINEAR ? delta * rL : pdx_omega_other(kid, delta, L);
+}
+
 // finite-difference perturbation of J v (compute_perturbation, newton_krylov.jl:534-566):
 // eps = scale cbrt(eps_mach) (||u|| + 1) / ||v||, with sc = scale cbrt(eps_mach) and the
 // squared norms on the device (a vanishing ||v|| counts as 1)
@@ -400,6 +429,7 @@ struct pdx_step_args {
     // material
     double bc, epsc, epsc_lo, epsc_hi, delta, G, K, lambda, mu, beta_zem, maxdmg, nu;
     int model;
+    int kernel;  // influence function (PDX_KERNEL_*)
     double *pstate;
     uint8_t *killed;
     double *defgrad, *cauchy;

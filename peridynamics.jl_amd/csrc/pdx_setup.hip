@@ -197,7 +197,12 @@ extern "C" pdx_status pdx_set_material(pdx_ctx *c, int32_t kind, const pdx_param
     if (kind == PDX_CPD && p->model != PDX_SVK && p->model != PDX_NHP && p->model != PDX_LINEAR_ELASTIC &&
         p->model != PDX_NEO_HOOKE)
         return pdx_fail(c, PDX_ERR_ARG, "unknown constitutive model %d", p->model);
-    if (p->kernel != 0) return pdx_fail(c, PDX_ERR_UNSUPPORTED, "only linear_kernel is built");
+    if (p->kernel < PDX_KERNEL_LINEAR || p->kernel > PDX_KERNEL_CUBIC_BSPLINE_NORM)
+        return pdx_fail(c, PDX_ERR_ARG, "unknown influence function %d", p->kernel);
+    if (kind == PDX_BB && p->kernel != PDX_KERNEL_LINEAR)
+        return pdx_fail(c, PDX_ERR_ARG, "BBMaterial has no influence function");
+    if (p->kernel != PDX_KERNEL_LINEAR && c->corr)
+        return pdx_fail(c, PDX_ERR_UNSUPPORTED, "the surface correction is built for linear_kernel");
     if (c->bonds_built && c->kind != kind)
         return pdx_fail(c, PDX_ERR_STATE, "material kind changed after the bonds were built");
     c->kind = kind;
@@ -226,6 +231,8 @@ extern "C" pdx_status pdx_set_material_sets(pdx_ctx *c, int32_t kind, int32_t n_
             return pdx_fail(c, PDX_ERR_ARG, "horizon and rho have to be larger than zero");
         if (sets[k].delta != sets[0].delta)
             return pdx_fail(c, PDX_ERR_UNSUPPORTED, "the parameter sets of a body must share the horizon");
+        if (sets[k].kernel != sets[0].kernel)
+            return pdx_fail(c, PDX_ERR_ARG, "the parameter sets of a body share the material's influence function");
     }
     std::vector<uint8_t> of(c->n);
     for (int64_t i = 0; i < c->n; i++) {

@@ -1138,7 +1138,7 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_osb_pass1(pdx_step_args A)
         const double rL = pdx_rcp1(Lb);
         const bool act = bond_fail(A, A.bw + b, wd, e, Lb, e * rL);
         if (act) {
-            const double w = A.delta * rL;  // linear_kernel(delta, L)
+            const double w = pdx_omega(A.kernel, A.delta, Lb, rL);
             wvol += (w * (Lb * Lb)) * rj.w;
             S += ((w * Lb) * e) * rj.w;
             na++;
@@ -1180,7 +1180,7 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_osb_pass2(pdx_step_args A)
         const double Lb = A.L[b];
         double rl;
         const double l = pdx_sqrt_rsq((dx * dx + dy * dy) + dz * dz, rl);
-        const double w = A.delta * pdx_rcp1(Lb);
+        const double w = pdx_omega(A.kernel, A.delta, Lb, pdx_rcp1(Lb));
         const double e = l - Lb;
         const double fi = ci.y * Lb + ci.x * e;
         const double fj = cj.y * Lb + cj.x * e;
@@ -1306,7 +1306,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds1(pdx_step_args A)
                     act = bond_fail16(wbase + o, wd, e, Lb, e * rL, epsc, epsc_lo, epsc_hi);
                 }
                 if (act) {
-                    double w = A.delta * rL;  // linear_kernel(delta, L)
+                    double w = (F & PDX_FEAT_KERN) ? pdx_omega(A.kernel, A.delta, Lb, rL) : A.delta * rL;
                     if (SC) w *= A.scf[b0 + o];  // omega * beta
                     wvol += (w * (Lb * Lb)) * rj.w;
                     Ssum += ((w * Lb) * e) * rj.w;
@@ -1507,7 +1507,8 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds2(pdx_step_args A)
                 const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
                 double rl;
                 const double l = pdx_sqrt_rsq((dx * dx + dy * dy) + dz * dz, rl);
-                double w = A.delta * pdx_rcp1(Lb);
+                const double rLb = pdx_rcp1(Lb);
+                double w = (F & PDX_FEAT_KERN) ? pdx_omega(A.kernel, A.delta, Lb, rLb) : A.delta * rLb;
                 if (SC) w *= A.scf[b0 + o];  // beta is symmetric: both ends' copies of the bond carry it
                 const double e = l - Lb;
                 if (SPLIT) {
@@ -1815,7 +1816,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_pass1(pdx_step_args A)
             const bool act = bond_fail(A, A.bw + b, wd, e, Lb, e * rL);
             if (act && !dead) {
                 na++;
-                const double temp = (A.delta * rL) * rj.w;
+                const double temp = pdx_omega(A.kernel, A.delta, Lb, rL) * rj.w;
                 Km[0] += temp * (DX[0] * DX[0]);
                 Km[1] += temp * (DX[1] * DX[0]);
                 Km[2] += temp * (DX[2] * DX[0]);
@@ -1880,7 +1881,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_pass2(pdx_step_args A)
             const double4 Xj = A.Xr[j];
             const double dx[3] = {rj.x - ri.x, rj.y - ri.y, rj.z - ri.z};
             const double DX[3] = {Xj.x - Xi.x, Xj.y - Xi.y, Xj.z - Xi.z};
-            const double wv = (A.delta * pdx_rcp1(Lu[u])) * rj.w;  // w V_j
+            const double wv = pdx_omega(A.kernel, A.delta, Lu[u], pdx_rcp1(Lu[u])) * rj.w;  // w V_j
             const bool live_j = !A.killed[j];
             const double *qj = A.pstate + 9 * (int64_t)j;
 #pragma unroll
@@ -2000,7 +2001,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds1(pdx_step_args A)
             }
             if (act && !dead) {
                 na++;
-                const double temp = (A.delta * rL) * rj.w;
+                const double temp = pdx_omega(A.kernel, A.delta, Lb, rL) * rj.w;
                 Km[0] += temp * (DX[0] * DX[0]);
                 Km[1] += temp * (DX[1] * DX[0]);
                 Km[2] += temp * (DX[2] * DX[0]);
@@ -2103,7 +2104,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2(pdx_step_args A)
                 const double *const qj = hq + NQ * (int64_t)(slot - T0);
                 const double dx[3] = {rj.x - ri.x, rj.y - ri.y, rj.z - ri.z};
                 const double DX[3] = {Xj.x - Xi.x, Xj.y - Xi.y, Xj.z - Xi.z};
-                const double wv = (A.delta * pdx_rcp1(Lb)) * rj.w;  // w V_j
+                const double wv = pdx_omega(A.kernel, A.delta, Lb, pdx_rcp1(Lb)) * rj.w;  // w V_j
 #pragma unroll
                 for (int r = 0; r < 3; r++) {
                     double tt = 0.0;
@@ -2480,6 +2481,7 @@ pdx_status pdx_fill_step_args(pdx_ctx *c, pdx_step_args *a, int64_t step, int fl
     a->beta_zem = (c->P.zem_cs * c->P.bc) / c->P.delta;  // correspondence.jl:232
     a->maxdmg = c->P.maxdmg;
     a->model = c->P.model;
+    a->kernel = c->kind == PDX_BB ? PDX_KERNEL_LINEAR : c->P.kernel;
     a->pstate = c->pstate;
     a->killed = c->killed;
     a->defgrad = c->defgrad;
@@ -2491,7 +2493,8 @@ pdx_status pdx_fill_step_args(pdx_ctx *c, pdx_step_args *a, int64_t step, int fl
 // launches of the LDS-staged kernels with a feature mask F (PDX_FEAT_*)
 static int pdx_features(const pdx_ctx *c)
 {
-    return (c->multi() ? PDX_FEAT_MULTI : 0) | (c->corr ? PDX_FEAT_SC : 0);
+    return (c->multi() ? PDX_FEAT_MULTI : 0) | (c->corr ? PDX_FEAT_SC : 0) |
+           (c->kind == PDX_OSB && c->P.kernel != PDX_KERNEL_LINEAR ? PDX_FEAT_KERN : 0);
 }
 
 // A kernel whose table entries take `entry` LDS bytes stages the whole table if the longest
@@ -2557,7 +2560,8 @@ static void launch_bb_feat(pdx_ctx *c, bool mixed, dim3 G, dim3 BL, size_t lds, 
     else hipLaunchKernelGGL((k_bb_lds<2, PDX_LDS_LANES, F>), G, BL, lds, c->stream, a);
 }
 
-// feature masks 1..7 (PDX_FEAT_MULTI | PDX_FEAT_SC | PDX_FEAT_CAP)
+// feature masks 1..7 (PDX_FEAT_MULTI | PDX_FEAT_SC | PDX_FEAT_CAP); for OSB also those with
+// PDX_FEAT_KERN (never with the surface correction)
 #define PDX_FEAT_SWITCH(feat, CALL)                                                         \
     switch (feat) {                                                                         \
     case 1: CALL(1); break;                                                                 \
@@ -2568,6 +2572,14 @@ static void launch_bb_feat(pdx_ctx *c, bool mixed, dim3 G, dim3 BL, size_t lds, 
     case 6: CALL(6); break;                                                                 \
     case 7: CALL(7); break;                                                                 \
     default: break;                                                                         \
+    }
+#define PDX_FEAT_SWITCH_OSB(feat, CALL)                                                     \
+    switch (feat) {                                                                         \
+    case 16: CALL(16); break;                                                               \
+    case 17: CALL(17); break;                                                               \
+    case 20: CALL(20); break;                                                               \
+    case 21: CALL(21); break;                                                               \
+    default: PDX_FEAT_SWITCH(feat, CALL)                                                    \
     }
 
 template <int F>
@@ -2650,7 +2662,8 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
         PDX_TRY(pdx_timed_end(c, "step"));
         break;
     case PDX_OSB:
-        if (c->lds_mode && mixed && !c->multi()) {
+        // (an influence function other than linear_kernel runs the FP64 passes)
+        if (c->lds_mode && mixed && !c->multi() && c->P.kernel == PDX_KERNEL_LINEAR) {
             const dim3 BL(PDX_BLOCK * PDX_LDS_LANES);
             // pass 1 entries (D, V), (U, 0): 32 B; pass 2 also (c2 V)_j: 36 B
             const int sc = c->corr ? PDX_FEAT_SC : 0;
@@ -2689,7 +2702,7 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
 #define PDX_OSB2_LAUNCH(F) launch_osb_feat<F>(c, 2, G, BL, e2, a)
             PDX_TRY(pdx_timed_begin(c));
             if (f1) {
-                PDX_FEAT_SWITCH(f1, PDX_OSB1_LAUNCH)
+                PDX_FEAT_SWITCH_OSB(f1, PDX_OSB1_LAUNCH)
             } else {
                 launch_osb_feat<0>(c, 1, G, BL, e1, a);
             }
@@ -2697,7 +2710,7 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
             PDX_TRY(pdx_timed_end(c, "pass1"));
             PDX_TRY(pdx_timed_begin(c));
             if (f2) {
-                PDX_FEAT_SWITCH(f2, PDX_OSB2_LAUNCH)
+                PDX_FEAT_SWITCH_OSB(f2, PDX_OSB2_LAUNCH)
             } else {
                 launch_osb_feat<0>(c, 2, G, BL, e2, a);
             }
@@ -2722,7 +2735,8 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
             const int bud = pdx_lds_budget_full(c);
             af.lds_budget = bud;
             const dim3 BL(PDX_BLOCK * PDX_LDS_LANES);
-            if (mixed && !c->multi() && c->P.zem != PDX_ZEM_WAN) {  // otherwise the FP64 passes
+            if (mixed && !c->multi() && c->P.zem != PDX_ZEM_WAN &&
+                c->P.kernel == PDX_KERNEL_LINEAR) {  // otherwise the FP64 passes
                 // pass 1 entries (D, V), (U, 0): 32 B; pass 2 (D, V), (U, beta l), l Q': 68 B
                 const size_t l1 = lds_bytes(c, 32, bud), l2b = lds_bytes(c, 68, bud);
                 PDX_TRY(pdx_timed_begin(c));

@@ -183,6 +183,8 @@ extern "C" pdx_status pdx_set_surface_correction(pdx_ctx *c, int32_t kind)
     if (!c) return PDX_ERR_ARG;
     if (c->bonds_built) return pdx_fail(c, PDX_ERR_STATE, "the surface correction must precede pdx_build_bonds");
     if (kind != 0 && kind != 1) return pdx_fail(c, PDX_ERR_ARG, "unknown surface correction %d", kind);
+    if (kind != 0 && c->P.kernel != PDX_KERNEL_LINEAR)
+        return pdx_fail(c, PDX_ERR_UNSUPPORTED, "the surface correction is built for linear_kernel");
     c->corr = kind;
     return PDX_OK;
 }

@@ -102,13 +102,18 @@ model_id(::CMaterial{<:LinearElastic}) = Int32(2)
 model_id(::CMaterial{<:NeoHooke}) = Int32(3)
 model_id(_) = Int32(0)
 
-# influence function of OSBMaterial / CMaterial (src/discretization/kernels.jl): the device
-# passes compute linear_kernel δ/L in registers; any other kernel is refused, not replaced
+# influence function of OSBMaterial / CMaterial: the four of src/discretization/kernels.jl
+# (PDX_KERNEL_* of pdx.h); a user function has no device implementation and is refused, not
+# replaced.  BBMaterial has none.
 kernel_id(::BBMaterial) = Int32(0)
 function kernel_id(mat::Union{OSBMaterial,CMaterial})
     mat.kernel === linear_kernel && return Int32(0)
-    throw(ArgumentError("kernel $(mat.kernel) has no MI355X implementation; " *
-                        "the device passes provide linear_kernel only"))
+    mat.kernel === const_one_kernel && return Int32(1)
+    mat.kernel === cubic_b_spline_kernel && return Int32(2)
+    mat.kernel === cubic_b_spline_kernel_norm && return Int32(3)
+    throw(ArgumentError("kernel $(mat.kernel) has no MI355X implementation; the device " *
+                        "evaluates linear_kernel, const_one_kernel, cubic_b_spline_kernel " *
+                        "and cubic_b_spline_kernel_norm"))
 end
 
 function pdx_params(body, p)

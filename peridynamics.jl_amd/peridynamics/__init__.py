@@ -11,7 +11,8 @@ from .data_handler import MI355XBodyDataHandler
 from .job import Job, submit
 from .materials import (BBMaterial, CMaterial, CriticalStretch, EnergySurfaceCorrection,
                         LinearElastic, NeoHooke, NeoHookePenalty, NoCorrection, OSBMaterial,
-                        SaintVenantKirchhoff, ZEMSilling, ZEMWan, linear_kernel)
+                        SaintVenantKirchhoff, ZEMSilling, ZEMWan, const_one_kernel,
+                        cubic_b_spline_kernel, cubic_b_spline_kernel_norm, linear_kernel)
 from .point_generators import jitter, lattice_block, uniform_box
 from .postprocess import crack_tip, mode_i_plate
 from .solvers import DynamicRelaxation, NewtonKrylov, VelocityVerlet
@@ -23,7 +24,9 @@ __all__ = [
     "n_points", "no_failure", "point_set", "precrack", "velocity_bc", "velocity_ic",
     "MI355XBodyDataHandler",
     "Job", "submit", "BBMaterial", "CMaterial", "CriticalStretch", "NeoHookePenalty",
-    "OSBMaterial", "SaintVenantKirchhoff", "LinearElastic", "NeoHooke", "ZEMSilling", "ZEMWan", "linear_kernel", "jitter",
+    "OSBMaterial", "SaintVenantKirchhoff", "LinearElastic", "NeoHooke", "ZEMSilling", "ZEMWan",
+    "linear_kernel", "const_one_kernel", "cubic_b_spline_kernel", "cubic_b_spline_kernel_norm",
+    "jitter",
     "lattice_block", "uniform_box", "NewtonKrylov", "VelocityVerlet", "crack_tip",
     "mode_i_plate", "DynamicRelaxation", "EnergySurfaceCorrection", "NoCorrection",
     "VTKExporter", "process_each_export", "read_vtk", "write_pvtu", "write_vtu",

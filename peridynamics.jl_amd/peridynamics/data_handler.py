@@ -10,7 +10,7 @@ import numpy as np
 from . import _lib
 from ._lib import ArgumentError, Context, NaNError, PdxNkOpts, PdxNkStats, PdxParams
 from .body import eval_posdep, initial_velocity
-from .materials import CMaterial
+from .materials import KERNEL_IDS, CMaterial, OSBMaterial
 
 
 def _arg_error(msg):
@@ -68,6 +68,8 @@ class MI355XBodyDataHandler:
         pp = PdxParams(delta=p.delta, rho=p.rho, E=p.E, nu=p.nu, G=p.G, K=p.K, lambda_=p.lam,
                        mu=p.mu, Gc=p.Gc, epsc=p.epsc, bc=p.bc, model=0, kernel=0, zem_cs=0.5,
                        maxdmg=0.85)
+        if isinstance(mat, (OSBMaterial, CMaterial)):
+            pp.kernel = KERNEL_IDS[mat.kernel]
         if isinstance(mat, CMaterial):
             pp.model = mat.constitutive_model.code
             pp.zem = mat.zem.code

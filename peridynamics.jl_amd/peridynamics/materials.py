@@ -5,8 +5,8 @@ src/physics/bond_based.jl:74-113, ordinary_state_based.jl:82-96, correspondence.
 import math
 from dataclasses import dataclass, field
 
-from ._lib import (ArgumentError, PDX_BB, PDX_CPD, PDX_LINEAR_ELASTIC, PDX_NEO_HOOKE, PDX_NHP,
-                   PDX_OSB, PDX_SVK)
+from ._lib import (ArgumentError, PdxError, PDX_BB, PDX_CPD, PDX_ERR_UNSUPPORTED,
+                   PDX_LINEAR_ELASTIC, PDX_NEO_HOOKE, PDX_NHP, PDX_OSB, PDX_SVK)
 
 
 def _arg_error(msg):
@@ -67,9 +67,50 @@ class CriticalStretch:
         return "CriticalStretch()"
 
 
+def const_one_kernel(delta, L):
+    """kernels.jl:10: omega = 1."""
+    return 1
+
+
 def linear_kernel(delta, L):
-    """kernels.jl:22."""
+    """kernels.jl:22: omega = delta / L."""
     return delta / L
+
+
+def cubic_b_spline_kernel(delta, L):
+    """kernels.jl:43-52: the cubic B-spline of xi = L / delta."""
+    xi = L / delta
+    if 0 < xi <= 0.5:
+        return 2 / 3 - 4 * xi ** 2 + 4 * xi ** 3
+    if 0.5 < xi <= 1:
+        return 4 / 3 - 4 * xi + 4 * xi ** 2 - 4 / 3 * xi ** 3
+    return 0
+
+
+def cubic_b_spline_kernel_norm(delta, L):
+    """kernels.jl:74-84: the cubic B-spline scaled to unit integral over the horizon."""
+    xi = L / delta
+    C = 8.0 / (math.pi * delta ** 3)
+    if 0 < xi <= 0.5:
+        return C * (1 - 6 * xi ** 2 + 6 * xi ** 3)
+    if 0.5 < xi <= 1:
+        return C * 2 * (1 - xi) ** 3
+    return 0
+
+
+# the influence functions the device evaluates (PDX_KERNEL_* of include/pdx.h)
+KERNEL_IDS = {linear_kernel: 0, const_one_kernel: 1, cubic_b_spline_kernel: 2,
+              cubic_b_spline_kernel_norm: 3}
+
+
+def _kernel(kernel, correction=None):
+    if kernel not in KERNEL_IDS:
+        raise _arg_error("the influence functions are linear_kernel, const_one_kernel, "
+                         "cubic_b_spline_kernel and cubic_b_spline_kernel_norm")
+    if kernel is not linear_kernel and correction is EnergySurfaceCorrection:
+        raise PdxError(PDX_ERR_UNSUPPORTED,
+                       "the MI355X build provides EnergySurfaceCorrection with linear_kernel only")
+    return kernel
 
 
 class NoCorrection:
@@ -111,11 +152,9 @@ class OSBMaterial:
     kind = PDX_OSB
 
     def __init__(self, kernel=linear_kernel, dmgmodel=None, correction=None):
-        if kernel is not linear_kernel:
-            raise _arg_error("only linear_kernel is provided by the MI355X build")
-        self.kernel = kernel
-        self.dmgmodel = dmgmodel or CriticalStretch()
         self.correction = _correction(correction)
+        self.kernel = _kernel(kernel, self.correction)
+        self.dmgmodel = dmgmodel or CriticalStretch()
 
     def __repr__(self):
         return f"OSBMaterial{{{self.correction.__name__}}}()"
@@ -126,9 +165,7 @@ class CMaterial:
     kind = PDX_CPD
 
     def __init__(self, kernel=linear_kernel, model=None, zem=None, dmgmodel=None, maxdmg=0.85):
-        if kernel is not linear_kernel:
-            raise _arg_error("only linear_kernel is provided by the MI355X build")
-        self.kernel = kernel
+        self.kernel = _kernel(kernel)
         self.constitutive_model = model or SaintVenantKirchhoff()
         if not isinstance(self.constitutive_model,
                           (SaintVenantKirchhoff, NeoHookePenalty, LinearElastic, NeoHooke)):

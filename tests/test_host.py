@@ -172,3 +172,27 @@ def test_library_version_and_null_context():
     assert b"gfx950" in lib.pdx_version()
     assert lib.pdx_last_error(None) == b"null context"
     assert lib.pdx_set_profiling(None, 1) == -1
+
+
+def test_influence_functions_of_the_materials():
+    """OSBMaterial / CMaterial(kernel=...) take the four influence functions of kernels.jl
+    and pass their ids to the library (PDX_KERNEL_* of include/pdx.h); another function is
+    an ArgumentError (the device evaluates these four only), and the surface correction is
+    built for linear_kernel only."""
+    from peridynamics.materials import KERNEL_IDS
+    assert [KERNEL_IDS[k] for k in (pd.linear_kernel, pd.const_one_kernel,
+                                    pd.cubic_b_spline_kernel, pd.cubic_b_spline_kernel_norm)] == [0, 1, 2, 3]
+    for k in KERNEL_IDS:
+        assert pd.OSBMaterial(kernel=k).kernel is k
+        assert pd.CMaterial(kernel=k).kernel is k
+    with pytest.raises(pd.ArgumentError):
+        pd.OSBMaterial(kernel=lambda d, L: 1.0)
+    with pytest.raises(pd.ArgumentError):
+        pd.CMaterial(kernel=lambda d, L: 1.0)
+    with pytest.raises(pd.PdxError):
+        pd.OSBMaterial(kernel=pd.cubic_b_spline_kernel, correction=pd.EnergySurfaceCorrection)
+    assert pd.OSBMaterial(kernel=pd.linear_kernel, correction=pd.EnergySurfaceCorrection)
+    # the mirror's scalar functions agree with kernels.jl's test values
+    assert pd.cubic_b_spline_kernel(1.0, 0.5) == pytest.approx(1 / 6)
+    assert pd.cubic_b_spline_kernel_norm(1.0, 0.75) == pytest.approx(8 / np.pi * 2 * 0.25 ** 3)
+    assert pd.const_one_kernel(2.0, 1.0) == 1 and pd.linear_kernel(2.0, 1.0) == 2.0

@@ -63,7 +63,7 @@ class NewtonKrylov:
     def __init__(self, time=-1, steps=-1, stepsize=1.0, maxiter=100, tol=1e-4,
                  perturbation_scale=1.0, gmres_maxiter=200, gmres_reltol=1e-4,
                  gmres_abstol=1e-8, gmres_restart=100, rel_tol=0.0, precond="none",
-                 learn_steps=3, train_steps=150, train_batch=128, train_lr=3e-3,
+                 learn_steps=3, train_steps=40, train_batch=128, train_lr=6e-3,
                  krylov_pairs=512, train_seed=0, collect_precond="two_level", gate=None):
         if time > 0 and steps > 0:
             raise _arg_error("specify either time or number of steps, not both!")

@@ -148,8 +148,8 @@ def test_training_lowers_loss(need_gpu):
 
 def test_learned_preconditioner_gain(need_gpu):
     """Load steps 1-3 are solved with the two-level preconditioner of the translations while
-    their error samples are collected; the network is trained on them and its coarse space
-    preconditions load steps 4-10.  The learned preconditioner needs fewer Krylov iterations
+    their error samples are collected; the network is trained on them (the default 40 Adam
+    steps) and its coarse space preconditions load steps 4-10.  The learned preconditioner needs fewer Krylov iterations
     than the two-level one without the network (strict gain) and than block Jacobi, uses no
     fallback, and every solve reaches the 1e-8 relative Newton tolerance."""
     say("gnn gain")
@@ -158,8 +158,7 @@ def test_learned_preconditioner_gain(need_gpu):
         b = block(30, 30, 8)
         dh = pd.MI355XBodyDataHandler(b)
         nk = pd.NewtonKrylov(steps=10, tol=1e-30, rel_tol=1e-8, maxiter=30, gmres_restart=30,
-                             gmres_maxiter=300, precond=pre, learn_steps=3, train_steps=150,
-                             train_batch=128, train_lr=3e-3, krylov_pairs=512)
+                             gmres_maxiter=300, precond=pre, learn_steps=3, krylov_pairs=512)
         stats = dh.solve_nk(nk)
         late = stats[3:]
         res[pre] = (sum(s.krylov_iters for s in late), sum(s.seconds for s in late),

@@ -197,6 +197,29 @@ def jfnk_leg(shape, train_steps):
                                                      res["two_level"]["seconds_steps_4_10"])}
 
 
+def jfnk_profile_leg(shape, steps):
+    """The first `steps` load steps of the quasi-static block with block Jacobi, then with the
+    two-level preconditioner (the workload of the Krylov-iteration kernel trace): Krylov
+    iterations and milliseconds per iteration of every load step."""
+    import peridynamics as pd
+    out = {}
+    for pre in ("jacobi", "two_level"):
+        dh = pd.MI355XBodyDataHandler(jfnk_block(*shape))
+        nk = pd.NewtonKrylov(steps=10, tol=1e-30, rel_tol=1e-8, maxiter=40, gmres_restart=30,
+                             gmres_maxiter=200, precond=pre)
+        dh.init_newton_krylov(nk)
+        rows = []
+        for n in range(1, steps + 1):
+            st = dh.nk_load_step(nk, n)
+            rows.append({"step": n, "newton_iters": st.newton_iters, "krylov_iters": st.krylov_iters,
+                         "ms": st.seconds * 1e3,
+                         "ms_per_krylov_iteration": st.seconds * 1e3 / max(st.krylov_iters, 1)})
+            log(f"[bench] jfnk profile {pre} step {n}: {rows[-1]['ms_per_krylov_iteration']:.4f} ms/it")
+        out[pre] = rows
+        dh.close()
+    return out
+
+
 def state_based_bytes(kind, n_points, n_bonds, layout, precision="fp64", zem="Silling"):
     """Algorithmic bytes of the two passes of an OSB or CPD step (one parameter set).  Per
     bond and pass the bond word and the FP64 length (LDS layout 2 + 8, global layout
@@ -223,10 +246,10 @@ def state_based_bytes(kind, n_points, n_bonds, layout, precision="fp64", zem="Si
     return 24 * n_bonds + per_pt * n_points, f"24*N_b + {per_pt}*N_p"
 
 
-def state_based_leg(kind, shape, steps, warmup=10, dx=1e-3, precision="fp64", zem="Silling"):
-    """Configuration 3: OSB or CMaterial (Saint Venant-Kirchhoff; ZEMSilling, or ZEMWan
-    with zem="Wan") bar, torsion 10 rad/s on one end (three point layers), the other end
-    held."""
+def torsion_bar(kind, shape, dx=1e-3, zem="Silling"):
+    """Configuration 3's body: OSB or CMaterial (Saint Venant-Kirchhoff; ZEMSilling, or
+    ZEMWan with zem="Wan") bar, torsion 10 rad/s on one end (three point layers), the other
+    end held."""
     import peridynamics as pd
     pos, vol = pd.lattice_block(*shape, dx)
     cmat = (lambda: pd.CMaterial(zem=pd.ZEMWan())) if zem == "Wan" else pd.CMaterial
@@ -240,6 +263,15 @@ def state_based_leg(kind, shape, steps, warmup=10, dx=1e-3, precision="fp64", ze
     pd.velocity_bc(lambda p, t: 10.0 * p[1], body, "end", "z")
     for d in "xyz":
         pd.velocity_bc(lambda t: 0.0, body, "fixed", d)
+    return body
+
+
+def state_based_leg(kind, shape, steps, warmup=10, dx=1e-3, precision="fp64", zem="Silling"):
+    """Configuration 3: OSB or CMaterial (Saint Venant-Kirchhoff; ZEMSilling, or ZEMWan
+    with zem="Wan") bar, torsion 10 rad/s on one end (three point layers), the other end
+    held."""
+    import peridynamics as pd
+    body = torsion_bar(kind, shape, dx, zem)
     dh = pd.MI355XBodyDataHandler(body, precision=precision)
     vv = pd.VelocityVerlet(steps=warmup + steps)
     dh.init_time_solver(vv)
@@ -480,13 +512,112 @@ def export_leg(shape, steps, freq, fields=("displacement", "damage")):
     return res
 
 
-def cpu_baseline(sample_shape, target_seconds=12.0, max_steps=5000):
-    import numpy as np
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     oracle.build()
+    return oracle
+
+
+def _cpu_threads():
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
-    threads = min(threads, 16)
+    return min(threads, 16)
+
+
+def _oracle_params(body, kind):
+    import peridynamics as pd
+    p = body.point_params[0]
+    d = dict(delta=p.delta, rho=p.rho, E=p.E, nu=p.nu, G=p.G, K=p.K, bc=p.bc, epsc=p.epsc,
+             Gc=p.Gc, mu=p.mu)
+    d["lambda"] = p.lam
+    if kind == "CPD":
+        names = {pd.NeoHookePenalty: "NHP", pd.LinearElastic: "LE", pd.NeoHooke: "NH",
+                 pd.SaintVenantKirchhoff: "SVK"}
+        wan = isinstance(body.mat.zem, pd.ZEMWan)
+        d.update(model=names[type(body.mat.constitutive_model)],
+                 Cs=0.0 if wan else body.mat.zem.Cs, maxdmg=body.mat.maxdmg,
+                 zem="Wan" if wan else "Silling")
+    return d
+
+
+def cpu_state_based(kind, shape, target_seconds=10.0, max_steps=500):
+    """CPU baseline of the torsion-bar workload (the same body, material and velocity
+    conditions as state_based_leg) on the oracle's OpenMP loops: per-point phase and the
+    ordered gather in parallel, Velocity Verlet sweeps in numpy.  One untimed step, then
+    steps until target_seconds."""
+    import numpy as np
+    oracle = _oracle()
+    threads = _cpu_threads()
+    body = torsion_bar(kind, shape)
+    t0 = time.perf_counter()
+    sim = oracle.OracleSim(body.xyz, body.volume, kind, _oracle_params(body, kind),
+                           fail_permit=body.fail_permit.astype(np.uint8), nthreads=threads)
+    setup = time.perf_counter() - t0
+    dt = sim.stable_dt(0.7)
+    end, fixed = body.point_sets["end"], body.point_sets["fixed"]
+    bcs = [("sd", fixed, d, lambda t: 0.0) for d in range(3)]
+    bcs += [("pdv", end, 1, lambda X, t: -10.0 * X[:, 2]),
+            ("pdv", end, 2, lambda X, t: 10.0 * X[:, 1])]
+    sim.force()
+    sim.vv_run(1, dt, bcs, first_step=1)  # warm-up step
+    steps, elapsed = 0, 0.0
+    t0 = time.perf_counter()
+    while elapsed < target_seconds and steps < max_steps:
+        sim.vv_run(1, dt, bcs, first_step=2 + steps)
+        steps += 1
+        elapsed = time.perf_counter() - t0
+    nb = int(sim.row_ptr[-1])
+    return {"value": nb * steps / elapsed, "unit": "bond updates/s", "cores": threads,
+            "kind": "port", "ms_per_step": elapsed / steps * 1e3,
+            "sample": (f"{kind} torsion bar {shape[0]}x{shape[1]}x{shape[2]} ({body.n_points} "
+                       f"points, {nb} bonds), FP64, {steps} Velocity Verlet steps in "
+                       f"{elapsed:.1f} s after one untimed step (setup {setup:.1f} s)")}
+
+
+def cpu_jfnk(shape, target_seconds=10.0):
+    """CPU baseline of one Krylov iteration of the quasi-static block (the same body and
+    conditions as jfnk_leg): the oracle's central-difference J v (two OpenMP bond-based
+    force evaluations) and its restarted GMRES(30) with modified Gram-Schmidt in numpy,
+    unpreconditioned as the reference's, on the first Newton iteration of load step 1;
+    whole cycles until target_seconds."""
+    import numpy as np
+    oracle = _oracle()
+    threads = _cpu_threads()
+    body = jfnk_block(*shape)
+    sim = oracle.OracleSim(body.xyz, body.volume, "BB", _oracle_params(body, "BB"),
+                           fail_permit=body.fail_permit.astype(np.uint8), nthreads=threads)
+    bcs = ([(body.point_sets["bottom"], d, lambda p: 0.0) for d in range(3)] +
+           [(body.point_sets["top"], 1, lambda p: 1e-5)])  # load step 1 of 10 (1e-4 m)
+    cons = sim.constrained_dofs(bcs)
+    for pts, dim, f in bcs:
+        sim.u[np.asarray(pts, np.int64), dim] = f(None)
+    sim.x[...] = sim.X + sim.u
+    sim.force()
+    r = sim.residual(cons)
+    n_jv = [0]
+
+    def A(v):
+        n_jv[0] += 1
+        return sim.jv(v, cons)
+
+    # whole GMRES(30) cycles (30 products and the restart residual's) until target_seconds
+    el, cycles = 0.0, 0
+    t0 = time.perf_counter()
+    while el < target_seconds and cycles < 20:
+        oracle.gmres(A, -r, maxiter=30, reltol=1e-30, abstol=0.0, restart=30)
+        cycles += 1
+        el = time.perf_counter() - t0
+    return {"value": n_jv[0] / el, "unit": "Krylov iterations/s", "cores": threads,
+            "kind": "port", "ms_per_iteration": el / n_jv[0] * 1e3,
+            "sample": (f"bond-based block {shape[0]}x{shape[1]}x{shape[2]} ({body.n_points} "
+                       f"points, {int(sim.row_ptr[-1])} bonds), {n_jv[0]} unpreconditioned "
+                       f"GMRES(30) iterations (J v products) in {el:.1f} s")}
+
+
+def cpu_baseline(sample_shape, target_seconds=12.0, max_steps=5000):
+    import numpy as np
+    oracle = _oracle()
+    threads = _cpu_threads()
     body = make_body(*sample_shape)
     p = body.point_params[0]
     params = dict(delta=p.delta, rho=p.rho, bc=p.bc, epsc=p.epsc, G=p.G, K=p.K)
@@ -547,12 +678,19 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the state-based (512k-point bar) and mode-I plate legs")
     ap.add_argument("--jfnk-shape", type=str, default="100,100,10")
-    ap.add_argument("--jfnk-train-steps", type=int, default=150)
+    ap.add_argument("--jfnk-train-steps", type=int, default=40)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB device copy that calibrates the HBM counters")
+    ap.add_argument("--jfnk-profile", type=int, default=0, metavar="STEPS",
+                    help="only run the first STEPS load steps of the JFNK block with block "
+                         "Jacobi and two-level (the Krylov-iteration profile workload)")
     args = ap.parse_args()
     if args.copy_calibration:
         print(json.dumps({"copy_calibration": copy_calibration()}), flush=True)
+        return
+    if args.jfnk_profile:
+        shape = tuple(int(v) for v in args.jfnk_shape.split(","))
+        print(json.dumps({"jfnk_profile": jfnk_profile_leg(shape, args.jfnk_profile)}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
@@ -709,8 +847,27 @@ def main():
                                args.jfnk_train_steps)
 
     if not args.no_cpu:
+        # numpy's BLAS threads spin between calls and steal the cores of the oracle's OpenMP
+        # loops: the CPU legs run BLAS single-threaded (their vector work is memory-bound)
+        from threadpoolctl import threadpool_limits
+        blas1 = threadpool_limits(1, user_api="blas")
         log("[bench] CPU baseline")
         out["cpu_baseline"] = cpu_baseline(tuple(int(s) for s in args.cpu_sample.split(",")))
+        if not args.no_extra:
+            for k in ("OSB", "CPD"):
+                log(f"[bench] CPU baseline, {k} torsion bar")
+                c = cpu_state_based(k, (160, 80, 40))
+                out["state_based"][k]["cpu_baseline"] = c
+                c["gpu_speedup"] = out["state_based"][k]["bond_updates_per_s"] / c["value"]
+        if not args.no_jfnk:
+            log("[bench] CPU baseline, JFNK")
+            j = out["jfnk"]
+            c = cpu_jfnk(tuple(int(v) for v in args.jfnk_shape.split(",")))
+            gpu_it = j["jacobi"]["seconds"] / j["jacobi"]["krylov_iters"]
+            c["gpu_ms_per_iteration_block_jacobi"] = gpu_it * 1e3
+            c["gpu_speedup_per_iteration"] = c["ms_per_iteration"] / (gpu_it * 1e3)
+            j["cpu_baseline"] = c
+        blas1.unregister()
     log(summary_line(out))
     print(json.dumps(out), flush=True)
 

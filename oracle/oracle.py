@@ -124,7 +124,7 @@ def _declare(L):
     L.orc_update_acc_and_vel_sets.argtypes = [_I64, _P, _P, _P, _P, _D, _P, _P]
     L.orc_esc_factors.restype = None
     L.orc_esc_factors.argtypes = [_I64, _INT, _INT, _P, _P, _P, _P, _P, _P, _D, _P, _P, _P, _P,
-                                  _P, _P, _P]
+                                  _P, _P, _P, _P]
     L.orc_has_nan.restype = _INT
     L.orc_has_nan.argtypes = [_I64, _P]
     L.orc_max_threads.restype = _INT
@@ -332,7 +332,9 @@ class OracleSim:
                                   _p(self.X), _p(self.vol), _p(self.row_ptr), _p(self.nbr),
                                   _p(self.L), _p(self.active), float(self.p["delta"]),
                                   _p(pp["lambda"]), _p(pp["mu"]), _p(pp["bc"]), _p(pp["G"]),
-                                  _p(pp["K"]), _p(self.mfactor), _p(self.scf))
+                                  _p(pp["K"]), _p(self.mfactor), _p(self.scf),
+                                  _p(self.omega) if kind == "OSB" and
+                                  self.p.get("kernel", "linear") != "linear" else None)
         elif correction is not None:
             raise ValueError(correction)
 

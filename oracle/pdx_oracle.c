@@ -659,7 +659,9 @@ int orc_osb_force_sets(int64_t n, const double *X, const double *x, const double
  * one parameter set, :124-127), multi = 1 averages them over the active bonds (:129-145).
  * kind 0 = BB (strain_energy_density_point!, bond_based.jl:183-201), 1 = OSB
  * (ordinary_state_based.jl:207-235), both at scfactor = 1 and the reference configuration
- * stretched by 1.001 along one axis at a time.  mfactor[3 i + d], scfactor[b]. */
+ * stretched by 1.001 along one axis at a time.  mfactor[3 i + d], scfactor[b].  omega: the
+ * OSB influence function per bond (kernel(system, bond_id), ordinary_state_based.jl:225), or
+ * null for linear_kernel delta / L. */
 static double orc_norm3(double a, double b, double c) { return sqrt((a * a + b * b) + c * c); }
 static double sq(double a) { return a * a; } /* Julia's literal x^2 */
 #ifndef M_PI
@@ -670,7 +672,8 @@ void orc_esc_factors(int64_t n, int kind, int multi, const double *X, const doub
                      const int64_t *row_ptr, const int32_t *nbr, const double *L,
                      const uint8_t *active, double delta, const double *lam_pt,
                      const double *mu_pt, const double *bc_pt, const double *G_pt,
-                     const double *K_pt, double *mfactor, double *scfactor)
+                     const double *K_pt, double *mfactor, double *scfactor,
+                     const double *omega)
 {
     const double st = 1.001;
     double *x = (double *)malloc(sizeof(double) * 3 * n);
@@ -710,7 +713,7 @@ void orc_esc_factors(int64_t n, int kind, int multi, const double *X, const doub
                     int64_t j = nbr[b];
                     double Dx = X[3 * j] - X[3 * i], Dy = X[3 * j + 1] - X[3 * i + 1],
                            Dz = X[3 * j + 2] - X[3 * i + 2];
-                    double w = active[b] ? delta / L[b] : 0.0;
+                    double w = active[b] ? (omega ? omega[b] : delta / L[b]) : 0.0;
                     wvol += ((w * 1.0) * ((Dx * Dx + Dy * Dy) + Dz * Dz)) * vol[j];
                 }
                 if (wvol != 0.0) {
@@ -719,7 +722,7 @@ void orc_esc_factors(int64_t n, int kind, int multi, const double *X, const doub
                         int64_t j = nbr[b];
                         double l = orc_norm3(x[3 * j] - x[3 * i], x[3 * j + 1] - x[3 * i + 1],
                                              x[3 * j + 2] - x[3 * i + 2]);
-                        double w = active[b] ? delta / L[b] : 0.0;
+                        double w = active[b] ? (omega ? omega[b] : delta / L[b]) : 0.0;
                         dil += ((((w * 1.0) * c1) * L[b]) * (l - L[b])) * vol[j];
                     }
                     double psi_vol = (0.5 * K_pt[i]) * (dil * dil), psi_dev = 0.0;
@@ -729,7 +732,7 @@ void orc_esc_factors(int64_t n, int kind, int multi, const double *X, const doub
                                              x[3 * j + 2] - x[3 * i + 2]);
                         double e = l - L[b];
                         double edev = e - ((1.0 / 3.0) * dil) * L[b];
-                        double w = active[b] ? delta / L[b] : 0.0;
+                        double w = active[b] ? (omega ? omega[b] : delta / L[b]) : 0.0;
                         double G = (G_pt[i] + G_pt[j]) / 2.0;
                         double cdev = (15.0 * G) / (2.0 * wvol);
                         psi_dev += ((((cdev * w) * 1.0) * edev) * edev) * vol[j];

@@ -201,8 +201,6 @@ extern "C" pdx_status pdx_set_material(pdx_ctx *c, int32_t kind, const pdx_param
         return pdx_fail(c, PDX_ERR_ARG, "unknown influence function %d", p->kernel);
     if (kind == PDX_BB && p->kernel != PDX_KERNEL_LINEAR)
         return pdx_fail(c, PDX_ERR_ARG, "BBMaterial has no influence function");
-    if (p->kernel != PDX_KERNEL_LINEAR && c->corr)
-        return pdx_fail(c, PDX_ERR_UNSUPPORTED, "the surface correction is built for linear_kernel");
     if (c->bonds_built && c->kind != kind)
         return pdx_fail(c, PDX_ERR_STATE, "material kind changed after the bonds were built");
     c->kind = kind;

@@ -2561,7 +2561,7 @@ static void launch_bb_feat(pdx_ctx *c, bool mixed, dim3 G, dim3 BL, size_t lds, 
 }
 
 // feature masks 1..7 (PDX_FEAT_MULTI | PDX_FEAT_SC | PDX_FEAT_CAP); for OSB also those with
-// PDX_FEAT_KERN (never with the surface correction)
+// PDX_FEAT_KERN
 #define PDX_FEAT_SWITCH(feat, CALL)                                                         \
     switch (feat) {                                                                         \
     case 1: CALL(1); break;                                                                 \
@@ -2577,8 +2577,12 @@ static void launch_bb_feat(pdx_ctx *c, bool mixed, dim3 G, dim3 BL, size_t lds, 
     switch (feat) {                                                                         \
     case 16: CALL(16); break;                                                               \
     case 17: CALL(17); break;                                                               \
+    case 18: CALL(18); break;                                                               \
+    case 19: CALL(19); break;                                                               \
     case 20: CALL(20); break;                                                               \
     case 21: CALL(21); break;                                                               \
+    case 22: CALL(22); break;                                                               \
+    case 23: CALL(23); break;                                                               \
     default: PDX_FEAT_SWITCH(feat, CALL)                                                    \
     }
 

@@ -45,7 +45,14 @@ __device__ __forceinline__ double3 esc_stretched(const double4 &X, int d)
                         d == 2 ? ESC_STRETCH * X.z : X.z);
 }
 
-__global__ void k_esc_mfactor(int64_t n, int d, int32_t kind, const int64_t *slice_ptr,
+// the OSB bond weight kernel(system, bond_id) (ordinary_state_based.jl:225): linear_kernel
+// delta / L as Julia divides, the other influence functions as the step kernels evaluate them
+__device__ __forceinline__ double esc_omega(int kid, double delta, double L)
+{
+    return kid == PDX_KERNEL_LINEAR ? delta / L : pdx_omega_other(kid, delta, L);
+}
+
+__global__ void k_esc_mfactor(int64_t n, int d, int32_t kind, int kid, const int64_t *slice_ptr,
                               const int32_t *cnt, const uint32_t *bw, const double *L,
                               const double4 *Xr, double delta, esc_par one, const pdx_pset *ps,
                               const uint8_t *pid, double *mfactor)
@@ -99,7 +106,7 @@ __global__ void k_esc_mfactor(int64_t n, int d, int32_t kind, const int64_t *sli
             const uint32_t wd = bw[b];
             const double4 Xj = Xr[wd & PDX_BW_NBR];
             const double Dx = Xj.x - Xi.x, Dy = Xj.y - Xi.y, Dz = Xj.z - Xi.z;
-            const double w = (wd & PDX_BW_ACT) ? delta / L[b] : 0.0;
+            const double w = (wd & PDX_BW_ACT) ? esc_omega(kid, delta, L[b]) : 0.0;
             wvol += ((w * 1.0) * ((Dx * Dx + Dy * Dy) + Dz * Dz)) * Xj.w;
         }
         if (wvol != 0.0) {  // a point without weighted volume keeps psi = 0
@@ -113,7 +120,7 @@ __global__ void k_esc_mfactor(int64_t n, int d, int32_t kind, const int64_t *sli
                 const double dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
                 const double l = sqrt((dx * dx + dy * dy) + dz * dz);
                 const double Lb = L[b];
-                const double w = (wd & PDX_BW_ACT) ? delta / Lb : 0.0;
+                const double w = (wd & PDX_BW_ACT) ? esc_omega(kid, delta, Lb) : 0.0;
                 dil += ((((w * 1.0) * c1) * Lb) * (l - Lb)) * Xj.w;
             }
             const double psi_vol = (0.5 * Pi.K) * (dil * dil);
@@ -129,7 +136,7 @@ __global__ void k_esc_mfactor(int64_t n, int d, int32_t kind, const int64_t *sli
                 const double Lb = L[b];
                 const double e = l - Lb;
                 const double edev = e - ((1.0 / 3.0) * dil) * Lb;
-                const double w = (wd & PDX_BW_ACT) ? delta / Lb : 0.0;
+                const double w = (wd & PDX_BW_ACT) ? esc_omega(kid, delta, Lb) : 0.0;
                 const double Gb = (Pi.G + esc_params(j, one, ps, pid).G) / 2.0;
                 const double cdev = (15.0 * Gb) / (2.0 * wvol);
                 psi_dev += ((((cdev * w) * 1.0) * edev) * edev) * Xj.w;
@@ -183,8 +190,6 @@ extern "C" pdx_status pdx_set_surface_correction(pdx_ctx *c, int32_t kind)
     if (!c) return PDX_ERR_ARG;
     if (c->bonds_built) return pdx_fail(c, PDX_ERR_STATE, "the surface correction must precede pdx_build_bonds");
     if (kind != 0 && kind != 1) return pdx_fail(c, PDX_ERR_ARG, "unknown surface correction %d", kind);
-    if (kind != 0 && c->P.kernel != PDX_KERNEL_LINEAR)
-        return pdx_fail(c, PDX_ERR_UNSUPPORTED, "the surface correction is built for linear_kernel");
     c->corr = kind;
     return PDX_OK;
 }
@@ -204,7 +209,8 @@ pdx_status pdx_build_correction(pdx_ctx *c)
     const esc_par one = {c->P.lambda, c->P.mu, c->P.bc, c->P.G, c->P.K};
     const dim3 G((unsigned)((n + 255) / 256)), B(256);
     for (int d = 0; d < 3; d++)
-        hipLaunchKernelGGL(k_esc_mfactor, G, B, 0, c->stream, n, d, c->kind, c->slice_ptr, c->cnt, c->bw,
+        hipLaunchKernelGGL(k_esc_mfactor, G, B, 0, c->stream, n, d, c->kind, (int)c->P.kernel, c->slice_ptr,
+                           c->cnt, c->bw,
                            c->L, c->Xr, c->P.delta, one, c->multi() ? c->d_psets : nullptr,
                            c->multi() ? c->d_pid : nullptr, c->mfactor);
     PDX_HIP(c, hipGetLastError());

@@ -5,8 +5,8 @@ src/physics/bond_based.jl:74-113, ordinary_state_based.jl:82-96, correspondence.
 import math
 from dataclasses import dataclass, field
 
-from ._lib import (ArgumentError, PdxError, PDX_BB, PDX_CPD, PDX_ERR_UNSUPPORTED,
-                   PDX_LINEAR_ELASTIC, PDX_NEO_HOOKE, PDX_NHP, PDX_OSB, PDX_SVK)
+from ._lib import (ArgumentError, PDX_BB, PDX_CPD, PDX_LINEAR_ELASTIC, PDX_NEO_HOOKE,
+                   PDX_NHP, PDX_OSB, PDX_SVK)
 
 
 def _arg_error(msg):
@@ -103,13 +103,10 @@ KERNEL_IDS = {linear_kernel: 0, const_one_kernel: 1, cubic_b_spline_kernel: 2,
               cubic_b_spline_kernel_norm: 3}
 
 
-def _kernel(kernel, correction=None):
+def _kernel(kernel):
     if kernel not in KERNEL_IDS:
         raise _arg_error("the influence functions are linear_kernel, const_one_kernel, "
                          "cubic_b_spline_kernel and cubic_b_spline_kernel_norm")
-    if kernel is not linear_kernel and correction is EnergySurfaceCorrection:
-        raise PdxError(PDX_ERR_UNSUPPORTED,
-                       "the MI355X build provides EnergySurfaceCorrection with linear_kernel only")
     return kernel
 
 
@@ -153,7 +150,7 @@ class OSBMaterial:
 
     def __init__(self, kernel=linear_kernel, dmgmodel=None, correction=None):
         self.correction = _correction(correction)
-        self.kernel = _kernel(kernel, self.correction)
+        self.kernel = _kernel(kernel)
         self.dmgmodel = dmgmodel or CriticalStretch()
 
     def __repr__(self):

@@ -224,3 +224,33 @@ def test_correction_rejected_for_cpd(need_gpu):
     with pytest.raises(pd.PdxError):
         ctx.build_bonds()
     ctx.close()
+
+
+@pytest.mark.parametrize("kernel", ["const_one", "cubic_b_spline", "cubic_b_spline_norm"])
+def test_osb_correction_with_influence_function(need_gpu, oracle, kernel):
+    """OSBMaterial{EnergySurfaceCorrection} with an influence function other than
+    linear_kernel: the strain energies of the factors weight the bonds by kernel(system,
+    bond_id) (ordinary_state_based.jl:225), so mfactor, scfactor and the corrected force
+    depend on it; all three against the oracle."""
+    say("esc osb kernel", kernel)
+    fn = {"const_one": pd.const_one_kernel, "cubic_b_spline": pd.cubic_b_spline_kernel,
+          "cubic_b_spline_norm": pd.cubic_b_spline_kernel_norm}[kernel]
+    pos, vol = pd.lattice_block(14, 12, 10, 1e-3, jitter_amplitude=0.05, seed=7)
+    b = pd.Body(pd.OSBMaterial(kernel=fn, correction=pd.EnergySurfaceCorrection), pos, vol)
+    pd.material(b, horizon=3.015e-3, rho=7850.0, E=210e9, nu=0.25)
+    dh = pd.MI355XBodyDataHandler(b)
+    orc = oracle.OracleSim(b.xyz, b.volume, "OSB", as_oracle_params(b.point_params[0], kernel=kernel),
+                           fail_permit=b.fail_permit.astype(np.uint8), correction="energy")
+    sc, _ = factors_match(dh, orc)
+    # the correction differs from linear_kernel's
+    lin = oracle.OracleSim(b.xyz, b.volume, "OSB", as_oracle_params(b.point_params[0]),
+                           fail_permit=b.fail_permit.astype(np.uint8), correction="energy")
+    assert np.max(np.abs(sc - lin.scf)) > 1e-6
+    x = b.xyz + 1e-6 * np.random.default_rng(4).standard_normal(b.xyz.shape)
+    dh.set("position", x)
+    dh.calc_force_density(0.0, 1e-7)
+    orc.force(x)
+    err = scaled_error(dh.get("b_int"), orc)
+    say(f"  force scaled error {err:.3e}")
+    assert err <= FP64_TOL
+    dh.close()

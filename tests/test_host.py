@@ -177,8 +177,8 @@ def test_library_version_and_null_context():
 def test_influence_functions_of_the_materials():
     """OSBMaterial / CMaterial(kernel=...) take the four influence functions of kernels.jl
     and pass their ids to the library (PDX_KERNEL_* of include/pdx.h); another function is
-    an ArgumentError (the device evaluates these four only), and the surface correction is
-    built for linear_kernel only."""
+    an ArgumentError (the device evaluates these four only); each combines with the surface
+    correction."""
     from peridynamics.materials import KERNEL_IDS
     assert [KERNEL_IDS[k] for k in (pd.linear_kernel, pd.const_one_kernel,
                                     pd.cubic_b_spline_kernel, pd.cubic_b_spline_kernel_norm)] == [0, 1, 2, 3]
@@ -189,9 +189,8 @@ def test_influence_functions_of_the_materials():
         pd.OSBMaterial(kernel=lambda d, L: 1.0)
     with pytest.raises(pd.ArgumentError):
         pd.CMaterial(kernel=lambda d, L: 1.0)
-    with pytest.raises(pd.PdxError):
-        pd.OSBMaterial(kernel=pd.cubic_b_spline_kernel, correction=pd.EnergySurfaceCorrection)
-    assert pd.OSBMaterial(kernel=pd.linear_kernel, correction=pd.EnergySurfaceCorrection)
+    m = pd.OSBMaterial(kernel=pd.cubic_b_spline_kernel, correction=pd.EnergySurfaceCorrection)
+    assert m.kernel is pd.cubic_b_spline_kernel and m.correction is pd.EnergySurfaceCorrection
     # the mirror's scalar functions agree with kernels.jl's test values
     assert pd.cubic_b_spline_kernel(1.0, 0.5) == pytest.approx(1 / 6)
     assert pd.cubic_b_spline_kernel_norm(1.0, 0.75) == pytest.approx(8 / np.pi * 2 * 0.25 ** 3)

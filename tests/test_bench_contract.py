@@ -47,3 +47,16 @@ def test_measured_traffic_lookup(tmp_path, monkeypatch):
     assert bench.measured_traffic("fp64", 11, 20) is None
     monkeypatch.setattr(bench, "TRAFFIC_FILE", str(tmp_path / "missing.md"))
     assert bench.measured_traffic("fp64", 10, 20) is None
+
+
+def test_cpu_baseline_legs_on_small_samples():
+    """The CPU-baseline legs run the oracle (its OpenMP loops) on the workloads of the GPU
+    legs and report a positive rate with the cores used; small samples here."""
+    sys.path.insert(0, os.path.join(ROOT, "peridynamics.jl_amd"))
+    for kind in ("OSB", "CPD"):
+        r = bench.cpu_state_based(kind, (12, 8, 6), target_seconds=0.2, max_steps=5)
+        assert r["value"] > 0 and r["unit"] == "bond updates/s" and r["cores"] >= 1
+        assert r["kind"] == "port" and kind in r["sample"]
+    j = bench.cpu_jfnk((12, 12, 4), target_seconds=0.1)
+    assert j["value"] > 0 and j["unit"] == "Krylov iterations/s" and j["ms_per_iteration"] > 0
+    assert "GMRES(30)" in j["sample"]

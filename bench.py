@@ -141,14 +141,15 @@ def jfnk_block(nx, ny, nz, dx=1e-3, jitter=0.05, seed=0x5EED, top=1e-4):
 
 def jfnk_leg(shape, train_steps):
     """Wall time of the 10 load steps to a 1e-8 relative Newton residual with GMRES(30),
-    200 Krylov iterations per Newton step: block Jacobi ("plain"), the two-level
-    preconditioner, and the learned one (trained on the error samples of load steps 1-3, which it
-    solves with the two-level preconditioner; applied while it passes validation)."""
+    200 Krylov iterations per Newton step: unpreconditioned (the reference's GMRES), block
+    Jacobi ("plain"), the two-level preconditioner, and the learned one (trained on the error
+    samples of load steps 1-3, which it solves with the two-level preconditioner; applied
+    while it passes validation).  Displacements are compared with block Jacobi's."""
     import numpy as np
     import peridynamics as pd
     res = {}
     u_ref = None
-    for pre in ("jacobi", "two_level", "learned"):
+    for pre in ("jacobi", "none", "two_level", "learned"):
         body = jfnk_block(*shape)
         dh = pd.MI355XBodyDataHandler(body)
         nk = pd.NewtonKrylov(steps=10, tol=1e-30, rel_tol=1e-8, maxiter=40, gmres_restart=30,
@@ -160,7 +161,7 @@ def jfnk_leg(shape, train_steps):
         dh.ctx.synchronize()
         wall = time.perf_counter() - t0
         u = dh.get("displacement")
-        if u_ref is None:
+        if pre == "jacobi":
             u_ref = u
         info = getattr(dh, "train_info", None)
         res[pre] = {"seconds": wall,
@@ -181,13 +182,17 @@ def jfnk_leg(shape, train_steps):
         log(f"[bench] jfnk {pre}: {wall:.3f} s, krylov {res[pre]['krylov_iters']}")
         dh.close()
     plain = res["jacobi"]
+    ref = res["none"]
     summary = {"plain_s": plain["seconds"], "two_level_s": res["two_level"]["seconds"],
                "learned_s": res["learned"]["seconds"],
-               "learned_vs_plain": plain["seconds"] / res["learned"]["seconds"]}
+               "learned_vs_plain": plain["seconds"] / res["learned"]["seconds"],
+               "unpreconditioned_s": ref["seconds"],
+               "learned_vs_unpreconditioned": ref["seconds"] / res["learned"]["seconds"]}
     return {"summary": summary, "workload": (f"bond-based block {shape[0]}x{shape[1]}x{shape[2]}, bottom fixed, top "
                          f"displaced 1e-4 m in y over 10 load steps, Newton rel tol 1e-8, GMRES(30)"),
             "points": shape[0] * shape[1] * shape[2], **res,
             "speedup_learned_vs_plain": plain["seconds"] / res["learned"]["seconds"],
+            "speedup_learned_vs_unpreconditioned": ref["seconds"] / res["learned"]["seconds"],
             "speedup_learned_vs_plain_steps_4_10": (plain["seconds_steps_4_10"] /
                                                    res["learned"]["seconds_steps_4_10"]),
             "speedup_learned_vs_two_level_steps_4_10": (res["two_level"]["seconds_steps_4_10"] /

@@ -54,6 +54,58 @@ __device__ __forceinline__ int64_t pdx_bond0(const int64_t *slice_ptr, int64_t p
     return slice_ptr[p >> 6] + (p & 63);
 }
 
+// Bond words and lengths of G bonds of a lane's row: t, t + S, ..., t + (G - 1) S.  Bonds
+// past the row's end read its last bond (bond 0 of an empty row), an in-range address whose
+// value the caller never uses, so the loads need no branch and can be issued a whole group
+// ahead of their use.
+template <int G, int S>
+__device__ __forceinline__ void pdx_load_group(const uint16_t *wbase, const double *Lbase, int t, int c,
+                                               uint32_t (&w)[G], double (&L)[G])
+{
+    const int last = c > 0 ? c - 1 : 0;
+#pragma unroll
+    for (int u = 0; u < G; u++) {
+        const int tt = t + u * S < c ? t + u * S : last;
+        w[u] = wbase[(int64_t)tt * 64];
+        L[u] = Lbase[(int64_t)tt * 64];
+    }
+}
+
+// The bonds of a lane's row whose neighbour slots lie below the staged tile's end T1, in row
+// order from bond t, as groups of G: the words and lengths of the next group are loaded
+// before the current group is evaluated, so their latency hides behind G bond evaluations.
+// Returns the first bond not evaluated (past the tile, or c).  BOND(wd, L, t) evaluates one.
+template <int G, int S, class BOND>
+__device__ __forceinline__ int pdx_walk_tile(const uint16_t *wbase, const double *Lbase, int t, int c,
+                                             int T1, BOND &&bond)
+{
+    if (t >= c) return t;
+    uint32_t gw[G];
+    double gL[G];
+    pdx_load_group<G, S>(wbase, Lbase, t, c, gw, gL);
+    while (true) {
+        uint32_t nw[G];
+        double nL[G];
+        pdx_load_group<G, S>(wbase, Lbase, t + G * S, c, nw, nL);
+        // the leading bonds of the group inside the row and the tile (slots increase along
+        // the row, so every bond after the first one past T1 is past it too)
+        int kk = 0;
+#pragma unroll
+        for (int u = 0; u < G; u++)
+            if (kk == u && t + u * S < c && (int)(gw[u] & PDX_W16_SLOT) < T1) kk = u + 1;
+#pragma unroll
+        for (int u = 0; u < G; u++)
+            if (u < kk) bond(gw[u], gL[u], t + u * S);
+        t += kk * S;
+        if (kk < G) return t;
+#pragma unroll
+        for (int u = 0; u < G; u++) {
+            gw[u] = nw[u];
+            gL[u] = nL[u];
+        }
+    }
+}
+
 // ----------------------------------------------------------------------------------------
 // per-point tail: NaN flag (check_for_nans, nans.jl:10-18), acceleration and velocity
 // (_update_acc_and_vel!), then the first half of the next step (update_vel_half!,
@@ -1902,6 +1954,14 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_pass2(pdx_step_args A)
     point_tail4(A, p, ri, b3, sub, A.rho);
 }
 
+// bonds per group of CPD pass 2's bond walk (pdx_walk_tile): the loads of the next group are
+// in flight while a group is evaluated.  The variants with features (several parameter sets,
+// ZEMWan) hold more per-point state and take groups of 2, which keeps them in registers; pass
+// 1, whose bond arithmetic is longer, runs fastest with groups of 1.
+#ifndef PDX_CPD_GROUP
+#define PDX_CPD_GROUP 4
+#endif
+
 // ----------------------------------------------------------------------------------------
 // CPD passes on the LDS-staged layout (FP64), for neighbour tables longer than one LDS
 // budget: the block's table is staged in tiles of TL entries.  A point's bonds are sorted
@@ -1957,8 +2017,6 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds1(pdx_step_args A)
     double Ft[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     int na = 0;
     int t = sub;
-    uint32_t wd = t < c ? wbase[(int64_t)t * 64] : 0u;
-    double Lb = t < c ? Lbase[(int64_t)t * 64] : 1.0;
     for (int T0 = 0; T0 < H; T0 += TL) {
         const int T1 = H < T0 + TL ? H : T0 + TL;
         if (T0 > 0) __syncthreads();  // the previous tile is consumed
@@ -1969,12 +2027,8 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds1(pdx_step_args A)
             if (MULTI) hpid[h - T0] = A.pid[j];
         }
         __syncthreads();
-        while (t < c) {
+        t = pdx_walk_tile<1, S>(wbase, Lbase, t, c, T1, [&](uint32_t wd, double Lb, int tb) {
             const int slot = (int)(wd & PDX_W16_SLOT);
-            if (slot >= T1) break;
-            const int tn = t + S;
-            const uint32_t wn = tn < c ? wbase[(int64_t)tn * 64] : 0u;
-            const double Ln = tn < c ? Lbase[(int64_t)tn * 64] : 1.0;
             const double4 rj = hx[slot - T0];
             const double4 Xj = hX[slot - T0];
             const double dx[3] = {rj.x - ri.x, rj.y - ri.y, rj.z - ri.z};
@@ -1994,29 +2048,28 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds1(pdx_step_args A)
                     if ((eps > epsc_hi) ? true : ((e / Lb) > epsc)) nw &= ~PDX_W16_ACT;
                 if ((wd & PDX_W16_RACT) && (wd & PDX_W16_FP) && eps >= Pj.epsc_lo)
                     if ((eps > Pj.epsc_hi) ? true : ((e / Lb) > Pj.epsc)) nw &= ~PDX_W16_RACT;
-                if (nw != wd) wbase[(int64_t)t * 64] = (uint16_t)nw;
+                if (nw != wd) wbase[(int64_t)tb * 64] = (uint16_t)nw;
                 act = (nw & PDX_W16_ACT) != 0;
             } else {
-                act = bond_fail16(wbase + (int64_t)t * 64, wd, e, Lb, e * rL, epsc, epsc_lo, epsc_hi);
+                act = bond_fail16(wbase + (int64_t)tb * 64, wd, e, Lb, e * rL, epsc, epsc_lo, epsc_hi);
             }
             if (act && !dead) {
                 na++;
+                // K += w V_j DX DX', Ft += w V_j dx DX' as fused products of (w V_j DX)
                 const double temp = pdx_omega(A.kernel, A.delta, Lb, rL) * rj.w;
-                Km[0] += temp * (DX[0] * DX[0]);
-                Km[1] += temp * (DX[1] * DX[0]);
-                Km[2] += temp * (DX[2] * DX[0]);
-                Km[3] += temp * (DX[1] * DX[1]);
-                Km[4] += temp * (DX[2] * DX[1]);
-                Km[5] += temp * (DX[2] * DX[2]);
+                const double tD[3] = {temp * DX[0], temp * DX[1], temp * DX[2]};
+                Km[0] = fma(tD[0], DX[0], Km[0]);
+                Km[1] = fma(tD[1], DX[0], Km[1]);
+                Km[2] = fma(tD[2], DX[0], Km[2]);
+                Km[3] = fma(tD[1], DX[1], Km[3]);
+                Km[4] = fma(tD[2], DX[1], Km[4]);
+                Km[5] = fma(tD[2], DX[2], Km[5]);
 #pragma unroll
                 for (int cc = 0; cc < 3; cc++)
 #pragma unroll
-                    for (int r = 0; r < 3; r++) Ft[r + 3 * cc] += temp * (dx[r] * DX[cc]);
+                    for (int r = 0; r < 3; r++) Ft[r + 3 * cc] = fma(tD[cc], dx[r], Ft[r + 3 * cc]);
             }
-            wd = wn;
-            Lb = Ln;
-            t = tn;
-        }
+        });
     }
 #pragma unroll
     for (int o = 1; o < S; o <<= 1) {
@@ -2068,13 +2121,13 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2(pdx_step_args A)
         beta = Pi.beta;
         rho = Pi.rho;
     }
+    // a dead point's own terms vanish: its state (zero since pass 1) and its beta are zeroed
     double Qi[NQ];
 #pragma unroll
-    for (int q = 0; q < NQ; q++) Qi[q] = valid ? A.pstate[NQ * p + q] : 0.0;
+    for (int q = 0; q < NQ; q++) Qi[q] = dead_i ? 0.0 : A.pstate[NQ * p + q];
+    const double beta_own = dead_i ? 0.0 : beta;
     double b3[3] = {0.0, 0.0, 0.0};
     int t = sub;
-    uint32_t wd = t < c ? wbase[(int64_t)t * 64] : 0u;
-    double Lb = t < c ? Lbase[(int64_t)t * 64] : 1.0;
     for (int T0 = 0; T0 < H; T0 += TL) {
         const int T1 = H < T0 + TL ? H : T0 + TL;
         if (T0 > 0) __syncthreads();
@@ -2090,12 +2143,8 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2(pdx_step_args A)
             for (int q = 0; q < NQ; q++) qd[q] = live ? A.pstate[NQ * (int64_t)j + q] : 0.0;
         }
         __syncthreads();
-        while (t < c) {
+        t = pdx_walk_tile<F == 0 ? PDX_CPD_GROUP : 2, S>(wbase, Lbase, t, c, T1, [&](uint32_t wd, double Lb, int) {
             const int slot = (int)(wd & PDX_W16_SLOT);
-            if (slot >= T1) break;
-            const int tn = t + S;
-            const uint32_t wn = tn < c ? wbase[(int64_t)tn * 64] : 0u;
-            const double Ln = tn < c ? Lbase[(int64_t)tn * 64] : 1.0;
             const bool act = (wd & PDX_W16_ACT) != 0;
             const bool ract = MULTI ? (wd & PDX_W16_RACT) != 0 : act;
             if (act || ract) {
@@ -2105,31 +2154,27 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2(pdx_step_args A)
                 const double dx[3] = {rj.x - ri.x, rj.y - ri.y, rj.z - ri.z};
                 const double DX[3] = {Xj.x - Xi.x, Xj.y - Xi.y, Xj.z - Xi.z};
                 const double wv = pdx_omega(A.kernel, A.delta, Lb, pdx_rcp1(Lb)) * rj.w;  // w V_j
+                // both ends' terms summed before the products: (Q_i + Q_j) DX + (beta_i + beta_j) dx,
+                // ZEMWan (Q_i + Q_j) DX + (C_1i + C_1j) dx; the own term counts for the owner's
+                // copy of the bond (ACT), the neighbour's for its copy (RACT; the same bit with
+                // one parameter set, and this body runs only when it is set).  A dead neighbour
+                // was staged as zeros, a dead owner's state and beta are zero.
+                const double fo = act ? 1.0 : 0.0, fn = ract ? 1.0 : 0.0;
+                double M[NQ];
+#pragma unroll
+                for (int q = 0; q < NQ; q++) M[q] = MULTI ? fma(fo, Qi[q], fn * qj[q]) : Qi[q] + qj[q];
+                const double bs = ZW ? 0.0 : (MULTI ? fma(fo, beta_own, fn * Xj.w) : beta_own + Xj.w);
 #pragma unroll
                 for (int r = 0; r < 3; r++) {
-                    double tt = 0.0;
-                    if (ZW) {
-                        // Q DX + C_1 dx at each end (a dead neighbour was staged as zeros)
-                        if (!dead_i && act)
-                            tt += ((Qi[r] * DX[0] + Qi[r + 3] * DX[1]) + Qi[r + 6] * DX[2]) +
-                                  ((Qi[9 + SY[r]] * dx[0] + Qi[9 + SY[r + 3]] * dx[1]) +
-                                   Qi[9 + SY[r + 6]] * dx[2]);
-                        if (ract)
-                            tt += ((qj[r] * DX[0] + qj[r + 3] * DX[1]) + qj[r + 6] * DX[2]) +
-                                  ((qj[9 + SY[r]] * dx[0] + qj[9 + SY[r + 3]] * dx[1]) +
-                                   qj[9 + SY[r + 6]] * dx[2]);
-                    } else {
-                        if (!dead_i && act) tt += ((Qi[r] * DX[0] + Qi[r + 3] * DX[1]) + Qi[r + 6] * DX[2]) + beta * dx[r];
-                        // a dead neighbour was staged with Q_j = 0 and beta = 0: its term adds 0
-                        if (ract) tt += ((qj[r] * DX[0] + qj[r + 3] * DX[1]) + qj[r + 6] * DX[2]) + Xj.w * dx[r];
-                    }
-                    b3[r] += wv * tt;
+                    double tt = fma(M[r + 6], DX[2], fma(M[r + 3], DX[1], M[r] * DX[0]));
+                    if (ZW)
+                        tt = fma(M[9 + SY[r + 6]], dx[2], fma(M[9 + SY[r + 3]], dx[1], fma(M[9 + SY[r]], dx[0], tt)));
+                    else
+                        tt = fma(bs, dx[r], tt);
+                    b3[r] = fma(wv, tt, b3[r]);
                 }
             }
-            wd = wn;
-            Lb = Ln;
-            t = tn;
-        }
+        });
     }
 #pragma unroll
     for (int o = 1; o < S; o <<= 1)

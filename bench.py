@@ -139,16 +139,32 @@ def jfnk_block(nx, ny, nz, dx=1e-3, jitter=0.05, seed=0x5EED, top=1e-4):
     return body
 
 
+def jfnk_warmup(precs=("jacobi", "none", "two_level", "learned")):
+    """Two load steps of a 16x16x8 block with each preconditioner, untimed: the first solve of
+    a process pays one-time costs (code objects, rocSOLVER handles, first allocations) that
+    no timed solve should carry."""
+    import peridynamics as pd
+    for pre in precs:
+        dh = pd.MI355XBodyDataHandler(jfnk_block(16, 16, 8))
+        nk = pd.NewtonKrylov(steps=2, tol=1e-30, rel_tol=1e-8, maxiter=40, gmres_restart=30,
+                             gmres_maxiter=200, precond=pre, learn_steps=1, train_steps=2)
+        dh.solve_nk(nk)
+        dh.close()
+
+
 def jfnk_leg(shape, train_steps):
     """Wall time of the 10 load steps to a 1e-8 relative Newton residual with GMRES(30),
     200 Krylov iterations per Newton step: unpreconditioned (the reference's GMRES), block
     Jacobi ("plain"), the two-level preconditioner, and the learned one (trained on the error
     samples of load steps 1-3, which it solves with the two-level preconditioner; applied
-    while it passes validation).  Displacements are compared with block Jacobi's."""
+    while it passes validation).  Displacements are compared with block Jacobi's.  A small
+    untimed solve with each preconditioner comes first, so that no timed run carries the
+    process's one-time start-up costs."""
     import numpy as np
     import peridynamics as pd
     res = {}
     u_ref = None
+    jfnk_warmup()
     for pre in ("jacobi", "none", "two_level", "learned"):
         body = jfnk_block(*shape)
         dh = pd.MI355XBodyDataHandler(body)
@@ -222,6 +238,30 @@ def jfnk_profile_leg(shape, steps):
             log(f"[bench] jfnk profile {pre} step {n}: {rows[-1]['ms_per_krylov_iteration']:.4f} ms/it")
         out[pre] = rows
         dh.close()
+    return out
+
+
+def jfnk_learned_leg(shape, train_steps):
+    """The 10 load steps of the quasi-static block with the learned preconditioner alone, after
+    a small untimed warm-up solve (the workload of the training and learned-solve kernel
+    trace): wall time, Krylov iterations and the training figures."""
+    import peridynamics as pd
+    jfnk_warmup(("learned",))
+    dh = pd.MI355XBodyDataHandler(jfnk_block(*shape))
+    nk = pd.NewtonKrylov(steps=10, tol=1e-30, rel_tol=1e-8, maxiter=40, gmres_restart=30,
+                         gmres_maxiter=200, precond="learned", learn_steps=3, train_steps=train_steps)
+    dh.ctx.synchronize()
+    t0 = time.perf_counter()
+    stats = dh.solve_nk(nk)
+    dh.ctx.synchronize()
+    out = {"seconds": time.perf_counter() - t0,
+           "krylov_iters": sum(s.krylov_iters for s in stats),
+           "krylov_iters_steps_4_10": sum(s.krylov_iters for s in stats[3:]),
+           "train_seconds": dh.train_info["seconds"], "train_steps": train_steps,
+           "validation_loss": dh.train_info["val_loss"]}
+    dh.close()
+    log(f"[bench] jfnk learned: {out['seconds']:.3f} s, krylov {out['krylov_iters']}, "
+        f"training {out['train_seconds']:.3f} s")
     return out
 
 
@@ -689,9 +729,16 @@ def main():
     ap.add_argument("--jfnk-profile", type=int, default=0, metavar="STEPS",
                     help="only run the first STEPS load steps of the JFNK block with block "
                          "Jacobi and two-level (the Krylov-iteration profile workload)")
+    ap.add_argument("--jfnk-learned", action="store_true",
+                    help="only run the JFNK block with the learned preconditioner (the training "
+                         "and learned-solve profile workload)")
     args = ap.parse_args()
     if args.copy_calibration:
         print(json.dumps({"copy_calibration": copy_calibration()}), flush=True)
+        return
+    if args.jfnk_learned:
+        shape = tuple(int(v) for v in args.jfnk_shape.split(","))
+        print(json.dumps({"jfnk_learned": jfnk_learned_leg(shape, args.jfnk_train_steps)}), flush=True)
         return
     if args.jfnk_profile:
         shape = tuple(int(v) for v in args.jfnk_shape.split(","))

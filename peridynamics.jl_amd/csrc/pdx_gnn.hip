@@ -127,6 +127,8 @@ __device__ __forceinline__ bf16x8 load_bf8(const uint16_t *p)
 }
 
 static inline dim3 g1(int64_t m, int bs = 256) { return dim3((unsigned)((m + bs - 1) / bs)); }
+// workgroups of k_gnn_reduce_add for len outputs (16 per workgroup)
+static inline dim3 g16(int64_t len) { return dim3((unsigned)((len + 15) / 16)); }
 
 // ----------------------------------------------------------------------------------------
 // graph
@@ -339,20 +341,34 @@ __global__ __launch_bounds__(256) void k_gnn_gemm_fwd(int64_t n, const uint16_t 
 // loss: per aggregate, the ridge least-squares fit of every error sample by the coarse
 // fields, and the gradient with respect to the fields.  One workgroup per aggregate; the
 // aggregate's basis B (rows = its 3 |I| dofs, columns = 3 translations + GKF fields, zero on
-// constrained dofs) and the gradient accumulate in LDS.
+// constrained dofs) lives in LDS, the Gram matrix is factored once, and the samples are fitted
+// in chunks of G_LSQ_CHUNK:
+//   1. B^T e of every sample of the chunk: 16 threads per sample, each over every 16th row,
+//      folded with lane shuffles;
+//   2. the coefficients c = (B^T B + lambda I)^-1 B^T e, one thread per sample (Cholesky
+//      substitution in FP64);
+//   3. the residuals e - B c: each thread over its own rows and all samples of the chunk,
+//      accumulating the gradient of its rows in registers.
+// The ridge term's gradient, 2 ridge / (3 + K) |c|^2 B per sample, is applied once at the end
+// with the chunk sums of w |c|^2.
 // ----------------------------------------------------------------------------------------
+#define G_LSQ_CHUNK 16
+#define G_LSQ_ROWS ((3 * G_AGG_MAX + 255) / 256)  // rows per thread in the residual pass
 __global__ __launch_bounds__(256) void k_gnn_lsq(int64_t n, int agg, const float *O,
                                                  const uint8_t *cmask, const float *E,
                                                  const float *einv, const int32_t *sidx,
                                                  int ns, int64_t m, int use_fields, float wscale,
                                                  float *dO, double *lpart)
 {
+    static_assert(G_LSQ_CHUNK * 16 == 256, "16 threads per sample of a chunk");
     __shared__ float B[3 * G_AGG_MAX][GNB];
-    __shared__ float dB[3 * G_AGG_MAX][GKF];
     __shared__ double G[GNB][GNB], Lc[GNB][GNB];
-    __shared__ float red[GNB][256];
-    __shared__ double bv[GNB], cv[GNB];
+    __shared__ float red[G_LSQ_CHUNK][GNB];
+    __shared__ double cv[G_LSQ_CHUNK][GNB];
+    __shared__ float wq[G_LSQ_CHUNK];
+    __shared__ int64_t srow[G_LSQ_CHUNK];
     __shared__ double lsum[256];
+    __shared__ double lam_sh, c2w_sh;
     const int64_t I = blockIdx.x;
     const int64_t p0 = I * agg, p1 = min(n, p0 + agg);
     const int nr = (int)(3 * (p1 - p0));
@@ -362,10 +378,7 @@ __global__ __launch_bounds__(256) void k_gnn_lsq(int64_t n, int agg, const float
         const int d = r % 3;
         const bool fr = !cmask[3 * p + d];
         for (int f = 0; f < 3; f++) B[r][f] = (fr && f == d) ? 1.f : 0.f;
-        for (int f = 0; f < GKF; f++) {
-            B[r][3 + f] = (fr && use_fields) ? O[p * GNO + 3 * f + d] : 0.f;
-            dB[r][f] = 0.f;
-        }
+        for (int f = 0; f < GKF; f++) B[r][3 + f] = (fr && use_fields) ? O[p * GNO + 3 * f + d] : 0.f;
     }
     __syncthreads();
     if (t < GNB * GNB) {
@@ -386,70 +399,99 @@ __global__ __launch_bounds__(256) void k_gnn_lsq(int64_t n, int agg, const float
                 for (int k = 0; k < b; k++) s -= Lc[a][k] * Lc[b][k];
                 Lc[a][b] = (a == b) ? sqrt(fmax(s, 1e-300)) : s / Lc[b][b];
             }
-        G[0][0] = lam;  // G is no longer needed: keep lambda
+        lam_sh = lam;
+        c2w_sh = 0.0;
     }
     __syncthreads();
-    const double lam = G[0][0];
+    const double lam = lam_sh;
+    float dBr[G_LSQ_ROWS][GKF];
+#pragma unroll
+    for (int u = 0; u < G_LSQ_ROWS; u++)
+#pragma unroll
+        for (int f = 0; f < GKF; f++) dBr[u][f] = 0.f;
     double lacc = 0.0;
-    for (int q = 0; q < ns; q++) {
-        const int64_t sI = sidx ? sidx[q] : q;
-        const float *e = E + sI * m + 3 * p0;
-        float ev[(3 * G_AGG_MAX + 255) / 256];
-        float part[GNB];
-        for (int a = 0; a < GNB; a++) part[a] = 0.f;
-        int u = 0;
-        for (int r = t; r < nr; r += 256, u++) {
-            ev[u] = e[r];
-            for (int a = 0; a < GNB; a++) part[a] += B[r][a] * ev[u];
+    const int qs = t >> 4, tg = t & 15;  // phase 1: sample of the chunk, lane in its group
+    for (int q0 = 0; q0 < ns; q0 += G_LSQ_CHUNK) {
+        const int nq = min(G_LSQ_CHUNK, ns - q0);
+        // 1. B^T e
+        if (qs < nq) {
+            const int64_t sI = sidx ? sidx[q0 + qs] : q0 + qs;
+            const float *e = E + sI * m + 3 * p0;
+            float part[GNB];
+#pragma unroll
+            for (int a = 0; a < GNB; a++) part[a] = 0.f;
+            for (int r = tg; r < nr; r += 16) {
+                const float ev = e[r];
+#pragma unroll
+                for (int a = 0; a < GNB; a++) part[a] += B[r][a] * ev;
+            }
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1)
+#pragma unroll
+                for (int a = 0; a < GNB; a++) part[a] += __shfl_xor(part[a], o);
+            if (tg == 0) {
+#pragma unroll
+                for (int a = 0; a < GNB; a++) red[qs][a] = part[a];
+                srow[qs] = sI * m + 3 * p0;
+                wq[qs] = einv[sI] * wscale;
+            }
         }
-        for (int a = 0; a < GNB; a++) red[a][t] = part[a];
         __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
-            if (t < o)
-                for (int a = 0; a < GNB; a++) red[a][t] += red[a][t + o];
-            __syncthreads();
-        }
-        if (t == 0) {
-            // c = (G + lam I)^-1 B^T e by the Cholesky factor
-            double y[GNB];
+        // 2. c = (G + lam I)^-1 B^T e by the Cholesky factor, one thread per sample
+        if (t < nq) {
+            double y[GNB], c[GNB];
             for (int a = 0; a < GNB; a++) {
-                double s = red[a][0];
+                double s = red[t][a];
                 for (int k = 0; k < a; k++) s -= Lc[a][k] * y[k];
                 y[a] = s / Lc[a][a];
             }
             for (int a = GNB - 1; a >= 0; a--) {
                 double s = y[a];
-                for (int k = a + 1; k < GNB; k++) s -= Lc[k][a] * cv[k];
-                cv[a] = s / Lc[a][a];
+                for (int k = a + 1; k < GNB; k++) s -= Lc[k][a] * c[k];
+                c[a] = s / Lc[a][a];
             }
             double c2 = 0.0;
-            for (int a = 0; a < GNB; a++) c2 += cv[a] * cv[a];
-            bv[0] = c2;
+            for (int a = 0; a < GNB; a++) {
+                cv[t][a] = c[a];
+                c2 += c[a] * c[a];
+            }
+            // loss: lambda |c|^2 of the sample; the ridge gradient's weight sum
+            lacc += (double)wq[t] * lam * c2;
+            atomicAdd(&c2w_sh, (double)wq[t] * c2);
         }
         __syncthreads();
-        const float w = einv[sI] * wscale;
-        double r2 = 0.0;
-        u = 0;
-        for (int r = t; r < nr; r += 256, u++) {
-            double bc = 0.0;
-            for (int a = 0; a < GNB; a++) bc += (double)B[r][a] * cv[a];
-            const float res = (float)((double)ev[u] - bc);
-            r2 += (double)res * res;
-            for (int f = 0; f < GKF; f++) dB[r][f] += -2.f * w * res * (float)cv[3 + f];
+        // 3. residuals of the thread's rows (each row of B read once) for every sample of the
+        // chunk
+#pragma unroll
+        for (int u = 0; u < G_LSQ_ROWS; u++) {
+            const int r = t + 256 * u;
+            if (r >= nr) break;
+            float br[GNB];
+#pragma unroll
+            for (int a = 0; a < GNB; a++) br[a] = B[r][a];
+            for (int q = 0; q < nq; q++) {
+                double bc = 0.0;
+#pragma unroll
+                for (int a = 0; a < GNB; a++) bc += (double)br[a] * cv[q][a];
+                const float res = (float)((double)E[srow[q] + r] - bc);
+                const float w = wq[q];
+                lacc += (double)w * ((double)res * res);
+#pragma unroll
+                for (int f = 0; f < GKF; f++) dBr[u][f] += -2.f * w * res * (float)cv[q][3 + f];
+            }
         }
-        // lambda = ridge tr(B^T B) / (3 + K) depends on the fields too: d lambda / dB = 2 ridge B / (3 + K)
-        u = 0;
-        for (int r = t; r < nr; r += 256, u++)
-            for (int f = 0; f < GKF; f++)
-                dB[r][f] += w * (float)(2.0 * G_RIDGE / GNB * bv[0]) * B[r][3 + f];
-        lacc += (double)w * r2;
-        if (t == 0) lacc += (double)w * lam * bv[0];
-        __syncthreads();
+        __syncthreads();  // red, cv, wq and srow are rewritten by the next chunk
     }
-    for (int r = t; r < nr; r += 256) {
+    // lambda = ridge tr(B^T B) / (3 + K) depends on the fields too: d lambda / dB = 2 ridge B / (3 + K)
+    const float rg = (float)(2.0 * G_RIDGE / GNB * c2w_sh);
+#pragma unroll
+    for (int u = 0; u < G_LSQ_ROWS; u++) {
+        const int r = t + 256 * u;
+        if (r >= nr) break;
         const int64_t p = p0 + r / 3;
         const int d = r % 3;
-        for (int f = 0; f < GKF; f++) dO[p * GNO + 3 * f + d] = dB[r][f];
+#pragma unroll
+        for (int f = 0; f < GKF; f++) dO[p * GNO + 3 * f + d] = dBr[u][f] + rg * B[r][3 + f];
     }
     lsum[t] = lacc;
     __syncthreads();
@@ -492,36 +534,54 @@ __global__ __launch_bounds__(256) void k_gnn_decode_bwd(int64_t n, int64_t npad,
 }
 
 // partial outer products sum_p A_p B_p^T and sum_p B_p over chunks of points: output
-// [(ka + 1) x kb] per workgroup (row ka = bias gradient)
+// [(ka + 1) x kb] per workgroup (row ka = bias gradient).  The chunk's rows of A and B are
+// staged in LDS (dynamic: chunk (ka + kb) floats) and every output sums them from there.
 template <typename TA>
 __global__ __launch_bounds__(256) void k_gnn_outer(int64_t npad, int64_t chunk, const TA *A, int ka,
                                                    const float *B, int kb, float *part)
 {
-    const int64_t p0 = (int64_t)blockIdx.x * chunk, p1 = min(npad, p0 + chunk);
+    extern __shared__ float osm[];
+    const int64_t p0 = (int64_t)blockIdx.x * chunk;
+    const int np_ = (int)(min(npad, p0 + chunk) - p0);
+    float *const As = osm;
+    float *const Bs = osm + chunk * ka;
+    for (int i = threadIdx.x; i < np_ * ka; i += 256) {
+        if constexpr (sizeof(TA) == 2) As[i] = bf2f(A[p0 * ka + i]);
+        else As[i] = A[p0 * ka + i];
+    }
+    for (int i = threadIdx.x; i < np_ * kb; i += 256) Bs[i] = B[p0 * kb + i];
+    __syncthreads();
     const int tot = (ka + 1) * kb;
     for (int e = threadIdx.x; e < tot; e += 256) {
         const int a = e / kb, b = e - a * kb;
         float s = 0.f;
-        for (int64_t p = p0; p < p1; p++) {
-            float av = 1.f;
-            if (a < ka) {
-                if constexpr (sizeof(TA) == 2) av = bf2f(A[p * ka + a]);
-                else av = A[p * ka + a];
-            }
-            s += av * B[p * kb + b];
-        }
+        if (a < ka)
+            for (int p = 0; p < np_; p++) s += As[p * ka + a] * Bs[p * kb + b];
+        else
+            for (int p = 0; p < np_; p++) s += Bs[p * kb + b];
         part[(int64_t)blockIdx.x * tot + e] = s;
     }
 }
 
-// out[e] += sum_w part[w * len + e]
-__global__ void k_gnn_reduce_add(int64_t nw, int64_t len, const float *part, float *out)
+// out[e] += sum_w part[w * len + e]: a workgroup sums 16 outputs, each over 16 threads that
+// take every 16th partial; the 16 sums are added in a fixed order (the result does not depend
+// on scheduling)
+__global__ __launch_bounds__(256) void k_gnn_reduce_add(int64_t nw, int64_t len, const float *part, float *out)
 {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= len) return;
+    __shared__ float sh[16][17];
+    const int el = threadIdx.x & 15, ws = threadIdx.x >> 4;
+    const int64_t e = (int64_t)blockIdx.x * 16 + el;
     float s = 0.f;
-    for (int64_t w = 0; w < nw; w++) s += part[w * len + e];
-    out[e] += s;
+    if (e < len)
+        for (int64_t w = ws; w < nw; w += 16) s += part[w * len + e];
+    sh[ws][el] = s;
+    __syncthreads();
+    if (threadIdx.x < 16 && e < len) {
+        float t = 0.f;
+#pragma unroll
+        for (int k = 0; k < 16; k++) t += sh[k][el];
+        out[e] += t;
+    }
 }
 
 // dZ = dH * [Z > 0] (bf16, and float when dZf is given); bias partials per 256 points.
@@ -969,20 +1029,20 @@ static pdx_status gnn_backward(pdx_ctx *c)
     const int64_t nch = np / 256;  // 256-point chunks
     hipLaunchKernelGGL(k_gnn_decode_bwd, dim3((unsigned)(np / 4)), dim3(256), 0, s, n, np, g->theta, g->dO, g->dH);
     // W_d, b_d
-    hipLaunchKernelGGL(k_gnn_outer<uint16_t>, dim3((unsigned)nch), dim3(256), 0, s, np, (int64_t)256,
-                       g->H[GL], GF, g->dO, GNO, g->wpart);
-    hipLaunchKernelGGL(k_gnn_reduce_add, g1((GF + 1) * GNO), dim3(256), 0, s, nch, (int64_t)(GF + 1) * GNO,
+    hipLaunchKernelGGL(k_gnn_outer<uint16_t>, dim3((unsigned)nch), dim3(256), 256 * (GF + GNO) * sizeof(float),
+                       s, np, (int64_t)256, g->H[GL], GF, g->dO, GNO, g->wpart);
+    hipLaunchKernelGGL(k_gnn_reduce_add, g16((GF + 1) * GNO), dim3(256), 0, s, nch, (int64_t)(GF + 1) * GNO,
                        g->wpart, g->grad + OFF_WD);
     PDX_HIP(c, hipGetLastError());
     float *dHo = g->dH, *dHi = g->dH2;
     for (int l = GL - 1; l >= 0; l--) {
         hipLaunchKernelGGL(k_gnn_relu_bwd, dim3((unsigned)nch), dim3(256), 0, s, n, dHo, g->Z[l], g->dZ,
                            nullptr, g->wpart);
-        hipLaunchKernelGGL(k_gnn_reduce_add, dim3(1), dim3(GF), 0, s, nch, (int64_t)GF, g->wpart,
+        hipLaunchKernelGGL(k_gnn_reduce_add, g16(GF), dim3(256), 0, s, nch, (int64_t)GF, g->wpart,
                            g->grad + OFF_B(l));
         hipLaunchKernelGGL(k_gnn_gemm_dw, dim3((unsigned)(np / GDW_CHUNK)), dim3(256), 0, s, g->X[l],
                            g->dZ, g->wpart);
-        hipLaunchKernelGGL(k_gnn_reduce_add, g1(GK * GF), dim3(256), 0, s, np / GDW_CHUNK,
+        hipLaunchKernelGGL(k_gnn_reduce_add, g16(GK * GF), dim3(256), 0, s, np / GDW_CHUNK,
                            (int64_t)GK * GF, g->wpart, g->grad + OFF_W(l));
         hipLaunchKernelGGL(k_gnn_gemm_dx, dim3((unsigned)(np / 64)), dim3(256), 0, s, g->dZ, g->wN[l],
                            g->dX);
@@ -999,9 +1059,9 @@ static pdx_status gnn_backward(pdx_ctx *c)
     // encoder
     hipLaunchKernelGGL(k_gnn_relu_bwd, dim3((unsigned)nch), dim3(256), 0, s, n, dHo, g->Z0, nullptr,
                        g->dZf, nullptr);
-    hipLaunchKernelGGL(k_gnn_outer<float>, dim3((unsigned)nch), dim3(256), 0, s, np, (int64_t)256, g->F,
-                       GFIN, g->dZf, GF, g->wpart);
-    hipLaunchKernelGGL(k_gnn_reduce_add, g1((GFIN + 1) * GF), dim3(256), 0, s, nch,
+    hipLaunchKernelGGL(k_gnn_outer<float>, dim3((unsigned)nch), dim3(256), 256 * (GFIN + GF) * sizeof(float),
+                       s, np, (int64_t)256, g->F, GFIN, g->dZf, GF, g->wpart);
+    hipLaunchKernelGGL(k_gnn_reduce_add, g16((GFIN + 1) * GF), dim3(256), 0, s, nch,
                        (int64_t)(GFIN + 1) * GF, g->wpart, g->grad + OFF_WE);
     PDX_HIP(c, hipGetLastError());
     return PDX_OK;

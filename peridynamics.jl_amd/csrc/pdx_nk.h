@@ -38,6 +38,7 @@ struct pdx_nk {
     int32_t *pJ = nullptr;       // n x NK_TP aggregates reached by each point's bonds
     int *dflag = nullptr;        // overflow flag and the rocSOLVER infos
     double *phi = nullptr;       // n x nf x 3 basis fields, orthonormal per aggregate
+    double *phis = nullptr;      // the same by 64-point chunk, field, component and lane
     double *Ycs = nullptr;       // n x NK_TP x nf x 3 per-point stiffness products
     double *Ac = nullptr;        // mc x mc Galerkin operator, then its Cholesky inverse
     float *Ainv = nullptr;       // mc x mc inverse of the Galerkin operator (float32)

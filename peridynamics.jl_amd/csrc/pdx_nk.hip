@@ -41,7 +41,7 @@ void pdx_nk_free(pdx_ctx *c)
                     (void *)k->recp, (void *)k->recm, (void *)k->fp, (void *)k->fm,
                     (void *)k->red, (void *)k->dinv, (void *)k->h_dev, (void *)k->dsc,
                     (void *)k->Ac, (void *)k->rc, (void *)k->zc, (void *)k->Ainv, (void *)k->phi,
-                    (void *)k->Ycs, (void *)k->pJ, (void *)k->aslot, (void *)k->dflag,
+                    (void *)k->phis, (void *)k->Ycs, (void *)k->pJ, (void *)k->aslot, (void *)k->dflag,
                     (void *)k->vparts, (void *)k->zparts})
         if (p) (void)hipFree(p);
     if (k->rb) rocblas_destroy_handle(k->rb);
@@ -866,42 +866,92 @@ __global__ __launch_bounds__(256) void k_coarse_orth(int64_t n, int agg, int nf,
     }
 }
 
-// rc[I*nf + f] = sum_{i in I} phi_i[f] . v_i; one workgroup per aggregate
-__global__ __launch_bounds__(256) void k_coarse_restrict(int64_t n, int agg, int nf, const double *phi,
-                                                         const double *v, double *rc)
+// phis[((c nf + f) 3 + d) 64 + l] = component d of field f of point 64 c + l (zero past n): the
+// basis by 64-point chunk, field, component and lane, the layout in which the per-iteration
+// kernels read it with consecutive lanes on consecutive addresses
+__global__ __launch_bounds__(256) void k_coarse_soa(int64_t n, int nf, const double *phi, double *phis)
 {
-    __shared__ double sh[NK_MAXF][256];
-    const int64_t I = blockIdx.x;
-    const int64_t p0 = I * agg, p1 = min(n, p0 + agg);
-    double s[NK_MAXF];
-    for (int f = 0; f < NK_MAXF; f++) s[f] = 0.0;
-    for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
-        const double v0 = v[3 * p], v1 = v[3 * p + 1], v2 = v[3 * p + 2];
-        for (int f = 0; f < nf; f++) {
-            const double *a = phi + (p * nf + f) * 3;
-            s[f] += (a[0] * v0 + a[1] * v1) + a[2] * v2;
-        }
-    }
-    for (int f = 0; f < nf; f++) sh[f][threadIdx.x] = s[f];
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o)
-            for (int f = 0; f < nf; f++) sh[f][threadIdx.x] += sh[f][threadIdx.x + o];
-        __syncthreads();
-    }
-    if ((int)threadIdx.x < nf) rc[I * nf + threadIdx.x] = sh[threadIdx.x][0];
+    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= (n + 63) / 64 * 64) return;
+    const int64_t cc = p >> 6;
+    const int l = (int)(p & 63);
+    for (int f = 0; f < nf; f++)
+        for (int d = 0; d < 3; d++)
+            phis[((cc * nf + f) * 3 + d) * 64 + l] = p < n ? phi[(p * nf + f) * 3 + d] : 0.0;
 }
 
-// zc = Ainv rc: one wavefront per row, coalesced along the row, shuffle reduction
-__global__ __launch_bounds__(64) void k_coarse_matvec(int64_t mc, const float *Ainv, const double *rc,
+// rc[I*nf + f] = sum_{i in I} phi_i[f] . v_i; one workgroup per aggregate (a whole number of
+// 64-point chunks: the aggregate size is a power of two >= 64) with a wavefront per chunk (at
+// most 16, which then take every 16th chunk) and a lane per point, then a lane-shuffle and a
+// sum over the wavefronts in a fixed order
+__global__ __launch_bounds__(1024) void k_coarse_restrict(int64_t n, int agg, int nf, const double *phis,
+                                                          const double *v, double *rc)
+{
+    __shared__ double sh[16][NK_MAXF];
+    const int64_t I = blockIdx.x;
+    const int nw = (int)(blockDim.x >> 6);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int64_t nch = (n + 63) / 64;
+    const int64_t c0 = I * (agg / 64), c1 = min(nch, (I + 1) * (agg / 64));
+    double s[NK_MAXF];
+#pragma unroll
+    for (int f = 0; f < NK_MAXF; f++) s[f] = 0.0;
+    for (int64_t cc = c0 + w; cc < c1; cc += nw) {
+        const int64_t p = cc * 64 + l;
+        double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+        if (p < n) {
+            v0 = v[3 * p];
+            v1 = v[3 * p + 1];
+            v2 = v[3 * p + 2];
+        }
+        const double *b = phis + cc * nf * 3 * 64 + l;
+#pragma unroll
+        for (int f = 0; f < NK_MAXF; f++)
+            if (f < nf) s[f] += (b[(3 * f) * 64] * v0 + b[(3 * f + 1) * 64] * v1) + b[(3 * f + 2) * 64] * v2;
+    }
+#pragma unroll
+    for (int f = 0; f < NK_MAXF; f++) {
+        if (f < nf) {
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) s[f] += __shfl_xor(s[f], o);
+            if (l == 0) sh[w][f] = s[f];
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nf) {
+        double t = 0.0;
+        for (int q = 0; q < nw; q++) t += sh[q][threadIdx.x];
+        rc[I * nf + threadIdx.x] = t;
+    }
+}
+
+// wavefronts of k_coarse_restrict: one per 64-point chunk of an aggregate, at most 16
+static inline dim3 restrict_block(int agg) { return dim3((unsigned)(64 * std::min(agg / 64, 16))); }
+
+// zc = Ainv rc: four rows per workgroup (a wavefront each, four independent partial sums per
+// lane), rc staged in LDS (dynamic: mc doubles)
+__global__ __launch_bounds__(256) void k_coarse_matvec(int64_t mc, const float *Ainv, const double *rc,
                                                       double *zc)
 {
-    const int64_t r = blockIdx.x;
+    extern __shared__ double rcs[];
+    for (int64_t k = threadIdx.x; k < mc; k += 256) rcs[k] = rc[k];
+    __syncthreads();
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= mc) return;  // after the only barrier
+    const int l = threadIdx.x & 63;
     const float *row = Ainv + r * mc;
-    double s = 0.0;
-    for (int64_t k = threadIdx.x; k < mc; k += 64) s += (double)row[k] * rc[k];
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int64_t k = l;
+    for (; k + 192 < mc; k += 256) {
+        s0 += (double)row[k] * rcs[k];
+        s1 += (double)row[k + 64] * rcs[k + 64];
+        s2 += (double)row[k + 128] * rcs[k + 128];
+        s3 += (double)row[k + 192] * rcs[k + 192];
+    }
+    for (; k < mc; k += 64) s0 += (double)row[k] * rcs[k];
+    double s = (s0 + s1) + (s2 + s3);
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if (threadIdx.x == 0) zc[r] = s;
+    if (l == 0) zc[r] = s;
 }
 
 // z = a D^-1 v + b P zc  (a = 0: the prolongation alone, added to z when b is given and
@@ -918,7 +968,7 @@ __device__ __forceinline__ void nk_block_part(double x, double *parts)
 
 __global__ __launch_bounds__(256) void k_apply_two_level(int64_t n, int agg, int nf, const double *dinv,
                                                          const double *v, const double *zc,
-                                                         const double *phi, double alpha, int add,
+                                                         const double *phis, double alpha, int add,
                                                          double *z, double *parts)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -937,12 +987,12 @@ __global__ __launch_bounds__(256) void k_apply_two_level(int64_t n, int agg, int
             o[2] = R[2] * a + R[5] * b + R[8] * cc;
         }
         const int64_t I = i / agg;
+        const double *b = phis + (i >> 6) * nf * 3 * 64 + (i & 63);
         for (int f = 0; f < nf; f++) {
             const double w = alpha * zc[I * nf + f];
-            const double *p = phi + (i * nf + f) * 3;
-            o[0] += w * p[0];
-            o[1] += w * p[1];
-            o[2] += w * p[2];
+            o[0] += w * b[(3 * f) * 64];
+            o[1] += w * b[(3 * f + 1) * 64];
+            o[2] += w * b[(3 * f + 2) * 64];
         }
         for (int d = 0; d < 3; d++) z[3 * i + d] = o[d];
         zz = (o[0] * o[0] + o[1] * o[1]) + o[2] * o[2];
@@ -1001,13 +1051,14 @@ static pdx_status build_coarse(pdx_ctx *c, const float *learned_fields, int n_le
             return pdx_fail(c, PDX_ERR_HIP, "rocblas_create_handle failed");
     }
     if (nf > k->nf_cap) {
-        for (void *p : {(void *)k->phi, (void *)k->Ycs, (void *)k->Ac, (void *)k->Ainv, (void *)k->rc,
-                        (void *)k->zc})
+        for (void *p : {(void *)k->phi, (void *)k->phis, (void *)k->Ycs, (void *)k->Ac, (void *)k->Ainv,
+                        (void *)k->rc, (void *)k->zc})
             if (p) (void)hipFree(p);
-        k->phi = k->Ycs = k->Ac = k->rc = k->zc = nullptr;
+        k->phi = k->phis = k->Ycs = k->Ac = k->rc = k->zc = nullptr;
         k->Ainv = nullptr;
         const int64_t mc = k->nagg * nf;
         PDX_TRY(pdx_alloc(c, &k->phi, (size_t)n * nf * 3));
+        PDX_TRY(pdx_alloc(c, &k->phis, (size_t)((n + 63) / 64) * 64 * nf * 3));
         PDX_TRY(pdx_alloc(c, &k->Ycs, (size_t)n * NK_TP * nf * 3));
         PDX_TRY(pdx_alloc(c, &k->Ac, (size_t)(mc * mc)));
         PDX_TRY(pdx_alloc(c, &k->Ainv, (size_t)(mc * mc)));
@@ -1020,6 +1071,7 @@ static pdx_status build_coarse(pdx_ctx *c, const float *learned_fields, int n_le
     const int64_t mc = k->mc, mm = mc * mc;
     hipLaunchKernelGGL(k_coarse_basis, grid1(n), dim3(256), 0, s, n, nf - 3, learned_fields, k->cmask, k->phi);
     hipLaunchKernelGGL(k_coarse_orth, dim3((unsigned)k->nagg), dim3(256), 0, s, n, k->agg, nf, k->phi);
+    hipLaunchKernelGGL(k_coarse_soa, grid1((n + 63) / 64 * 64), dim3(256), 0, s, n, nf, k->phi, k->phis);
     PDX_HIP(c, hipMemsetAsync(k->Ycs, 0, sizeof(double) * n * NK_TP * nf * 3, s));
     PDX_HIP(c, hipMemsetAsync(k->dflag, 0, sizeof(int) * 4, s));
     hipLaunchKernelGGL(k_coarse_y, grid1(n), dim3(256), 0, s, n, k->agg, nf, c->slice_ptr, c->cnt, c->bw,
@@ -1085,11 +1137,12 @@ pdx_status pdx_nk_two_level(pdx_ctx *c, const double *v, double *z, double *part
 {
     pdx_nk *k = c->nk;
     if (!k->coarse_ready) return pdx_fail(c, PDX_ERR_STATE, "coarse space not built");
-    hipLaunchKernelGGL(k_coarse_restrict, dim3((unsigned)k->nagg), dim3(256), 0, c->stream, c->n, k->agg,
-                       k->nf, k->phi, v, k->rc);
-    hipLaunchKernelGGL(k_coarse_matvec, dim3((unsigned)k->mc), dim3(64), 0, c->stream, k->mc, k->Ainv, k->rc, k->zc);
+    hipLaunchKernelGGL(k_coarse_restrict, dim3((unsigned)k->nagg), restrict_block(k->agg), 0, c->stream, c->n, k->agg,
+                       k->nf, k->phis, v, k->rc);
+    hipLaunchKernelGGL(k_coarse_matvec, dim3((unsigned)((k->mc + 3) / 4)), dim3(256), sizeof(double) * k->mc,
+                       c->stream, k->mc, k->Ainv, k->rc, k->zc);
     hipLaunchKernelGGL(k_apply_two_level, grid1(c->n), dim3(256), 0, c->stream, c->n, k->agg, k->nf, k->dinv, v,
-                       k->zc, k->phi, 1.0, 0, z, parts);
+                       k->zc, k->phis, 1.0, 0, z, parts);
     PDX_HIP(c, hipGetLastError());
     return PDX_OK;
 }
@@ -1099,11 +1152,12 @@ pdx_status pdx_nk_coarse_add(pdx_ctx *c, const double *v, double *z, double alph
 {
     pdx_nk *k = c->nk;
     if (!k->coarse_ready) return pdx_fail(c, PDX_ERR_STATE, "coarse space not built");
-    hipLaunchKernelGGL(k_coarse_restrict, dim3((unsigned)k->nagg), dim3(256), 0, c->stream, c->n, k->agg,
-                       k->nf, k->phi, v, k->rc);
-    hipLaunchKernelGGL(k_coarse_matvec, dim3((unsigned)k->mc), dim3(64), 0, c->stream, k->mc, k->Ainv, k->rc, k->zc);
+    hipLaunchKernelGGL(k_coarse_restrict, dim3((unsigned)k->nagg), restrict_block(k->agg), 0, c->stream, c->n, k->agg,
+                       k->nf, k->phis, v, k->rc);
+    hipLaunchKernelGGL(k_coarse_matvec, dim3((unsigned)((k->mc + 3) / 4)), dim3(256), sizeof(double) * k->mc,
+                       c->stream, k->mc, k->Ainv, k->rc, k->zc);
     hipLaunchKernelGGL(k_apply_two_level, grid1(c->n), dim3(256), 0, c->stream, c->n, k->agg, k->nf, k->dinv, v,
-                       k->zc, k->phi, alpha, 1, z, (double *)nullptr);
+                       k->zc, k->phis, alpha, 1, z, (double *)nullptr);
     PDX_HIP(c, hipGetLastError());
     return PDX_OK;
 }

@@ -5,6 +5,16 @@
 
 #include "pdx_internal.h"
 
+// Krylov columns the FGMRES loop keeps queued beyond the one whose convergence flag the host
+// reads: after queuing column j it waits for column j - PDX_NK_LAG, so the device always has a
+// column queued while the host wakes up and queues the next; PDX_NK_SLOTS flag slots and
+// events rotate over the columns in flight.  One column suffices: two measured the same
+// time per iteration and queue one more column past convergence.
+#ifndef PDX_NK_LAG
+#define PDX_NK_LAG 1
+#endif
+#define PDX_NK_SLOTS (PDX_NK_LAG + 1)
+
 struct pdx_nk {
     int64_t m = 0;              // 3n
     int64_t nc = 0;             // constrained dofs
@@ -22,12 +32,12 @@ struct pdx_nk {
     double *dinv = nullptr;     // 9n point-block Jacobi inverse
     double *dsc = nullptr;      // device scalars (||u||^2, ||v||^2)
     double *gm = nullptr;       // GMRES state on the device: Hessenberg, rotations, g, y, flags
-    int *flag_host = nullptr;   // pinned (coherent): the convergence flag of the last two columns
+    int *flag_host = nullptr;   // pinned (coherent): the convergence flags of the columns in flight
     int *flag_dev = nullptr;    // the same memory as the kernels address it
     bool jv_fused = false;      // J v by the one-kernel central difference (k_bb_jv)
     double *vparts = nullptr;   // partial sums of |v|^2 for a J v whose caller has none
     double *zparts = nullptr;   // partial sums of |z|^2 left by the preconditioner's kernels
-    hipEvent_t ev_col[2] = {nullptr, nullptr};
+    hipEvent_t ev_col[PDX_NK_SLOTS] = {};
     // two-level preconditioner: aggregates of `agg` consecutive points, nf basis fields per
     // aggregate, mc = nf nagg coarse dofs
     int agg = 0;

@@ -47,7 +47,7 @@ void pdx_nk_free(pdx_ctx *c)
     if (k->rb) rocblas_destroy_handle(k->rb);
     if (k->gm) (void)hipFree(k->gm);
     if (k->flag_host) (void)hipHostFree(k->flag_host);
-    for (int e = 0; e < 2; e++)
+    for (int e = 0; e < PDX_NK_SLOTS; e++)
         if (k->ev_col[e]) (void)hipEventDestroy(k->ev_col[e]);
     delete k;
     c->nk = nullptr;
@@ -130,7 +130,9 @@ __global__ void k_lincomb_add(int64_t m, int nv, const double *Z, int64_t ldz, c
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     double s = x[i];
-    for (int k = 0; k < nv; k++) s += y[k] * Z[k * ldz + i];
+    // columns queued past convergence carry y = 0 and take no part, whatever they hold
+    for (int k = 0; k < nv; k++)
+        if (y[k] != 0.0) s += y[k] * Z[k * ldz + i];
     x[i] = s;
 }
 
@@ -493,9 +495,9 @@ static pdx_status nk_ensure(pdx_ctx *c, int restart)
         // evaluations of the step kernel (the same numbers, for comparison)
         const char *two = std::getenv("PDX_JV_TWO_KERNELS");
         k->jv_fused = pdx_bb_jv_fusable(c) && !(two && two[0] == '1');
-        PDX_HIP(c, hipHostMalloc((void **)&k->flag_host, sizeof(int) * 2, hipHostMallocCoherent));
+        PDX_HIP(c, hipHostMalloc((void **)&k->flag_host, sizeof(int) * PDX_NK_SLOTS, hipHostMallocCoherent));
         PDX_HIP(c, hipHostGetDevicePointer((void **)&k->flag_dev, k->flag_host, 0));
-        for (int e = 0; e < 2; e++)
+        for (int e = 0; e < PDX_NK_SLOTS; e++)
             PDX_HIP(c, hipEventCreateWithFlags(&k->ev_col[e], hipEventDisableTiming));
     }
     if (restart > k->m_cap) {
@@ -1282,7 +1284,7 @@ __global__ void k_gm_start(int R, const double *b2, double *g, int *flags)
 // orthonormal, |w - V h2|^2 = n1 - |h2|^2.  The rotations run on the column staged in
 // LDS.  Stores the new norm in hn[0], the columns so far in flags[1]; sets flags[0] (done)
 // when the residual estimate |g[j + 1]| reaches thr, the norm vanishes or the iteration is
-// the last one allowed, and copies flags[0] to flag_host[j & 1] (pinned host memory, may
+// the last one allowed, and copies flags[0] to flag_host[j % PDX_NK_SLOTS] (pinned host memory, may
 // be null).  One workgroup of 1024 threads.
 __global__ __launch_bounds__(1024) void k_arnoldi(int j, int R, const double *h1, const double *parts,
                                                   int np, double *h2, double *hn, double thr,
@@ -1326,7 +1328,7 @@ __global__ __launch_bounds__(1024) void k_arnoldi(int j, int R, const double *h1
     if (t == 0) {
         const double hnv = hn_s;
         if (done_before) {
-            if (flag_host) __hip_atomic_store(flag_host + (j & 1), 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            if (flag_host) __hip_atomic_store(flag_host + j % PDX_NK_SLOTS, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         } else {
             col[j + 1] = hnv;
             // the previous rotations, eight at a time: their inputs are read from LDS ahead
@@ -1362,7 +1364,7 @@ __global__ __launch_bounds__(1024) void k_arnoldi(int j, int R, const double *h1
             flags[1] = j + 1;
             const int done = (fabs(-sj * gj) <= thr || last_allowed || hnv == 0) ? 1 : 0;
             flags[0] = done;
-            if (flag_host) __hip_atomic_store(flag_host + (j & 1), done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            if (flag_host) __hip_atomic_store(flag_host + j % PDX_NK_SLOTS, done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         }
     }
     __syncthreads();
@@ -1498,12 +1500,13 @@ static pdx_status fgmres(pdx_ctx *c, const double *b, int restart, int maxiter, 
             // Z has R columns: the last column's V needs no image
             PDX_TRY(nk_next(c, precond, nv, G.h2, G.hn, 0, w, w, j + 1 < R ? k->Zb + (int64_t)(j + 1) * m : nullptr,
                             &zp, &znp));
-            PDX_HIP(c, hipEventRecord(k->ev_col[j & 1], s));
+            PDX_HIP(c, hipEventRecord(k->ev_col[j % PDX_NK_SLOTS], s));
             enq++;
-            // with this column queued, wait for the previous one's flag
-            if (j >= 1) {
-                PDX_HIP(c, hipEventSynchronize(k->ev_col[(j - 1) & 1]));
-                if (__atomic_load_n(k->flag_host + ((j - 1) & 1), __ATOMIC_ACQUIRE)) break;
+            // with this column queued, wait for the flag of the column PDX_NK_LAG before it
+            if (j >= PDX_NK_LAG) {
+                const int jw = j - PDX_NK_LAG;
+                PDX_HIP(c, hipEventSynchronize(k->ev_col[jw % PDX_NK_SLOTS]));
+                if (__atomic_load_n(k->flag_host + jw % PDX_NK_SLOTS, __ATOMIC_ACQUIRE)) break;
             }
         }
         hipLaunchKernelGGL(k_gm_solve, dim3(1), dim3(64), 0, s, R, G.H, G.g, G.flags, G.y, G.hn + 1);

@@ -6,14 +6,15 @@
 #include "pdx_internal.h"
 
 // Krylov columns the FGMRES loop keeps queued beyond the one whose convergence flag the host
-// reads: after queuing column j it waits for column j - PDX_NK_LAG, so the device always has a
-// column queued while the host wakes up and queues the next; PDX_NK_SLOTS flag slots and
-// events rotate over the columns in flight.  One column suffices: two measured the same
+// reads: after queuing column j it spins on the flag word of column j - PDX_NK_LAG (no event
+// in the stream), so the device always has a column queued; PDX_NK_SLOTS flag slots rotate
+// over the columns in flight.  One column suffices: two measured the same
 // time per iteration and queue one more column past convergence.
 #ifndef PDX_NK_LAG
 #define PDX_NK_LAG 1
 #endif
 #define PDX_NK_SLOTS (PDX_NK_LAG + 1)
+
 
 struct pdx_nk {
     int64_t m = 0;              // 3n
@@ -37,7 +38,6 @@ struct pdx_nk {
     bool jv_fused = false;      // J v by the one-kernel central difference (k_bb_jv)
     double *vparts = nullptr;   // partial sums of |v|^2 for a J v whose caller has none
     double *zparts = nullptr;   // partial sums of |z|^2 left by the preconditioner's kernels
-    hipEvent_t ev_col[PDX_NK_SLOTS] = {};
     // two-level preconditioner: aggregates of `agg` consecutive points, nf basis fields per
     // aggregate, mc = nf nagg coarse dofs
     int agg = 0;
@@ -56,6 +56,8 @@ struct pdx_nk {
     rocblas_handle rb = nullptr;
     bool coarse_ready = false;
     int64_t n_jv = 0;
+    int col_tok = 0;             // token of the last queued Krylov column (1 .. 2^30 - 1)
+    int tok[PDX_NK_SLOTS] = {};  // tokens of the columns in the flag slots
     bool verbose = false;       // PDX_NK_VERBOSE: a line per Newton iteration on stderr
 };
 

@@ -47,8 +47,6 @@ void pdx_nk_free(pdx_ctx *c)
     if (k->rb) rocblas_destroy_handle(k->rb);
     if (k->gm) (void)hipFree(k->gm);
     if (k->flag_host) (void)hipHostFree(k->flag_host);
-    for (int e = 0; e < PDX_NK_SLOTS; e++)
-        if (k->ev_col[e]) (void)hipEventDestroy(k->ev_col[e]);
     delete k;
     c->nk = nullptr;
 }
@@ -497,8 +495,7 @@ static pdx_status nk_ensure(pdx_ctx *c, int restart)
         k->jv_fused = pdx_bb_jv_fusable(c) && !(two && two[0] == '1');
         PDX_HIP(c, hipHostMalloc((void **)&k->flag_host, sizeof(int) * PDX_NK_SLOTS, hipHostMallocCoherent));
         PDX_HIP(c, hipHostGetDevicePointer((void **)&k->flag_dev, k->flag_host, 0));
-        for (int e = 0; e < PDX_NK_SLOTS; e++)
-            PDX_HIP(c, hipEventCreateWithFlags(&k->ev_col[e], hipEventDisableTiming));
+        std::memset(k->flag_host, 0, sizeof(int) * PDX_NK_SLOTS);
     }
     if (restart > k->m_cap) {
         if (k->Vb) (void)hipFree(k->Vb);
@@ -1284,12 +1281,13 @@ __global__ void k_gm_start(int R, const double *b2, double *g, int *flags)
 // orthonormal, |w - V h2|^2 = n1 - |h2|^2.  The rotations run on the column staged in
 // LDS.  Stores the new norm in hn[0], the columns so far in flags[1]; sets flags[0] (done)
 // when the residual estimate |g[j + 1]| reaches thr, the norm vanishes or the iteration is
-// the last one allowed, and copies flags[0] to flag_host[j % PDX_NK_SLOTS] (pinned host memory, may
-// be null).  One workgroup of 1024 threads.
+// the last one allowed, and writes 2 tok + flags[0] to flag_host[j % PDX_NK_SLOTS] (pinned host
+// memory, may be null; tok tells the host which column wrote the slot).  One workgroup of
+// 1024 threads.
 __global__ __launch_bounds__(1024) void k_arnoldi(int j, int R, const double *h1, const double *parts,
                                                   int np, double *h2, double *hn, double thr,
                                                   int last_allowed, double *H, double *cs, double *sn,
-                                                  double *g, int *flags, int *flag_host)
+                                                  double *g, int *flags, int *flag_host, int tok)
 {
     __shared__ double h2s[PDX_MAX_RESTART + 2];
     __shared__ double col[PDX_MAX_RESTART + 2], c_[PDX_MAX_RESTART], s_[PDX_MAX_RESTART];
@@ -1328,7 +1326,7 @@ __global__ __launch_bounds__(1024) void k_arnoldi(int j, int R, const double *h1
     if (t == 0) {
         const double hnv = hn_s;
         if (done_before) {
-            if (flag_host) __hip_atomic_store(flag_host + j % PDX_NK_SLOTS, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            if (flag_host) __hip_atomic_store(flag_host + j % PDX_NK_SLOTS, 2 * tok + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         } else {
             col[j + 1] = hnv;
             // the previous rotations, eight at a time: their inputs are read from LDS ahead
@@ -1364,7 +1362,7 @@ __global__ __launch_bounds__(1024) void k_arnoldi(int j, int R, const double *h1
             flags[1] = j + 1;
             const int done = (fabs(-sj * gj) <= thr || last_allowed || hnv == 0) ? 1 : 0;
             flags[0] = done;
-            if (flag_host) __hip_atomic_store(flag_host + j % PDX_NK_SLOTS, done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            if (flag_host) __hip_atomic_store(flag_host + j % PDX_NK_SLOTS, 2 * tok + done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         }
     }
     __syncthreads();
@@ -1399,6 +1397,29 @@ __global__ __launch_bounds__(64) void k_gm_solve(int R, const double *H, const d
     if (t < R) y[t] = y0;
     if (t + 64 < R) y[t + 64] = y1;
     if (t == 0) *est = fabs(g[j]);
+}
+
+// Waits until the Arnoldi kernel of the column with token tok has written its flag into slot
+// (2 tok + done, a system-scope store into pinned memory) by spinning on that word, with no
+// event in the stream; after 2 s without it, synchronises the stream and reads it once more.
+static pdx_status nk_wait_flag(pdx_ctx *c, int slot, int tok, int *done)
+{
+    pdx_nk *k = c->nk;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t spin = 1;; spin++) {
+        const int v = __atomic_load_n(k->flag_host + slot, __ATOMIC_ACQUIRE);
+        if ((v >> 1) == tok) {
+            *done = v & 1;
+            return PDX_OK;
+        }
+        if ((spin & 1023) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) {
+            PDX_HIP(c, hipStreamSynchronize(c->stream));
+            const int w = __atomic_load_n(k->flag_host + slot, __ATOMIC_ACQUIRE);
+            if ((w >> 1) != tok) return pdx_fail(c, PDX_ERR_HIP, "a Krylov column's flag was not written");
+            *done = w & 1;
+            return PDX_OK;
+        }
+    }
 }
 
 // |x|^2 into out_dev
@@ -1493,20 +1514,22 @@ static pdx_status fgmres(pdx_ctx *c, const double *b, int restart, int maxiter, 
                 hipLaunchKernelGGL(k_cgs_axdn_loop, dim3(CB), dim3(256), 0, s, m, nv, (const double *)G.h1,
                                    (const double *)k->Vb, m, w, k->red);
             const int last_allowed = (it + enq + 1 >= maxiter) ? 1 : 0;
+            const int tok = k->col_tok = (k->col_tok & 0x3fffffff) + 1;
+            k->tok[j % PDX_NK_SLOTS] = tok;
             hipLaunchKernelGGL(k_arnoldi, dim3(1), dim3(1024), 0, s, j, R, (const double *)G.h1,
                                (const double *)k->red, CB, G.h2, G.hn, thr, last_allowed, G.H, G.cs, G.sn,
-                               G.g, G.flags, k->flag_dev);
+                               G.g, G.flags, k->flag_dev, tok);
             PDX_HIP(c, hipGetLastError());
             // Z has R columns: the last column's V needs no image
             PDX_TRY(nk_next(c, precond, nv, G.h2, G.hn, 0, w, w, j + 1 < R ? k->Zb + (int64_t)(j + 1) * m : nullptr,
                             &zp, &znp));
-            PDX_HIP(c, hipEventRecord(k->ev_col[j % PDX_NK_SLOTS], s));
             enq++;
             // with this column queued, wait for the flag of the column PDX_NK_LAG before it
             if (j >= PDX_NK_LAG) {
                 const int jw = j - PDX_NK_LAG;
-                PDX_HIP(c, hipEventSynchronize(k->ev_col[jw % PDX_NK_SLOTS]));
-                if (__atomic_load_n(k->flag_host + jw % PDX_NK_SLOTS, __ATOMIC_ACQUIRE)) break;
+                int done = 0;
+                PDX_TRY(nk_wait_flag(c, jw % PDX_NK_SLOTS, k->tok[jw % PDX_NK_SLOTS], &done));
+                if (done) break;
             }
         }
         hipLaunchKernelGGL(k_gm_solve, dim3(1), dim3(64), 0, s, R, G.H, G.g, G.flags, G.y, G.hn + 1);

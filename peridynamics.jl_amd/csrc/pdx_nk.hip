@@ -1514,7 +1514,7 @@ static pdx_status fgmres(pdx_ctx *c, const double *b, int restart, int maxiter, 
                 hipLaunchKernelGGL(k_cgs_axdn_loop, dim3(CB), dim3(256), 0, s, m, nv, (const double *)G.h1,
                                    (const double *)k->Vb, m, w, k->red);
             const int last_allowed = (it + enq + 1 >= maxiter) ? 1 : 0;
-            const int tok = k->col_tok = (k->col_tok & 0x3fffffff) + 1;
+            const int tok = k->col_tok = k->col_tok % 0x3fffffff + 1;  // 1 .. 2^30 - 1: 2 tok + 1 fits an int
             k->tok[j % PDX_NK_SLOTS] = tok;
             hipLaunchKernelGGL(k_arnoldi, dim3(1), dim3(1024), 0, s, j, R, (const double *)G.h1,
                                (const double *)k->red, CB, G.h2, G.hn, thr, last_allowed, G.H, G.cs, G.sn,

@@ -861,6 +861,8 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds_mx(pdx_step_args A)
             wd[u] = (t + u * S < c) ? wbase[(int64_t)(t + u * S) * 64] : 0u;
             sc[u] = (SC && t + u * S < c) ? A.scf32[b0 + (int64_t)(t + u * S) * 64] : 1.f;
         }
+        // the group's bond forces are summed in FP32 (fused), then once into the FP64 sums
+        float gx = 0.f, gy = 0.f, gz = 0.f;
 #pragma unroll
         for (int u = 0; u < U; u++) {
             if (t + u * S >= c) continue;
@@ -904,11 +906,14 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds_mx(pdx_step_args A)
             }
             const float bcb = MULTI ? (bcf + (float)A.psets[(!CAP || slot < HL) ? hpid[slot] : A.pid[jg]].bc) * 0.5f : bcf;
             const float sf = act ? (((SC ? sc[u] * bcb : bcb) * eps) * a.w) * __builtin_amdgcn_rcpf(l) : 0.f;
-            bx += (double)(sf * (Dx + Ux));
-            by += (double)(sf * (Dy + Uy));
-            bz += (double)(sf * (Dz + Uz));
+            gx = __builtin_fmaf(sf, Dx + Ux, gx);
+            gy = __builtin_fmaf(sf, Dy + Uy, gy);
+            gz = __builtin_fmaf(sf, Dz + Uz, gz);
             na += act ? 1 : 0;
         }
+        bx += (double)gx;
+        by += (double)gy;
+        bz += (double)gz;
     }
 #pragma unroll
     for (int o = 1; o < S; o <<= 1) {
@@ -1115,6 +1120,8 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds2_mx(pdx_step_args A)
             wd[u] = (t + u * S < c) ? wbase[(int64_t)(t + u * S) * 64] : 0u;
             sc[u] = (SC && t + u * S < c) ? A.scf32[b0 + (int64_t)(t + u * S) * 64] : 1.f;
         }
+        // the group's bond forces are summed in FP32 (fused), then once into the FP64 sums
+        float gx = 0.f, gy = 0.f, gz = 0.f;
 #pragma unroll
         for (int u = 0; u < U; u++) {
             if (t + u * S >= c) continue;
@@ -1148,10 +1155,13 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds2_mx(pdx_step_args A)
             const float fi = __builtin_fmaf(c1i, e, c2i * Lb) * a.w;
             const float fj = __builtin_fmaf(b.w, e, c2Vj * Lb);
             const float sf = (w * __builtin_amdgcn_rcpf(l)) * (fi + fj);
-            bx += (double)(sf * (Dx + Ux));
-            by += (double)(sf * (Dy + Uy));
-            bz += (double)(sf * (Dz + Uz));
+            gx = __builtin_fmaf(sf, Dx + Ux, gx);
+            gy = __builtin_fmaf(sf, Dy + Uy, gy);
+            gz = __builtin_fmaf(sf, Dz + Uz, gz);
         }
+        bx += (double)gx;
+        by += (double)gy;
+        bz += (double)gz;
     }
 #pragma unroll
     for (int o = 1; o < S; o <<= 1) {

@@ -998,6 +998,8 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds1_mx(pdx_step_args A)
             wd[u] = (t + u * S < c) ? wbase[(int64_t)(t + u * S) * 64] : 0u;
             sc[u] = (SC && t + u * S < c) ? A.scf32[b0 + (int64_t)(t + u * S) * 64] : 1.f;
         }
+        // the group's sums in FP32 (fused), then once into the FP64 sums
+        float gv = 0.f, gs = 0.f;
 #pragma unroll
         for (int u = 0; u < U; u++) {
             if (t + u * S >= c) continue;
@@ -1040,11 +1042,13 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds1_mx(pdx_step_args A)
                 float w = deltaf * rL;  // linear_kernel(delta, L)
                 if (SC) w *= sc[u];
                 const float wVL = (w * a.w) * Lb;
-                wvol += (double)(wVL * Lb);
-                Ssum += (double)(wVL * e);
+                gv = __builtin_fmaf(wVL, Lb, gv);
+                gs = __builtin_fmaf(wVL, e, gs);
                 na++;
             }
         }
+        wvol += (double)gv;
+        Ssum += (double)gs;
     }
 #pragma unroll
     for (int o = 1; o < S; o <<= 1) {
@@ -2290,16 +2294,17 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_pass1_mx(pdx_step_args A)
             if (act && !dead) {
                 na++;
                 const float wv = (deltaf * rL) * a.w;  // w V_j
-                Km[0] += (double)(wv * (D[0] * D[0]));
-                Km[1] += (double)(wv * (D[1] * D[0]));
-                Km[2] += (double)(wv * (D[2] * D[0]));
-                Km[3] += (double)(wv * (D[1] * D[1]));
-                Km[4] += (double)(wv * (D[2] * D[1]));
-                Km[5] += (double)(wv * (D[2] * D[2]));
+                const float wD[3] = {wv * D[0], wv * D[1], wv * D[2]};  // the products of w V_j D
+                Km[0] += (double)(wD[0] * D[0]);
+                Km[1] += (double)(wD[1] * D[0]);
+                Km[2] += (double)(wD[2] * D[0]);
+                Km[3] += (double)(wD[1] * D[1]);
+                Km[4] += (double)(wD[2] * D[1]);
+                Km[5] += (double)(wD[2] * D[2]);
 #pragma unroll
                 for (int cc = 0; cc < 3; cc++)
 #pragma unroll
-                    for (int r = 0; r < 3; r++) Ht[r + 3 * cc] += (double)(wv * (Uv[r] * D[cc]));
+                    for (int r = 0; r < 3; r++) Ht[r + 3 * cc] += (double)(Uv[r] * wD[cc]);
             }
         }
     }

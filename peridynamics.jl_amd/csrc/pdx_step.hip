@@ -1374,11 +1374,14 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds1(pdx_step_args A)
                 if (act) {
                     double w = (F & PDX_FEAT_KERN) ? pdx_omega(A.kernel, A.delta, Lb, rL) : A.delta * rL;
                     if (SC) w *= A.scf[b0 + o];  // omega * beta
-                    wvol += (w * (Lb * Lb)) * rj.w;
-                    Ssum += ((w * Lb) * e) * rj.w;
+                    // w V_j formed once: wvol += (w V_j L) L, Ssum += (w V_j L) e
+                    const double wV = w * rj.w;
+                    const double wVL = wV * Lb;
+                    wvol = __builtin_fma(wVL, Lb, wvol);
+                    Ssum = __builtin_fma(wVL, e, Ssum);
                     na++;
                     if (SPLIT) {
-                        const double q = (w * rl) * rj.w;
+                        const double q = wV * rl;
                         const double qL = q * Lb, qe = q * e;
                         ax = __builtin_fma(qL, dx, ax);
                         ay = __builtin_fma(qL, dy, ay);
@@ -1579,7 +1582,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds2(pdx_step_args A)
                 const double e = l - Lb;
                 if (SPLIT) {
                     // the reverse part (c2 V)_j L + (c1 V)_j e
-                    const double s = (w * rl) * (cj.y * Lb + cj.x * e);
+                    const double s = (w * rl) * __builtin_fma(cj.x, e, cj.y * Lb);
                     bx = __builtin_fma(s, dx, bx);
                     by = __builtin_fma(s, dy, by);
                     bz = __builtin_fma(s, dz, bz);

@@ -426,6 +426,29 @@ __device__ __forceinline__ void bb_bond64(double4 ri, double4 rj, uint16_t *wp, 
     bb_bond64_r(ri, rj, wp, wd, Lb, pdx_rcp1(Lb), sc, bcb, epsc, epsc_lo, epsc_hi, bx, by, bz, na);
 }
 
+// The same bond with one parameter set, whose neighbour record carries V_j bc in its fourth
+// word (the neighbour-table staging forms it once per entry): one multiply fewer per bond.
+__device__ __forceinline__ void bb_bond64_vbc(double4 ri, double4 rj, uint16_t *wp, uint32_t wd, double Lb,
+                                              double rL, double sc, double epsc, double epsc_lo,
+                                              double epsc_hi, double &bx, double &by, double &bz, int &na)
+{
+    const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
+    const double d2 = (dx * dx + dy * dy) + dz * dz;
+    double rl;
+    const double l = pdx_sqrt_rsq(d2, rl);
+    const double e = l - Lb;
+    const double eps = e * rL;
+    const bool act = bond_fail16(wp, wd, e, Lb, eps, epsc, epsc_lo, epsc_hi);
+    const double s = ((act ? sc * eps : 0.0) * rj.w) * rl;  // w eps (V_j bc) / l, w = active * sc
+    bx = __builtin_fma(s, dx, bx);
+    by = __builtin_fma(s, dy, by);
+    bz = __builtin_fma(s, dz, bz);
+    na += act ? 1 : 0;
+}
+
+// a neighbour record (x, V) with V_j bc in the fourth word, for bb_bond64_vbc
+__device__ __forceinline__ double4 pdx_rec_vbc(double4 r, double bc) { return make_double4(r.x, r.y, r.z, r.w * bc); }
+
 // F = feature mask.  PDX_FEAT_MULTI: several parameter sets (force_density_point! with a
 // ParameterHandler, bond_based.jl:163-180): eps_c of the row's point, (bc_i + bc_j) / 2 per
 // bond, rho_i.  PDX_FEAT_SC: EnergySurfaceCorrection, w = bond_active * scfactor[bond]
@@ -460,7 +483,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds(pdx_step_args A)
         // the whole table fits: stage it once
         for (int h = threadIdx.x; h < H; h += PDX_BLOCK * S) {
             const int32_t j = hidx[h];
-            hrec[h] = A.rec_in[j];
+            hrec[h] = MULTI ? A.rec_in[j] : pdx_rec_vbc(A.rec_in[j], A.bc);
             if (MULTI) hpid[h] = A.pid[j];
         }
         __syncthreads();
@@ -498,9 +521,12 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds(pdx_step_args A)
             for (int u = 0; u < U; u++) {
                 if (t + u * S >= c) continue;
                 const int slot = (int)(wd[u] & PDX_W16_SLOT);
-                const double bcb = MULTI ? (bc + A.psets[hpid[slot]].bc) / 2.0 : bc;
-                bb_bond64(ri, hrec[slot], wbase + (int64_t)(t + u * S) * 64, wd[u], Ld[u], Sd[u], bcb,
-                          epsc, epsc_lo, epsc_hi, bx, by, bz, na);
+                if (MULTI)
+                    bb_bond64(ri, hrec[slot], wbase + (int64_t)(t + u * S) * 64, wd[u], Ld[u], Sd[u],
+                              (bc + A.psets[hpid[slot]].bc) / 2.0, epsc, epsc_lo, epsc_hi, bx, by, bz, na);
+                else
+                    bb_bond64_vbc(ri, hrec[slot], wbase + (int64_t)(t + u * S) * 64, wd[u], Ld[u],
+                                  pdx_rcp1(Ld[u]), Sd[u], epsc, epsc_lo, epsc_hi, bx, by, bz, na);
             }
         }
     } else {
@@ -526,7 +552,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds(pdx_step_args A)
             if (T0 > 0) __syncthreads();  // the previous tile is consumed
             for (int h = T0 + (int)threadIdx.x; h < T1; h += PDX_BLOCK * S) {
                 const int32_t j = hidx[h];
-                hrec[h - T0] = A.rec_in[j];
+                hrec[h - T0] = MULTI ? A.rec_in[j] : pdx_rec_vbc(A.rec_in[j], A.bc);
                 if (MULTI) hpid[h - T0] = A.pid[j];
             }
             __syncthreads();
@@ -554,9 +580,12 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_lds(pdx_step_args A)
                 for (int u = 0; u < U; u++) {
                     if (u >= kk) continue;
                     const int ls = (int)(wd[u] & PDX_W16_SLOT) - T0;
-                    const double bcb = MULTI ? (bc + A.psets[hpid[ls]].bc) / 2.0 : bc;
-                    bb_bond64(ri, hrec[ls], wbase + (int64_t)(t + u * S) * 64, wd[u], Ld[u], Sd[u],
-                              bcb, epsc, epsc_lo, epsc_hi, bx, by, bz, na);
+                    if (MULTI)
+                        bb_bond64(ri, hrec[ls], wbase + (int64_t)(t + u * S) * 64, wd[u], Ld[u], Sd[u],
+                                  (bc + A.psets[hpid[ls]].bc) / 2.0, epsc, epsc_lo, epsc_hi, bx, by, bz, na);
+                    else
+                        bb_bond64_vbc(ri, hrec[ls], wbase + (int64_t)(t + u * S) * 64, wd[u], Ld[u],
+                                      pdx_rcp1(Ld[u]), Sd[u], epsc, epsc_lo, epsc_hi, bx, by, bz, na);
                 }
                 if (kk < U) {
                     t += kk * S;  // resume at the first bond past the tile (or the row's end)
@@ -643,7 +672,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_jv(pdx_step_args A, const 
         if (pass == 1) __syncthreads();  // the + table is consumed
         for (int h = threadIdx.x; h < H; h += PDX_BLOCK * S) {
             const int32_t j = hidx[h];
-            hrec[h] = jv_position(A.Xr[j], u, v, j, es);
+            hrec[h] = pdx_rec_vbc(jv_position(A.Xr[j], u, v, j, es), A.bc);
         }
         __syncthreads();
         if (!valid) continue;  // all S lanes of the point together
@@ -667,8 +696,8 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_jv(pdx_step_args A, const 
             for (int q = 0; q < U; q++) {
                 if (t + q * S >= c) continue;
                 const int slot = (int)(wd[q] & PDX_W16_SLOT);
-                bb_bond64(ri, hrec[slot], wbase + (int64_t)(t + q * S) * 64, wd[q], Ld[q], 1.0, A.bc,
-                          A.epsc, A.epsc_lo, A.epsc_hi, bx, by, bz, nap);
+                bb_bond64_vbc(ri, hrec[slot], wbase + (int64_t)(t + q * S) * 64, wd[q], Ld[q],
+                              pdx_rcp1(Ld[q]), 1.0, A.epsc, A.epsc_lo, A.epsc_hi, bx, by, bz, nap);
             }
         }
 #pragma unroll
@@ -722,8 +751,8 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_jv1(pdx_step_args A, const
     for (int h = threadIdx.x; h < H; h += PDX_BLOCK * S) {
         const int32_t j = hidx[h];
         const double4 X = A.Xr[j];
-        hrec[h] = jv_position(X, u, v, j, e);
-        hrec[H + h] = jv_position(X, u, v, j, -e);
+        hrec[h] = pdx_rec_vbc(jv_position(X, u, v, j, e), A.bc);
+        hrec[H + h] = pdx_rec_vbc(jv_position(X, u, v, j, -e), A.bc);
     }
     __syncthreads();
     if (valid) {
@@ -755,11 +784,11 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_bb_jv1(pdx_step_args A, const
                 uint16_t *const wp = wbase + (int64_t)(t + q * S) * 64;
                 const int before = nap;
                 const double rL = pdx_rcp1(Ld[q]);
-                bb_bond64_r(rp, hrec[slot], wp, wd[q], Ld[q], rL, 1.0, A.bc, A.epsc, A.epsc_lo, A.epsc_hi,
-                            px, py, pz, nap);
+                bb_bond64_vbc(rp, hrec[slot], wp, wd[q], Ld[q], rL, 1.0, A.epsc, A.epsc_lo, A.epsc_hi,
+                              px, py, pz, nap);
                 const uint32_t wm = nap != before ? wd[q] : (wd[q] & ~(uint32_t)PDX_W16_ACT);
-                bb_bond64_r(rm, hrec[H + slot], wp, wm, Ld[q], rL, 1.0, A.bc, A.epsc, A.epsc_lo, A.epsc_hi,
-                            mx, my, mz, nam);
+                bb_bond64_vbc(rm, hrec[H + slot], wp, wm, Ld[q], rL, 1.0, A.epsc, A.epsc_lo, A.epsc_hi,
+                              mx, my, mz, nam);
             }
         }
 #pragma unroll

@@ -1321,7 +1321,10 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds1(pdx_step_args A)
     auto stage = [&](int T0, int T1) {
         for (int h = T0 + (int)threadIdx.x; h < T1; h += PDX_BLOCK * S) {
             const int32_t j = hidx[h];
-            hrec[h - T0] = A.rec_in[j];
+            // with the linear influence function the record carries delta V_j (omega V_j =
+            // (delta V_j) / L)
+            const double4 r = A.rec_in[j];
+            hrec[h - T0] = (F & PDX_FEAT_KERN) ? r : make_double4(r.x, r.y, r.z, A.delta * r.w);
             if (MULTI) hpid[h - T0] = A.pid[j];
         }
     };
@@ -1401,7 +1404,8 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds1(pdx_step_args A)
                     act = bond_fail16(wbase + o, wd, e, Lb, e * rL, epsc, epsc_lo, epsc_hi);
                 }
                 if (act) {
-                    double w = (F & PDX_FEAT_KERN) ? pdx_omega(A.kernel, A.delta, Lb, rL) : A.delta * rL;
+                    // omega, or 1 / L with delta already in the record
+                    double w = (F & PDX_FEAT_KERN) ? pdx_omega(A.kernel, A.delta, Lb, rL) : rL;
                     if (SC) w *= A.scf[b0 + o];  // omega * beta
                     // w V_j formed once: wvol += (w V_j L) L, Ssum += (w V_j L) e
                     const double wV = w * rj.w;
@@ -1525,8 +1529,10 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds2(pdx_step_args A)
             const double4 r = A.rec_in[j];
             const double2 cj = cst[j];
             if (SPLIT) {
-                hrec[h - T0] = make_double4(r.x, r.y, r.z, cj.x);
-                hc2[h - T0] = cj.y;
+                // with the linear influence function: delta (c1 V)_j, delta (c2 V)_j
+                const double dl = (F & PDX_FEAT_KERN) ? 1.0 : A.delta;
+                hrec[h - T0] = make_double4(r.x, r.y, r.z, dl * cj.x);
+                hc2[h - T0] = dl * cj.y;
             } else {
                 hrec[h - T0] = r;
                 hc[h - T0] = cj;
@@ -1606,7 +1612,8 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_osb_lds2(pdx_step_args A)
                 double rl;
                 const double l = pdx_sqrt_rsq((dx * dx + dy * dy) + dz * dz, rl);
                 const double rLb = pdx_rcp1(Lb);
-                double w = (F & PDX_FEAT_KERN) ? pdx_omega(A.kernel, A.delta, Lb, rLb) : A.delta * rLb;
+                // omega, or 1 / L where delta is already in the staged constants (one set)
+                double w = (F & PDX_FEAT_KERN) ? pdx_omega(A.kernel, A.delta, Lb, rLb) : (SPLIT ? rLb : A.delta * rLb);
                 if (SC) w *= A.scf[b0 + o];  // beta is symmetric: both ends' copies of the bond carry it
                 const double e = l - Lb;
                 if (SPLIT) {

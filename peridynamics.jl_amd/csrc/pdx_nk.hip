@@ -309,9 +309,17 @@ static inline dim3 grid1(int64_t m, int bs = 256) { return dim3((unsigned)((m + 
 // butterfly); all indices into the register arrays are compile-time constants, so the
 // arrays stay in registers.
 #define CGS_REG 32
+// threads per block of the Gram-Schmidt kernels (one dof per thread).  256 keeps five blocks
+// per CU resident (78-95 VGPRs), so a 300 k-dof vector's 1172 blocks run in one round; 512
+// and 1024 leave 4x fewer partial sums to reduce but measured 0.116 and 0.125 ms per
+// block-Jacobi Krylov iteration against 0.114 ms.
+#ifndef CGS_T
+#define CGS_T 256
+#endif
+#define CGS_NW (CGS_T / 64)
 // one dof per thread (grid-stride beyond CGS_MAX_BLOCKS blocks): the loads of all vectors
 // of a dof are in flight at once
-static inline int cgs_blocks(int64_t m) { return (int)std::min<int64_t>((m + 255) / 256, CGS_MAX_BLOCKS); }
+static inline int cgs_blocks(int64_t m) { return (int)std::min<int64_t>((m + CGS_T - 1) / CGS_T, CGS_MAX_BLOCKS); }
 
 // the block's sums 0..kn-1 (after pdx_wave_reduce32) into column q of the partial sums
 // (partial[q * nblk + block]: the reductions read a sum's partials contiguously), in a fixed
@@ -324,7 +332,10 @@ __device__ __forceinline__ void cgs_block_store(const double (&s)[CGS_REG], int 
     __syncthreads();
     if ((int)threadIdx.x < kn) {
         const int q = threadIdx.x;
-        partial[(int64_t)(q0 + q) * gridDim.x + blockIdx.x] = (sh[0][q] + sh[1][q]) + (sh[2][q] + sh[3][q]);
+        double t = 0.0;
+#pragma unroll
+        for (int b = 0; b < CGS_NW; b++) t += sh[b][q];
+        partial[(int64_t)(q0 + q) * gridDim.x + blockIdx.x] = t;
     }
     __syncthreads();
 }
@@ -339,19 +350,24 @@ __device__ __forceinline__ double cgs_wave_sum(double t)
 // the block's partial of one more sum (x of every thread) into column q
 __device__ __forceinline__ void cgs_block_store1(double x, double *partial, int q)
 {
-    __shared__ double sh1[4];
+    __shared__ double sh1[CGS_NW];
     x = cgs_wave_sum(x);
     if ((threadIdx.x & 63) == 0) sh1[threadIdx.x >> 6] = x;
     __syncthreads();
-    if (threadIdx.x == 0) partial[(int64_t)q * gridDim.x + blockIdx.x] = (sh1[0] + sh1[1]) + (sh1[2] + sh1[3]);
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+#pragma unroll
+        for (int b = 0; b < CGS_NW; b++) t += sh1[b];
+        partial[(int64_t)q * gridDim.x + blockIdx.x] = t;
+    }
 }
 
 // partial (column q, one per block) = V_q . w for q < nv
-__global__ __launch_bounds__(256) void k_cgs_dots(int64_t m, int nv, const double *V, int64_t ldv,
+__global__ __launch_bounds__(CGS_T) void k_cgs_dots(int64_t m, int nv, const double *V, int64_t ldv,
                                                   const double *w, double *partial)
 {
-    __shared__ double sh[4][CGS_REG];
-    const int64_t i0 = (int64_t)blockIdx.x * 256, stride = (int64_t)gridDim.x * 256;
+    __shared__ double sh[CGS_NW][CGS_REG];
+    const int64_t i0 = (int64_t)blockIdx.x * CGS_T, stride = (int64_t)gridDim.x * CGS_T;
     if (m <= stride && nv <= CGS_REG) {
         // one dof per thread: the products go straight into the register array
         const int64_t i = i0 + threadIdx.x;
@@ -381,16 +397,16 @@ __global__ __launch_bounds__(256) void k_cgs_dots(int64_t m, int nv, const doubl
 }
 
 // w -= V h; partial columns q < nv: V_q . w (the updated w), column nv: w . w.  One dof
-// per thread (m <= 256 x blocks) and nv <= 32: the products overwrite the loaded V values,
+// per thread (m <= CGS_T x blocks) and nv <= 32: the products overwrite the loaded V values,
 // so only one 32-wide register array is live; h is read from LDS.
-__global__ __launch_bounds__(256) void k_cgs_axdn(int64_t m, int nv, const double *h, const double *V,
+__global__ __launch_bounds__(CGS_T) void k_cgs_axdn(int64_t m, int nv, const double *h, const double *V,
                                                   int64_t ldv, double *w, double *partial)
 {
-    __shared__ double sh[4][CGS_REG];
+    __shared__ double sh[CGS_NW][CGS_REG];
     __shared__ double hs[CGS_REG];
     if (threadIdx.x < CGS_REG) hs[threadIdx.x] = (int)threadIdx.x < nv ? h[threadIdx.x] : 0.0;
     __syncthreads();
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * CGS_T + threadIdx.x;
     double v[CGS_REG];
     double x = 0.0;
 #pragma unroll
@@ -414,11 +430,11 @@ __global__ __launch_bounds__(256) void k_cgs_axdn(int64_t m, int nv, const doubl
 
 // the same for several dofs per thread or a long basis: the update first, then the dots in
 // chunks (V read again)
-__global__ __launch_bounds__(256) void k_cgs_axdn_loop(int64_t m, int nv, const double *h, const double *V,
+__global__ __launch_bounds__(CGS_T) void k_cgs_axdn_loop(int64_t m, int nv, const double *h, const double *V,
                                                        int64_t ldv, double *w, double *partial)
 {
-    __shared__ double sh[4][CGS_REG];
-    const int64_t i0 = (int64_t)blockIdx.x * 256, stride = (int64_t)gridDim.x * 256;
+    __shared__ double sh[CGS_NW][CGS_REG];
+    const int64_t i0 = (int64_t)blockIdx.x * CGS_T, stride = (int64_t)gridDim.x * CGS_T;
     double nrm = 0.0;
     for (int64_t i = i0 + threadIdx.x; i < m; i += stride) {
         double x = w[i];
@@ -575,7 +591,7 @@ pdx_status pdx_nk_jv_dev(pdx_ctx *c, const double *v, double *jv, double scale, 
     const double sc = scale * std::cbrt(2.220446049250313e-16);
     if (!parts) {
         const int nb = cgs_blocks(k->m);
-        hipLaunchKernelGGL(k_cgs_dots, dim3(nb), dim3(256), 0, c->stream, k->m, 1, v, k->m, v, k->vparts);
+        hipLaunchKernelGGL(k_cgs_dots, dim3(nb), dim3(CGS_T), 0, c->stream, k->m, 1, v, k->m, v, k->vparts);
         parts = k->vparts;
         np = nb;
     }
@@ -1427,7 +1443,7 @@ static pdx_status nk_norm2_dev(pdx_ctx *c, const double *x, double *out_dev)
 {
     pdx_nk *k = c->nk;
     const int nb = cgs_blocks(k->m);
-    hipLaunchKernelGGL(k_cgs_dots, dim3(nb), dim3(256), 0, c->stream, k->m, 1, x, k->m, x, k->red);
+    hipLaunchKernelGGL(k_cgs_dots, dim3(nb), dim3(CGS_T), 0, c->stream, k->m, 1, x, k->m, x, k->red);
     hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, c->stream, nb, (const double *)k->red, out_dev);
     PDX_HIP(c, hipGetLastError());
     return PDX_OK;
@@ -1504,14 +1520,14 @@ static pdx_status fgmres(pdx_ctx *c, const double *b, int restart, int maxiter, 
             // partial sums that the Arnoldi step reduces into h2; then the next basis vector
             // (w - V h2) / |w - V h2| and its preconditioned image
             const int nv = j + 1;
-            hipLaunchKernelGGL(k_cgs_dots, dim3(CB), dim3(256), 0, s, m, nv, (const double *)k->Vb, m,
+            hipLaunchKernelGGL(k_cgs_dots, dim3(CB), dim3(CGS_T), 0, s, m, nv, (const double *)k->Vb, m,
                                (const double *)w, k->red);
             hipLaunchKernelGGL(k_reduce_cols, dim3((unsigned)nv), dim3(256), 0, s, CB, (const double *)k->red, G.h1);
-            if (nv <= CGS_REG && m <= (int64_t)CB * 256)
-                hipLaunchKernelGGL(k_cgs_axdn, dim3(CB), dim3(256), 0, s, m, nv, (const double *)G.h1,
+            if (nv <= CGS_REG && m <= (int64_t)CB * CGS_T)
+                hipLaunchKernelGGL(k_cgs_axdn, dim3(CB), dim3(CGS_T), 0, s, m, nv, (const double *)G.h1,
                                    (const double *)k->Vb, m, w, k->red);
             else
-                hipLaunchKernelGGL(k_cgs_axdn_loop, dim3(CB), dim3(256), 0, s, m, nv, (const double *)G.h1,
+                hipLaunchKernelGGL(k_cgs_axdn_loop, dim3(CB), dim3(CGS_T), 0, s, m, nv, (const double *)G.h1,
                                    (const double *)k->Vb, m, w, k->red);
             const int last_allowed = (it + enq + 1 >= maxiter) ? 1 : 0;
             const int tok = k->col_tok = k->col_tok % 0x3fffffff + 1;  // 1 .. 2^30 - 1: 2 tok + 1 fits an int

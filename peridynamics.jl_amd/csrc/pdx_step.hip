@@ -1796,7 +1796,7 @@ __device__ void pk1(int model, const double *F, double lambda, double mu, double
 // Q = P K^-1 - beta F stored for pass 2; defgrad, cauchy_stress, damage and the active-bond
 // count on export steps.  Km = K's six distinct entries (00 10 20 11 21 22).
 template <bool ZW = false>
-__device__ __forceinline__ void cpd_point_state(const pdx_step_args &A, int64_t p, const double Km[6],
+__device__ __forceinline__ bool cpd_point_state(const pdx_step_args &A, int64_t p, const double Km[6],
                                                const double Ft[9], int na, int c, bool dead,
                                                double lambda, double mu, double G, double K,
                                                double nu, double beta)
@@ -1870,6 +1870,7 @@ __device__ __forceinline__ void cpd_point_state(const pdx_step_args &A, int64_t 
         A.nact_out[p] = kill ? 0 : na;
         A.dmg_out[p] = dmg;
     }
+    return kill;
 }
 
 // CPD pass 1: calc_failure! (bond_system.jl:309-327), calc_damage!,
@@ -2068,6 +2069,9 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds1(pdx_step_args A)
     }
     double Km[6] = {0, 0, 0, 0, 0, 0};  // 00 10 20 11 21 22
     double Ft[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    // one set, ZEMSilling: the sums of w V_j DX and w V_j dx, for the own bond terms of pass 2
+    constexpr bool OWN = F == 0;
+    double S1[3] = {0, 0, 0}, S2[3] = {0, 0, 0};
     int na = 0;
     int t = sub;
     for (int T0 = 0; T0 < H; T0 += TL) {
@@ -2121,6 +2125,13 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds1(pdx_step_args A)
                 for (int cc = 0; cc < 3; cc++)
 #pragma unroll
                     for (int r = 0; r < 3; r++) Ft[r + 3 * cc] = fma(tD[cc], dx[r], Ft[r + 3 * cc]);
+                if (OWN) {
+#pragma unroll
+                    for (int r = 0; r < 3; r++) {
+                        S1[r] += tD[r];
+                        S2[r] = fma(temp, dx[r], S2[r]);
+                    }
+                }
             }
         });
     }
@@ -2130,10 +2141,33 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds1(pdx_step_args A)
         for (int q = 0; q < 6; q++) Km[q] += __shfl_xor(Km[q], o);
 #pragma unroll
         for (int q = 0; q < 9; q++) Ft[q] += __shfl_xor(Ft[q], o);
+        if (OWN) {
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                S1[r] += __shfl_xor(S1[r], o);
+                S2[r] += __shfl_xor(S2[r], o);
+            }
+        }
         na += __shfl_xor(na, o);
     }
     if (!valid || sub != 0) return;
-    cpd_point_state<(F & PDX_FEAT_ZW) != 0>(A, p, Km, Ft, na, c, dead, lam, mu, Gp, Kp, nup, beta);
+    const bool kill = cpd_point_state<(F & PDX_FEAT_ZW) != 0>(A, p, Km, Ft, na, c, dead, lam, mu, Gp, Kp, nup, beta);
+    if constexpr (OWN) {
+        // the own term of the point's bonds, Q_i S1 + beta S2, and a_i = Q_i X_i + beta x_i
+        // (zero for a dead point, whose Q is zero), behind the Q block of the state
+        // (x_i, X_i read again rather than held through the point state: its stress code
+        // takes the registers)
+        const double *const Q = A.pstate + 9 * p;
+        const double be = kill ? 0.0 : beta;
+        const double4 Xp = A.Xr[p], xp = A.rec_in[p];
+        const double X3[3] = {Xp.x, Xp.y, Xp.z}, x3[3] = {xp.x, xp.y, xp.z};
+        double *const ex = A.pstate + 9 * A.n + 6 * p;
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            ex[r] = fma(be, S2[r], fma(Q[r + 6], S1[2], fma(Q[r + 3], S1[1], Q[r] * S1[0])));
+            ex[3 + r] = fma(be, x3[r], fma(Q[r + 6], X3[2], fma(Q[r + 3], X3[1], Q[r] * X3[0])));
+        }
+    }
 }
 
 template <int S, int F>
@@ -2236,6 +2270,89 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2(pdx_step_args A)
     if (!valid) return;
     static_assert(S == 4, "the tail takes four lanes per point");
     point_tail4(A, p, ri, b3, sub, rho);
+}
+
+// CPD pass 2 with one parameter set and ZEMSilling.  The own terms of the point's bonds,
+// w V_j [Q_i DX + beta dx] summed, came from pass 1 as Q_i S1 + beta S2; the neighbour's term
+// of a bond, Q_j DX + beta_j dx, is a_j - Q_j X_i - beta_j x_i with a_j = Q_j X_j + beta_j x_j
+// from pass 1, in coordinates relative to the block's first point (the staging moves a_j
+// there), so that the products stay of the size of the bond vectors.  Table entries:
+// [j alive] (Q_j, a_j, beta), V_j (112 B, seven 16-byte reads per bond) instead of the
+// 136 B of x, X and Q, so a 160 KB tile holds 1463 entries.
+template <int S>
+__global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2_own(pdx_step_args A)
+{
+    extern __shared__ double4 hrec[];
+    double2 *const hq = reinterpret_cast<double2 *>(hrec);
+    const int TL = pdx_lds_entries(A.halo_max, A.lds_budget, 112);
+    const int64_t k = pdx_xcd_block();
+    const int64_t p0 = k * PDX_BLOCK;
+    if (p0 >= A.n) return;  // whole workgroup
+    const int64_t h0 = A.halo_ptr[k];
+    const int H = (int)(A.halo_ptr[k + 1] - h0);
+    const int32_t *const hidx = A.halo + h0;
+    const int sub = threadIdx.x % S;
+    const int64_t p = p0 + threadIdx.x / S;
+    const bool valid = p < A.n;
+    const int c = valid ? A.cnt[p] : 0;
+    const int64_t b0 = valid ? pdx_bond0(A.slice_ptr, p) : 0;
+    const uint16_t *const wbase = A.bw16 + b0;
+    const double *const Lbase = A.L + b0;
+    const double4 OX = A.Xr[p0], Ox = A.rec_in[p0];
+    const double4 ri = valid ? A.rec_in[p] : Ox;
+    const double4 Xi = valid ? A.Xr[p] : OX;
+    const double xr[3] = {ri.x - Ox.x, ri.y - Ox.y, ri.z - Ox.z};
+    const double Xr[3] = {Xi.x - OX.x, Xi.y - OX.y, Xi.z - OX.z};
+    const double O3x[3] = {Ox.x, Ox.y, Ox.z}, O3X[3] = {OX.x, OX.y, OX.z};
+    const double *const ext = A.pstate + 9 * A.n;
+    double b3[3] = {0.0, 0.0, 0.0};
+    int t = sub;
+    for (int T0 = 0; T0 < H; T0 += TL) {
+        const int T1 = H < T0 + TL ? H : T0 + TL;
+        if (T0 > 0) __syncthreads();  // the previous tile is consumed
+        for (int h = T0 + (int)threadIdx.x; h < T1; h += PDX_BLOCK * S) {
+            const int64_t j = hidx[h];
+            const bool live = !A.killed[j];
+            const double *const q = A.pstate + 9 * j;
+            const double *const aj = ext + 6 * j + 3;
+            double e[14];
+#pragma unroll
+            for (int m = 0; m < 9; m++) e[m] = live ? q[m] : 0.0;
+            const double bj = live ? A.beta_zem : 0.0;
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+                e[9 + r] = live ? aj[r] - fma(bj, O3x[r], fma(e[r + 6], O3X[2], fma(e[r + 3], O3X[1], e[r] * O3X[0])))
+                                : 0.0;
+            e[12] = bj;
+            e[13] = A.rec_in[j].w;
+            double2 *const d = hq + 7 * (int64_t)(h - T0);
+#pragma unroll
+            for (int m = 0; m < 7; m++) d[m] = make_double2(e[2 * m], e[2 * m + 1]);
+        }
+        __syncthreads();
+        t = pdx_walk_tile<PDX_CPD_GROUP, S>(wbase, Lbase, t, c, T1, [&](uint32_t wd, double Lb, int) {
+            if (!(wd & PDX_W16_ACT)) return;
+            const double2 *const d = hq + 7 * (int64_t)((int)(wd & PDX_W16_SLOT) - T0);
+            const double2 d0 = d[0], d1 = d[1], d2 = d[2], d3 = d[3], d4 = d[4], d5 = d[5], d6 = d[6];
+            const double Q[9] = {d0.x, d0.y, d1.x, d1.y, d2.x, d2.y, d3.x, d3.y, d4.x};
+            const double a[3] = {d4.y, d5.x, d5.y};
+            const double wv = pdx_omega(A.kernel, A.delta, Lb, pdx_rcp1(Lb)) * d6.y;  // w V_j
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                const double tt = a[r] - fma(d6.x, xr[r], fma(Q[r + 6], Xr[2], fma(Q[r + 3], Xr[1], Q[r] * Xr[0])));
+                b3[r] = fma(wv, tt, b3[r]);
+            }
+        });
+    }
+#pragma unroll
+    for (int o = 1; o < S; o <<= 1)
+#pragma unroll
+        for (int r = 0; r < 3; r++) b3[r] += __shfl_xor(b3[r], o);
+    if (!valid) return;
+#pragma unroll
+    for (int r = 0; r < 3; r++) b3[r] += ext[6 * p + r];  // the own terms (pass 1)
+    static_assert(S == 4, "the tail takes four lanes per point");
+    point_tail4(A, p, ri, b3, sub, A.rho);
 }
 
 // ----------------------------------------------------------------------------------------
@@ -2859,7 +2976,10 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
                 break;
             }
             const int fc = (c->multi() ? PDX_FEAT_MULTI : 0) | (c->P.zem == PDX_ZEM_WAN ? PDX_FEAT_ZW : 0);
-            const int e1 = 64 + (c->multi() ? 1 : 0), e2 = c->P.zem == PDX_ZEM_WAN ? 184 : 136;
+            // pass 2 entries: 112 B with one set and ZEMSilling (k_cpd_lds2_own), else 136 B, 184 B
+            // with ZEMWan
+            const int e1 = 64 + (c->multi() ? 1 : 0),
+                      e2 = c->P.zem == PDX_ZEM_WAN ? 184 : (c->multi() ? 136 : 112);
             PDX_TRY(pdx_timed_begin(c));
             switch (fc) {
             case 0: hipLaunchKernelGGL((k_cpd_lds1<PDX_LDS_LANES, 0>), G, BL, lds_bytes(c, e1, bud), c->stream, af); break;
@@ -2871,7 +2991,7 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
             PDX_TRY(pdx_timed_end(c, "pass1"));
             PDX_TRY(pdx_timed_begin(c));
             switch (fc) {
-            case 0: hipLaunchKernelGGL((k_cpd_lds2<PDX_LDS_LANES, 0>), G, BL, lds_bytes(c, e2, bud), c->stream, af); break;
+            case 0: hipLaunchKernelGGL((k_cpd_lds2_own<PDX_LDS_LANES>), G, BL, lds_bytes(c, e2, bud), c->stream, af); break;
             case PDX_FEAT_MULTI: hipLaunchKernelGGL((k_cpd_lds2<PDX_LDS_LANES, PDX_FEAT_MULTI>), G, BL, lds_bytes(c, e2, bud), c->stream, af); break;
             case PDX_FEAT_ZW: hipLaunchKernelGGL((k_cpd_lds2<PDX_LDS_LANES, PDX_FEAT_ZW>), G, BL, lds_bytes(c, e2, bud), c->stream, af); break;
             default: hipLaunchKernelGGL((k_cpd_lds2<PDX_LDS_LANES, PDX_FEAT_MULTI | PDX_FEAT_ZW>), G, BL, lds_bytes(c, e2, bud), c->stream, af); break;

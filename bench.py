@@ -277,8 +277,9 @@ def state_based_bytes(kind, n_points, n_bonds, layout, precision="fp64", zem="Si
     Q' = P K^-1 - beta (F - I) written and read back); CPD 511 B (as mixed CPD with
     Q = P K^-1 - beta F, plus the own bond terms Q_i S1 + beta S2 and a_i = Q_i X_i + beta x_i,
     six numbers written by pass 1 and read by pass 2; 415 B in the global layout, which has
-    no such split); CPD with ZEMWan 511 B (the state is
-    Q = P K^-1 - C_1 F and the six entries of the symmetric C_1, 15 numbers).  LDS layout: plus the 4-byte
+    no such split); CPD with ZEMWan 607 B (the state is Q = P K^-1 - C_1 F and the six
+    entries of the symmetric C_1, 15 numbers, plus the six numbers of the own terms and
+    a_i = Q_i X_i + C_1i x_i).  LDS layout: plus the 4-byte
     neighbour-table index per entry and pass."""
     if precision == "mixed" and layout["lds_mode"]:
         per_pt = 300 if kind == "OSB" else 414
@@ -287,7 +288,7 @@ def state_based_bytes(kind, n_points, n_bonds, layout, precision="fp64", zem="Si
     if kind == "OSB":
         per_pt = 268 if layout["lds_mode"] else 236
     else:
-        per_pt = 511 if zem == "Wan" or layout["lds_mode"] else 415
+        per_pt = (607 if zem == "Wan" else 511) if layout["lds_mode"] else 415
     if layout["lds_mode"]:
         return (20 * n_bonds + per_pt * n_points + 8 * layout["halo_entries"],
                 f"20*N_b + {per_pt}*N_p + 8*N_halo")

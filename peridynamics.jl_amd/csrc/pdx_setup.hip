@@ -862,10 +862,10 @@ extern "C" pdx_status pdx_build_bonds(pdx_ctx *c, int64_t *n_bonds_out)
     // OSB: the per-point pair of pass 1, then (LDS layout, one set) the own force part
     if (c->kind == PDX_OSB) TRY_C(pdx_alloc(c, &c->pstate, 5 * n));
     if (c->kind == PDX_CPD) {
-        // Q per point; with ZEMWan also the stabilisation stiffness C_1 (15 numbers per point);
-        // with ZEMSilling the LDS passes keep the own bond terms and a = Q X + beta x (six per
-        // point) behind the 9 n numbers of Q
-        TRY_C(pdx_alloc(c, &c->pstate, 15 * n));
+        // Q per point; with ZEMWan also the stabilisation stiffness C_1 (15 numbers per point).
+        // The LDS passes with one parameter set keep six more per point behind those: the own
+        // bond terms and a = Q X + beta x (ZEMWan: Q X + C_1 x)
+        TRY_C(pdx_alloc(c, &c->pstate, (c->P.zem == PDX_ZEM_WAN ? 21 : 15) * n));
         TRY_C(pdx_alloc(c, &c->defgrad, 9 * n));
         TRY_C(pdx_alloc(c, &c->cauchy, 9 * n));
         HIP_C(hipMemsetAsync(c->defgrad, 0, sizeof(double) * 9 * n, c->stream));

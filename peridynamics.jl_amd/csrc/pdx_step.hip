@@ -2069,8 +2069,9 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds1(pdx_step_args A)
     }
     double Km[6] = {0, 0, 0, 0, 0, 0};  // 00 10 20 11 21 22
     double Ft[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    // one set, ZEMSilling: the sums of w V_j DX and w V_j dx, for the own bond terms of pass 2
-    constexpr bool OWN = F == 0;
+    // one set: the sums of w V_j DX and w V_j dx, for the own bond terms of pass 2
+    constexpr bool OWN = !MULTI;
+    constexpr bool ZWF = (F & PDX_FEAT_ZW) != 0;
     double S1[3] = {0, 0, 0}, S2[3] = {0, 0, 0};
     int na = 0;
     int t = sub;
@@ -2151,21 +2152,31 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds1(pdx_step_args A)
         na += __shfl_xor(na, o);
     }
     if (!valid || sub != 0) return;
-    const bool kill = cpd_point_state<(F & PDX_FEAT_ZW) != 0>(A, p, Km, Ft, na, c, dead, lam, mu, Gp, Kp, nup, beta);
+    const bool kill = cpd_point_state<ZWF>(A, p, Km, Ft, na, c, dead, lam, mu, Gp, Kp, nup, beta);
     if constexpr (OWN) {
         // the own term of the point's bonds, Q_i S1 + beta S2, and a_i = Q_i X_i + beta x_i
-        // (zero for a dead point, whose Q is zero), behind the Q block of the state
-        // (x_i, X_i read again rather than held through the point state: its stress code
-        // takes the registers)
-        const double *const Q = A.pstate + 9 * p;
+        // (ZEMWan: C_1i in place of beta; zero for a dead point, whose state is zero), behind
+        // the state block (x_i, X_i read again rather than held through the point state:
+        // its stress code takes the registers)
+        constexpr int NQ = ZWF ? 15 : 9;
+        constexpr int SY[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};  // (r, c) -> stored entry of C_1
+        const double *const Q = A.pstate + NQ * p;
+        const double *const C1 = Q + 9;
         const double be = kill ? 0.0 : beta;
         const double4 Xp = A.Xr[p], xp = A.rec_in[p];
         const double X3[3] = {Xp.x, Xp.y, Xp.z}, x3[3] = {xp.x, xp.y, xp.z};
-        double *const ex = A.pstate + 9 * A.n + 6 * p;
+        double *const ex = A.pstate + NQ * A.n + 6 * p;
 #pragma unroll
         for (int r = 0; r < 3; r++) {
-            ex[r] = fma(be, S2[r], fma(Q[r + 6], S1[2], fma(Q[r + 3], S1[1], Q[r] * S1[0])));
-            ex[3 + r] = fma(be, x3[r], fma(Q[r + 6], X3[2], fma(Q[r + 3], X3[1], Q[r] * X3[0])));
+            const double qs = fma(Q[r + 6], S1[2], fma(Q[r + 3], S1[1], Q[r] * S1[0]));
+            const double qx = fma(Q[r + 6], X3[2], fma(Q[r + 3], X3[1], Q[r] * X3[0]));
+            if (ZWF) {
+                ex[r] = fma(C1[SY[r + 6]], S2[2], fma(C1[SY[r + 3]], S2[1], fma(C1[SY[r]], S2[0], qs)));
+                ex[3 + r] = fma(C1[SY[r + 6]], x3[2], fma(C1[SY[r + 3]], x3[1], fma(C1[SY[r]], x3[0], qx)));
+            } else {
+                ex[r] = fma(be, S2[r], qs);
+                ex[3 + r] = fma(be, x3[r], qx);
+            }
         }
     }
 }
@@ -2272,19 +2283,25 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2(pdx_step_args A)
     point_tail4(A, p, ri, b3, sub, rho);
 }
 
-// CPD pass 2 with one parameter set and ZEMSilling.  The own terms of the point's bonds,
-// w V_j [Q_i DX + beta dx] summed, came from pass 1 as Q_i S1 + beta S2; the neighbour's term
-// of a bond, Q_j DX + beta_j dx, is a_j - Q_j X_i - beta_j x_i with a_j = Q_j X_j + beta_j x_j
-// from pass 1, in coordinates relative to the block's first point (the staging moves a_j
-// there), so that the products stay of the size of the bond vectors.  Table entries:
-// [j alive] (Q_j, a_j, beta), V_j (112 B, seven 16-byte reads per bond) instead of the
-// 136 B of x, X and Q, so a 160 KB tile holds 1463 entries.
-template <int S>
+// CPD pass 2 with one parameter set.  The own terms of the point's bonds, w V_j [Q_i DX +
+// beta dx] summed, came from pass 1 as Q_i S1 + beta S2; the neighbour's term of a bond,
+// Q_j DX + beta_j dx, is a_j - Q_j X_i - beta_j x_i with a_j = Q_j X_j + beta_j x_j from
+// pass 1, in coordinates relative to the block's first point (the staging moves a_j there),
+// so that the products stay of the size of the block rather than of the coordinates.  ZW
+// (ZEMWan): the symmetric C_1 of each end in place of beta.  Table entries as 16-byte pairs:
+// [j alive] (Q_j, a_j), then [j alive] beta, V_j (112 B, 1463 entries in a 160 KB tile), or
+// with ZEMWan [j alive] C_1j (six entries), V_j and a pad (160 B, 1024 entries), against
+// the 136 B (184 B) of x, X and the state in the per-bond form.
+template <int S, bool ZW>
 __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2_own(pdx_step_args A)
 {
+    constexpr int NQ = ZW ? 15 : 9;  // state numbers per point
+    constexpr int ND = ZW ? 10 : 7;  // 16-byte pairs per entry
+    constexpr int IV = ZW ? 18 : 13;  // V_j in the entry
+    constexpr int SY[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};  // (r, c) -> stored entry of C_1
     extern __shared__ double4 hrec[];
     double2 *const hq = reinterpret_cast<double2 *>(hrec);
-    const int TL = pdx_lds_entries(A.halo_max, A.lds_budget, 112);
+    const int TL = pdx_lds_entries(A.halo_max, A.lds_budget, 16 * ND);
     const int64_t k = pdx_xcd_block();
     const int64_t p0 = k * PDX_BLOCK;
     if (p0 >= A.n) return;  // whole workgroup
@@ -2304,7 +2321,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2_own(pdx_step_args A)
     const double xr[3] = {ri.x - Ox.x, ri.y - Ox.y, ri.z - Ox.z};
     const double Xr[3] = {Xi.x - OX.x, Xi.y - OX.y, Xi.z - OX.z};
     const double O3x[3] = {Ox.x, Ox.y, Ox.z}, O3X[3] = {OX.x, OX.y, OX.z};
-    const double *const ext = A.pstate + 9 * A.n;
+    const double *const ext = A.pstate + NQ * A.n;
     double b3[3] = {0.0, 0.0, 0.0};
     int t = sub;
     for (int T0 = 0; T0 < H; T0 += TL) {
@@ -2313,34 +2330,52 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2_own(pdx_step_args A)
         for (int h = T0 + (int)threadIdx.x; h < T1; h += PDX_BLOCK * S) {
             const int64_t j = hidx[h];
             const bool live = !A.killed[j];
-            const double *const q = A.pstate + 9 * j;
+            const double *const q = A.pstate + NQ * j;
             const double *const aj = ext + 6 * j + 3;
-            double e[14];
+            double e[2 * ND];
 #pragma unroll
             for (int m = 0; m < 9; m++) e[m] = live ? q[m] : 0.0;
-            const double bj = live ? A.beta_zem : 0.0;
+            if (ZW) {
 #pragma unroll
-            for (int r = 0; r < 3; r++)
-                e[9 + r] = live ? aj[r] - fma(bj, O3x[r], fma(e[r + 6], O3X[2], fma(e[r + 3], O3X[1], e[r] * O3X[0])))
-                                : 0.0;
-            e[12] = bj;
-            e[13] = A.rec_in[j].w;
-            double2 *const d = hq + 7 * (int64_t)(h - T0);
+                for (int m = 0; m < 6; m++) e[12 + m] = live ? q[9 + m] : 0.0;
+                e[19] = 0.0;
+            } else {
+                e[12] = live ? A.beta_zem : 0.0;
+            }
+            e[IV] = A.rec_in[j].w;
 #pragma unroll
-            for (int m = 0; m < 7; m++) d[m] = make_double2(e[2 * m], e[2 * m + 1]);
+            for (int r = 0; r < 3; r++) {
+                double o = fma(e[r + 6], O3X[2], fma(e[r + 3], O3X[1], e[r] * O3X[0]));
+                if (ZW)
+                    o = fma(e[12 + SY[r + 6]], O3x[2], fma(e[12 + SY[r + 3]], O3x[1], fma(e[12 + SY[r]], O3x[0], o)));
+                else
+                    o = fma(e[12], O3x[r], o);
+                e[9 + r] = live ? aj[r] - o : 0.0;
+            }
+            double2 *const d = hq + ND * (int64_t)(h - T0);
+#pragma unroll
+            for (int m = 0; m < ND; m++) d[m] = make_double2(e[2 * m], e[2 * m + 1]);
         }
         __syncthreads();
         t = pdx_walk_tile<PDX_CPD_GROUP, S>(wbase, Lbase, t, c, T1, [&](uint32_t wd, double Lb, int) {
             if (!(wd & PDX_W16_ACT)) return;
-            const double2 *const d = hq + 7 * (int64_t)((int)(wd & PDX_W16_SLOT) - T0);
-            const double2 d0 = d[0], d1 = d[1], d2 = d[2], d3 = d[3], d4 = d[4], d5 = d[5], d6 = d[6];
-            const double Q[9] = {d0.x, d0.y, d1.x, d1.y, d2.x, d2.y, d3.x, d3.y, d4.x};
-            const double a[3] = {d4.y, d5.x, d5.y};
-            const double wv = pdx_omega(A.kernel, A.delta, Lb, pdx_rcp1(Lb)) * d6.y;  // w V_j
+            const double2 *const d = hq + ND * (int64_t)((int)(wd & PDX_W16_SLOT) - T0);
+            double e[2 * ND];
+#pragma unroll
+            for (int m = 0; m < ND; m++) {
+                const double2 v = d[m];
+                e[2 * m] = v.x;
+                e[2 * m + 1] = v.y;
+            }
+            const double wv = pdx_omega(A.kernel, A.delta, Lb, pdx_rcp1(Lb)) * e[IV];  // w V_j
 #pragma unroll
             for (int r = 0; r < 3; r++) {
-                const double tt = a[r] - fma(d6.x, xr[r], fma(Q[r + 6], Xr[2], fma(Q[r + 3], Xr[1], Q[r] * Xr[0])));
-                b3[r] = fma(wv, tt, b3[r]);
+                double o = fma(e[r + 6], Xr[2], fma(e[r + 3], Xr[1], e[r] * Xr[0]));
+                if (ZW)
+                    o = fma(e[12 + SY[r + 6]], xr[2], fma(e[12 + SY[r + 3]], xr[1], fma(e[12 + SY[r]], xr[0], o)));
+                else
+                    o = fma(e[12], xr[r], o);
+                b3[r] = fma(wv, e[9 + r] - o, b3[r]);
             }
         });
     }
@@ -2949,7 +2984,7 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
         break;
     case PDX_CPD:
         if (c->lds_mode) {
-            // The CPD passes hold 80-113 VGPRs, so one 1024-thread block runs per CU: their
+            // The CPD passes hold 97-122 VGPRs, so one 1024-thread block runs per CU: their
             // tables take the CU's whole LDS (fewer tiles, fewer entries read past the cap).
             pdx_step_args af = a;
             const int bud = pdx_lds_budget_full(c);
@@ -2976,10 +3011,11 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
                 break;
             }
             const int fc = (c->multi() ? PDX_FEAT_MULTI : 0) | (c->P.zem == PDX_ZEM_WAN ? PDX_FEAT_ZW : 0);
-            // pass 2 entries: 112 B with one set and ZEMSilling (k_cpd_lds2_own), else 136 B, 184 B
-            // with ZEMWan
+            // pass 2 entries: with one set (k_cpd_lds2_own) 112 B, 160 B with ZEMWan; with
+            // several sets 136 B, 184 B with ZEMWan
+            const bool wan = c->P.zem == PDX_ZEM_WAN;
             const int e1 = 64 + (c->multi() ? 1 : 0),
-                      e2 = c->P.zem == PDX_ZEM_WAN ? 184 : (c->multi() ? 136 : 112);
+                      e2 = c->multi() ? (wan ? 184 : 136) : (wan ? 160 : 112);
             PDX_TRY(pdx_timed_begin(c));
             switch (fc) {
             case 0: hipLaunchKernelGGL((k_cpd_lds1<PDX_LDS_LANES, 0>), G, BL, lds_bytes(c, e1, bud), c->stream, af); break;
@@ -2991,9 +3027,9 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
             PDX_TRY(pdx_timed_end(c, "pass1"));
             PDX_TRY(pdx_timed_begin(c));
             switch (fc) {
-            case 0: hipLaunchKernelGGL((k_cpd_lds2_own<PDX_LDS_LANES>), G, BL, lds_bytes(c, e2, bud), c->stream, af); break;
+            case 0: hipLaunchKernelGGL((k_cpd_lds2_own<PDX_LDS_LANES, false>), G, BL, lds_bytes(c, e2, bud), c->stream, af); break;
             case PDX_FEAT_MULTI: hipLaunchKernelGGL((k_cpd_lds2<PDX_LDS_LANES, PDX_FEAT_MULTI>), G, BL, lds_bytes(c, e2, bud), c->stream, af); break;
-            case PDX_FEAT_ZW: hipLaunchKernelGGL((k_cpd_lds2<PDX_LDS_LANES, PDX_FEAT_ZW>), G, BL, lds_bytes(c, e2, bud), c->stream, af); break;
+            case PDX_FEAT_ZW: hipLaunchKernelGGL((k_cpd_lds2_own<PDX_LDS_LANES, true>), G, BL, lds_bytes(c, e2, bud), c->stream, af); break;
             default: hipLaunchKernelGGL((k_cpd_lds2<PDX_LDS_LANES, PDX_FEAT_MULTI | PDX_FEAT_ZW>), G, BL, lds_bytes(c, e2, bud), c->stream, af); break;
             }
             PDX_HIP(c, hipGetLastError());

@@ -35,7 +35,7 @@ def test_step_bytes_formulas():
     assert bench.state_based_bytes("CPD", n_p, n_b, lds)[0] == 20 * n_b + 511 * n_p + 800
     assert bench.reference_layout_bytes("OSB", n_p, n_b, "mixed")[0] == 22 * n_b + 248 * n_p
     assert bench.state_based_bytes("CPD", n_p, n_b, lds, "mixed")[0] == 4 * n_b + 414 * n_p + 800
-    assert bench.state_based_bytes("CPD", n_p, n_b, lds, "fp64", "Wan")[0] == 20 * n_b + 511 * n_p + 800
+    assert bench.state_based_bytes("CPD", n_p, n_b, lds, "fp64", "Wan")[0] == 20 * n_b + 607 * n_p + 800
 
 
 def test_measured_traffic_lookup(tmp_path, monkeypatch):

@@ -863,8 +863,8 @@ extern "C" pdx_status pdx_build_bonds(pdx_ctx *c, int64_t *n_bonds_out)
     if (c->kind == PDX_OSB) TRY_C(pdx_alloc(c, &c->pstate, 5 * n));
     if (c->kind == PDX_CPD) {
         // Q per point; with ZEMWan also the stabilisation stiffness C_1 (15 numbers per point).
-        // The LDS passes with one parameter set keep six more per point behind those: the own
-        // bond terms and a = Q X + beta x (ZEMWan: Q X + C_1 x)
+        // The LDS passes keep six more per point behind those: the own bond terms and
+        // a = Q X + beta x (ZEMWan: Q X + C_1 x)
         TRY_C(pdx_alloc(c, &c->pstate, (c->P.zem == PDX_ZEM_WAN ? 21 : 15) * n));
         TRY_C(pdx_alloc(c, &c->defgrad, 9 * n));
         TRY_C(pdx_alloc(c, &c->cauchy, 9 * n));

@@ -2069,8 +2069,8 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds1(pdx_step_args A)
     }
     double Km[6] = {0, 0, 0, 0, 0, 0};  // 00 10 20 11 21 22
     double Ft[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    // one set: the sums of w V_j DX and w V_j dx, for the own bond terms of pass 2
-    constexpr bool OWN = !MULTI;
+    // the sums of w V_j DX and w V_j dx over the owner's copies of the bonds, for the own
+    // bond terms of pass 2
     constexpr bool ZWF = (F & PDX_FEAT_ZW) != 0;
     double S1[3] = {0, 0, 0}, S2[3] = {0, 0, 0};
     int na = 0;
@@ -2126,12 +2126,10 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds1(pdx_step_args A)
                 for (int cc = 0; cc < 3; cc++)
 #pragma unroll
                     for (int r = 0; r < 3; r++) Ft[r + 3 * cc] = fma(tD[cc], dx[r], Ft[r + 3 * cc]);
-                if (OWN) {
 #pragma unroll
-                    for (int r = 0; r < 3; r++) {
-                        S1[r] += tD[r];
-                        S2[r] = fma(temp, dx[r], S2[r]);
-                    }
+                for (int r = 0; r < 3; r++) {
+                    S1[r] += tD[r];
+                    S2[r] = fma(temp, dx[r], S2[r]);
                 }
             }
         });
@@ -2142,18 +2140,16 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds1(pdx_step_args A)
         for (int q = 0; q < 6; q++) Km[q] += __shfl_xor(Km[q], o);
 #pragma unroll
         for (int q = 0; q < 9; q++) Ft[q] += __shfl_xor(Ft[q], o);
-        if (OWN) {
 #pragma unroll
-            for (int r = 0; r < 3; r++) {
-                S1[r] += __shfl_xor(S1[r], o);
-                S2[r] += __shfl_xor(S2[r], o);
-            }
+        for (int r = 0; r < 3; r++) {
+            S1[r] += __shfl_xor(S1[r], o);
+            S2[r] += __shfl_xor(S2[r], o);
         }
         na += __shfl_xor(na, o);
     }
     if (!valid || sub != 0) return;
     const bool kill = cpd_point_state<ZWF>(A, p, Km, Ft, na, c, dead, lam, mu, Gp, Kp, nup, beta);
-    if constexpr (OWN) {
+    {
         // the own term of the point's bonds, Q_i S1 + beta S2, and a_i = Q_i X_i + beta x_i
         // (ZEMWan: C_1i in place of beta; zero for a dead point, whose state is zero), behind
         // the state block (x_i, X_i read again rather than held through the point state:
@@ -2181,109 +2177,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds1(pdx_step_args A)
     }
 }
 
-template <int S, int F>
-__global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2(pdx_step_args A)
-{
-    // PDX_FEAT_MULTI: beta of each end's set; the own term follows the owner's copy of the
-    // bond (ACT), the neighbour's term the neighbour's copy (RACT)
-    constexpr bool MULTI = (F & PDX_FEAT_MULTI) != 0;
-    // PDX_FEAT_ZW: ZEMWan, the stabilisation C_1 dx of each end instead of beta dx; entries
-    // also carry [j alive] C_1 of the neighbour, six entries of the symmetric C_1 (184 B)
-    constexpr bool ZW = (F & PDX_FEAT_ZW) != 0;
-    constexpr int NQ = ZW ? 15 : 9;
-    constexpr int SY[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};  // (r, c) -> stored entry of C_1
-    extern __shared__ double4 hrec[];
-    const int TL = pdx_lds_entries(A.halo_max, A.lds_budget, 64 + 8 * NQ);
-    double4 *const hx = hrec;
-    double4 *const hX = hrec + TL;
-    double *const hq = reinterpret_cast<double *>(hrec + 2 * TL);
-    const int64_t k = pdx_xcd_block();
-    const int64_t p0 = k * PDX_BLOCK;
-    if (p0 >= A.n) return;  // whole workgroup
-    const int64_t h0 = A.halo_ptr[k];
-    const int H = (int)(A.halo_ptr[k + 1] - h0);
-    const int32_t *const hidx = A.halo + h0;
-    const int sub = threadIdx.x % S;
-    const int64_t p = p0 + threadIdx.x / S;
-    const bool valid = p < A.n;
-    const int c = valid ? A.cnt[p] : 0;
-    const int64_t b0 = valid ? pdx_bond0(A.slice_ptr, p) : 0;
-    const uint16_t *const wbase = A.bw16 + b0;
-    const double *const Lbase = A.L + b0;
-    const double4 ri = valid ? A.rec_in[p] : make_double4(0.0, 0.0, 0.0, 0.0);
-    const double4 Xi = valid ? A.Xr[p] : make_double4(0.0, 0.0, 0.0, 0.0);
-    const bool dead_i = !valid || A.killed[p] != 0;
-    double beta = A.beta_zem, rho = A.rho;  // several sets: the point's own (velocity_verlet.jl:262-275)
-    if (MULTI && valid) {
-        const pdx_pset Pi = A.psets[A.pid[p]];
-        beta = Pi.beta;
-        rho = Pi.rho;
-    }
-    // a dead point's own terms vanish: its state (zero since pass 1) and its beta are zeroed
-    double Qi[NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; q++) Qi[q] = dead_i ? 0.0 : A.pstate[NQ * p + q];
-    const double beta_own = dead_i ? 0.0 : beta;
-    double b3[3] = {0.0, 0.0, 0.0};
-    int t = sub;
-    for (int T0 = 0; T0 < H; T0 += TL) {
-        const int T1 = H < T0 + TL ? H : T0 + TL;
-        if (T0 > 0) __syncthreads();
-        for (int h = T0 + (int)threadIdx.x; h < T1; h += PDX_BLOCK * S) {
-            const int32_t j = hidx[h];
-            const bool live = !A.killed[j];
-            const double4 X = A.Xr[j];
-            const double beta_j = MULTI ? A.psets[A.pid[j]].beta : beta;
-            hx[h - T0] = A.rec_in[j];
-            hX[h - T0] = make_double4(X.x, X.y, X.z, live ? beta_j : 0.0);
-            double *const qd = hq + NQ * (int64_t)(h - T0);
-#pragma unroll
-            for (int q = 0; q < NQ; q++) qd[q] = live ? A.pstate[NQ * (int64_t)j + q] : 0.0;
-        }
-        __syncthreads();
-        t = pdx_walk_tile<F == 0 ? PDX_CPD_GROUP : 2, S>(wbase, Lbase, t, c, T1, [&](uint32_t wd, double Lb, int) {
-            const int slot = (int)(wd & PDX_W16_SLOT);
-            const bool act = (wd & PDX_W16_ACT) != 0;
-            const bool ract = MULTI ? (wd & PDX_W16_RACT) != 0 : act;
-            if (act || ract) {
-                const double4 rj = hx[slot - T0];
-                const double4 Xj = hX[slot - T0];
-                const double *const qj = hq + NQ * (int64_t)(slot - T0);
-                const double dx[3] = {rj.x - ri.x, rj.y - ri.y, rj.z - ri.z};
-                const double DX[3] = {Xj.x - Xi.x, Xj.y - Xi.y, Xj.z - Xi.z};
-                const double wv = pdx_omega(A.kernel, A.delta, Lb, pdx_rcp1(Lb)) * rj.w;  // w V_j
-                // both ends' terms summed before the products: (Q_i + Q_j) DX + (beta_i + beta_j) dx,
-                // ZEMWan (Q_i + Q_j) DX + (C_1i + C_1j) dx; the own term counts for the owner's
-                // copy of the bond (ACT), the neighbour's for its copy (RACT; the same bit with
-                // one parameter set, and this body runs only when it is set).  A dead neighbour
-                // was staged as zeros, a dead owner's state and beta are zero.
-                const double fo = act ? 1.0 : 0.0, fn = ract ? 1.0 : 0.0;
-                double M[NQ];
-#pragma unroll
-                for (int q = 0; q < NQ; q++) M[q] = MULTI ? fma(fo, Qi[q], fn * qj[q]) : Qi[q] + qj[q];
-                const double bs = ZW ? 0.0 : (MULTI ? fma(fo, beta_own, fn * Xj.w) : beta_own + Xj.w);
-#pragma unroll
-                for (int r = 0; r < 3; r++) {
-                    double tt = fma(M[r + 6], DX[2], fma(M[r + 3], DX[1], M[r] * DX[0]));
-                    if (ZW)
-                        tt = fma(M[9 + SY[r + 6]], dx[2], fma(M[9 + SY[r + 3]], dx[1], fma(M[9 + SY[r]], dx[0], tt)));
-                    else
-                        tt = fma(bs, dx[r], tt);
-                    b3[r] = fma(wv, tt, b3[r]);
-                }
-            }
-        });
-    }
-#pragma unroll
-    for (int o = 1; o < S; o <<= 1)
-#pragma unroll
-        for (int r = 0; r < 3; r++) b3[r] += __shfl_xor(b3[r], o);
-    if (!valid) return;
-    static_assert(S == 4, "the tail takes four lanes per point");
-    point_tail4(A, p, ri, b3, sub, rho);
-}
-
-// CPD pass 2 with one parameter set.  The own terms of the point's bonds, w V_j [Q_i DX +
+// CPD pass 2 (correspondence.jl:234-249).  The own terms of the point's bonds, w V_j [Q_i DX +
 // beta dx] summed, came from pass 1 as Q_i S1 + beta S2; the neighbour's term of a bond,
 // Q_j DX + beta_j dx, is a_j - Q_j X_i - beta_j x_i with a_j = Q_j X_j + beta_j x_j from
 // pass 1, in coordinates relative to the block's first point (the staging moves a_j there),
@@ -2291,10 +2185,16 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2(pdx_step_args A)
 // (ZEMWan): the symmetric C_1 of each end in place of beta.  Table entries as 16-byte pairs:
 // [j alive] (Q_j, a_j), then [j alive] beta, V_j (112 B, 1463 entries in a 160 KB tile), or
 // with ZEMWan [j alive] C_1j (six entries), V_j and a pad (160 B, 1024 entries), against
-// the 136 B (184 B) of x, X and the state in the per-bond form.
-template <int S, bool ZW>
+// the 136 B (184 B) of x, X and the state in a per-bond form.  PDX_FEAT_MULTI (several
+// parameter sets): beta_j of the neighbour's set, the neighbour's term for the neighbour's
+// copy of the bond (RACT; the own terms followed the owner's copy, ACT, in pass 1), and the
+// point's own rho (velocity_verlet.jl:262-275).
+template <int S, int F>
 __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2_own(pdx_step_args A)
 {
+    constexpr bool MULTI = (F & PDX_FEAT_MULTI) != 0;
+    constexpr bool ZW = (F & PDX_FEAT_ZW) != 0;
+    constexpr uint32_t NBIT = MULTI ? PDX_W16_RACT : PDX_W16_ACT;  // the neighbour's copy
     constexpr int NQ = ZW ? 15 : 9;  // state numbers per point
     constexpr int ND = ZW ? 10 : 7;  // 16-byte pairs per entry
     constexpr int IV = ZW ? 18 : 13;  // V_j in the entry
@@ -2322,6 +2222,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2_own(pdx_step_args A)
     const double Xr[3] = {Xi.x - OX.x, Xi.y - OX.y, Xi.z - OX.z};
     const double O3x[3] = {Ox.x, Ox.y, Ox.z}, O3X[3] = {OX.x, OX.y, OX.z};
     const double *const ext = A.pstate + NQ * A.n;
+    const double rho = MULTI && valid ? A.psets[A.pid[p]].rho : A.rho;
     double b3[3] = {0.0, 0.0, 0.0};
     int t = sub;
     for (int T0 = 0; T0 < H; T0 += TL) {
@@ -2340,7 +2241,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2_own(pdx_step_args A)
                 for (int m = 0; m < 6; m++) e[12 + m] = live ? q[9 + m] : 0.0;
                 e[19] = 0.0;
             } else {
-                e[12] = live ? A.beta_zem : 0.0;
+                e[12] = live ? (MULTI ? A.psets[A.pid[j]].beta : A.beta_zem) : 0.0;
             }
             e[IV] = A.rec_in[j].w;
 #pragma unroll
@@ -2358,7 +2259,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2_own(pdx_step_args A)
         }
         __syncthreads();
         t = pdx_walk_tile<PDX_CPD_GROUP, S>(wbase, Lbase, t, c, T1, [&](uint32_t wd, double Lb, int) {
-            if (!(wd & PDX_W16_ACT)) return;
+            if (!(wd & NBIT)) return;
             const double2 *const d = hq + ND * (int64_t)((int)(wd & PDX_W16_SLOT) - T0);
             double e[2 * ND];
 #pragma unroll
@@ -2387,7 +2288,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_lds2_own(pdx_step_args A)
 #pragma unroll
     for (int r = 0; r < 3; r++) b3[r] += ext[6 * p + r];  // the own terms (pass 1)
     static_assert(S == 4, "the tail takes four lanes per point");
-    point_tail4(A, p, ri, b3, sub, A.rho);
+    point_tail4(A, p, ri, b3, sub, rho);
 }
 
 // ----------------------------------------------------------------------------------------
@@ -3011,11 +2912,8 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
                 break;
             }
             const int fc = (c->multi() ? PDX_FEAT_MULTI : 0) | (c->P.zem == PDX_ZEM_WAN ? PDX_FEAT_ZW : 0);
-            // pass 2 entries: with one set (k_cpd_lds2_own) 112 B, 160 B with ZEMWan; with
-            // several sets 136 B, 184 B with ZEMWan
-            const bool wan = c->P.zem == PDX_ZEM_WAN;
-            const int e1 = 64 + (c->multi() ? 1 : 0),
-                      e2 = c->multi() ? (wan ? 184 : 136) : (wan ? 160 : 112);
+            // pass 2 entries 112 B, 160 B with ZEMWan
+            const int e1 = 64 + (c->multi() ? 1 : 0), e2 = c->P.zem == PDX_ZEM_WAN ? 160 : 112;
             PDX_TRY(pdx_timed_begin(c));
             switch (fc) {
             case 0: hipLaunchKernelGGL((k_cpd_lds1<PDX_LDS_LANES, 0>), G, BL, lds_bytes(c, e1, bud), c->stream, af); break;
@@ -3027,10 +2925,10 @@ pdx_status pdx_launch_step(pdx_ctx *c, const pdx_step_args &a)
             PDX_TRY(pdx_timed_end(c, "pass1"));
             PDX_TRY(pdx_timed_begin(c));
             switch (fc) {
-            case 0: hipLaunchKernelGGL((k_cpd_lds2_own<PDX_LDS_LANES, false>), G, BL, lds_bytes(c, e2, bud), c->stream, af); break;
-            case PDX_FEAT_MULTI: hipLaunchKernelGGL((k_cpd_lds2<PDX_LDS_LANES, PDX_FEAT_MULTI>), G, BL, lds_bytes(c, e2, bud), c->stream, af); break;
-            case PDX_FEAT_ZW: hipLaunchKernelGGL((k_cpd_lds2_own<PDX_LDS_LANES, true>), G, BL, lds_bytes(c, e2, bud), c->stream, af); break;
-            default: hipLaunchKernelGGL((k_cpd_lds2<PDX_LDS_LANES, PDX_FEAT_MULTI | PDX_FEAT_ZW>), G, BL, lds_bytes(c, e2, bud), c->stream, af); break;
+            case 0: hipLaunchKernelGGL((k_cpd_lds2_own<PDX_LDS_LANES, 0>), G, BL, lds_bytes(c, e2, bud), c->stream, af); break;
+            case PDX_FEAT_MULTI: hipLaunchKernelGGL((k_cpd_lds2_own<PDX_LDS_LANES, PDX_FEAT_MULTI>), G, BL, lds_bytes(c, e2, bud), c->stream, af); break;
+            case PDX_FEAT_ZW: hipLaunchKernelGGL((k_cpd_lds2_own<PDX_LDS_LANES, PDX_FEAT_ZW>), G, BL, lds_bytes(c, e2, bud), c->stream, af); break;
+            default: hipLaunchKernelGGL((k_cpd_lds2_own<PDX_LDS_LANES, PDX_FEAT_MULTI | PDX_FEAT_ZW>), G, BL, lds_bytes(c, e2, bud), c->stream, af); break;
             }
             PDX_HIP(c, hipGetLastError());
             PDX_TRY(pdx_timed_end(c, "step"));

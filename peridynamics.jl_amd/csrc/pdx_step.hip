@@ -2009,11 +2009,11 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_cpd_pass2(pdx_step_args A)
 }
 
 // bonds per group of CPD pass 2's bond walk (pdx_walk_tile): the loads of the next group are
-// in flight while a group is evaluated.  The variants with features (several parameter sets,
-// ZEMWan) hold more per-point state and take groups of 2, which keeps them in registers; pass
-// 1, whose bond arithmetic is longer, runs fastest with groups of 1.
+// in flight while a group is evaluated.  Measured at 512 k points, pass 2 with ZEMSilling:
+// 0.330 / 0.306 / 0.319 / 0.328 / 0.365 ms for groups of 1 / 2 / 3 / 4 / 8 (ZEMWan within
+// 1 % for 2 to 4).  Pass 1, whose bond arithmetic is longer, runs fastest with groups of 1.
 #ifndef PDX_CPD_GROUP
-#define PDX_CPD_GROUP 4
+#define PDX_CPD_GROUP 2
 #endif
 
 // ----------------------------------------------------------------------------------------
